@@ -1,0 +1,296 @@
+"""ctypes view of include/fs_abi.h.
+
+The structs and the ``Solver`` wrapper are shared by the HIP library
+(``libfs_hip.so``, prefix ``fs_``) and by the CPU checker used in tests
+(prefix ``fso_``), so a parity test drives both sides with the same calls.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+FS_OK = 0
+FS_ERR_BAD_ARG = -1
+FS_ERR_OOM = -2
+FS_ERR_HIP = -3
+FS_ERR_NOT_CONVERGED = -4
+FS_ERR_NONFINITE = -5
+FS_ERR_UNSUPPORTED = -6
+
+FS_FP64 = 0
+FS_FP32_REFINE = 1
+FS_STOP_ABS_PRECOND = 0
+FS_STOP_REL_RESIDUAL = 1
+FS_SCHEDULE_COMPAT = 0
+FS_SCHEDULE_FIXED = 1
+
+ERROR_NAMES = {
+    FS_ERR_BAD_ARG: "FS_ERR_BAD_ARG",
+    FS_ERR_OOM: "FS_ERR_OOM",
+    FS_ERR_HIP: "FS_ERR_HIP",
+    FS_ERR_NOT_CONVERGED: "FS_ERR_NOT_CONVERGED",
+    FS_ERR_NONFINITE: "FS_ERR_NONFINITE",
+    FS_ERR_UNSUPPORTED: "FS_ERR_UNSUPPORTED",
+}
+
+_D = C.c_double
+_I = C.c_int
+
+
+class FsGrid(C.Structure):
+    _fields_ = [("nx", _I), ("ny", _I), ("nz", _I),
+                ("x_min", _D), ("y_min", _D), ("z_min", _D),
+                ("x_max", _D), ("y_max", _D), ("z_max", _D)]
+
+
+class FsParams(C.Structure):
+    _fields_ = [
+        ("air_bubble_rising", _I),
+        ("water_density", _D), ("air_density", _D),
+        ("water_viscosity", _D), ("air_viscosity", _D),
+        ("gravity", _D * 3),
+        ("surface_tension", _D),
+        ("use_delta_function_formulation", _I),
+        ("order_for_time_advancing", _I),
+        ("sussmanstyle_reinitialization", _I),
+        ("scaling_number_for_reinitialization", _I),
+        ("iteration_number_for_reinitialization", _I),
+        ("levelset_epsilon", _D),
+        ("velocity_epsilon", _D),
+        ("use_5th_weno_v", _I),
+        ("tolerance", _D),
+        ("max_iteration", _I),
+        ("use_previous_pressure", _I),
+        ("stop_rule", _I),
+        ("rel_tolerance", _D),
+        ("precision", _I),
+        ("schedule", _I),
+        ("is_levelset_advection_active", _I),
+        ("is_velocity_advection_active", _I),
+        ("is_external_force_active", _I),
+        ("is_sourcing_active", _I),
+        ("is_projection_active", _I),
+    ]
+
+
+class FsStepStats(C.Structure):
+    _fields_ = [("pcg_iterations", _I), ("refinement_sweeps", _I),
+                ("residual_precond", _D), ("residual_rel", _D),
+                ("max_velocity", _D * 3), ("projection_ms", _D), ("step_ms", _D)]
+
+    def as_dict(self):
+        return {"pcg_iterations": self.pcg_iterations,
+                "refinement_sweeps": self.refinement_sweeps,
+                "residual_precond": self.residual_precond,
+                "residual_rel": self.residual_rel,
+                "max_velocity": list(self.max_velocity),
+                "projection_ms": self.projection_ms,
+                "step_ms": self.step_ms}
+
+
+def default_params(**kw):
+    """Air/water defaults of the reference script (3D_script.sim:90-178),
+    with zero surface tension, gravity along -z."""
+    p = FsParams()
+    p.air_bubble_rising = 1
+    p.water_density, p.air_density = 1000.0, 1.0
+    p.water_viscosity, p.air_viscosity = 1e-3, 1.8e-5
+    p.gravity[0], p.gravity[1], p.gravity[2] = 0.0, 0.0, -9.8
+    p.surface_tension = 0.0
+    p.use_delta_function_formulation = 1
+    p.order_for_time_advancing = 1
+    p.sussmanstyle_reinitialization = 1
+    p.scaling_number_for_reinitialization = 5
+    p.iteration_number_for_reinitialization = 25
+    p.levelset_epsilon = 1e-6
+    p.velocity_epsilon = 1e-6
+    p.use_5th_weno_v = 0
+    p.tolerance = 1e-5
+    p.max_iteration = 10000
+    p.use_previous_pressure = 1
+    p.stop_rule = FS_STOP_ABS_PRECOND
+    p.rel_tolerance = 1e-8
+    p.precision = FS_FP64
+    p.schedule = FS_SCHEDULE_COMPAT
+    p.is_levelset_advection_active = 1
+    p.is_velocity_advection_active = 1
+    p.is_external_force_active = 1
+    p.is_sourcing_active = 1
+    p.is_projection_active = 1
+    for k, v in kw.items():
+        if k == "gravity":
+            for i in range(3):
+                p.gravity[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def unit_cube_grid(nx, ny=None, nz=None):
+    """Cell-centred grid on the unit cube: first/last centres at dx/2 and
+    1-dx/2, so that the reference's (max-min)/(res-1) spacing is 1/n."""
+    ny = nx if ny is None else ny
+    nz = nx if nz is None else nz
+    g = FsGrid()
+    g.nx, g.ny, g.nz = nx, ny, nz
+    g.x_min, g.y_min, g.z_min = 0.5 / nx, 0.5 / ny, 0.5 / nz
+    g.x_max, g.y_max, g.z_max = 1 - 0.5 / nx, 1 - 0.5 / ny, 1 - 0.5 / nz
+    return g
+
+
+_PD = C.POINTER(C.c_double)
+
+
+def _ptr(a):
+    if a is None:
+        return None
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PD)
+
+
+def bind(lib, prefix):
+    """Declare the argument types of the ABI functions in ``lib``."""
+    f = lambda n: getattr(lib, prefix + n)
+    vp = C.c_void_p
+    f("create").argtypes = ([C.POINTER(FsParams), C.POINTER(FsGrid)] +
+                            ([_I] if prefix == "fs_" else []) + [C.POINTER(vp)])
+    f("create").restype = _I
+    f("destroy").argtypes = [vp]
+    f("destroy").restype = None
+    f("set_fields").argtypes = [vp, _PD, _PD, _PD, _PD, _PD]
+    f("get_fields").argtypes = [vp, _PD, _PD, _PD, _PD, _PD]
+    for n in ("advance_frame", "step"):
+        f(n).argtypes = [vp, _D, C.POINTER(FsStepStats)]
+    f("cfl_dt").argtypes = [vp, _D, _I, C.POINTER(_D)]
+    f("compute_curvatures").argtypes = [vp]
+    for n in ("levelset_advection", "velocity_advection", "apply_gravity", "sourcing", "reinitialization"):
+        f(n).argtypes = [vp, _D]
+    f("projection").argtypes = [vp, _D, C.POINTER(FsStepStats)]
+    f("assemble_pressure_system").argtypes = [vp, _D, _PD, _PD, _PD, _PD, _PD]
+    f("apply_pressure_operator").argtypes = [vp, _PD, _PD]
+    if prefix == "fs_":
+        lib.fs_time_pcg_kernels.argtypes = [vp, _I, _I, _PD]
+        lib.fs_last_error.argtypes = [vp]
+        lib.fs_last_error.restype = C.c_char_p
+    return lib
+
+
+class FsError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"{ERROR_NAMES.get(code, code)}: {msg}")
+        self.code = code
+
+
+class Solver:
+    """Thin wrapper over one handle of either library.
+
+    Mirrors EULERIAN_FLUID_SOLVER_3D's call surface: fields in and out,
+    one frame / one step, the phase functions and the CFL time step."""
+
+    def __init__(self, lib, prefix, params, grid, device=0):
+        self.lib, self.prefix = lib, prefix
+        self.params, self.grid = params, grid
+        self.nx, self.ny, self.nz = grid.nx, grid.ny, grid.nz
+        h = C.c_void_p()
+        args = [C.byref(params), C.byref(grid)] + ([device] if prefix == "fs_" else []) + [C.byref(h)]
+        rc = self._f("create")(*args)
+        if rc != FS_OK:
+            raise FsError(rc, "create failed")
+        self.h = h
+
+    def _f(self, n):
+        return getattr(self.lib, self.prefix + n)
+
+    def _check(self, rc, what, allow=()):
+        if rc != FS_OK and rc not in allow:
+            msg = what
+            if self.prefix == "fs_":
+                m = self.lib.fs_last_error(self.h)
+                msg = f"{what}: {m.decode() if m else ''}"
+            raise FsError(rc, msg)
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            self._f("destroy")(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # shapes, x fastest: arrays are indexed [k, j, i]
+    def shape(self, comp):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        return {"phi": (nz, ny, nx), "p": (nz, ny, nx), "u": (nz, ny, nx + 1),
+                "v": (nz, ny + 1, nx), "w": (nz + 1, ny, nx)}[comp]
+
+    def set_fields(self, phi=None, u=None, v=None, w=None, p=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        phi, u, v, w, p = map(conv, (phi, u, v, w, p))
+        self._check(self._f("set_fields")(self.h, _ptr(phi), _ptr(u), _ptr(v), _ptr(w), _ptr(p)), "set_fields")
+
+    def get_fields(self):
+        out = {c: np.empty(self.shape(c)) for c in ("phi", "u", "v", "w", "p")}
+        self._check(self._f("get_fields")(self.h, *[_ptr(out[c]) for c in ("phi", "u", "v", "w", "p")]),
+                    "get_fields")
+        return out
+
+    def advance_frame(self, dt, allow_not_converged=False):
+        st = FsStepStats()
+        self._check(self._f("advance_frame")(self.h, dt, C.byref(st)), "advance_frame",
+                    (FS_ERR_NOT_CONVERGED,) if allow_not_converged else ())
+        return st.as_dict()
+
+    def step(self, dt, allow_not_converged=False):
+        st = FsStepStats()
+        self._check(self._f("step")(self.h, dt, C.byref(st)), "step",
+                    (FS_ERR_NOT_CONVERGED,) if allow_not_converged else ())
+        return st.as_dict()
+
+    def cfl_dt(self, cfl, gravity_norm=1):
+        d = C.c_double()
+        self._check(self._f("cfl_dt")(self.h, cfl, gravity_norm, C.byref(d)), "cfl_dt")
+        return d.value
+
+    def compute_curvatures(self):
+        self._check(self._f("compute_curvatures")(self.h), "compute_curvatures")
+
+    def phase(self, name, dt):
+        self._check(self._f(name)(self.h, dt), name)
+
+    def projection(self, dt, allow_not_converged=False):
+        st = FsStepStats()
+        self._check(self._f("projection")(self.h, dt, C.byref(st)), "projection",
+                    (FS_ERR_NOT_CONVERGED,) if allow_not_converged else ())
+        return st.as_dict()
+
+    def assemble(self, dt):
+        nx, ny, nz = self.nx, self.ny, self.nz
+        out = {"diag": np.empty((nz, ny, nx)), "bx": np.empty((nz, ny, nx + 1)),
+               "by": np.empty((nz, ny + 1, nx)), "bz": np.empty((nz + 1, ny, nx)),
+               "rhs": np.empty((nz, ny, nx))}
+        self._check(self._f("assemble_pressure_system")(self.h, dt, *[_ptr(out[c]) for c in
+                                                                      ("diag", "bx", "by", "bz", "rhs")]),
+                    "assemble")
+        return out
+
+    def apply_operator(self, x):
+        x = np.ascontiguousarray(x, dtype=np.float64)
+        q = np.empty_like(x)
+        self._check(self._f("apply_pressure_operator")(self.h, _ptr(x), _ptr(q)), "apply_operator")
+        return q
+
+
+def time_pcg_kernels(solver, precision, reps):
+    out = np.zeros(3)
+    solver._check(solver.lib.fs_time_pcg_kernels(solver.h, precision, reps, _ptr(out)), "time_pcg_kernels")
+    return out
+
+
+def load(path, prefix):
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    return bind(C.CDLL(path), prefix)

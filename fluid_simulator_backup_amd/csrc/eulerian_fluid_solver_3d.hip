@@ -1,0 +1,702 @@
+// eulerian_fluid_solver_3d.hip -- host class and C-ABI (include/fs_abi.h).
+//
+// EulerianFluidSolver3D keeps the call surface of the reference class
+// EULERIAN_FLUID_SOLVER_3D (EULERIAN_FLUID_SOLVER_3D.h): the same phase
+// functions, called in the same order, on device-resident fields.  All work
+// of a handle runs on its own HIP stream; each public call returns after the
+// stream has drained.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <utility>
+
+#include "../../include/fs_abi.h"
+#include "fs_kernels.h"
+#include "fs_layout.h"
+#include "fs_pcg.h"
+
+namespace {
+
+struct HipError {
+    hipError_t e;
+    const char* what;
+};
+
+#define HIPCHECK(x)                                  \
+    do {                                             \
+        hipError_t _e = (x);                         \
+        if (_e != hipSuccess) throw HipError{_e, #x}; \
+    } while (0)
+
+struct FsFail {
+    int code;
+    std::string msg;
+};
+
+}  // namespace
+
+class EulerianFluidSolver3D {
+   public:
+    fs_params P;
+    fs_grid G;
+    FsLayout L;
+    FsGridConst cell, macx, macy, macz;
+    FsMaterial mat;
+    int device = 0;
+    hipStream_t st = nullptr;
+    hipEvent_t ev[4] = {};
+
+    // persistent fields (padded layout, FP64)
+    double *phi = nullptr, *sghost = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+    // one spare per component: each keeps its component's extents, so ghost
+    // cells of every velocity buffer stay zero across the swaps
+    double *spare_u = nullptr, *spare_v = nullptr, *spare_w = nullptr;
+    double *gu = nullptr, *gv = nullptr, *gw = nullptr, *p = nullptr, *div = nullptr, *rho = nullptr,
+           *irho = nullptr;
+    double *sgn = nullptr, *tmp = nullptr, *bx = nullptr, *by = nullptr, *bz = nullptr, *diag = nullptr,
+           *rhs = nullptr;
+    // PCG workspaces
+    FsPcgWork w64{}, w32{};
+    float *bx32 = nullptr, *by32 = nullptr, *bz32 = nullptr, *diag32 = nullptr, *e32 = nullptr, *r32 = nullptr;
+    FsPcgState *S64 = nullptr, *S32 = nullptr, *hS = nullptr;  // hS: pinned host mirror
+    double* scratch = nullptr;                                   // mean removal partials (nz+1) + pin (2)
+    unsigned long long* vmax = nullptr;                          // 4 words
+    unsigned long long* hvmax = nullptr;                         // pinned
+    double* staging = nullptr;                                   // host <-> device exchange
+    long staging_n = 0;
+    bool assembled = false;
+    std::string last_error;
+
+    // last-projection statistics
+    fs_step_stats stats{};
+    double inner_rtol = 1e-6;  // floor of the FP32 inner relative tolerance
+    int max_sweeps = 30;
+    int chunk = 16;
+
+    void* dalloc(size_t bytes) {
+        void* ptr = nullptr;
+        if (hipMalloc(&ptr, bytes) != hipSuccess) throw FsFail{FS_ERR_OOM, "hipMalloc failed"};
+        HIPCHECK(hipMemsetAsync(ptr, 0, bytes, st));
+        return ptr;
+    }
+    double* dfield() { return (double*)dalloc(sizeof(double) * (size_t)L.total); }
+    float* ffield() { return (float*)dalloc(sizeof(float) * (size_t)L.total); }
+
+    // EULERIAN_FLUID_SOLVER_3D::InitializeFromScriptBlock, :192-800
+    void InitializeFromScriptBlock(const fs_params& params, const fs_grid& grid, int dev) {
+        P = params;
+        G = grid;
+        if (G.nx < 3 || G.ny < 3 || G.nz < 3) throw FsFail{FS_ERR_BAD_ARG, "grid must be at least 3 cells per axis"};
+        if ((long)G.nx * G.ny * G.nz > (1L << 31)) throw FsFail{FS_ERR_BAD_ARG, "grid too large"};
+        if (P.surface_tension != 0.0)
+            throw FsFail{FS_ERR_UNSUPPORTED, "surface_tension != 0 (jump terms) is outside this build"};
+        if (P.order_for_time_advancing != 1)
+            throw FsFail{FS_ERR_UNSUPPORTED, "only order_for_time_advancing = 1 is implemented"};
+        if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE) throw FsFail{FS_ERR_BAD_ARG, "precision"};
+        if (P.scaling_number_for_reinitialization <= 0) throw FsFail{FS_ERR_BAD_ARG, "scaling_number"};
+        if (dev >= 0) HIPCHECK(hipSetDevice(dev));
+        HIPCHECK(hipGetDevice(&device));
+        HIPCHECK(hipStreamCreate(&st));
+        for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+        L = fs_make_layout(G.nx, G.ny, G.nz);
+        cell.x = fs_make_axis(G.x_min, G.x_max, G.nx);
+        cell.y = fs_make_axis(G.y_min, G.y_max, G.ny);
+        cell.z = fs_make_axis(G.z_min, G.z_max, G.nz);
+        // MAC grids, EULERIAN_FLUID_SOLVER_3D.h:407,415,423
+        macx = cell;
+        macx.x = fs_make_axis(G.x_min - (double)0.5 * cell.x.d, G.x_max + (double)0.5 * cell.x.d, G.nx + 1);
+        macy = cell;
+        macy.y = fs_make_axis(G.y_min - (double)0.5 * cell.y.d, G.y_max + (double)0.5 * cell.y.d, G.ny + 1);
+        macz = cell;
+        macz.z = fs_make_axis(G.z_min - (double)0.5 * cell.z.d, G.z_max + (double)0.5 * cell.z.d, G.nz + 1);
+        mat = FsMaterial{P.air_bubble_rising, P.water_density, P.air_density, P.water_viscosity, P.air_viscosity};
+
+        double** f64[] = {&phi, &sghost, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
+                          &bx, &by, &bz, &diag, &rhs};
+        for (double** f : f64) *f = dfield();
+        int nb = fs_pcg_blocks(L);
+        for (FsPcgWork* wk : {&w64}) {
+            wk->r = dfield();
+            wk->s = dfield();
+            wk->p = dfield();
+            wk->q = dfield();
+            wk->nblocks = nb;
+            wk->partials = (double*)dalloc(sizeof(double) * 3 * nb);
+        }
+        if (P.precision == FS_FP32_REFINE) {
+            bx32 = ffield();
+            by32 = ffield();
+            bz32 = ffield();
+            diag32 = ffield();
+            e32 = ffield();
+            r32 = ffield();
+            w32.r = ffield();
+            w32.s = ffield();
+            w32.p = ffield();
+            w32.q = ffield();
+            w32.nblocks = nb;
+            w32.partials = (double*)dalloc(sizeof(double) * 3 * nb);
+        }
+        S64 = (FsPcgState*)dalloc(sizeof(FsPcgState));
+        S32 = (FsPcgState*)dalloc(sizeof(FsPcgState));
+        HIPCHECK(hipHostMalloc((void**)&hS, sizeof(FsPcgState)));
+        scratch = (double*)dalloc(sizeof(double) * (G.nz + 8));
+        vmax = (unsigned long long*)dalloc(sizeof(unsigned long long) * 4);
+        HIPCHECK(hipHostMalloc((void**)&hvmax, sizeof(unsigned long long) * 4));
+        staging_n = (long)(G.nx + 1) * (G.ny + 1) * (G.nz + 1);
+        staging = (double*)dalloc(sizeof(double) * staging_n);
+        HIPCHECK(hipStreamSynchronize(st));
+    }
+
+    ~EulerianFluidSolver3D() {
+        if (st) (void)hipStreamSynchronize(st);
+        double* f64[] = {phi, sghost, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
+                         scratch, staging, w64.partials, w32.partials};
+        for (double* f : f64)
+            if (f) (void)hipFree(f);
+        void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32,
+                      S64, S32, vmax};
+        for (void* f : fv)
+            if (f) (void)hipFree(f);
+        if (hS) (void)hipHostFree(hS);
+        if (hvmax) (void)hipHostFree(hvmax);
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    FsBox box(int comp) const { return fs_box(L, comp); }
+
+    void sync() { HIPCHECK(hipStreamSynchronize(st)); }
+    void check_launch() { HIPCHECK(hipGetLastError()); }
+
+    // ---- field exchange ---------------------------------------------------
+    void put(double* dst, const double* host, int comp) {
+        FsBox b = box(comp);
+        size_t n = (size_t)b.ni * b.nj * b.nk;
+        HIPCHECK(hipMemcpyAsync(staging, host, n * sizeof(double), hipMemcpyHostToDevice, st));
+        fs_scatter(st, dst, staging, L, b);
+        sync();
+    }
+    void get(const double* src, double* host, int comp) {
+        FsBox b = box(comp);
+        size_t n = (size_t)b.ni * b.nj * b.nk;
+        fs_gather(st, staging, src, L, b);
+        HIPCHECK(hipMemcpyAsync(host, staging, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+    }
+
+    void SetFields(const double* hphi, const double* hu, const double* hv, const double* hw, const double* hp) {
+        if (hphi) {
+            put(phi, hphi, 0);
+            // level-set ghosts by constant extrapolation, EULERIAN_FLUID_SOLVER_3D.h:757
+            fs_fill_const(st, phi, phi, L, box(0), 2);
+        }
+        if (hu) put(u, hu, 1);
+        if (hv) put(v, hv, 2);
+        if (hw) put(w, hw, 3);
+        if (hp) put(p, hp, 0);
+        check_launch();
+        sync();
+    }
+    void GetFields(double* hphi, double* hu, double* hv, double* hw, double* hp) {
+        if (hphi) get(phi, hphi, 0);
+        if (hu) get(u, hu, 1);
+        if (hv) get(v, hv, 2);
+        if (hw) get(w, hw, 3);
+        if (hp) get(p, hp, 0);
+    }
+
+    // ---- phases -----------------------------------------------------------
+    // LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082: with zero
+    // surface tension the lasting effect is phi's ghost re-extrapolation (:998).
+    void ComputeCurvatures() { fs_fill_linear(st, phi, phi, L, box(0), 2, 1); }
+
+    // ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242
+    void Levelset_Advection(double dt) {
+        fs_weno5_stage(st, phi, sghost, gu, gv, gw, L, 0, cell, P.levelset_epsilon, dt);
+    }
+
+    // ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 (air/water)
+    void Velocity_Advection(double dt) {
+        if (P.use_5th_weno_v) {
+            fs_weno5_stage(st, u, gu, gu, gv, gw, L, 1, macx, P.velocity_epsilon, dt);
+            fs_weno5_stage(st, v, gu, gu, gv, gw, L, 2, macy, P.velocity_epsilon, dt);
+            fs_weno5_stage(st, w, gu, gu, gv, gw, L, 3, macz, P.velocity_epsilon, dt);
+        } else {
+            fs_eno3_stage(st, u, gu, gu, gv, gw, L, 1, macx, dt);
+            fs_eno3_stage(st, v, gv, gu, gv, gw, L, 2, macy, dt);
+            fs_eno3_stage(st, w, gw, gu, gv, gw, L, 3, macz, dt);
+        }
+    }
+
+    // ApplyGravity, EULERIAN_FLUID_SOLVER_3D.h:2237-2270
+    void ApplyGravity(double dt) {
+        fs_gravity(st, u, v, w, L, dt * P.gravity[0], dt * P.gravity[1], dt * P.gravity[2]);
+    }
+
+    // Sourcing -> VISCOSITY_3D::ApplyViscosity (air/water explicit), each
+    // component out of place into its spare buffer, then swapped in.
+    void Sourcing(double dt) {
+        if (!P.use_delta_function_formulation) return;
+        fs_viscosity(st, 1, spare_u, u, v, w, phi, L, mat, cell, dt);
+        std::swap(u, spare_u);
+        fs_viscosity(st, 2, spare_v, u, v, w, phi, L, mat, cell, dt);
+        std::swap(v, spare_v);
+        fs_viscosity(st, 3, spare_w, u, v, w, phi, L, mat, cell, dt);
+        std::swap(w, spare_w);
+    }
+
+    // PROJECTION_3D::Solve up to the linear system (:416-434, :596-834 and
+    // POISSON_SOLVER_3D::Solve :165-246).
+    void AssemblePressureSystem(double dt) {
+        double light = P.air_bubble_rising ? P.air_density : P.water_density;
+        double heavy = P.air_bubble_rising ? P.water_density : P.air_density;
+        fs_density(st, phi, rho, irho, L, light, heavy);
+        // DetermineDivergence: ghost copies of u, v, w with zero faces
+        FsBox bu = box(1), bv = box(2), bw = box(3);
+        fs_copy_box(st, gu, u, L, bu);
+        fs_fill_zero_faces(st, gu, L, bu, 3);
+        fs_copy_box(st, gv, v, L, bv);
+        fs_fill_zero_faces(st, gv, L, bv, 3);
+        fs_copy_box(st, gw, w, L, bw);
+        fs_fill_zero_faces(st, gw, L, bw, 3);
+        fs_divergence(st, gu, gv, gw, div, L, cell, 1 / dt);
+        // DetermineJumpConditionField's ComputeCurvatures refresh of phi's
+        // ghosts is overwritten by ComputeNormals below before anything reads
+        // it, so it is not repeated here.
+        if (!P.use_previous_pressure) fs_fill_value(st, p, L.total, 0.0);
+        fs_fill_const(st, phi, phi, L, box(0), 2);   // ComputeNormals, LEVELSET_3D.h:931
+        fs_fill_const(st, irho, irho, L, box(0), 1); // PROJECTION_3D.h:813
+        fs_assemble(st, phi, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
+        fs_remove_mean(st, rhs, scratch, L);
+        assembled = true;
+    }
+
+    FsPcgOp op64() const {
+        FsPcgOp o;
+        o.bx = bx;
+        o.by = by;
+        o.bz = bz;
+        o.idx2 = (double)1 / (cell.x.d * cell.x.d);
+        o.idy2 = (double)1 / (cell.y.d * cell.y.d);
+        o.idz2 = (double)1 / (cell.z.d * cell.z.d);
+        return o;
+    }
+
+    // Host copy of the solver configuration into a device state, keeping the
+    // pin extras computed on device.
+    void init_state(FsPcgState* dS, int stop_rule, double tol, double rtol, int max_iter) {
+        FsPcgState h{};
+        h.max_iter = max_iter;
+        h.stop_rule = stop_rule;
+        h.tol2 = tol * tol;
+        h.rtol2 = rtol * rtol;
+        HIPCHECK(hipMemcpyAsync(dS, &h, sizeof(h), hipMemcpyHostToDevice, st));
+        // pin extras written by the diagonal kernel into S64 (see PinExtras)
+        HIPCHECK(hipMemcpyAsync(&dS->pin_b, &S64pin()[0], 2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        sync();
+    }
+    double* S64pin() { return scratch + G.nz + 2; }
+
+    void read_state(FsPcgState* dS) {
+        HIPCHECK(hipMemcpyAsync(hS, dS, sizeof(FsPcgState), hipMemcpyDeviceToHost, st));
+        sync();
+    }
+
+    // Jacobi-PCG to completion; returns reference-style iteration count.
+    int run_pcg64(int stop_rule, double tol, double rtol, int max_iter, double* x) {
+        FsPcgOp o = op64();
+        init_state(S64, stop_rule, tol, rtol, max_iter);
+        fs_pcg64_begin(st, L, o, rhs, diag, x, w64, S64);
+        read_state(S64);
+        while (!hS->done) {
+            fs_pcg64_iterations(st, L, o, diag, x, w64, S64, chunk);
+            check_launch();
+            read_state(S64);
+        }
+        stats.residual_precond = std::sqrt(hS->rz);
+        return hS->iter;
+    }
+
+    int run_pcg32(double rtol, int max_iter) {
+        FsPcgOp o;
+        o.bx = bx32;
+        o.by = by32;
+        o.bz = bz32;
+        o.idx2 = (double)1 / (cell.x.d * cell.x.d);
+        o.idy2 = (double)1 / (cell.y.d * cell.y.d);
+        o.idz2 = (double)1 / (cell.z.d * cell.z.d);
+        init_state(S32, FS_STOP_REL_RESIDUAL, 0, rtol, max_iter);
+        HIPCHECK(hipMemsetAsync(e32, 0, sizeof(float) * L.total, st));
+        fs_pcg32_begin(st, L, o, r32, diag32, e32, w32, S32);
+        read_state(S32);
+        while (!hS->done) {
+            fs_pcg32_iterations(st, L, o, diag32, e32, w32, S32, chunk);
+            check_launch();
+            read_state(S32);
+        }
+        return hS->iter + (hS->done == 1 ? 1 : 0);
+    }
+
+    // Pressure solve on p (warm start), FP64 or FP32-inner / FP64-refined.
+    void SolvePressure() {
+        FsPcgOp o = op64();
+        if (P.precision == FS_FP64) {
+            stats.pcg_iterations = run_pcg64(P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, p);
+            stats.refinement_sweeps = 0;
+            bool converged = hS->done == 1;
+            fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32 ? r32 : (float*)w64.q, w64, S64);
+            read_state(S64);
+            stats.residual_rel = hS->bb > 0 ? std::sqrt(hS->rr / hS->bb) : 0;
+            if (!converged) throw FsFail{FS_ERR_NOT_CONVERGED, "PCG reached max_iteration"};
+            return;
+        }
+        // FP32 inner PCG inside FP64 iterative refinement: the outer residual
+        // b - A x, the correction update and all dot products stay FP64.
+        fs_make_op32(st, L, bx, by, bz, diag, bx32, by32, bz32, diag32);
+        init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
+        fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32, w64, S64);
+        read_state(S64);
+        double rr = hS->rr, bb = hS->bb;
+        double rt2 = P.rel_tolerance * P.rel_tolerance;
+        int sweeps = 0, inner = 0;
+        while (bb > 0 && rr > rt2 * bb && sweeps < max_sweeps && inner < P.max_iteration) {
+            // ask the inner solve for what is still missing (with margin),
+            // but not below what FP32 recursion reliably attains
+            double want = 0.3 * P.rel_tolerance * std::sqrt(bb / rr);
+            inner += run_pcg32(want > inner_rtol ? want : inner_rtol, P.max_iteration - inner);
+            fs_add_correction(st, L, p, e32);
+            fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32, w64, S64);
+            read_state(S64);
+            rr = hS->rr;
+            sweeps++;
+        }
+        stats.pcg_iterations = inner;
+        stats.refinement_sweeps = sweeps;
+        stats.residual_rel = bb > 0 ? std::sqrt(rr / bb) : 0;
+        stats.residual_precond = 0;
+        if (bb > 0 && rr > rt2 * bb) throw FsFail{FS_ERR_NOT_CONVERGED, "refinement did not reach rel_tolerance"};
+    }
+
+    // PROJECTION_3D::UpdateVelocity, :836-841 (z faces only in air/water
+    // mode; all faces in the fixed schedule).
+    void UpdateVelocity(double dt) {
+        HIPCHECK(hipMemsetAsync(vmax, 0, sizeof(unsigned long long) * 4, st));
+        int all = P.schedule == FS_SCHEDULE_FIXED;
+        fs_gradient_update(st, 1, all, u, p, phi, rho, L, mat, cell, dt, vmax + 0);
+        fs_gradient_update(st, 2, all, v, p, phi, rho, L, mat, cell, dt, vmax + 1);
+        fs_gradient_update(st, 3, 1, w, p, phi, rho, L, mat, cell, dt, vmax + 2);
+        HIPCHECK(hipMemcpyAsync(hvmax, vmax, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, st));
+        sync();
+        for (int c = 0; c < 3; c++) {
+            double m;
+            memcpy(&m, &hvmax[c], sizeof(double));
+            stats.max_velocity[c] = m;
+        }
+    }
+
+    // PROJECTION_3D::Solve, PROJECTION_3D.h:416-594
+    int Projection(double dt) {
+        HIPCHECK(hipEventRecord(ev[2], st));
+        AssemblePressureSystem(dt);
+        // pin extras of cell (0,0,0) for the operator kernels
+        int rc = FS_OK;
+        try {
+            SolvePressure();
+        } catch (FsFail& f) {
+            if (f.code != FS_ERR_NOT_CONVERGED) throw;
+            rc = f.code;
+            last_error = f.msg;
+        }
+        UpdateVelocity(dt);
+        HIPCHECK(hipEventRecord(ev[3], st));
+        sync();
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
+        stats.projection_ms = ms;
+        for (int c = 0; c < 3; c++)
+            if (!std::isfinite(stats.max_velocity[c])) throw FsFail{FS_ERR_NONFINITE, "non-finite velocity after projection"};
+        return rc;
+    }
+
+    // EULERIAN_FLUID_SOLVER_3D::Reinitialization, :2198-2225
+    void Reinitialization() {
+        if (!P.sussmanstyle_reinitialization) return;
+        double fictious_dt = cell.x.d / P.scaling_number_for_reinitialization;
+        fs_sign(st, phi, sgn, L, cell.x.d * cell.x.d);
+        for (int it = 0; it < P.iteration_number_for_reinitialization; it++) {
+            // ADVECTION_3D::ReinitializationBySussman, ADVECTION_3D.h:360-378
+            fs_copy_box(st, sghost, phi, L, box(0));
+            fs_fill_linear(st, sghost, phi, L, box(0), 3, 0);
+            fs_reinit_sweep(st, phi, sghost, sgn, L, cell, P.levelset_epsilon, fictious_dt);
+        }
+    }
+
+    // EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread, order 1, :806-858
+    int AdvanceOneTimeStepThread(double dt) {
+        int rc = FS_OK;
+        if (P.is_levelset_advection_active) {
+            fs_copy_box(st, tmp, phi, L, box(0));
+            Levelset_Advection(dt);
+            Levelset_Advection(dt);
+            fs_average(st, phi, tmp, L);
+        }
+        // SolveForRK, :1453-1476
+        if (P.is_velocity_advection_active) Velocity_Advection(dt);
+        if (P.is_external_force_active) ApplyGravity(dt);
+        if (P.is_sourcing_active) Sourcing(dt);
+        if (P.is_projection_active) rc = Projection(dt);
+        fs_copy_box(st, tmp, phi, L, box(0));
+        Reinitialization();
+        Reinitialization();
+        fs_average(st, phi, tmp, L);
+        check_launch();
+        return rc;
+    }
+
+    // EULERIAN_FLUID_SOLVER_3D::CFLOneTimeStep, :2673-2785
+    double CFLOneTimeStep(int gravity_norm) {
+        HIPCHECK(hipMemsetAsync(vmax, 0, sizeof(unsigned long long) * 4, st));
+        fs_abs_max(st, u, L, box(1), vmax + 0);
+        fs_abs_max(st, v, L, box(2), vmax + 1);
+        fs_abs_max(st, w, L, box(3), vmax + 2);
+        HIPCHECK(hipMemcpyAsync(hvmax, vmax, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, st));
+        sync();
+        double m[3];
+        for (int c = 0; c < 3; c++) memcpy(&m[c], &hvmax[c], sizeof(double));
+        for (double x : m)
+            if (!std::isfinite(x)) throw FsFail{FS_ERR_NONFINITE, "non-finite velocity"};
+        double c_f = m[0] * macx.x.inv + m[1] * macy.y.inv + m[2] * macz.z.inv;
+        double g_f = 0;
+        if (P.is_external_force_active) {
+            double g = gravity_norm ? std::sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] +
+                                                P.gravity[2] * P.gravity[2])
+                                    : std::fabs(P.gravity[1]);
+            g_f = std::sqrt(g * macy.y.inv);
+        }
+        double s_f = 0;
+        double a = P.water_viscosity / P.water_density, b = P.air_viscosity / P.air_density;
+        double v_f = (a > b ? a : b) * ((double)2 * cell.x.inv_d2 + (double)2 * cell.y.inv_d2 + (double)2 * cell.z.inv_d2);
+        double cfl = (double)0.5 * (c_f + v_f + std::sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) +
+                                                          (double)4 * (s_f * s_f)));
+        return (double)1 / cfl;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// C-ABI
+// ---------------------------------------------------------------------------
+
+struct fs_solver {
+    EulerianFluidSolver3D* impl = nullptr;
+    std::string err;
+};
+
+template <typename F>
+static int guarded(fs_solver* s, F&& f) {
+    if (!s || !s->impl) return FS_ERR_BAD_ARG;
+    try {
+        int rc = f(*s->impl);
+        if (rc != FS_OK) s->err = s->impl->last_error;
+        return rc;
+    } catch (FsFail& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (HipError& e) {
+        s->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return FS_ERR_HIP;
+    }
+}
+
+extern "C" {
+
+int fs_create(const fs_params* params, const fs_grid* grid, int device, fs_solver** out) {
+    if (!params || !grid || !out) return FS_ERR_BAD_ARG;
+    *out = nullptr;
+    auto* s = new fs_solver();
+    s->impl = new EulerianFluidSolver3D();
+    try {
+        s->impl->InitializeFromScriptBlock(*params, *grid, device);
+    } catch (FsFail& e) {
+        delete s->impl;
+        delete s;
+        return e.code;
+    } catch (HipError& e) {
+        delete s->impl;
+        delete s;
+        return e.e == hipErrorOutOfMemory ? FS_ERR_OOM : FS_ERR_HIP;
+    }
+    *out = s;
+    return FS_OK;
+}
+
+void fs_destroy(fs_solver* s) {
+    if (!s) return;
+    delete s->impl;
+    delete s;
+}
+
+const char* fs_last_error(const fs_solver* s) { return s ? s->err.c_str() : "null handle"; }
+
+int fs_set_fields(fs_solver* s, const double* phi, const double* u, const double* v, const double* w,
+                  const double* p) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.SetFields(phi, u, v, w, p);
+        return FS_OK;
+    });
+}
+
+int fs_get_fields(fs_solver* s, double* phi, double* u, double* v, double* w, double* p) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.GetFields(phi, u, v, w, p);
+        return FS_OK;
+    });
+}
+
+static int step_impl(EulerianFluidSolver3D& S, double dt, fs_step_stats* stats, bool frame) {
+    if (!(dt > 0) || !std::isfinite(dt)) throw FsFail{FS_ERR_BAD_ARG, "dt must be positive and finite"};
+    S.stats = fs_step_stats{};
+    HIPCHECK(hipEventRecord(S.ev[0], S.st));
+    if (frame) S.ComputeCurvatures();
+    int rc = S.AdvanceOneTimeStepThread(dt);
+    HIPCHECK(hipEventRecord(S.ev[1], S.st));
+    S.sync();
+    float ms = 0;
+    HIPCHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+    S.stats.step_ms = ms;
+    if (stats) *stats = S.stats;
+    return rc;
+}
+
+int fs_advance_frame(fs_solver* s, double dt, fs_step_stats* stats) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) { return step_impl(S, dt, stats, true); });
+}
+
+int fs_step(fs_solver* s, double dt, fs_step_stats* stats) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) { return step_impl(S, dt, stats, false); });
+}
+
+int fs_cfl_dt(fs_solver* s, double cfl, int gravity_norm, double* dt) {
+    if (!dt || !(cfl > 0)) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        *dt = cfl * S.CFLOneTimeStep(gravity_norm);
+        return FS_OK;
+    });
+}
+
+int fs_compute_curvatures(fs_solver* s) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.ComputeCurvatures();
+        S.check_launch();
+        S.sync();
+        return FS_OK;
+    });
+}
+
+#define PHASE(name, call)                                                              \
+    int name(fs_solver* s, double dt) {                                                \
+        if (!(dt > 0)) return FS_ERR_BAD_ARG;                                          \
+        return guarded(s, [&](EulerianFluidSolver3D& S) {                              \
+            call;                                                                      \
+            S.check_launch();                                                          \
+            S.sync();                                                                  \
+            return FS_OK;                                                              \
+        });                                                                            \
+    }
+
+PHASE(fs_levelset_advection, S.Levelset_Advection(dt))
+PHASE(fs_velocity_advection, S.Velocity_Advection(dt))
+PHASE(fs_apply_gravity, S.ApplyGravity(dt))
+PHASE(fs_sourcing, S.Sourcing(dt))
+PHASE(fs_reinitialization, S.Reinitialization())
+
+int fs_projection(fs_solver* s, double dt, fs_step_stats* stats) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.stats = fs_step_stats{};
+        int rc = S.Projection(dt);
+        if (stats) *stats = S.stats;
+        return rc;
+    });
+}
+
+int fs_assemble_pressure_system(fs_solver* s, double dt, double* diag, double* bxh, double* byh, double* bzh,
+                                double* rhs) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.AssemblePressureSystem(dt);
+        S.check_launch();
+        if (diag) S.get(S.diag, diag, 0);
+        if (bxh) S.get(S.bx, bxh, 1);
+        if (byh) S.get(S.by, byh, 2);
+        if (bzh) S.get(S.bz, bzh, 3);
+        if (rhs) S.get(S.rhs, rhs, 0);
+        return FS_OK;
+    });
+}
+
+int fs_apply_pressure_operator(fs_solver* s, const double* x, double* q) {
+    if (!x || !q) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        if (!S.assembled) throw FsFail{FS_ERR_BAD_ARG, "assemble the pressure system first"};
+        FsBox b = S.box(0);
+        double* xd = (double*)S.w64.r;
+        double* qd = (double*)S.w64.q;
+        S.put(xd, x, 0);
+        S.init_state(S.S64, 0, 0, 0, 0);
+        fs_apply_op64(S.st, S.L, S.op64(), xd, qd, S.S64);
+        S.check_launch();
+        S.get(qd, q, 0);
+        (void)b;
+        return FS_OK;
+    });
+}
+
+int fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms) {
+    if (!avg_ms || reps <= 0 || (precision != FS_FP64 && precision != FS_FP32_REFINE)) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        if (!S.assembled) throw FsFail{FS_ERR_BAD_ARG, "no assembled pressure system"};
+        if (precision == FS_FP32_REFINE && !S.bx32) throw FsFail{FS_ERR_BAD_ARG, "handle has no FP32 workspace"};
+        // a state that never stops, with the iteration's scalars finite
+        FsPcgState* dS = precision == FS_FP64 ? S.S64 : S.S32;
+        S.init_state(dS, FS_STOP_ABS_PRECOND, 0, 0, 1 << 30);
+        FsPcgState h{};
+        HIPCHECK(hipMemcpy(&h, dS, sizeof(h), hipMemcpyDeviceToHost));
+        h.rz = 1;
+        h.alpha = 0;
+        h.beta = 0;
+        HIPCHECK(hipMemcpy(dS, &h, sizeof(h), hipMemcpyHostToDevice));
+        FsPcgOp o = S.op64();
+        FsPcgWork* w = &S.w64;
+        const void* dg = S.diag;
+        void* x = S.tmp;  // scratch x (tmp only lives inside a step): the pressure is left untouched
+        if (precision == FS_FP32_REFINE) {
+            o.bx = S.bx32;
+            o.by = S.by32;
+            o.bz = S.bz32;
+            w = &S.w32;
+            dg = S.diag32;
+            x = S.e32;
+        }
+        for (int which = 0; which < 3; which++) {
+            fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which);  // warm
+            HIPCHECK(hipEventRecord(S.ev[0], S.st));
+            for (int r = 0; r < reps; r++)
+                fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which);
+            HIPCHECK(hipEventRecord(S.ev[1], S.st));
+            S.check_launch();
+            S.sync();
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+            avg_ms[which] = ms / reps;
+        }
+        return FS_OK;
+    });
+}
+
+}  // extern "C"

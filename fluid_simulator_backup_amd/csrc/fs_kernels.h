@@ -1,0 +1,58 @@
+// fs_kernels.h -- launchers of the step kernels (host-callable, internal).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "fs_device.h"
+#include "fs_layout.h"
+
+// Box extents of a component: 0 cell, 1 x-faces, 2 y-faces, 3 z-faces.
+struct FsBox {
+    int ni, nj, nk;
+};
+inline FsBox fs_box(const FsLayout& L, int comp) {
+    return FsBox{L.nx + (comp == 1), L.ny + (comp == 2), L.nz + (comp == 3)};
+}
+
+// ---- ghost fills (FIELD_STRUCTURE_3D.h) ---------------------------------
+void fs_copy_box(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b);
+void fs_fill_zero_faces(hipStream_t st, double* dst, const FsLayout& L, FsBox b, int depth);
+void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth);
+void fs_fill_linear(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth, int edges);
+
+// ---- advection ------------------------------------------------------------
+void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
+                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt);
+void fs_eno3_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
+                   const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double dt);
+
+// ---- forces ---------------------------------------------------------------
+void fs_gravity(hipStream_t st, double* u, double* v, double* w, const FsLayout& L, double gx, double gy, double gz);
+void fs_viscosity(hipStream_t st, int comp, double* dst, const double* U, const double* V, const double* W,
+                  const double* phi, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt);
+
+// ---- projection -----------------------------------------------------------
+void fs_density(hipStream_t st, const double* phi, double* rho, double* irho, const FsLayout& L, double light,
+                double heavy);
+void fs_divergence(hipStream_t st, const double* gu, const double* gv, const double* gw, double* div,
+                   const FsLayout& L, const FsGridConst& cell, double one_over_dt);
+// beta of every face between two interior cells (0 elsewhere), A(c,c) and
+// b = -div.  pin[0..1]: the first cell's Neumann extras beta_b, beta_d.
+void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, double* bx, double* by,
+                 double* bz, double* diag, double* rhs, double* pin, const FsLayout& L, const FsGridConst& cell);
+// b -= mean(b) with the per-plane row-order sum; scratch: nz+1 doubles.
+void fs_remove_mean(hipStream_t st, double* rhs, double* scratch, const FsLayout& L);
+void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,
+                        const double* rho, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
+                        double dt, unsigned long long* vmax_bits);
+void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, unsigned long long* bits);
+
+// ---- reinitialization -----------------------------------------------------
+void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, double dx2);
+void fs_reinit_sweep(hipStream_t st, double* phi, const double* ghost, const double* sgn, const FsLayout& L,
+                     const FsGridConst& cell, double eps, double dt);
+
+// ---- misc -----------------------------------------------------------------
+void fs_average(hipStream_t st, double* phi, const double* tmp, const FsLayout& L);  // phi = 0.5*(tmp+phi)
+void fs_scatter(hipStream_t st, double* dst, const double* lin, const FsLayout& L, FsBox b);
+void fs_gather(hipStream_t st, double* lin, const double* src, const FsLayout& L, FsBox b);
+void fs_fill_value(hipStream_t st, double* dst, long n, double v);

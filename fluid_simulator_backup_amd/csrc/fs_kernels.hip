@@ -1,0 +1,625 @@
+// fs_kernels.hip -- step kernels for gfx950 (FP64).
+//
+// One thread per cell/face, 64-wide x rows (one wavefront per x row segment,
+// coalesced loads), 4 rows per workgroup.  Stencils read the padded layout
+// directly; neighbours along y and z come through L2.
+#include "fs_kernels.h"
+
+#define BX 64
+#define BY 4
+
+static inline dim3 grid_for(FsBox b) { return dim3((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, b.nk); }
+
+// ---------------------------------------------------------------------------
+// ghost fills
+// ---------------------------------------------------------------------------
+
+__global__ void k_copy_box(double* __restrict__ dst, const double* __restrict__ src, FsLayout L, FsBox b) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    long c = L.idx(i, j, k);
+    dst[c] = src[c];
+}
+
+void fs_copy_box(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b) {
+    hipLaunchKernelGGL(k_copy_box, grid_for(b), dim3(BX, BY), 0, st, dst, src, L, b);
+}
+
+// FillGhostCellsZero face blocks, FIELD_STRUCTURE_3D.h:567-660.
+__global__ void k_zero_faces(double* __restrict__ dst, FsLayout L, FsBox b, int d, int axis) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (axis == 0) {
+        if (a >= b.nj) return;
+        for (int g = 1; g <= d; g++) {
+            dst[L.idx(-g, a, c)] = 0;
+            dst[L.idx(b.ni - 1 + g, a, c)] = 0;
+        }
+    } else if (axis == 1) {
+        if (a >= b.ni) return;
+        for (int g = 1; g <= d; g++) {
+            dst[L.idx(a, -g, c)] = 0;
+            dst[L.idx(a, b.nj - 1 + g, c)] = 0;
+        }
+    } else {
+        if (a >= b.ni) return;
+        for (int g = 1; g <= d; g++) {
+            dst[L.idx(a, c, -g)] = 0;
+            dst[L.idx(a, c, b.nk - 1 + g)] = 0;
+        }
+    }
+}
+
+void fs_fill_zero_faces(hipStream_t st, double* dst, const FsLayout& L, FsBox b, int depth) {
+    hipLaunchKernelGGL(k_zero_faces, dim3((b.nj + 63) / 64, b.nk), dim3(64), 0, st, dst, L, b, depth, 0);
+    hipLaunchKernelGGL(k_zero_faces, dim3((b.ni + 63) / 64, b.nk), dim3(64), 0, st, dst, L, b, depth, 1);
+    hipLaunchKernelGGL(k_zero_faces, dim3((b.ni + 63) / 64, b.nj), dim3(64), 0, st, dst, L, b, depth, 2);
+}
+
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+// FillGhostCellsFrom, FIELD_STRUCTURE_3D.h:663-1086: every ghost cell of the
+// depth-d shell takes its clamped interior value (reads interior only).
+__global__ void k_fill_const(double* dst, const double* src, FsLayout L, FsBox b, int d) {
+    int jj = blockIdx.x * blockDim.x + threadIdx.x - d, kk = blockIdx.y - d;
+    if (jj >= b.nj + d) return;
+    bool row_inside = jj >= 0 && jj < b.nj && kk >= 0 && kk < b.nk;
+    int jc = clampi(jj, 0, b.nj - 1), kc = clampi(kk, 0, b.nk - 1);
+    if (row_inside) {
+        for (int g = 1; g <= d; g++) {
+            dst[L.idx(-g, jj, kk)] = src[L.idx(0, jc, kc)];
+            dst[L.idx(b.ni - 1 + g, jj, kk)] = src[L.idx(b.ni - 1, jc, kc)];
+        }
+    } else {
+        for (int ii = -d; ii < b.ni + d; ii++) dst[L.idx(ii, jj, kk)] = src[L.idx(clampi(ii, 0, b.ni - 1), jc, kc)];
+    }
+}
+
+void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth) {
+    hipLaunchKernelGGL(k_fill_const, dim3((b.nj + 2 * depth + 63) / 64, b.nk + 2 * depth), dim3(64), 0, st, dst, src,
+                       L, b, depth);
+}
+
+// FillGhostCellsContinuousDerivativesFrom face blocks,
+// FIELD_STRUCTURE_3D.h:1806-1894: 2a-b outwards, deeper layers reading the
+// layers just written when src == dst (one thread walks its own row).
+__global__ void k_linear_faces(double* dst, const double* src, FsLayout L, FsBox b, int d, int axis) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (axis == 0) {
+        if (a >= b.nj) return;
+        for (int i = -1; i >= -d; i--) dst[L.idx(i, a, c)] = (double)2 * src[L.idx(i + 1, a, c)] - src[L.idx(i + 2, a, c)];
+        for (int i = b.ni; i <= b.ni - 1 + d; i++)
+            dst[L.idx(i, a, c)] = (double)2 * src[L.idx(i - 1, a, c)] - src[L.idx(i - 2, a, c)];
+    } else if (axis == 1) {
+        if (a >= b.ni) return;
+        for (int j = -1; j >= -d; j--) dst[L.idx(a, j, c)] = (double)2 * src[L.idx(a, j + 1, c)] - src[L.idx(a, j + 2, c)];
+        for (int j = b.nj; j <= b.nj - 1 + d; j++)
+            dst[L.idx(a, j, c)] = (double)2 * src[L.idx(a, j - 1, c)] - src[L.idx(a, j - 2, c)];
+    } else {
+        if (a >= b.ni) return;
+        for (int k = -1; k >= -d; k--) dst[L.idx(a, c, k)] = (double)2 * src[L.idx(a, c, k + 1)] - src[L.idx(a, c, k + 2)];
+        for (int k = b.nk; k <= b.nk - 1 + d; k++)
+            dst[L.idx(a, c, k)] = (double)2 * src[L.idx(a, c, k - 1)] - src[L.idx(a, c, k - 2)];
+    }
+}
+
+// The three filled edge blocks normal to z (FIELD_STRUCTURE_3D.h:1896-1946),
+// in the reference's loop order; one thread per z-plane.
+__global__ void k_linear_edges(double* dst, const double* src, FsLayout L, FsBox b, int d) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= b.nk) return;
+    for (int j = -1; j >= -d; j--)
+        for (int i = -1; i >= -d; i--)
+            dst[L.idx(i, j, k)] = (double)0.5 * (((double)2 * src[L.idx(i + 1, j, k)] - src[L.idx(i + 2, j, k)]) +
+                                                 ((double)2 * src[L.idx(i, j + 1, k)] - src[L.idx(i, j + 2, k)]));
+    for (int j = b.nj; j <= b.nj - 1 + d; j++)
+        for (int i = b.ni; i <= b.ni - 1 + d; i++)
+            dst[L.idx(i, j, k)] = (double)0.5 * (((double)2 * src[L.idx(i - 1, j, k)] - src[L.idx(i - 2, j, k)]) +
+                                                 ((double)2 * src[L.idx(i, j - 1, k)] - src[L.idx(i, j - 2, k)]));
+    for (int j = b.nj; j <= b.nj - 1 + d; j++)
+        for (int i = -1; i >= -d; i--)
+            dst[L.idx(i, j, k)] = (double)0.5 * (((double)2 * src[L.idx(i + 1, j, k)] - src[L.idx(i + 2, j, k)]) +
+                                                 ((double)2 * src[L.idx(i, j - 1, k)] - src[L.idx(i, j - 2, k)]));
+}
+
+void fs_fill_linear(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth,
+                    int edges) {
+    hipLaunchKernelGGL(k_linear_faces, dim3((b.nj + 63) / 64, b.nk), dim3(64), 0, st, dst, src, L, b, depth, 0);
+    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nk), dim3(64), 0, st, dst, src, L, b, depth, 1);
+    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nj), dim3(64), 0, st, dst, src, L, b, depth, 2);
+    if (edges) hipLaunchKernelGGL(k_linear_edges, dim3((b.nk + 63) / 64), dim3(64), 0, st, dst, src, L, b, depth);
+}
+
+// ---------------------------------------------------------------------------
+// advection (ADVECTION_METHOD_3D.h)
+// ---------------------------------------------------------------------------
+
+// Velocity at the sample point from the MAC ghost copies,
+// ADVECTION_METHOD_3D.h:201-224 (same for ENO3rd, :1862-1879).
+__device__ __forceinline__ void vel_at(const double* __restrict__ gu, const double* __restrict__ gv,
+                                       const double* __restrict__ gw, const FsLayout& L, int comp, int i, int j,
+                                       int k, double& uu, double& vv, double& ww) {
+    long c = L.idx(i, j, k);
+    const long sx = L.sx, sy = L.sy;
+    if (comp == 0) {
+        uu = (gu[c + 1] + gu[c]) * (double)0.5;
+        vv = (gv[c + sx] + gv[c]) * (double)0.5;
+        ww = (gw[c + sy] + gw[c]) * (double)0.5;
+    } else if (comp == 1) {
+        uu = gu[c];
+        vv = (gv[c] + gv[c + sx] + gv[c - 1] + gv[c - 1 + sx]) * (double)0.25;
+        ww = (gw[c] + gw[c + sy] + gw[c - 1] + gw[c - 1 + sy]) * (double)0.25;
+    } else if (comp == 2) {
+        uu = (gu[c] + gu[c + 1] + gu[c - sx] + gu[c + 1 - sx]) * (double)0.25;
+        vv = gv[c];
+        ww = (gw[c] + gw[c + sy] + gw[c - sx] + gw[c - sx + sy]) * (double)0.25;
+    } else {
+        uu = (gu[c] + gu[c + 1] + gu[c - sy] + gu[c + 1 - sy]) * (double)0.25;
+        vv = (gv[c] + gv[c + sx] + gv[c - sy] + gv[c + sx - sy]) * (double)0.25;
+        ww = gw[c];
+    }
+}
+
+__device__ __forceinline__ void gather7(const double* __restrict__ a, long c, long stride, double* f) {
+#pragma unroll
+    for (int o = -3; o <= 3; o++) f[o + 3] = a[c + o * stride];
+}
+
+__device__ __forceinline__ double upwind(double rho, double dt, double uu, double vv, double ww, double mx, double px,
+                                         double my, double py, double mz, double pz) {
+    double X = uu > 0 ? mx : px;
+    double Y = vv > 0 ? my : py;
+    double Z = ww > 0 ? mz : pz;
+    return rho - dt * (uu * X + vv * Y + ww * Z);
+}
+
+__global__ void __launch_bounds__(BX* BY) k_weno5_stage(double* __restrict__ rho, const double* __restrict__ gh,
+                                                        const double* __restrict__ gu, const double* __restrict__ gv,
+                                                        const double* __restrict__ gw, FsLayout L, FsBox b, int comp,
+                                                        double ix, double iy, double iz, double eps, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    long c = L.idx(i, j, k);
+    double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+    gather7(gh, c, 1, f);
+    fs_weno5(f + 3, ix, eps, mx, px);
+    gather7(gh, c, L.sx, f);
+    fs_weno5(f + 3, iy, eps, my, py);
+    gather7(gh, c, L.sy, f);
+    fs_weno5(f + 3, iz, eps, mz, pz);
+    vel_at(gu, gv, gw, L, comp, i, j, k, uu, vv, ww);
+    rho[c] = upwind(rho[c], dt, uu, vv, ww, mx, px, my, py, mz, pz);
+}
+
+static const FsGridConst& gc_dummy(const FsGridConst& g) { return g; }
+
+void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
+                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt) {
+    FsBox b = fs_box(L, comp);
+    hipLaunchKernelGGL(k_weno5_stage, grid_for(b), dim3(BX, BY), 0, st, rho, ghost, gu, gv, gw, L, b, comp, gc.x.inv,
+                       gc.y.inv, gc.z.inv, eps, dt);
+    (void)gc_dummy;
+}
+
+__global__ void __launch_bounds__(BX* BY) k_eno3_stage(double* __restrict__ rho, const double* __restrict__ gh,
+                                                       const double* __restrict__ gu, const double* __restrict__ gv,
+                                                       const double* __restrict__ gw, FsLayout L, FsBox b, int comp,
+                                                       FsGridConst gc, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    long c = L.idx(i, j, k);
+    // one_over_3dz is built from one_over_dx (ADVECTION_METHOD_3D.h:1649)
+    double inv3x = gc.x.inv * (double)1 / 3, inv3y = gc.y.inv * (double)1 / 3, inv3z = gc.x.inv * (double)1 / 3;
+    double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+    gather7(gh, c, 1, f);
+    fs_eno3(f + 3, gc.x.d, gc.x.inv, gc.x.inv2d, inv3x, mx, px);
+    gather7(gh, c, L.sx, f);
+    fs_eno3(f + 3, gc.y.d, gc.y.inv, gc.y.inv2d, inv3y, my, py);
+    gather7(gh, c, L.sy, f);
+    fs_eno3(f + 3, gc.z.d, gc.z.inv, gc.z.inv2d, inv3z, mz, pz);
+    vel_at(gu, gv, gw, L, comp, i, j, k, uu, vv, ww);
+    rho[c] = upwind(rho[c], dt, uu, vv, ww, mx, px, my, py, mz, pz);
+}
+
+void fs_eno3_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
+                   const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double dt) {
+    FsBox b = fs_box(L, comp);
+    hipLaunchKernelGGL(k_eno3_stage, grid_for(b), dim3(BX, BY), 0, st, rho, ghost, gu, gv, gw, L, b, comp, gc, dt);
+}
+
+// ---------------------------------------------------------------------------
+// forces
+// ---------------------------------------------------------------------------
+
+// ApplyGravity, EULERIAN_FLUID_SOLVER_3D.h:2237-2270: every face.
+__global__ void k_gravity(double* u, double* v, double* w, FsLayout L, double gx, double gy, double gz) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i > L.nx || j > L.ny) return;
+    long c = L.idx(i, j, k);
+    if (j < L.ny && k < L.nz) u[c] += gx;
+    if (i < L.nx && k < L.nz) v[c] += gy;
+    if (i < L.nx && j < L.ny) w[c] += gz;
+}
+
+void fs_gravity(hipStream_t st, double* u, double* v, double* w, const FsLayout& L, double gx, double gy, double gz) {
+    FsBox b{L.nx + 1, L.ny + 1, L.nz + 1};
+    hipLaunchKernelGGL(k_gravity, grid_for(b), dim3(BX, BY), 0, st, u, v, w, L, gx, gy, gz);
+}
+
+// VISCOSITY_3D::ApplyViscosity, air/water explicit branch
+// (VISCOSITY_3D.h:452-563), one component, out of place: dst = src + update,
+// all reads from U, V, W (the component being updated is read from its
+// pre-sweep copy).  The smoothed density/viscosity fields of the reference
+// (:1258-1347) are evaluated at the face on the fly.
+__global__ void __launch_bounds__(BX* BY) k_viscosity(int comp, double* __restrict__ dst, const double* __restrict__ U,
+                                                      const double* __restrict__ V, const double* __restrict__ W,
+                                                      const double* __restrict__ f, FsLayout L, FsBox b, FsMaterial m,
+                                                      double eps, double odx, double ody, double odz, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    const long c = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+#define MU(x) m.viscosity(fs_heaviside_visc((x), eps))
+    double upd, self;
+    if (comp == 1) {
+        double lh = (double)0.5 * (f[c - 1] + f[c]);
+        double lcu = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 + sx] + f[c + sx]);
+        double lcd = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 - sx] + f[c - sx]);
+        double lct = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 + sy] + f[c + sy]);
+        double lcb = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 - sy] + f[c - sy]);
+        double rho_h = m.density(fs_heaviside_visc(lh, eps));
+        double mcu = MU(lcu), mcd = MU(lcd), mr = MU(f[c]), mc = MU(f[c - 1]), mct = MU(lct), mcb = MU(lcb);
+        double coef = dt * ((double)1 / rho_h);
+        double first = (double)2 * coef * odx * (mr * odx * (U[c + 1] - U[c]) - mc * ody * (U[c] - U[c - 1]));
+        double second = coef * ody * (mcu * (ody * (U[c + sx] - U[c]) + odx * (V[c + sx] - V[c - 1 + sx])) -
+                                      mcd * (ody * (U[c] - U[c - sx]) + odx * (V[c] - V[c - 1])));
+        double third = coef * odz * (mct * (odz * (U[c + sy] - U[c]) + odx * (W[c + sy] - W[c - 1 + sy])) -
+                                     mcb * (odz * (U[c] - U[c - sy]) + odx * (W[c] - W[c - 1])));
+        upd = first + second + third;
+        self = U[c];
+    } else if (comp == 2) {
+        double lh = (double)0.5 * (f[c - sx] + f[c]);
+        double lcu = (double)0.25 * (f[c - sx] + f[c + 1 - sx] + f[c] + f[c + 1]);
+        double lcl = (double)0.25 * (f[c - 1 - sx] + f[c - sx] + f[c - 1] + f[c]);
+        double lct = (double)0.25 * (f[c - sx + sy] + f[c - sx] + f[c + sy] + f[c]);
+        double lcb = (double)0.25 * (f[c - sx - sy] + f[c - sx] + f[c - sy] + f[c]);
+        double rho_h = m.density(fs_heaviside_visc(lh, eps));
+        double mcu = MU(lcu), mcl = MU(lcl), mu_ = MU(f[c]), mc = MU(f[c - sx]), mct = MU(lct), mcb = MU(lcb);
+        double coef = dt * ((double)1 / rho_h);
+        double first = coef * odx * (mcu * (ody * (U[c + 1] - U[c + 1 - sx]) + odx * (V[c + 1] - V[c])) -
+                                     mcl * (ody * (U[c] - U[c - sx]) + odx * (V[c] - V[c - 1])));
+        double second = (double)2 * coef * ody * (mu_ * ody * (V[c + sx] - V[c]) - mc * ody * (V[c] - V[c - sx]));
+        double third = coef * odz * (mct * (odz * (V[c + sy] - V[c]) + ody * (W[c] - W[c - sx])) -
+                                     mcb * (odz * (V[c] - V[c - sy]) + ody * (W[c - sy] - W[c - sx - sy])));
+        upd = first + second + third;
+        self = V[c];
+    } else {
+        double lh = (double)0.5 * (f[c - sy] + f[c]);
+        double lct = (double)0.25 * (f[c - sy] + f[c + 1 - sy] + f[c] + f[c + 1]);
+        double lcb = (double)0.25 * (f[c - sy] + f[c - 1 - sy] + f[c] + f[c - 1]);
+        double lcu = (double)0.25 * (f[c - sy] + f[c + sx - sy] + f[c] + f[c + sx]);
+        double lcd = (double)0.25 * (f[c - sy] + f[c - sx - sy] + f[c] + f[c - sx]);
+        double rho_h = m.density(fs_heaviside_visc(lh, eps));
+        double mct = MU(lct), mcb = MU(lcb), mcu = MU(lcu), mcd = MU(lcd), mt = MU(f[c]), mc = MU(f[c - sy]);
+        double coef = dt * ((double)1 / rho_h);
+        double first = coef * odx * (mct * (odz * (U[c] - U[c - sy]) + odx * (W[c + 1] - W[c])) -
+                                     mcb * (odz * (U[c - 1] - U[c - 1 - sy]) + odx * (W[c] - W[c - 1])));
+        double second = coef * ody * (mcu * (odz * (V[c] - V[c - sy]) + ody * (W[c + sx] - W[c])) -
+                                      mcd * (odz * (V[c - sx] - V[c - sx - sy]) + ody * (W[c] - W[c - sx])));
+        double third = (double)2 * coef * odz * (mt * odz * (W[c + sy] - W[c]) - mc * odz * (W[c] - W[c - sy]));
+        upd = first + second + third;
+        self = W[c];
+    }
+#undef MU
+    dst[c] = self + upd;
+}
+
+void fs_viscosity(hipStream_t st, int comp, double* dst, const double* U, const double* V, const double* W,
+                  const double* phi, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt) {
+    FsBox b = fs_box(L, comp);
+    double eps = 1.5 * cell.x.d;  // epsilon_for_mollification, EULERIAN_FLUID_SOLVER_3D.h:393
+    hipLaunchKernelGGL(k_viscosity, grid_for(b), dim3(BX, BY), 0, st, comp, dst, U, V, W, phi, L, b, m, eps,
+                       cell.x.inv, cell.y.inv, cell.z.inv, dt);
+}
+
+// ---------------------------------------------------------------------------
+// projection (PROJECTION_3D.h, POISSON_SOLVER_3D.h)
+// ---------------------------------------------------------------------------
+
+// DetermineProjectionDensity, PROJECTION_3D.h:596-679 (bit-exact sign test).
+__global__ void k_density(const double* __restrict__ phi, double* __restrict__ rho, double* __restrict__ irho,
+                          FsLayout L, double light, double heavy) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    double f = phi[c], r = rho[c];
+    if (f <= 0) r = light;
+    else if (f > 0) r = heavy;
+    rho[c] = r;
+    irho[c] = 1 / r;
+}
+
+void fs_density(hipStream_t st, const double* phi, double* rho, double* irho, const FsLayout& L, double light,
+                double heavy) {
+    hipLaunchKernelGGL(k_density, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, rho, irho, L, light, heavy);
+}
+
+// DetermineDivergence, PROJECTION_3D.h:690-725.
+__global__ void k_divergence(const double* __restrict__ gu, const double* __restrict__ gv,
+                             const double* __restrict__ gw, double* __restrict__ div, FsLayout L, double odx,
+                             double ody, double odz, double one_over_dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    double d = odx * (gu[c + 1] - gu[c]) + ody * (gv[c + L.sx] - gv[c]) + odz * (gw[c + L.sy] - gw[c]);
+    div[c] = d * one_over_dt;
+}
+
+void fs_divergence(hipStream_t st, const double* gu, const double* gv, const double* gw, double* div,
+                   const FsLayout& L, const FsGridConst& cell, double one_over_dt) {
+    hipLaunchKernelGGL(k_divergence, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, gu, gv, gw, div, L, cell.x.inv,
+                       cell.y.inv, cell.z.inv, one_over_dt);
+}
+
+// GFM face coefficient of BuildLinearSystemNodeJumpConditionVaribleCoefficient,
+// POISSON_SOLVER_3D.h:1080-1085, written from the cell on the + side (beta_l
+// form).  The - side cell's beta_r of the same face evaluates the same
+// products and sums, so both rows see one value.
+__device__ __forceinline__ double gfm_beta(double vc, double vn, double fn, double fc) {
+    return vc * vn * (fabs(fn) + fabs(fc)) / (vc * fabs(fn) + vn * fabs(fc));
+}
+
+// Faces between two interior (full) cells get beta, every other face 0.
+__global__ void k_beta(const double* __restrict__ f, const double* __restrict__ v, double* __restrict__ bx,
+                       double* __restrict__ by, double* __restrict__ bz, FsLayout L) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i > L.nx || j > L.ny) return;
+    long c = L.idx(i, j, k);
+    bool in_c = i < L.nx && j < L.ny && k < L.nz;
+    if (j < L.ny && k < L.nz) bx[c] = (in_c && i > 0) ? gfm_beta(v[c], v[c - 1], f[c - 1], f[c]) : 0.0;
+    if (i < L.nx && k < L.nz) by[c] = (in_c && j > 0) ? gfm_beta(v[c], v[c - L.sx], f[c - L.sx], f[c]) : 0.0;
+    if (i < L.nx && j < L.ny) bz[c] = (in_c && k > 0) ? gfm_beta(v[c], v[c - L.sy], f[c - L.sy], f[c]) : 0.0;
+}
+
+// A(c,c) = inv_dxdx * coef (:1119-1213) with the Neumann pin on the first
+// full cell (:1145-1179: + beta_b + beta_d, + beta_u if its top is Neumann)
+// and b = -div (:1215-1328 with zero jump terms).
+__global__ void k_diag_rhs(const double* __restrict__ f, const double* __restrict__ v, const double* __restrict__ bx,
+                           const double* __restrict__ by, const double* __restrict__ bz, const double* __restrict__ div,
+                           double* __restrict__ diag, double* __restrict__ rhs, double* __restrict__ pin, FsLayout L,
+                           double inv_dxdx) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    double coef = 0;
+    coef += bx[c];
+    coef += bx[c + 1];
+    coef += by[c];
+    coef += by[c + L.sx];
+    coef += bz[c];
+    coef += bz[c + L.sy];
+    if (i == 0 && j == 0 && k == 0) {
+        double vc = v[c], fc = f[c];
+        double pb = gfm_beta(vc, v[c - L.sx], f[c - L.sx], fc);  // beta_b, face towards j-1
+        double pd = gfm_beta(vc, v[c - L.sy], f[c - L.sy], fc);  // beta_d, face towards k-1
+        coef += pb;
+        coef += pd;
+        pin[0] = pb;
+        pin[1] = pd;
+    }
+    if (coef == 0) coef = 1;
+    diag[c] = inv_dxdx * coef;
+    rhs[c] = -div[c];
+}
+
+void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, double* bx, double* by,
+                 double* bz, double* diag, double* rhs, double* pin, const FsLayout& L, const FsGridConst& cell) {
+    FsBox bb{L.nx + 1, L.ny + 1, L.nz + 1};
+    hipLaunchKernelGGL(k_beta, grid_for(bb), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, L);
+    double inv_dxdx = (double)1 / (cell.x.d * cell.x.d);
+    hipLaunchKernelGGL(k_diag_rhs, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, div, diag, rhs,
+                       pin, L, inv_dxdx);
+}
+
+// Neumann mean removal, POISSON_SOLVER_3D.h:225-246.  The sum runs in row
+// order inside each z-plane, then over planes in order (one slab per plane).
+__global__ void k_plane_sums(const double* __restrict__ rhs, double* __restrict__ part, FsLayout L) {
+    int k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= L.nz) return;
+    double s = 0;
+    for (int j = 0; j < L.ny; j++) {
+        const double* row = rhs + L.idx(0, j, k);
+        for (int i = 0; i < L.nx; i++) s += row[i];
+    }
+    part[k] = s;
+}
+__global__ void k_total(double* part, int nz) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double s = 0;
+    for (int k = 0; k < nz; k++) s += part[k];
+    part[nz] = s;
+}
+__global__ void k_sub_mean(double* __restrict__ rhs, const double* __restrict__ part, FsLayout L, double inv_n) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    rhs[c] -= inv_n * part[L.nz];
+}
+
+void fs_remove_mean(hipStream_t st, double* rhs, double* scratch, const FsLayout& L) {
+    hipLaunchKernelGGL(k_plane_sums, dim3((L.nz + 63) / 64), dim3(64), 0, st, rhs, scratch, L);
+    hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, st, scratch, L.nz);
+    double inv_n = 1 / (double)((long)L.nx * L.ny * L.nz);
+    hipLaunchKernelGGL(k_sub_mean, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, rhs, scratch, L, inv_n);
+}
+
+__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* bits, double a) {
+    // non-negative doubles order like their bit patterns
+    atomicMax(bits, (unsigned long long)__double_as_longlong(a));
+}
+
+// UpdateVelocityByPressureGradientVariableDensity, PROJECTION_3D.h:1099-1449,
+// one component (comp 1,2,3).  Faces whose + cell is not full are skipped
+// (:1165-1168); the bc product test (:1191) lets a face next to a Neumann
+// ghost through only for the cell numbered 0.  The face level set comes from
+// the projection's ghost copy (linear extrapolation at depth 1, :1115).
+__global__ void __launch_bounds__(BX* BY) k_gradient(int comp, int apply, double* __restrict__ vel,
+                                                     const double* __restrict__ p, const double* __restrict__ phi,
+                                                     const double* __restrict__ rho, FsLayout L, FsBox b, FsMaterial m,
+                                                     double eps, double od, double dt, unsigned long long* vmax) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    bool valid = i < b.ni && j < b.nj && i < L.nx && j < L.ny && k < L.nz;
+    double a = 0;
+    if (valid) {
+        long c = L.idx(i, j, k);
+        long st = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
+        int nidx = comp == 1 ? i - 1 : comp == 2 ? j - 1 : k - 1;
+        bool neighbour_full = nidx >= 0;
+        double lc = phi[c], ln = phi[c - st];
+        // projection ghost copy: interior values, or 2*phi(0) - phi(1) below 0
+        double gn = neighbour_full ? ln : (double)2 * phi[c] - phi[c + st];
+        double lface = (double)0.5 * (lc + gn);
+        double H = fs_heaviside_proj(lface, eps);
+        double density_half = m.density(H);
+        double one_over_density_ijk = (double)1 / rho[c];
+        double jump_gamma = ((double)0 * fabs(ln) + (double)0 * fabs(lc)) / (fabs(ln) + fabs(lc));
+        double one_over_density_half = 1 / density_half;
+        bool first_cell = (i == 0 && j == 0 && k == 0);
+        double pc = p[c], pn = p[c - st];
+        double vr = vel[c];
+        if (apply && (neighbour_full || first_cell)) {
+            if (lc * ln > 0) {
+                vr -= dt * (pc - pn) * one_over_density_ijk * od;
+            } else {
+                if (lc <= 0) vr -= dt * (pc - pn + jump_gamma) * one_over_density_half * od;
+                else if (lc > 0) vr -= dt * (pc - pn - jump_gamma) * one_over_density_half * od;
+            }
+            vel[c] = vr;
+        }
+        a = fabs(vr);
+    }
+    // wave-level max, then one atomic per wave
+    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax, a);
+}
+
+void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,
+                        const double* rho, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
+                        double dt, unsigned long long* vmax_bits) {
+    FsBox b = fs_box(L, comp);
+    double dg = comp == 1 ? cell.x.d : comp == 2 ? cell.y.d : cell.z.d;
+    double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
+    hipLaunchKernelGGL(k_gradient, grid_for(b), dim3(BX, BY), 0, st, comp, apply, vel, p, phi, rho, L, b, m,
+                       (double)1.5 * dg, od, dt, vmax_bits);
+}
+
+__global__ void k_abs_max(const double* __restrict__ a, FsLayout L, FsBox b, unsigned long long* bits) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    double v = 0;
+    if (i < b.ni && j < b.nj) v = fabs(a[L.idx(i, j, k)]);
+    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
+    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(bits, v);
+}
+
+void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, unsigned long long* bits) {
+    hipLaunchKernelGGL(k_abs_max, grid_for(b), dim3(BX, BY), 0, st, a, L, b, bits);
+}
+
+// ---------------------------------------------------------------------------
+// reinitialization
+// ---------------------------------------------------------------------------
+
+// Sign function S = phi/sqrt(phi^2 + dx^2), EULERIAN_FLUID_SOLVER_3D.h:2211-2215.
+__global__ void k_sign(const double* __restrict__ phi, double* __restrict__ sgn, FsLayout L, double dx2) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    double f = phi[c];
+    sgn[c] = f / sqrt(f * f + dx2);
+}
+
+void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, double dx2) {
+    hipLaunchKernelGGL(k_sign, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, sgn, L, dx2);
+}
+
+// WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637: WENO5 one-sided
+// derivatives of the ghost copy, Godunov case per axis, the misplaced
+// parenthesis of :1463/:1605 (x shock, y shock towards +, z expansion), and
+// the sign-preserving clamp (:1624-1634).
+__global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, const double* __restrict__ gh,
+                                                   const double* __restrict__ sgn, FsLayout L, double ix, double iy,
+                                                   double iz, double eps, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    double f[7], mx, px, my, py, mz, pz, X, Y, Z;
+    gather7(gh, c, 1, f);
+    fs_weno5(f + 3, ix, eps, mx, px);
+    gather7(gh, c, L.sx, f);
+    fs_weno5(f + 3, iy, eps, my, py);
+    gather7(gh, c, L.sy, f);
+    fs_weno5(f + 3, iz, eps, mz, pz);
+    double sg = sgn[c];
+    double rho = phi[c];
+    int cx = fs_godunov_axis(sg, mx, px, X);
+    int cy = cx ? fs_godunov_axis(sg, my, py, Y) : FS_AX_NONE;
+    int cz = cy ? fs_godunov_axis(sg, mz, pz, Z) : FS_AX_NONE;
+    if (cx && cy && cz) {
+        double g;
+        if ((cx == FS_AX_SM || cx == FS_AX_SP) && cy == FS_AX_SP && cz == FS_AX_DROP)
+            g = sqrt(X * X) + Y * Y - 1;
+        else
+            g = sqrt(X * X + Y * Y + Z * Z) - 1;
+        rho = rho - dt * sg * g;
+    }
+    if (sg * rho < (double)0) {
+        if (sg < 0) rho = -eps;
+        if (sg > 0) rho = eps;
+    }
+    phi[c] = rho;
+}
+
+void fs_reinit_sweep(hipStream_t st, double* phi, const double* ghost, const double* sgn, const FsLayout& L,
+                     const FsGridConst& cell, double eps, double dt) {
+    hipLaunchKernelGGL(k_reinit, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, ghost, sgn, L, cell.x.inv,
+                       cell.y.inv, cell.z.inv, eps, dt);
+}
+
+// ---------------------------------------------------------------------------
+// misc
+// ---------------------------------------------------------------------------
+
+__global__ void k_average(double* __restrict__ phi, const double* __restrict__ tmp, FsLayout L) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    phi[c] = (double)0.5 * (tmp[c] + phi[c]);
+}
+
+void fs_average(hipStream_t st, double* phi, const double* tmp, const FsLayout& L) {
+    hipLaunchKernelGGL(k_average, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, tmp, L);
+}
+
+__global__ void k_scatter(double* __restrict__ dst, const double* __restrict__ lin, FsLayout L, FsBox b) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    dst[L.idx(i, j, k)] = lin[i + (long)b.ni * (j + (long)b.nj * k)];
+}
+__global__ void k_gather(double* __restrict__ lin, const double* __restrict__ src, FsLayout L, FsBox b) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    lin[i + (long)b.ni * (j + (long)b.nj * k)] = src[L.idx(i, j, k)];
+}
+
+void fs_scatter(hipStream_t st, double* dst, const double* lin, const FsLayout& L, FsBox b) {
+    hipLaunchKernelGGL(k_scatter, grid_for(b), dim3(BX, BY), 0, st, dst, lin, L, b);
+}
+void fs_gather(hipStream_t st, double* lin, const double* src, const FsLayout& L, FsBox b) {
+    hipLaunchKernelGGL(k_gather, grid_for(b), dim3(BX, BY), 0, st, lin, src, L, b);
+}
+
+__global__ void k_fill_value(double* dst, long n, double v) {
+    long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    for (; t < n; t += (long)gridDim.x * blockDim.x) dst[t] = v;
+}
+void fs_fill_value(hipStream_t st, double* dst, long n, double v) {
+    hipLaunchKernelGGL(k_fill_value, dim3(1024), dim3(256), 0, st, dst, n, v);
+}

@@ -1,0 +1,188 @@
+/*
+ * fs_abi.h -- C-ABI of the MI355X-native timestep hot path of a two-phase
+ * level-set Navier-Stokes solver on a uniform MAC grid.
+ *
+ * The library (libfs_hip.so) replaces the per-timestep work of the
+ * reference class EULERIAN_FLUID_SOLVER_3D (EULERIAN_FLUID_SOLVER_3D.h) with
+ * hand-written HIP kernels for gfx950.  Every entry point below names the
+ * reference function it stands in for.  Nothing here uses C++ or torch types:
+ * plain structs, pointers and sizes only.
+ *
+ * Conventions
+ *   - every function returns 0 (FS_OK) or a negative FS_ERR_* code;
+ *     fs_last_error() gives a message for the last failure on a handle;
+ *   - one host thread per handle; calls on one handle are not re-entrant;
+ *   - all GPU work of a handle runs on one HIP stream owned by the handle and
+ *     every call returns after that work has finished;
+ *   - field buffers exchanged with the caller are interior-only and x-fastest:
+ *       cell field (phi, p)   nx*ny*nz        index i + nx*(j + ny*k)
+ *       x-faces (u)           (nx+1)*ny*nz    index i + (nx+1)*(j + ny*k)
+ *       y-faces (v)           nx*(ny+1)*nz    index i + nx*(j + (ny+1)*k)
+ *       z-faces (w)           nx*ny*(nz+1)    index i + nx*(j + ny*k)
+ *     which is the k-j-i order of ARRAY_3D (ARRAY_3D.h:98) and of the
+ *     level-set dump format (LEVELSET_3D.h:1132-1200).
+ */
+#ifndef FS_ABI_H
+#define FS_ABI_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- status codes --------------------------------------------------------- */
+#define FS_OK                 0
+#define FS_ERR_BAD_ARG       -1   /* invalid argument (null, dt <= 0, bad size) */
+#define FS_ERR_OOM           -2   /* device allocation failed or over the size cap */
+#define FS_ERR_HIP           -3   /* a HIP runtime call failed */
+#define FS_ERR_NOT_CONVERGED -4   /* PCG hit max_iteration; fields are still updated */
+#define FS_ERR_NONFINITE     -5   /* NaN/Inf found in the velocity or level set */
+#define FS_ERR_UNSUPPORTED   -6   /* option outside the implemented hot path */
+
+/* ---- option values -------------------------------------------------------- */
+/* Pressure solve precision. */
+#define FS_FP64          0   /* Jacobi-PCG in FP64 (PCG_METHOD.cpp:4-126) */
+#define FS_FP32_REFINE   1   /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+
+/* PCG stop rule. */
+#define FS_STOP_ABS_PRECOND  0   /* r^T D^-1 r < tol^2, the reference rule (PCG_METHOD.cpp:85-90) */
+#define FS_STOP_REL_RESIDUAL 1   /* ||b - A x||_2 / ||b||_2 < rel_tolerance */
+
+/* Step schedule. */
+#define FS_SCHEDULE_COMPAT   0   /* reference call order, ghost refresh points and quirks */
+#define FS_SCHEDULE_FIXED    1   /* same kernels; all faces projected, fresh ghosts */
+
+/* ---- plain-data structs --------------------------------------------------- */
+
+/* Grid of cell centres.  Spacing follows GRID_STRUCTURE_3D::Initialize
+ * (GRID_STRUCTURE_3D.h:91-137): dx = (x_max - x_min)/(nx - 1), with
+ * x_min/x_max the first/last cell centre. */
+typedef struct fs_grid {
+    int    nx, ny, nz;
+    double x_min, y_min, z_min;
+    double x_max, y_max, z_max;
+} fs_grid;
+
+/* The script keys of the hot path (EULERIAN_FLUID_SOLVER_3D.h:192-260,
+ * ADVECTION_3D.h:102-113, PROJECTION_3D.h:178-227), air/water mode. */
+typedef struct fs_params {
+    /* phase labels: 1 = air_bubble_rising (phi <= 0 is the light phase),
+     * 0 = water_drop (phi <= 0 is water); PROJECTION_3D.h:605-639 */
+    int    air_bubble_rising;
+    double water_density, air_density;
+    double water_viscosity, air_viscosity;
+    double gravity[3];
+    double surface_tension;                 /* must be 0 (jump terms vanish) */
+    int    use_delta_function_formulation;  /* explicit viscosity on/off */
+    int    order_for_time_advancing;        /* 1 (2 and 3: FS_ERR_UNSUPPORTED) */
+    int    sussmanstyle_reinitialization;
+    int    scaling_number_for_reinitialization;   /* fictitious dt = dx/scaling */
+    int    iteration_number_for_reinitialization; /* sweeps per call */
+    double levelset_epsilon;   /* LEVELSET_ADVECTION epsilon: WENO5 eps and reinit clamp */
+    double velocity_epsilon;   /* VELOCITY_ADVECTION epsilon (WENO5 velocity) */
+    int    use_5th_weno_v;     /* 1: WENO5 on MAC velocity, 0: ENO3 (use_3rd_eno_v) */
+    /* PROJECTION_WATER */
+    double tolerance;          /* absolute tolerance of the reference stop rule */
+    int    max_iteration;
+    int    use_previous_pressure;
+    int    stop_rule;          /* FS_STOP_* */
+    double rel_tolerance;      /* for FS_STOP_REL_RESIDUAL */
+    int    precision;          /* FS_FP64 | FS_FP32_REFINE */
+    int    schedule;           /* FS_SCHEDULE_* */
+    /* debugging switches, EULERIAN_FLUID_SOLVER_3D.h:103-107 */
+    int    is_levelset_advection_active;
+    int    is_velocity_advection_active;
+    int    is_external_force_active;
+    int    is_sourcing_active;
+    int    is_projection_active;
+} fs_params;
+
+/* Per-step (or per-phase) statistics. */
+typedef struct fs_step_stats {
+    int    pcg_iterations;       /* total inner iterations of the last projection */
+    int    refinement_sweeps;    /* outer FP64 sweeps (FS_FP32_REFINE), else 0 */
+    double residual_precond;     /* sqrt(r^T D^-1 r) at exit, PCG_METHOD.cpp:111 */
+    double residual_rel;         /* ||b - A x||_2 / ||b||_2 at exit (recomputed in FP64) */
+    double max_velocity[3];      /* max |u|,|v|,|w| after the gradient update, PROJECTION_3D.h:1236,1345,1427 */
+    double projection_ms;        /* device time of the last projection */
+    double step_ms;              /* device time of the last fs_advance_frame/fs_step */
+} fs_step_stats;
+
+typedef struct fs_solver fs_solver;
+
+/* ---- lifecycle ------------------------------------------------------------ */
+
+/* Replaces EULERIAN_FLUID_SOLVER_3D::InitializeFromScriptBlock
+ * (EULERIAN_FLUID_SOLVER_3D.h:192-800): allocates the MAC fields, ghost
+ * fields and level set on device.  device < 0 keeps the current device. */
+int  fs_create(const fs_params* params, const fs_grid* grid, int device, fs_solver** out);
+void fs_destroy(fs_solver* s);
+const char* fs_last_error(const fs_solver* s);
+
+/* Copy host fields in.  phi is the initial level set written by
+ * SIMULATION_WORLD.cpp:116-129; u,v,w the MAC velocity
+ * (EULERIAN_FLUID_SOLVER_3D.h:23-25).  Ghost cells are set as at the end of
+ * InitializeFromScriptBlock (level-set ghosts by constant extrapolation,
+ * EULERIAN_FLUID_SOLVER_3D.h:757; every other ghost and p zero).
+ * Any pointer may be NULL to leave that field as it is. */
+int  fs_set_fields(fs_solver* s, const double* phi, const double* u,
+                   const double* v, const double* w, const double* p);
+/* Copy fields out (any pointer may be NULL). */
+int  fs_get_fields(fs_solver* s, double* phi, double* u, double* v, double* w, double* p);
+
+/* ---- the timestep -------------------------------------------------------- */
+
+/* One frame as SIMULATION_WORLD::AdvanceOneFrame runs it
+ * (SIMULATION_WORLD.h:320-330): the level-set ghost refresh of
+ * LEVELSET_3D::ComputeCurvatures (LEVELSET_3D.h:998) followed by
+ * EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread(dt)
+ * (EULERIAN_FLUID_SOLVER_3D.h:803-1111).  stats may be NULL. */
+int  fs_advance_frame(fs_solver* s, double dt, fs_step_stats* stats);
+
+/* EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread alone. */
+int  fs_step(fs_solver* s, double dt, fs_step_stats* stats);
+
+/* CFL time step: EULERIAN_FLUID_SOLVER_3D::CFLOneTimeStep()
+ * (EULERIAN_FLUID_SOLVER_3D.h:2673-2785) times cfl.  gravity_norm = 0 uses
+ * |g_y| as the reference does (:2717); 1 uses |g|.  Returns in *dt. */
+int  fs_cfl_dt(fs_solver* s, double cfl, int gravity_norm, double* dt);
+
+/* ---- phases (EULERIAN_FLUID_SOLVER_3D.h:1590-1606, 2076-2237) ------------ */
+int  fs_compute_curvatures(fs_solver* s);           /* LEVELSET_3D::ComputeCurvatures ghost refresh, :998 */
+int  fs_levelset_advection(fs_solver* s, double dt); /* ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242 */
+int  fs_velocity_advection(fs_solver* s, double dt); /* ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 */
+int  fs_apply_gravity(fs_solver* s, double dt);      /* ApplyGravity, EULERIAN_FLUID_SOLVER_3D.h:2237-2270 */
+int  fs_sourcing(fs_solver* s, double dt);           /* Sourcing -> VISCOSITY_3D::ApplyViscosity, VISCOSITY_3D.h:448-563 */
+int  fs_projection(fs_solver* s, double dt, fs_step_stats* stats); /* PROJECTION_3D::Solve, PROJECTION_3D.h:416-594 */
+int  fs_reinitialization(fs_solver* s, double dt);   /* Reinitialization, EULERIAN_FLUID_SOLVER_3D.h:2198-2225 */
+
+/* ---- inspection of the pressure system (per-kernel parity) ------------- */
+
+/* Assemble the projection's linear system for the current fields without
+ * solving it (PROJECTION_3D.h:596-833 up to POISSON_SOLVER_3D::Solve's
+ * mean removal, POISSON_SOLVER_3D.h:225-246).  Outputs (host, may be NULL):
+ *   diag  nx*ny*nz   A(c,c) of each cell          (POISSON_SOLVER_3D.h:1213)
+ *   bx    (nx+1)*ny*nz  beta of x-faces between two full cells, else 0
+ *   by, bz likewise for y- and z-faces            (POISSON_SOLVER_3D.h:1080-1085)
+ *   rhs   nx*ny*nz   b after the Neumann mean removal */
+int  fs_assemble_pressure_system(fs_solver* s, double dt, double* diag, double* bx,
+                                 double* by, double* bz, double* rhs);
+
+/* q = A x with the matrix-free operator of the last assembly
+ * (CSR_MATRIX::Multiply, CSR_MATRIX.h:236-260).  x, q: nx*ny*nz host. */
+int  fs_apply_pressure_operator(fs_solver* s, const double* x, double* q);
+
+/* ---- measurement ---------------------------------------------------------- */
+
+/* Times the three PCG kernels of one iteration (SpMV + p.q, the x/r/z update
+ * with r.z, the p update) on the operator of the last assembly: `reps`
+ * back-to-back launches each, timed with HIP events on the handle's stream.
+ * precision: FS_FP64 or FS_FP32_REFINE (needs a handle created with it).
+ * avg_ms[0..2] receives the mean duration per launch.  No reference
+ * counterpart: it replaces the wall-clock LOG timers around PCGMethod
+ * (PCG_METHOD.cpp:6, LOG.h:191-338) for roofline accounting. */
+int  fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FS_ABI_H */

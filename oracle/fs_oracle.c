@@ -1,0 +1,1114 @@
+/*
+ * fs_oracle.c -- TEST INFRASTRUCTURE ONLY: the CPU checker for the HIP path.
+ *
+ * A plain-C, FP64, sequential-order restatement of the reference's per-step
+ * hot path in air/water mode (EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread,
+ * EULERIAN_FLUID_SOLVER_3D.h:803-1111, order_for_time_advancing = 1).  Each
+ * function cites the reference lines it restates.  Operation order inside every
+ * expression follows the reference text (no FMA contraction: build with
+ * -ffp-contract=off) so that the HIP kernels, which keep the same order, can be
+ * compared bit for bit where no libm call or reduction intervenes.
+ *
+ * State model.  The reference keeps persistent ghosted arrays whose ghost
+ * cells are refreshed only at specific points; several stencils read ghost
+ * copies made earlier in the step (stale).  This file keeps the same arrays
+ * with the same refresh points:
+ *   phi     level set, ghost width 2 (LEVELSET_3D, EULERIAN_FLUID_SOLVER_3D.h:457-459)
+ *   sghost  shared scalar ghost copy, width 3 (scalar_field_ghost, :435), written
+ *           only by reinitialization sweeps (ADVECTION_3D.h:375)
+ *   gu,gv,gw  MAC ghost copies, width 3 (:436-438), written only by the
+ *           projection's divergence (PROJECTION_3D.h:686-688)
+ *   u,v,w   MAC velocity, width 2; nothing writes their ghosts (stay 0)
+ *   p       pressure, width 1 (PROJECTION_3D.h:362), ghosts stay 0
+ * Every array shares one padded layout of ghost width 3, x fastest.
+ *
+ * Reductions (dot products, the Neumann mean) are summed per z-plane in
+ * row order and the plane partials in plane order: the reference's
+ * MULTITHREADING::SyncSum (MULTITHREADING.h:229-250) with one slab per plane.
+ *
+ * Parity unpinned here: the libm boundary (sin in the smoothed Heaviside,
+ * sqrt in reinitialization / sign function) is whatever the host libm gives.
+ */
+#include "fs_oracle.h"
+
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#define G 3
+#define FS_BC_FULL 0
+#define FS_BC_NEUM (-4) /* MACROS.h:10 */
+#define FS_PI 3.14159265358979323846 /* MACROS.h:13 */
+
+typedef struct {
+    double d, d2, inv, inv2d, inv_d2;
+} ax_t; /* GRID_STRUCTURE_3D.h:114-133 */
+typedef struct {
+    ax_t x, y, z;
+} gc_t;
+
+struct fso_state {
+    fs_params P;
+    fs_grid gr;
+    int nx, ny, nz;
+    long sx, sy, sz, ntot;
+    gc_t cell, mx, my, mz;
+    double *phi, *sghost, *u, *v, *w, *gu, *gv, *gw, *p, *div, *rho, *irho, *sgn, *tmp, *pg, *vtmp;
+    double *fbx, *fby, *fbz; /* face betas of the last assembly (inspection only) */
+    int* bc;
+    /* CSR of the last assembly (POISSON_SOLVER_3D.h:1036-1038) */
+    int N, nnz;
+    int *rp, *ci;
+    double *val, *b, *x, *r, *zv, *pp, *Ap;
+    int visc_inplace;
+    double max_vel[3];
+    int last_iters;
+    double last_res_abs, last_res_rel;
+};
+
+static int g_threads = 1;
+void fso_set_threads(int n) { g_threads = n > 0 ? n : 1; }
+
+#define IX(s, i, j, k) ((((long)(k) + G) * (s)->sy + ((long)(j) + G)) * (s)->sx + ((long)(i) + G))
+#define A3(a, i, j, k) ((a)[IX(s, i, j, k)])
+
+static ax_t mk_ax(double mn, double mx, int res) {
+    ax_t a;
+    a.d = (mx - mn) / (res - 1);
+    a.d2 = a.d * a.d;
+    a.inv = (double)1 / a.d;
+    a.inv2d = (double)0.5 / a.d;
+    a.inv_d2 = (double)1 / a.d2;
+    return a;
+}
+
+static double* zalloc(long n) { return (double*)calloc((size_t)n, sizeof(double)); }
+
+int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
+    if (!P || !g || !out || g->nx < 3 || g->ny < 3 || g->nz < 3) return FS_ERR_BAD_ARG;
+    if (P->surface_tension != 0.0 || P->order_for_time_advancing != 1) return FS_ERR_UNSUPPORTED;
+    fso_state* s = (fso_state*)calloc(1, sizeof(fso_state));
+    s->P = *P;
+    s->gr = *g;
+    s->nx = g->nx;
+    s->ny = g->ny;
+    s->nz = g->nz;
+    s->sx = g->nx + 1 + 2 * G;
+    s->sy = g->ny + 1 + 2 * G;
+    s->sz = g->nz + 1 + 2 * G;
+    s->ntot = s->sx * s->sy * s->sz;
+    s->cell.x = mk_ax(g->x_min, g->x_max, g->nx);
+    s->cell.y = mk_ax(g->y_min, g->y_max, g->ny);
+    s->cell.z = mk_ax(g->z_min, g->z_max, g->nz);
+    /* MAC grids, EULERIAN_FLUID_SOLVER_3D.h:407,415,423 */
+    double dx = s->cell.x.d, dy = s->cell.y.d, dz = s->cell.z.d;
+    s->mx = s->cell;
+    s->mx.x = mk_ax(g->x_min - (double)0.5 * dx, g->x_max + (double)0.5 * dx, g->nx + 1);
+    s->my = s->cell;
+    s->my.y = mk_ax(g->y_min - (double)0.5 * dy, g->y_max + (double)0.5 * dy, g->ny + 1);
+    s->mz = s->cell;
+    s->mz.z = mk_ax(g->z_min - (double)0.5 * dz, g->z_max + (double)0.5 * dz, g->nz + 1);
+    long n = s->ntot;
+    s->phi = zalloc(n); s->sghost = zalloc(n); s->u = zalloc(n); s->v = zalloc(n); s->w = zalloc(n);
+    s->gu = zalloc(n); s->gv = zalloc(n); s->gw = zalloc(n); s->p = zalloc(n); s->div = zalloc(n);
+    s->rho = zalloc(n); s->irho = zalloc(n); s->sgn = zalloc(n); s->tmp = zalloc(n); s->pg = zalloc(n);
+    s->vtmp = zalloc(n);
+    s->fbx = zalloc(n); s->fby = zalloc(n); s->fbz = zalloc(n);
+    s->bc = (int*)calloc((size_t)n, sizeof(int));
+    s->visc_inplace = 1;
+    *out = s;
+    return FS_OK;
+}
+
+static void free_csr(fso_state* s) {
+    free(s->rp); free(s->ci); free(s->val); free(s->b); free(s->x); free(s->r); free(s->zv); free(s->pp); free(s->Ap);
+    s->rp = s->ci = NULL;
+    s->val = s->b = s->x = s->r = s->zv = s->pp = s->Ap = NULL;
+}
+
+void fso_destroy(fso_state* s) {
+    if (!s) return;
+    free(s->phi); free(s->sghost); free(s->u); free(s->v); free(s->w); free(s->gu); free(s->gv); free(s->gw);
+    free(s->p); free(s->div); free(s->rho); free(s->irho); free(s->sgn); free(s->tmp); free(s->pg); free(s->vtmp); free(s->fbx); free(s->fby); free(s->fbz);
+    free(s->bc);
+    free_csr(s);
+    free(s);
+}
+
+void fso_set_viscosity_inplace(fso_state* s, int inplace) { s->visc_inplace = inplace; }
+
+/* ------------------------------------------------------------------------- */
+/* Ghost fills (FIELD_STRUCTURE_3D.h)                                        */
+/* ------------------------------------------------------------------------- */
+
+static void copy_interior(fso_state* s, double* dst, const double* src, int ni, int nj, int nk) {
+    if (dst == src) return;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) A3(dst, i, j, k) = A3(src, i, j, k);
+}
+
+/* FillGhostCellsZero, FIELD_STRUCTURE_3D.h:538-661: copy the interior and
+ * zero the six face blocks; edge and vertex blocks are left as they are. */
+static void fill_zero(fso_state* s, double* dst, const double* src, int ni, int nj, int nk, int d) {
+    copy_interior(s, dst, src, ni, nj, nk);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int g = 1; g <= d; g++) {
+                A3(dst, -g, j, k) = 0;
+                A3(dst, ni - 1 + g, j, k) = 0;
+            }
+    for (int k = 0; k < nk; k++)
+        for (int g = 1; g <= d; g++)
+            for (int i = 0; i < ni; i++) {
+                A3(dst, i, -g, k) = 0;
+                A3(dst, i, nj - 1 + g, k) = 0;
+            }
+    for (int g = 1; g <= d; g++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                A3(dst, i, j, -g) = 0;
+                A3(dst, i, j, nk - 1 + g) = 0;
+            }
+}
+
+static int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+/* FillGhostCellsFrom, FIELD_STRUCTURE_3D.h:663-1086: constant extrapolation
+ * of faces, edges and vertices (every ghost takes its clamped interior value). */
+static void fill_const(fso_state* s, double* dst, const double* src, int ni, int nj, int nk, int d, int copy) {
+    if (copy) copy_interior(s, dst, src, ni, nj, nk);
+    for (int k = -d; k < nk + d; k++)
+        for (int j = -d; j < nj + d; j++)
+            for (int i = -d; i < ni + d; i++) {
+                if (i >= 0 && i < ni && j >= 0 && j < nj && k >= 0 && k < nk) continue;
+                A3(dst, i, j, k) = A3(src, clampi(i, 0, ni - 1), clampi(j, 0, nj - 1), clampi(k, 0, nk - 1));
+            }
+}
+
+/* FillGhostCellsContinuousDerivativesFrom, FIELD_STRUCTURE_3D.h:1776-2199:
+ * linear extrapolation 2a-b on the six faces in the reference's block and
+ * loop order (src may equal dst: later ghost layers then read earlier ones);
+ * with edges != 0 also the three filled edge blocks normal to z (1898-1946).
+ * The other edges and all vertices are commented out in the reference. */
+static void fill_linear(fso_state* s, double* dst, const double* src, int ni, int nj, int nk, int d, int copy,
+                        int edges) {
+    if (copy) copy_interior(s, dst, src, ni, nj, nk);
+    int i, j, k;
+    for (k = 0; k < nk; k++)
+        for (j = 0; j < nj; j++)
+            for (i = -1; i >= -d; i--) A3(dst, i, j, k) = (double)2 * A3(src, i + 1, j, k) - A3(src, i + 2, j, k);
+    for (k = 0; k < nk; k++)
+        for (j = 0; j < nj; j++)
+            for (i = ni; i <= ni - 1 + d; i++) A3(dst, i, j, k) = (double)2 * A3(src, i - 1, j, k) - A3(src, i - 2, j, k);
+    for (k = 0; k < nk; k++)
+        for (j = -1; j >= -d; j--)
+            for (i = 0; i < ni; i++) A3(dst, i, j, k) = (double)2 * A3(src, i, j + 1, k) - A3(src, i, j + 2, k);
+    for (k = 0; k < nk; k++)
+        for (j = nj; j <= nj - 1 + d; j++)
+            for (i = 0; i < ni; i++) A3(dst, i, j, k) = (double)2 * A3(src, i, j - 1, k) - A3(src, i, j - 2, k);
+    for (k = -1; k >= -d; k--)
+        for (j = 0; j < nj; j++)
+            for (i = 0; i < ni; i++) A3(dst, i, j, k) = (double)2 * A3(src, i, j, k + 1) - A3(src, i, j, k + 2);
+    for (k = nk; k <= nk - 1 + d; k++)
+        for (j = 0; j < nj; j++)
+            for (i = 0; i < ni; i++) A3(dst, i, j, k) = (double)2 * A3(src, i, j, k - 1) - A3(src, i, j, k - 2);
+    if (!edges) return;
+    for (k = 0; k < nk; k++)
+        for (j = -1; j >= -d; j--)
+            for (i = -1; i >= -d; i--)
+                A3(dst, i, j, k) = (double)0.5 * (((double)2 * A3(src, i + 1, j, k) - A3(src, i + 2, j, k)) +
+                                                  ((double)2 * A3(src, i, j + 1, k) - A3(src, i, j + 2, k)));
+    for (k = 0; k < nk; k++)
+        for (j = nj; j <= nj - 1 + d; j++)
+            for (i = ni; i <= ni - 1 + d; i++)
+                A3(dst, i, j, k) = (double)0.5 * (((double)2 * A3(src, i - 1, j, k) - A3(src, i - 2, j, k)) +
+                                                  ((double)2 * A3(src, i, j - 1, k) - A3(src, i, j - 2, k)));
+    for (k = 0; k < nk; k++)
+        for (j = nj; j <= nj - 1 + d; j++)
+            for (i = -1; i >= -d; i--)
+                A3(dst, i, j, k) = (double)0.5 * (((double)2 * A3(src, i + 1, j, k) - A3(src, i + 2, j, k)) +
+                                                  ((double)2 * A3(src, i, j - 1, k) - A3(src, i, j - 2, k)));
+}
+
+/* ------------------------------------------------------------------------- */
+/* 1-D derivative kernels                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* WENO5-JS (Jiang-Shu 1996) one-sided first derivatives at v[3], restating
+ * ADVECTION_METHOD_3D.h:55-183 term by term. */
+void fso_weno5_1d(const double* f, double inv, double eps, double* dmo, double* dpo) {
+    const double* c = f + 3;
+    double m1 = inv * (c[-2] - c[-3]), m2 = inv * (c[-1] - c[-2]), m3 = inv * (c[0] - c[-1]);
+    double m4 = inv * (c[1] - c[0]), m5 = inv * (c[2] - c[1]);
+    double p1 = inv * (c[3] - c[2]), p2 = inv * (c[2] - c[1]), p3 = inv * (c[1] - c[0]);
+    double p4 = inv * (c[0] - c[-1]), p5 = inv * (c[-1] - c[-2]);
+    const double k13 = (double)13 / 12, q = (double)1 / 4;
+    double sm1 = k13 * (m1 - 2 * m2 + m3) * (m1 - 2 * m2 + m3) + q * (m1 - 4 * m2 + 3 * m3) * (m1 - 4 * m2 + 3 * m3);
+    double sm2 = k13 * (m2 - 2 * m3 + m4) * (m2 - 2 * m3 + m4) + q * (m2 - m4) * (m2 - m4);
+    double sm3 = k13 * (m3 - 2 * m4 + m5) * (m3 - 2 * m4 + m5) + q * (3 * m3 - 4 * m4 + m5) * (3 * m3 - 4 * m4 + m5);
+    double sp1 = k13 * (p1 - 2 * p2 + p3) * (p1 - 2 * p2 + p3) + q * (p1 - 4 * p2 + 3 * p3) * (p1 - 4 * p2 + 3 * p3);
+    double sp2 = k13 * (p2 - 2 * p3 + p4) * (p2 - 2 * p3 + p4) + q * (p2 - p4) * (p2 - p4);
+    double sp3 = k13 * (p3 - 2 * p4 + p5) * (p3 - 2 * p4 + p5) + q * (3 * p3 - 4 * p4 + p5) * (3 * p3 - 4 * p4 + p5);
+    double am1 = (double)1 / 10 * (double)1 / ((eps + sm1) * (eps + sm1));
+    double am2 = (double)6 / 10 * (double)1 / ((eps + sm2) * (eps + sm2));
+    double am3 = (double)3 / 10 * (double)1 / ((eps + sm3) * (eps + sm3));
+    double summ = am1 + am2 + am3;
+    double wm1 = am1 / summ, wm2 = am2 / summ, wm3 = am3 / summ;
+    double ap1 = (double)1 / 10 * (double)1 / ((eps + sp1) * (eps + sp1));
+    double ap2 = (double)6 / 10 * (double)1 / ((eps + sp2) * (eps + sp2));
+    double ap3 = (double)3 / 10 * (double)1 / ((eps + sp3) * (eps + sp3));
+    double sump = ap1 + ap2 + ap3;
+    double wp1 = ap1 / sump, wp2 = ap2 / sump, wp3 = ap3 / sump;
+    *dmo = wm1 * (m1 * (double)1 / 3 - m2 * (double)7 / 6 + m3 * (double)11 / 6) +
+           wm2 * (m2 * (-(double)1 / 6) + m3 * (double)5 / 6 + m4 * (double)1 / 3) +
+           wm3 * (m3 * (double)1 / 3 + m4 * (double)5 / 6 - m5 * (double)1 / 6);
+    *dpo = wp1 * (p1 * (double)1 / 3 - p2 * (double)7 / 6 + p3 * (double)11 / 6) +
+           wp2 * (p2 * (-(double)1 / 6) + p3 * (double)5 / 6 + p4 * (double)1 / 3) +
+           wp3 * (p3 * (double)1 / 3 + p4 * (double)5 / 6 - p5 * (double)1 / 6);
+}
+
+/* ENO3 one-sided first derivatives at v[3] with the reference's divided
+ * differences and stencil coefficients, ADVECTION_METHOD_3D.h:1653-1718. */
+void fso_eno3_1d(const double* f, double dx, double inv, double inv2, double inv3, double* dmo, double* dpo) {
+    const double* c = f + 3;
+    double d1n3 = inv * (c[-2] - c[-3]), d1n2 = inv * (c[-1] - c[-2]), d1n1 = inv * (c[0] - c[-1]);
+    double d10 = inv * (c[1] - c[0]), d1p1 = inv * (c[2] - c[1]), d1p2 = inv * (c[3] - c[2]);
+    double d2n2 = inv2 * (d1n2 - d1n3), d2n1 = inv2 * (d1n1 - d1n2), d20 = inv2 * (d10 - d1n1);
+    double d2p1 = inv2 * (d1p1 - d10), d2p2 = inv2 * (d1p2 - d1p1);
+    double d3n2 = inv3 * (d2n1 - d2n2), d3n1 = inv3 * (d20 - d2n1), d30 = inv3 * (d2p1 - d20);
+    double d3p1 = inv3 * (d2p2 - d2p1);
+    double m, p;
+    if (fabs(d2n1) <= fabs(d20)) {
+        if (fabs(d3n2) <= fabs(d3n1)) m = d1n1 + d2n1 * dx + 2 * d3n2 * dx * dx;
+        else m = d1n1 + d2n1 * dx + 2 * d3n1 * dx * dx;
+    } else {
+        if (fabs(d3n1) <= fabs(d30)) m = d1n1 + d20 * dx - d3n1 * dx * dx;
+        else m = d1n1 + d20 * dx - d30 * dx * dx;
+    }
+    if (fabs(d20) <= fabs(d2p1)) {
+        if (fabs(d3n1) <= fabs(d30)) p = d10 - d20 * dx - d3n1 * dx * dx;
+        else p = d10 - d20 * dx - d30 * dx * dx;
+    } else {
+        if (fabs(d30) <= fabs(d3p1)) p = d10 - d2p1 * dx + 2 * d30 * dx * dx;
+        else p = d10 - d2p1 * dx + 2 * d3p1 * dx * dx;
+    }
+    *dmo = m;
+    *dpo = p;
+}
+
+/* gather 7 samples along an axis */
+static void gather7(fso_state* s, const double* a, int i, int j, int k, int axis, double* f) {
+    for (int o = -3; o <= 3; o++)
+        f[o + 3] = axis == 0 ? A3(a, i + o, j, k) : axis == 1 ? A3(a, i, j + o, k) : A3(a, i, j, k + o);
+}
+
+/* component: 0 scalar (cell), 1 x-face, 2 y-face, 3 z-face; velocity at the
+ * sample point from the MAC ghost arrays (ADVECTION_METHOD_3D.h:201-224). */
+static void vel_at(fso_state* s, int comp, int i, int j, int k, double* uu, double* vv, double* ww) {
+    const double *gu = s->gu, *gv = s->gv, *gw = s->gw;
+    if (comp == 0) {
+        *uu = (A3(gu, i + 1, j, k) + A3(gu, i, j, k)) * (double)0.5;
+        *vv = (A3(gv, i, j + 1, k) + A3(gv, i, j, k)) * (double)0.5;
+        *ww = (A3(gw, i, j, k + 1) + A3(gw, i, j, k)) * (double)0.5;
+    } else if (comp == 1) {
+        *uu = A3(gu, i, j, k);
+        *vv = (A3(gv, i, j, k) + A3(gv, i, j + 1, k) + A3(gv, i - 1, j, k) + A3(gv, i - 1, j + 1, k)) * (double)0.25;
+        *ww = (A3(gw, i, j, k) + A3(gw, i, j, k + 1) + A3(gw, i - 1, j, k) + A3(gw, i - 1, j, k + 1)) * (double)0.25;
+    } else if (comp == 2) {
+        *uu = (A3(gu, i, j, k) + A3(gu, i + 1, j, k) + A3(gu, i, j - 1, k) + A3(gu, i + 1, j - 1, k)) * (double)0.25;
+        *vv = A3(gv, i, j, k);
+        *ww = (A3(gw, i, j, k) + A3(gw, i, j, k + 1) + A3(gw, i, j - 1, k) + A3(gw, i, j - 1, k + 1)) * (double)0.25;
+    } else {
+        *uu = (A3(gu, i, j, k) + A3(gu, i + 1, j, k) + A3(gu, i, j, k - 1) + A3(gu, i + 1, j, k - 1)) * (double)0.25;
+        *vv = (A3(gv, i, j, k) + A3(gv, i, j + 1, k) + A3(gv, i, j, k - 1) + A3(gv, i, j + 1, k - 1)) * (double)0.25;
+        *ww = A3(gw, i, j, k);
+    }
+}
+
+/* Upwind Euler update, ADVECTION_METHOD_3D.h:226-275. */
+static double upwind_update(double rho, double dt, double uu, double vv, double ww, double mx, double px, double my,
+                            double py, double mz, double pz) {
+    double X = uu > 0 ? mx : px;
+    double Y = vv > 0 ? my : py;
+    double Z = ww > 0 ? mz : pz;
+    return rho - dt * (uu * X + vv * Y + ww * Z);
+}
+
+static void extents(fso_state* s, int comp, int* ni, int* nj, int* nk) {
+    *ni = s->nx + (comp == 1);
+    *nj = s->ny + (comp == 2);
+    *nk = s->nz + (comp == 3);
+}
+
+static const gc_t* grid_of(fso_state* s, int comp) {
+    return comp == 0 ? &s->cell : comp == 1 ? &s->mx : comp == 2 ? &s->my : &s->mz;
+}
+
+/* ADVECTION_METHOD_3D::WENO5th, ADVECTION_METHOD_3D.h:42-279: one Euler
+ * stage on field `rho` (component comp) reading the ghost copy `ghost`. */
+static void weno5_stage(fso_state* s, double* rho, const double* ghost, int comp, double dt, double eps) {
+    int ni, nj, nk;
+    extents(s, comp, &ni, &nj, &nk);
+    const gc_t* gc = grid_of(s, comp);
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+                gather7(s, ghost, i, j, k, 0, f);
+                fso_weno5_1d(f, gc->x.inv, eps, &mx, &px);
+                gather7(s, ghost, i, j, k, 1, f);
+                fso_weno5_1d(f, gc->y.inv, eps, &my, &py);
+                gather7(s, ghost, i, j, k, 2, f);
+                fso_weno5_1d(f, gc->z.inv, eps, &mz, &pz);
+                vel_at(s, comp, i, j, k, &uu, &vv, &ww);
+                A3(rho, i, j, k) = upwind_update(A3(rho, i, j, k), dt, uu, vv, ww, mx, px, my, py, mz, pz);
+            }
+}
+
+/* ADVECTION_METHOD_3D::ENO3rd, ADVECTION_METHOD_3D.h:1639-1934 (note
+ * one_over_3dz is built from one_over_dx, :1649). */
+static void eno3_stage(fso_state* s, double* rho, const double* ghost, int comp, double dt) {
+    int ni, nj, nk;
+    extents(s, comp, &ni, &nj, &nk);
+    const gc_t* gc = grid_of(s, comp);
+    double inv3x = gc->x.inv * (double)1 / 3, inv3y = gc->y.inv * (double)1 / 3, inv3z = gc->x.inv * (double)1 / 3;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+                gather7(s, ghost, i, j, k, 0, f);
+                fso_eno3_1d(f, gc->x.d, gc->x.inv, gc->x.inv2d, inv3x, &mx, &px);
+                gather7(s, ghost, i, j, k, 1, f);
+                fso_eno3_1d(f, gc->y.d, gc->y.inv, gc->y.inv2d, inv3y, &my, &py);
+                gather7(s, ghost, i, j, k, 2, f);
+                fso_eno3_1d(f, gc->z.d, gc->z.inv, gc->z.inv2d, inv3z, &mz, &pz);
+                vel_at(s, comp, i, j, k, &uu, &vv, &ww);
+                A3(rho, i, j, k) = upwind_update(A3(rho, i, j, k), dt, uu, vv, ww, mx, px, my, py, mz, pz);
+            }
+}
+
+/* ------------------------------------------------------------------------- */
+/* Phases                                                                     */
+/* ------------------------------------------------------------------------- */
+
+/* LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082.  With zero surface
+ * tension the curvature never reaches the fields; the call's lasting effect
+ * is the linear re-extrapolation of phi's own ghost cells (:998). */
+int fso_compute_curvatures(fso_state* s) {
+    fill_linear(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0, 1);
+    return FS_OK;
+}
+
+/* LEVELSET_3D::ComputeNormals, LEVELSET_3D.h:929-956: lasting effect is the
+ * constant re-extrapolation of phi's ghost cells (:931); normals only feed
+ * jump terms that vanish with zero surface tension. */
+static void compute_normals(fso_state* s) { fill_const(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0); }
+
+/* ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242 (air/water, WENO5):
+ * reads the shared ghost copy and the MAC ghost copies as they are. */
+int fso_levelset_advection(fso_state* s, double dt) {
+    weno5_stage(s, s->phi, s->sghost, 0, dt, s->P.levelset_epsilon);
+    return FS_OK;
+}
+
+/* ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 (air/water branch).
+ * ENO3 reads each component's own ghost copy; the WENO5 branch passes the
+ * x ghost copy for all three components (:314-316). */
+int fso_velocity_advection(fso_state* s, double dt) {
+    if (s->P.use_5th_weno_v) {
+        weno5_stage(s, s->u, s->gu, 1, dt, s->P.velocity_epsilon);
+        weno5_stage(s, s->v, s->gu, 2, dt, s->P.velocity_epsilon);
+        weno5_stage(s, s->w, s->gu, 3, dt, s->P.velocity_epsilon);
+    } else {
+        eno3_stage(s, s->u, s->gu, 1, dt);
+        eno3_stage(s, s->v, s->gv, 2, dt);
+        eno3_stage(s, s->w, s->gw, 3, dt);
+    }
+    return FS_OK;
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::ApplyGravity, :2237-2270: every face. */
+int fso_apply_gravity(fso_state* s, double dt) {
+    double gx = dt * s->P.gravity[0], gy = dt * s->P.gravity[1], gz = dt * s->P.gravity[2];
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i <= s->nx; i++) A3(s->u, i, j, k) += gx;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j <= s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->v, i, j, k) += gy;
+    for (int k = 0; k <= s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->w, i, j, k) += gz;
+    return FS_OK;
+}
+
+/* VISCOSITY_3D::HeavisideFunction, VISCOSITY_3D.h:1234-1256. */
+static double heaviside_visc(double phi, double eps) {
+    double one_over_epsilon = (double)1 / eps, one_over_pi = (double)1 / FS_PI;
+    if (phi < -eps) return 0;
+    if (phi > eps) return 1;
+    return (double)0.5 * ((double)1 + phi * one_over_epsilon + one_over_pi * sin(FS_PI * phi * one_over_epsilon));
+}
+
+/* PROJECTION_3D::HeavisideFunction, PROJECTION_3D.h:1452-1472. */
+static double heaviside_proj(double phi, double eps) {
+    double one_over_epsilon = (double)1 / eps, one_over_pi = (double)1 / FS_PI;
+    if (phi < -eps) return 0;
+    if (phi > eps) return 1;
+    return (double)0.5 + phi * one_over_epsilon * (double)0.5 + (double)0.5 * one_over_pi * sin(FS_PI * phi * one_over_epsilon);
+}
+
+static double mix_density(const fs_params* P, double H) {
+    return P->air_bubble_rising ? P->air_density + (P->water_density - P->air_density) * H
+                                : P->water_density + (P->air_density - P->water_density) * H;
+}
+static double mix_viscosity(const fs_params* P, double H) {
+    return P->air_bubble_rising ? P->air_viscosity + (P->water_viscosity - P->air_viscosity) * H
+                                : P->water_viscosity + (P->air_viscosity - P->water_viscosity) * H;
+}
+
+/* One face of the explicit viscous update; reads velocities from (ux,vy,wz). */
+static double visc_face(fso_state* s, int comp, int i, int j, int k, double dt, const double* ux, const double* vy,
+                        const double* wz) {
+    const fs_params* P = &s->P;
+    const double* f = s->phi;
+    const double eps = (double)1.5 * s->cell.x.d; /* epsilon_for_mollification, EULERIAN_FLUID_SOLVER_3D.h:393 */
+    const double odx = s->cell.x.inv, ody = s->cell.y.inv, odz = s->cell.z.inv;
+#define PH(a, b, c) A3(f, a, b, c)
+#define U(a, b, c) A3(ux, a, b, c)
+#define V(a, b, c) A3(vy, a, b, c)
+#define W(a, b, c) A3(wz, a, b, c)
+#define MU(x) mix_viscosity(P, heaviside_visc((x), eps))
+    if (comp == 1) { /* VISCOSITY_3D.h:462-491 */
+        double lh = (double)0.5 * (PH(i - 1, j, k) + PH(i, j, k));
+        double lcu = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j + 1, k) + PH(i, j + 1, k));
+        double lcd = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j - 1, k) + PH(i, j - 1, k));
+        double lct = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j, k + 1) + PH(i, j, k + 1));
+        double lcb = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j, k - 1) + PH(i, j, k - 1));
+        double rho_h = mix_density(P, heaviside_visc(lh, eps));
+        double mcu = MU(lcu), mcd = MU(lcd), mr = MU(PH(i, j, k)), mc = MU(PH(i - 1, j, k)), mct = MU(lct), mcb = MU(lcb);
+        double coef = dt * ((double)1 / rho_h);
+        double first = (double)2 * coef * odx * (mr * odx * (U(i + 1, j, k) - U(i, j, k)) - mc * ody * (U(i, j, k) - U(i - 1, j, k)));
+        double second = coef * ody * (mcu * (ody * (U(i, j + 1, k) - U(i, j, k)) + odx * (V(i, j + 1, k) - V(i - 1, j + 1, k))) -
+                                      mcd * (ody * (U(i, j, k) - U(i, j - 1, k)) + odx * (V(i, j, k) - V(i - 1, j, k))));
+        double third = coef * odz * (mct * (odz * (U(i, j, k + 1) - U(i, j, k)) + odx * (W(i, j, k + 1) - W(i - 1, j, k + 1))) -
+                                     mcb * (odz * (U(i, j, k) - U(i, j, k - 1)) + odx * (W(i, j, k) - W(i - 1, j, k))));
+        return first + second + third;
+    } else if (comp == 2) { /* VISCOSITY_3D.h:496-524 */
+        double lh = (double)0.5 * (PH(i, j - 1, k) + PH(i, j, k));
+        double lcu = (double)0.25 * (PH(i, j - 1, k) + PH(i + 1, j - 1, k) + PH(i, j, k) + PH(i + 1, j, k));
+        double lcl = (double)0.25 * (PH(i - 1, j - 1, k) + PH(i, j - 1, k) + PH(i - 1, j, k) + PH(i, j, k));
+        double lct = (double)0.25 * (PH(i, j - 1, k + 1) + PH(i, j - 1, k) + PH(i, j, k + 1) + PH(i, j, k));
+        double lcb = (double)0.25 * (PH(i, j - 1, k - 1) + PH(i, j - 1, k) + PH(i, j, k - 1) + PH(i, j, k));
+        double rho_h = mix_density(P, heaviside_visc(lh, eps));
+        double mcu = MU(lcu), mcl = MU(lcl), mu_ = MU(PH(i, j, k)), mc = MU(PH(i, j - 1, k)), mct = MU(lct), mcb = MU(lcb);
+        double coef = dt * ((double)1 / rho_h);
+        double first = coef * odx * (mcu * (ody * (U(i + 1, j, k) - U(i + 1, j - 1, k)) + odx * (V(i + 1, j, k) - V(i, j, k))) -
+                                     mcl * (ody * (U(i, j, k) - U(i, j - 1, k)) + odx * (V(i, j, k) - V(i - 1, j, k))));
+        double second = (double)2 * coef * ody * (mu_ * ody * (V(i, j + 1, k) - V(i, j, k)) - mc * ody * (V(i, j, k) - V(i, j - 1, k)));
+        double third = coef * odz * (mct * (odz * (V(i, j, k + 1) - V(i, j, k)) + ody * (W(i, j, k) - W(i, j - 1, k))) -
+                                     mcb * (odz * (V(i, j, k) - V(i, j, k - 1)) + ody * (W(i, j, k - 1) - W(i, j - 1, k - 1))));
+        return first + second + third;
+    } else { /* VISCOSITY_3D.h:530-559 */
+        double lh = (double)0.5 * (PH(i, j, k - 1) + PH(i, j, k));
+        double lct = (double)0.25 * (PH(i, j, k - 1) + PH(i + 1, j, k - 1) + PH(i, j, k) + PH(i + 1, j, k));
+        double lcb = (double)0.25 * (PH(i, j, k - 1) + PH(i - 1, j, k - 1) + PH(i, j, k) + PH(i - 1, j, k));
+        double lcu = (double)0.25 * (PH(i, j, k - 1) + PH(i, j + 1, k - 1) + PH(i, j, k) + PH(i, j + 1, k));
+        double lcd = (double)0.25 * (PH(i, j, k - 1) + PH(i, j - 1, k - 1) + PH(i, j, k) + PH(i, j - 1, k));
+        double rho_h = mix_density(P, heaviside_visc(lh, eps));
+        double mct = MU(lct), mcb = MU(lcb), mcu = MU(lcu), mcd = MU(lcd), mt = MU(PH(i, j, k)), mc = MU(PH(i, j, k - 1));
+        double coef = dt * ((double)1 / rho_h);
+        double first = coef * odx * (mct * (odz * (U(i, j, k) - U(i, j, k - 1)) + odx * (W(i + 1, j, k) - W(i, j, k))) -
+                                     mcb * (odz * (U(i - 1, j, k) - U(i - 1, j, k - 1)) + odx * (W(i, j, k) - W(i - 1, j, k))));
+        double second = coef * ody * (mcu * (odz * (V(i, j, k) - V(i, j, k - 1)) + ody * (W(i, j + 1, k) - W(i, j, k))) -
+                                      mcd * (odz * (V(i, j - 1, k) - V(i, j - 1, k - 1)) + ody * (W(i, j, k) - W(i, j - 1, k))));
+        double third = (double)2 * coef * odz * (mt * odz * (W(i, j, k + 1) - W(i, j, k)) - mc * odz * (W(i, j, k) - W(i, j, k - 1)));
+        return first + second + third;
+    }
+#undef PH
+#undef U
+#undef V
+#undef W
+#undef MU
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::Sourcing (:2076-2100) -> VISCOSITY_3D::ApplyViscosity
+ * air/water explicit branch, VISCOSITY_3D.h:448-563: components x, y, z in
+ * turn, each updated in place (in-place mode) or from a copy of itself. */
+int fso_sourcing(fso_state* s, double dt) {
+    if (!s->P.use_delta_function_formulation) return FS_OK;
+    double* comps[3] = {s->u, s->v, s->w};
+    for (int c = 1; c <= 3; c++) {
+        int ni, nj, nk;
+        extents(s, c, &ni, &nj, &nk);
+        double* fld = comps[c - 1];
+        if (s->visc_inplace) {
+            for (int k = 0; k < nk; k++)
+                for (int j = 0; j < nj; j++)
+                    for (int i = 0; i < ni; i++) A3(fld, i, j, k) += visc_face(s, c, i, j, k, dt, s->u, s->v, s->w);
+        } else {
+            memcpy(s->vtmp, fld, sizeof(double) * (size_t)s->ntot);
+            const double* ux = c == 1 ? s->vtmp : s->u;
+            const double* vy = c == 2 ? s->vtmp : s->v;
+            const double* wz = c == 3 ? s->vtmp : s->w;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+            for (int k = 0; k < nk; k++)
+                for (int j = 0; j < nj; j++)
+                    for (int i = 0; i < ni; i++) A3(fld, i, j, k) = A3(s->vtmp, i, j, k) + visc_face(s, c, i, j, k, dt, ux, vy, wz);
+        }
+    }
+    return FS_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Projection                                                                 */
+/* ------------------------------------------------------------------------- */
+
+static double plane_sum_dot(fso_state* s, const double* a, const double* b) {
+    /* rows are numbered plane by plane; per-plane partials in order */
+    int per = s->nx * s->ny;
+    double total = 0;
+    for (int k = 0; k < s->nz; k++) {
+        double part = 0;
+        const double* aa = a + (long)k * per;
+        const double* bb = b + (long)k * per;
+        for (int r = 0; r < per; r++) part += aa[r] * bb[r];
+        total += part;
+    }
+    return total;
+}
+
+/* PROJECTION_3D::DetermineProjectionDensity, PROJECTION_3D.h:596-679. */
+static void projection_density(fso_state* s) {
+    const fs_params* P = &s->P;
+    double light = P->air_bubble_rising ? P->air_density : P->water_density;
+    double heavy = P->air_bubble_rising ? P->water_density : P->air_density;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                double f = A3(s->phi, i, j, k);
+                if (f <= 0) A3(s->rho, i, j, k) = light;
+                else if (f > 0) A3(s->rho, i, j, k) = heavy;
+            }
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->irho, i, j, k) = 1 / A3(s->rho, i, j, k);
+}
+
+/* PROJECTION_3D::DetermineDivergence, PROJECTION_3D.h:681-744 (air/water). */
+static void projection_divergence(fso_state* s, double dt) {
+    fill_zero(s, s->gu, s->u, s->nx + 1, s->ny, s->nz, 3);
+    fill_zero(s, s->gv, s->v, s->nx, s->ny + 1, s->nz, 3);
+    fill_zero(s, s->gw, s->w, s->nx, s->ny, s->nz + 1, 3);
+    double odx = s->cell.x.inv, ody = s->cell.y.inv, odz = s->cell.z.inv;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++)
+                A3(s->div, i, j, k) = odx * (A3(s->gu, i + 1, j, k) - A3(s->gu, i, j, k)) +
+                                      ody * (A3(s->gv, i, j + 1, k) - A3(s->gv, i, j, k)) +
+                                      odz * (A3(s->gw, i, j, k + 1) - A3(s->gw, i, j, k));
+    const double one_over_dt = 1 / dt;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->div, i, j, k) *= one_over_dt;
+}
+
+/* PROJECTION_3D::SetupBoundaryConditions (Neumann, air/water, :1000-1016) and
+ * POISSON_SOLVER_3D::AssignSequentialindicesToFullCells (:527-552). */
+static void setup_bc(fso_state* s) {
+    int row = 0;
+    for (int k = -1; k <= s->nz; k++)
+        for (int j = -1; j <= s->ny; j++)
+            for (int i = -1; i <= s->nx; i++) {
+                int inside = i >= 0 && i < s->nx && j >= 0 && j < s->ny && k >= 0 && k < s->nz;
+                s->bc[IX(s, i, j, k)] = inside ? FS_BC_FULL : FS_BC_NEUM;
+            }
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++)
+                if (s->bc[IX(s, i, j, k)] > -1) s->bc[IX(s, i, j, k)] = row++;
+    s->N = row;
+}
+
+#define GFM_BETA(vc, vn, fn, fc) ((vc) * (vn) * (fabs(fn) + fabs(fc)) / ((vc) * fabs(fn) + (vn) * fabs(fc)))
+#define GFM_BETA_R(vc, vn, fn, fc) ((vc) * (vn) * (fabs(fn) + fabs(fc)) / ((vn) * fabs(fc) + (vc) * fabs(fn)))
+
+/* PROJECTION_3D::DeterminePressure (:794-834) up to the solve:
+ * POISSON_SOLVER_3D::BuildLinearSystemNodeJumpConditionVaribleCoefficient
+ * (:1030-1358, air/water, zero jump) and the Neumann mean removal (:225-246). */
+static void assemble(fso_state* s) {
+    const fs_params* P = &s->P;
+    if (!P->use_previous_pressure)
+        for (long n = 0; n < s->ntot; n++) s->p[n] = 0;
+    setup_bc(s);
+    compute_normals(s);
+    fill_const(s, s->irho, s->irho, s->nx, s->ny, s->nz, 2, 0); /* :813 */
+    free_csr(s);
+    int N = s->N;
+    s->rp = (int*)calloc((size_t)N + 1, sizeof(int));
+    s->ci = (int*)calloc((size_t)N * 7, sizeof(int));
+    s->val = (double*)calloc((size_t)N * 7, sizeof(double));
+    s->b = (double*)calloc((size_t)N, sizeof(double));
+    s->x = (double*)calloc((size_t)N, sizeof(double));
+    s->r = (double*)calloc((size_t)N, sizeof(double));
+    s->zv = (double*)calloc((size_t)N, sizeof(double));
+    s->pp = (double*)calloc((size_t)N, sizeof(double));
+    s->Ap = (double*)calloc((size_t)N, sizeof(double));
+    const double inv_dxdx = (double)1 / (s->cell.x.d * s->cell.x.d);
+    const double inv_dydy = (double)1 / (s->cell.y.d * s->cell.y.d);
+    const double inv_dzdz = (double)1 / (s->cell.z.d * s->cell.z.d);
+    const double* vv = s->irho;
+    const double* f = s->phi;
+    int nz_ix = 0;
+    int i0 = 0, j0 = 0, k0 = 0;
+    memset(s->fbx, 0, sizeof(double) * (size_t)s->ntot);
+    memset(s->fby, 0, sizeof(double) * (size_t)s->ntot);
+    memset(s->fbz, 0, sizeof(double) * (size_t)s->ntot);
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                int row = s->bc[IX(s, i, j, k)];
+                if (row < 0) continue;
+                double vc = A3(vv, i, j, k), fc = A3(f, i, j, k);
+                double bl = GFM_BETA(vc, A3(vv, i - 1, j, k), A3(f, i - 1, j, k), fc);
+                double br = GFM_BETA_R(vc, A3(vv, i + 1, j, k), A3(f, i + 1, j, k), fc);
+                double bb = GFM_BETA(vc, A3(vv, i, j - 1, k), A3(f, i, j - 1, k), fc);
+                double bt = GFM_BETA_R(vc, A3(vv, i, j + 1, k), A3(f, i, j + 1, k), fc);
+                double bd = GFM_BETA(vc, A3(vv, i, j, k - 1), A3(f, i, j, k - 1), fc);
+                double bu = GFM_BETA_R(vc, A3(vv, i, j, k + 1), A3(f, i, j, k + 1), fc);
+                int cl = s->bc[IX(s, i - 1, j, k)], cr = s->bc[IX(s, i + 1, j, k)];
+                int cb = s->bc[IX(s, i, j - 1, k)], ct = s->bc[IX(s, i, j + 1, k)];
+                int cd = s->bc[IX(s, i, j, k - 1)], cu = s->bc[IX(s, i, j, k + 1)];
+                double coef = 0;
+                s->rp[row] = nz_ix;
+                if (cl > -1) A3(s->fbx, i, j, k) = bl;
+                if (cb > -1) A3(s->fby, i, j, k) = bb;
+                if (cd > -1) A3(s->fbz, i, j, k) = bd;
+                if (cl > -1) { coef += bl; s->ci[nz_ix] = cl; s->val[nz_ix++] = -bl * inv_dxdx; }
+                if (cr > -1) { coef += br; s->ci[nz_ix] = cr; s->val[nz_ix++] = -br * inv_dxdx; }
+                if (cb > -1) { coef += bb; s->ci[nz_ix] = cb; s->val[nz_ix++] = -bb * inv_dydy; }
+                if (ct > -1) { coef += bt; s->ci[nz_ix] = ct; s->val[nz_ix++] = -bt * inv_dydy; }
+                if (cd > -1) { coef += bd; s->ci[nz_ix] = cd; s->val[nz_ix++] = -bd * inv_dzdz; }
+                if (cu > -1) { coef += bu; s->ci[nz_ix] = cu; s->val[nz_ix++] = -bu * inv_dzdz; }
+                if (row == 0) { i0 = i; j0 = j; k0 = k; }
+                if (i == i0 && j == j0 && k == k0) { /* Neumann pin, :1145-1179 */
+                    if (cb == FS_BC_NEUM) coef += bb;
+                    if (cd == FS_BC_NEUM) coef += bd;
+                    if (cu == FS_BC_NEUM) coef += bu;
+                }
+                if (coef == 0) coef = 1;
+                s->ci[nz_ix] = row;
+                s->val[nz_ix++] = inv_dxdx * coef;
+                s->b[row] = -A3(s->div, i, j, k);
+                s->x[row] = A3(s->p, i, j, k);
+            }
+    s->rp[N] = nz_ix;
+    s->nnz = nz_ix;
+    /* Neumann compatibility: b -= mean(b), POISSON_SOLVER_3D.h:225-246 */
+    double sum = 0;
+    int per = s->nx * s->ny;
+    for (int k = 0; k < s->nz; k++) {
+        double part = 0;
+        for (int r = k * per; r < (k + 1) * per; r++) part += s->b[r];
+        sum += part;
+    }
+    double one_over_b_dimension = 1 / (double)N;
+    for (int r = 0; r < N; r++) s->b[r] -= one_over_b_dimension * sum;
+}
+
+static void get(fso_state* s, const double* src, double* dst, int ni, int nj, int nk);
+
+static double csr_diag(const fso_state* s, int row) {
+    for (int e = s->rp[row]; e < s->rp[row + 1]; e++)
+        if (s->ci[e] == row) return s->val[e];
+    return 0;
+}
+
+/* CSR_MATRIX::Multiply, CSR_MATRIX.h:236-260 */
+static void csr_mul(fso_state* s, const double* x, double* y) {
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int row = 0; row < s->N; row++) {
+        double v = 0;
+        for (int e = s->rp[row]; e < s->rp[row + 1]; e++) v += s->val[e] * x[s->ci[e]];
+        y[row] = v;
+    }
+}
+
+/* PCG_METHOD::PCGMethod, PCG_METHOD.cpp:4-126, with the Jacobi
+ * preconditioner of MultiplicationByMinverseAsDiagonal (:128-160). */
+static void pcg(fso_state* s) {
+    const fs_params* P = &s->P;
+    int N = s->N;
+    double *x = s->x, *r = s->r, *sv = s->zv, *p = s->pp, *Ap = s->Ap, *b = s->b;
+    double* dg = (double*)malloc(sizeof(double) * (size_t)N);
+    for (int i = 0; i < N; i++) dg[i] = csr_diag(s, i);
+    csr_mul(s, x, Ap);
+    for (int i = 0; i < N; i++) r[i] = b[i] - Ap[i];
+    for (int i = 0; i < N; i++) p[i] = r[i] / dg[i];
+    double res_new = plane_sum_dot(s, r, p);
+    double bnorm = sqrt(plane_sum_dot(s, b, b));
+    double sqr_tol = P->tolerance * P->tolerance;
+    int it = 0;
+    double rr = plane_sum_dot(s, r, r);
+    int rel = P->stop_rule == FS_STOP_REL_RESIDUAL;
+    if (rel && (bnorm == 0 || sqrt(rr) <= P->rel_tolerance * bnorm)) goto done;
+    while (it < P->max_iteration) {
+        csr_mul(s, p, Ap);
+        double dot_result = plane_sum_dot(s, p, Ap);
+        double alpha = res_new / dot_result;
+        for (int i = 0; i < N; i++) x[i] += alpha * p[i];
+        for (int i = 0; i < N; i++) r[i] -= alpha * Ap[i];
+        for (int i = 0; i < N; i++) sv[i] = r[i] / dg[i];
+        double res_old = res_new;
+        res_new = plane_sum_dot(s, r, sv);
+        if (rel) {
+            rr = plane_sum_dot(s, r, r);
+            if (sqrt(rr) < P->rel_tolerance * bnorm) break;
+        } else if (res_new < sqr_tol)
+            break;
+        double beta = res_new / res_old;
+        for (int i = 0; i < N; i++) {
+            p[i] *= beta;
+            p[i] += sv[i];
+        }
+        it++;
+    }
+done:
+    s->last_iters = it;
+    s->last_res_abs = sqrt(res_new);
+    /* FP64 true residual for the report */
+    csr_mul(s, x, Ap);
+    double tr = 0;
+    for (int i = 0; i < N; i++) {
+        double d = b[i] - Ap[i];
+        tr += d * d;
+    }
+    s->last_res_rel = bnorm > 0 ? sqrt(tr) / bnorm : 0;
+    free(dg);
+}
+
+/* PROJECTION_3D::UpdateVelocityByPressureGradientVariableDensity,
+ * PROJECTION_3D.h:1099-1449 (air/water).  comp 1,2,3 = x,y,z faces. */
+static double gradient_update(fso_state* s, int comp, double dt, int apply) {
+    const fs_params* P = &s->P;
+    int ni, nj, nk;
+    extents(s, comp, &ni, &nj, &nk);
+    int di = comp == 1, dj = comp == 2, dk = comp == 3;
+    double* vel = comp == 1 ? s->u : comp == 2 ? s->v : s->w;
+    double od = comp == 1 ? s->cell.x.inv : comp == 2 ? s->cell.y.inv : s->cell.z.inv;
+    double dgrid = comp == 1 ? s->cell.x.d : comp == 2 ? s->cell.y.d : s->cell.z.d;
+    double eps = (double)1.5 * dgrid; /* :1126, :1137, :1148 */
+    double vmax = 0;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                int bcc = s->bc[IX(s, i, j, k)];
+                if (bcc < 0) continue;
+                double lface = (double)0.5 * (A3(s->pg, i, j, k) + A3(s->pg, i - di, j - dj, k - dk));
+                double density_half = 0;
+                /* PROJECTION_3D::DetermineDensityField, :1474-1519 */
+                double H = heaviside_proj(lface, eps);
+                density_half = P->air_bubble_rising ? P->air_density + (P->water_density - P->air_density) * H
+                                                    : P->water_density + (P->air_density - P->water_density) * H;
+                double density_ijk = A3(s->rho, i, j, k);
+                double one_over_density_ijk = (double)1 / density_ijk;
+                double lc = A3(s->phi, i, j, k), ln = A3(s->phi, i - di, j - dj, k - dk);
+                double jump_gamma = ((double)0 * fabs(ln) + (double)0 * fabs(lc)) / (fabs(ln) + fabs(lc));
+                double one_over_density_half = 1 / density_half;
+                double bcd = (double)bcc, bcn = (double)s->bc[IX(s, i - di, j - dj, k - dk)];
+                double* vr = &A3(vel, i, j, k);
+                double pc = A3(s->p, i, j, k), pn = A3(s->p, i - di, j - dj, k - dk);
+                if (apply && bcd * bcn >= 0) {
+                    if (lc * ln > 0) {
+                        *vr -= dt * (pc - pn) * one_over_density_ijk * od;
+                    } else {
+                        if (lc <= 0) *vr -= dt * (pc - pn + jump_gamma) * one_over_density_half * od;
+                        else if (lc > 0) *vr -= dt * (pc - pn - jump_gamma) * one_over_density_half * od;
+                    }
+                }
+                double a = fabs(*vr);
+                vmax = vmax > a ? vmax : a;
+            }
+    return vmax;
+}
+
+static void projection_solve(fso_state* s, double dt) {
+    projection_density(s);
+    projection_divergence(s, dt);
+    /* DetermineJumpConditionField, :746-792: ComputeCurvatures' ghost refresh;
+     * with zero surface tension jc = 0 */
+    fso_compute_curvatures(s);
+    assemble(s);
+    pcg(s);
+    /* POISSON_SOLVER_3D::VectorToGrid, :4060-4078 */
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->p, i, j, k) = s->x[s->bc[IX(s, i, j, k)]];
+    /* UpdateVelocity, :836-841; the projection's own ghost copy of phi is
+     * refreshed by linear extrapolation before each component (:1115) */
+    fill_linear(s, s->pg, s->phi, s->nx, s->ny, s->nz, 2, 1, 0);
+    /* air/water corrects z-faces only: the x and y updates sit inside the
+     * pipe-mode branches (:1188-1232, :1297-1341) while the max |u| is still
+     * taken over every visited face */
+    int all = s->P.schedule == FS_SCHEDULE_FIXED;
+    s->max_vel[0] = gradient_update(s, 1, dt, all);
+    s->max_vel[1] = gradient_update(s, 2, dt, all);
+    s->max_vel[2] = gradient_update(s, 3, dt, 1);
+}
+
+int fso_projection(fso_state* s, double dt, fs_step_stats* st) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    projection_solve(s, dt);
+    if (st) {
+        st->pcg_iterations = s->last_iters;
+        st->refinement_sweeps = 0;
+        st->residual_precond = s->last_res_abs;
+        st->residual_rel = s->last_res_rel;
+        st->max_velocity[0] = s->max_vel[0];
+        st->max_velocity[1] = s->max_vel[1];
+        st->max_velocity[2] = s->max_vel[2];
+    }
+    return s->last_iters >= s->P.max_iteration ? FS_ERR_NOT_CONVERGED : FS_OK;
+}
+
+int fso_assemble_pressure_system(fso_state* s, double dt, double* diag, double* bx, double* by, double* bz, double* rhs) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    projection_density(s);
+    projection_divergence(s, dt);
+    fso_compute_curvatures(s);
+    assemble(s);
+    int nx = s->nx, ny = s->ny, nz = s->nz;
+    for (int k = 0; k < nz; k++)
+        for (int j = 0; j < ny; j++)
+            for (int i = 0; i < nx; i++) {
+                int row = s->bc[IX(s, i, j, k)];
+                long c = i + (long)nx * (j + (long)ny * k);
+                if (diag) diag[c] = csr_diag(s, row);
+                if (rhs) rhs[c] = s->b[row];
+            }
+    /* face betas between two full cells, as each row computed them (beta_l,
+     * beta_b, beta_d of the cell on the + side of the face) */
+    if (bx) get(s, s->fbx, bx, nx + 1, ny, nz);
+    if (by) get(s, s->fby, by, nx, ny + 1, nz);
+    if (bz) get(s, s->fbz, bz, nx, ny, nz + 1);
+    return FS_OK;
+}
+
+int fso_apply_pressure_operator(fso_state* s, const double* x, double* q) {
+    if (!s->rp) return FS_ERR_BAD_ARG;
+    csr_mul(s, x, q);
+    return FS_OK;
+}
+
+int fso_csr_info(fso_state* s, int* n, int* nnz) {
+    if (!s->rp) return FS_ERR_BAD_ARG;
+    *n = s->N;
+    *nnz = s->nnz;
+    return FS_OK;
+}
+
+int fso_csr_copy(fso_state* s, int* row_ptr, int* col, double* val, double* b) {
+    if (!s->rp) return FS_ERR_BAD_ARG;
+    if (row_ptr) memcpy(row_ptr, s->rp, sizeof(int) * (size_t)(s->N + 1));
+    if (col) memcpy(col, s->ci, sizeof(int) * (size_t)s->nnz);
+    if (val) memcpy(val, s->val, sizeof(double) * (size_t)s->nnz);
+    if (b) memcpy(b, s->b, sizeof(double) * (size_t)s->N);
+    return FS_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Reinitialization                                                           */
+/* ------------------------------------------------------------------------- */
+
+enum { AX_M = 1, AX_P = 2, AX_DROP = 3, AX_SM = 4, AX_SP = 5, AX_NONE = 0 };
+
+/* One axis of the Godunov case tree, ADVECTION_METHOD_3D.h:908-1622. */
+static int godunov_axis(double sg, double m, double p, double* val) {
+    double sm = sg * m, sp = sg * p;
+    if (sp >= 0 && sm >= 0) { *val = m; return AX_M; }
+    if (sp <= 0 && sm <= 0) { *val = p; return AX_P; }
+    if (sp > 0 && sm < 0) { *val = 0; return AX_DROP; }
+    if (sp < 0 && sm > 0) {
+        double ss = sg * (fabs(p) - fabs(m)) / (p - m);
+        if (ss > 0) { *val = m; return AX_SM; }
+        *val = p;
+        return AX_SP;
+    }
+    return AX_NONE;
+}
+
+/* ADVECTION_METHOD_3D::WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637. */
+static void reinit_sweep(fso_state* s, double dt) {
+    const double eps = s->P.levelset_epsilon;
+    const gc_t* gc = &s->cell;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                double f[7], mx, px, my, py, mz, pz, X, Y, Z;
+                gather7(s, s->sghost, i, j, k, 0, f);
+                fso_weno5_1d(f, gc->x.inv, eps, &mx, &px);
+                gather7(s, s->sghost, i, j, k, 1, f);
+                fso_weno5_1d(f, gc->y.inv, eps, &my, &py);
+                gather7(s, s->sghost, i, j, k, 2, f);
+                fso_weno5_1d(f, gc->z.inv, eps, &mz, &pz);
+                double sg = A3(s->sgn, i, j, k);
+                double* rho = &A3(s->phi, i, j, k);
+                int cx = godunov_axis(sg, mx, px, &X);
+                int cy = cx ? godunov_axis(sg, my, py, &Y) : AX_NONE;
+                int cz = cy ? godunov_axis(sg, mz, pz, &Z) : AX_NONE;
+                if (cx && cy && cz) {
+                    double g;
+                    if ((cx == AX_SM || cx == AX_SP) && cy == AX_SP && cz == AX_DROP)
+                        g = sqrt(X * X) + Y * Y - 1; /* :1463, :1605 */
+                    else
+                        g = sqrt(X * X + Y * Y + Z * Z) - 1;
+                    *rho = *rho - dt * sg * g;
+                }
+                if (sg * *rho < (double)0) { /* :1624-1634 */
+                    if (sg < 0) *rho = -eps;
+                    if (sg > 0) *rho = eps;
+                }
+            }
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::Reinitialization, :2198-2225, Sussman branch;
+ * each sweep: ADVECTION_3D::ReinitializationBySussman, ADVECTION_3D.h:360-378. */
+int fso_reinitialization(fso_state* s, double dt) {
+    (void)dt;
+    if (!s->P.sussmanstyle_reinitialization) return FS_OK;
+    double fictious_dt = s->cell.x.d / s->P.scaling_number_for_reinitialization;
+    double dx2 = s->cell.x.d * s->cell.x.d;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                double f = A3(s->phi, i, j, k);
+                A3(s->sgn, i, j, k) = f / sqrt(f * f + dx2);
+            }
+    for (int it = 0; it < s->P.iteration_number_for_reinitialization; it++) {
+        fill_linear(s, s->sghost, s->phi, s->nx, s->ny, s->nz, 3, 1, 0);
+        reinit_sweep(s, fictious_dt);
+    }
+    return FS_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Step                                                                       */
+/* ------------------------------------------------------------------------- */
+
+static void save_phi(fso_state* s) {
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->tmp, i, j, k) = A3(s->phi, i, j, k);
+}
+static void average_phi(fso_state* s) {
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++)
+                A3(s->phi, i, j, k) = (double)0.5 * (A3(s->tmp, i, j, k) + A3(s->phi, i, j, k));
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread, order 1, :806-858;
+ * SolveForRK :1453-1476. */
+int fso_step(fso_state* s, double dt, fs_step_stats* st) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    const fs_params* P = &s->P;
+    int rc = FS_OK;
+    if (P->is_levelset_advection_active) {
+        save_phi(s);
+        fso_levelset_advection(s, dt);
+        fso_levelset_advection(s, dt);
+        average_phi(s);
+    }
+    if (P->is_velocity_advection_active) fso_velocity_advection(s, dt);
+    if (P->is_external_force_active) fso_apply_gravity(s, dt);
+    if (P->is_sourcing_active) fso_sourcing(s, dt);
+    if (P->is_projection_active) rc = fso_projection(s, dt, st);
+    save_phi(s);
+    fso_reinitialization(s, dt);
+    fso_reinitialization(s, dt);
+    average_phi(s);
+    return rc;
+}
+
+/* SIMULATION_WORLD::AdvanceOneFrame, SIMULATION_WORLD.h:320-330. */
+int fso_advance_frame(fso_state* s, double dt, fs_step_stats* st) {
+    fso_compute_curvatures(s);
+    return fso_step(s, dt, st);
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::CFLOneTimeStep, :2673-2785 (air/water). */
+int fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt) {
+    const fs_params* P = &s->P;
+    double um = 0, vm = 0, wm = 0;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i <= s->nx; i++)
+                if (um <= fabs(A3(s->u, i, j, k))) um = fabs(A3(s->u, i, j, k));
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j <= s->ny; j++)
+            for (int i = 0; i < s->nx; i++)
+                if (vm <= fabs(A3(s->v, i, j, k))) vm = fabs(A3(s->v, i, j, k));
+    for (int k = 0; k <= s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++)
+                if (wm <= fabs(A3(s->w, i, j, k))) wm = fabs(A3(s->w, i, j, k));
+    if (!isfinite(um) || !isfinite(vm) || !isfinite(wm)) return FS_ERR_NONFINITE;
+    double c_f = um * s->mx.x.inv + vm * s->my.y.inv + wm * s->mz.z.inv;
+    double g_f = 0;
+    if (P->is_external_force_active) {
+        double gmag = gravity_norm ? sqrt(P->gravity[0] * P->gravity[0] + P->gravity[1] * P->gravity[1] +
+                                          P->gravity[2] * P->gravity[2])
+                                   : fabs(P->gravity[1]);
+        g_f = sqrt(gmag * s->my.y.inv);
+    }
+    double s_f = 0;
+    double a = P->water_viscosity / P->water_density, b = P->air_viscosity / P->air_density;
+    double v_f = (a > b ? a : b) * ((double)2 * s->cell.x.inv_d2 + (double)2 * s->cell.y.inv_d2 + (double)2 * s->cell.z.inv_d2);
+    double c = (double)0.5 * (c_f + v_f + sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) + (double)4 * (s_f * s_f)));
+    *dt = cfl * ((double)1 / c);
+    return FS_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Field exchange                                                             */
+/* ------------------------------------------------------------------------- */
+
+static void put(fso_state* s, double* dst, const double* src, int ni, int nj, int nk) {
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) A3(dst, i, j, k) = src[i + (long)ni * (j + (long)nj * k)];
+}
+static void get(fso_state* s, const double* src, double* dst, int ni, int nj, int nk) {
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) dst[i + (long)ni * (j + (long)nj * k)] = A3(src, i, j, k);
+}
+
+int fso_set_fields(fso_state* s, const double* phi, const double* u, const double* v, const double* w, const double* p) {
+    if (phi) {
+        put(s, s->phi, phi, s->nx, s->ny, s->nz);
+        fill_const(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0); /* EULERIAN_FLUID_SOLVER_3D.h:757 */
+    }
+    if (u) put(s, s->u, u, s->nx + 1, s->ny, s->nz);
+    if (v) put(s, s->v, v, s->nx, s->ny + 1, s->nz);
+    if (w) put(s, s->w, w, s->nx, s->ny, s->nz + 1);
+    if (p) put(s, s->p, p, s->nx, s->ny, s->nz);
+    return FS_OK;
+}
+
+int fso_get_fields(fso_state* s, double* phi, double* u, double* v, double* w, double* p) {
+    if (phi) get(s, s->phi, phi, s->nx, s->ny, s->nz);
+    if (u) get(s, s->u, u, s->nx + 1, s->ny, s->nz);
+    if (v) get(s, s->v, v, s->nx, s->ny + 1, s->nz);
+    if (w) get(s, s->w, w, s->nx, s->ny, s->nz + 1);
+    if (p) get(s, s->p, p, s->nx, s->ny, s->nz);
+    return FS_OK;
+}

@@ -1,0 +1,71 @@
+/*
+ * fs_oracle.h -- TEST INFRASTRUCTURE ONLY.
+ *
+ * CPU restatement (plain C, FP64) of the reference's per-timestep hot path
+ * (EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread and everything under
+ * it, air/water mode).  It is the checker for the HIP library: only tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load it.  The
+ * product path never calls it.
+ *
+ * Entry points mirror include/fs_abi.h with the prefix fso_ and the same
+ * argument meaning, so a parity test calls both sides the same way.
+ */
+#ifndef FS_ORACLE_H
+#define FS_ORACLE_H
+#include "../include/fs_abi.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct fso_state fso_state;
+
+int  fso_create(const fs_params* params, const fs_grid* grid, fso_state** out);
+void fso_destroy(fso_state* s);
+int  fso_set_fields(fso_state* s, const double* phi, const double* u, const double* v,
+                    const double* w, const double* p);
+int  fso_get_fields(fso_state* s, double* phi, double* u, double* v, double* w, double* p);
+
+/* 1 = explicit viscosity updated in place in lexicographic order, as the
+ * reference's single-thread sweep does (VISCOSITY_3D.h:483-559);
+ * 0 = out-of-place (every face reads the pre-sweep component), the order the
+ * HIP kernels use.  Default 1. */
+void fso_set_viscosity_inplace(fso_state* s, int inplace);
+/* Worker threads for the stencil loops (results do not depend on it). */
+void fso_set_threads(int n);
+
+int  fso_advance_frame(fso_state* s, double dt, fs_step_stats* st);
+int  fso_step(fso_state* s, double dt, fs_step_stats* st);
+int  fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt);
+
+int  fso_compute_curvatures(fso_state* s);
+int  fso_levelset_advection(fso_state* s, double dt);
+int  fso_velocity_advection(fso_state* s, double dt);
+int  fso_apply_gravity(fso_state* s, double dt);
+int  fso_sourcing(fso_state* s, double dt);
+int  fso_projection(fso_state* s, double dt, fs_step_stats* st);
+int  fso_reinitialization(fso_state* s, double dt);
+
+int  fso_assemble_pressure_system(fso_state* s, double dt, double* diag, double* bx,
+                                  double* by, double* bz, double* rhs);
+int  fso_apply_pressure_operator(fso_state* s, const double* x, double* q);
+
+/* Sparse view of the last assembly, for tests that check the operator:
+ * n rows, nnz entries, CSR arrays copied into caller buffers when non-NULL. */
+int  fso_csr_info(fso_state* s, int* n, int* nnz);
+int  fso_csr_copy(fso_state* s, int* row_ptr, int* col, double* val, double* b);
+
+/* Raw 1-D kernels for closed-form pins. */
+/* WENO5-JS one-sided derivatives from 7 samples v[0..6] centred on v[3]
+ * (ADVECTION_METHOD_3D.h:55-183): writes d_minus, d_plus. */
+void fso_weno5_1d(const double* v, double one_over_dx, double epsilon,
+                  double* d_minus, double* d_plus);
+/* ENO3 one-sided derivatives from v[0..6] centred on v[3]
+ * (ADVECTION_METHOD_3D.h:1653-1718). */
+void fso_eno3_1d(const double* v, double dx, double one_over_dx, double one_over_2dx,
+                 double one_over_3dx, double* d_minus, double* d_plus);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

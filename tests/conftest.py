@@ -1,0 +1,39 @@
+import os
+import sys
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+ORACLE_LIB = os.path.join(ROOT, "oracle", "libfs_oracle.so")
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (runs through libfs_hip.so)")
+
+
+def _gpu_available():
+    try:
+        import torch
+        return torch.cuda.is_available()
+    except Exception:
+        return False
+
+
+@pytest.fixture(scope="session")
+def oracle_lib():
+    from fluid_simulator_backup_amd import _abi
+    if not os.path.exists(ORACLE_LIB):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    return _abi.load(ORACLE_LIB, "fso_")
+
+
+@pytest.fixture(scope="session")
+def hip_lib():
+    if not _gpu_available():
+        pytest.skip("no GPU visible")
+    import fluid_simulator_backup_amd as fsa
+    return fsa.load_library()  # raises if the HIP build is missing: no fallback
