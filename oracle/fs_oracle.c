@@ -1060,7 +1060,10 @@ int fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt) {
         for (int j = 0; j < s->ny; j++)
             for (int i = 0; i < s->nx; i++)
                 if (wm <= fabs(A3(s->w, i, j, k))) wm = fabs(A3(s->w, i, j, k));
-    if (!isfinite(um) || !isfinite(vm) || !isfinite(wm)) return FS_ERR_NONFINITE;
+    /* the reference's max skips NaN (a comparison with NaN is false); the
+     * build reports non-finite velocities instead */
+    for (long n = 0; n < s->ntot; n++)
+        if (!isfinite(s->u[n]) || !isfinite(s->v[n]) || !isfinite(s->w[n])) return FS_ERR_NONFINITE;
     double c_f = um * s->mx.x.inv + vm * s->my.y.inv + wm * s->mz.z.inv;
     double g_f = 0;
     if (P->is_external_force_active) {

@@ -1,0 +1,324 @@
+"""Pins of the CPU checker against facts the mathematics fixes.
+
+The reference has no tests or golden data and cannot be built here (it
+needs Boost, Windows headers and MSVC template leniency), so each oracle
+function is pinned by a known answer that a plausible slip (dropped term,
+wrong sign, swapped operand, wrong stencil index) would break:
+  - WENO5 / ENO3: exact on quadratics / cubics (golden KATs in
+    tests/golden/derivative_kats.json), fifth-order convergence;
+  - reinitialization: a plane distance function is a fixed point; a
+    stretched sphere returns to unit gradient with the zero level held and
+    signs preserved;
+  - pressure system: bitwise symmetry, the textbook Neumann Laplacian for
+    one phase, the ghost-fluid effective density across an interface, a
+    dense direct solve, and a divergence-free projection;
+  - viscosity: the exact discrete diffusion of a shear wave;
+  - gravity, CFL: closed forms.
+"""
+import ctypes as C
+import json
+import os
+
+import numpy as np
+import pytest
+
+from fluid_simulator_backup_amd import _abi, scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PD = C.POINTER(C.c_double)
+
+
+@pytest.fixture(scope="module")
+def lib(oracle_lib):
+    oracle_lib.fso_weno5_1d.argtypes = [PD, C.c_double, C.c_double, PD, PD]
+    oracle_lib.fso_eno3_1d.argtypes = [PD, C.c_double, C.c_double, C.c_double, C.c_double, PD, PD]
+    oracle_lib.fso_csr_info.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]
+    oracle_lib.fso_csr_copy.argtypes = [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int), PD, PD]
+    oracle_lib.fso_set_viscosity_inplace.argtypes = [C.c_void_p, C.c_int]
+    return oracle_lib
+
+
+def weno(lib, s, h, eps=1e-6):
+    m, p = C.c_double(), C.c_double()
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    lib.fso_weno5_1d(s.ctypes.data_as(PD), 1 / h, eps, C.byref(m), C.byref(p))
+    return m.value, p.value
+
+
+def eno(lib, s, h):
+    m, p = C.c_double(), C.c_double()
+    s = np.ascontiguousarray(s, dtype=np.float64)
+    lib.fso_eno3_1d(s.ctypes.data_as(PD), h, 1 / h, 0.5 / h, (1 / h) / 3, C.byref(m), C.byref(p))
+    return m.value, p.value
+
+
+KATS = json.load(open(os.path.join(HERE, "golden", "derivative_kats.json")))
+
+
+@pytest.mark.parametrize("k", range(len(KATS["weno5_quadratic"])))
+def test_weno5_exact_on_quadratics(lib, k):
+    c = KATS["weno5_quadratic"][k]
+    m, p = weno(lib, c["samples"], c["h"])
+    scale = max(1.0, abs(c["derivative"]), max(abs(x) for x in c["samples"]) / c["h"])
+    assert abs(m - c["derivative"]) < 1e-13 * scale
+    assert abs(p - c["derivative"]) < 1e-13 * scale
+
+
+@pytest.mark.parametrize("k", range(len(KATS["eno3_cubic"])))
+def test_eno3_exact_on_cubics(lib, k):
+    c = KATS["eno3_cubic"][k]
+    m, p = eno(lib, c["samples"], c["h"])
+    scale = max(1.0, abs(c["derivative"]), max(abs(x) for x in c["samples"]) / c["h"])
+    assert abs(m - c["derivative"]) < 1e-12 * scale
+    assert abs(p - c["derivative"]) < 1e-12 * scale
+
+
+def test_weno5_fifth_order(lib):
+    x0 = 0.3
+    errs = []
+    for n in (20, 40, 80):
+        h = 1.0 / n
+        s = np.sin(2 * np.pi * (x0 + (np.arange(7) - 3) * h))
+        m, p = weno(lib, s, h, eps=1e-40)
+        errs.append(abs(m - 2 * np.pi * np.cos(2 * np.pi * x0)))
+    rates = np.log2(np.array(errs[:-1]) / np.array(errs[1:]))
+    assert (rates > 4.5).all(), rates
+
+
+def test_weno5_picks_smooth_side_at_a_kink(lib):
+    # |x| sampled across the kink: the backward derivative at x=2h must be +1
+    h = 0.1
+    s = np.abs((np.arange(7) - 1) * h)
+    m, p = weno(lib, s, h)
+    assert abs(m - 1) < 1e-3 and abs(p - 1) < 1e-12
+
+
+def _solver(lib, name, n, **over):
+    phi, vel, P, g = scenes.make(name, n, **over)
+    s = _abi.Solver(lib, "fso_", P, g)
+    return s, phi, vel
+
+
+def test_reinit_plane_is_fixed_point(lib):
+    n = 16
+    s, _, vel = _solver(lib, "rising_drop", n)
+    x, y, z = scenes.cell_centres(n)
+    phi = (z - 0.43) * 0.8 + (x - 0.5) * 0.6  # unit normal (0.6, 0, 0.8)
+    s.set_fields(phi, *vel)
+    s.phase("reinitialization", 1e-3)
+    out = s.get_fields()["phi"]
+    assert np.abs(out - phi).max() < 1e-12
+
+
+def test_reinit_restores_unit_gradient(lib):
+    n = 32
+    s, _, vel = _solver(lib, "rising_drop", n)
+    x, y, z = scenes.cell_centres(n)
+    r = np.sqrt((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2)
+    phi0 = 2.0 * (r - 0.25)
+    s.set_fields(phi0, *vel)
+    for _ in range(3):
+        s.phase("reinitialization", 1e-3)
+    phi = s.get_fields()["phi"]
+    assert (np.sign(phi) == np.sign(phi0)).all()
+    h = 1.0 / n
+    gz, gy, gx = np.gradient(phi, h)
+    band = (np.abs(phi) < 3 * h) & (r > 4 * h)
+    gm = np.sqrt(gx ** 2 + gy ** 2 + gz ** 2)[band]
+    assert abs(np.median(gm) - 1) < 0.05
+    # zero level: the interpolated crossing along the x axis stays at r = 0.25
+    j = k = n // 2
+    line = phi[k, j, :]
+    i = np.where(np.diff(np.sign(line)) != 0)[0][-1]
+    xc = (i + 0.5) * h + h * line[i] / (line[i] - line[i + 1])
+    rc = np.sqrt((xc - .5) ** 2 + ((j + .5) * h - .5) ** 2 + ((k + .5) * h - .5) ** 2)
+    assert abs(rc - 0.25) < h / 10
+
+
+def _csr(lib, s):
+    n, nnz = C.c_int(), C.c_int()
+    assert lib.fso_csr_info(s.h, C.byref(n), C.byref(nnz)) == 0
+    rp = np.zeros(n.value + 1, np.int32)
+    ci = np.zeros(nnz.value, np.int32)
+    val = np.zeros(nnz.value)
+    b = np.zeros(n.value)
+    IP = C.POINTER(C.c_int)
+    lib.fso_csr_copy(s.h, rp.ctypes.data_as(IP), ci.ctypes.data_as(IP), val.ctypes.data_as(PD), b.ctypes.data_as(PD))
+    import scipy.sparse as sp
+    return sp.csr_matrix((val, ci, rp), shape=(n.value, n.value)), b
+
+
+def test_pressure_matrix_symmetric_bitwise(lib):
+    s, phi, vel = _solver(lib, "rising_drop", 12)
+    s.set_fields(phi, *[np.random.default_rng(0).standard_normal(v.shape) for v in vel])
+    s.assemble(1e-3)
+    A, _ = _csr(lib, s)
+    D = (A - A.T).tocoo()
+    assert np.all(D.data == 0)
+
+
+def _neumann_laplacian(n, h, inv_rho):
+    """Textbook cell-centred 7-point Neumann Laplacian on n^3 (negative
+    definite part flipped), lexicographic x-fastest order, scaled by 1/rho."""
+    import scipy.sparse as sp
+    e = np.ones(n)
+    T = sp.diags([-e[:-1], 2 * e, -e[:-1]], [-1, 0, 1]).tolil()
+    T[0, 0] = 1
+    T[n - 1, n - 1] = 1
+    I = sp.identity(n)
+    L = (sp.kron(sp.kron(I, I), T) + sp.kron(sp.kron(I, T), I) + sp.kron(sp.kron(T, I), I)) / (h * h)
+    return (L * inv_rho).tocsr()
+
+
+def test_single_phase_operator_is_neumann_laplacian(lib):
+    n = 8
+    s, phi, vel = _solver(lib, "rising_drop", n, air_density=1000.0, water_density=1000.0)
+    s.set_fields(phi, *vel)
+    s.assemble(1e-3)
+    A, _ = _csr(lib, s)
+    h = 1.0 / n
+    ref = _neumann_laplacian(n, h, 1e-3)
+    # the reference's pin: cell 0 gains its two Neumann faces below (y and z)
+    ref = ref.tolil()
+    ref[0, 0] += 2 * 1e-3 / (h * h)
+    diff = abs(A - ref.tocsr()).max()
+    assert diff < 1e-12 * abs(ref).max()
+
+
+def test_two_phase_face_coefficient_is_effective_density(lib):
+    """Across the interface beta = 1/rho_hat with rho_hat the |phi|-weighted
+    blend of the two densities (ghost-fluid method, variable coefficient)."""
+    n = 12
+    s, phi, vel = _solver(lib, "rising_drop", n)
+    s.set_fields(phi, *vel)
+    a = s.assemble(1e-3)
+    P = s.params
+    rho = np.where(phi <= 0, P.air_density, P.water_density)
+    fn, fc = np.abs(phi[:, :, :-1]), np.abs(phi[:, :, 1:])
+    expect = (fn + fc) / (fn * rho[:, :, :-1] + fc * rho[:, :, 1:])
+    got = a["bx"][:, :, 1:-1]
+    assert np.allclose(got, expect, rtol=1e-14, atol=0)
+    assert a["bx"][:, :, 0].max() == 0 and a["bx"][:, :, -1].max() == 0
+
+
+def test_pcg_matches_dense_solve(lib):
+    n = 8
+    s, phi, vel = _solver(lib, "rising_drop", n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-13)
+    rng = np.random.default_rng(5)
+    vel = [rng.standard_normal(v.shape) * 0.1 for v in vel]
+    s.set_fields(phi, *vel)
+    st = s.projection(1e-2)
+    p = s.get_fields()["p"].ravel()
+    A, b = _csr(lib, s)
+    x = np.linalg.solve(A.toarray(), b)
+    assert st["residual_rel"] < 1e-12
+    assert np.abs(p - x).max() < 1e-8 * np.abs(x).max()
+
+
+def test_fixed_schedule_projection_is_divergence_free(lib):
+    """All faces corrected, one phase, walls at rest: the discrete divergence
+    vanishes to solver tolerance everywhere but the pinned cell, whose x-face
+    is corrected against a zero ghost pressure the pin does not include."""
+    n = 12
+    s, phi, _ = _solver(lib, "rising_drop", n, air_density=1.0, water_density=1.0, stop_rule=1,
+                        rel_tolerance=1e-13, schedule=_abi.FS_SCHEDULE_FIXED)
+    rng = np.random.default_rng(2)
+    u = rng.standard_normal((n, n, n + 1))
+    v = rng.standard_normal((n, n + 1, n))
+    w = rng.standard_normal((n + 1, n, n))
+    u[:, :, [0, -1]] = 0
+    v[:, [0, -1], :] = 0
+    w[[0, -1], :, :] = 0
+    s.set_fields(phi, u, v, w)
+    s.projection(1e-2)
+    f = s.get_fields()
+    div = (np.diff(f["u"], axis=2) + np.diff(f["v"], axis=1) + np.diff(f["w"], axis=0)) * n
+    div[0, 0, 0] = 0
+    before = (np.diff(u, axis=2) + np.diff(v, axis=1) + np.diff(w, axis=0)) * n
+    assert np.abs(div).max() < 1e-9 * np.abs(before).max()
+
+
+def test_viscosity_shear_wave_is_discrete_diffusion(lib):
+    n = 16
+    mu, rho = 0.02, 2.0
+    s, phi, _ = _solver(lib, "rising_drop", n, air_density=rho, water_density=rho, air_viscosity=mu,
+                        water_viscosity=mu)
+    lib.fso_set_viscosity_inplace(s.h, 0)
+    h = 1.0 / n
+    yc = (np.arange(n) + 0.5) * h
+    u = np.broadcast_to(np.sin(2 * np.pi * yc)[None, :, None], (n, n, n + 1)).copy()
+    v = np.zeros((n, n + 1, n))
+    w = np.zeros((n + 1, n, n))
+    s.set_fields(phi, u, v, w)
+    dt = 1e-3
+    s.phase("sourcing", dt)
+    out = s.get_fields()["u"]
+    lap = (np.roll(u, -1, axis=1) - 2 * u + np.roll(u, 1, axis=1)) / (h * h)
+    expect = u + dt * mu / rho * lap
+    inner = (slice(1, -1), slice(1, -1), slice(1, -1))
+    assert np.abs(out[inner] - expect[inner]).max() < 1e-13
+
+
+def test_gravity_every_face(lib):
+    s, phi, vel = _solver(lib, "rising_drop", 8, gravity=(1.0, -2.0, -9.8))
+    s.set_fields(phi, *vel)
+    s.phase("apply_gravity", 0.25)
+    f = s.get_fields()
+    assert np.all(f["u"] == 0.25) and np.all(f["v"] == -0.5) and np.all(f["w"] == 0.25 * -9.8)
+
+
+def test_cfl_closed_form(lib):
+    n = 16
+    s, phi, vel = _solver(lib, "rising_drop", n)
+    u, v, w = vel
+    u = u + 0.3
+    w = w - 0.7
+    s.set_fields(phi, u, v, w)
+    P = s.params
+    h = 1.0 / n
+    c = 0.3 / h + 0.7 / h
+    g = np.sqrt(9.8 / h)
+    vf = max(P.water_viscosity / P.water_density, P.air_viscosity / P.air_density) * 6 / (h * h)
+    expect = 0.5 / (0.5 * (c + vf + np.sqrt((c + vf) ** 2 + 4 * g * g)))
+    assert abs(s.cfl_dt(0.5, 1) - expect) < 1e-12 * expect
+    # the reference's own rule takes |g_y| only
+    expect_ref = 0.5 / (0.5 * (c + vf + np.sqrt((c + vf) ** 2)))
+    assert abs(s.cfl_dt(0.5, 0) - expect_ref) < 1e-12 * expect_ref
+
+
+def test_viscosity_order_sensitivity_is_small(lib):
+    """The HIP kernels update each component out of place; the reference's
+    single-thread sweep updates in place.  The difference stays at the
+    second order of the explicit increment."""
+    out = []
+    for inplace in (1, 0):
+        s, phi, vel = _solver(lib, "dam_break", 16)
+        lib.fso_set_viscosity_inplace(s.h, inplace)
+        rng = np.random.default_rng(4)
+        s.set_fields(phi, *[rng.standard_normal(x.shape) for x in vel])
+        s.phase("sourcing", 1e-3)
+        out.append(s.get_fields())
+    for c in ("u", "v", "w"):
+        d = np.abs(out[0][c] - out[1][c]).max() / np.abs(out[1][c]).max()
+        assert d < 1e-7, (c, d)
+
+
+def test_error_paths(lib):
+    s, phi, vel = _solver(lib, "rising_drop", 8, max_iteration=1, stop_rule=1, rel_tolerance=1e-14)
+    s.set_fields(phi, *[np.random.default_rng(1).standard_normal(v.shape) for v in vel])
+    with pytest.raises(_abi.FsError) as e:
+        s.projection(1e-2)
+    assert e.value.code == _abi.FS_ERR_NOT_CONVERGED
+    with pytest.raises(_abi.FsError) as e:
+        s.step(-1.0)
+    assert e.value.code == _abi.FS_ERR_BAD_ARG
+    P = _abi.default_params(surface_tension=0.07)
+    with pytest.raises(_abi.FsError) as e:
+        _abi.Solver(lib, "fso_", P, _abi.unit_cube_grid(8))
+    assert e.value.code == _abi.FS_ERR_UNSUPPORTED
+    u = vel[0].copy()
+    u[2, 2, 2] = np.nan
+    s.set_fields(u=u)
+    with pytest.raises(_abi.FsError) as e:
+        s.cfl_dt(0.5)
+    assert e.value.code == _abi.FS_ERR_NONFINITE
