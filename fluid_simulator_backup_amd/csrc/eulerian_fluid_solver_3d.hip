@@ -118,15 +118,12 @@ class EulerianFluidSolver3D {
         double** f64[] = {&phi, &sghost, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
                           &bx, &by, &bz, &diag, &rhs};
         for (double** f : f64) *f = dfield();
-        int nb = fs_pcg_blocks(L);
-        for (FsPcgWork* wk : {&w64}) {
-            wk->r = dfield();
-            wk->s = dfield();
-            wk->p = dfield();
-            wk->q = dfield();
-            wk->nblocks = nb;
-            wk->partials = (double*)dalloc(sizeof(double) * 3 * nb);
-        }
+        const size_t npart = sizeof(double) * 4 * FS_PCG_MAX_BLOCKS;
+        w64.r = dfield();
+        w64.s = dfield();
+        w64.p = dfield();
+        w64.q = dfield();
+        w64.partials = (double*)dalloc(npart);
         if (P.precision == FS_FP32_REFINE) {
             bx32 = ffield();
             by32 = ffield();
@@ -138,8 +135,7 @@ class EulerianFluidSolver3D {
             w32.s = ffield();
             w32.p = ffield();
             w32.q = ffield();
-            w32.nblocks = nb;
-            w32.partials = (double*)dalloc(sizeof(double) * 3 * nb);
+            w32.partials = (double*)dalloc(npart);
         }
         S64 = (FsPcgState*)dalloc(sizeof(FsPcgState));
         S32 = (FsPcgState*)dalloc(sizeof(FsPcgState));
@@ -314,12 +310,14 @@ class EulerianFluidSolver3D {
         init_state(S64, stop_rule, tol, rtol, max_iter);
         fs_pcg64_begin(st, L, o, rhs, diag, x, w64, S64);
         read_state(S64);
+        int n = 0;
         while (!hS->done) {
-            fs_pcg64_iterations(st, L, o, diag, x, w64, S64, chunk);
+            fs_pcg64_iterations(st, L, o, diag, x, w64, S64, n, chunk);
+            n += chunk;
             check_launch();
             read_state(S64);
         }
-        stats.residual_precond = std::sqrt(hS->rz);
+        stats.residual_precond = std::sqrt(hS->rs_last);
         return hS->iter;
     }
 
@@ -335,8 +333,10 @@ class EulerianFluidSolver3D {
         HIPCHECK(hipMemsetAsync(e32, 0, sizeof(float) * L.total, st));
         fs_pcg32_begin(st, L, o, r32, diag32, e32, w32, S32);
         read_state(S32);
+        int n = 0;
         while (!hS->done) {
-            fs_pcg32_iterations(st, L, o, diag32, e32, w32, S32, chunk);
+            fs_pcg32_iterations(st, L, o, diag32, e32, w32, S32, n, chunk);
+            n += chunk;
             check_launch();
             read_state(S32);
         }
@@ -667,9 +667,9 @@ int fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms) {
         S.init_state(dS, FS_STOP_ABS_PRECOND, 0, 0, 1 << 30);
         FsPcgState h{};
         HIPCHECK(hipMemcpy(&h, dS, sizeof(h), hipMemcpyDeviceToHost));
-        h.rz = 1;
-        h.alpha = 0;
-        h.beta = 0;
+        h.rz[0] = 1;
+        h.rz[1] = 1;
+        h.done = 0;
         HIPCHECK(hipMemcpy(dS, &h, sizeof(h), hipMemcpyHostToDevice));
         FsPcgOp o = S.op64();
         FsPcgWork* w = &S.w64;
@@ -684,10 +684,10 @@ int fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms) {
             x = S.e32;
         }
         for (int which = 0; which < 3; which++) {
-            fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which);  // warm
+            fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which, 0);  // warm
             HIPCHECK(hipEventRecord(S.ev[0], S.st));
             for (int r = 0; r < reps; r++)
-                fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which);
+                fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which, 0);
             HIPCHECK(hipEventRecord(S.ev[1], S.st));
             S.check_launch();
             S.sync();

@@ -421,15 +421,37 @@ void fs_assemble(hipStream_t st, const double* phi, const double* irho, const do
 
 // Neumann mean removal, POISSON_SOLVER_3D.h:225-246.  The sum runs in row
 // order inside each z-plane, then over planes in order (one slab per plane).
-__global__ void k_plane_sums(const double* __restrict__ rhs, double* __restrict__ part, FsLayout L) {
-    int k = blockIdx.x * blockDim.x + threadIdx.x;
-    if (k >= L.nz) return;
+// One workgroup per plane: rows are staged through LDS with coalesced loads
+// and one thread adds them in row order.
+#define PS_CHUNK 4096
+__global__ void __launch_bounds__(256) k_plane_sums(const double* __restrict__ rhs, double* __restrict__ part,
+                                                    FsLayout L) {
+    __shared__ double buf[PS_CHUNK];
+    int k = blockIdx.x;
+    int rows = PS_CHUNK / L.nx;
+    if (rows < 1) rows = 1;
     double s = 0;
-    for (int j = 0; j < L.ny; j++) {
-        const double* row = rhs + L.idx(0, j, k);
-        for (int i = 0; i < L.nx; i++) s += row[i];
+    for (int j0 = 0; j0 < L.ny; j0 += rows) {
+        int nrow = min(rows, L.ny - j0);
+        int n = nrow * L.nx;
+        __syncthreads();
+        for (int t = threadIdx.x; t < n && t < PS_CHUNK; t += blockDim.x) {
+            int jj = t / L.nx, ii = t - jj * L.nx;
+            buf[t] = rhs[L.idx(ii, j0 + jj, k)];
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            if (L.nx <= PS_CHUNK) {
+                for (int t = 0; t < n; t++) s += buf[t];
+            } else {  // rows wider than the stage: read straight from memory
+                for (int jj = 0; jj < nrow; jj++) {
+                    const double* row = rhs + L.idx(0, j0 + jj, k);
+                    for (int i = 0; i < L.nx; i++) s += row[i];
+                }
+            }
+        }
     }
-    part[k] = s;
+    if (threadIdx.x == 0) part[k] = s;
 }
 __global__ void k_total(double* part, int nz) {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
@@ -445,16 +467,30 @@ __global__ void k_sub_mean(double* __restrict__ rhs, const double* __restrict__ 
 }
 
 void fs_remove_mean(hipStream_t st, double* rhs, double* scratch, const FsLayout& L) {
-    hipLaunchKernelGGL(k_plane_sums, dim3((L.nz + 63) / 64), dim3(64), 0, st, rhs, scratch, L);
+    hipLaunchKernelGGL(k_plane_sums, dim3(L.nz), dim3(256), 0, st, rhs, scratch, L);
     hipLaunchKernelGGL(k_total, dim3(1), dim3(64), 0, st, scratch, L.nz);
     double inv_n = 1 / (double)((long)L.nx * L.ny * L.nz);
     hipLaunchKernelGGL(k_sub_mean, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, rhs, scratch, L, inv_n);
 }
 
-__device__ __forceinline__ void atomic_max_nonneg(unsigned long long* bits, double a) {
-    // non-negative doubles order like their bit patterns
-    atomicMax(bits, (unsigned long long)__double_as_longlong(a));
+// max that keeps a NaN (so a non-finite field is reported, not skipped)
+__device__ __forceinline__ double nanmax(double a, double b) { return (a != a || a > b) ? a : b; }
+
+// Workgroup max of non-negative a (BX x BY threads), one atomic per
+// workgroup; non-negative doubles (and NaN above +inf) order like their bits.
+__device__ __forceinline__ void wg_max_atomic(double a, unsigned long long* bits) {
+    __shared__ double lds[BY];
+    for (int o = 32; o > 0; o >>= 1) a = nanmax(a, __shfl_xor(a, o));
+    if (threadIdx.x == 0) lds[threadIdx.y] = a;
+    __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double m = lds[0];
+        for (int w = 1; w < BY; w++) m = nanmax(m, lds[w]);
+        atomicMax(bits, (unsigned long long)__double_as_longlong(m));
+    }
 }
+
+#define KZ 16  // z-planes per workgroup in the max-reducing kernels
 
 // UpdateVelocityByPressureGradientVariableDensity, PROJECTION_3D.h:1099-1449,
 // one component (comp 1,2,3).  Faces whose + cell is not full are skipped
@@ -465,10 +501,11 @@ __global__ void __launch_bounds__(BX* BY) k_gradient(int comp, int apply, double
                                                      const double* __restrict__ p, const double* __restrict__ phi,
                                                      const double* __restrict__ rho, FsLayout L, FsBox b, FsMaterial m,
                                                      double eps, double od, double dt, unsigned long long* vmax) {
-    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
-    bool valid = i < b.ni && j < b.nj && i < L.nx && j < L.ny && k < L.nz;
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double a = 0;
-    if (valid) {
+    for (int k = blockIdx.z * KZ; k < min(blockIdx.z * KZ + KZ, b.nk); k++) {
+        bool valid = i < b.ni && j < b.nj && i < L.nx && j < L.ny && k < L.nz;
+        if (!valid) continue;
         long c = L.idx(i, j, k);
         long st = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
         int nidx = comp == 1 ? i - 1 : comp == 2 ? j - 1 : k - 1;
@@ -494,11 +531,9 @@ __global__ void __launch_bounds__(BX* BY) k_gradient(int comp, int apply, double
             }
             vel[c] = vr;
         }
-        a = fabs(vr);
+        a = nanmax(a, fabs(vr));
     }
-    // wave-level max, then one atomic per wave
-    for (int o = 32; o > 0; o >>= 1) a = fmax(a, __shfl_xor(a, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(vmax, a);
+    wg_max_atomic(a, vmax);
 }
 
 void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,
@@ -507,20 +542,23 @@ void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const 
     FsBox b = fs_box(L, comp);
     double dg = comp == 1 ? cell.x.d : comp == 2 ? cell.y.d : cell.z.d;
     double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
-    hipLaunchKernelGGL(k_gradient, grid_for(b), dim3(BX, BY), 0, st, comp, apply, vel, p, phi, rho, L, b, m,
+    dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
+    hipLaunchKernelGGL(k_gradient, g, dim3(BX, BY), 0, st, comp, apply, vel, p, phi, rho, L, b, m,
                        (double)1.5 * dg, od, dt, vmax_bits);
 }
 
-__global__ void k_abs_max(const double* __restrict__ a, FsLayout L, FsBox b, unsigned long long* bits) {
-    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+__global__ void __launch_bounds__(BX* BY) k_abs_max(const double* __restrict__ a, FsLayout L, FsBox b,
+                                                    unsigned long long* bits) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double v = 0;
-    if (i < b.ni && j < b.nj) v = fabs(a[L.idx(i, j, k)]);
-    for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o));
-    if ((threadIdx.x & 63) == 0) atomic_max_nonneg(bits, v);
+    if (i < b.ni && j < b.nj)
+        for (int k = blockIdx.z * KZ; k < min(blockIdx.z * KZ + KZ, b.nk); k++) v = nanmax(v, fabs(a[L.idx(i, j, k)]));
+    wg_max_atomic(v, bits);
 }
 
 void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, unsigned long long* bits) {
-    hipLaunchKernelGGL(k_abs_max, grid_for(b), dim3(BX, BY), 0, st, a, L, b, bits);
+    dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
+    hipLaunchKernelGGL(k_abs_max, g, dim3(BX, BY), 0, st, a, L, b, bits);
 }
 
 // ---------------------------------------------------------------------------
