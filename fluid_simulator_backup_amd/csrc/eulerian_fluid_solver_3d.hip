@@ -10,6 +10,7 @@
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <utility>
@@ -73,8 +74,10 @@ class EulerianFluidSolver3D {
 
     // last-projection statistics
     fs_step_stats stats{};
-    double inner_rtol = 1e-6;  // floor of the FP32 inner relative tolerance
-    int max_sweeps = 30;
+    double update_drop = 0.01;  // reliable update when ||r32||^2 fell by this factor
+    double hbb = 0;             // host staging of ||b||^2 for the FP32 state
+    bool debug = false;         // FS_DEBUG: trace refinement on stderr
+    int dbg_lines = 0;
     int chunk = 16;
 
     void* dalloc(size_t bytes) {
@@ -98,6 +101,12 @@ class EulerianFluidSolver3D {
             throw FsFail{FS_ERR_UNSUPPORTED, "only order_for_time_advancing = 1 is implemented"};
         if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE) throw FsFail{FS_ERR_BAD_ARG, "precision"};
         if (P.scaling_number_for_reinitialization <= 0) throw FsFail{FS_ERR_BAD_ARG, "scaling_number"};
+        debug = std::getenv("FS_DEBUG") != nullptr;
+        // tuning override of the reliable-update trigger (residual^2 drop, default 0.01)
+        if (const char* e = std::getenv("FS_FP32_UPDATE_DROP")) {
+            double v = std::atof(e);
+            if (v > 0 && v < 1) update_drop = v;
+        }
         if (dev >= 0) HIPCHECK(hipSetDevice(dev));
         HIPCHECK(hipGetDevice(&device));
         HIPCHECK(hipStreamCreate(&st));
@@ -118,7 +127,7 @@ class EulerianFluidSolver3D {
         double** f64[] = {&phi, &sghost, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
                           &bx, &by, &bz, &diag, &rhs};
         for (double** f : f64) *f = dfield();
-        const size_t npart = sizeof(double) * 4 * FS_PCG_MAX_BLOCKS;
+        const size_t npart = sizeof(double) * FS_PCG_SLOTS * FS_PCG_MAX_BLOCKS;
         w64.r = dfield();
         w64.s = dfield();
         w64.p = dfield();
@@ -286,16 +295,32 @@ class EulerianFluidSolver3D {
 
     // Host copy of the solver configuration into a device state, keeping the
     // pin extras computed on device.
-    void init_state(FsPcgState* dS, int stop_rule, double tol, double rtol, int max_iter) {
+    void init_state(FsPcgState* dS, int stop_rule, double tol, double rtol, int max_iter, bool with_pin = true) {
         FsPcgState h{};
         h.max_iter = max_iter;
         h.stop_rule = stop_rule;
         h.tol2 = tol * tol;
         h.rtol2 = rtol * rtol;
         HIPCHECK(hipMemcpyAsync(dS, &h, sizeof(h), hipMemcpyHostToDevice, st));
-        // pin extras written by the diagonal kernel into S64 (see PinExtras)
-        HIPCHECK(hipMemcpyAsync(&dS->pin_b, &S64pin()[0], 2 * sizeof(double), hipMemcpyDeviceToDevice, st));
+        // pin extras written by the diagonal kernel (fs_assemble)
+        if (with_pin)
+            HIPCHECK(hipMemcpyAsync(&dS->pin_b, &S64pin()[0], 2 * sizeof(double), hipMemcpyDeviceToDevice, st));
         sync();
+    }
+
+    // FP64 residual r = b - A p of the pinned system with its constant mode
+    // settled in FP64 (fs_deflate_constant); returns r.r, sets bb.
+    double true_residual(double& bb) {
+        FsPcgOp o = op64();
+        fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32, w64, S64);
+        read_state(S64);
+        double pin = o.idx2 * (hS->pin_b + hS->pin_d);
+        if (pin > 0) {
+            fs_deflate_constant(st, L, p, (double*)w64.r, r32, S64, pin);
+            read_state(S64);
+        }
+        bb = hS->bb;
+        return hS->rr;
     }
     double* S64pin() { return scratch + G.nz + 2; }
 
@@ -321,26 +346,78 @@ class EulerianFluidSolver3D {
         return hS->iter;
     }
 
-    int run_pcg32(double rtol, int max_iter) {
-        FsPcgOp o;
+    FsPcgOp op32() const {
+        FsPcgOp o = op64();
         o.bx = bx32;
         o.by = by32;
         o.bz = bz32;
-        o.idx2 = (double)1 / (cell.x.d * cell.x.d);
-        o.idy2 = (double)1 / (cell.y.d * cell.y.d);
-        o.idz2 = (double)1 / (cell.z.d * cell.z.d);
-        init_state(S32, FS_STOP_REL_RESIDUAL, 0, rtol, max_iter);
+        return o;
+    }
+
+    // FP32 Jacobi-PCG on the correction e with FP64 reliable updates: when
+    // the FP32 recursive residual has dropped by `update_drop` since the last
+    // update (or reached the target), e is folded into the FP64 pressure and
+    // the residual b - A p is recomputed in FP64 and replaces the
+    // recursion's residual between its r-update and p-update, so beta and the
+    // new direction come from the refreshed residual and the Krylov history
+    // is kept.  Returns FP32 iterations; `updates` counts FP64 residual
+    // recomputations after the first.
+    //
+    // The reference's system is a Neumann operator L pinned at one cell,
+    // A = L + pin e0 e0^T, with a zero-sum right-hand side.  A constant
+    // shift of x changes the residual only in the pinned cell, so each FP64
+    // residual first settles the constant (x += sum(r)/pin, leaving a
+    // zero-sum residual) and the FP32 recursion solves the pin-free L on
+    // zero-sum vectors: its corrections carry no large constant (which FP32
+    // could not resolve the small variations on) and it converges faster
+    // than the pinned system, whose tiny pin eigenvalue slows Jacobi-PCG.
+    int run_pcg32_reliable(double& rr64, double& bb, int& updates) {
+        FsPcgOp o32 = op32();
+        const double rt2 = P.rel_tolerance * P.rel_tolerance;
+        init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
+        rr64 = true_residual(bb);
+        updates = 0;
+        if (bb == 0 || rr64 <= rt2 * bb) return 0;
+        init_state(S32, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, P.max_iteration, false);
         HIPCHECK(hipMemsetAsync(e32, 0, sizeof(float) * L.total, st));
-        fs_pcg32_begin(st, L, o, r32, diag32, e32, w32, S32);
-        read_state(S32);
-        int n = 0;
-        while (!hS->done) {
-            fs_pcg32_iterations(st, L, o, diag32, e32, w32, S32, n, chunk);
-            n += chunk;
+        fs_pcg32_begin(st, L, o32, r32, diag32, e32, w32, S32);
+        // the recursion stops against the global target ||b||
+        hbb = bb;
+        HIPCHECK(hipMemcpyAsync(&S32->bb, &hbb, sizeof(double), hipMemcpyHostToDevice, st));
+        double rr_ref = rr64;
+        int n = 0, pending = 0;
+        while (true) {
+            fs_pcg32_chunk(st, L, o32, diag32, e32, w32, S32, n, chunk, pending);
             check_launch();
             read_state(S32);
+            if (hS->done == 2) {
+                n = hS->iter;
+                break;
+            }
+            bool stopped = hS->done == 1;
+            if (debug && dbg_lines++ < 120)
+                fprintf(stderr, "[fs]   chunk n0=%d done=%d iter=%d rel32=%.3e rz=%.3e,%.3e\n", n, hS->done, hS->iter,
+                        std::sqrt(hS->rr / bb), hS->rz[0], hS->rz[1]);
+            // the iteration whose p-update is pending: the one that met the
+            // target, or the chunk's last
+            int last = stopped ? hS->iter : n + chunk - 1;
+            n = last + 1;
+            pending = 1;
+            double rr32 = hS->rr;
+            if (stopped || rr32 < update_drop * rr_ref) {
+                fs_add_correction(st, L, p, e32);
+                HIPCHECK(hipMemsetAsync(e32, 0, sizeof(float) * L.total, st));
+                rr64 = true_residual(bb);
+                updates++;
+                if (debug)
+                    fprintf(stderr, "[fs] fp32 n=%d stopped=%d rel32=%.3e rel64=%.3e\n", n, (int)stopped,
+                            std::sqrt(rr32 / bb), std::sqrt(rr64 / bb));
+                if (rr64 <= rt2 * bb || updates > 400) break;
+                fs_pcg32_reliable_update(st, L, r32, diag32, w32, S32);
+                rr_ref = rr64;
+            }
         }
-        return hS->iter + (hS->done == 1 ? 1 : 0);
+        return n;
     }
 
     // Pressure solve on p (warm start), FP64 or FP32-inner / FP64-refined.
@@ -358,28 +435,15 @@ class EulerianFluidSolver3D {
         }
         // FP32 inner PCG inside FP64 iterative refinement: the outer residual
         // b - A x, the correction update and all dot products stay FP64.
+        (void)o;
         fs_make_op32(st, L, bx, by, bz, diag, bx32, by32, bz32, diag32);
-        init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
-        fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32, w64, S64);
-        read_state(S64);
-        double rr = hS->rr, bb = hS->bb;
-        double rt2 = P.rel_tolerance * P.rel_tolerance;
-        int sweeps = 0, inner = 0;
-        while (bb > 0 && rr > rt2 * bb && sweeps < max_sweeps && inner < P.max_iteration) {
-            // ask the inner solve for what is still missing (with margin),
-            // but not below what FP32 recursion reliably attains
-            double want = 0.3 * P.rel_tolerance * std::sqrt(bb / rr);
-            inner += run_pcg32(want > inner_rtol ? want : inner_rtol, P.max_iteration - inner);
-            fs_add_correction(st, L, p, e32);
-            fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32, w64, S64);
-            read_state(S64);
-            rr = hS->rr;
-            sweeps++;
-        }
-        stats.pcg_iterations = inner;
-        stats.refinement_sweeps = sweeps;
+        double rr = 0, bb = 0;
+        int updates = 0;
+        stats.pcg_iterations = run_pcg32_reliable(rr, bb, updates);
+        stats.refinement_sweeps = updates;
         stats.residual_rel = bb > 0 ? std::sqrt(rr / bb) : 0;
         stats.residual_precond = 0;
+        double rt2 = P.rel_tolerance * P.rel_tolerance;
         if (bb > 0 && rr > rt2 * bb) throw FsFail{FS_ERR_NOT_CONVERGED, "refinement did not reach rel_tolerance"};
     }
 

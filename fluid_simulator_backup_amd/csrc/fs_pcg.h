@@ -13,6 +13,7 @@
 struct FsPcgState {
     double rz[2];         // r.z of iteration n lives in rz[n & 1]
     double rr, bb;        // r.r and b.b of the last iteration / of the start
+    double rsum;          // sum of the FP64 residual (fs_residual64)
     double rs_last;       // r.z at exit (the reference's residual^2, PCG_METHOD.cpp:111)
     double pin_b, pin_d;  // Neumann pin extras of cell (0,0,0), POISSON_SOLVER_3D.h:1145-1179
     int iter, done;       // done: 1 converged, 2 max_iteration reached
@@ -30,10 +31,11 @@ struct FsPcgOp {
 // Workspace of one precision.
 struct FsPcgWork {
     void *r, *s, *p, *q;    // padded-layout vectors (ghosts stay zero)
-    double* partials;       // 4 * FS_PCG_MAX_BLOCKS per-workgroup partial sums
+    double* partials;       // FS_PCG_SLOTS * FS_PCG_MAX_BLOCKS per-workgroup partial sums
 };
 
 #define FS_PCG_MAX_BLOCKS 2048
+#define FS_PCG_SLOTS 8
 
 // FP64 / FP32 solves on x (warm start).  begin: r = b - A x, z = D^-1 r,
 // p = z and the scalars; iterations: n0 .. n0+count-1.
@@ -49,9 +51,27 @@ void fs_pcg32_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, c
 // q = A x (FP64, exact CSR row order).
 void fs_apply_op64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* x, double* q,
                    const FsPcgState* dstate);
-// r = b - A x in FP64; r32 = (float) r; state->rr = r.r, state->bb = b.b.
+// The constant mode of the singly-pinned Neumann system, kept in FP64: with
+// the residual r of x just computed (state->rsum = sum(r)), shift
+// x += rsum / pin everywhere, which changes r only in the pinned cell
+// (r0 -= rsum, exactly for the flux form of the operator), and update r32
+// and state->rr to match.  pin = the pinned cell's extra diagonal.
+void fs_deflate_constant(hipStream_t st, const FsLayout& L, double* x, double* r, float* r32, FsPcgState* dstate,
+                         double pin);
+// r = b - A x in FP64; r32 = (float) r; state->rr = r.r, state->bb = b.b,
+// state->rsum = sum(r).
 void fs_residual64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* b, const double* x, double* r,
                    float* r32, FsPcgWork& w, FsPcgState* dstate);
+// FP32 iterations n0 .. n0+count-1 ending after the r-update (K2) of the
+// last one, whose p-update (K3) is left pending; pending_k3 != 0 first runs
+// the pending K3 of iteration n0-1.  state->rr receives the last r.r.
+void fs_pcg32_chunk(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x, FsPcgWork& w,
+                    FsPcgState* dstate, int n0, int count, int pending_k3);
+// Reliable update between K2 and the pending K3, with r32 holding the
+// rounded FP64 residual: the recursion's r = r32, s = r32/D and the r.s / r.r
+// partials the pending K3 reduces; clears `done`.
+void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r32, const float* diag, FsPcgWork& w,
+                              FsPcgState* dstate);
 // x += (double) e
 void fs_add_correction(hipStream_t st, const FsLayout& L, double* x, const float* e);
 // FP32 copies of the face coefficients and the FP32 Jacobi diagonal.

@@ -1,23 +1,29 @@
 // fs_pcg.hip -- matrix-free Jacobi-PCG kernels (FP64 and FP32 vectors).
 //
 // Layout: the padded grid of fs_layout.h; every vector keeps zero ghosts so
-// the 7-point stencil needs no bounds tests.  Work unit: a tile of 64 x 4
-// cells in x-y marching ZC planes in z (p(k-1), p(k), p(k+1) and
-// beta_z(k), beta_z(k+1) stay in registers).  A fixed grid of NB workgroups
-// walks the tiles; workgroup b lives on XCD b % 8 and takes tiles from that
-// XCD's contiguous share, so y-neighbour rows are reused in one L2.
+// the 7-point stencil needs no bounds tests.  Work unit: a tile of (64*V) x 4
+// cells in x-y marching ZC planes in z.  Each thread owns V consecutive
+// x-cells (V = 2 in FP64, 4 in FP32: 16-byte loads and stores), keeps
+// p(k-1), p(k), p(k+1) and beta_z(k), beta_z(k+1) in registers, and takes
+// its x-neighbours at the vector ends with two scalar loads that hit L1.
+// A fixed grid of NB workgroups walks the tiles; workgroup b lives on XCD
+// b % 8 and takes tiles from that XCD's contiguous share, so y-neighbour rows
+// are reused in one L2.
 //
 // Operator row (CSR_MATRIX::Multiply, CSR_MATRIX.h:236-260, on the rows of
 // POISSON_SOLVER_3D.h:1088-1213): entries in the order l, r, b, t, d, u,
 // diagonal; off-diagonals -beta*inv_d2; A(c,c) = inv_dx2 * sum of the six
 // betas of full neighbours (+ the Neumann pin on cell (0,0,0)).  Faces to
-// non-full neighbours carry beta = 0, which adds an exact -0 term.
+// non-full neighbours carry beta = 0, which adds an exact -0 term.  The
+// per-cell operation order does not depend on V.
 //
 // Reductions use no device-scope fence or atomic: each workgroup stores its
 // partial sum (fixed wave/LDS tree), and every workgroup of the consuming
 // kernel re-reduces the NB partials in the same fixed order, so all of them
 // see the same scalar and results do not depend on scheduling.  Dot products
 // accumulate in FP64 for both precisions.
+#include <cstring>
+
 #include "fs_pcg.h"
 
 #define TBX 64
@@ -30,14 +36,55 @@
 #define SLOT_RS 1
 #define SLOT_RR 2
 #define SLOT_BB 3
+#define SLOT_SUM 4
 
-struct Tiling {
-    int ntx, nty, ntz, ntiles, nb;
+// ---- vectors ----------------------------------------------------------------
+template <typename T, int V>
+struct VecT;
+template <>
+struct VecT<double, 1> {
+    typedef double type;
+};
+template <>
+struct VecT<double, 2> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecT<float, 1> {
+    typedef float type;
+};
+template <>
+struct VecT<float, 2> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <>
+struct VecT<float, 4> {
+    typedef float type __attribute__((ext_vector_type(4)));
 };
 
-static Tiling make_tiling(const FsLayout& L) {
+template <typename T, int V>
+__device__ __forceinline__ void vld(const T* __restrict__ p, T (&o)[V]) {
+    typedef typename VecT<T, V>::type vt;
+    vt t = *(const vt*)p;
+    memcpy(o, &t, sizeof(o));
+}
+template <typename T, int V>
+__device__ __forceinline__ void vst(T* __restrict__ p, const T (&o)[V]) {
+    typedef typename VecT<T, V>::type vt;
+    vt t;
+    memcpy(&t, o, sizeof(o));
+    *(vt*)p = t;
+}
+
+// ---- tiling -------------------------------------------------------------------
+struct Tiling {
+    int ntx, nty, ntz, ntiles, nb, v;
+};
+
+static Tiling make_tiling(const FsLayout& L, int v) {
     Tiling t;
-    t.ntx = (L.nx + TBX - 1) / TBX;
+    t.v = v;
+    t.ntx = (L.nx + TBX * v - 1) / (TBX * v);
     t.nty = (L.ny + TBY - 1) / TBY;
     t.ntz = (L.nz + ZC - 1) / ZC;
     t.ntiles = t.ntx * t.nty * t.ntz;
@@ -45,6 +92,12 @@ static Tiling make_tiling(const FsLayout& L) {
     if (nb >= 8) nb = nb / 8 * 8;
     t.nb = nb;
     return t;
+}
+
+// Vector width for a precision: the widest that divides nx.
+static int vec_width(const FsLayout& L, int is_float) {
+    if (is_float) return L.nx % 4 == 0 ? 4 : (L.nx % 2 == 0 ? 2 : 1);
+    return L.nx % 2 == 0 ? 2 : 1;
 }
 
 // Tiles of workgroup b: XCD x = b % 8 owns tiles [x*T/8, (x+1)*T/8); its
@@ -69,14 +122,17 @@ struct TileIter {
     __device__ void next() { t += step; }
 };
 
-__device__ __forceinline__ void tile_origin(const Tiling& tl, int t, int& i, int& j, int& k0) {
+// First cell (i0, j, k0) of this thread in tile t.
+template <int V>
+__device__ __forceinline__ void tile_origin(const Tiling& tl, int t, int& i0, int& j, int& k0) {
     int tx = t % tl.ntx, r = t / tl.ntx;
     int ty = r % tl.nty, tz = r / tl.nty;
-    i = tx * TBX + threadIdx.x;
+    i0 = (tx * TBX + threadIdx.x) * V;
     j = ty * TBY + threadIdx.y;
     k0 = tz * ZC;
 }
 
+// ---- reductions ---------------------------------------------------------------
 // Sum over the workgroup, returned to every thread; fixed order.
 __device__ __forceinline__ double wg_sum_all(double v, double* lds) {
     for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
@@ -89,14 +145,14 @@ __device__ __forceinline__ double wg_sum_all(double v, double* lds) {
     return t;
 }
 
-// Store this workgroup's partial of slot s (plain store; consumers run in a
+// Store this workgroup's partial of a slot (plain store; consumers run in a
 // later kernel).
 __device__ __forceinline__ void put_partial(double* part, int slot, double v, double* lds) {
     double t = wg_sum_all(v, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) part[slot * FS_PCG_MAX_BLOCKS + blockIdx.x] = t;
 }
 
-// Total of slot s over the nb workgroups of the producing kernel.
+// Total of a slot over the nb workgroups of the producing kernel.
 __device__ __forceinline__ double get_total(const double* part, int slot, int nb, double* lds) {
     int tid = threadIdx.y * TBX + threadIdx.x;
     double a = 0;
@@ -104,79 +160,15 @@ __device__ __forceinline__ double get_total(const double* part, int slot, int nb
     return wg_sum_all(a, lds);
 }
 
+// ---- operator rows ----------------------------------------------------------
 template <typename T>
 struct OpT {
     const T* __restrict__ bx;
     const T* __restrict__ by;
     const T* __restrict__ bz;
     T idx2, idy2, idz2;
+    bool cubic;
 };
-
-// q = A x at cell c; xm/xc/xp = x at k-1, k, k+1; bzd/bzu = beta_z below/above.
-template <typename T>
-__device__ __forceinline__ T apply_row(const OpT<T>& op, const T* __restrict__ x, long c, long sx, T xm, T xc, T xp,
-                                       T bzd, T bzu, bool pin, T pin_b, T pin_d) {
-    T bl = op.bx[c], br = op.bx[c + 1], bb = op.by[c], bt = op.by[c + sx];
-    T coef = 0;
-    coef += bl;
-    coef += br;
-    coef += bb;
-    coef += bt;
-    coef += bzd;
-    coef += bzu;
-    if (pin) {
-        coef += pin_b;
-        coef += pin_d;
-    }
-    if (coef == 0) coef = 1;
-    T v = 0;
-    v += -bl * op.idx2 * x[c - 1];
-    v += -br * op.idx2 * x[c + 1];
-    v += -bb * op.idy2 * x[c - sx];
-    v += -bt * op.idy2 * x[c + sx];
-    v += -bzd * op.idz2 * xm;
-    v += -bzu * op.idz2 * xp;
-    v += op.idx2 * coef * xc;
-    return v;
-}
-
-// FP32 inner operator in flux form: sum over faces of beta*(x_c - x_n)/d^2
-// (+ the pin on cell (0,0,0)).  Constants are then an exact null vector of
-// every non-pinned row in any precision, as they are for the FP64 matrix up
-// to its own rounding; the diagonal-sum form rounded to FP32 would shift the
-// near-null eigenvalue of this singly-pinned Neumann system by O(eps32) per
-// row and stall the refinement.  Non-cubic cells keep the FP64 row's
-// scaling (1/dx^2 on the diagonal for every face).
-__device__ __forceinline__ float apply_row_flux(const OpT<float>& op, const float* __restrict__ x, long c, long sx,
-                                                float xm, float xc, float xp, float bzd, float bzu, bool pin,
-                                                float pin_b, float pin_d, bool cubic) {
-    float bl = op.bx[c], br = op.bx[c + 1], bb = op.by[c], bt = op.by[c + sx];
-    float v;
-    if (cubic) {
-        float s = op.idx2;
-        v = bl * (s * (xc - x[c - 1])) + br * (s * (xc - x[c + 1])) + bb * (s * (xc - x[c - sx])) +
-            bt * (s * (xc - x[c + sx])) + bzd * (s * (xc - xm)) + bzu * (s * (xc - xp));
-    } else {
-        float cx = op.idx2 * xc;
-        v = bl * (cx - op.idx2 * x[c - 1]) + br * (cx - op.idx2 * x[c + 1]) + bb * (cx - op.idy2 * x[c - sx]) +
-            bt * (cx - op.idy2 * x[c + sx]) + bzd * (cx - op.idz2 * xm) + bzu * (cx - op.idz2 * xp);
-    }
-    if (pin) v += (pin_b + pin_d) * (op.idx2 * xc);
-    return v;
-}
-
-template <typename T>
-__device__ __forceinline__ T row(const OpT<T>& op, const T* __restrict__ x, long c, long sx, T xm, T xc, T xp, T bzd,
-                                 T bzu, bool pin, T pin_b, T pin_d) {
-    return apply_row<T>(op, x, c, sx, xm, xc, xp, bzd, bzu, pin, pin_b, pin_d);
-}
-template <>
-__device__ __forceinline__ float row<float>(const OpT<float>& op, const float* __restrict__ x, long c, long sx,
-                                            float xm, float xc, float xp, float bzd, float bzu, bool pin, float pin_b,
-                                            float pin_d) {
-    bool cubic = op.idx2 == op.idy2 && op.idx2 == op.idz2;
-    return apply_row_flux(op, x, c, sx, xm, xc, xp, bzd, bzu, pin, pin_b, pin_d, cubic);
-}
 
 template <typename T>
 __device__ __forceinline__ OpT<T> make_op(const FsPcgOp& o) {
@@ -187,11 +179,117 @@ __device__ __forceinline__ OpT<T> make_op(const FsPcgOp& o) {
     r.idx2 = (T)o.idx2;
     r.idy2 = (T)o.idy2;
     r.idz2 = (T)o.idz2;
+    r.cubic = o.idx2 == o.idy2 && o.idx2 == o.idz2;
     return r;
 }
 
-// ---- begin: r = b - A x, s = r/D, p = s; partials of r.s, r.r, b.b -------
+// The neighbourhood of one cell.
 template <typename T>
+struct Nb {
+    T xc, xl, xr, xb, xt, xd, xu;
+    T bl, br, bb, bt, bd, bu;
+};
+
+// FP64 row in the CSR order of the reference.
+__device__ __forceinline__ double row_val(const OpT<double>& op, const Nb<double>& n, bool pin, double pin_b,
+                                          double pin_d) {
+    double coef = 0;
+    coef += n.bl;
+    coef += n.br;
+    coef += n.bb;
+    coef += n.bt;
+    coef += n.bd;
+    coef += n.bu;
+    if (pin) {
+        coef += pin_b;
+        coef += pin_d;
+    }
+    if (coef == 0) coef = 1;
+    double v = 0;
+    v += -n.bl * op.idx2 * n.xl;
+    v += -n.br * op.idx2 * n.xr;
+    v += -n.bb * op.idy2 * n.xb;
+    v += -n.bt * op.idy2 * n.xt;
+    v += -n.bd * op.idz2 * n.xd;
+    v += -n.bu * op.idz2 * n.xu;
+    v += op.idx2 * coef * n.xc;
+    return v;
+}
+
+// FP32 inner operator in flux form: sum over faces of beta*(x_c - x_n)/d^2
+// (+ the pin on cell (0,0,0)).  Constants are then an exact null vector of
+// every non-pinned row in any precision, as they are for the FP64 matrix up
+// to its own rounding; the diagonal-sum form rounded to FP32 would shift the
+// near-null eigenvalue of this singly-pinned Neumann system by O(eps32) per
+// row and stall the refinement.  Non-cubic cells keep the FP64 row's
+// scaling (1/dx^2 on the diagonal for every face).
+__device__ __forceinline__ float row_val(const OpT<float>& op, const Nb<float>& n, bool pin, float pin_b,
+                                         float pin_d) {
+    float v;
+    if (op.cubic) {
+        float s = op.idx2;
+        v = n.bl * (s * (n.xc - n.xl)) + n.br * (s * (n.xc - n.xr)) + n.bb * (s * (n.xc - n.xb)) +
+            n.bt * (s * (n.xc - n.xt)) + n.bd * (s * (n.xc - n.xd)) + n.bu * (s * (n.xc - n.xu));
+    } else {
+        float cx = op.idx2 * n.xc;
+        v = n.bl * (cx - op.idx2 * n.xl) + n.br * (cx - op.idx2 * n.xr) + n.bb * (cx - op.idy2 * n.xb) +
+            n.bt * (cx - op.idy2 * n.xt) + n.bd * (cx - op.idz2 * n.xd) + n.bu * (cx - op.idz2 * n.xu);
+    }
+    if (pin) v += (pin_b + pin_d) * (op.idx2 * n.xc);
+    return v;
+}
+
+// Marches one thread's V cells through planes k0..k1-1, calling f(c, v, k,
+// Ax) for each cell with the operator applied to x.
+template <typename T, int V, typename F>
+__device__ __forceinline__ void march_apply(const FsLayout& L, const OpT<T>& op, const T* __restrict__ x, int i0, int j,
+                                            int k0, T pin_b, T pin_d, F&& f) {
+    int k1 = min(k0 + ZC, L.nz);
+    long c = L.idx(i0, j, k0);
+    T xm[V], xc[V], bzd[V];
+    vld<T, V>(x + c - L.sy, xm);
+    vld<T, V>(x + c, xc);
+    vld<T, V>(op.bz + c, bzd);
+    for (int k = k0; k < k1; k++, c += L.sy) {
+        T xp[V], bzu[V], xb[V], xt[V], bl[V], bb[V], bt[V];
+        vld<T, V>(x + c + L.sy, xp);
+        vld<T, V>(op.bz + c + L.sy, bzu);
+        vld<T, V>(x + c - L.sx, xb);
+        vld<T, V>(x + c + L.sx, xt);
+        vld<T, V>(op.bx + c, bl);
+        vld<T, V>(op.by + c, bb);
+        vld<T, V>(op.by + c + L.sx, bt);
+        T xleft = x[c - 1], xright = x[c + V], bxr = op.bx[c + V];
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            Nb<T> n;
+            n.xc = xc[v];
+            n.xl = v == 0 ? xleft : xc[v - 1];
+            n.xr = v == V - 1 ? xright : xc[v + 1];
+            n.xb = xb[v];
+            n.xt = xt[v];
+            n.xd = xm[v];
+            n.xu = xp[v];
+            n.bl = bl[v];
+            n.br = v == V - 1 ? bxr : bl[v + 1];
+            n.bb = bb[v];
+            n.bt = bt[v];
+            n.bd = bzd[v];
+            n.bu = bzu[v];
+            bool pin = (v == 0) && ((i0 | j | k) == 0);
+            f(c, v, k, row_val(op, n, pin, pin_b, pin_d));
+        }
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            xm[v] = xc[v];
+            xc[v] = xp[v];
+            bzd[v] = bzu[v];
+        }
+    }
+}
+
+// ---- begin: r = b - A x, s = r/D, p = s; partials of r.s, r.r, b.b -------
+template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_begin(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ b,
                                               const T* __restrict__ D, const T* __restrict__ x, T* __restrict__ r,
                                               T* __restrict__ s, T* __restrict__ p, double* part, FsPcgState* S) {
@@ -200,27 +298,20 @@ __global__ void __launch_bounds__(NT) k_begin(FsLayout L, Tiling tl, FsPcgOp o, 
     double a_rs = 0, a_rr = 0, a_bb = 0;
     T pin_b = (T)S->pin_b, pin_d = (T)S->pin_d;
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
-        T xm = x[c - L.sy], xc = x[c], bzd = op.bz[c];
-        for (int k = k0; k < k1; k++, c += L.sy) {
-            T xp = x[c + L.sy], bzu = op.bz[c + L.sy];
-            T ax = row<T>(op, x, c, L.sx, xm, xc, xp, bzd, bzu, (i | j | k) == 0, pin_b, pin_d);
-            T rv = b[c] - ax;
-            T sv = rv / D[c];
-            r[c] = rv;
-            s[c] = sv;
-            p[c] = sv;
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        march_apply<T, V>(L, op, x, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, T ax) {
+            T bv = b[c + v];
+            T rv = bv - ax;
+            T sv = rv / D[c + v];
+            r[c + v] = rv;
+            s[c + v] = sv;
+            p[c + v] = sv;
             a_rs += (double)rv * (double)sv;
             a_rr += (double)rv * (double)rv;
-            a_bb += (double)b[c] * (double)b[c];
-            xm = xc;
-            xc = xp;
-            bzd = bzu;
-        }
+            a_bb += (double)bv * (double)bv;
+        });
     }
     put_partial(part, SLOT_RS, a_rs, lds);
     put_partial(part, SLOT_RR, a_rr, lds);
@@ -246,7 +337,7 @@ __global__ void __launch_bounds__(NT) k_begin_final(const double* part, int nb, 
 }
 
 // ---- K1: q = A p, partial p.q ---------------------------------------------
-template <typename T>
+template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_spmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ p,
                                                  T* __restrict__ q, double* part, const FsPcgState* S) {
     if (S->done) return;
@@ -255,27 +346,21 @@ __global__ void __launch_bounds__(NT) k_spmv_dot(FsLayout L, Tiling tl, FsPcgOp 
     double acc = 0;
     T pin_b = (T)S->pin_b, pin_d = (T)S->pin_d;
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
-        T xm = p[c - L.sy], xc = p[c], bzd = op.bz[c];
-        for (int k = k0; k < k1; k++, c += L.sy) {
-            T xp = p[c + L.sy], bzu = op.bz[c + L.sy];
-            T v = row<T>(op, p, c, L.sx, xm, xc, xp, bzd, bzu, (i | j | k) == 0, pin_b, pin_d);
-            q[c] = v;
-            acc += (double)xc * (double)v;
-            xm = xc;
-            xc = xp;
-            bzd = bzu;
-        }
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        T qv[V];
+        march_apply<T, V>(L, op, p, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, T ap) {
+            qv[v] = ap;
+            acc += (double)p[c + v] * (double)ap;
+            if (v == V - 1) vst<T, V>(q + c, qv);
+        });
     }
     put_partial(part, SLOT_PQ, acc, lds);
 }
 
 // ---- K2: alpha = rz/pq; x += a p, r -= a q, s = r/D; partials r.s, r.r ----
-template <typename T>
+template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_update(FsLayout L, Tiling tl, const T* __restrict__ D, T* __restrict__ x,
                                                T* __restrict__ r, T* __restrict__ s, const T* __restrict__ p,
                                                const T* __restrict__ q, double* part, const FsPcgState* S, int n) {
@@ -285,20 +370,29 @@ __global__ void __launch_bounds__(NT) k_update(FsLayout L, Tiling tl, const T* _
     T alpha = (T)(S->rz[n & 1] / pq);
     double a_rs = 0, a_rr = 0;
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
         int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
+        long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy) {
-            x[c] += alpha * p[c];
-            T rv = r[c];
-            rv -= alpha * q[c];
-            r[c] = rv;
-            T sv = rv / D[c];
-            s[c] = sv;
-            a_rs += (double)rv * (double)sv;
-            a_rr += (double)rv * (double)rv;
+            T xv[V], pv[V], rv[V], qv[V], dv[V], sv[V];
+            vld<T, V>(x + c, xv);
+            vld<T, V>(p + c, pv);
+            vld<T, V>(r + c, rv);
+            vld<T, V>(q + c, qv);
+            vld<T, V>(D + c, dv);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                xv[v] += alpha * pv[v];
+                rv[v] -= alpha * qv[v];
+                sv[v] = rv[v] / dv[v];
+                a_rs += (double)rv[v] * (double)sv[v];
+                a_rr += (double)rv[v] * (double)rv[v];
+            }
+            vst<T, V>(x + c, xv);
+            vst<T, V>(r + c, rv);
+            vst<T, V>(s + c, sv);
         }
     }
     put_partial(part, SLOT_RS, a_rs, lds);
@@ -307,7 +401,7 @@ __global__ void __launch_bounds__(NT) k_update(FsLayout L, Tiling tl, const T* _
 
 // ---- K3: stop test, beta = rz_new/rz, p = p*beta + s ----------------------
 // Workgroup 0 records the outcome for the host and the next iteration.
-template <typename T>
+template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_pupdate(FsLayout L, Tiling tl, T* __restrict__ p, const T* __restrict__ s,
                                                 const double* part, FsPcgState* S, int n) {
     if (S->done) return;
@@ -331,49 +425,73 @@ __global__ void __launch_bounds__(NT) k_pupdate(FsLayout L, Tiling tl, T* __rest
     if (stop) return;
     T beta = (T)(rs / rz_old);
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
         int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
+        long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy) {
-            T pv = p[c];
-            pv *= beta;
-            pv += s[c];
-            p[c] = pv;
+            T pv[V], sv[V];
+            vld<T, V>(p + c, pv);
+            vld<T, V>(s + c, sv);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                pv[v] *= beta;
+                pv[v] += sv[v];
+            }
+            vst<T, V>(p + c, pv);
         }
     }
+}
+
+// ---- launch helpers ---------------------------------------------------------
+template <typename T, int V>
+static void begin_v(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* rhs, const T* diag, T* x,
+                    FsPcgWork& w, FsPcgState* S) {
+    Tiling tl = make_tiling(L, V);
+    hipLaunchKernelGGL((k_begin<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, rhs, diag, x, (T*)w.r,
+                       (T*)w.s, (T*)w.p, w.partials, S);
+    hipLaunchKernelGGL(k_begin_final, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S);
+}
+
+template <typename T, int V>
+static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
+                       FsPcgState* S, int which, int n) {
+    Tiling tl = make_tiling(L, V);
+    if (which == 0)
+        hipLaunchKernelGGL((k_spmv_dot<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const T*)w.p,
+                           (T*)w.q, w.partials, S);
+    else if (which == 1)
+        hipLaunchKernelGGL((k_update<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, diag, x, (T*)w.r, (T*)w.s,
+                           (const T*)w.p, (const T*)w.q, w.partials, S, n);
+    else
+        hipLaunchKernelGGL((k_pupdate<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, (T*)w.p, (const T*)w.s,
+                           w.partials, S, n);
 }
 
 template <typename T>
 static void begin_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* rhs, const T* diag, T* x,
                     FsPcgWork& w, FsPcgState* S) {
-    Tiling tl = make_tiling(L);
-    hipLaunchKernelGGL(k_begin<T>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, rhs, diag, x, (T*)w.r, (T*)w.s,
-                       (T*)w.p, w.partials, S);
-    hipLaunchKernelGGL(k_begin_final, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S);
+    int v = vec_width(L, sizeof(T) == 4);
+    if (v == 4) begin_v<T, (sizeof(T) == 4 ? 4 : 2)>(st, L, op, rhs, diag, x, w, S);
+    else if (v == 2) begin_v<T, 2>(st, L, op, rhs, diag, x, w, S);
+    else begin_v<T, 1>(st, L, op, rhs, diag, x, w, S);
 }
 
 template <typename T>
-static void launch_one(hipStream_t st, const FsLayout& L, const Tiling& tl, const FsPcgOp& op, const T* diag, T* x,
-                       FsPcgWork& w, FsPcgState* S, int which, int n) {
-    if (which == 0)
-        hipLaunchKernelGGL(k_spmv_dot<T>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const T*)w.p, (T*)w.q,
-                           w.partials, S);
-    else if (which == 1)
-        hipLaunchKernelGGL(k_update<T>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, diag, x, (T*)w.r, (T*)w.s,
-                           (const T*)w.p, (const T*)w.q, w.partials, S, n);
-    else
-        hipLaunchKernelGGL(k_pupdate<T>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, (T*)w.p, (const T*)w.s,
-                           w.partials, S, n);
+static void one_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
+                  FsPcgState* S, int which, int n) {
+    int v = vec_width(L, sizeof(T) == 4);
+    if (v == 4) launch_one<T, (sizeof(T) == 4 ? 4 : 2)>(st, L, op, diag, x, w, S, which, n);
+    else if (v == 2) launch_one<T, 2>(st, L, op, diag, x, w, S, which, n);
+    else launch_one<T, 1>(st, L, op, diag, x, w, S, which, n);
 }
 
 template <typename T>
 static void iterations_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
                          FsPcgState* S, int n0, int count) {
-    Tiling tl = make_tiling(L);
     for (int n = n0; n < n0 + count; n++)
-        for (int which = 0; which < 3; which++) launch_one<T>(st, L, tl, op, diag, x, w, S, which, n);
+        for (int which = 0; which < 3; which++) one_t<T>(st, L, op, diag, x, w, S, which, n);
 }
 
 void fs_pcg64_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* rhs, const double* diag,
@@ -395,102 +513,204 @@ void fs_pcg32_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, c
 
 void fs_pcg_time_kernels(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, void* x,
                          FsPcgWork& w, FsPcgState* S, int precision, int which, int n) {
-    Tiling tl = make_tiling(L);
     if (precision == 0)
-        launch_one<double>(st, L, tl, op, (const double*)diag, (double*)x, w, S, which, n);
+        one_t<double>(st, L, op, (const double*)diag, (double*)x, w, S, which, n);
     else
-        launch_one<float>(st, L, tl, op, (const float*)diag, (float*)x, w, S, which, n);
+        one_t<float>(st, L, op, (const float*)diag, (float*)x, w, S, which, n);
 }
 
 // ---- plain operator application (FP64) ----------------------------------
+template <int V>
 __global__ void __launch_bounds__(NT) k_apply64(FsLayout L, Tiling tl, FsPcgOp o, const double* __restrict__ x,
                                                 double* __restrict__ q, const FsPcgState* S) {
     OpT<double> op = make_op<double>(o);
+    double pin_b = S->pin_b, pin_d = S->pin_d;
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
-        double xm = x[c - L.sy], xc = x[c], bzd = op.bz[c];
-        for (int k = k0; k < k1; k++, c += L.sy) {
-            double xp = x[c + L.sy], bzu = op.bz[c + L.sy];
-            q[c] = apply_row<double>(op, x, c, L.sx, xm, xc, xp, bzd, bzu, (i | j | k) == 0, S->pin_b, S->pin_d);
-            xm = xc;
-            xc = xp;
-            bzd = bzu;
-        }
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        march_apply<double, V>(L, op, x, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, double ax) { q[c + v] = ax; });
     }
 }
 
 void fs_apply_op64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* x, double* q,
                    const FsPcgState* S) {
-    Tiling tl = make_tiling(L);
-    hipLaunchKernelGGL(k_apply64, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, x, q, S);
+    int v = vec_width(L, 0);
+    Tiling tl = make_tiling(L, v);
+    if (v == 2)
+        hipLaunchKernelGGL(k_apply64<2>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, x, q, S);
+    else
+        hipLaunchKernelGGL(k_apply64<1>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, x, q, S);
 }
 
 // ---- refinement pieces ----------------------------------------------------
+template <int V>
 __global__ void __launch_bounds__(NT) k_residual64(FsLayout L, Tiling tl, FsPcgOp o, const double* __restrict__ b,
                                                    const double* __restrict__ x, double* __restrict__ r,
                                                    float* __restrict__ r32, double* part, const FsPcgState* S) {
     __shared__ double lds[4];
     OpT<double> op = make_op<double>(o);
-    double a_rr = 0, a_bb = 0;
+    double a_rr = 0, a_bb = 0, a_sum = 0;
+    double pin_b = S->pin_b, pin_d = S->pin_d;
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
-        double xm = x[c - L.sy], xc = x[c], bzd = op.bz[c];
-        for (int k = k0; k < k1; k++, c += L.sy) {
-            double xp = x[c + L.sy], bzu = op.bz[c + L.sy];
-            double ax = apply_row<double>(op, x, c, L.sx, xm, xc, xp, bzd, bzu, (i | j | k) == 0, S->pin_b, S->pin_d);
-            double rv = b[c] - ax;
-            r[c] = rv;
-            r32[c] = (float)rv;
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        march_apply<double, V>(L, op, x, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, double ax) {
+            double bv = b[c + v];
+            double rv = bv - ax;
+            r[c + v] = rv;
+            r32[c + v] = (float)rv;
             a_rr += rv * rv;
-            a_bb += b[c] * b[c];
-            xm = xc;
-            xc = xp;
-            bzd = bzu;
-        }
+            a_bb += bv * bv;
+            a_sum += rv;
+        });
     }
     put_partial(part, SLOT_RR, a_rr, lds);
     put_partial(part, SLOT_BB, a_bb, lds);
+    put_partial(part, SLOT_SUM, a_sum, lds);
 }
 
 __global__ void __launch_bounds__(NT) k_residual_final(const double* part, int nb, FsPcgState* S) {
     __shared__ double lds[4];
     double rr = get_total(part, SLOT_RR, nb, lds);
     double bb = get_total(part, SLOT_BB, nb, lds);
+    double sum = get_total(part, SLOT_SUM, nb, lds);
     if (threadIdx.x == 0 && threadIdx.y == 0) {
         S->rr = rr;
         S->bb = bb;
+        S->rsum = sum;
     }
+}
+
+// x += rsum/pin on every interior cell; the pinned cell's residual drops by
+// rsum (its row is the only one a constant reaches), r.r follows.
+__global__ void __launch_bounds__(NT) k_shift(FsLayout L, double* __restrict__ x, double* __restrict__ r,
+                                              float* __restrict__ r32, FsPcgState* S, double pin) {
+    double sum = S->rsum;
+    double cst = sum / pin;
+    int i = blockIdx.x * TBX + threadIdx.x, j = blockIdx.y * TBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    x[c] += cst;
+    if ((i | j | k) == 0) {
+        double r0 = r[c], r1 = r0 - sum;
+        r[c] = r1;
+        r32[c] = (float)r1;
+        // rsum itself stays: every workgroup of this kernel reads it
+        S->rr = S->rr - r0 * r0 + r1 * r1;
+    }
+}
+
+void fs_deflate_constant(hipStream_t st, const FsLayout& L, double* x, double* r, float* r32, FsPcgState* S,
+                         double pin) {
+    dim3 g((L.nx + TBX - 1) / TBX, (L.ny + TBY - 1) / TBY, L.nz);
+    hipLaunchKernelGGL(k_shift, g, dim3(TBX, TBY), 0, st, L, x, r, r32, S, pin);
 }
 
 void fs_residual64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* b, const double* x, double* r,
                    float* r32, FsPcgWork& w, FsPcgState* S) {
-    Tiling tl = make_tiling(L);
-    hipLaunchKernelGGL(k_residual64, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, b, x, r, r32, w.partials, S);
+    int v = vec_width(L, 0);
+    Tiling tl = make_tiling(L, v);
+    if (v == 2)
+        hipLaunchKernelGGL(k_residual64<2>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, b, x, r, r32, w.partials,
+                           S);
+    else
+        hipLaunchKernelGGL(k_residual64<1>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, b, x, r, r32, w.partials,
+                           S);
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S);
+}
+
+// Reliable update of the FP32 recursion, between the r-update (K2) and the
+// p-update (K3) of an iteration: the recursion's residual is replaced by the
+// rounded FP64 residual, s = r/D and the partials of r.s and r.r are
+// rewritten, so the pending K3 takes beta and its stop test from the
+// refreshed residual.
+template <int V>
+__global__ void __launch_bounds__(NT) k_reprecond(FsLayout L, Tiling tl, const float* __restrict__ r,
+                                                  const float* __restrict__ D, float* __restrict__ s,
+                                                  float* __restrict__ rrec, double* part) {
+    __shared__ double lds[4];
+    double a_rs = 0, a_rr = 0;
+    for (TileIter it(tl); it.valid(); it.next()) {
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        int k1 = min(k0 + ZC, L.nz);
+        long c = L.idx(i0, j, k0);
+        for (int k = k0; k < k1; k++, c += L.sy) {
+            float rv[V], dv[V], sv[V];
+            vld<float, V>(r + c, rv);
+            vld<float, V>(D + c, dv);
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sv[v] = rv[v] / dv[v];
+                a_rs += (double)rv[v] * (double)sv[v];
+                a_rr += (double)rv[v] * (double)rv[v];
+            }
+            vst<float, V>(s + c, sv);
+            vst<float, V>(rrec + c, rv);  // the recursion continues from the FP64 residual
+        }
+    }
+    put_partial(part, SLOT_RS, a_rs, lds);
+    put_partial(part, SLOT_RR, a_rr, lds);
+}
+
+// The r.r of the last K2, for the host's reliable-update decision; also
+// clears `done` when asked (a recursion that met its target resumes).
+__global__ void __launch_bounds__(NT) k_peek(const double* part, int nb, FsPcgState* S, int clear_done) {
+    __shared__ double lds[4];
+    double rr = get_total(part, SLOT_RR, nb, lds);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        S->rr = rr;
+        if (clear_done) S->done = 0;
+    }
+}
+
+void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r32, const float* diag, FsPcgWork& w,
+                              FsPcgState* S) {
+    int v = vec_width(L, 1);
+    Tiling tl = make_tiling(L, v);
+    if (v == 4)
+        hipLaunchKernelGGL(k_reprecond<4>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, r32, diag, (float*)w.s,
+                           (float*)w.r, w.partials);
+    else if (v == 2)
+        hipLaunchKernelGGL(k_reprecond<2>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, r32, diag, (float*)w.s,
+                           (float*)w.r, w.partials);
+    else
+        hipLaunchKernelGGL(k_reprecond<1>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, r32, diag, (float*)w.s,
+                           (float*)w.r, w.partials);
+    hipLaunchKernelGGL(k_peek, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, 1);
+}
+
+// FP32 iterations n0 .. n0+count-1 that end after the r-update of the last
+// one: [K3(n0-1) if pending] K1 K2 K3 ... K1 K2, then the r.r peek.
+void fs_pcg32_chunk(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x, FsPcgWork& w,
+                    FsPcgState* S, int n0, int count, int pending_k3) {
+    if (pending_k3) one_t<float>(st, L, op, diag, x, w, S, 2, n0 - 1);
+    for (int n = n0; n < n0 + count; n++) {
+        one_t<float>(st, L, op, diag, x, w, S, 0, n);
+        one_t<float>(st, L, op, diag, x, w, S, 1, n);
+        if (n < n0 + count - 1) one_t<float>(st, L, op, diag, x, w, S, 2, n);
+    }
+    Tiling tl = make_tiling(L, vec_width(L, 1));
+    hipLaunchKernelGGL(k_peek, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, 0);
 }
 
 __global__ void __launch_bounds__(NT) k_add_corr(FsLayout L, Tiling tl, double* __restrict__ x,
                                                  const float* __restrict__ e) {
     for (TileIter it(tl); it.valid(); it.next()) {
-        int i, j, k0;
-        tile_origin(tl, it.t, i, j, k0);
-        if (i >= L.nx || j >= L.ny) continue;
+        int i0, j, k0;
+        tile_origin<1>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
         int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i, j, k0);
+        long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy) x[c] += (double)e[c];
     }
 }
 
 void fs_add_correction(hipStream_t st, const FsLayout& L, double* x, const float* e) {
-    Tiling tl = make_tiling(L);
+    Tiling tl = make_tiling(L, 1);
     hipLaunchKernelGGL(k_add_corr, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, x, e);
 }
 

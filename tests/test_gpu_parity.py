@@ -123,6 +123,27 @@ def test_frames_fp64_tight(hip_lib, oracle_lib, name, n, frames):
         assert e < 1e-8, (c, e)
 
 
+@pytest.mark.parametrize("name,n", [("rising_drop", 32), ("dam_break", 24), ("bubbles", 32)])
+def test_fp32_refined_projection_matches_fp64(hip_lib, oracle_lib, name, n):
+    """FP32 PCG with FP64 refinement reaches 1e-8 relative residual (checked
+    in FP64 on the device) and its fields match the FP64 path within what
+    two solutions at that residual can differ by."""
+    over = dict(stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8)
+    g64, cpu = make_pair(hip_lib, oracle_lib, name, n=n, **over)
+    g32, _ = make_pair(hip_lib, oracle_lib, name, n=n, precision=_abi.FS_FP32_REFINE, **over)
+    dt = 2e-3
+    s64, s32, sc = g64.projection(dt), g32.projection(dt), cpu.projection(dt)
+    print(name, n, "fp64", s64, "fp32+refine", s32)
+    assert s32["residual_rel"] <= 1e-8 and s64["residual_rel"] <= 1e-8
+    assert s32["pcg_iterations"] <= 1.5 * s64["pcg_iterations"] + 20
+    f64, f32, fc = g64.get_fields(), g32.get_fields(), cpu.get_fields()
+    for c in ("w", "p"):
+        a, b = f32[c], fc[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        assert rel_err(a, b) < 1e-5, (c, rel_err(a, b))
+
+
 @pytest.mark.parametrize("name,n,frames", [("rising_drop", 24, 4), ("dam_break", 24, 4)])
 def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
     """The reference's absolute stop rule at its script tolerance (1e-5):
