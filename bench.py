@@ -38,13 +38,17 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-fp64", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run a 1 GiB device copy first (profiler byte-counter calibration)")
     return ap.parse_args()
 
 
 def run_frames(solver, dt, warmup, steps, sync):
+    from fluid_simulator_backup_amd import _abi
     for _ in range(warmup):
         solver.advance_frame(dt)
     sync()
+    _abi.set_kernel_sampling(solver, True)
     stats = []
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -54,17 +58,41 @@ def run_frames(solver, dt, warmup, steps, sync):
     return (t1 - t0) * 1e3 / steps, stats
 
 
-def cpu_baseline(scene, n, iters_per_step, threads):
-    """Composed per-step time of the CPU checker at n^3 from a bounded sample:
-    one projection capped at 4 and at 12 PCG iterations (fixed cost and
-    per-iteration cost), one 2-sweep reinitialization call, one level-set
-    stage, one velocity-advection stage, gravity and viscosity."""
+def measured_traffic(scene, n, kernel):
+    """HBM bytes per launch of `kernel` (FETCH_SIZE and WRITE_SIZE per
+    dispatch from rocprofv3 PMC runs of this bench, scaled by the counters'
+    reading of a 1 GiB copy), when a profile of this scene and size is
+    committed under profiles/; else None."""
+    path = os.path.join(ROOT, "profiles", f"pcg_traffic_{scene}_{n}.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel]
+        return int(k["read_bytes"] + k["write_bytes"])
+    except (OSError, KeyError, ValueError):
+        return None
+
+
+def divergent_velocity(n):
+    """Smooth MAC velocity with nonzero divergence, so that a projection on
+    its own has work to do (the scenes start at rest)."""
+    c = (np.arange(n) + 0.5) / n
+    f = np.arange(n + 1) / n
+    u = 0.1 * np.sin(np.pi * f)[None, None, :] * np.cos(np.pi * c)[None, :, None] * np.ones((n, 1, 1))
+    v = 0.1 * np.sin(2 * np.pi * f)[None, :, None] * np.cos(np.pi * c)[:, None, None] * np.ones((1, 1, n))
+    w = 0.1 * np.sin(3 * np.pi * f)[:, None, None] * np.cos(np.pi * c)[None, None, :] * np.ones((1, n, 1))
+    return u, v, w
+
+
+def cpu_baseline(scene, n, iters_per_step, dt, threads):
+    """Per-step time of the CPU checker at n^3, composed from a bounded
+    sample of the same frame: the projection timed with PCG capped at 4 and
+    at 24 iterations (assembly + per-iteration cost), and each other phase of
+    the frame timed once with the frame's own parameters."""
     from fluid_simulator_backup_amd import _abi, scenes
     lib = _abi.load(os.path.join(ROOT, "oracle", "libfs_oracle.so"), "fso_")
     lib.fso_set_threads(threads)
-    phi, vel, P, g = scenes.make(scene, n)
-    P.iteration_number_for_reinitialization = 2
-    P.stop_rule = _abi.FS_STOP_REL_RESIDUAL
+    phi, _, P, g = scenes.make(scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8)
+    vel = divergent_velocity(n)
 
     def timed(fn):
         t = time.perf_counter()
@@ -74,32 +102,38 @@ def cpu_baseline(scene, n, iters_per_step, threads):
     def fresh(max_it):
         P.max_iteration = max_it
         s = _abi.Solver(lib, "fso_", P, g)
-        lib.fso_set_viscosity_inplace(s.h, 0)
+        lib.fso_set_viscosity_inplace(s.h, 0)  # the HIP path's out-of-place order
         s.set_fields(phi, *vel)
         return s
 
-    dt = 1e-3
-    s = fresh(4)
-    t4 = timed(lambda: s.projection(dt, allow_not_converged=True))
-    s.close()
-    s = fresh(12)
-    t12 = timed(lambda: s.projection(dt, allow_not_converged=True))
-    t_iter = max((t12 - t4) / 8, 0.0)
-    t_fixed = max(t4 - 4 * t_iter, 0.0)
-    t_reinit2 = timed(lambda: s.phase("reinitialization", dt))
+    t_proj = {}
+    for cap in (4, 24):
+        s = fresh(cap)
+        s.projection(dt, allow_not_converged=True)  # first touch of the system's pages
+        s.set_fields(phi, *vel)
+        t_proj[cap] = timed(lambda: s.projection(dt, allow_not_converged=True))
+        if cap == 4:
+            s.close()
+    t_iter = (t_proj[24] - t_proj[4]) / 20
+    t_fixed = t_proj[4] - 4 * t_iter
+    t_curv = timed(s.compute_curvatures)
     t_ls = timed(lambda: s.phase("levelset_advection", dt))
     t_vel = timed(lambda: s.phase("velocity_advection", dt))
     t_grav = timed(lambda: s.phase("apply_gravity", dt))
     t_visc = timed(lambda: s.phase("sourcing", dt))
+    t_reinit = timed(lambda: s.phase("reinitialization", dt))
     s.close()
-    per_step = (t_fixed + iters_per_step * t_iter + 25 * t_reinit2 + 2 * t_ls + t_vel + t_grav + t_visc)
-    sample_s = t4 + t12 + t_reinit2 + t_ls + t_vel + t_grav + t_visc
+    # a frame: curvature ghost refresh, 2 level-set stages, ENO3 MAC stage,
+    # gravity, viscosity, projection, 2 reinitialization calls of 25 sweeps
+    per_step = t_curv + 2 * t_ls + t_vel + t_grav + t_visc + t_fixed + iters_per_step * t_iter + 2 * t_reinit
+    timed_s = t_proj[4] + t_proj[24] + t_curv + t_ls + t_vel + t_grav + t_visc + t_reinit
     return {
-        "value": per_step * 1e3, "unit": "ms/step", "cores": threads, "kind": "port",
-        "sample": (f"CPU checker at {n}^3, {threads} threads: projection capped at 4 and 12 PCG iterations "
-                   f"(fixed {t_fixed:.2f}s + {t_iter*1e3:.1f} ms/iteration, scaled to {iters_per_step} iterations), "
-                   f"one 2-sweep reinitialization call x25, level-set stage x2, ENO3 MAC stage, gravity, "
-                   f"out-of-place viscosity; {sample_s:.1f}s of CPU work"),
+        "value": round(per_step * 1e3, 1), "unit": "ms/step", "cores": threads, "kind": "port",
+        "sample": (f"CPU checker (C, OpenMP, FP64) at {n}^3 with {threads} threads on a divergent velocity field: "
+                   f"projection capped at 4 and 24 PCG iterations -> {t_fixed:.2f} s assembly + "
+                   f"{t_iter * 1e3:.1f} ms/iteration, scaled to the FP64 path's {iters_per_step:.0f} iterations; "
+                   f"one 25-sweep reinitialization call (x2), level-set stage (x2), ENO3 MAC stage, gravity, "
+                   f"viscosity, curvature ghost refresh, each timed once; {timed_s:.1f} s timed"),
     }
 
 
@@ -116,6 +150,13 @@ def main():
     torch.cuda.set_device(0)
     lib = fsa.load_library()
     sync = torch.cuda.synchronize
+    if a.copy_calibration:
+        src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        sync()
+        del src, dst
     phi, vel, P32, g = scenes.make(a.scene, a.n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
                                    precision=_abi.FS_FP32_REFINE, max_iteration=20000)
     cells = a.n ** 3
@@ -126,8 +167,10 @@ def main():
           flush=True)
     ms32, st32 = run_frames(s32, dt, a.warmup, a.steps, sync)
     print(f"[bench] fp32+refine {ms32:.2f} ms/step; last {st32[-1]}", file=sys.stderr, flush=True)
-    # dominant kernels of the FP32 inner loop
-    k32 = _abi.time_pcg_kernels(s32, _abi.FS_FP32_REFINE, 20)
+    # FP32 inner-loop kernels: in-loop event samples from the timed frames,
+    # and back-to-back repeats of each kernel for comparison
+    k32, n32 = _abi.kernel_samples(s32, _abi.FS_FP32_REFINE)
+    k32_rep = _abi.time_pcg_kernels(s32, _abi.FS_FP32_REFINE, 20)
     s32.close()
 
     res = {}
@@ -139,25 +182,31 @@ def main():
         s64.set_fields(phi, *vel)
         ms64, st64 = run_frames(s64, dt, a.warmup, a.steps, sync)
         print(f"[bench] fp64 {ms64:.2f} ms/step; last {st64[-1]}", file=sys.stderr, flush=True)
-        k64 = _abi.time_pcg_kernels(s64, _abi.FS_FP64, 20)
+        k64, _ = _abi.kernel_samples(s64, _abi.FS_FP64)
         s64.close()
         res["fp64"] = {"ms_per_step": ms64, "pcg_iterations_per_step": float(np.mean([s["pcg_iterations"] for s in st64])),
                        "projection_ms": float(np.mean([s["projection_ms"] for s in st64])),
                        "residual_rel": st64[-1]["residual_rel"],
-                       "kernel_ms": {"spmv_dot": k64[0], "update": k64[1], "p_update": k64[2]}}
+                       "kernel_ms": {"spmv_dot": float(k64[0]), "update": float(k64[1]), "p_update": float(k64[2])}}
 
     # roofline: algorithmic bytes per launch / measured duration
     words = {"spmv_dot": 5, "update": 8, "p_update": 3}
     names = ["spmv_dot", "update", "p_update"]
-    dom = int(np.argmax(k32 * np.array([1, 1, 1])))
+    kname = {"spmv_dot": "k_spmv_dot", "update": "k_update", "p_update": "k_pupdate"}
+    dom = int(np.argmax(k32))
     name = names[dom]
     bytes_launch = words[name] * cells * 4
     achieved = bytes_launch / (k32[dom] * 1e-3) / 1e9
-    roof = {"bound": "hbm", "kernel": f"k_{'update' if name == 'update' else name}<float>",
-            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-            "bytes_per_launch": bytes_launch,
-            "all_kernels_gbs": {nm: round(words[nm] * cells * 4 / (k32[i] * 1e-3) / 1e9, 1) for i, nm in enumerate(names)}}
+    gbs = lambda ms, i, wb: round(words[names[i]] * cells * wb / (ms * 1e-3) / 1e9, 1)
+    roof = {"bound": "hbm", "kernel": f"{kname[name]}<float>", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(a.scene, a.n, kname[name]),
+            "bytes_per_launch": bytes_launch, "avg_launch_ms": round(float(k32[dom]), 5),
+            "timing": f"HIP events on the solver stream around one iteration per 16 of the timed frames "
+                      f"({int(n32[dom])} samples)",
+            "all_kernels_gbs": {nm: gbs(k32[i], i, 4) for i, nm in enumerate(names)},
+            "all_kernels_ms": {nm: round(float(k32[i]), 5) for i, nm in enumerate(names)},
+            "back_to_back_repeat_gbs": {nm: gbs(k32_rep[i], i, 4) for i, nm in enumerate(names)}}
     if "fp64" in res:
         k64 = res["fp64"]["kernel_ms"]
         res["fp64"]["kernel_gbs"] = {nm: round(words[nm] * cells * 8 / (k64[nm] * 1e-3) / 1e9, 1) for nm in names}
@@ -168,7 +217,7 @@ def main():
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
         iters = res["fp64"]["pcg_iterations_per_step"] if "fp64" in res else iters32
         try:
-            cpu = cpu_baseline(a.scene, a.n, iters, threads)
+            cpu = cpu_baseline(a.scene, a.n, iters, dt, threads)
         except Exception as e:  # the baseline must not sink the GPU line
             cpu = {"error": str(e)}
     out = {

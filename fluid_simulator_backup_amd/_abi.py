@@ -170,6 +170,8 @@ def bind(lib, prefix):
     f("apply_pressure_operator").argtypes = [vp, _PD, _PD]
     if prefix == "fs_":
         lib.fs_time_pcg_kernels.argtypes = [vp, _I, _I, _PD]
+        lib.fs_set_kernel_sampling.argtypes = [vp, _I]
+        lib.fs_get_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(_I)]
         lib.fs_last_error.argtypes = [vp]
         lib.fs_last_error.restype = C.c_char_p
     return lib
@@ -288,6 +290,18 @@ def time_pcg_kernels(solver, precision, reps):
     out = np.zeros(3)
     solver._check(solver.lib.fs_time_pcg_kernels(solver.h, precision, reps, _ptr(out)), "time_pcg_kernels")
     return out
+
+
+def set_kernel_sampling(solver, enable):
+    solver._check(solver.lib.fs_set_kernel_sampling(solver.h, int(bool(enable))), "set_kernel_sampling")
+
+
+def kernel_samples(solver, precision):
+    """Mean in-loop duration (ms) and sample count of the three PCG kernels."""
+    avg = np.zeros(3)
+    cnt = (_I * 3)()
+    solver._check(solver.lib.fs_get_kernel_samples(solver.h, precision, _ptr(avg), cnt), "get_kernel_samples")
+    return avg, np.array(list(cnt))
 
 
 def load(path, prefix):

@@ -79,6 +79,13 @@ class EulerianFluidSolver3D {
     bool debug = false;         // FS_DEBUG: trace refinement on stderr
     int dbg_lines = 0;
     int chunk = 16;
+    // in-loop kernel sampling (measurement only): one iteration per launch
+    // batch bracketed with events; sums per precision (0 FP64, 1 FP32) and
+    // kernel (0 SpMV+dot, 1 update, 2 p-update)
+    FsKernelProbe probe{};
+    bool sampling = false;
+    double ksum[2][3] = {};
+    int kcount[2][3] = {};
 
     void* dalloc(size_t bytes) {
         void* ptr = nullptr;
@@ -111,6 +118,7 @@ class EulerianFluidSolver3D {
         HIPCHECK(hipGetDevice(&device));
         HIPCHECK(hipStreamCreate(&st));
         for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
+        for (auto& e : probe.ev) HIPCHECK(hipEventCreate(&e));
         L = fs_make_layout(G.nx, G.ny, G.nz);
         cell.x = fs_make_axis(G.x_min, G.x_max, G.nx);
         cell.y = fs_make_axis(G.y_min, G.y_max, G.ny);
@@ -170,6 +178,8 @@ class EulerianFluidSolver3D {
         if (hS) (void)hipHostFree(hS);
         if (hvmax) (void)hipHostFree(hvmax);
         for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+        for (auto& e : probe.ev)
             if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -324,6 +334,20 @@ class EulerianFluidSolver3D {
     }
     double* S64pin() { return scratch + G.nz + 2; }
 
+    // After a launch batch and read_state: add the probed iteration's kernel
+    // durations, if that iteration ran in full (it began before the stop).
+    void harvest(int prec) {
+        if (!sampling || !probe.recorded) return;
+        probe.recorded = false;
+        if (hS->done && hS->iter <= probe.iter) return;
+        for (int k = 0; k < 3; k++) {
+            float ms = 0;
+            HIPCHECK(hipEventElapsedTime(&ms, probe.ev[2 * k], probe.ev[2 * k + 1]));
+            ksum[prec][k] += ms;
+            kcount[prec][k]++;
+        }
+    }
+
     void read_state(FsPcgState* dS) {
         HIPCHECK(hipMemcpyAsync(hS, dS, sizeof(FsPcgState), hipMemcpyDeviceToHost, st));
         sync();
@@ -337,10 +361,11 @@ class EulerianFluidSolver3D {
         read_state(S64);
         int n = 0;
         while (!hS->done) {
-            fs_pcg64_iterations(st, L, o, diag, x, w64, S64, n, chunk);
+            fs_pcg64_iterations(st, L, o, diag, x, w64, S64, n, chunk, sampling ? &probe : nullptr);
             n += chunk;
             check_launch();
             read_state(S64);
+            harvest(0);
         }
         stats.residual_precond = std::sqrt(hS->rs_last);
         return hS->iter;
@@ -387,9 +412,10 @@ class EulerianFluidSolver3D {
         double rr_ref = rr64;
         int n = 0, pending = 0;
         while (true) {
-            fs_pcg32_chunk(st, L, o32, diag32, e32, w32, S32, n, chunk, pending);
+            fs_pcg32_chunk(st, L, o32, diag32, e32, w32, S32, n, chunk, pending, sampling ? &probe : nullptr);
             check_launch();
             read_state(S32);
+            harvest(1);
             if (hS->done == 2) {
                 n = hS->iter;
                 break;
@@ -758,6 +784,31 @@ int fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms) {
             float ms = 0;
             HIPCHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
             avg_ms[which] = ms / reps;
+        }
+        return FS_OK;
+    });
+}
+
+int fs_set_kernel_sampling(fs_solver* s, int enable) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.sampling = enable != 0;
+        S.probe.recorded = false;
+        for (int p = 0; p < 2; p++)
+            for (int k = 0; k < 3; k++) {
+                S.ksum[p][k] = 0;
+                S.kcount[p][k] = 0;
+            }
+        return FS_OK;
+    });
+}
+
+int fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* count) {
+    if (!avg_ms || !count || (precision != FS_FP64 && precision != FS_FP32_REFINE)) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        int p = precision == FS_FP64 ? 0 : 1;
+        for (int k = 0; k < 3; k++) {
+            count[k] = S.kcount[p][k];
+            avg_ms[k] = S.kcount[p][k] ? S.ksum[p][k] / S.kcount[p][k] : 0;
         }
         return FS_OK;
     });

@@ -37,12 +37,22 @@ struct FsPcgWork {
 #define FS_PCG_MAX_BLOCKS 2048
 #define FS_PCG_SLOTS 8
 
+// Measurement only: HIP events recorded on the solve's stream around the
+// three kernels of the first iteration of a launch batch, ev[2w] before and
+// ev[2w+1] after kernel w (0 SpMV+dot, 1 update, 2 p-update).  `iter` is that
+// iteration; `recorded` is set when the events were enqueued.
+struct FsKernelProbe {
+    hipEvent_t ev[6];
+    int iter;
+    bool recorded;
+};
+
 // FP64 / FP32 solves on x (warm start).  begin: r = b - A x, z = D^-1 r,
 // p = z and the scalars; iterations: n0 .. n0+count-1.
 void fs_pcg64_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* rhs, const double* diag,
                     double* x, FsPcgWork& w, FsPcgState* dstate);
 void fs_pcg64_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* diag, double* x,
-                         FsPcgWork& w, FsPcgState* dstate, int n0, int count);
+                         FsPcgWork& w, FsPcgState* dstate, int n0, int count, FsKernelProbe* probe = nullptr);
 void fs_pcg32_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* rhs, const float* diag,
                     float* x, FsPcgWork& w, FsPcgState* dstate);
 void fs_pcg32_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x,
@@ -64,9 +74,10 @@ void fs_residual64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const d
                    float* r32, FsPcgWork& w, FsPcgState* dstate);
 // FP32 iterations n0 .. n0+count-1 ending after the r-update (K2) of the
 // last one, whose p-update (K3) is left pending; pending_k3 != 0 first runs
-// the pending K3 of iteration n0-1.  state->rr receives the last r.r.
+// the pending K3 of iteration n0-1.  state->rr receives the last r.r.  The
+// probe, if any, brackets iteration n0 when count >= 2.
 void fs_pcg32_chunk(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x, FsPcgWork& w,
-                    FsPcgState* dstate, int n0, int count, int pending_k3);
+                    FsPcgState* dstate, int n0, int count, int pending_k3, FsKernelProbe* probe = nullptr);
 // Reliable update between K2 and the pending K3, with r32 holding the
 // rounded FP64 residual: the recursion's r = r32, s = r32/D and the r.s / r.r
 // partials the pending K3 reduces; clears `done`.

@@ -487,11 +487,29 @@ static void one_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T*
     else launch_one<T, 1>(st, L, op, diag, x, w, S, which, n);
 }
 
+// Kernel `which` of iteration n, bracketed by the probe's events when n is
+// the probed iteration.
+template <typename T>
+static void one_probed(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
+                       FsPcgState* S, int which, int n, FsKernelProbe* probe) {
+    bool rec = probe && probe->iter == n;
+    if (rec) (void)hipEventRecord(probe->ev[2 * which], st);
+    one_t<T>(st, L, op, diag, x, w, S, which, n);
+    if (rec) {
+        (void)hipEventRecord(probe->ev[2 * which + 1], st);
+        if (which == 2) probe->recorded = true;
+    }
+}
+
 template <typename T>
 static void iterations_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
-                         FsPcgState* S, int n0, int count) {
+                         FsPcgState* S, int n0, int count, FsKernelProbe* probe) {
+    if (probe) {
+        probe->iter = n0;
+        probe->recorded = false;
+    }
     for (int n = n0; n < n0 + count; n++)
-        for (int which = 0; which < 3; which++) one_t<T>(st, L, op, diag, x, w, S, which, n);
+        for (int which = 0; which < 3; which++) one_probed<T>(st, L, op, diag, x, w, S, which, n, probe);
 }
 
 void fs_pcg64_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* rhs, const double* diag,
@@ -499,8 +517,8 @@ void fs_pcg64_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const 
     begin_t<double>(st, L, op, rhs, diag, x, w, S);
 }
 void fs_pcg64_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* diag, double* x,
-                         FsPcgWork& w, FsPcgState* S, int n0, int count) {
-    iterations_t<double>(st, L, op, diag, x, w, S, n0, count);
+                         FsPcgWork& w, FsPcgState* S, int n0, int count, FsKernelProbe* probe) {
+    iterations_t<double>(st, L, op, diag, x, w, S, n0, count, probe);
 }
 void fs_pcg32_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* rhs, const float* diag,
                     float* x, FsPcgWork& w, FsPcgState* S) {
@@ -508,7 +526,7 @@ void fs_pcg32_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const 
 }
 void fs_pcg32_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x,
                          FsPcgWork& w, FsPcgState* S, int n0, int count) {
-    iterations_t<float>(st, L, op, diag, x, w, S, n0, count);
+    iterations_t<float>(st, L, op, diag, x, w, S, n0, count, nullptr);
 }
 
 void fs_pcg_time_kernels(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, void* x,
@@ -686,12 +704,16 @@ void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r3
 // FP32 iterations n0 .. n0+count-1 that end after the r-update of the last
 // one: [K3(n0-1) if pending] K1 K2 K3 ... K1 K2, then the r.r peek.
 void fs_pcg32_chunk(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x, FsPcgWork& w,
-                    FsPcgState* S, int n0, int count, int pending_k3) {
+                    FsPcgState* S, int n0, int count, int pending_k3, FsKernelProbe* probe) {
+    if (probe) {
+        probe->iter = count >= 2 ? n0 : -1;
+        probe->recorded = false;
+    }
     if (pending_k3) one_t<float>(st, L, op, diag, x, w, S, 2, n0 - 1);
     for (int n = n0; n < n0 + count; n++) {
-        one_t<float>(st, L, op, diag, x, w, S, 0, n);
-        one_t<float>(st, L, op, diag, x, w, S, 1, n);
-        if (n < n0 + count - 1) one_t<float>(st, L, op, diag, x, w, S, 2, n);
+        one_probed<float>(st, L, op, diag, x, w, S, 0, n, probe);
+        one_probed<float>(st, L, op, diag, x, w, S, 1, n, probe);
+        if (n < n0 + count - 1) one_probed<float>(st, L, op, diag, x, w, S, 2, n, probe);
     }
     Tiling tl = make_tiling(L, vec_width(L, 1));
     hipLaunchKernelGGL(k_peek, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, 0);

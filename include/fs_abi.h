@@ -182,6 +182,16 @@ int  fs_apply_pressure_operator(fs_solver* s, const double* x, double* q);
  * (PCG_METHOD.cpp:6, LOG.h:191-338) for roofline accounting. */
 int  fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms);
 
+/* In-loop kernel sampling.  While enabled, ordinary solves bracket the three
+ * PCG kernels of one iteration per launch batch (16 iterations) with HIP
+ * events on the handle's stream; fs_get_kernel_samples returns, per kernel
+ * (0 SpMV + p.q, 1 update, 2 p update) of the FP64 or the FP32 inner
+ * iteration, the mean duration and the sample count since sampling was last
+ * enabled (enabling clears them).  No reference counterpart: measurement
+ * hooks beside fs_time_pcg_kernels. */
+int  fs_set_kernel_sampling(fs_solver* s, int enable);
+int  fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* count);
+
 #ifdef __cplusplus
 }
 #endif

@@ -64,6 +64,7 @@ struct fso_state {
     int N, nnz;
     int *rp, *ci;
     double *val, *b, *x, *r, *zv, *pp, *Ap;
+    double* plane_part; /* per-plane partial sums of plane_sum_dot */
     int visc_inplace;
     double max_vel[3];
     int last_iters;
@@ -119,6 +120,7 @@ int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     s->vtmp = zalloc(n);
     s->fbx = zalloc(n); s->fby = zalloc(n); s->fbz = zalloc(n);
     s->bc = (int*)calloc((size_t)n, sizeof(int));
+    s->plane_part = (double*)calloc((size_t)s->nz, sizeof(double));
     s->visc_inplace = 1;
     *out = s;
     return FS_OK;
@@ -135,6 +137,7 @@ void fso_destroy(fso_state* s) {
     free(s->phi); free(s->sghost); free(s->u); free(s->v); free(s->w); free(s->gu); free(s->gv); free(s->gw);
     free(s->p); free(s->div); free(s->rho); free(s->irho); free(s->sgn); free(s->tmp); free(s->pg); free(s->vtmp); free(s->fbx); free(s->fby); free(s->fbz);
     free(s->bc);
+    free(s->plane_part);
     free_csr(s);
     free(s);
 }
@@ -572,16 +575,20 @@ int fso_sourcing(fso_state* s, double dt) {
 /* ------------------------------------------------------------------------- */
 
 static double plane_sum_dot(fso_state* s, const double* a, const double* b) {
-    /* rows are numbered plane by plane; per-plane partials in order */
+    /* rows are numbered plane by plane; per-plane partials (each summed in
+     * row order, planes in parallel), then added in plane order */
     int per = s->nx * s->ny;
-    double total = 0;
+    double* part = s->plane_part;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
     for (int k = 0; k < s->nz; k++) {
-        double part = 0;
+        double acc = 0;
         const double* aa = a + (long)k * per;
         const double* bb = b + (long)k * per;
-        for (int r = 0; r < per; r++) part += aa[r] * bb[r];
-        total += part;
+        for (int r = 0; r < per; r++) acc += aa[r] * bb[r];
+        part[k] = acc;
     }
+    double total = 0;
+    for (int k = 0; k < s->nz; k++) total += part[k];
     return total;
 }
 
@@ -763,9 +770,12 @@ static void pcg(fso_state* s) {
         csr_mul(s, p, Ap);
         double dot_result = plane_sum_dot(s, p, Ap);
         double alpha = res_new / dot_result;
-        for (int i = 0; i < N; i++) x[i] += alpha * p[i];
-        for (int i = 0; i < N; i++) r[i] -= alpha * Ap[i];
-        for (int i = 0; i < N; i++) sv[i] = r[i] / dg[i];
+#pragma omp parallel for num_threads(g_threads) schedule(static)
+        for (int i = 0; i < N; i++) {
+            x[i] += alpha * p[i];
+            r[i] -= alpha * Ap[i];
+            sv[i] = r[i] / dg[i];
+        }
         double res_old = res_new;
         res_new = plane_sum_dot(s, r, sv);
         if (rel) {
@@ -774,6 +784,7 @@ static void pcg(fso_state* s) {
         } else if (res_new < sqr_tol)
             break;
         double beta = res_new / res_old;
+#pragma omp parallel for num_threads(g_threads) schedule(static)
         for (int i = 0; i < N; i++) {
             p[i] *= beta;
             p[i] += sv[i];
