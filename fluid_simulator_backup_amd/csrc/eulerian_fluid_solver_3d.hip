@@ -56,6 +56,9 @@ class EulerianFluidSolver3D {
     // one spare per component: each keeps its component's extents, so ghost
     // cells of every velocity buffer stay zero across the swaps
     double *spare_u = nullptr, *spare_v = nullptr, *spare_w = nullptr;
+    // velocity at the start of a Runge-Kutta step (orders 2 and 3), the
+    // reference's water_velocity_field_*_rk
+    double *u_rk = nullptr, *v_rk = nullptr, *w_rk = nullptr;
     double *gu = nullptr, *gv = nullptr, *gw = nullptr, *p = nullptr, *div = nullptr, *rho = nullptr,
            *irho = nullptr;
     double *sgn = nullptr, *tmp = nullptr, *bx = nullptr, *by = nullptr, *bz = nullptr, *diag = nullptr,
@@ -104,8 +107,8 @@ class EulerianFluidSolver3D {
         if ((long)G.nx * G.ny * G.nz > (1L << 31)) throw FsFail{FS_ERR_BAD_ARG, "grid too large"};
         if (P.surface_tension != 0.0)
             throw FsFail{FS_ERR_UNSUPPORTED, "surface_tension != 0 (jump terms) is outside this build"};
-        if (P.order_for_time_advancing != 1)
-            throw FsFail{FS_ERR_UNSUPPORTED, "only order_for_time_advancing = 1 is implemented"};
+        if (P.order_for_time_advancing < 1 || P.order_for_time_advancing > 3)
+            throw FsFail{FS_ERR_BAD_ARG, "order_for_time_advancing must be 1, 2 or 3"};
         if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE) throw FsFail{FS_ERR_BAD_ARG, "precision"};
         if (P.scaling_number_for_reinitialization <= 0) throw FsFail{FS_ERR_BAD_ARG, "scaling_number"};
         debug = std::getenv("FS_DEBUG") != nullptr;
@@ -135,6 +138,11 @@ class EulerianFluidSolver3D {
         double** f64[] = {&phi, &sghost, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
                           &bx, &by, &bz, &diag, &rhs};
         for (double** f : f64) *f = dfield();
+        if (P.order_for_time_advancing > 1) {
+            u_rk = dfield();
+            v_rk = dfield();
+            w_rk = dfield();
+        }
         const size_t npart = sizeof(double) * FS_PCG_SLOTS * FS_PCG_MAX_BLOCKS;
         w64.r = dfield();
         w64.s = dfield();
@@ -168,7 +176,7 @@ class EulerianFluidSolver3D {
     ~EulerianFluidSolver3D() {
         if (st) (void)hipStreamSynchronize(st);
         double* f64[] = {phi, sghost, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
-                         scratch, staging, w64.partials, w32.partials};
+                         scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk};
         for (double* f : f64)
             if (f) (void)hipFree(f);
         void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32,
@@ -450,8 +458,7 @@ class EulerianFluidSolver3D {
     void SolvePressure() {
         FsPcgOp o = op64();
         if (P.precision == FS_FP64) {
-            stats.pcg_iterations = run_pcg64(P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, p);
-            stats.refinement_sweeps = 0;
+            stats.pcg_iterations += run_pcg64(P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, p);
             bool converged = hS->done == 1;
             fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32 ? r32 : (float*)w64.q, w64, S64);
             read_state(S64);
@@ -465,8 +472,8 @@ class EulerianFluidSolver3D {
         fs_make_op32(st, L, bx, by, bz, diag, bx32, by32, bz32, diag32);
         double rr = 0, bb = 0;
         int updates = 0;
-        stats.pcg_iterations = run_pcg32_reliable(rr, bb, updates);
-        stats.refinement_sweeps = updates;
+        stats.pcg_iterations += run_pcg32_reliable(rr, bb, updates);
+        stats.refinement_sweeps += updates;
         stats.residual_rel = bb > 0 ? std::sqrt(rr / bb) : 0;
         stats.residual_precond = 0;
         double rt2 = P.rel_tolerance * P.rel_tolerance;
@@ -508,7 +515,7 @@ class EulerianFluidSolver3D {
         sync();
         float ms = 0;
         HIPCHECK(hipEventElapsedTime(&ms, ev[2], ev[3]));
-        stats.projection_ms = ms;
+        stats.projection_ms += ms;
         for (int c = 0; c < 3; c++)
             if (!std::isfinite(stats.max_velocity[c])) throw FsFail{FS_ERR_NONFINITE, "non-finite velocity after projection"};
         return rc;
@@ -527,24 +534,70 @@ class EulerianFluidSolver3D {
         }
     }
 
-    // EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread, order 1, :806-858
-    int AdvanceOneTimeStepThread(double dt) {
-        int rc = FS_OK;
-        if (P.is_levelset_advection_active) {
-            fs_copy_box(st, tmp, phi, L, box(0));
-            Levelset_Advection(dt);
-            Levelset_Advection(dt);
-            fs_average(st, phi, tmp, L);
-        }
-        // SolveForRK, :1453-1476
+    // EULERIAN_FLUID_SOLVER_3D::SolveForRK, :1453-1476; a non-converged
+    // projection is remembered and the step goes on, as the reference does.
+    void SolveForRK(double dt, int& rc) {
         if (P.is_velocity_advection_active) Velocity_Advection(dt);
         if (P.is_external_force_active) ApplyGravity(dt);
         if (P.is_sourcing_active) Sourcing(dt);
-        if (P.is_projection_active) rc = Projection(dt);
+        if (P.is_projection_active) {
+            int r = Projection(dt);
+            if (r != FS_OK) rc = r;
+        }
+    }
+
+    // velocity = ca*velocity_rk + cb*velocity on every MAC component
+    void BlendVelocity(double ca, double cb) {
+        fs_combine(st, u, u_rk, ca, cb, L, box(1));
+        fs_combine(st, v, v_rk, ca, cb, L, box(2));
+        fs_combine(st, w, w_rk, ca, cb, L, box(3));
+    }
+    void SaveVelocity() {
+        fs_copy_box(st, u_rk, u, L, box(1));
+        fs_copy_box(st, v_rk, v, L, box(2));
+        fs_copy_box(st, w_rk, w, L, box(3));
+    }
+
+    // EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread, :803-1111.  tmp
+    // plays temp_for_levelset.
+    int AdvanceOneTimeStepThread(double dt) {
+        int rc = FS_OK;
+        const int order = P.order_for_time_advancing;
+        if (P.is_levelset_advection_active) {
+            if (order == 3) {
+                Levelset_Advection(dt);  // :1010-1013, one Euler stage
+            } else {
+                fs_copy_box(st, tmp, phi, L, box(0));  // :810-832, :888-911
+                Levelset_Advection(dt);
+                Levelset_Advection(dt);
+                fs_average(st, phi, tmp, L);
+            }
+        }
+        if (order == 1) {
+            SolveForRK(dt, rc);  // :834
+        } else {
+            SaveVelocity();  // :913-932, :1015-1034
+            SolveForRK(dt, rc);
+            SolveForRK(dt, rc);
+            if (order == 2) {
+                BlendVelocity((double)0.5, (double)0.5);  // :937-953
+            } else {
+                BlendVelocity((double)0.75, (double)0.25);  // :1039-1055
+                SolveForRK(dt, rc);
+                BlendVelocity((double)1 / (double)3, (double)2 / (double)3);  // :1059-1075
+            }
+        }
+        // reinitialization, RK2 (:836-858, :958-980) or RK3 (:1080-1109)
         fs_copy_box(st, tmp, phi, L, box(0));
         Reinitialization();
         Reinitialization();
-        fs_average(st, phi, tmp, L);
+        if (order == 3) {
+            fs_combine(st, phi, tmp, (double)0.75, (double)0.25, L, box(0));
+            Reinitialization();
+            fs_combine(st, phi, tmp, (double)1 / (double)3, (double)2 / (double)3, L, box(0));
+        } else {
+            fs_average(st, phi, tmp, L);
+        }
         check_launch();
         return rc;
     }

@@ -636,6 +636,24 @@ void fs_average(hipStream_t st, double* phi, const double* tmp, const FsLayout& 
     hipLaunchKernelGGL(k_average, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, tmp, L);
 }
 
+// dst = ca*a + cb*dst over a box: the Runge-Kutta stage blends of
+// EULERIAN_FLUID_SOLVER_3D.h:939-951 (0.5/0.5), :1041-1053 (0.75/0.25),
+// :1061-1073 ((1/3)/(2/3)), :1099 and :1107, each product rounded before
+// the sum as written there.
+__global__ void k_combine(double* __restrict__ dst, const double* __restrict__ a, double ca, double cb, FsLayout L,
+                          FsBox b) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    long c = L.idx(i, j, k);
+    double x = ca * a[c];
+    double y = cb * dst[c];
+    dst[c] = x + y;
+}
+
+void fs_combine(hipStream_t st, double* dst, const double* a, double ca, double cb, const FsLayout& L, FsBox b) {
+    hipLaunchKernelGGL(k_combine, grid_for(b), dim3(BX, BY), 0, st, dst, a, ca, cb, L, b);
+}
+
 __global__ void k_scatter(double* __restrict__ dst, const double* __restrict__ lin, FsLayout L, FsBox b) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= b.ni || j >= b.nj) return;

@@ -3,7 +3,7 @@
  *
  * A plain-C, FP64, sequential-order restatement of the reference's per-step
  * hot path in air/water mode (EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread,
- * EULERIAN_FLUID_SOLVER_3D.h:803-1111, order_for_time_advancing = 1).  Each
+ * EULERIAN_FLUID_SOLVER_3D.h:803-1111, order_for_time_advancing 1, 2, 3).  Each
  * function cites the reference lines it restates.  Operation order inside every
  * expression follows the reference text (no FMA contraction: build with
  * -ffp-contract=off) so that the HIP kernels, which keep the same order, can be
@@ -58,6 +58,7 @@ struct fso_state {
     long sx, sy, sz, ntot;
     gc_t cell, mx, my, mz;
     double *phi, *sghost, *u, *v, *w, *gu, *gv, *gw, *p, *div, *rho, *irho, *sgn, *tmp, *pg, *vtmp;
+    double *urk, *vrk, *wrk; /* water_velocity_field_*_rk (orders 2 and 3) */
     double *fbx, *fby, *fbz; /* face betas of the last assembly (inspection only) */
     int* bc;
     /* CSR of the last assembly (POISSON_SOLVER_3D.h:1036-1038) */
@@ -91,7 +92,8 @@ static double* zalloc(long n) { return (double*)calloc((size_t)n, sizeof(double)
 
 int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     if (!P || !g || !out || g->nx < 3 || g->ny < 3 || g->nz < 3) return FS_ERR_BAD_ARG;
-    if (P->surface_tension != 0.0 || P->order_for_time_advancing != 1) return FS_ERR_UNSUPPORTED;
+    if (P->order_for_time_advancing < 1 || P->order_for_time_advancing > 3) return FS_ERR_BAD_ARG;
+    if (P->surface_tension != 0.0) return FS_ERR_UNSUPPORTED;
     fso_state* s = (fso_state*)calloc(1, sizeof(fso_state));
     s->P = *P;
     s->gr = *g;
@@ -118,6 +120,7 @@ int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     s->gu = zalloc(n); s->gv = zalloc(n); s->gw = zalloc(n); s->p = zalloc(n); s->div = zalloc(n);
     s->rho = zalloc(n); s->irho = zalloc(n); s->sgn = zalloc(n); s->tmp = zalloc(n); s->pg = zalloc(n);
     s->vtmp = zalloc(n);
+    s->urk = zalloc(n); s->vrk = zalloc(n); s->wrk = zalloc(n);
     s->fbx = zalloc(n); s->fby = zalloc(n); s->fbz = zalloc(n);
     s->bc = (int*)calloc((size_t)n, sizeof(int));
     s->plane_part = (double*)calloc((size_t)s->nz, sizeof(double));
@@ -137,6 +140,7 @@ void fso_destroy(fso_state* s) {
     free(s->phi); free(s->sghost); free(s->u); free(s->v); free(s->w); free(s->gu); free(s->gv); free(s->gw);
     free(s->p); free(s->div); free(s->rho); free(s->irho); free(s->sgn); free(s->tmp); free(s->pg); free(s->vtmp); free(s->fbx); free(s->fby); free(s->fbz);
     free(s->bc);
+    free(s->urk); free(s->vrk); free(s->wrk);
     free(s->plane_part);
     free_csr(s);
     free(s);
@@ -1026,26 +1030,87 @@ static void average_phi(fso_state* s) {
                 A3(s->phi, i, j, k) = (double)0.5 * (A3(s->tmp, i, j, k) + A3(s->phi, i, j, k));
 }
 
-/* EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread, order 1, :806-858;
- * SolveForRK :1453-1476. */
-int fso_step(fso_state* s, double dt, fs_step_stats* st) {
-    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+/* dst = ca*a + cb*dst over an ni x nj x nk interior, each product rounded
+ * before the sum: the stage blends of EULERIAN_FLUID_SOLVER_3D.h:939-951,
+ * :1041-1053, :1061-1073, :1099, :1107. */
+static void blend(fso_state* s, double* dst, const double* a, double ca, double cb, int ni, int nj, int nk) {
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                double x = ca * A3(a, i, j, k);
+                double y = cb * A3(dst, i, j, k);
+                A3(dst, i, j, k) = x + y;
+            }
+}
+static void blend_velocity(fso_state* s, double ca, double cb) {
+    blend(s, s->u, s->urk, ca, cb, s->nx + 1, s->ny, s->nz);
+    blend(s, s->v, s->vrk, ca, cb, s->nx, s->ny + 1, s->nz);
+    blend(s, s->w, s->wrk, ca, cb, s->nx, s->ny, s->nz + 1);
+}
+/* the water_velocity_field_*_rk copies, :913-932 / :1015-1034 */
+static void save_velocity(fso_state* s) {
+    copy_interior(s, s->urk, s->u, s->nx + 1, s->ny, s->nz);
+    copy_interior(s, s->vrk, s->v, s->nx, s->ny + 1, s->nz);
+    copy_interior(s, s->wrk, s->w, s->nx, s->ny, s->nz + 1);
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::SolveForRK, :1453-1476.  Statistics: PCG
+ * iterations add up over the step's projections, the rest is the last. */
+static void solve_for_rk(fso_state* s, double dt, fs_step_stats* st, int* rc) {
     const fs_params* P = &s->P;
-    int rc = FS_OK;
-    if (P->is_levelset_advection_active) {
-        save_phi(s);
-        fso_levelset_advection(s, dt);
-        fso_levelset_advection(s, dt);
-        average_phi(s);
-    }
     if (P->is_velocity_advection_active) fso_velocity_advection(s, dt);
     if (P->is_external_force_active) fso_apply_gravity(s, dt);
     if (P->is_sourcing_active) fso_sourcing(s, dt);
-    if (P->is_projection_active) rc = fso_projection(s, dt, st);
+    if (P->is_projection_active) {
+        int its = st ? st->pcg_iterations : 0;
+        int r = fso_projection(s, dt, st);
+        if (st) st->pcg_iterations += its;
+        if (r != FS_OK) *rc = r;
+    }
+}
+
+/* EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread, :803-1111: order 1
+ * (:806-858), order 2 (:886-980), order 3 (:1008-1109). */
+int fso_step(fso_state* s, double dt, fs_step_stats* st) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    const fs_params* P = &s->P;
+    const int order = P->order_for_time_advancing;
+    int rc = FS_OK;
+    if (st) memset(st, 0, sizeof(*st));
+    if (P->is_levelset_advection_active) {
+        if (order == 3) {
+            fso_levelset_advection(s, dt);
+        } else {
+            save_phi(s);
+            fso_levelset_advection(s, dt);
+            fso_levelset_advection(s, dt);
+            average_phi(s);
+        }
+    }
+    if (order == 1) {
+        solve_for_rk(s, dt, st, &rc);
+    } else {
+        save_velocity(s);
+        solve_for_rk(s, dt, st, &rc);
+        solve_for_rk(s, dt, st, &rc);
+        if (order == 2) {
+            blend_velocity(s, (double)0.5, (double)0.5);
+        } else {
+            blend_velocity(s, (double)0.75, (double)0.25);
+            solve_for_rk(s, dt, st, &rc);
+            blend_velocity(s, (double)1 / (double)3, (double)2 / (double)3);
+        }
+    }
     save_phi(s);
     fso_reinitialization(s, dt);
     fso_reinitialization(s, dt);
-    average_phi(s);
+    if (order == 3) {
+        blend(s, s->phi, s->tmp, (double)0.75, (double)0.25, s->nx, s->ny, s->nz);
+        fso_reinitialization(s, dt);
+        blend(s, s->phi, s->tmp, (double)1 / (double)3, (double)2 / (double)3, s->nx, s->ny, s->nz);
+    } else {
+        average_phi(s);
+    }
     return rc;
 }
 

@@ -123,6 +123,17 @@ def test_frames_fp64_tight(hip_lib, oracle_lib, name, n, frames):
         assert e < 1e-8, (c, e)
 
 
+@pytest.mark.parametrize("name,order", [("dam_break", 2), ("dam_break", 3), ("rising_drop", 3)])
+def test_frames_runge_kutta_orders(hip_lib, oracle_lib, name, order):
+    """Orders 2 and 3 of the step schedule (two / three projections per
+    step, stage blends, RK reinitialization) agree to 1e-8 after 2 frames
+    when each solve reaches 1e-11 relative residual."""
+    its, errs = _frames(hip_lib, oracle_lib, name, 20, 2, order_for_time_advancing=order,
+                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11)
+    for c, e in errs.items():
+        assert e < 1e-8, (c, e)
+
+
 @pytest.mark.parametrize("name,n", [("rising_drop", 32), ("dam_break", 24), ("bubbles", 32)])
 def test_fp32_refined_projection_matches_fp64(hip_lib, oracle_lib, name, n):
     """FP32 PCG with FP64 refinement reaches 1e-8 relative residual (checked

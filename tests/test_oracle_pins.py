@@ -13,6 +13,9 @@ wrong sign, swapped operand, wrong stencil index) would break:
     one phase, the ghost-fluid effective density across an interface, a
     dense direct solve, and a divergence-free projection;
   - viscosity: the exact discrete diffusion of a shear wave;
+  - time advancing (orders 1-3): one step multiplies a shear wave by the
+    Taylor polynomial of exp(z) of the order's degree; constant forcing
+    advances exactly;
   - gravity, CFL: closed forms.
 """
 import ctypes as C
@@ -257,6 +260,52 @@ def test_viscosity_shear_wave_is_discrete_diffusion(lib):
     expect = u + dt * mu / rho * lap
     inner = (slice(1, -1), slice(1, -1), slice(1, -1))
     assert np.abs(out[inner] - expect[inner]).max() < 1e-13
+
+
+@pytest.mark.parametrize("order", [1, 2, 3])
+def test_time_advancing_is_runge_kutta_polynomial(lib, order):
+    """With viscosity the only active phase, a shear wave u = sin(2 pi y)
+    is an eigenvector of each Euler stage, u <- (1 + z) u with
+    z = dt nu lambda, lambda the discrete Laplacian's eigenvalue.  One step
+    of order 1, 2, 3 must then multiply u by the Taylor polynomial of
+    exp(z) of that degree (Forward Euler, Heun, Shu-Osher TVD-RK3): a missing
+    stage or a wrong blend weight changes the factor."""
+    n = 16
+    mu, rho = 0.05, 2.0  # dt*nu*12/h^2 = 1.5: inside explicit stability
+    s, phi, _ = _solver(lib, "rising_drop", n, air_density=rho, water_density=rho, air_viscosity=mu,
+                        water_viscosity=mu, order_for_time_advancing=order, is_levelset_advection_active=0,
+                        is_velocity_advection_active=0, is_external_force_active=0, is_projection_active=0,
+                        sussmanstyle_reinitialization=0)
+    lib.fso_set_viscosity_inplace(s.h, 0)
+    h = 1.0 / n
+    yc = (np.arange(n) + 0.5) * h
+    u = np.broadcast_to(np.sin(4 * np.pi * yc)[None, :, None], (n, n, n + 1)).copy()
+    s.set_fields(phi, u, np.zeros((n, n + 1, n)), np.zeros((n + 1, n, n)))
+    dt = 2e-2
+    s.step(dt)
+    out = s.get_fields()["u"]
+    lam = (2 * np.cos(4 * np.pi * h) - 2) / (h * h)
+    z = dt * mu / rho * lam
+    R = {1: 1 + z, 2: 1 + z + z * z / 2, 3: 1 + z + z * z / 2 + z ** 3 / 6}[order]
+    inner = (slice(4, -4), slice(4, -4), slice(4, -4))  # walls reach one cell per stage
+    assert np.abs(out[inner] - R * u[inner]).max() < 1e-13
+    # the neighbouring polynomials differ from R by far more than rounding
+    assert all(abs(R - Q) > 1e-6 for o, Q in {1: 1 + z, 2: 1 + z + z * z / 2,
+                                              3: 1 + z + z * z / 2 + z ** 3 / 6}.items() if o != order)
+
+
+@pytest.mark.parametrize("order", [2, 3])
+def test_time_advancing_constant_force_is_exact(lib, order):
+    """Gravity alone: every order advances u by exactly dt*g (the blends are
+    consistent), within the rounding of the stage sums."""
+    s, phi, vel = _solver(lib, "rising_drop", 8, order_for_time_advancing=order, is_levelset_advection_active=0,
+                          is_velocity_advection_active=0, is_sourcing_active=0, is_projection_active=0,
+                          sussmanstyle_reinitialization=0, gravity=(0.5, -1.0, -9.8))
+    s.set_fields(phi, *vel)
+    s.step(0.125)
+    f = s.get_fields()
+    for c, g in zip("uvw", (0.5, -1.0, -9.8)):
+        assert np.abs(f[c] - (vel["uvw".index(c)] + 0.125 * g)).max() < 1e-14, c
 
 
 def test_gravity_every_face(lib):
