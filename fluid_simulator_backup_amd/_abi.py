@@ -70,13 +70,15 @@ class FsParams(C.Structure):
         ("is_external_force_active", _I),
         ("is_sourcing_active", _I),
         ("is_projection_active", _I),
+        ("semi_implicit_viscosity", _I),
     ]
 
 
 class FsStepStats(C.Structure):
     _fields_ = [("pcg_iterations", _I), ("refinement_sweeps", _I),
                 ("residual_precond", _D), ("residual_rel", _D),
-                ("max_velocity", _D * 3), ("projection_ms", _D), ("step_ms", _D)]
+                ("max_velocity", _D * 3), ("projection_ms", _D), ("step_ms", _D),
+                ("viscosity_iterations", _I)]
 
     def as_dict(self):
         return {"pcg_iterations": self.pcg_iterations,
@@ -85,7 +87,8 @@ class FsStepStats(C.Structure):
                 "residual_rel": self.residual_rel,
                 "max_velocity": list(self.max_velocity),
                 "projection_ms": self.projection_ms,
-                "step_ms": self.step_ms}
+                "step_ms": self.step_ms,
+                "viscosity_iterations": self.viscosity_iterations}
 
 
 def default_params(**kw):
@@ -117,6 +120,7 @@ def default_params(**kw):
     p.is_external_force_active = 1
     p.is_sourcing_active = 1
     p.is_projection_active = 1
+    p.semi_implicit_viscosity = 0
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):

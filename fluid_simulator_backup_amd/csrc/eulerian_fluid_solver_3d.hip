@@ -59,6 +59,11 @@ class EulerianFluidSolver3D {
     // velocity at the start of a Runge-Kutta step (orders 2 and 3), the
     // reference's water_velocity_field_*_rk
     double *u_rk = nullptr, *v_rk = nullptr, *w_rk = nullptr;
+    // semi-implicit viscosity rows (entries to i-1, i+1, j-1, j+1, k-1, k+1,
+    // diagonal, right-hand side) and its PCG vectors
+    double* va[6] = {};
+    double *vdiag = nullptr, *vrhs = nullptr;
+    FsPcgWork wv{};
     double *gu = nullptr, *gv = nullptr, *gw = nullptr, *p = nullptr, *div = nullptr, *rho = nullptr,
            *irho = nullptr;
     double *sgn = nullptr, *tmp = nullptr, *bx = nullptr, *by = nullptr, *bz = nullptr, *diag = nullptr,
@@ -162,6 +167,16 @@ class EulerianFluidSolver3D {
             w32.q = ffield();
             w32.partials = (double*)dalloc(npart);
         }
+        if (P.semi_implicit_viscosity) {
+            for (auto& a : va) a = dfield();
+            vdiag = dfield();
+            vrhs = dfield();
+            wv.r = dfield();
+            wv.s = dfield();
+            wv.p = dfield();
+            wv.q = dfield();
+            wv.partials = (double*)dalloc(npart);
+        }
         S64 = (FsPcgState*)dalloc(sizeof(FsPcgState));
         S32 = (FsPcgState*)dalloc(sizeof(FsPcgState));
         HIPCHECK(hipHostMalloc((void**)&hS, sizeof(FsPcgState)));
@@ -176,11 +191,12 @@ class EulerianFluidSolver3D {
     ~EulerianFluidSolver3D() {
         if (st) (void)hipStreamSynchronize(st);
         double* f64[] = {phi, sghost, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
-                         scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk};
+                         scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk,
+                         va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials};
         for (double* f : f64)
             if (f) (void)hipFree(f);
         void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32,
-                      S64, S32, vmax};
+                      S64, S32, vmax, wv.r, wv.s, wv.p, wv.q};
         for (void* f : fv)
             if (f) (void)hipFree(f);
         if (hS) (void)hipHostFree(hS);
@@ -264,14 +280,69 @@ class EulerianFluidSolver3D {
 
     // Sourcing -> VISCOSITY_3D::ApplyViscosity (air/water explicit), each
     // component out of place into its spare buffer, then swapped in.
-    void Sourcing(double dt) {
-        if (!P.use_delta_function_formulation) return;
+    int Sourcing(double dt) {
+        if (!P.use_delta_function_formulation) return FS_OK;
+        if (P.semi_implicit_viscosity) return SemiImplicitViscosity(dt);
         fs_viscosity(st, 1, spare_u, u, v, w, phi, L, mat, cell, dt);
         std::swap(u, spare_u);
         fs_viscosity(st, 2, spare_v, u, v, w, phi, L, mat, cell, dt);
         std::swap(v, spare_v);
         fs_viscosity(st, 3, spare_w, u, v, w, phi, L, mat, cell, dt);
         std::swap(w, spare_w);
+        return FS_OK;
+    }
+
+    // VISCOSITY_3D::ApplyViscosity, semi-implicit branch (VISCOSITY_3D.h:853-989):
+    // the three systems are built from the velocities at entry (the spares
+    // hold them) and solved in place one after the other with Jacobi-PCG
+    // (POISSON_SOLVER_3D::SolveForViscosity, :254-266), warm-started from the
+    // current component.  The unknowns are the component's faces, so the PCG
+    // runs on a layout whose extents are that face box.
+    int SemiImplicitViscosity(double dt) {
+        fs_copy_box(st, spare_u, u, L, box(1));
+        fs_copy_box(st, spare_v, v, L, box(2));
+        fs_copy_box(st, spare_w, w, L, box(3));
+        int rc = FS_OK;
+        double* comps[3] = {u, v, w};
+        for (int comp = 1; comp <= 3; comp++) {
+            double* x = comps[comp - 1];
+            fs_visc_system(st, comp, va, vdiag, vrhs, spare_u, spare_v, spare_w, x, phi, L, mat, cell, dt);
+            FsBox b = box(comp);
+            FsLayout Lf = L;
+            Lf.nx = b.ni;
+            Lf.ny = b.nj;
+            Lf.nz = b.nk;
+            // vectors of the previous component's box must not leave values
+            // in this box's zero ghost ring
+            for (void* f : {wv.r, wv.s, wv.p, wv.q})
+                HIPCHECK(hipMemsetAsync(f, 0, sizeof(double) * (size_t)L.total, st));
+            FsPcgOp o{};
+            o.kind = FS_OP_ROWS;
+            for (int e = 0; e < 6; e++) o.a[e] = va[e];
+            o.dg = vdiag;
+            int its = run_pcg_rows(Lf, o, x);
+            stats.viscosity_iterations += its;
+            if (hS->done != 1) {
+                rc = FS_ERR_NOT_CONVERGED;
+                last_error = "semi-implicit viscosity PCG reached max_iteration";
+            }
+        }
+        return rc;
+    }
+
+    // Jacobi-PCG on an FS_OP_ROWS system with the PROJECTION_WATER stop rule.
+    int run_pcg_rows(const FsLayout& Lf, const FsPcgOp& o, double* x) {
+        init_state(S64, P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, false);
+        fs_pcg64_begin(st, Lf, o, vrhs, vdiag, x, wv, S64);
+        read_state(S64);
+        int n = 0;
+        while (!hS->done) {
+            fs_pcg64_iterations(st, Lf, o, vdiag, x, wv, S64, n, chunk);
+            n += chunk;
+            check_launch();
+            read_state(S64);
+        }
+        return hS->iter;
     }
 
     // PROJECTION_3D::Solve up to the linear system (:416-434, :596-834 and
@@ -301,7 +372,7 @@ class EulerianFluidSolver3D {
     }
 
     FsPcgOp op64() const {
-        FsPcgOp o;
+        FsPcgOp o{};
         o.bx = bx;
         o.by = by;
         o.bz = bz;
@@ -539,7 +610,10 @@ class EulerianFluidSolver3D {
     void SolveForRK(double dt, int& rc) {
         if (P.is_velocity_advection_active) Velocity_Advection(dt);
         if (P.is_external_force_active) ApplyGravity(dt);
-        if (P.is_sourcing_active) Sourcing(dt);
+        if (P.is_sourcing_active) {
+            int r = Sourcing(dt);
+            if (r != FS_OK) rc = r;
+        }
         if (P.is_projection_active) {
             int r = Projection(dt);
             if (r != FS_OK) rc = r;
@@ -625,6 +699,7 @@ class EulerianFluidSolver3D {
         double s_f = 0;
         double a = P.water_viscosity / P.water_density, b = P.air_viscosity / P.air_density;
         double v_f = (a > b ? a : b) * ((double)2 * cell.x.inv_d2 + (double)2 * cell.y.inv_d2 + (double)2 * cell.z.inv_d2);
+        if (P.semi_implicit_viscosity) v_f = 0;  // :2770-2773
         double cfl = (double)0.5 * (c_f + v_f + std::sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) +
                                                           (double)4 * (s_f * s_f)));
         return (double)1 / cfl;
@@ -755,7 +830,17 @@ int fs_compute_curvatures(fs_solver* s) {
 PHASE(fs_levelset_advection, S.Levelset_Advection(dt))
 PHASE(fs_velocity_advection, S.Velocity_Advection(dt))
 PHASE(fs_apply_gravity, S.ApplyGravity(dt))
-PHASE(fs_sourcing, S.Sourcing(dt))
+
+int fs_sourcing(fs_solver* s, double dt) {
+    if (!(dt > 0)) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.stats = fs_step_stats{};
+        int rc = S.Sourcing(dt);
+        S.check_launch();
+        S.sync();
+        return rc;
+    });
+}
 PHASE(fs_reinitialization, S.Reinitialization())
 
 int fs_projection(fs_solver* s, double dt, fs_step_stats* stats) {

@@ -31,6 +31,11 @@ void fs_viscosity(hipStream_t st, int comp, double* dst, const double* U, const 
                   const double* phi, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt);
 
 // ---- projection -----------------------------------------------------------
+// Semi-implicit viscosity rows of MAC component comp (1..3): off-diagonal
+// entries a[0..5] (i-1, i+1, j-1, j+1, k-1, k+1), diagonal and right-hand side.
+void fs_visc_system(hipStream_t st, int comp, double* const a[6], double* diag, double* rhs, const double* U,
+                    const double* V, const double* W, const double* self, const double* phi, const FsLayout& L,
+                    const FsMaterial& m, const FsGridConst& cell, double dt);
 void fs_density(hipStream_t st, const double* phi, double* rho, double* irho, const FsLayout& L, double light,
                 double heavy);
 void fs_divergence(hipStream_t st, const double* gu, const double* gv, const double* gw, double* div,

@@ -320,6 +320,122 @@ void fs_viscosity(hipStream_t st, int comp, double* dst, const double* U, const 
                        cell.x.inv, cell.y.inv, cell.z.inv, dt);
 }
 
+// Semi-implicit viscosity system of one MAC component (VISCOSITY_3D.h:853-989
+// coefficients and explicit terms; POISSON_SOLVER_3D.h:3359-3496 rows), one
+// row per face of the component's box.  Unknown faces are the box; the ring
+// beyond it is Dirichlet with the zero ghost velocity
+// (SetupBoundaryConditionsForVelocity, regular_boundary, VISCOSITY_3D.h:1403-1417).
+//   c[0..5]: the coefficients to the i+1, i-1, j+1, j-1, k+1, k-1 faces
+//            (coef_1..coef_6), 2 dt mu/rho_half across the normal pair and
+//            dt mu/rho_half across the tangential ones;
+//   a[0..5]: matrix entries to i-1, i+1, j-1, j+1, k-1, k+1 (-c/d^2, 0 to a
+//            Dirichlet face), diag = 1 + inv_dxdx * (full coefs, then
+//            Dirichlet coefs, in that order; 1 if zero) as :3402-3468 sums it;
+//   rhs:     the explicit term plus the Dirichlet contributions (:3472-3495).
+// U, V, W are the velocities at entry (the reference reads ghost copies made
+// before the first solve); self is the live component (its ghosts, the
+// Dirichlet values, are zero).  The cross terms of x and z are those of
+// VISCOSITY_3D.h:887,975; the y term's z part follows the explicit update
+// (:522) where :930 differences w across x.
+__global__ void __launch_bounds__(BX* BY) k_visc_system(int comp, double* __restrict__ a0, double* __restrict__ a1,
+                                                        double* __restrict__ a2, double* __restrict__ a3,
+                                                        double* __restrict__ a4, double* __restrict__ a5,
+                                                        double* __restrict__ diag, double* __restrict__ rhs,
+                                                        const double* __restrict__ U, const double* __restrict__ V,
+                                                        const double* __restrict__ W, const double* __restrict__ self,
+                                                        const double* __restrict__ f, FsLayout L, FsBox b,
+                                                        FsMaterial m, double eps, double odx, double ody, double odz,
+                                                        double idxx, double idyy, double idzz, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    const long c = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+#define MU(x) m.viscosity(fs_heaviside_visc((x), eps))
+    double cf[6];  // coef_1 .. coef_6
+    double ex;
+    if (comp == 1) {
+        double lh = (double)0.5 * (f[c - 1] + f[c]);
+        double lcu = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 + sx] + f[c + sx]);
+        double lcd = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 - sx] + f[c - sx]);
+        double lct = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 + sy] + f[c + sy]);
+        double lcb = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 - sy] + f[c - sy]);
+        double rh = m.density(fs_heaviside_visc(lh, eps));
+        double mcu = MU(lcu), mcd = MU(lcd), mr = MU(f[c]), mc = MU(f[c - 1]), mct = MU(lct), mcb = MU(lcb);
+        cf[0] = (double)2 * dt / rh * mr;
+        cf[1] = (double)2 * dt / rh * mc;
+        cf[2] = dt / rh * mcu;
+        cf[3] = dt / rh * mcd;
+        cf[4] = dt / rh * mct;
+        cf[5] = dt / rh * mcb;
+        ex = U[c] + dt / rh * (ody * (mcu * odx * (V[c + sx] - V[c - 1 + sx]) - mcd * odx * (V[c] - V[c - 1]))) +
+             dt / rh * (odz * (mct * odx * (W[c + sy] - W[c - 1 + sy]) - mcb * odx * (W[c] - W[c - 1])));
+    } else if (comp == 2) {
+        double lh = (double)0.5 * (f[c - sx] + f[c]);
+        double lcu = (double)0.25 * (f[c - sx] + f[c + 1 - sx] + f[c] + f[c + 1]);
+        double lcl = (double)0.25 * (f[c - 1 - sx] + f[c - sx] + f[c - 1] + f[c]);
+        double lct = (double)0.25 * (f[c - sx + sy] + f[c - sx] + f[c + sy] + f[c]);
+        double lcb = (double)0.25 * (f[c - sx - sy] + f[c - sx] + f[c - sy] + f[c]);
+        double rh = m.density(fs_heaviside_visc(lh, eps));
+        double mcu = MU(lcu), mcl = MU(lcl), mu_ = MU(f[c]), mc = MU(f[c - sx]), mct = MU(lct), mcb = MU(lcb);
+        cf[0] = dt / rh * mcu;
+        cf[1] = dt / rh * mcl;
+        cf[2] = (double)2 * dt / rh * mu_;
+        cf[3] = (double)2 * dt / rh * mc;
+        cf[4] = dt / rh * mct;
+        cf[5] = dt / rh * mcb;
+        ex = V[c] + dt / rh * (odx * (mcu * ody * (U[c + 1] - U[c + 1 - sx]) - mcl * ody * (U[c] - U[c - sx]))) +
+             dt / rh * (odz * (mct * ody * (W[c] - W[c - sx]) - mcb * ody * (W[c - sy] - W[c - sx - sy])));
+    } else {
+        double lh = (double)0.5 * (f[c - sy] + f[c]);
+        double lct = (double)0.25 * (f[c - sy] + f[c + 1 - sy] + f[c] + f[c + 1]);
+        double lcb = (double)0.25 * (f[c - sy] + f[c - 1 - sy] + f[c] + f[c - 1]);
+        double lcu = (double)0.25 * (f[c - sy] + f[c + sx - sy] + f[c] + f[c + sx]);
+        double lcd = (double)0.25 * (f[c - sy] + f[c - sx - sy] + f[c] + f[c - sx]);
+        double rh = m.density(fs_heaviside_visc(lh, eps));
+        double mct = MU(lct), mcb = MU(lcb), mcu = MU(lcu), mcd = MU(lcd), mt = MU(f[c]), mc = MU(f[c - sy]);
+        cf[0] = dt / rh * mct;
+        cf[1] = dt / rh * mcb;
+        cf[2] = dt / rh * mcu;
+        cf[3] = dt / rh * mcd;
+        cf[4] = (double)2 * dt / rh * mt;
+        cf[5] = (double)2 * dt / rh * mc;
+        ex = self[c] + dt / rh * (odx * odz * (mct * (U[c] - U[c - sy]) - mcb * (U[c - 1] - U[c - 1 - sy]))) +
+             dt / rh * (ody * odz * (mcu * (V[c] - V[c - sy]) - mcd * (V[c - sx] - V[c - sx - sy])));
+    }
+#undef MU
+    // neighbours in assignment order i-1, i+1, j-1, j+1, k-1, k+1 with their
+    // coefficient, inverse square spacing and face offset
+    const double nc[6] = {cf[1], cf[0], cf[3], cf[2], cf[5], cf[4]};
+    const double ni[6] = {idxx, idxx, idyy, idyy, idzz, idzz};
+    const long no[6] = {-1, 1, -sx, sx, -sy, sy};
+    const bool full[6] = {i > 0, i < b.ni - 1, j > 0, j < b.nj - 1, k > 0, k < b.nk - 1};
+    double* const out[6] = {a0, a1, a2, a3, a4, a5};
+    double coef = 0;
+    for (int e = 0; e < 6; e++) {
+        if (full[e]) coef += nc[e];
+        out[e][c] = full[e] ? -nc[e] * ni[e] : (double)0;
+    }
+    for (int e = 0; e < 6; e++)
+        if (!full[e]) coef += nc[e];
+    if (coef == 0) coef = 1;
+    diag[c] = (double)1 + idxx * coef;
+    double bv = ex;
+    for (int e = 0; e < 6; e++)
+        if (!full[e]) bv += nc[e] * ni[e] * self[c + no[e]];
+    rhs[c] = bv;
+}
+
+void fs_visc_system(hipStream_t st, int comp, double* const a[6], double* diag, double* rhs, const double* U,
+                    const double* V, const double* W, const double* self, const double* phi, const FsLayout& L,
+                    const FsMaterial& m, const FsGridConst& cell, double dt) {
+    FsBox b = fs_box(L, comp);
+    double eps = 1.5 * cell.x.d;  // epsilon_for_mollification, EULERIAN_FLUID_SOLVER_3D.h:393
+    double idxx = (double)1 / (cell.x.d * cell.x.d), idyy = (double)1 / (cell.y.d * cell.y.d),
+           idzz = (double)1 / (cell.z.d * cell.z.d);
+    hipLaunchKernelGGL(k_visc_system, grid_for(b), dim3(BX, BY), 0, st, comp, a[0], a[1], a[2], a[3], a[4], a[5],
+                       diag, rhs, U, V, W, self, phi, L, b, m, eps, cell.x.inv, cell.y.inv, cell.z.inv, idxx, idyy,
+                       idzz, dt);
+}
+
 // ---------------------------------------------------------------------------
 // projection (PROJECTION_3D.h, POISSON_SOLVER_3D.h)
 // ---------------------------------------------------------------------------

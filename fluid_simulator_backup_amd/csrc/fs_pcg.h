@@ -21,11 +21,22 @@ struct FsPcgState {
     double tol2, rtol2;   // squared absolute / relative tolerances
 };
 
+// kind FS_OP_GFM: the pressure operator from face coefficients bx, by, bz
+// (POISSON_SOLVER_3D.h:1030-1213).  kind FS_OP_ROWS (FP64 only): rows with
+// explicit entries a[0..5] to the i-1, i+1, j-1, j+1, k-1, k+1 neighbours
+// (0 where the neighbour is not an unknown) and diagonal dg, summed in that
+// order as CSR_MATRIX::Multiply sums the rows the reference assigns in it
+// (the semi-implicit viscosity system, POISSON_SOLVER_3D.h:3359-3496).
+#define FS_OP_GFM 0
+#define FS_OP_ROWS 1
 struct FsPcgOp {
     const void* bx;
     const void* by;
     const void* bz;
     double idx2, idy2, idz2;  // 1/dx^2, 1/dy^2, 1/dz^2 (POISSON_SOLVER_3D.h:1076)
+    int kind;
+    const double* a[6];
+    const double* dg;
 };
 
 // Workspace of one precision.

@@ -444,13 +444,88 @@ __global__ void __launch_bounds__(NT) k_pupdate(FsLayout L, Tiling tl, T* __rest
     }
 }
 
+// ---- FS_OP_ROWS operator (FP64): explicit entries per row -----------------
+__device__ __forceinline__ double rows_val(const FsPcgOp& o, const double* __restrict__ x, long c, long sx,
+                                           long sy) {
+    double v = 0;
+    v += o.a[0][c] * x[c - 1];
+    v += o.a[1][c] * x[c + 1];
+    v += o.a[2][c] * x[c - sx];
+    v += o.a[3][c] * x[c + sx];
+    v += o.a[4][c] * x[c - sy];
+    v += o.a[5][c] * x[c + sy];
+    v += o.dg[c] * x[c];
+    return v;
+}
+
+// begin on FS_OP_ROWS: r = b - A x, s = r/D, p = s and the partials.  The
+// tiling (hence the partial count) is the one the update kernels use.
+template <int V>
+__global__ void __launch_bounds__(NT) k_rows_begin(FsLayout L, Tiling tl, FsPcgOp o, const double* __restrict__ b,
+                                                   const double* __restrict__ D, const double* __restrict__ x,
+                                                   double* __restrict__ r, double* __restrict__ s,
+                                                   double* __restrict__ p, double* part) {
+    __shared__ double lds[4];
+    double a_rs = 0, a_rr = 0, a_bb = 0;
+    for (TileIter it(tl); it.valid(); it.next()) {
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        int k1 = min(k0 + ZC, L.nz);
+        long c = L.idx(i0, j, k0);
+        for (int k = k0; k < k1; k++, c += L.sy)
+            for (int v = 0; v < V; v++) {
+                double bv = b[c + v];
+                double rv = bv - rows_val(o, x, c + v, L.sx, L.sy);
+                double sv = rv / D[c + v];
+                r[c + v] = rv;
+                s[c + v] = sv;
+                p[c + v] = sv;
+                a_rs += rv * sv;
+                a_rr += rv * rv;
+                a_bb += bv * bv;
+            }
+    }
+    put_partial(part, SLOT_RS, a_rs, lds);
+    put_partial(part, SLOT_RR, a_rr, lds);
+    put_partial(part, SLOT_BB, a_bb, lds);
+}
+
+// K1 on FS_OP_ROWS: q = A p, partial p.q.
+template <int V>
+__global__ void __launch_bounds__(NT) k_rows_spmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const double* __restrict__ p,
+                                                      double* __restrict__ q, double* part, const FsPcgState* S) {
+    if (S->done) return;
+    __shared__ double lds[4];
+    double acc = 0;
+    for (TileIter it(tl); it.valid(); it.next()) {
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        if (i0 >= L.nx || j >= L.ny) continue;
+        int k1 = min(k0 + ZC, L.nz);
+        long c = L.idx(i0, j, k0);
+        for (int k = k0; k < k1; k++, c += L.sy)
+            for (int v = 0; v < V; v++) {
+                double ap = rows_val(o, p, c + v, L.sx, L.sy);
+                q[c + v] = ap;
+                acc += p[c + v] * ap;
+            }
+    }
+    put_partial(part, SLOT_PQ, acc, lds);
+}
+
 // ---- launch helpers ---------------------------------------------------------
 template <typename T, int V>
 static void begin_v(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* rhs, const T* diag, T* x,
                     FsPcgWork& w, FsPcgState* S) {
     Tiling tl = make_tiling(L, V);
-    hipLaunchKernelGGL((k_begin<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, rhs, diag, x, (T*)w.r,
-                       (T*)w.s, (T*)w.p, w.partials, S);
+    if (op.kind == FS_OP_ROWS)
+        hipLaunchKernelGGL((k_rows_begin<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)rhs,
+                           (const double*)diag, (const double*)x, (double*)w.r, (double*)w.s, (double*)w.p,
+                           w.partials);
+    else
+        hipLaunchKernelGGL((k_begin<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, rhs, diag, x, (T*)w.r,
+                           (T*)w.s, (T*)w.p, w.partials, S);
     hipLaunchKernelGGL(k_begin_final, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S);
 }
 
@@ -458,7 +533,10 @@ template <typename T, int V>
 static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
                        FsPcgState* S, int which, int n) {
     Tiling tl = make_tiling(L, V);
-    if (which == 0)
+    if (which == 0 && op.kind == FS_OP_ROWS)
+        hipLaunchKernelGGL((k_rows_spmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)w.p,
+                           (double*)w.q, w.partials, S);
+    else if (which == 0)
         hipLaunchKernelGGL((k_spmv_dot<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const T*)w.p,
                            (T*)w.q, w.partials, S);
     else if (which == 1)

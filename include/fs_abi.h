@@ -73,7 +73,7 @@ typedef struct fs_params {
     double gravity[3];
     double surface_tension;                 /* must be 0 (jump terms vanish) */
     int    use_delta_function_formulation;  /* explicit viscosity on/off */
-    int    order_for_time_advancing;        /* 1 (2 and 3: FS_ERR_UNSUPPORTED) */
+    int    order_for_time_advancing;        /* 1 Euler, 2 RK2, 3 TVD-RK3 (EULERIAN_FLUID_SOLVER_3D.h:806-1109) */
     int    sussmanstyle_reinitialization;
     int    scaling_number_for_reinitialization;   /* fictitious dt = dx/scaling */
     int    iteration_number_for_reinitialization; /* sweeps per call */
@@ -94,17 +94,25 @@ typedef struct fs_params {
     int    is_external_force_active;
     int    is_sourcing_active;
     int    is_projection_active;
+    /* VISCOSITY_3D semi_implicit_approach (VISCOSITY_3D.h:227, :853-989):
+     * per MAC component solve (I + sum_f c_f/d^2) u = u* + cross terms with
+     * Dirichlet walls, Jacobi-PCG with the PROJECTION_WATER tolerance,
+     * max_iteration and stop rule; the CFL drops its viscous term
+     * (EULERIAN_FLUID_SOLVER_3D.h:2770-2773).  Needs
+     * use_delta_function_formulation like the explicit path. */
+    int    semi_implicit_viscosity;
 } fs_params;
 
 /* Per-step (or per-phase) statistics. */
 typedef struct fs_step_stats {
-    int    pcg_iterations;       /* total inner iterations of the last projection */
+    int    pcg_iterations;       /* inner PCG iterations, summed over the step's projections */
     int    refinement_sweeps;    /* outer FP64 sweeps (FS_FP32_REFINE), else 0 */
     double residual_precond;     /* sqrt(r^T D^-1 r) at exit, PCG_METHOD.cpp:111 */
     double residual_rel;         /* ||b - A x||_2 / ||b||_2 at exit (recomputed in FP64) */
     double max_velocity[3];      /* max |u|,|v|,|w| after the gradient update, PROJECTION_3D.h:1236,1345,1427 */
     double projection_ms;        /* device time of the last projection */
     double step_ms;              /* device time of the last fs_advance_frame/fs_step */
+    int    viscosity_iterations; /* PCG iterations of the semi-implicit viscosity solves */
 } fs_step_stats;
 
 typedef struct fs_solver fs_solver;

@@ -69,8 +69,18 @@ struct fso_state {
     int visc_inplace;
     double max_vel[3];
     int last_iters;
+    int visc_iters; /* semi-implicit viscosity PCG iterations since the step began */
     double last_res_abs, last_res_rel;
 };
+
+/* A CSR system with rows numbered plane by plane (per rows in each of
+ * nplanes planes) and its PCG work vectors. */
+typedef struct {
+    int N, per, nplanes;
+    const int *rp, *ci;
+    const double *val, *b;
+    double *x, *r, *zv, *pp, *Ap;
+} sys_t;
 
 static int g_threads = 1;
 void fso_set_threads(int n) { g_threads = n > 0 ? n : 1; }
@@ -123,7 +133,7 @@ int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     s->urk = zalloc(n); s->vrk = zalloc(n); s->wrk = zalloc(n);
     s->fbx = zalloc(n); s->fby = zalloc(n); s->fbz = zalloc(n);
     s->bc = (int*)calloc((size_t)n, sizeof(int));
-    s->plane_part = (double*)calloc((size_t)s->nz, sizeof(double));
+    s->plane_part = (double*)calloc((size_t)s->nz + 1, sizeof(double));
     s->visc_inplace = 1;
     *out = s;
     return FS_OK;
@@ -546,11 +556,190 @@ static double visc_face(fso_state* s, int comp, int i, int j, int k, double dt, 
 #undef MU
 }
 
+typedef struct {
+    int N, per, nplanes;
+    int *rp, *ci;
+    double *val, *b, *x, *r, *zv, *pp, *Ap;
+} vsys_t;
+
+static sys_t as_sys(vsys_t* V) {
+    sys_t S = {V->N, V->per, V->nplanes, V->rp, V->ci, V->val, V->b, V->x, V->r, V->zv, V->pp, V->Ap};
+    return S;
+}
+static void pcg_sys(fso_state* s, const sys_t* S);
+
+/* Rows of the semi-implicit viscosity system of component comp on its face
+ * box (ni, nj, nk), numbered i fastest:
+ *   coefficients and explicit term, VISCOSITY_3D.h:853-989 (air/water
+ *   density and viscosity mixes, :1258-1347), built from the velocities at
+ *   entry (ou, ov, ow: the reference reads ghost copies made before the first
+ *   solve); live = the component itself (x0 and the Dirichlet values);
+ *   rows, BuildLinearSystemNodeForSemiImplicitViscosity,
+ *   POISSON_SOLVER_3D.h:3359-3496, with every box face an unknown and the
+ *   ring beyond it Dirichlet (regular_boundary, VISCOSITY_3D.h:1403-1417).
+ * The cross terms of x and z are those of :887 and :975; the y term's z part
+ * is the explicit update's (:522), where :930 differences w across x. */
+static void visc_rows(fso_state* s, int comp, double dt, const double* ou, const double* ov, const double* ow,
+                      const double* live, vsys_t* V) {
+    const fs_params* P = &s->P;
+    const double* f = s->phi;
+    const double eps = (double)1.5 * s->cell.x.d;
+    const double odx = s->cell.x.inv, ody = s->cell.y.inv, odz = s->cell.z.inv;
+    const double idxx = (double)1 / (s->cell.x.d * s->cell.x.d), idyy = (double)1 / (s->cell.y.d * s->cell.y.d),
+                 idzz = (double)1 / (s->cell.z.d * s->cell.z.d);
+    int ni, nj, nk;
+    extents(s, comp, &ni, &nj, &nk);
+#define PH(a, b, c) A3(f, a, b, c)
+#define U(a, b, c) A3(ou, a, b, c)
+#define V_(a, b, c) A3(ov, a, b, c)
+#define W(a, b, c) A3(ow, a, b, c)
+#define MU(x) mix_viscosity(P, heaviside_visc((x), eps))
+    int nnz = 0;
+    V->rp[0] = 0;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                int row = i + ni * (j + nj * k);
+                double c1, c2, c3, c4, c5, c6, ex;
+                if (comp == 1) {
+                    double lh = (double)0.5 * (PH(i - 1, j, k) + PH(i, j, k));
+                    double lcu = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j + 1, k) + PH(i, j + 1, k));
+                    double lcd = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j - 1, k) + PH(i, j - 1, k));
+                    double lct = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j, k + 1) + PH(i, j, k + 1));
+                    double lcb = (double)0.25 * (PH(i - 1, j, k) + PH(i, j, k) + PH(i - 1, j, k - 1) + PH(i, j, k - 1));
+                    double rh = mix_density(P, heaviside_visc(lh, eps));
+                    double mcu = MU(lcu), mcd = MU(lcd), mr = MU(PH(i, j, k)), mc = MU(PH(i - 1, j, k)), mct = MU(lct),
+                           mcb = MU(lcb);
+                    c1 = (double)2 * dt / rh * mr;
+                    c2 = (double)2 * dt / rh * mc;
+                    c3 = dt / rh * mcu;
+                    c4 = dt / rh * mcd;
+                    c5 = dt / rh * mct;
+                    c6 = dt / rh * mcb;
+                    ex = U(i, j, k) +
+                         dt / rh * (ody * (mcu * odx * (V_(i, j + 1, k) - V_(i - 1, j + 1, k)) - mcd * odx * (V_(i, j, k) - V_(i - 1, j, k)))) +
+                         dt / rh * (odz * (mct * odx * (W(i, j, k + 1) - W(i - 1, j, k + 1)) - mcb * odx * (W(i, j, k) - W(i - 1, j, k))));
+                } else if (comp == 2) {
+                    double lh = (double)0.5 * (PH(i, j - 1, k) + PH(i, j, k));
+                    double lcu = (double)0.25 * (PH(i, j - 1, k) + PH(i + 1, j - 1, k) + PH(i, j, k) + PH(i + 1, j, k));
+                    double lcl = (double)0.25 * (PH(i - 1, j - 1, k) + PH(i, j - 1, k) + PH(i - 1, j, k) + PH(i, j, k));
+                    double lct = (double)0.25 * (PH(i, j - 1, k + 1) + PH(i, j - 1, k) + PH(i, j, k + 1) + PH(i, j, k));
+                    double lcb = (double)0.25 * (PH(i, j - 1, k - 1) + PH(i, j - 1, k) + PH(i, j, k - 1) + PH(i, j, k));
+                    double rh = mix_density(P, heaviside_visc(lh, eps));
+                    double mcu = MU(lcu), mcl = MU(lcl), mu_ = MU(PH(i, j, k)), mc = MU(PH(i, j - 1, k)), mct = MU(lct),
+                           mcb = MU(lcb);
+                    c1 = dt / rh * mcu;
+                    c2 = dt / rh * mcl;
+                    c3 = (double)2 * dt / rh * mu_;
+                    c4 = (double)2 * dt / rh * mc;
+                    c5 = dt / rh * mct;
+                    c6 = dt / rh * mcb;
+                    ex = V_(i, j, k) +
+                         dt / rh * (odx * (mcu * ody * (U(i + 1, j, k) - U(i + 1, j - 1, k)) - mcl * ody * (U(i, j, k) - U(i, j - 1, k)))) +
+                         dt / rh * (odz * (mct * ody * (W(i, j, k) - W(i, j - 1, k)) - mcb * ody * (W(i, j, k - 1) - W(i, j - 1, k - 1))));
+                } else {
+                    double lh = (double)0.5 * (PH(i, j, k - 1) + PH(i, j, k));
+                    double lct = (double)0.25 * (PH(i, j, k - 1) + PH(i + 1, j, k - 1) + PH(i, j, k) + PH(i + 1, j, k));
+                    double lcb = (double)0.25 * (PH(i, j, k - 1) + PH(i - 1, j, k - 1) + PH(i, j, k) + PH(i - 1, j, k));
+                    double lcu = (double)0.25 * (PH(i, j, k - 1) + PH(i, j + 1, k - 1) + PH(i, j, k) + PH(i, j + 1, k));
+                    double lcd = (double)0.25 * (PH(i, j, k - 1) + PH(i, j - 1, k - 1) + PH(i, j, k) + PH(i, j - 1, k));
+                    double rh = mix_density(P, heaviside_visc(lh, eps));
+                    double mct = MU(lct), mcb = MU(lcb), mcu = MU(lcu), mcd = MU(lcd), mt = MU(PH(i, j, k)),
+                           mc = MU(PH(i, j, k - 1));
+                    c1 = dt / rh * mct;
+                    c2 = dt / rh * mcb;
+                    c3 = dt / rh * mcu;
+                    c4 = dt / rh * mcd;
+                    c5 = (double)2 * dt / rh * mt;
+                    c6 = (double)2 * dt / rh * mc;
+                    ex = A3(live, i, j, k) +
+                         dt / rh * (odx * odz * (mct * (U(i, j, k) - U(i, j, k - 1)) - mcb * (U(i - 1, j, k) - U(i - 1, j, k - 1)))) +
+                         dt / rh * (ody * odz * (mcu * (V_(i, j, k) - V_(i, j, k - 1)) - mcd * (V_(i, j - 1, k) - V_(i, j - 1, k - 1))));
+                }
+                /* neighbours in the order the reference assigns them */
+                const double nc[6] = {c2, c1, c4, c3, c6, c5};
+                const double nv[6] = {idxx, idxx, idyy, idyy, idzz, idzz};
+                const int di[6] = {-1, 1, 0, 0, 0, 0}, dj[6] = {0, 0, -1, 1, 0, 0}, dk[6] = {0, 0, 0, 0, -1, 1};
+                double coef = 0;
+                for (int e = 0; e < 6; e++) {
+                    int a = i + di[e], bb = j + dj[e], cc = k + dk[e];
+                    if (a >= 0 && a < ni && bb >= 0 && bb < nj && cc >= 0 && cc < nk) {
+                        coef += nc[e];
+                        V->ci[nnz] = a + ni * (bb + nj * cc);
+                        V->val[nnz++] = -nc[e] * nv[e];
+                    }
+                }
+                for (int e = 0; e < 6; e++) {
+                    int a = i + di[e], bb = j + dj[e], cc = k + dk[e];
+                    if (!(a >= 0 && a < ni && bb >= 0 && bb < nj && cc >= 0 && cc < nk)) coef += nc[e];
+                }
+                if (coef == 0) coef = 1;
+                V->ci[nnz] = row;
+                V->val[nnz++] = (double)1 + idxx * coef;
+                V->rp[row + 1] = nnz;
+                double bv = ex;
+                for (int e = 0; e < 6; e++) {
+                    int a = i + di[e], bb = j + dj[e], cc = k + dk[e];
+                    if (!(a >= 0 && a < ni && bb >= 0 && bb < nj && cc >= 0 && cc < nk))
+                        bv += nc[e] * nv[e] * A3(live, a, bb, cc);
+                }
+                V->b[row] = bv;
+                V->x[row] = A3(live, i, j, k);
+            }
+#undef PH
+#undef U
+#undef V_
+#undef W
+#undef MU
+}
+
+/* VISCOSITY_3D::ApplyViscosity, semi-implicit branch (VISCOSITY_3D.h:853-989)
+ * in the air/water box: components x, y, z solved in turn
+ * (POISSON_SOLVER_3D::SolveForViscosity, :254-266) with the Jacobi-PCG and
+ * stop rule of PROJECTION_WATER; the systems read the velocities at entry. */
+static int semi_implicit_viscosity(fso_state* s, double dt, int* iterations) {
+    int rc = FS_OK;
+    long n = s->ntot;
+    double* old[3];
+    for (int c = 0; c < 3; c++) {
+        old[c] = zalloc(n);
+        memcpy(old[c], c == 0 ? s->u : (c == 1 ? s->v : s->w), sizeof(double) * (size_t)n);
+    }
+    double* comps[3] = {s->u, s->v, s->w};
+    for (int comp = 1; comp <= 3; comp++) {
+        int ni, nj, nk;
+        extents(s, comp, &ni, &nj, &nk);
+        vsys_t V;
+        V.N = ni * nj * nk;
+        V.per = ni * nj;
+        V.nplanes = nk;
+        V.rp = (int*)calloc((size_t)V.N + 1, sizeof(int));
+        V.ci = (int*)calloc((size_t)V.N * 7, sizeof(int));
+        V.val = (double*)calloc((size_t)V.N * 7, sizeof(double));
+        double** vecs[6] = {&V.b, &V.x, &V.r, &V.zv, &V.pp, &V.Ap};
+        for (int q = 0; q < 6; q++) *vecs[q] = (double*)calloc((size_t)V.N, sizeof(double));
+        visc_rows(s, comp, dt, old[0], old[1], old[2], comps[comp - 1], &V);
+        sys_t S = as_sys(&V);
+        pcg_sys(s, &S);
+        if (iterations) *iterations += s->last_iters;
+        if (s->last_iters >= s->P.max_iteration) rc = FS_ERR_NOT_CONVERGED;
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++) A3(comps[comp - 1], i, j, k) = V.x[i + ni * (j + nj * k)];
+        free(V.rp); free(V.ci); free(V.val);
+        for (int q = 0; q < 6; q++) free(*vecs[q]);
+    }
+    for (int c = 0; c < 3; c++) free(old[c]);
+    return rc;
+}
+
 /* EULERIAN_FLUID_SOLVER_3D::Sourcing (:2076-2100) -> VISCOSITY_3D::ApplyViscosity
  * air/water explicit branch, VISCOSITY_3D.h:448-563: components x, y, z in
- * turn, each updated in place (in-place mode) or from a copy of itself. */
+ * turn, each updated in place (in-place mode) or from a copy of itself;
+ * the semi-implicit branch when semi_implicit_viscosity is set. */
 int fso_sourcing(fso_state* s, double dt) {
     if (!s->P.use_delta_function_formulation) return FS_OK;
+    if (s->P.semi_implicit_viscosity) return semi_implicit_viscosity(s, dt, &s->visc_iters);
     double* comps[3] = {s->u, s->v, s->w};
     for (int c = 1; c <= 3; c++) {
         int ni, nj, nk;
@@ -578,13 +767,12 @@ int fso_sourcing(fso_state* s, double dt) {
 /* Projection                                                                 */
 /* ------------------------------------------------------------------------- */
 
-static double plane_sum_dot(fso_state* s, const double* a, const double* b) {
-    /* rows are numbered plane by plane; per-plane partials (each summed in
-     * row order, planes in parallel), then added in plane order */
-    int per = s->nx * s->ny;
+/* rows are numbered plane by plane (per rows each); per-plane partials
+ * (each summed in row order, planes in parallel), then added in plane order */
+static double planes_dot(fso_state* s, int per, int nplanes, const double* a, const double* b) {
     double* part = s->plane_part;
 #pragma omp parallel for num_threads(g_threads) schedule(static)
-    for (int k = 0; k < s->nz; k++) {
+    for (int k = 0; k < nplanes; k++) {
         double acc = 0;
         const double* aa = a + (long)k * per;
         const double* bb = b + (long)k * per;
@@ -592,9 +780,10 @@ static double plane_sum_dot(fso_state* s, const double* a, const double* b) {
         part[k] = acc;
     }
     double total = 0;
-    for (int k = 0; k < s->nz; k++) total += part[k];
+    for (int k = 0; k < nplanes; k++) total += part[k];
     return total;
 }
+
 
 /* PROJECTION_3D::DetermineProjectionDensity, PROJECTION_3D.h:596-679. */
 static void projection_density(fso_state* s) {
@@ -743,36 +932,51 @@ static double csr_diag(const fso_state* s, int row) {
 }
 
 /* CSR_MATRIX::Multiply, CSR_MATRIX.h:236-260 */
-static void csr_mul(fso_state* s, const double* x, double* y) {
+static void sys_mul(const sys_t* S, const double* x, double* y) {
 #pragma omp parallel for num_threads(g_threads) schedule(static)
-    for (int row = 0; row < s->N; row++) {
+    for (int row = 0; row < S->N; row++) {
         double v = 0;
-        for (int e = s->rp[row]; e < s->rp[row + 1]; e++) v += s->val[e] * x[s->ci[e]];
+        for (int e = S->rp[row]; e < S->rp[row + 1]; e++) v += S->val[e] * x[S->ci[e]];
         y[row] = v;
     }
 }
+static double sys_diag(const sys_t* S, int row) {
+    for (int e = S->rp[row]; e < S->rp[row + 1]; e++)
+        if (S->ci[e] == row) return S->val[e];
+    return 0;
+}
+static sys_t pressure_sys(fso_state* s) {
+    sys_t S = {s->N, s->nx * s->ny, s->nz, s->rp, s->ci, s->val, s->b, s->x, s->r, s->zv, s->pp, s->Ap};
+    return S;
+}
+static void csr_mul(fso_state* s, const double* x, double* y) {
+    sys_t S = pressure_sys(s);
+    sys_mul(&S, x, y);
+}
 
 /* PCG_METHOD::PCGMethod, PCG_METHOD.cpp:4-126, with the Jacobi
- * preconditioner of MultiplicationByMinverseAsDiagonal (:128-160). */
-static void pcg(fso_state* s) {
+ * preconditioner of MultiplicationByMinverseAsDiagonal (:128-160), on any
+ * system; the stop rule, tolerances and max_iteration of PROJECTION_WATER. */
+static void pcg_sys(fso_state* s, const sys_t* S) {
     const fs_params* P = &s->P;
-    int N = s->N;
-    double *x = s->x, *r = s->r, *sv = s->zv, *p = s->pp, *Ap = s->Ap, *b = s->b;
+    int N = S->N;
+    double *x = S->x, *r = S->r, *sv = S->zv, *p = S->pp, *Ap = S->Ap;
+    const double* b = S->b;
     double* dg = (double*)malloc(sizeof(double) * (size_t)N);
-    for (int i = 0; i < N; i++) dg[i] = csr_diag(s, i);
-    csr_mul(s, x, Ap);
+    for (int i = 0; i < N; i++) dg[i] = sys_diag(S, i);
+    sys_mul(S, x, Ap);
     for (int i = 0; i < N; i++) r[i] = b[i] - Ap[i];
     for (int i = 0; i < N; i++) p[i] = r[i] / dg[i];
-    double res_new = plane_sum_dot(s, r, p);
-    double bnorm = sqrt(plane_sum_dot(s, b, b));
+    double res_new = planes_dot(s, S->per, S->nplanes, r, p);
+    double bnorm = sqrt(planes_dot(s, S->per, S->nplanes, b, b));
     double sqr_tol = P->tolerance * P->tolerance;
     int it = 0;
-    double rr = plane_sum_dot(s, r, r);
+    double rr = planes_dot(s, S->per, S->nplanes, r, r);
     int rel = P->stop_rule == FS_STOP_REL_RESIDUAL;
     if (rel && (bnorm == 0 || sqrt(rr) <= P->rel_tolerance * bnorm)) goto done;
     while (it < P->max_iteration) {
-        csr_mul(s, p, Ap);
-        double dot_result = plane_sum_dot(s, p, Ap);
+        sys_mul(S, p, Ap);
+        double dot_result = planes_dot(s, S->per, S->nplanes, p, Ap);
         double alpha = res_new / dot_result;
 #pragma omp parallel for num_threads(g_threads) schedule(static)
         for (int i = 0; i < N; i++) {
@@ -781,9 +985,9 @@ static void pcg(fso_state* s) {
             sv[i] = r[i] / dg[i];
         }
         double res_old = res_new;
-        res_new = plane_sum_dot(s, r, sv);
+        res_new = planes_dot(s, S->per, S->nplanes, r, sv);
         if (rel) {
-            rr = plane_sum_dot(s, r, r);
+            rr = planes_dot(s, S->per, S->nplanes, r, r);
             if (sqrt(rr) < P->rel_tolerance * bnorm) break;
         } else if (res_new < sqr_tol)
             break;
@@ -799,7 +1003,7 @@ done:
     s->last_iters = it;
     s->last_res_abs = sqrt(res_new);
     /* FP64 true residual for the report */
-    csr_mul(s, x, Ap);
+    sys_mul(S, x, Ap);
     double tr = 0;
     for (int i = 0; i < N; i++) {
         double d = b[i] - Ap[i];
@@ -807,6 +1011,10 @@ done:
     }
     s->last_res_rel = bnorm > 0 ? sqrt(tr) / bnorm : 0;
     free(dg);
+}
+static void pcg(fso_state* s) {
+    sys_t S = pressure_sys(s);
+    pcg_sys(s, &S);
 }
 
 /* PROJECTION_3D::UpdateVelocityByPressureGradientVariableDensity,
@@ -1060,7 +1268,10 @@ static void solve_for_rk(fso_state* s, double dt, fs_step_stats* st, int* rc) {
     const fs_params* P = &s->P;
     if (P->is_velocity_advection_active) fso_velocity_advection(s, dt);
     if (P->is_external_force_active) fso_apply_gravity(s, dt);
-    if (P->is_sourcing_active) fso_sourcing(s, dt);
+    if (P->is_sourcing_active) {
+        int r = fso_sourcing(s, dt);
+        if (r != FS_OK) *rc = r;
+    }
     if (P->is_projection_active) {
         int its = st ? st->pcg_iterations : 0;
         int r = fso_projection(s, dt, st);
@@ -1077,6 +1288,7 @@ int fso_step(fso_state* s, double dt, fs_step_stats* st) {
     const int order = P->order_for_time_advancing;
     int rc = FS_OK;
     if (st) memset(st, 0, sizeof(*st));
+    s->visc_iters = 0;
     if (P->is_levelset_advection_active) {
         if (order == 3) {
             fso_levelset_advection(s, dt);
@@ -1111,6 +1323,7 @@ int fso_step(fso_state* s, double dt, fs_step_stats* st) {
     } else {
         average_phi(s);
     }
+    if (st) st->viscosity_iterations = s->visc_iters;
     return rc;
 }
 
@@ -1151,6 +1364,7 @@ int fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt) {
     double s_f = 0;
     double a = P->water_viscosity / P->water_density, b = P->air_viscosity / P->air_density;
     double v_f = (a > b ? a : b) * ((double)2 * s->cell.x.inv_d2 + (double)2 * s->cell.y.inv_d2 + (double)2 * s->cell.z.inv_d2);
+    if (P->semi_implicit_viscosity) v_f = 0; /* :2770-2773 */
     double c = (double)0.5 * (c_f + v_f + sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) + (double)4 * (s_f * s_f)));
     *dt = cfl * ((double)1 / c);
     return FS_OK;

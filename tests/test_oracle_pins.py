@@ -371,3 +371,76 @@ def test_error_paths(lib):
     with pytest.raises(_abi.FsError) as e:
         s.cfl_dt(0.5)
     assert e.value.code == _abi.FS_ERR_NONFINITE
+
+
+def _faces(n, comp):
+    """Face-centre coordinates (x, y, z) of MAC component comp on the unit
+    cube with n cells per axis, arrays indexed [k, j, i]."""
+    h = 1.0 / n
+    c = (np.arange(n) + 0.5) * h
+    f = np.arange(n + 1) * h
+    ax = [f if comp == 1 else c, f if comp == 2 else c, f if comp == 3 else c]
+    z, y, x = np.meshgrid(ax[2], ax[1], ax[0], indexing="ij")
+    return x, y, z
+
+
+def _visc_solver(lib, n, dt_coef, **over):
+    """Uniform mu/rho = 0.01 with dt chosen so that dt*nu/h^2 = dt_coef."""
+    mu, rho = 0.02, 2.0
+    over.setdefault("stop_rule", _abi.FS_STOP_REL_RESIDUAL)
+    over.setdefault("rel_tolerance", 1e-13)
+    s, phi, _ = _solver(lib, "rising_drop", n, air_density=rho, water_density=rho, air_viscosity=mu,
+                        water_viscosity=mu, semi_implicit_viscosity=1, max_iteration=500, **over)
+    return s, phi, dt_coef * (1.0 / n) ** 2 / (mu / rho)
+
+
+@pytest.mark.parametrize("comp", [1, 2, 3])
+def test_semi_implicit_viscosity_eigenvector(lib, comp):
+    """With uniform mu/rho, a product of sines that vanishes on the
+    Dirichlet ring beyond the face box is an eigenvector of the implicit
+    operator I - dt nu (2 d_nn + d_tt + d_ss): one step divides it by
+    1 + c [2(2 - 2cos a_n) + (2 - 2cos a_t) + (2 - 2cos a_s)], c = dt nu/h^2
+    (normal pair 2 mu, tangential pairs mu).  The other components are zero,
+    so the component's cross terms vanish."""
+    n = 12
+    s, phi, dt = _visc_solver(lib, n, 0.5)
+    modes = (1, 2, 3)
+    shape = [n, n, n]
+    shape[comp - 1] = n + 1  # x, y, z extents of the face box
+    idx = np.meshgrid(np.arange(shape[2]), np.arange(shape[1]), np.arange(shape[0]), indexing="ij")
+    vec = np.ones(idx[0].shape)
+    lam = 1.0
+    for d in range(3):  # d = 0 x, 1 y, 2 z
+        m, nn = modes[d], shape[d]
+        ang = np.pi * m / (nn + 1)
+        vec = vec * np.sin(ang * (idx[2 - d] + 1))
+        lam += 0.5 * (2 if d == comp - 1 else 1) * (2 - 2 * np.cos(ang))
+    zeros = [np.zeros((n, n, n + 1)), np.zeros((n, n + 1, n)), np.zeros((n + 1, n, n))]
+    zeros[comp - 1] = vec
+    s.set_fields(phi, *zeros)
+    s.phase("sourcing", dt)
+    out = s.get_fields()["uvw"[comp - 1]]
+    assert np.abs(out - vec / lam).max() < 1e-13
+    assert abs(lam - 1) > 0.2  # far from the identity
+
+
+@pytest.mark.parametrize("comp", [1, 2, 3])
+def test_semi_implicit_viscosity_cross_terms(lib, comp):
+    """Bilinear other components make the explicit cross term
+    dt nu (d_c d_a a + d_c d_b b) = 2 dt nu in the interior; with a small
+    c = dt nu/h^2 the implicit solve keeps that constant away from the walls
+    (wall influence decays like c^distance)."""
+    n = 16
+    s, phi, dt = _visc_solver(lib, n, 1e-3)
+    others = [c for c in (1, 2, 3) if c != comp]
+    fields = [np.zeros((n, n, n + 1)), np.zeros((n, n + 1, n)), np.zeros((n + 1, n, n))]
+    coord = {1: 0, 2: 1, 3: 2}
+    for o in others:
+        xyz = _faces(n, o)
+        fields[o - 1] = xyz[coord[comp]] * xyz[coord[o]]
+    s.set_fields(phi, *fields)
+    s.phase("sourcing", dt)
+    out = s.get_fields()["uvw"[comp - 1]]
+    expect = 2 * dt * 0.01
+    inner = (slice(3, -3), slice(3, -3), slice(3, -3))
+    assert np.abs(out[inner] / expect - 1).max() < 1e-6
