@@ -171,6 +171,7 @@ def main():
     # and back-to-back repeats of each kernel for comparison
     k32, n32 = _abi.kernel_samples(s32, _abi.FS_FP32_REFINE)
     k32_rep = _abi.time_pcg_kernels(s32, _abi.FS_FP32_REFINE, 20)
+    f32 = s32.get_fields()
     s32.close()
 
     res = {}
@@ -183,11 +184,24 @@ def main():
         ms64, st64 = run_frames(s64, dt, a.warmup, a.steps, sync)
         print(f"[bench] fp64 {ms64:.2f} ms/step; last {st64[-1]}", file=sys.stderr, flush=True)
         k64, _ = _abi.kernel_samples(s64, _abi.FS_FP64)
+        f64 = s64.get_fields()
         s64.close()
         res["fp64"] = {"ms_per_step": ms64, "pcg_iterations_per_step": float(np.mean([s["pcg_iterations"] for s in st64])),
                        "projection_ms": float(np.mean([s["projection_ms"] for s in st64])),
                        "residual_rel": st64[-1]["residual_rel"],
                        "kernel_ms": {"spmv_dot": float(k64[0]), "update": float(k64[1]), "p_update": float(k64[2])}}
+        # both paths ran the same frames from the same state: per-field error
+        # of FP32+refine against FP64 (pressure compared with its mean
+        # removed: the additive constant is held only by the weak pin)
+        err = {}
+        for c in ("phi", "u", "v", "w", "p"):
+            a32, a64 = f32[c], f64[c]
+            if c == "p":
+                a32, a64 = a32 - a32.mean(), a64 - a64.mean()
+            d = a32 - a64
+            err[c] = {"rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(a64), 1e-300)),
+                      "rel_linf": float(np.abs(d).max() / max(np.abs(a64).max(), 1e-300))}
+        res["fp32_refine_vs_fp64_error"] = err
 
     # roofline: algorithmic bytes per launch / measured duration
     words = {"spmv_dot": 5, "update": 8, "p_update": 3}
