@@ -189,7 +189,7 @@ def main():
         res["fp64"] = {"ms_per_step": ms64, "pcg_iterations_per_step": float(np.mean([s["pcg_iterations"] for s in st64])),
                        "projection_ms": float(np.mean([s["projection_ms"] for s in st64])),
                        "residual_rel": st64[-1]["residual_rel"],
-                       "kernel_ms": {"spmv_dot": float(k64[0]), "update": float(k64[1]), "p_update": float(k64[2])}}
+                       "kernel_ms": {"pspmv_dot": float(k64[0]), "update": float(k64[1])}}
         # both paths ran the same frames from the same state: per-field error
         # of FP32+refine against FP64 (pressure compared with its mean
         # removed: the additive constant is held only by the weak pin)
@@ -203,10 +203,15 @@ def main():
                       "rel_linf": float(np.abs(d).max() / max(np.abs(a64).max(), 1e-300))}
         res["fp32_refine_vs_fp64_error"] = err
 
-    # roofline: algorithmic bytes per launch / measured duration
-    words = {"spmv_dot": 5, "update": 8, "p_update": 3}
-    names = ["spmv_dot", "update", "p_update"]
-    kname = {"spmv_dot": "k_spmv_dot", "update": "k_update", "p_update": "k_pupdate"}
+    # roofline: algorithmic bytes per launch / measured duration (two kernels
+    # per iteration)
+    # algorithmic words per cell: p-update fused with SpMV + p.q reads s,
+    # p_old, beta_x, beta_y, beta_z and writes p, q; the update reads x, p, r,
+    # q, D and writes x, r, s
+    words = {"pspmv_dot": 7, "update": 8}
+    names = ["pspmv_dot", "update"]
+    kname = {"pspmv_dot": "k_pspmv_dot", "update": "k_update"}
+    k32 = k32[:2]
     dom = int(np.argmax(k32))
     name = names[dom]
     bytes_launch = words[name] * cells * 4

@@ -7,6 +7,7 @@
 // stream has drained.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -89,7 +90,7 @@ class EulerianFluidSolver3D {
     int chunk = 16;
     // in-loop kernel sampling (measurement only): one iteration per launch
     // batch bracketed with events; sums per precision (0 FP64, 1 FP32) and
-    // kernel (0 SpMV+dot, 1 update, 2 p-update)
+    // kernel (0 p-update + SpMV + dot, 1 update; 2 unused)
     FsKernelProbe probe{};
     bool sampling = false;
     double ksum[2][3] = {};
@@ -153,6 +154,7 @@ class EulerianFluidSolver3D {
         w64.s = dfield();
         w64.p = dfield();
         w64.q = dfield();
+        w64.p2 = dfield();
         w64.partials = (double*)dalloc(npart);
         if (P.precision == FS_FP32_REFINE) {
             bx32 = ffield();
@@ -165,6 +167,7 @@ class EulerianFluidSolver3D {
             w32.s = ffield();
             w32.p = ffield();
             w32.q = ffield();
+            w32.p2 = ffield();
             w32.partials = (double*)dalloc(npart);
         }
         if (P.semi_implicit_viscosity) {
@@ -175,6 +178,7 @@ class EulerianFluidSolver3D {
             wv.s = dfield();
             wv.p = dfield();
             wv.q = dfield();
+            wv.p2 = dfield();
             wv.partials = (double*)dalloc(npart);
         }
         S64 = (FsPcgState*)dalloc(sizeof(FsPcgState));
@@ -196,7 +200,7 @@ class EulerianFluidSolver3D {
         for (double* f : f64)
             if (f) (void)hipFree(f);
         void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32,
-                      S64, S32, vmax, wv.r, wv.s, wv.p, wv.q};
+                      S64, S32, vmax, wv.r, wv.s, wv.p, wv.q, w64.p2, w32.p2, wv.p2};
         for (void* f : fv)
             if (f) (void)hipFree(f);
         if (hS) (void)hipHostFree(hS);
@@ -314,7 +318,7 @@ class EulerianFluidSolver3D {
             Lf.nz = b.nk;
             // vectors of the previous component's box must not leave values
             // in this box's zero ghost ring
-            for (void* f : {wv.r, wv.s, wv.p, wv.q})
+            for (void* f : {wv.r, wv.s, wv.p, wv.q, wv.p2})
                 HIPCHECK(hipMemsetAsync(f, 0, sizeof(double) * (size_t)L.total, st));
             FsPcgOp o{};
             o.kind = FS_OP_ROWS;
@@ -419,7 +423,7 @@ class EulerianFluidSolver3D {
         if (!sampling || !probe.recorded) return;
         probe.recorded = false;
         if (hS->done && hS->iter <= probe.iter) return;
-        for (int k = 0; k < 3; k++) {
+        for (int k = 0; k < FS_PCG_KERNELS; k++) {
             float ms = 0;
             HIPCHECK(hipEventElapsedTime(&ms, probe.ev[2 * k], probe.ev[2 * k + 1]));
             ksum[prec][k] += ms;
@@ -489,9 +493,9 @@ class EulerianFluidSolver3D {
         hbb = bb;
         HIPCHECK(hipMemcpyAsync(&S32->bb, &hbb, sizeof(double), hipMemcpyHostToDevice, st));
         double rr_ref = rr64;
-        int n = 0, pending = 0;
+        int n = 0;
         while (true) {
-            fs_pcg32_chunk(st, L, o32, diag32, e32, w32, S32, n, chunk, pending, sampling ? &probe : nullptr);
+            fs_pcg32_chunk(st, L, o32, diag32, e32, w32, S32, n, chunk, sampling ? &probe : nullptr);
             check_launch();
             read_state(S32);
             harvest(1);
@@ -503,11 +507,10 @@ class EulerianFluidSolver3D {
             if (debug && dbg_lines++ < 120)
                 fprintf(stderr, "[fs]   chunk n0=%d done=%d iter=%d rel32=%.3e rz=%.3e,%.3e\n", n, hS->done, hS->iter,
                         std::sqrt(hS->rr / bb), hS->rz[0], hS->rz[1]);
-            // the iteration whose p-update is pending: the one that met the
-            // target, or the chunk's last
+            // the last iteration that ran: the one that met the target, or
+            // the chunk's last; the next K1 continues from it
             int last = stopped ? hS->iter : n + chunk - 1;
             n = last + 1;
-            pending = 1;
             double rr32 = hS->rr;
             if (stopped || rr32 < update_drop * rr_ref) {
                 fs_add_correction(st, L, p, e32);
@@ -904,18 +907,18 @@ int fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms) {
         const void* dg = S.diag;
         void* x = S.tmp;  // scratch x (tmp only lives inside a step): the pressure is left untouched
         if (precision == FS_FP32_REFINE) {
-            o.bx = S.bx32;
-            o.by = S.by32;
-            o.bz = S.bz32;
+            o = S.op32();
             w = &S.w32;
             dg = S.diag32;
             x = S.e32;
         }
-        for (int which = 0; which < 3; which++) {
-            fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which, 0);  // warm
+        avg_ms[2] = 0;  // the p-update runs inside kernel 0
+        for (int which = 0; which < FS_PCG_KERNELS; which++) {
+            // as iteration 1: kernel 0 forms the direction on the fly
+            fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which, 1);  // warm
             HIPCHECK(hipEventRecord(S.ev[0], S.st));
             for (int r = 0; r < reps; r++)
-                fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which, 0);
+                fs_pcg_time_kernels(S.st, S.L, o, dg, x, *w, dS, precision == FS_FP64 ? 0 : 1, which, 1);
             HIPCHECK(hipEventRecord(S.ev[1], S.st));
             S.check_launch();
             S.sync();

@@ -6,10 +6,10 @@
 
 #include "fs_layout.h"
 
-// Device-resident solver scalars.  Written by one thread between kernels
-// (begin-finalize, the p-update's block 0); every kernel of iteration n
-// reads rz[n & 1] and `done` at entry and returns at once when done != 0,
-// so the host may enqueue iterations past convergence.
+// Device-resident solver scalars.  Written by one thread (begin-finalize,
+// workgroup 0 of the fused K1, the loop-tail kernel); every kernel of
+// iteration n reads rz and `done` at entry and returns at once when
+// done != 0, so the host may enqueue iterations past convergence.
 struct FsPcgState {
     double rz[2];         // r.z of iteration n lives in rz[n & 1]
     double rr, bb;        // r.r and b.b of the last iteration / of the start
@@ -39,27 +39,34 @@ struct FsPcgOp {
     const double* dg;
 };
 
-// Workspace of one precision.
+// Workspace of one precision.  The direction of iteration n lives in p
+// (n even) or p2 (n odd).
 struct FsPcgWork {
     void *r, *s, *p, *q;    // padded-layout vectors (ghosts stay zero)
     double* partials;       // FS_PCG_SLOTS * FS_PCG_MAX_BLOCKS per-workgroup partial sums
+    void* p2;
 };
+
+// Kernels per PCG iteration: the fused p-update + SpMV + p.q, and the
+// x/r/s update with r.s and r.r.
+#define FS_PCG_KERNELS 2
 
 #define FS_PCG_MAX_BLOCKS 2048
 #define FS_PCG_SLOTS 8
 
 // Measurement only: HIP events recorded on the solve's stream around the
-// three kernels of the first iteration of a launch batch, ev[2w] before and
-// ev[2w+1] after kernel w (0 SpMV+dot, 1 update, 2 p-update).  `iter` is that
-// iteration; `recorded` is set when the events were enqueued.
+// kernels of the first iteration of a launch batch, ev[2w] before and
+// ev[2w+1] after kernel w (0 p-update + SpMV + dot, 1 update).  `iter` is
+// that iteration; `recorded` is set when the events were enqueued.
 struct FsKernelProbe {
-    hipEvent_t ev[6];
+    hipEvent_t ev[2 * FS_PCG_KERNELS];
     int iter;
     bool recorded;
 };
 
 // FP64 / FP32 solves on x (warm start).  begin: r = b - A x, z = D^-1 r,
-// p = z and the scalars; iterations: n0 .. n0+count-1.
+// p = z and the scalars; iterations: n0 .. n0+count-1, then the loop tail of
+// the last (the host sees its stop without another batch).
 void fs_pcg64_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* rhs, const double* diag,
                     double* x, FsPcgWork& w, FsPcgState* dstate);
 void fs_pcg64_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* diag, double* x,
@@ -83,15 +90,14 @@ void fs_deflate_constant(hipStream_t st, const FsLayout& L, double* x, double* r
 // state->rsum = sum(r).
 void fs_residual64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* b, const double* x, double* r,
                    float* r32, FsPcgWork& w, FsPcgState* dstate);
-// FP32 iterations n0 .. n0+count-1 ending after the r-update (K2) of the
-// last one, whose p-update (K3) is left pending; pending_k3 != 0 first runs
-// the pending K3 of iteration n0-1.  state->rr receives the last r.r.  The
-// probe, if any, brackets iteration n0 when count >= 2.
+// FP32 iterations n0 .. n0+count-1 and the loop tail of the last (its r.r
+// into state->rr, done = 1 if it met the target).  The probe, if any,
+// brackets iteration n0.
 void fs_pcg32_chunk(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x, FsPcgWork& w,
-                    FsPcgState* dstate, int n0, int count, int pending_k3, FsKernelProbe* probe = nullptr);
-// Reliable update between K2 and the pending K3, with r32 holding the
-// rounded FP64 residual: the recursion's r = r32, s = r32/D and the r.s / r.r
-// partials the pending K3 reduces; clears `done`.
+                    FsPcgState* dstate, int n0, int count, FsKernelProbe* probe = nullptr);
+// Reliable update between an iteration's K2 and the next K1, with r32
+// holding the rounded FP64 residual: the recursion's r = r32, s = r32/D and
+// the r.s / r.r partials the next K1 reduces; clears `done`.
 void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r32, const float* diag, FsPcgWork& w,
                               FsPcgState* dstate);
 // x += (double) e
@@ -99,7 +105,7 @@ void fs_add_correction(hipStream_t st, const FsLayout& L, double* x, const float
 // FP32 copies of the face coefficients and the FP32 Jacobi diagonal.
 void fs_make_op32(hipStream_t st, const FsLayout& L, const double* bx, const double* by, const double* bz,
                   const double* diag, float* bx32, float* by32, float* bz32, float* diag32);
-// One launch of iteration kernel `which` (0 SpMV+dot, 1 update, 2 p update)
-// as iteration n, for timing.
+// One launch of iteration kernel `which` (0 p-update + SpMV + dot, 1
+// update) as iteration n, for timing.
 void fs_pcg_time_kernels(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, void* x,
                          FsPcgWork& w, FsPcgState* dstate, int precision, int which, int n);

@@ -2,10 +2,16 @@
 //
 // Layout: the padded grid of fs_layout.h; every vector keeps zero ghosts so
 // the 7-point stencil needs no bounds tests.  Work unit: a tile of (64*V) x 4
-// cells in x-y marching ZC planes in z.  Each thread owns V consecutive
-// x-cells (V = 2 in FP64, 4 in FP32: 16-byte loads and stores), keeps
-// p(k-1), p(k), p(k+1) and beta_z(k), beta_z(k+1) in registers, and takes
-// its x-neighbours at the vector ends with two scalar loads that hit L1.
+// cells in x-y marching zc planes in z (Tiling::zc).  Each thread owns V
+// consecutive x-cells (V = 2 in FP64, 4 in FP32: 16-byte loads and stores),
+// keeps p(k-1), p(k), p(k+1) and beta_z(k), beta_z(k+1) in registers, and
+// takes its x-neighbours at the vector ends with two scalar loads that hit L1.
+//
+// One iteration is two kernels: K1 runs the loop tail of the previous
+// iteration (stop test, beta), forms its direction p = p_old*beta + s on the
+// fly for every cell the stencil touches, stores it, and computes q = A p
+// with the partial p.q; K2 updates x, r, s = r/D with the partials of r.s and
+// r.r.  The two directions alternate between two buffers.
 // A fixed grid of NB workgroups walks the tiles; workgroup b lives on XCD
 // b % 8 and takes tiles from that XCD's contiguous share, so y-neighbour rows
 // are reused in one L2.
@@ -28,8 +34,14 @@
 
 #define TBX 64
 #define TBY 4
-#define ZC 8
 #define NT (TBX * TBY)
+// vector width of the fused SpMV (at most the update kernels' width)
+#ifndef FS_SPMV_V32
+#define FS_SPMV_V32 4
+#endif
+#ifndef FS_SPMV_V64
+#define FS_SPMV_V64 2
+#endif
 
 // partial-sum slots inside FsPcgWork::partials
 #define SLOT_PQ 0
@@ -78,6 +90,7 @@ __device__ __forceinline__ void vst(T* __restrict__ p, const T (&o)[V]) {
 
 // ---- tiling -------------------------------------------------------------------
 struct Tiling {
+    int zc;  // z-planes per tile
     int ntx, nty, ntz, ntiles, nb, v;
 };
 
@@ -86,10 +99,24 @@ static Tiling make_tiling(const FsLayout& L, int v) {
     t.v = v;
     t.ntx = (L.nx + TBX * v - 1) / (TBX * v);
     t.nty = (L.ny + TBY - 1) / TBY;
-    t.ntz = (L.nz + ZC - 1) / ZC;
+    // the deepest z-chunk that still leaves at least 512 tiles: a longer
+    // z-march re-reads fewer halo planes (measured at 256^3: 32 planes beat
+    // 16 and 8), enough tiles keep every CU busy on small grids
+    for (t.zc = 32; t.zc > 4; t.zc >>= 1)
+        if ((long)t.ntx * t.nty * ((L.nz + t.zc - 1) / t.zc) >= 512) break;
+    t.ntz = (L.nz + t.zc - 1) / t.zc;
     t.ntiles = t.ntx * t.nty * t.ntz;
     int nb = t.ntiles < FS_PCG_MAX_BLOCKS ? t.ntiles : FS_PCG_MAX_BLOCKS;
     if (nb >= 8) nb = nb / 8 * 8;
+    t.nb = nb;
+    return t;
+}
+
+// Tiles of width v walked by the workgroup count of another tiling: kernels
+// of one iteration may use different vector widths, but producer and
+// consumer of the partial sums must agree on the number of workgroups.
+static Tiling make_tiling_nb(const FsLayout& L, int v, int nb) {
+    Tiling t = make_tiling(L, v);
     t.nb = nb;
     return t;
 }
@@ -129,7 +156,7 @@ __device__ __forceinline__ void tile_origin(const Tiling& tl, int t, int& i0, in
     int ty = r % tl.nty, tz = r / tl.nty;
     i0 = (tx * TBX + threadIdx.x) * V;
     j = ty * TBY + threadIdx.y;
-    k0 = tz * ZC;
+    k0 = tz * tl.zc;
 }
 
 // ---- reductions ---------------------------------------------------------------
@@ -239,27 +266,61 @@ __device__ __forceinline__ float row_val(const OpT<float>& op, const Nb<float>& 
     return v;
 }
 
+// Operand loaders for march_apply: the vector x itself, or the direction
+// p = p_old*beta + s formed on the fly (the p-update of PCG_METHOD.cpp,
+// p *= beta; p += s, with the same two roundings).
+template <typename T, int V>
+struct LdPlain {
+    const T* __restrict__ x;
+    __device__ __forceinline__ void vec(long c, T (&out)[V]) const { vld<T, V>(x + c, out); }
+    __device__ __forceinline__ T at(long c) const { return x[c]; }
+};
+template <typename T, int V>
+struct LdDir {
+    const T* __restrict__ po;
+    const T* __restrict__ s;
+    T beta;
+    __device__ __forceinline__ void vec(long c, T (&out)[V]) const {
+        T sv[V];
+        vld<T, V>(po + c, out);
+        vld<T, V>(s + c, sv);
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            out[v] *= beta;
+            out[v] += sv[v];
+        }
+    }
+    __device__ __forceinline__ T at(long c) const {
+        T r = po[c];
+        r *= beta;
+        r += s[c];
+        return r;
+    }
+};
+
 // Marches one thread's V cells through planes k0..k1-1, calling f(c, v, k,
-// Ax) for each cell with the operator applied to x.
-template <typename T, int V, typename F>
-__device__ __forceinline__ void march_apply(const FsLayout& L, const OpT<T>& op, const T* __restrict__ x, int i0, int j,
-                                            int k0, T pin_b, T pin_d, F&& f) {
-    int k1 = min(k0 + ZC, L.nz);
+// x_c, Ax) for each cell with the operator applied to the loader's vector.
+// (Issuing the next plane's loads ahead of the current plane's arithmetic
+// measured no faster at 256^3 and cost FP64 occupancy.)
+template <typename T, int V, typename LD, typename F>
+__device__ __forceinline__ void march_apply(const FsLayout& L, const OpT<T>& op, const LD& x, int i0, int j, int k0,
+                                            int zc, T pin_b, T pin_d, F&& f) {
+    int k1 = min(k0 + zc, L.nz);
     long c = L.idx(i0, j, k0);
     T xm[V], xc[V], bzd[V];
-    vld<T, V>(x + c - L.sy, xm);
-    vld<T, V>(x + c, xc);
+    x.vec(c - L.sy, xm);
+    x.vec(c, xc);
     vld<T, V>(op.bz + c, bzd);
     for (int k = k0; k < k1; k++, c += L.sy) {
         T xp[V], bzu[V], xb[V], xt[V], bl[V], bb[V], bt[V];
-        vld<T, V>(x + c + L.sy, xp);
+        x.vec(c + L.sy, xp);
         vld<T, V>(op.bz + c + L.sy, bzu);
-        vld<T, V>(x + c - L.sx, xb);
-        vld<T, V>(x + c + L.sx, xt);
+        x.vec(c - L.sx, xb);
+        x.vec(c + L.sx, xt);
         vld<T, V>(op.bx + c, bl);
         vld<T, V>(op.by + c, bb);
         vld<T, V>(op.by + c + L.sx, bt);
-        T xleft = x[c - 1], xright = x[c + V], bxr = op.bx[c + V];
+        T xleft = x.at(c - 1), xright = x.at(c + V), bxr = op.bx[c + V];
 #pragma unroll
         for (int v = 0; v < V; v++) {
             Nb<T> n;
@@ -277,7 +338,7 @@ __device__ __forceinline__ void march_apply(const FsLayout& L, const OpT<T>& op,
             n.bd = bzd[v];
             n.bu = bzu[v];
             bool pin = (v == 0) && ((i0 | j | k) == 0);
-            f(c, v, k, row_val(op, n, pin, pin_b, pin_d));
+            f(c, v, k, xc[v], row_val(op, n, pin, pin_b, pin_d));
         }
 #pragma unroll
         for (int v = 0; v < V; v++) {
@@ -301,7 +362,7 @@ __global__ void __launch_bounds__(NT) k_begin(FsLayout L, Tiling tl, FsPcgOp o, 
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        march_apply<T, V>(L, op, x, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, T ax) {
+        march_apply<T, V>(L, op, LdPlain<T, V>{x}, i0, j, k0, tl.zc, pin_b, pin_d, [&](long c, int v, int, T, T ax) {
             T bv = b[c + v];
             T rv = bv - ax;
             T sv = rv / D[c + v];
@@ -336,12 +397,50 @@ __global__ void __launch_bounds__(NT) k_begin_final(const double* part, int nb, 
     }
 }
 
-// ---- K1: q = A p, partial p.q ---------------------------------------------
+// ---- state step shared by the fused K1 and the chunk-end test ------------
+// The reference's loop tail for iteration m = n-1 (PCG_METHOD.cpp:81-104):
+// with r.s and r.r of K2(m), stop or take r.s as the next rz.  Workgroup 0
+// records the outcome.  Returns true if iteration n is not to run.
+__device__ __forceinline__ bool loop_tail(const double* part, int nb, FsPcgState* S, int n, double& rs,
+                                          double* lds) {
+    rs = get_total(part, SLOT_RS, nb, lds);
+    double rr = get_total(part, SLOT_RR, nb, lds);
+    bool stop = S->stop_rule == 1 ? (rr < S->rtol2 * S->bb) : (rs < S->tol2);
+    bool cap = !stop && n >= S->max_iter;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
+        S->rs_last = rs;
+        S->rr = rr;
+        if (stop) {
+            S->iter = n - 1;
+            S->done = 1;
+        } else {
+            S->rz[n & 1] = rs;
+            S->iter = n;
+            if (cap) S->done = 2;
+        }
+    }
+    return stop || cap;
+}
+
+// ---- K1 (fused with the p-update of the previous iteration) ----------------
+// n = 0: q = A p with p from begin.  n > 0: loop tail of iteration n-1, then
+// p(n) = p(n-1)*beta + s formed on the fly for every cell the stencil
+// touches (p(n-1) from pold, written to pnew: the two directions live in
+// separate buffers so no workgroup reads a value another has replaced),
+// q = A p(n) and the partial p.q.
 template <typename T, int V>
-__global__ void __launch_bounds__(NT) k_spmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ p,
-                                                 T* __restrict__ q, double* part, const FsPcgState* S) {
+__global__ void __launch_bounds__(NT) k_pspmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ s,
+                                                  const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
+                                                  double* part, FsPcgState* S, int n) {
     if (S->done) return;
     __shared__ double lds[4];
+    T beta = 0;
+    if (n > 0) {
+        double rz_old = S->rz[(n - 1) & 1];
+        double rs;
+        if (loop_tail(part, tl.nb, S, n, rs, lds)) return;
+        beta = (T)(rs / rz_old);
+    }
     OpT<T> op = make_op<T>(o);
     double acc = 0;
     T pin_b = (T)S->pin_b, pin_d = (T)S->pin_d;
@@ -349,14 +448,40 @@ __global__ void __launch_bounds__(NT) k_spmv_dot(FsLayout L, Tiling tl, FsPcgOp 
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        T qv[V];
-        march_apply<T, V>(L, op, p, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, T ap) {
+        T qv[V], pv[V];
+        auto body = [&](long c, int v, int, T pc, T ap) {
             qv[v] = ap;
-            acc += (double)p[c + v] * (double)ap;
-            if (v == V - 1) vst<T, V>(q + c, qv);
-        });
+            pv[v] = pc;
+            acc += (double)pc * (double)ap;
+            if (v == V - 1) {
+                vst<T, V>(q + c, qv);
+                if (n > 0) vst<T, V>(pnew + c, pv);
+            }
+        };
+        if (n > 0)
+            march_apply<T, V>(L, op, LdDir<T, V>{pold, s, beta}, i0, j, k0, tl.zc, pin_b, pin_d, body);
+        else
+            march_apply<T, V>(L, op, LdPlain<T, V>{pold}, i0, j, k0, tl.zc, pin_b, pin_d, body);
     }
     put_partial(part, SLOT_PQ, acc, lds);
+}
+
+// The loop tail of iteration n alone (one workgroup), after the last K2 of a
+// launch batch: the host sees the stop and r.r without another batch.
+__global__ void __launch_bounds__(NT) k_tail(const double* part, int nb, FsPcgState* S, int n) {
+    if (S->done) return;
+    __shared__ double lds[4];
+    double rs = get_total(part, SLOT_RS, nb, lds);
+    double rr = get_total(part, SLOT_RR, nb, lds);
+    bool stop = S->stop_rule == 1 ? (rr < S->rtol2 * S->bb) : (rs < S->tol2);
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        S->rr = rr;
+        if (stop) {
+            S->rs_last = rs;
+            S->iter = n;
+            S->done = 1;
+        }
+    }
 }
 
 // ---- K2: alpha = rz/pq; x += a p, r -= a q, s = r/D; partials r.s, r.r ----
@@ -373,7 +498,7 @@ __global__ void __launch_bounds__(NT) k_update(FsLayout L, Tiling tl, const T* _
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
+        int k1 = min(k0 + tl.zc, L.nz);
         long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy) {
             T xv[V], pv[V], rv[V], qv[V], dv[V], sv[V];
@@ -399,62 +524,17 @@ __global__ void __launch_bounds__(NT) k_update(FsLayout L, Tiling tl, const T* _
     put_partial(part, SLOT_RR, a_rr, lds);
 }
 
-// ---- K3: stop test, beta = rz_new/rz, p = p*beta + s ----------------------
-// Workgroup 0 records the outcome for the host and the next iteration.
-template <typename T, int V>
-__global__ void __launch_bounds__(NT) k_pupdate(FsLayout L, Tiling tl, T* __restrict__ p, const T* __restrict__ s,
-                                                const double* part, FsPcgState* S, int n) {
-    if (S->done) return;
-    __shared__ double lds[4];
-    double rs = get_total(part, SLOT_RS, tl.nb, lds);
-    double rr = get_total(part, SLOT_RR, tl.nb, lds);
-    double rz_old = S->rz[n & 1];
-    bool stop = S->stop_rule == 1 ? (rr < S->rtol2 * S->bb) : (rs < S->tol2);
-    if (blockIdx.x == 0 && threadIdx.x == 0 && threadIdx.y == 0) {
-        S->rs_last = rs;
-        S->rr = rr;
-        if (stop) {
-            S->iter = n;
-            S->done = 1;
-        } else {
-            S->rz[(n + 1) & 1] = rs;
-            S->iter = n + 1;
-            if (n + 1 >= S->max_iter) S->done = 2;
-        }
-    }
-    if (stop) return;
-    T beta = (T)(rs / rz_old);
-    for (TileIter it(tl); it.valid(); it.next()) {
-        int i0, j, k0;
-        tile_origin<V>(tl, it.t, i0, j, k0);
-        if (i0 >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
-        long c = L.idx(i0, j, k0);
-        for (int k = k0; k < k1; k++, c += L.sy) {
-            T pv[V], sv[V];
-            vld<T, V>(p + c, pv);
-            vld<T, V>(s + c, sv);
-#pragma unroll
-            for (int v = 0; v < V; v++) {
-                pv[v] *= beta;
-                pv[v] += sv[v];
-            }
-            vst<T, V>(p + c, pv);
-        }
-    }
-}
-
 // ---- FS_OP_ROWS operator (FP64): explicit entries per row -----------------
-__device__ __forceinline__ double rows_val(const FsPcgOp& o, const double* __restrict__ x, long c, long sx,
-                                           long sy) {
+template <typename LD>
+__device__ __forceinline__ double rows_val(const FsPcgOp& o, const LD& x, long c, long sx, long sy) {
     double v = 0;
-    v += o.a[0][c] * x[c - 1];
-    v += o.a[1][c] * x[c + 1];
-    v += o.a[2][c] * x[c - sx];
-    v += o.a[3][c] * x[c + sx];
-    v += o.a[4][c] * x[c - sy];
-    v += o.a[5][c] * x[c + sy];
-    v += o.dg[c] * x[c];
+    v += o.a[0][c] * x.at(c - 1);
+    v += o.a[1][c] * x.at(c + 1);
+    v += o.a[2][c] * x.at(c - sx);
+    v += o.a[3][c] * x.at(c + sx);
+    v += o.a[4][c] * x.at(c - sy);
+    v += o.a[5][c] * x.at(c + sy);
+    v += o.dg[c] * x.at(c);
     return v;
 }
 
@@ -471,12 +551,12 @@ __global__ void __launch_bounds__(NT) k_rows_begin(FsLayout L, Tiling tl, FsPcgO
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
+        int k1 = min(k0 + tl.zc, L.nz);
         long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy)
             for (int v = 0; v < V; v++) {
                 double bv = b[c + v];
-                double rv = bv - rows_val(o, x, c + v, L.sx, L.sy);
+                double rv = bv - rows_val(o, LdPlain<double, 1>{x}, c + v, L.sx, L.sy);
                 double sv = rv / D[c + v];
                 r[c + v] = rv;
                 s[c + v] = sv;
@@ -491,24 +571,43 @@ __global__ void __launch_bounds__(NT) k_rows_begin(FsLayout L, Tiling tl, FsPcgO
     put_partial(part, SLOT_BB, a_bb, lds);
 }
 
-// K1 on FS_OP_ROWS: q = A p, partial p.q.
+// K1 on FS_OP_ROWS, fused like k_pspmv_dot: loop tail of n-1, p(n) on the
+// fly, q = A p(n), partial p.q.
 template <int V>
-__global__ void __launch_bounds__(NT) k_rows_spmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const double* __restrict__ p,
-                                                      double* __restrict__ q, double* part, const FsPcgState* S) {
+__global__ void __launch_bounds__(NT) k_rows_pspmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const double* __restrict__ s,
+                                                       const double* __restrict__ pold, double* __restrict__ pnew,
+                                                       double* __restrict__ q, double* part, FsPcgState* S, int n) {
     if (S->done) return;
     __shared__ double lds[4];
+    double beta = 0;
+    if (n > 0) {
+        double rz_old = S->rz[(n - 1) & 1];
+        double rs;
+        if (loop_tail(part, tl.nb, S, n, rs, lds)) return;
+        beta = rs / rz_old;
+    }
+    LdDir<double, 1> dir{pold, s, beta};
+    LdPlain<double, 1> plain{pold};
     double acc = 0;
     for (TileIter it(tl); it.valid(); it.next()) {
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
+        int k1 = min(k0 + tl.zc, L.nz);
         long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy)
             for (int v = 0; v < V; v++) {
-                double ap = rows_val(o, p, c + v, L.sx, L.sy);
+                double pc, ap;
+                if (n > 0) {
+                    pc = dir.at(c + v);
+                    ap = rows_val(o, dir, c + v, L.sx, L.sy);
+                    pnew[c + v] = pc;
+                } else {
+                    pc = plain.at(c + v);
+                    ap = rows_val(o, plain, c + v, L.sx, L.sy);
+                }
                 q[c + v] = ap;
-                acc += p[c + v] * ap;
+                acc += pc * ap;
             }
     }
     put_partial(part, SLOT_PQ, acc, lds);
@@ -529,22 +628,32 @@ static void begin_v(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const 
     hipLaunchKernelGGL(k_begin_final, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S);
 }
 
+// Direction of iteration n (begin writes p(0) into w.p).
+static inline void* dir_buf(FsPcgWork& w, int n) { return (n & 1) ? w.p2 : w.p; }
+
+// Kernel `which` of iteration n: 0 the fused K1, 1 the update K2, 2 the
+// one-workgroup loop tail of n.
 template <typename T, int V>
 static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
                        FsPcgState* S, int which, int n) {
     Tiling tl = make_tiling(L, V);
+    T* pn = (T*)dir_buf(w, n);
+    const T* po = (const T*)dir_buf(w, n > 0 ? n - 1 : 0);
+    // the fused SpMV keeps three planes and six coefficient vectors live:
+    // it may run narrower than the update (fewer registers, more waves)
+    constexpr int VW = sizeof(T) == 4 ? FS_SPMV_V32 : FS_SPMV_V64;
+    constexpr int VK = VW < V ? VW : V;
     if (which == 0 && op.kind == FS_OP_ROWS)
-        hipLaunchKernelGGL((k_rows_spmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)w.p,
-                           (double*)w.q, w.partials, S);
+        hipLaunchKernelGGL((k_rows_pspmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)w.s,
+                           (const double*)po, (double*)pn, (double*)w.q, w.partials, S, n);
     else if (which == 0)
-        hipLaunchKernelGGL((k_spmv_dot<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const T*)w.p,
-                           (T*)w.q, w.partials, S);
+        hipLaunchKernelGGL((k_pspmv_dot<T, VK>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, make_tiling_nb(L, VK, tl.nb),
+                           op, (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n);
     else if (which == 1)
         hipLaunchKernelGGL((k_update<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, diag, x, (T*)w.r, (T*)w.s,
-                           (const T*)w.p, (const T*)w.q, w.partials, S, n);
+                           (const T*)pn, (const T*)w.q, w.partials, S, n);
     else
-        hipLaunchKernelGGL((k_pupdate<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, (T*)w.p, (const T*)w.s,
-                           w.partials, S, n);
+        hipLaunchKernelGGL(k_tail, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, n);
 }
 
 template <typename T>
@@ -575,10 +684,11 @@ static void one_probed(hipStream_t st, const FsLayout& L, const FsPcgOp& op, con
     one_t<T>(st, L, op, diag, x, w, S, which, n);
     if (rec) {
         (void)hipEventRecord(probe->ev[2 * which + 1], st);
-        if (which == 2) probe->recorded = true;
+        if (which == FS_PCG_KERNELS - 1) probe->recorded = true;
     }
 }
 
+// Iterations n0 .. n0+count-1 (K1, K2 each) and the loop tail of the last.
 template <typename T>
 static void iterations_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
                          FsPcgState* S, int n0, int count, FsKernelProbe* probe) {
@@ -587,7 +697,8 @@ static void iterations_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, c
         probe->recorded = false;
     }
     for (int n = n0; n < n0 + count; n++)
-        for (int which = 0; which < 3; which++) one_probed<T>(st, L, op, diag, x, w, S, which, n, probe);
+        for (int which = 0; which < FS_PCG_KERNELS; which++) one_probed<T>(st, L, op, diag, x, w, S, which, n, probe);
+    one_t<T>(st, L, op, diag, x, w, S, 2, n0 + count - 1);
 }
 
 void fs_pcg64_begin(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const double* rhs, const double* diag,
@@ -609,6 +720,7 @@ void fs_pcg32_iterations(hipStream_t st, const FsLayout& L, const FsPcgOp& op, c
 
 void fs_pcg_time_kernels(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, void* x,
                          FsPcgWork& w, FsPcgState* S, int precision, int which, int n) {
+    if (which < 0 || which >= FS_PCG_KERNELS) return;
     if (precision == 0)
         one_t<double>(st, L, op, (const double*)diag, (double*)x, w, S, which, n);
     else
@@ -625,7 +737,8 @@ __global__ void __launch_bounds__(NT) k_apply64(FsLayout L, Tiling tl, FsPcgOp o
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        march_apply<double, V>(L, op, x, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, double ax) { q[c + v] = ax; });
+        march_apply<double, V>(L, op, LdPlain<double, V>{x}, i0, j, k0, tl.zc, pin_b, pin_d,
+                               [&](long c, int v, int, double, double ax) { q[c + v] = ax; });
     }
 }
 
@@ -652,7 +765,7 @@ __global__ void __launch_bounds__(NT) k_residual64(FsLayout L, Tiling tl, FsPcgO
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        march_apply<double, V>(L, op, x, i0, j, k0, pin_b, pin_d, [&](long c, int v, int, double ax) {
+        march_apply<double, V>(L, op, LdPlain<double, V>{x}, i0, j, k0, tl.zc, pin_b, pin_d, [&](long c, int v, int, double, double ax) {
             double bv = b[c + v];
             double rv = bv - ax;
             r[c + v] = rv;
@@ -717,11 +830,11 @@ void fs_residual64(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const d
     hipLaunchKernelGGL(k_residual_final, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S);
 }
 
-// Reliable update of the FP32 recursion, between the r-update (K2) and the
-// p-update (K3) of an iteration: the recursion's residual is replaced by the
-// rounded FP64 residual, s = r/D and the partials of r.s and r.r are
-// rewritten, so the pending K3 takes beta and its stop test from the
-// refreshed residual.
+// Reliable update of the FP32 recursion, between the update (K2) of an
+// iteration and the next fused K1: the recursion's residual is replaced by
+// the rounded FP64 residual, s = r/D and the partials of r.s and r.r are
+// rewritten, so that K1 takes its stop test and beta from the refreshed
+// residual.
 template <int V>
 __global__ void __launch_bounds__(NT) k_reprecond(FsLayout L, Tiling tl, const float* __restrict__ r,
                                                   const float* __restrict__ D, float* __restrict__ s,
@@ -732,7 +845,7 @@ __global__ void __launch_bounds__(NT) k_reprecond(FsLayout L, Tiling tl, const f
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
+        int k1 = min(k0 + tl.zc, L.nz);
         long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy) {
             float rv[V], dv[V], sv[V];
@@ -752,8 +865,8 @@ __global__ void __launch_bounds__(NT) k_reprecond(FsLayout L, Tiling tl, const f
     put_partial(part, SLOT_RR, a_rr, lds);
 }
 
-// The r.r of the last K2, for the host's reliable-update decision; also
-// clears `done` when asked (a recursion that met its target resumes).
+// The r.r of the refreshed residual into the state; clears `done` when
+// asked (a recursion that met its target resumes).
 __global__ void __launch_bounds__(NT) k_peek(const double* part, int nb, FsPcgState* S, int clear_done) {
     __shared__ double lds[4];
     double rr = get_total(part, SLOT_RR, nb, lds);
@@ -779,22 +892,11 @@ void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r3
     hipLaunchKernelGGL(k_peek, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, 1);
 }
 
-// FP32 iterations n0 .. n0+count-1 that end after the r-update of the last
-// one: [K3(n0-1) if pending] K1 K2 K3 ... K1 K2, then the r.r peek.
+// FP32 iterations n0 .. n0+count-1 and the loop tail of the last, which
+// leaves its r.r in the state for the host's reliable-update decision.
 void fs_pcg32_chunk(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const float* diag, float* x, FsPcgWork& w,
-                    FsPcgState* S, int n0, int count, int pending_k3, FsKernelProbe* probe) {
-    if (probe) {
-        probe->iter = count >= 2 ? n0 : -1;
-        probe->recorded = false;
-    }
-    if (pending_k3) one_t<float>(st, L, op, diag, x, w, S, 2, n0 - 1);
-    for (int n = n0; n < n0 + count; n++) {
-        one_probed<float>(st, L, op, diag, x, w, S, 0, n, probe);
-        one_probed<float>(st, L, op, diag, x, w, S, 1, n, probe);
-        if (n < n0 + count - 1) one_probed<float>(st, L, op, diag, x, w, S, 2, n, probe);
-    }
-    Tiling tl = make_tiling(L, vec_width(L, 1));
-    hipLaunchKernelGGL(k_peek, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, 0);
+                    FsPcgState* S, int n0, int count, FsKernelProbe* probe) {
+    iterations_t<float>(st, L, op, diag, x, w, S, n0, count, probe);
 }
 
 __global__ void __launch_bounds__(NT) k_add_corr(FsLayout L, Tiling tl, double* __restrict__ x,
@@ -803,7 +905,7 @@ __global__ void __launch_bounds__(NT) k_add_corr(FsLayout L, Tiling tl, double* 
         int i0, j, k0;
         tile_origin<1>(tl, it.t, i0, j, k0);
         if (i0 >= L.nx || j >= L.ny) continue;
-        int k1 = min(k0 + ZC, L.nz);
+        int k1 = min(k0 + tl.zc, L.nz);
         long c = L.idx(i0, j, k0);
         for (int k = k0; k < k1; k++, c += L.sy) x[c] += (double)e[c];
     }

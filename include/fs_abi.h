@@ -181,22 +181,23 @@ int  fs_apply_pressure_operator(fs_solver* s, const double* x, double* q);
 
 /* ---- measurement ---------------------------------------------------------- */
 
-/* Times the three PCG kernels of one iteration (SpMV + p.q, the x/r/z update
- * with r.z, the p update) on the operator of the last assembly: `reps`
- * back-to-back launches each, timed with HIP events on the handle's stream.
- * precision: FS_FP64 or FS_FP32_REFINE (needs a handle created with it).
- * avg_ms[0..2] receives the mean duration per launch.  No reference
+/* Times the PCG kernels of one iteration on the operator of the last
+ * assembly: [0] the p update fused with SpMV + p.q, [1] the x/r/z update with
+ * r.z and r.r; [2] is 0 (no separate p-update kernel).  `reps` back-to-back
+ * launches each, timed with HIP events on the handle's stream.  precision:
+ * FS_FP64 or FS_FP32_REFINE (needs a handle created with it).  avg_ms[0..2]
+ * receives the mean duration per launch.  No reference
  * counterpart: it replaces the wall-clock LOG timers around PCGMethod
  * (PCG_METHOD.cpp:6, LOG.h:191-338) for roofline accounting. */
 int  fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms);
 
-/* In-loop kernel sampling.  While enabled, ordinary solves bracket the three
- * PCG kernels of one iteration per launch batch (16 iterations) with HIP
- * events on the handle's stream; fs_get_kernel_samples returns, per kernel
- * (0 SpMV + p.q, 1 update, 2 p update) of the FP64 or the FP32 inner
- * iteration, the mean duration and the sample count since sampling was last
- * enabled (enabling clears them).  No reference counterpart: measurement
- * hooks beside fs_time_pcg_kernels. */
+/* In-loop kernel sampling.  While enabled, ordinary solves bracket the PCG
+ * kernels of one iteration per launch batch (16 iterations) with HIP events
+ * on the handle's stream; fs_get_kernel_samples returns, per kernel (as in
+ * fs_time_pcg_kernels) of the FP64 or the FP32 inner iteration, the mean
+ * duration and the sample count since sampling was last enabled (enabling
+ * clears them).  No reference counterpart: measurement hooks beside
+ * fs_time_pcg_kernels. */
 int  fs_set_kernel_sampling(fs_solver* s, int enable);
 int  fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* count);
 
