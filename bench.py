@@ -40,6 +40,10 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (profiler byte-counter calibration)")
+    ap.add_argument("--workload", default="timestep", choices=["timestep", "batch"],
+                    help="batch: 48 uneven scenes advanced together vs one at a time (own JSON line)")
+    ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--threads", type=int, default=0)
     return ap.parse_args()
 
 
@@ -137,11 +141,75 @@ def cpu_baseline(scene, n, iters_per_step, dt, threads):
     }
 
 
+def batch_main(a):
+    """Uneven scenes (edges 32..128, dam break or rising drop, density ratio
+    1.25..1000) advanced together through fs_batch_advance_frame, against
+    the same scenes advanced one at a time; FP32+refine projection to 1e-8."""
+    import torch
+
+    import fluid_simulator_backup_amd as fsa
+    from fluid_simulator_backup_amd import _abi, scenes
+    torch.cuda.set_device(0)
+    lib = fsa.load_library()
+    sc = scenes.batch_scenes(a.scenes, 3, precision=_abi.FS_FP32_REFINE, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
+                             rel_tolerance=1e-8, max_iteration=20000)
+    cells = sum(g.nx * g.ny * g.nz for _, _, _, _, g in sc)
+    batch = _abi.Batch(lib, [s[3] for s in sc], [s[4] for s in sc], 0, a.threads)
+    for (_, phi, vel, _, _), bs in zip(sc, batch.scenes):
+        bs.set_fields(phi, *vel)
+    dts = [bs.cfl_dt(0.5, 1) for bs in batch.scenes]
+    print(f"[bench] batch of {len(sc)} scenes, {cells / 1e6:.1f} M cells, warmup={a.warmup} steps={a.steps}",
+          file=sys.stderr, flush=True)
+    for _ in range(a.warmup):
+        batch.advance_frame(dts)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    its = 0
+    for _ in range(a.steps):
+        its += sum(st["pcg_iterations"] for st in batch.advance_frame(dts))
+    torch.cuda.synchronize()
+    t_batch = time.perf_counter() - t0
+    batch.close()
+    print(f"[bench] batch {t_batch * 1e3 / a.steps:.1f} ms/frame", file=sys.stderr, flush=True)
+    # the same scenes, one at a time
+    t_seq = 0.0
+    its_seq = 0
+    for (_, phi, vel, P, g), dt in zip(sc, dts):
+        s = _abi.Solver(lib, "fs_", P, g, 0)
+        s.set_fields(phi, *vel)
+        for _ in range(a.warmup):
+            s.advance_frame(dt)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(a.steps):
+            its_seq += s.advance_frame(dt)["pcg_iterations"]
+        torch.cuda.synchronize()
+        t_seq += time.perf_counter() - t0
+        s.close()
+    print(f"[bench] one at a time {t_seq * 1e3 / a.steps:.1f} ms/frame", file=sys.stderr, flush=True)
+    out = {
+        "metric": "aggregate cells/s, uneven scenes advanced together", "value": round(cells * a.steps / t_batch, 1),
+        "unit": "cells/s", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(t_batch * 1e3 / a.steps, 3), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere", "data": "synthetic",
+        "config": {"workload": f"{len(sc)} scenes, edges 32-128, dam break / rising drop, density ratio 1.25-1000, "
+                               f"FP32+refine to 1e-8, host threads {a.threads or min(len(sc), 16)}",
+                   "cells": cells},
+        "one_at_a_time_cells_per_s": round(cells * a.steps / t_seq, 1),
+        "speedup_vs_one_at_a_time": round(t_seq / t_batch, 3),
+        "pcg_iterations_per_step": its / a.steps, "pcg_iterations_per_step_one_at_a_time": its_seq / a.steps,
+    }
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU; --gpus must be 1 (no multi-GPU path)"}))
         sys.exit(2)
+    if a.workload == "batch":
+        batch_main(a)
+        return
     import torch
 
     import fluid_simulator_backup_amd as fsa

@@ -178,6 +178,14 @@ def bind(lib, prefix):
         lib.fs_get_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(_I)]
         lib.fs_last_error.argtypes = [vp]
         lib.fs_last_error.restype = C.c_char_p
+        lib.fs_batch_create.argtypes = [_I, C.POINTER(FsParams), C.POINTER(FsGrid), _I, _I, C.POINTER(vp)]
+        lib.fs_batch_destroy.argtypes = [vp]
+        lib.fs_batch_destroy.restype = None
+        lib.fs_batch_scene.argtypes = [vp, _I]
+        lib.fs_batch_scene.restype = vp
+        lib.fs_batch_advance_frame.argtypes = [vp, _PD, C.POINTER(FsStepStats), C.POINTER(_I)]
+        lib.fs_batch_last_error.argtypes = [vp]
+        lib.fs_batch_last_error.restype = C.c_char_p
     return lib
 
 
@@ -193,10 +201,14 @@ class Solver:
     Mirrors EULERIAN_FLUID_SOLVER_3D's call surface: fields in and out,
     one frame / one step, the phase functions and the CFL time step."""
 
-    def __init__(self, lib, prefix, params, grid, device=0):
+    def __init__(self, lib, prefix, params, grid, device=0, handle=None):
         self.lib, self.prefix = lib, prefix
         self.params, self.grid = params, grid
         self.nx, self.ny, self.nz = grid.nx, grid.ny, grid.nz
+        self.owned = handle is None
+        if handle is not None:  # a scene of a batch: the batch owns it
+            self.h = C.c_void_p(handle)
+            return
         h = C.c_void_p()
         args = [C.byref(params), C.byref(grid)] + ([device] if prefix == "fs_" else []) + [C.byref(h)]
         rc = self._f("create")(*args)
@@ -218,7 +230,8 @@ class Solver:
 
     def close(self):
         if getattr(self, "h", None):
-            self._f("destroy")(self.h)
+            if self.owned:
+                self._f("destroy")(self.h)
             self.h = None
 
     def __del__(self):
@@ -288,6 +301,47 @@ class Solver:
         q = np.empty_like(x)
         self._check(self._f("apply_pressure_operator")(self.h, _ptr(x), _ptr(q)), "apply_operator")
         return q
+
+
+class Batch:
+    """Many independent scenes advanced together (fs_batch_*)."""
+
+    def __init__(self, lib, params, grids, device=0, threads=0):
+        self.lib = lib
+        n = len(params)
+        self.params = (FsParams * n)(*params)
+        self.grids = (FsGrid * n)(*grids)
+        h = C.c_void_p()
+        rc = lib.fs_batch_create(n, self.params, self.grids, device, threads, C.byref(h))
+        if rc != FS_OK:
+            raise FsError(rc, "fs_batch_create failed")
+        self.h = h
+        self.scenes = [Solver(lib, "fs_", params[i], grids[i], handle=lib.fs_batch_scene(h, i)) for i in range(n)]
+
+    def advance_frame(self, dts, allow_not_converged=False):
+        n = len(self.scenes)
+        dt = np.ascontiguousarray(dts, dtype=np.float64)
+        st = (FsStepStats * n)()
+        rcs = (_I * n)()
+        rc = self.lib.fs_batch_advance_frame(self.h, _ptr(dt), st, rcs)
+        bad = [r for r in rcs if r != FS_OK and not (allow_not_converged and r == FS_ERR_NOT_CONVERGED)]
+        if bad:
+            m = self.lib.fs_batch_last_error(self.h)
+            raise FsError(bad[0], m.decode() if m else "batch")
+        return [x.as_dict() for x in st]
+
+    def close(self):
+        if getattr(self, "h", None):
+            for sc in self.scenes:
+                sc.h = None
+            self.lib.fs_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def time_pcg_kernels(solver, precision, reps):

@@ -14,7 +14,10 @@
 #include <cstdlib>
 #include <cstring>
 #include <string>
+#include <atomic>
+#include <thread>
 #include <utility>
+#include <vector>
 
 #include "../../include/fs_abi.h"
 #include "fs_kernels.h"
@@ -792,6 +795,86 @@ static int step_impl(EulerianFluidSolver3D& S, double dt, fs_step_stats* stats, 
     S.stats.step_ms = ms;
     if (stats) *stats = S.stats;
     return rc;
+}
+
+// ---- batches of independent scenes ------------------------------------------
+
+struct fs_batch {
+    std::vector<fs_solver*> scenes;
+    std::vector<long> order;  // scenes by decreasing cell count (largest first)
+    int device = 0;
+    int threads = 1;
+    std::string err;
+};
+
+int fs_batch_create(int n, const fs_params* params, const fs_grid* grids, int device, int threads, fs_batch** out) {
+    if (n <= 0 || !params || !grids || !out) return FS_ERR_BAD_ARG;
+    *out = nullptr;
+    auto* b = new fs_batch();
+    for (int i = 0; i < n; i++) {
+        fs_solver* s = nullptr;
+        int rc = fs_create(&params[i], &grids[i], device, &s);
+        if (rc != FS_OK) {
+            for (fs_solver* t : b->scenes) fs_destroy(t);
+            delete b;
+            return rc;
+        }
+        b->scenes.push_back(s);
+        b->order.push_back(i);
+    }
+    std::stable_sort(b->order.begin(), b->order.end(), [&](long a, long c) {
+        auto cells = [&](long k) { return (long)grids[k].nx * grids[k].ny * grids[k].nz; };
+        return cells(a) > cells(c);
+    });
+    (void)hipGetDevice(&b->device);
+    b->threads = threads > 0 ? std::min(threads, n) : std::min(n, 16);
+    *out = b;
+    return FS_OK;
+}
+
+void fs_batch_destroy(fs_batch* b) {
+    if (!b) return;
+    for (fs_solver* s : b->scenes) fs_destroy(s);
+    delete b;
+}
+
+fs_solver* fs_batch_scene(fs_batch* b, int i) {
+    if (!b || i < 0 || i >= (int)b->scenes.size()) return nullptr;
+    return b->scenes[i];
+}
+
+const char* fs_batch_last_error(const fs_batch* b) { return b ? b->err.c_str() : "null handle"; }
+
+// Every scene advances one frame.  Host threads take scenes from a shared
+// counter, largest first; each scene's work runs on its own stream, so the
+// kernels of small scenes overlap on the device.
+int fs_batch_advance_frame(fs_batch* b, const double* dt, fs_step_stats* stats, int* rcs) {
+    if (!b || !dt) return FS_ERR_BAD_ARG;
+    const int n = (int)b->scenes.size();
+    std::vector<int> rc(n, FS_OK);
+    std::atomic<int> next{0};
+    auto work = [&]() {
+        if (hipSetDevice(b->device) != hipSuccess) return;
+        for (int q = next++; q < n; q = next++) {
+            int i = (int)b->order[q];
+            rc[i] = guarded(b->scenes[i], [&](EulerianFluidSolver3D& S) {
+                return step_impl(S, dt[i], stats ? &stats[i] : nullptr, true);
+            });
+        }
+    };
+    std::vector<std::thread> pool;
+    for (int t = 1; t < b->threads; t++) pool.emplace_back(work);
+    work();
+    for (auto& t : pool) t.join();
+    int first = FS_OK;
+    for (int i = 0; i < n; i++) {
+        if (rcs) rcs[i] = rc[i];
+        if (rc[i] != FS_OK && first == FS_OK) {
+            first = rc[i];
+            b->err = "scene " + std::to_string(i) + ": " + b->scenes[i]->err;
+        }
+    }
+    return first;
 }
 
 int fs_advance_frame(fs_solver* s, double dt, fs_step_stats* stats) {

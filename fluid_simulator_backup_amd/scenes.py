@@ -120,3 +120,22 @@ def make(name, n, seed=None, **param_overrides):
     phi, vel, pkw = fn(n) if seed is None else fn(n, seed)
     pkw.update(param_overrides)
     return phi, vel, _abi.default_params(**pkw), _abi.unit_cube_grid(n)
+
+
+def batch_scenes(count=48, seed=3, edges=(32, 48, 64, 96, 128), **param_overrides):
+    """Uneven scenes for batched serving: per scene an edge from `edges`, a
+    dam break or a rising drop by coin flip, a density ratio log-uniform in
+    [1.25, 1000] (heavy phase 1000 kg/m^3), scene seed 1000*seed + s.  Returns
+    a list of (name, phi, (u, v, w), FsParams, FsGrid)."""
+    rng = SplitMix64(seed)
+    out = []
+    for s in range(count):
+        n = edges[min(int(rng.uniform() * len(edges)), len(edges) - 1)]
+        name = "dam_break" if rng.uniform() < 0.5 else "rising_drop"
+        ratio = float(np.exp(np.log(1.25) + (np.log(1000.0) - np.log(1.25)) * rng.uniform()))
+        phi, vel, pkw = SCENES[name](n, 1000 * seed + s)
+        pkw["water_density"] = 1000.0
+        pkw["air_density"] = 1000.0 / ratio
+        pkw.update(param_overrides)
+        out.append((name, phi, vel, _abi.default_params(**pkw), _abi.unit_cube_grid(n)))
+    return out

@@ -201,6 +201,24 @@ int  fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms);
 int  fs_set_kernel_sampling(fs_solver* s, int enable);
 int  fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* count);
 
+/* ---- batches of independent scenes ---------------------------------------
+ * No reference counterpart: the reference advances one world per process
+ * (SIMULATION_WORLD::AdvanceOneFrame, SIMULATION_WORLD.h:320-330).  A batch
+ * serves many uneven scenes from one process: scene i is a full solver
+ * (params[i], grids[i]) with its own stream; fs_batch_advance_frame runs one
+ * fs_advance_frame on every scene, host threads (threads <= 0: up to 16)
+ * taking scenes largest first, so the small scenes' kernels overlap on the
+ * GPU.  Results are those of each scene advanced alone.  fs_batch_scene
+ * returns scene i's handle (owned by the batch) for fields and phases.
+ * rcs (may be NULL) receives each scene's return code; the function returns
+ * the first failure or FS_OK. */
+typedef struct fs_batch fs_batch;
+int  fs_batch_create(int n, const fs_params* params, const fs_grid* grids, int device, int threads, fs_batch** out);
+void fs_batch_destroy(fs_batch* b);
+fs_solver* fs_batch_scene(fs_batch* b, int i);
+int  fs_batch_advance_frame(fs_batch* b, const double* dt, fs_step_stats* stats, int* rcs);
+const char* fs_batch_last_error(const fs_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

@@ -195,3 +195,31 @@ def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
         assert abs(a - b) <= 2, its
     for c, e in errs.items():
         assert e < 1e-4, (c, e)
+
+
+def test_batch_matches_scenes_advanced_alone(hip_lib):
+    """A batch of uneven scenes (FP32+refine projection, one host thread per
+    scene stream) gives, bit for bit, what each scene gives advanced alone:
+    every kernel and reduction is deterministic and no state is shared."""
+    from fluid_simulator_backup_amd import scenes
+    sc = scenes.batch_scenes(6, seed=5, edges=(12, 16, 20, 24), precision=_abi.FS_FP32_REFINE,
+                             stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8, max_iteration=5000)
+    batch = _abi.Batch(hip_lib, [s[3] for s in sc], [s[4] for s in sc], 0, 4)
+    alone = []
+    for (name, phi, vel, P, g), bs in zip(sc, batch.scenes):
+        bs.set_fields(phi, *vel)
+        s = _abi.Solver(hip_lib, "fs_", P, g, 0)
+        s.set_fields(phi, *vel)
+        alone.append(s)
+    dts = [s.cfl_dt(0.5, 1) for s in alone]
+    for _ in range(2):
+        st_b = batch.advance_frame(dts)
+        st_a = [s.advance_frame(dt) for s, dt in zip(alone, dts)]
+        for a, b in zip(st_a, st_b):
+            assert a["pcg_iterations"] == b["pcg_iterations"]
+    for s, bs in zip(alone, batch.scenes):
+        fa, fb = s.get_fields(), bs.get_fields()
+        for c in ("phi", "u", "v", "w", "p"):
+            assert np.array_equal(fa[c], fb[c]), c
+        s.close()
+    batch.close()
