@@ -178,6 +178,10 @@ def bind(lib, prefix):
         lib.fs_get_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(_I)]
         lib.fs_last_error.argtypes = [vp]
         lib.fs_last_error.restype = C.c_char_p
+        lib.fs_adjoint_projection.argtypes = [vp, _D, _PD, _PD, _PD, C.POINTER(FsStepStats)]
+        lib.fs_adjoint_sourcing.argtypes = [vp, _D, _PD, _PD, _PD]
+        lib.fs_weno5_transport.argtypes = [vp, _D, _PD, _PD, _PD, _PD, _PD]
+        lib.fs_adjoint_weno5_transport.argtypes = [vp, _D] + [_PD] * 9
         lib.fs_batch_create.argtypes = [_I, C.POINTER(FsParams), C.POINTER(FsGrid), _I, _I, C.POINTER(vp)]
         lib.fs_batch_destroy.argtypes = [vp]
         lib.fs_batch_destroy.restype = None
@@ -294,6 +298,36 @@ class Solver:
         self._check(self._f("assemble_pressure_system")(self.h, dt, *[_ptr(out[c]) for c in
                                                                       ("diag", "bx", "by", "bz", "rhs")]),
                     "assemble")
+        return out
+
+    def adjoint_projection(self, dt, lu, lv, lw):
+        """lam -> P^T lam for the projection at the current level set."""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (lu, lv, lw)]
+        st = FsStepStats()
+        self._check(self.lib.fs_adjoint_projection(self.h, dt, *[_ptr(a) for a in out], C.byref(st)),
+                    "adjoint_projection")
+        return out, st.as_dict()
+
+    def adjoint_sourcing(self, dt, lu, lv, lw):
+        """lam -> V^T lam for the explicit viscous update at the current level set."""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (lu, lv, lw)]
+        self._check(self.lib.fs_adjoint_sourcing(self.h, dt, *[_ptr(a) for a in out]), "adjoint_sourcing")
+        return out
+
+    def weno5_transport(self, dt, phi, u, v, w):
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        phi, u, v, w = map(c, (phi, u, v, w))
+        out = np.empty_like(phi)
+        self._check(self.lib.fs_weno5_transport(self.h, dt, _ptr(phi), _ptr(u), _ptr(v), _ptr(w), _ptr(out)),
+                    "weno5_transport")
+        return out
+
+    def adjoint_weno5_transport(self, dt, phi, u, v, w, lam):
+        c = lambda a: np.ascontiguousarray(a, dtype=np.float64)
+        phi, u, v, w, lam = map(c, (phi, u, v, w, lam))
+        out = [np.empty_like(phi), np.empty_like(u), np.empty_like(v), np.empty_like(w)]
+        self._check(self.lib.fs_adjoint_weno5_transport(self.h, dt, _ptr(phi), _ptr(u), _ptr(v), _ptr(w), _ptr(lam),
+                                                        *[_ptr(a) for a in out]), "adjoint_weno5_transport")
         return out
 
     def apply_operator(self, x):

@@ -598,6 +598,74 @@ class EulerianFluidSolver3D {
         return rc;
     }
 
+    // Adjoint of Projection at the current interface (phi held fixed):
+    // u' = u + Gh A^-1 M B u, so lam <- lam + B^T M A^-1 Gh^T lam on the face
+    // fields lu, lv, lw (device,
+    // zero ghosts).  The operator is rebuilt from phi exactly as Projection
+    // builds it, without the persistent ghost copies the forward refreshes;
+    // the solve is the FP64 Jacobi-PCG with the forward's stop rule.
+    int AdjointProjection(double dt, double* lu, double* lv, double* lw) {
+        double light = P.air_bubble_rising ? P.air_density : P.water_density;
+        double heavy = P.air_bubble_rising ? P.water_density : P.air_density;
+        fs_density(st, phi, rho, irho, L, light, heavy);
+        fs_fill_const(st, irho, irho, L, box(0), 1);
+        fs_fill_value(st, div, L.total, 0.0);
+        fs_assemble(st, phi, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
+        assembled = true;
+        // rhs = G^T lam (the faces the forward corrects)
+        fs_fill_value(st, rhs, L.total, 0.0);
+        if (P.schedule == FS_SCHEDULE_FIXED) {
+            fs_adjoint_gradient(st, 1, lu, phi, rho, rhs, L, mat, cell, dt);
+            fs_adjoint_gradient(st, 2, lv, phi, rho, rhs, L, mat, cell, dt);
+        }
+        fs_adjoint_gradient(st, 3, lw, phi, rho, rhs, L, mat, cell, dt);
+        // y = A^-1 rhs (A symmetric), then M y
+        fs_fill_value(st, tmp, L.total, 0.0);
+        int its = run_pcg64(P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, tmp);
+        bool converged = hS->done == 1;
+        fs_remove_mean(st, tmp, scratch, L);
+        // rhs holds Gh^T lam with Gh = dU/dp (the minus sign of the update
+        // included), so lam <- lam + B^T M A^-1 Gh^T lam with B = -D/dt
+        fs_adjoint_divergence(st, lu, lv, lw, tmp, L, cell, -1 / dt);
+        check_launch();
+        sync();
+        stats.pcg_iterations = its;
+        return converged ? FS_OK : FS_ERR_NOT_CONVERGED;
+    }
+
+    // Adjoint of the explicit Sourcing at the current level set: the x, y, z
+    // updates run in that order, each reading the components already
+    // updated, so their transposes run z, y, x; each scatters from a copy of
+    // its component's adjoint.
+    void AdjointSourcing(double dt, double* lu, double* lv, double* lw) {
+        if (!P.use_delta_function_formulation) return;
+        if (P.semi_implicit_viscosity)
+            throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the semi-implicit viscosity is not built"};
+        double* lam[3] = {lu, lv, lw};
+        for (int comp = 3; comp >= 1; comp--) {
+            fs_copy_box(st, tmp, lam[comp - 1], L, box(comp));
+            fs_adjoint_viscosity(st, comp, tmp, lu, lv, lw, phi, L, mat, cell, dt);
+        }
+        check_launch();
+        sync();
+    }
+
+    // host adjoint fields in, through the spares (their ghosts hold scatter
+    // leftovers afterwards and are cleared: the forward relies on zero ghosts)
+    template <typename F>
+    int WithAdjointFields(double* hu, double* hv, double* hw, F&& f) {
+        put(spare_u, hu, 1);
+        put(spare_v, hv, 2);
+        put(spare_w, hw, 3);
+        int rc = f(spare_u, spare_v, spare_w);
+        get(spare_u, hu, 1);
+        get(spare_v, hv, 2);
+        get(spare_w, hw, 3);
+        for (double* a : {spare_u, spare_v, spare_w}) fs_fill_value(st, a, L.total, 0.0);
+        sync();
+        return rc;
+    }
+
     // EULERIAN_FLUID_SOLVER_3D::Reinitialization, :2198-2225
     void Reinitialization() {
         if (!P.sussmanstyle_reinitialization) return;
@@ -936,6 +1004,74 @@ int fs_projection(fs_solver* s, double dt, fs_step_stats* stats) {
         int rc = S.Projection(dt);
         if (stats) *stats = S.stats;
         return rc;
+    });
+}
+
+int fs_adjoint_projection(fs_solver* s, double dt, double* lu, double* lv, double* lw, fs_step_stats* stats) {
+    if (!(dt > 0) || !lu || !lv || !lw) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.stats = fs_step_stats{};
+        int rc = S.WithAdjointFields(lu, lv, lw, [&](double* a, double* b, double* c) {
+            return S.AdjointProjection(dt, a, b, c);
+        });
+        if (stats) *stats = S.stats;
+        return rc;
+    });
+}
+
+// The stage's inputs and outputs travel through scratch fields that no
+// later phase reads before rewriting them: tmp (phi), the spares (u, v, w;
+// cleared afterwards), div (output or lam), rhs and bx/by/bz (adjoints; the
+// next projection reassembles them).
+int fs_weno5_transport(fs_solver* s, double dt, const double* phi, const double* u, const double* v, const double* w,
+                       double* out) {
+    if (!(dt > 0) || !phi || !u || !v || !w || !out) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.put(S.tmp, phi, 0);
+        S.put(S.spare_u, u, 1);
+        S.put(S.spare_v, v, 2);
+        S.put(S.spare_w, w, 3);
+        fs_weno5_transport(S.st, S.tmp, S.spare_u, S.spare_v, S.spare_w, S.div, S.L, S.cell, S.P.levelset_epsilon,
+                           dt);
+        S.check_launch();
+        S.get(S.div, out, 0);
+        for (double* a : {S.spare_u, S.spare_v, S.spare_w}) fs_fill_value(S.st, a, S.L.total, 0.0);
+        S.sync();
+        return FS_OK;
+    });
+}
+
+int fs_adjoint_weno5_transport(fs_solver* s, double dt, const double* phi, const double* u, const double* v,
+                               const double* w, const double* lam, double* lphi, double* lu, double* lv, double* lw) {
+    if (!(dt > 0) || !phi || !u || !v || !w || !lam || !lphi || !lu || !lv || !lw) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.assembled = false;
+        S.put(S.tmp, phi, 0);
+        S.put(S.spare_u, u, 1);
+        S.put(S.spare_v, v, 2);
+        S.put(S.spare_w, w, 3);
+        S.put(S.div, lam, 0);
+        for (double* a : {S.rhs, S.bx, S.by, S.bz}) fs_fill_value(S.st, a, S.L.total, 0.0);
+        fs_adjoint_weno5_transport(S.st, S.tmp, S.spare_u, S.spare_v, S.spare_w, S.div, S.rhs, S.bx, S.by, S.bz, S.L,
+                                   S.cell, S.P.levelset_epsilon, dt);
+        S.check_launch();
+        S.get(S.rhs, lphi, 0);
+        S.get(S.bx, lu, 1);
+        S.get(S.by, lv, 2);
+        S.get(S.bz, lw, 3);
+        for (double* a : {S.spare_u, S.spare_v, S.spare_w}) fs_fill_value(S.st, a, S.L.total, 0.0);
+        S.sync();
+        return FS_OK;
+    });
+}
+
+int fs_adjoint_sourcing(fs_solver* s, double dt, double* lu, double* lv, double* lw) {
+    if (!(dt > 0) || !lu || !lv || !lw) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        return S.WithAdjointFields(lu, lv, lw, [&](double* a, double* b, double* c) {
+            S.AdjointSourcing(dt, a, b, c);
+            return FS_OK;
+        });
     });
 }
 

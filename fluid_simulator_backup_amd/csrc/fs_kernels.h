@@ -25,10 +25,23 @@ void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const doub
 void fs_eno3_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double dt);
 
+// One WENO5 level-set stage with the level set extended by constant
+// extrapolation and zero MAC ghosts, and its adjoint (accumulating).
+void fs_weno5_transport(hipStream_t st, const double* phi, const double* u, const double* v, const double* w,
+                        double* out, const FsLayout& L, const FsGridConst& gc, double eps, double dt);
+void fs_adjoint_weno5_transport(hipStream_t st, const double* phi, const double* u, const double* v, const double* w,
+                                const double* lam, double* lphi, double* lu, double* lv, double* lw, const FsLayout& L,
+                                const FsGridConst& gc, double eps, double dt);
+
 // ---- forces ---------------------------------------------------------------
 void fs_gravity(hipStream_t st, double* u, double* v, double* w, const FsLayout& L, double gx, double gy, double gz);
 void fs_viscosity(hipStream_t st, int comp, double* dst, const double* U, const double* V, const double* W,
                   const double* phi, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt);
+
+// lu, lv, lw += (the transpose of comp's explicit viscous update) src
+void fs_adjoint_viscosity(hipStream_t st, int comp, const double* src, double* lu, double* lv, double* lw,
+                          const double* phi, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
+                          double dt);
 
 // ---- projection -----------------------------------------------------------
 // Semi-implicit viscosity rows of MAC component comp (1..3): off-diagonal
@@ -49,6 +62,12 @@ void fs_remove_mean(hipStream_t st, double* rhs, double* scratch, const FsLayout
 void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,
                         const double* rho, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
                         double dt, unsigned long long* vmax_bits);
+// Adjoint pieces of the projection at fixed phi: g += G^T lam (comp's faces
+// to cells) and lam += scale * D^T z (cells to the faces of u, v, w).
+void fs_adjoint_gradient(hipStream_t st, int comp, const double* lam, const double* phi, const double* rho, double* g,
+                         const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt);
+void fs_adjoint_divergence(hipStream_t st, double* lu, double* lv, double* lw, const double* z, const FsLayout& L,
+                           const FsGridConst& cell, double scale);
 void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, unsigned long long* bits);
 
 // ---- reinitialization -----------------------------------------------------

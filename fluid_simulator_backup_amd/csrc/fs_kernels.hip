@@ -192,6 +192,154 @@ __global__ void __launch_bounds__(BX* BY) k_weno5_stage(double* __restrict__ rho
 
 static const FsGridConst& gc_dummy(const FsGridConst& g) { return g; }
 
+// ---- WENO5 transport and its adjoint -----------------------------------------
+// One level-set stage as k_weno5_stage computes it, on a level set whose
+// ghosts are its constant extrapolation (read by clamping the index) and
+// MAC velocities with zero ghosts: phi' = phi - dt (u X + v Y + w Z), X the
+// upwind WENO5 derivative.  The adjoint differentiates the WENO5 weights
+// (smoothness indicators included); the upwind choice is piecewise constant.
+
+// Gradient of the one-sided WENO5 value D(v1..v5) (fs_weno5's dmo with
+// v = m, dpo with v = p) with respect to v.
+__device__ __forceinline__ void weno5_grad(const double v[5], double eps, double g[5]) {
+    const double k13 = (double)13 / 12, q4 = (double)1 / 4;
+    double e1 = v[0] - 2 * v[1] + v[2], f1 = v[0] - 4 * v[1] + 3 * v[2];
+    double e2 = v[1] - 2 * v[2] + v[3], f2 = v[1] - v[3];
+    double e3 = v[2] - 2 * v[3] + v[4], f3 = 3 * v[2] - 4 * v[3] + v[4];
+    double s[3] = {k13 * e1 * e1 + q4 * f1 * f1, k13 * e2 * e2 + q4 * f2 * f2, k13 * e3 * e3 + q4 * f3 * f3};
+    const double d[3] = {(double)1 / 10, (double)6 / 10, (double)3 / 10};
+    double a[3], S = 0;
+    for (int k = 0; k < 3; k++) {
+        a[k] = d[k] / ((eps + s[k]) * (eps + s[k]));
+        S += a[k];
+    }
+    double q[3] = {v[0] / 3 - v[1] * 7 / 6 + v[2] * 11 / 6, -v[1] / 6 + v[2] * 5 / 6 + v[3] / 3,
+                   v[2] / 3 + v[3] * 5 / 6 - v[4] / 6};
+    double w[3] = {a[0] / S, a[1] / S, a[2] / S};
+    double D = w[0] * q[0] + w[1] * q[1] + w[2] * q[2];
+    // dD/ds_k = -2 w_k (q_k - D) / (eps + s_k)
+    double ds[3];
+    for (int k = 0; k < 3; k++) ds[k] = -2 * w[k] * (q[k] - D) / (eps + s[k]);
+    // w_k dq_k/dv
+    g[0] = w[0] / 3;
+    g[1] = -w[0] * 7 / 6 - w[1] / 6;
+    g[2] = w[0] * 11 / 6 + w[1] * 5 / 6 + w[2] / 3;
+    g[3] = w[1] / 3 + w[2] * 5 / 6;
+    g[4] = -w[2] / 6;
+    // ds_k/dv
+    double a1 = ds[0] * 2 * k13 * e1, b1 = ds[0] * 2 * q4 * f1;
+    g[0] += a1 + b1;
+    g[1] += -2 * a1 - 4 * b1;
+    g[2] += a1 + 3 * b1;
+    double a2 = ds[1] * 2 * k13 * e2, b2 = ds[1] * 2 * q4 * f2;
+    g[1] += a2 + b2;
+    g[2] += -2 * a2;
+    g[3] += a2 - b2;
+    double a3 = ds[2] * 2 * k13 * e3, b3 = ds[2] * 2 * q4 * f3;
+    g[2] += a3 + 3 * b3;
+    g[3] += -2 * a3 - 4 * b3;
+    g[4] += a3 + b3;
+}
+
+// the seven samples of the extended level set along one axis
+__device__ __forceinline__ void gather7_clamped(const double* __restrict__ phi, const FsLayout& L, int i, int j, int k,
+                                                int axis, double f[7], long cl[7]) {
+    for (int o = -3; o <= 3; o++) {
+        int ii = i, jj = j, kk = k;
+        if (axis == 0) ii = min(max(i + o, 0), L.nx - 1);
+        else if (axis == 1) jj = min(max(j + o, 0), L.ny - 1);
+        else kk = min(max(k + o, 0), L.nz - 1);
+        cl[o + 3] = L.idx(ii, jj, kk);
+        f[o + 3] = phi[cl[o + 3]];
+    }
+}
+
+__global__ void __launch_bounds__(BX* BY) k_weno5_transport(const double* __restrict__ phi, const double* __restrict__ u,
+                                                            const double* __restrict__ v, const double* __restrict__ w,
+                                                            double* __restrict__ out, FsLayout L, double ix, double iy,
+                                                            double iz, double eps, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k), cl[7];
+    double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+    gather7_clamped(phi, L, i, j, k, 0, f, cl);
+    fs_weno5(f + 3, ix, eps, mx, px);
+    gather7_clamped(phi, L, i, j, k, 1, f, cl);
+    fs_weno5(f + 3, iy, eps, my, py);
+    gather7_clamped(phi, L, i, j, k, 2, f, cl);
+    fs_weno5(f + 3, iz, eps, mz, pz);
+    vel_at(u, v, w, L, 0, i, j, k, uu, vv, ww);
+    out[c] = upwind(phi[c], dt, uu, vv, ww, mx, px, my, py, mz, pz);
+}
+
+// lphi += (d phi'/d phi)^T lam, lu/lv/lw += (d phi'/d velocity)^T lam.
+__global__ void __launch_bounds__(BX* BY) k_adj_weno5_transport(const double* __restrict__ phi,
+                                                                const double* __restrict__ u, const double* __restrict__ v,
+                                                                const double* __restrict__ w,
+                                                                const double* __restrict__ lam, double* lphi, double* lu,
+                                                                double* lv, double* lw, FsLayout L, double ix, double iy,
+                                                                double iz, double eps, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    double l = lam[c];
+    atomicAdd(&lphi[c], l);  // phi' = phi - ...
+    if (l == 0) return;
+    double uu, vv, ww;
+    vel_at(u, v, w, L, 0, i, j, k, uu, vv, ww);
+    const double vel[3] = {uu, vv, ww}, inv[3] = {ix, iy, iz};
+    const long st[3] = {1, L.sx, L.sy};
+    double* lface[3] = {lu, lv, lw};
+    for (int axis = 0; axis < 3; axis++) {
+        double f[7];
+        long cl[7];
+        gather7_clamped(phi, L, i, j, k, axis, f, cl);
+        double dm, dp;
+        fs_weno5(f + 3, inv[axis], eps, dm, dp);
+        bool minus = vel[axis] > 0;
+        // the selected one-sided derivative and its five differences
+        double d[5], g[5];
+        if (minus) {
+            for (int q = 0; q < 5; q++) d[q] = inv[axis] * (f[q + 1] - f[q]);  // m1..m5 over f[0..5]
+        } else {
+            d[0] = inv[axis] * (f[6] - f[5]);  // p1
+            for (int q = 1; q < 5; q++) d[q] = inv[axis] * (f[6 - q] - f[5 - q]);  // p2..p5 = m5..m2
+        }
+        weno5_grad(d, eps, g);
+        double sc = -dt * vel[axis] * l * inv[axis];
+        if (minus) {
+            for (int q = 0; q < 5; q++) {  // m_{q+1} = inv (f[q+1] - f[q])
+                atomicAdd(&lphi[cl[q + 1]], sc * g[q]);
+                atomicAdd(&lphi[cl[q]], -sc * g[q]);
+            }
+        } else {
+            atomicAdd(&lphi[cl[6]], sc * g[0]);
+            atomicAdd(&lphi[cl[5]], -sc * g[0]);
+            for (int q = 1; q < 5; q++) {
+                atomicAdd(&lphi[cl[6 - q]], sc * g[q]);
+                atomicAdd(&lphi[cl[5 - q]], -sc * g[q]);
+            }
+        }
+        // velocity: the cell value is the mean of the two faces along the axis
+        double lvel = -dt * l * (minus ? dm : dp);
+        atomicAdd(&lface[axis][c], (double)0.5 * lvel);
+        atomicAdd(&lface[axis][c + st[axis]], (double)0.5 * lvel);
+    }
+}
+
+void fs_weno5_transport(hipStream_t st, const double* phi, const double* u, const double* v, const double* w,
+                        double* out, const FsLayout& L, const FsGridConst& gc, double eps, double dt) {
+    hipLaunchKernelGGL(k_weno5_transport, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, u, v, w, out, L, gc.x.inv,
+                       gc.y.inv, gc.z.inv, eps, dt);
+}
+
+void fs_adjoint_weno5_transport(hipStream_t st, const double* phi, const double* u, const double* v, const double* w,
+                                const double* lam, double* lphi, double* lu, double* lv, double* lw, const FsLayout& L,
+                                const FsGridConst& gc, double eps, double dt) {
+    hipLaunchKernelGGL(k_adj_weno5_transport, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, u, v, w, lam, lphi,
+                       lu, lv, lw, L, gc.x.inv, gc.y.inv, gc.z.inv, eps, dt);
+}
+
 void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
                     const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt) {
     FsBox b = fs_box(L, comp);
@@ -317,6 +465,113 @@ void fs_viscosity(hipStream_t st, int comp, double* dst, const double* U, const 
     FsBox b = fs_box(L, comp);
     double eps = 1.5 * cell.x.d;  // epsilon_for_mollification, EULERIAN_FLUID_SOLVER_3D.h:393
     hipLaunchKernelGGL(k_viscosity, grid_for(b), dim3(BX, BY), 0, st, comp, dst, U, V, W, phi, L, b, m, eps,
+                       cell.x.inv, cell.y.inv, cell.z.inv, dt);
+}
+
+// Transpose of one component's explicit viscous update (k_viscosity): the
+// update X_comp += sum of coefficients times U, V, W values is linear for a
+// fixed phi, so its adjoint scatters coefficient * lam_comp(face) into the
+// adjoints of every value the face read (atomics; ghost targets collect what
+// belongs to the constant zero ghosts and are never read back).  src is the
+// adjoint of the component after the update; lu, lv, lw accumulate.
+__global__ void __launch_bounds__(BX* BY) k_adj_viscosity(int comp, const double* __restrict__ src, double* lu,
+                                                          double* lv, double* lw, const double* __restrict__ f,
+                                                          FsLayout L, FsBox b, FsMaterial m, double eps, double odx,
+                                                          double ody, double odz, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    const long c = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double l = src[c];
+    if (l == 0) return;
+#define MU(x) m.viscosity(fs_heaviside_visc((x), eps))
+#define ADD(arr, off, w) atomicAdd(&(arr)[c + (off)], (w) * l)
+    if (comp == 1) {
+        double lh = (double)0.5 * (f[c - 1] + f[c]);
+        double lcu = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 + sx] + f[c + sx]);
+        double lcd = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 - sx] + f[c - sx]);
+        double lct = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 + sy] + f[c + sy]);
+        double lcb = (double)0.25 * (f[c - 1] + f[c] + f[c - 1 - sy] + f[c - sy]);
+        double coef = dt * ((double)1 / m.density(fs_heaviside_visc(lh, eps)));
+        double mcu = MU(lcu), mcd = MU(lcd), mr = MU(f[c]), mc = MU(f[c - 1]), mct = MU(lct), mcb = MU(lcb);
+        double a1 = 2 * coef * odx * mr * odx, a2 = 2 * coef * odx * mc * ody;
+        double bu = coef * ody * mcu, bd = coef * ody * mcd, tt = coef * odz * mct, tb = coef * odz * mcb;
+        ADD(lu, 1, a1);
+        ADD(lu, 0, -a1 - a2 - bu * ody - bd * ody - tt * odz - tb * odz);
+        ADD(lu, -1, a2);
+        ADD(lu, sx, bu * ody);
+        ADD(lu, -sx, bd * ody);
+        ADD(lu, sy, tt * odz);
+        ADD(lu, -sy, tb * odz);
+        ADD(lv, sx, bu * odx);
+        ADD(lv, -1 + sx, -bu * odx);
+        ADD(lv, 0, -bd * odx);
+        ADD(lv, -1, bd * odx);
+        ADD(lw, sy, tt * odx);
+        ADD(lw, -1 + sy, -tt * odx);
+        ADD(lw, 0, -tb * odx);
+        ADD(lw, -1, tb * odx);
+    } else if (comp == 2) {
+        double lh = (double)0.5 * (f[c - sx] + f[c]);
+        double lcu = (double)0.25 * (f[c - sx] + f[c + 1 - sx] + f[c] + f[c + 1]);
+        double lcl = (double)0.25 * (f[c - 1 - sx] + f[c - sx] + f[c - 1] + f[c]);
+        double lct = (double)0.25 * (f[c - sx + sy] + f[c - sx] + f[c + sy] + f[c]);
+        double lcb = (double)0.25 * (f[c - sx - sy] + f[c - sx] + f[c - sy] + f[c]);
+        double coef = dt * ((double)1 / m.density(fs_heaviside_visc(lh, eps)));
+        double mcu = MU(lcu), mcl = MU(lcl), mu_ = MU(f[c]), mc = MU(f[c - sx]), mct = MU(lct), mcb = MU(lcb);
+        double au = coef * odx * mcu, al = coef * odx * mcl;
+        double s1 = 2 * coef * ody * mu_ * ody, s2 = 2 * coef * ody * mc * ody;
+        double tt = coef * odz * mct, tb = coef * odz * mcb;
+        ADD(lu, 1, au * ody);
+        ADD(lu, 1 - sx, -au * ody);
+        ADD(lu, 0, -al * ody);
+        ADD(lu, -sx, al * ody);
+        ADD(lv, 1, au * odx);
+        ADD(lv, 0, -au * odx - al * odx - s1 - s2 - tt * odz - tb * odz);
+        ADD(lv, -1, al * odx);
+        ADD(lv, sx, s1);
+        ADD(lv, -sx, s2);
+        ADD(lv, sy, tt * odz);
+        ADD(lv, -sy, tb * odz);
+        ADD(lw, 0, tt * ody);
+        ADD(lw, -sx, -tt * ody);
+        ADD(lw, -sy, -tb * ody);
+        ADD(lw, -sx - sy, tb * ody);
+    } else {
+        double lh = (double)0.5 * (f[c - sy] + f[c]);
+        double lct = (double)0.25 * (f[c - sy] + f[c + 1 - sy] + f[c] + f[c + 1]);
+        double lcb = (double)0.25 * (f[c - sy] + f[c - 1 - sy] + f[c] + f[c - 1]);
+        double lcu = (double)0.25 * (f[c - sy] + f[c + sx - sy] + f[c] + f[c + sx]);
+        double lcd = (double)0.25 * (f[c - sy] + f[c - sx - sy] + f[c] + f[c - sx]);
+        double coef = dt * ((double)1 / m.density(fs_heaviside_visc(lh, eps)));
+        double mct = MU(lct), mcb = MU(lcb), mcu = MU(lcu), mcd = MU(lcd), mt = MU(f[c]), mc = MU(f[c - sy]);
+        double at = coef * odx * mct, ab = coef * odx * mcb, bu = coef * ody * mcu, bd = coef * ody * mcd;
+        double s1 = 2 * coef * odz * mt * odz, s2 = 2 * coef * odz * mc * odz;
+        ADD(lu, 0, at * odz);
+        ADD(lu, -sy, -at * odz);
+        ADD(lu, -1, -ab * odz);
+        ADD(lu, -1 - sy, ab * odz);
+        ADD(lv, 0, bu * odz);
+        ADD(lv, -sy, -bu * odz);
+        ADD(lv, -sx, -bd * odz);
+        ADD(lv, -sx - sy, bd * odz);
+        ADD(lw, 1, at * odx);
+        ADD(lw, 0, -at * odx - ab * odx - bu * ody - bd * ody - s1 - s2);
+        ADD(lw, -1, ab * odx);
+        ADD(lw, sx, bu * ody);
+        ADD(lw, -sx, bd * ody);
+        ADD(lw, sy, s1);
+        ADD(lw, -sy, s2);
+    }
+#undef ADD
+#undef MU
+}
+
+void fs_adjoint_viscosity(hipStream_t st, int comp, const double* src, double* lu, double* lv, double* lw,
+                          const double* phi, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
+                          double dt) {
+    FsBox b = fs_box(L, comp);
+    double eps = 1.5 * cell.x.d;  // epsilon_for_mollification, EULERIAN_FLUID_SOLVER_3D.h:393
+    hipLaunchKernelGGL(k_adj_viscosity, grid_for(b), dim3(BX, BY), 0, st, comp, src, lu, lv, lw, phi, L, b, m, eps,
                        cell.x.inv, cell.y.inv, cell.z.inv, dt);
 }
 
@@ -661,6 +916,94 @@ void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const 
     dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
     hipLaunchKernelGGL(k_gradient, g, dim3(BX, BY), 0, st, comp, apply, vel, p, phi, rho, L, b, m,
                        (double)1.5 * dg, od, dt, vmax_bits);
+}
+
+// ---- adjoint of the projection (fixed interface) ---------------------------
+// The projection at fixed phi is linear in the velocity:
+//   u' = u + Gh A^-1 M B u,  B u = -div(u)/dt,  M = mean removal,
+// A the GFM pressure matrix (symmetric), Gh = du/dp of the gradient update
+// above (its minus sign included).  Its adjoint is
+// lambda + B^T M A^-1 Gh^T lambda: these kernels are Gh^T (faces to cells)
+// and D^T (cells to faces); the solve uses the same operator.
+
+// Coefficient of face f (the face below cell f along comp's axis, neighbour
+// cell f - st) in k_gradient's update u_f -= dt (p_f - p_n) coef; returns
+// false for a face k_gradient leaves alone.  phi's ghost next to the first
+// cell holds its own value (constant extrapolation of ComputeNormals).
+__device__ __forceinline__ bool grad_face_coef(const double* __restrict__ phi, const double* __restrict__ rho,
+                                               const FsLayout& L, int comp, int i, int j, int k, long st, FsMaterial m,
+                                               double eps, double od, double& coef) {
+    int nidx = comp == 1 ? i - 1 : comp == 2 ? j - 1 : k - 1;
+    bool neighbour_full = nidx >= 0;
+    bool first_cell = (i == 0 && j == 0 && k == 0);
+    if (!neighbour_full && !first_cell) return false;
+    long c = L.idx(i, j, k);
+    double lc = phi[c], ln = neighbour_full ? phi[c - st] : lc;
+    if (lc * ln > 0) {
+        coef = (double)1 / rho[c] * od;
+    } else {
+        double gn = neighbour_full ? ln : (double)2 * phi[c] - phi[c + st];
+        double H = fs_heaviside_proj((double)0.5 * (lc + gn), eps);
+        coef = (double)1 / m.density(H) * od;
+    }
+    return true;
+}
+
+// g_c += sum over comp's faces f touching cell c of (d u_f / d p_c) lam_f
+// (Gh^T lam).
+__global__ void __launch_bounds__(BX* BY) k_adj_gradient(int comp, const double* __restrict__ lam,
+                                                         const double* __restrict__ phi, const double* __restrict__ rho,
+                                                         double* __restrict__ g, FsLayout L, FsMaterial m, double eps,
+                                                         double od, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    long st = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
+    int idx = comp == 1 ? i : comp == 2 ? j : k;
+    int n = comp == 1 ? L.nx : comp == 2 ? L.ny : L.nz;
+    double acc = 0, coef;
+    // the face below c: u_f -= dt (p_c - p_n) coef
+    if (grad_face_coef(phi, rho, L, comp, i, j, k, st, m, eps, od, coef)) acc -= dt * coef * lam[c];
+    // the face above c, i.e. below the next cell (whose neighbour c is full)
+    if (idx + 1 < n) {
+        int i2 = i + (comp == 1), j2 = j + (comp == 2), k2 = k + (comp == 3);
+        if (grad_face_coef(phi, rho, L, comp, i2, j2, k2, st, m, eps, od, coef)) acc += dt * coef * lam[c + st];
+    }
+    g[c] += acc;
+}
+
+void fs_adjoint_gradient(hipStream_t st, int comp, const double* lam, const double* phi, const double* rho, double* g,
+                         const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt) {
+    double dg = comp == 1 ? cell.x.d : comp == 2 ? cell.y.d : cell.z.d;
+    double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
+    hipLaunchKernelGGL(k_adj_gradient, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, comp, lam, phi, rho, g, L, m,
+                       (double)1.5 * dg, od, dt);
+}
+
+// lam_face += scale * D^T z: the divergence reads u(i+1) - u(i) per cell, so
+// face i of each component gets od (z(i-1) - z(i)), z = 0 outside the cells.
+__global__ void __launch_bounds__(BX* BY) k_adj_divergence(int comp, double* __restrict__ lam,
+                                                           const double* __restrict__ z, FsLayout L, FsBox b, double od,
+                                                           double scale) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= b.ni || j >= b.nj) return;
+    long c = L.idx(i, j, k);
+    long st = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
+    int idx = comp == 1 ? i : comp == 2 ? j : k;
+    int n = comp == 1 ? L.nx : comp == 2 ? L.ny : L.nz;
+    double below = idx > 0 ? z[c - st] : 0.0, above = idx < n ? z[c] : 0.0;
+    lam[c] += scale * (od * (below - above));
+}
+
+void fs_adjoint_divergence(hipStream_t st, double* lu, double* lv, double* lw, const double* z, const FsLayout& L,
+                           const FsGridConst& cell, double scale) {
+    double* lam[3] = {lu, lv, lw};
+    double od[3] = {cell.x.inv, cell.y.inv, cell.z.inv};
+    for (int comp = 1; comp <= 3; comp++) {
+        FsBox b = fs_box(L, comp);
+        hipLaunchKernelGGL(k_adj_divergence, grid_for(b), dim3(BX, BY), 0, st, comp, lam[comp - 1], z, L, b,
+                           od[comp - 1], scale);
+    }
 }
 
 __global__ void __launch_bounds__(BX* BY) k_abs_max(const double* __restrict__ a, FsLayout L, FsBox b,

@@ -201,6 +201,41 @@ int  fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms);
 int  fs_set_kernel_sampling(fs_solver* s, int enable);
 int  fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* count);
 
+/* ---- adjoints (no reference counterpart) ---------------------------------
+ * Reverse-mode pieces of the timestep for gradients through the GPU path.
+ *
+ * fs_adjoint_projection: the projection (fs_projection) at the current
+ * level set is linear in the velocity, u' = (I - G A^-1 M B) u with
+ * B u = -div(u)/dt, M the Neumann mean removal, A the GFM matrix and G the
+ * gradient update (z faces, or all faces in FS_SCHEDULE_FIXED).  Given the
+ * adjoint lam of u', v', w' (host face fields, interior only, in place) it
+ * returns lam - B^T M A^-1 G^T lam: one more solve with the same symmetric
+ * operator, FP64, with the stop rule of the forward solve.  The level set is
+ * held fixed (its influence on the coefficients is not differentiated);
+ * nothing of the forward state is changed. */
+int  fs_adjoint_projection(fs_solver* s, double dt, double* lu, double* lv, double* lw, fs_step_stats* stats);
+
+/* fs_adjoint_sourcing: the explicit viscous update of fs_sourcing (x, then
+ * y reading the new u, then z reading the new u and v) is linear in the
+ * velocity at a fixed level set; this applies its transpose (z, y, x) to the
+ * adjoint lam of u, v, w (host face fields, in place).  FS_ERR_UNSUPPORTED
+ * with semi_implicit_viscosity. */
+int  fs_adjoint_sourcing(fs_solver* s, double dt, double* lu, double* lv, double* lw);
+
+/* fs_weno5_transport: one level-set stage of ADVECTION_3D::Solve_Levelset
+ * (ADVECTION_3D.h:204-242, ADVECTION_METHOD_3D.h:42-279) on explicit inputs,
+ * with the level set's ghosts its constant extrapolation and the MAC ghosts
+ * zero: out = phi - dt (u X + v Y + w Z), X the upwind WENO5 derivative.
+ * fs_adjoint_weno5_transport: its vector-Jacobian product at (phi, u, v,
+ * w): given lam (cell field) it returns lphi = (d out/d phi)^T lam, with the
+ * derivatives of the WENO5 weights and smoothness indicators, and lu, lv, lw
+ * = (d out/d velocity)^T lam; the upwind choice is piecewise constant and
+ * not differentiated.  Host buffers, interior only. */
+int  fs_weno5_transport(fs_solver* s, double dt, const double* phi, const double* u, const double* v,
+                        const double* w, double* out);
+int  fs_adjoint_weno5_transport(fs_solver* s, double dt, const double* phi, const double* u, const double* v,
+                                const double* w, const double* lam, double* lphi, double* lu, double* lv, double* lw);
+
 /* ---- batches of independent scenes ---------------------------------------
  * No reference counterpart: the reference advances one world per process
  * (SIMULATION_WORLD::AdvanceOneFrame, SIMULATION_WORLD.h:320-330).  A batch
