@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "../../include/fs_abi.h"
+#include "fs_adjoint.h"
 #include "fs_kernels.h"
 #include "fs_layout.h"
 #include "fs_pcg.h"
@@ -98,6 +99,47 @@ class EulerianFluidSolver3D {
     bool sampling = false;
     double ksum[2][3] = {};
     int kcount[2][3] = {};
+
+    // ---- tape of one fixed-schedule step (what its reverse pass reads) ----
+    // Fields are drawn from one pool per component box (0 cells, 1..3 MAC)
+    // so a recycled field's ghosts stay zero.
+    struct RkRecord {
+        double *u = nullptr, *v = nullptr, *w = nullptr;     // velocity at the stage's start
+        double *u2 = nullptr, *v2 = nullptr, *w2 = nullptr;  // at Sourcing's start
+        double *ux = nullptr, *vy = nullptr;                 // after the x / y viscosity updates
+        double *p = nullptr, *phi = nullptr;                 // solved pressure; the stage's level set
+    };
+    struct StepTape {
+        std::vector<double*> pool[4];
+        size_t used[4] = {};
+        double *phi = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;  // step input
+        std::vector<double*> ls;                   // level set before each advection stage
+        std::vector<RkRecord> rk;                  // one per SolveForRK
+        double* phi_adv = nullptr;                 // level set before reinitialization
+        std::vector<std::vector<double*>> reinit;  // per call: level set before each sweep
+        void reset() {
+            for (auto& u_ : used) u_ = 0;
+            ls.clear();
+            rk.clear();
+            reinit.clear();
+            phi = u = v = w = p = phi_adv = nullptr;
+        }
+    };
+    StepTape tape;
+    bool recording = false;
+    // adjoint fields (lazily allocated) and saved states of a rollout
+    struct AdjointWork {
+        double *lphi = nullptr, *lu = nullptr, *lv = nullptr, *lw = nullptr;  // adjoint of the state
+        double *ladv = nullptr, *a0 = nullptr, *a1 = nullptr, *lsg = nullptr, *ghw = nullptr, *y = nullptr;
+        double *phiw = nullptr, *tgt = nullptr, *part = nullptr;
+        double *aU = nullptr, *aV = nullptr, *aW = nullptr, *src = nullptr, *du = nullptr, *dv = nullptr,
+               *dw = nullptr;
+        bool ready = false;
+    } A;
+    struct SavedState {
+        double *phi = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
+    };
+    std::vector<SavedState> saved;
 
     void* dalloc(size_t bytes) {
         void* ptr = nullptr;
@@ -206,6 +248,15 @@ class EulerianFluidSolver3D {
                       S64, S32, vmax, wv.r, wv.s, wv.p, wv.q, w64.p2, w32.p2, wv.p2};
         for (void* f : fv)
             if (f) (void)hipFree(f);
+        for (auto& pl : tape.pool)
+            for (double* f : pl) (void)hipFree(f);
+        double* adj[] = {A.lphi, A.lu, A.lv, A.lw, A.ladv, A.a0, A.a1, A.lsg, A.ghw, A.y, A.phiw, A.tgt, A.part,
+                         A.aU, A.aV, A.aW, A.src, A.du, A.dv, A.dw};
+        for (double* f : adj)
+            if (f) (void)hipFree(f);
+        for (auto& sv : saved)
+            for (double* f : {sv.phi, sv.u, sv.v, sv.w, sv.p})
+                if (f) (void)hipFree(f);
         if (hS) (void)hipHostFree(hS);
         if (hvmax) (void)hipHostFree(hvmax);
         for (auto& e : ev)
@@ -262,17 +313,62 @@ class EulerianFluidSolver3D {
     // surface tension the lasting effect is phi's ghost re-extrapolation (:998).
     void ComputeCurvatures() { fs_fill_linear(st, phi, phi, L, box(0), 2, 1); }
 
-    // ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242
+    bool fixed() const { return P.schedule == FS_SCHEDULE_FIXED; }
+
+    // copy of field src (component box comp) onto the step tape
+    double* Record(const double* src, int comp) {
+        auto& pl = tape.pool[comp];
+        size_t& n = tape.used[comp];
+        if (n == pl.size()) pl.push_back(dfield());
+        double* f = pl[n++];
+        fs_copy_box(st, f, src, L, box(comp));
+        return f;
+    }
+
+    // Ghost copies as functions of the current fields (fixed schedule): the
+    // level set's by linear extrapolation of its own interior, layer g at
+    // (g+1) phi_0 - g phi_1, faces only; the MAC velocities' with zero faces.
+    void RefreshLevelsetGhost() {
+        fs_copy_box(st, sghost, phi, L, box(0));
+        fs_fill_linear(st, sghost, sghost, L, box(0), 3, 0);
+    }
+    void RefreshVelocityGhosts() {
+        fs_copy_box(st, gu, u, L, box(1));
+        fs_fill_zero_faces(st, gu, L, box(1), 3);
+        fs_copy_box(st, gv, v, L, box(2));
+        fs_fill_zero_faces(st, gv, L, box(2), 3);
+        fs_copy_box(st, gw, w, L, box(3));
+        fs_fill_zero_faces(st, gw, L, box(3), 3);
+    }
+    void ZeroWalls() {
+        fs_zero_walls(st, 1, u, L);
+        fs_zero_walls(st, 2, v, L);
+        fs_zero_walls(st, 3, w, L);
+    }
+
+    // ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242.  The compat
+    // schedule reads the ghost copies the earlier phases left (the last
+    // reinitialization sweep's, the last projection's); the fixed schedule
+    // refreshes them first.
     void Levelset_Advection(double dt) {
+        if (recording) tape.ls.push_back(Record(phi, 0));
+        if (fixed()) {
+            RefreshLevelsetGhost();
+            RefreshVelocityGhosts();
+        }
         fs_weno5_stage(st, phi, sghost, gu, gv, gw, L, 0, cell, P.levelset_epsilon, dt);
     }
 
-    // ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 (air/water)
+    // ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 (air/water).  The
+    // reference's WENO5 branch passes the x ghost copy for all three
+    // components (:314-316); the fixed schedule reads each component's own.
     void Velocity_Advection(double dt) {
+        if (fixed()) RefreshVelocityGhosts();
         if (P.use_5th_weno_v) {
+            bool own = fixed();
             fs_weno5_stage(st, u, gu, gu, gv, gw, L, 1, macx, P.velocity_epsilon, dt);
-            fs_weno5_stage(st, v, gu, gu, gv, gw, L, 2, macy, P.velocity_epsilon, dt);
-            fs_weno5_stage(st, w, gu, gu, gv, gw, L, 3, macz, P.velocity_epsilon, dt);
+            fs_weno5_stage(st, v, own ? gv : gu, gu, gv, gw, L, 2, macy, P.velocity_epsilon, dt);
+            fs_weno5_stage(st, w, own ? gw : gu, gu, gv, gw, L, 3, macz, P.velocity_epsilon, dt);
         } else {
             fs_eno3_stage(st, u, gu, gu, gv, gw, L, 1, macx, dt);
             fs_eno3_stage(st, v, gv, gu, gv, gw, L, 2, macy, dt);
@@ -281,21 +377,42 @@ class EulerianFluidSolver3D {
     }
 
     // ApplyGravity, EULERIAN_FLUID_SOLVER_3D.h:2237-2270
+    // (the fixed schedule then zeroes the normal wall faces)
     void ApplyGravity(double dt) {
         fs_gravity(st, u, v, w, L, dt * P.gravity[0], dt * P.gravity[1], dt * P.gravity[2]);
+        if (fixed()) ZeroWalls();
     }
 
     // Sourcing -> VISCOSITY_3D::ApplyViscosity (air/water explicit), each
-    // component out of place into its spare buffer, then swapped in.
+    // component out of place into its spare buffer, then swapped in.  The
+    // fixed schedule reads phi's ghosts as the constant extrapolation of the
+    // current level set and zeroes each component's wall faces after its
+    // update.
     int Sourcing(double dt) {
         if (!P.use_delta_function_formulation) return FS_OK;
-        if (P.semi_implicit_viscosity) return SemiImplicitViscosity(dt);
+        if (fixed()) fs_fill_const(st, phi, phi, L, box(0), 2);
+        if (P.semi_implicit_viscosity) {
+            int rc = SemiImplicitViscosity(dt);
+            if (fixed()) ZeroWalls();
+            return rc;
+        }
+        if (recording) {
+            RkRecord& r = tape.rk.back();
+            r.u2 = Record(u, 1);
+            r.v2 = Record(v, 2);
+            r.w2 = Record(w, 3);
+        }
         fs_viscosity(st, 1, spare_u, u, v, w, phi, L, mat, cell, dt);
         std::swap(u, spare_u);
+        if (fixed()) fs_zero_walls(st, 1, u, L);
+        if (recording) tape.rk.back().ux = Record(u, 1);
         fs_viscosity(st, 2, spare_v, u, v, w, phi, L, mat, cell, dt);
         std::swap(v, spare_v);
+        if (fixed()) fs_zero_walls(st, 2, v, L);
+        if (recording) tape.rk.back().vy = Record(v, 2);
         fs_viscosity(st, 3, spare_w, u, v, w, phi, L, mat, cell, dt);
         std::swap(w, spare_w);
+        if (fixed()) fs_zero_walls(st, 3, w, L);
         return FS_OK;
     }
 
@@ -558,13 +675,19 @@ class EulerianFluidSolver3D {
     }
 
     // PROJECTION_3D::UpdateVelocity, :836-841 (z faces only in air/water
-    // mode; all faces in the fixed schedule).
+    // mode).  The fixed schedule corrects every face with the matrix's own
+    // face coefficients.
     void UpdateVelocity(double dt) {
         HIPCHECK(hipMemsetAsync(vmax, 0, sizeof(unsigned long long) * 4, st));
-        int all = P.schedule == FS_SCHEDULE_FIXED;
-        fs_gradient_update(st, 1, all, u, p, phi, rho, L, mat, cell, dt, vmax + 0);
-        fs_gradient_update(st, 2, all, v, p, phi, rho, L, mat, cell, dt, vmax + 1);
-        fs_gradient_update(st, 3, 1, w, p, phi, rho, L, mat, cell, dt, vmax + 2);
+        if (fixed()) {
+            fs_gradient_update_beta(st, 1, u, p, bx, L, cell, dt, vmax + 0);
+            fs_gradient_update_beta(st, 2, v, p, by, L, cell, dt, vmax + 1);
+            fs_gradient_update_beta(st, 3, w, p, bz, L, cell, dt, vmax + 2);
+        } else {
+            fs_gradient_update(st, 1, 0, u, p, phi, rho, L, mat, cell, dt, vmax + 0);
+            fs_gradient_update(st, 2, 0, v, p, phi, rho, L, mat, cell, dt, vmax + 1);
+            fs_gradient_update(st, 3, 1, w, p, phi, rho, L, mat, cell, dt, vmax + 2);
+        }
         HIPCHECK(hipMemcpyAsync(hvmax, vmax, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, st));
         sync();
         for (int c = 0; c < 3; c++) {
@@ -608,17 +731,20 @@ class EulerianFluidSolver3D {
         double light = P.air_bubble_rising ? P.air_density : P.water_density;
         double heavy = P.air_bubble_rising ? P.water_density : P.air_density;
         fs_density(st, phi, rho, irho, L, light, heavy);
+        fs_fill_const(st, phi, phi, L, box(0), 2);  // as AssemblePressureSystem
         fs_fill_const(st, irho, irho, L, box(0), 1);
         fs_fill_value(st, div, L.total, 0.0);
         fs_assemble(st, phi, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
         assembled = true;
         // rhs = G^T lam (the faces the forward corrects)
         fs_fill_value(st, rhs, L.total, 0.0);
-        if (P.schedule == FS_SCHEDULE_FIXED) {
-            fs_adjoint_gradient(st, 1, lu, phi, rho, rhs, L, mat, cell, dt);
-            fs_adjoint_gradient(st, 2, lv, phi, rho, rhs, L, mat, cell, dt);
+        if (fixed()) {
+            fs_adjoint_gradient_beta(st, 1, lu, bx, rhs, L, cell, dt);
+            fs_adjoint_gradient_beta(st, 2, lv, by, rhs, L, cell, dt);
+            fs_adjoint_gradient_beta(st, 3, lw, bz, rhs, L, cell, dt);
+        } else {
+            fs_adjoint_gradient(st, 3, lw, phi, rho, rhs, L, mat, cell, dt);
         }
-        fs_adjoint_gradient(st, 3, lw, phi, rho, rhs, L, mat, cell, dt);
         // y = A^-1 rhs (A symmetric), then M y
         fs_fill_value(st, tmp, L.total, 0.0);
         int its = run_pcg64(P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, tmp);
@@ -666,22 +792,38 @@ class EulerianFluidSolver3D {
         return rc;
     }
 
-    // EULERIAN_FLUID_SOLVER_3D::Reinitialization, :2198-2225
+    // EULERIAN_FLUID_SOLVER_3D::Reinitialization, :2198-2225.  The fixed
+    // schedule extrapolates the ghost copy from phi's interior only and uses
+    // the Euclidean norm in every Godunov case.
     void Reinitialization() {
         if (!P.sussmanstyle_reinitialization) return;
         double fictious_dt = cell.x.d / P.scaling_number_for_reinitialization;
         fs_sign(st, phi, sgn, L, cell.x.d * cell.x.d);
+        if (recording) tape.reinit.emplace_back();
         for (int it = 0; it < P.iteration_number_for_reinitialization; it++) {
+            if (recording) tape.reinit.back().push_back(Record(phi, 0));
             // ADVECTION_3D::ReinitializationBySussman, ADVECTION_3D.h:360-378
-            fs_copy_box(st, sghost, phi, L, box(0));
-            fs_fill_linear(st, sghost, phi, L, box(0), 3, 0);
-            fs_reinit_sweep(st, phi, sghost, sgn, L, cell, P.levelset_epsilon, fictious_dt);
+            if (fixed()) {
+                RefreshLevelsetGhost();
+            } else {
+                fs_copy_box(st, sghost, phi, L, box(0));
+                fs_fill_linear(st, sghost, phi, L, box(0), 3, 0);
+            }
+            fs_reinit_sweep(st, phi, sghost, sgn, L, cell, P.levelset_epsilon, fictious_dt, fixed());
         }
     }
 
     // EULERIAN_FLUID_SOLVER_3D::SolveForRK, :1453-1476; a non-converged
     // projection is remembered and the step goes on, as the reference does.
     void SolveForRK(double dt, int& rc) {
+        if (recording) {
+            RkRecord r;
+            r.u = Record(u, 1);
+            r.v = Record(v, 2);
+            r.w = Record(w, 3);
+            r.phi = Record(phi, 0);
+            tape.rk.push_back(r);
+        }
         if (P.is_velocity_advection_active) Velocity_Advection(dt);
         if (P.is_external_force_active) ApplyGravity(dt);
         if (P.is_sourcing_active) {
@@ -691,6 +833,7 @@ class EulerianFluidSolver3D {
         if (P.is_projection_active) {
             int r = Projection(dt);
             if (r != FS_OK) rc = r;
+            if (recording) tape.rk.back().p = Record(p, 0);
         }
     }
 
@@ -711,6 +854,14 @@ class EulerianFluidSolver3D {
     int AdvanceOneTimeStepThread(double dt) {
         int rc = FS_OK;
         const int order = P.order_for_time_advancing;
+        if (recording) {
+            tape.reset();
+            tape.phi = Record(phi, 0);
+            tape.u = Record(u, 1);
+            tape.v = Record(v, 2);
+            tape.w = Record(w, 3);
+            tape.p = Record(p, 0);
+        }
         if (P.is_levelset_advection_active) {
             if (order == 3) {
                 Levelset_Advection(dt);  // :1010-1013, one Euler stage
@@ -736,6 +887,7 @@ class EulerianFluidSolver3D {
             }
         }
         // reinitialization, RK2 (:836-858, :958-980) or RK3 (:1080-1109)
+        if (recording) tape.phi_adv = Record(phi, 0);
         fs_copy_box(st, tmp, phi, L, box(0));
         Reinitialization();
         Reinitialization();

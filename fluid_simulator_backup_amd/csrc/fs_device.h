@@ -115,3 +115,84 @@ __device__ __forceinline__ int fs_godunov_axis(double sg, double m, double p, do
     }
     return FS_AX_NONE;
 }
+
+// Velocity at the sample point from the MAC ghost copies,
+// ADVECTION_METHOD_3D.h:201-224 (same for ENO3rd, :1862-1879).
+__device__ __forceinline__ void vel_at(const double* __restrict__ gu, const double* __restrict__ gv,
+                                       const double* __restrict__ gw, const FsLayout& L, int comp, int i, int j,
+                                       int k, double& uu, double& vv, double& ww) {
+    long c = L.idx(i, j, k);
+    const long sx = L.sx, sy = L.sy;
+    if (comp == 0) {
+        uu = (gu[c + 1] + gu[c]) * (double)0.5;
+        vv = (gv[c + sx] + gv[c]) * (double)0.5;
+        ww = (gw[c + sy] + gw[c]) * (double)0.5;
+    } else if (comp == 1) {
+        uu = gu[c];
+        vv = (gv[c] + gv[c + sx] + gv[c - 1] + gv[c - 1 + sx]) * (double)0.25;
+        ww = (gw[c] + gw[c + sy] + gw[c - 1] + gw[c - 1 + sy]) * (double)0.25;
+    } else if (comp == 2) {
+        uu = (gu[c] + gu[c + 1] + gu[c - sx] + gu[c + 1 - sx]) * (double)0.25;
+        vv = gv[c];
+        ww = (gw[c] + gw[c + sy] + gw[c - sx] + gw[c - sx + sy]) * (double)0.25;
+    } else {
+        uu = (gu[c] + gu[c + 1] + gu[c - sy] + gu[c + 1 - sy]) * (double)0.25;
+        vv = (gv[c] + gv[c + sx] + gv[c - sy] + gv[c + sx - sy]) * (double)0.25;
+        ww = gw[c];
+    }
+}
+
+__device__ __forceinline__ void gather7(const double* __restrict__ a, long c, long stride, double* f) {
+#pragma unroll
+    for (int o = -3; o <= 3; o++) f[o + 3] = a[c + o * stride];
+}
+
+// Gradient of the one-sided WENO5 value D(v1..v5) (fs_weno5's dmo with
+// v = m, dpo with v = p) with respect to v.
+__device__ __forceinline__ void weno5_grad(const double v[5], double eps, double g[5]) {
+    const double k13 = (double)13 / 12, q4 = (double)1 / 4;
+    double e1 = v[0] - 2 * v[1] + v[2], f1 = v[0] - 4 * v[1] + 3 * v[2];
+    double e2 = v[1] - 2 * v[2] + v[3], f2 = v[1] - v[3];
+    double e3 = v[2] - 2 * v[3] + v[4], f3 = 3 * v[2] - 4 * v[3] + v[4];
+    double s[3] = {k13 * e1 * e1 + q4 * f1 * f1, k13 * e2 * e2 + q4 * f2 * f2, k13 * e3 * e3 + q4 * f3 * f3};
+    const double d[3] = {(double)1 / 10, (double)6 / 10, (double)3 / 10};
+    double a[3], S = 0;
+    for (int k = 0; k < 3; k++) {
+        a[k] = d[k] / ((eps + s[k]) * (eps + s[k]));
+        S += a[k];
+    }
+    double q[3] = {v[0] / 3 - v[1] * 7 / 6 + v[2] * 11 / 6, -v[1] / 6 + v[2] * 5 / 6 + v[3] / 3,
+                   v[2] / 3 + v[3] * 5 / 6 - v[4] / 6};
+    double w[3] = {a[0] / S, a[1] / S, a[2] / S};
+    double D = w[0] * q[0] + w[1] * q[1] + w[2] * q[2];
+    // dD/ds_k = -2 w_k (q_k - D) / (eps + s_k)
+    double ds[3];
+    for (int k = 0; k < 3; k++) ds[k] = -2 * w[k] * (q[k] - D) / (eps + s[k]);
+    // w_k dq_k/dv
+    g[0] = w[0] / 3;
+    g[1] = -w[0] * 7 / 6 - w[1] / 6;
+    g[2] = w[0] * 11 / 6 + w[1] * 5 / 6 + w[2] / 3;
+    g[3] = w[1] / 3 + w[2] * 5 / 6;
+    g[4] = -w[2] / 6;
+    // ds_k/dv
+    double a1 = ds[0] * 2 * k13 * e1, b1 = ds[0] * 2 * q4 * f1;
+    g[0] += a1 + b1;
+    g[1] += -2 * a1 - 4 * b1;
+    g[2] += a1 + 3 * b1;
+    double a2 = ds[1] * 2 * k13 * e2, b2 = ds[1] * 2 * q4 * f2;
+    g[1] += a2 + b2;
+    g[2] += -2 * a2;
+    g[3] += a2 - b2;
+    double a3 = ds[2] * 2 * k13 * e3, b3 = ds[2] * 2 * q4 * f3;
+    g[2] += a3 + 3 * b3;
+    g[3] += -2 * a3 - 4 * b3;
+    g[4] += a3 + b3;
+}
+
+// GFM face coefficient of BuildLinearSystemNodeJumpConditionVaribleCoefficient,
+// POISSON_SOLVER_3D.h:1080-1085, written from the cell on the + side (beta_l
+// form).  The - side cell's beta_r of the same face evaluates the same
+// products and sums, so both rows see one value.
+__device__ __forceinline__ double gfm_beta(double vc, double vn, double fn, double fc) {
+    return vc * vn * (fabs(fn) + fabs(fc)) / (vc * fabs(fn) + vn * fabs(fc));
+}

@@ -68,12 +68,23 @@ void fs_adjoint_gradient(hipStream_t st, int comp, const double* lam, const doub
                          const FsLayout& L, const FsMaterial& m, const FsGridConst& cell, double dt);
 void fs_adjoint_divergence(hipStream_t st, double* lu, double* lv, double* lw, const double* z, const FsLayout& L,
                            const FsGridConst& cell, double scale);
+// g += Gh^T lam for fs_gradient_update_beta's faces of comp
+void fs_adjoint_gradient_beta(hipStream_t st, int comp, const double* lam, const double* beta, double* g,
+                              const FsLayout& L, const FsGridConst& cell, double dt);
 void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, unsigned long long* bits);
+// fixed schedule: u_f -= dt beta_f (p_c - p_n)/dx with the pressure matrix's
+// face coefficients (beta = bx, by or bz of fs_assemble)
+void fs_gradient_update_beta(hipStream_t st, int comp, double* vel, const double* p, const double* beta,
+                             const FsLayout& L, const FsGridConst& cell, double dt, unsigned long long* vmax_bits);
+// zero the two normal wall planes of MAC component comp (1..3)
+void fs_zero_walls(hipStream_t st, int comp, double* vel, const FsLayout& L);
 
 // ---- reinitialization -----------------------------------------------------
 void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, double dx2);
+// exact: the Euclidean gradient norm in every Godunov case (the reference's
+// misplaced parenthesis of ADVECTION_METHOD_3D.h:1605 otherwise)
 void fs_reinit_sweep(hipStream_t st, double* phi, const double* ghost, const double* sgn, const FsLayout& L,
-                     const FsGridConst& cell, double eps, double dt);
+                     const FsGridConst& cell, double eps, double dt, int exact);
 
 // ---- misc -----------------------------------------------------------------
 void fs_average(hipStream_t st, double* phi, const double* tmp, const FsLayout& L);  // phi = 0.5*(tmp+phi)

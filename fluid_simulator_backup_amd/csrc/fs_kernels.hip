@@ -133,36 +133,7 @@ void fs_fill_linear(hipStream_t st, double* dst, const double* src, const FsLayo
 // advection (ADVECTION_METHOD_3D.h)
 // ---------------------------------------------------------------------------
 
-// Velocity at the sample point from the MAC ghost copies,
-// ADVECTION_METHOD_3D.h:201-224 (same for ENO3rd, :1862-1879).
-__device__ __forceinline__ void vel_at(const double* __restrict__ gu, const double* __restrict__ gv,
-                                       const double* __restrict__ gw, const FsLayout& L, int comp, int i, int j,
-                                       int k, double& uu, double& vv, double& ww) {
-    long c = L.idx(i, j, k);
-    const long sx = L.sx, sy = L.sy;
-    if (comp == 0) {
-        uu = (gu[c + 1] + gu[c]) * (double)0.5;
-        vv = (gv[c + sx] + gv[c]) * (double)0.5;
-        ww = (gw[c + sy] + gw[c]) * (double)0.5;
-    } else if (comp == 1) {
-        uu = gu[c];
-        vv = (gv[c] + gv[c + sx] + gv[c - 1] + gv[c - 1 + sx]) * (double)0.25;
-        ww = (gw[c] + gw[c + sy] + gw[c - 1] + gw[c - 1 + sy]) * (double)0.25;
-    } else if (comp == 2) {
-        uu = (gu[c] + gu[c + 1] + gu[c - sx] + gu[c + 1 - sx]) * (double)0.25;
-        vv = gv[c];
-        ww = (gw[c] + gw[c + sy] + gw[c - sx] + gw[c - sx + sy]) * (double)0.25;
-    } else {
-        uu = (gu[c] + gu[c + 1] + gu[c - sy] + gu[c + 1 - sy]) * (double)0.25;
-        vv = (gv[c] + gv[c + sx] + gv[c - sy] + gv[c + sx - sy]) * (double)0.25;
-        ww = gw[c];
-    }
-}
 
-__device__ __forceinline__ void gather7(const double* __restrict__ a, long c, long stride, double* f) {
-#pragma unroll
-    for (int o = -3; o <= 3; o++) f[o + 3] = a[c + o * stride];
-}
 
 __device__ __forceinline__ double upwind(double rho, double dt, double uu, double vv, double ww, double mx, double px,
                                          double my, double py, double mz, double pz) {
@@ -199,47 +170,6 @@ static const FsGridConst& gc_dummy(const FsGridConst& g) { return g; }
 // upwind WENO5 derivative.  The adjoint differentiates the WENO5 weights
 // (smoothness indicators included); the upwind choice is piecewise constant.
 
-// Gradient of the one-sided WENO5 value D(v1..v5) (fs_weno5's dmo with
-// v = m, dpo with v = p) with respect to v.
-__device__ __forceinline__ void weno5_grad(const double v[5], double eps, double g[5]) {
-    const double k13 = (double)13 / 12, q4 = (double)1 / 4;
-    double e1 = v[0] - 2 * v[1] + v[2], f1 = v[0] - 4 * v[1] + 3 * v[2];
-    double e2 = v[1] - 2 * v[2] + v[3], f2 = v[1] - v[3];
-    double e3 = v[2] - 2 * v[3] + v[4], f3 = 3 * v[2] - 4 * v[3] + v[4];
-    double s[3] = {k13 * e1 * e1 + q4 * f1 * f1, k13 * e2 * e2 + q4 * f2 * f2, k13 * e3 * e3 + q4 * f3 * f3};
-    const double d[3] = {(double)1 / 10, (double)6 / 10, (double)3 / 10};
-    double a[3], S = 0;
-    for (int k = 0; k < 3; k++) {
-        a[k] = d[k] / ((eps + s[k]) * (eps + s[k]));
-        S += a[k];
-    }
-    double q[3] = {v[0] / 3 - v[1] * 7 / 6 + v[2] * 11 / 6, -v[1] / 6 + v[2] * 5 / 6 + v[3] / 3,
-                   v[2] / 3 + v[3] * 5 / 6 - v[4] / 6};
-    double w[3] = {a[0] / S, a[1] / S, a[2] / S};
-    double D = w[0] * q[0] + w[1] * q[1] + w[2] * q[2];
-    // dD/ds_k = -2 w_k (q_k - D) / (eps + s_k)
-    double ds[3];
-    for (int k = 0; k < 3; k++) ds[k] = -2 * w[k] * (q[k] - D) / (eps + s[k]);
-    // w_k dq_k/dv
-    g[0] = w[0] / 3;
-    g[1] = -w[0] * 7 / 6 - w[1] / 6;
-    g[2] = w[0] * 11 / 6 + w[1] * 5 / 6 + w[2] / 3;
-    g[3] = w[1] / 3 + w[2] * 5 / 6;
-    g[4] = -w[2] / 6;
-    // ds_k/dv
-    double a1 = ds[0] * 2 * k13 * e1, b1 = ds[0] * 2 * q4 * f1;
-    g[0] += a1 + b1;
-    g[1] += -2 * a1 - 4 * b1;
-    g[2] += a1 + 3 * b1;
-    double a2 = ds[1] * 2 * k13 * e2, b2 = ds[1] * 2 * q4 * f2;
-    g[1] += a2 + b2;
-    g[2] += -2 * a2;
-    g[3] += a2 - b2;
-    double a3 = ds[2] * 2 * k13 * e3, b3 = ds[2] * 2 * q4 * f3;
-    g[2] += a3 + 3 * b3;
-    g[3] += -2 * a3 - 4 * b3;
-    g[4] += a3 + b3;
-}
 
 // the seven samples of the extended level set along one axis
 __device__ __forceinline__ void gather7_clamped(const double* __restrict__ phi, const FsLayout& L, int i, int j, int k,
@@ -730,13 +660,6 @@ void fs_divergence(hipStream_t st, const double* gu, const double* gv, const dou
                        cell.y.inv, cell.z.inv, one_over_dt);
 }
 
-// GFM face coefficient of BuildLinearSystemNodeJumpConditionVaribleCoefficient,
-// POISSON_SOLVER_3D.h:1080-1085, written from the cell on the + side (beta_l
-// form).  The - side cell's beta_r of the same face evaluates the same
-// products and sums, so both rows see one value.
-__device__ __forceinline__ double gfm_beta(double vc, double vn, double fn, double fc) {
-    return vc * vn * (fabs(fn) + fabs(fc)) / (vc * fabs(fn) + vn * fabs(fc));
-}
 
 // Faces between two interior (full) cells get beta, every other face 0.
 __global__ void k_beta(const double* __restrict__ f, const double* __restrict__ v, double* __restrict__ bx,
@@ -918,6 +841,62 @@ void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const 
                        (double)1.5 * dg, od, dt, vmax_bits);
 }
 
+// Gradient update of the fixed schedule: every face takes the pressure
+// matrix's own GFM coefficient, u_f -= dt beta_f (p_c - p_n) / dx, so the
+// corrected velocity is discretely divergence-free (beta is 0 on wall faces,
+// which are left alone).
+__global__ void __launch_bounds__(BX* BY) k_gradient_beta(double* __restrict__ vel, const double* __restrict__ p,
+                                                          const double* __restrict__ beta, FsLayout L, FsBox b,
+                                                          long st, double od, double dt, unsigned long long* vmax) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double a = 0;
+    for (int k = blockIdx.z * KZ; k < min(blockIdx.z * KZ + KZ, b.nk); k++) {
+        if (i >= b.ni || j >= b.nj) continue;
+        long c = L.idx(i, j, k);
+        double bf = beta[c], vr = vel[c];
+        if (bf != 0) {
+            vr -= dt * bf * (p[c] - p[c - st]) * od;
+            vel[c] = vr;
+        }
+        a = nanmax(a, fabs(vr));
+    }
+    wg_max_atomic(a, vmax);
+}
+
+void fs_gradient_update_beta(hipStream_t st, int comp, double* vel, const double* p, const double* beta,
+                             const FsLayout& L, const FsGridConst& cell, double dt, unsigned long long* vmax_bits) {
+    FsBox b = fs_box(L, comp);
+    double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
+    long s = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
+    dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
+    hipLaunchKernelGGL(k_gradient_beta, g, dim3(BX, BY), 0, st, vel, p, beta, L, b, s, od, dt, vmax_bits);
+}
+
+// Normal wall faces of component comp set to zero (the fixed schedule's
+// no-penetration walls after forces).
+__global__ void k_zero_walls(double* __restrict__ vel, FsLayout L, FsBox b, int comp) {
+    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (comp == 1) {
+        if (a >= b.nj) return;
+        vel[L.idx(0, a, c)] = 0;
+        vel[L.idx(b.ni - 1, a, c)] = 0;
+    } else if (comp == 2) {
+        if (a >= b.ni) return;
+        vel[L.idx(a, 0, c)] = 0;
+        vel[L.idx(a, b.nj - 1, c)] = 0;
+    } else {
+        if (a >= b.ni) return;
+        vel[L.idx(a, c, 0)] = 0;
+        vel[L.idx(a, c, b.nk - 1)] = 0;
+    }
+}
+
+void fs_zero_walls(hipStream_t st, int comp, double* vel, const FsLayout& L) {
+    FsBox b = fs_box(L, comp);
+    int n = comp == 1 ? b.nj : b.ni, m = comp == 3 ? b.nj : b.nk;
+    hipLaunchKernelGGL(k_zero_walls, dim3((n + 63) / 64, m), dim3(64), 0, st, vel, L, b, comp);
+}
+
 // ---- adjoint of the projection (fixed interface) ---------------------------
 // The projection at fixed phi is linear in the velocity:
 //   u' = u + Gh A^-1 M B u,  B u = -div(u)/dt,  M = mean removal,
@@ -980,6 +959,24 @@ void fs_adjoint_gradient(hipStream_t st, int comp, const double* lam, const doub
                        (double)1.5 * dg, od, dt);
 }
 
+// g_c += (Gh^T lam)_c for k_gradient_beta's update u_f -= dt beta_f (p_c -
+// p_n) od: the face below c contributes -dt od beta lam, the face above +.
+__global__ void __launch_bounds__(BX* BY) k_adj_gradient_beta(const double* __restrict__ lam,
+                                                              const double* __restrict__ beta, double* __restrict__ g,
+                                                              FsLayout L, long st, double od, double dt) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    g[c] += dt * od * (beta[c + st] * lam[c + st] - beta[c] * lam[c]);
+}
+
+void fs_adjoint_gradient_beta(hipStream_t st, int comp, const double* lam, const double* beta, double* g,
+                              const FsLayout& L, const FsGridConst& cell, double dt) {
+    double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
+    long s = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
+    hipLaunchKernelGGL(k_adj_gradient_beta, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, lam, beta, g, L, s, od, dt);
+}
+
 // lam_face += scale * D^T z: the divergence reads u(i+1) - u(i) per cell, so
 // face i of each component gets od (z(i-1) - z(i)), z = 0 outside the cells.
 __global__ void __launch_bounds__(BX* BY) k_adj_divergence(int comp, double* __restrict__ lam,
@@ -1039,11 +1036,12 @@ void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, 
 
 // WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637: WENO5 one-sided
 // derivatives of the ghost copy, Godunov case per axis, the misplaced
-// parenthesis of :1463/:1605 (x shock, y shock towards +, z expansion), and
-// the sign-preserving clamp (:1624-1634).
+// parenthesis of :1463/:1605 (x shock, y shock towards +, z expansion; the
+// Euclidean norm instead when `exact` is set), and the sign-preserving clamp
+// (:1624-1634).
 __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, const double* __restrict__ gh,
                                                    const double* __restrict__ sgn, FsLayout L, double ix, double iy,
-                                                   double iz, double eps, double dt) {
+                                                   double iz, double eps, double dt, int exact) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
     long c = L.idx(i, j, k);
@@ -1061,7 +1059,7 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, con
     int cz = cy ? fs_godunov_axis(sg, mz, pz, Z) : FS_AX_NONE;
     if (cx && cy && cz) {
         double g;
-        if ((cx == FS_AX_SM || cx == FS_AX_SP) && cy == FS_AX_SP && cz == FS_AX_DROP)
+        if (!exact && (cx == FS_AX_SM || cx == FS_AX_SP) && cy == FS_AX_SP && cz == FS_AX_DROP)
             g = sqrt(X * X) + Y * Y - 1;
         else
             g = sqrt(X * X + Y * Y + Z * Z) - 1;
@@ -1075,9 +1073,9 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, con
 }
 
 void fs_reinit_sweep(hipStream_t st, double* phi, const double* ghost, const double* sgn, const FsLayout& L,
-                     const FsGridConst& cell, double eps, double dt) {
+                     const FsGridConst& cell, double eps, double dt, int exact) {
     hipLaunchKernelGGL(k_reinit, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, ghost, sgn, L, cell.x.inv,
-                       cell.y.inv, cell.z.inv, eps, dt);
+                       cell.y.inv, cell.z.inv, eps, dt, exact);
 }
 
 // ---------------------------------------------------------------------------

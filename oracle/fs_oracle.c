@@ -428,21 +428,57 @@ int fso_compute_curvatures(fso_state* s) {
  * jump terms that vanish with zero surface tension. */
 static void compute_normals(fso_state* s) { fill_const(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0); }
 
+static int is_fixed(const fso_state* s) { return s->P.schedule == FS_SCHEDULE_FIXED; }
+
+/* Fixed schedule: ghost copies made from the current fields.  The level
+ * set's ghost layers extrapolate its own interior linearly (faces only, each
+ * layer from the layers written before it); velocities get zero faces. */
+static void refresh_levelset_ghost(fso_state* s) {
+    copy_interior(s, s->sghost, s->phi, s->nx, s->ny, s->nz);
+    fill_linear(s, s->sghost, s->sghost, s->nx, s->ny, s->nz, 3, 0, 0);
+}
+static void refresh_velocity_ghosts(fso_state* s) {
+    fill_zero(s, s->gu, s->u, s->nx + 1, s->ny, s->nz, 3);
+    fill_zero(s, s->gv, s->v, s->nx, s->ny + 1, s->nz, 3);
+    fill_zero(s, s->gw, s->w, s->nx, s->ny, s->nz + 1, 3);
+}
+/* Fixed schedule: the two normal wall planes of a MAC component are zero. */
+static void zero_walls(fso_state* s, int comp) {
+    int ni, nj, nk;
+    extents(s, comp, &ni, &nj, &nk);
+    double* vel = comp == 1 ? s->u : comp == 2 ? s->v : s->w;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                int wall = comp == 1 ? (i == 0 || i == ni - 1) : comp == 2 ? (j == 0 || j == nj - 1)
+                                                                           : (k == 0 || k == nk - 1);
+                if (wall) A3(vel, i, j, k) = 0;
+            }
+}
+
 /* ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242 (air/water, WENO5):
- * reads the shared ghost copy and the MAC ghost copies as they are. */
+ * the compat schedule reads the shared ghost copy and the MAC ghost copies
+ * as they are; the fixed schedule refreshes them first. */
 int fso_levelset_advection(fso_state* s, double dt) {
+    if (is_fixed(s)) {
+        refresh_levelset_ghost(s);
+        refresh_velocity_ghosts(s);
+    }
     weno5_stage(s, s->phi, s->sghost, 0, dt, s->P.levelset_epsilon);
     return FS_OK;
 }
 
 /* ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 (air/water branch).
  * ENO3 reads each component's own ghost copy; the WENO5 branch passes the
- * x ghost copy for all three components (:314-316). */
+ * x ghost copy for all three components (:314-316), the fixed schedule each
+ * component's own, refreshed first. */
 int fso_velocity_advection(fso_state* s, double dt) {
+    if (is_fixed(s)) refresh_velocity_ghosts(s);
     if (s->P.use_5th_weno_v) {
+        int own = is_fixed(s);
         weno5_stage(s, s->u, s->gu, 1, dt, s->P.velocity_epsilon);
-        weno5_stage(s, s->v, s->gu, 2, dt, s->P.velocity_epsilon);
-        weno5_stage(s, s->w, s->gu, 3, dt, s->P.velocity_epsilon);
+        weno5_stage(s, s->v, own ? s->gv : s->gu, 2, dt, s->P.velocity_epsilon);
+        weno5_stage(s, s->w, own ? s->gw : s->gu, 3, dt, s->P.velocity_epsilon);
     } else {
         eno3_stage(s, s->u, s->gu, 1, dt);
         eno3_stage(s, s->v, s->gv, 2, dt);
@@ -463,6 +499,8 @@ int fso_apply_gravity(fso_state* s, double dt) {
     for (int k = 0; k <= s->nz; k++)
         for (int j = 0; j < s->ny; j++)
             for (int i = 0; i < s->nx; i++) A3(s->w, i, j, k) += gz;
+    if (is_fixed(s))
+        for (int c = 1; c <= 3; c++) zero_walls(s, c);
     return FS_OK;
 }
 
@@ -736,10 +774,18 @@ static int semi_implicit_viscosity(fso_state* s, double dt, int* iterations) {
 /* EULERIAN_FLUID_SOLVER_3D::Sourcing (:2076-2100) -> VISCOSITY_3D::ApplyViscosity
  * air/water explicit branch, VISCOSITY_3D.h:448-563: components x, y, z in
  * turn, each updated in place (in-place mode) or from a copy of itself;
- * the semi-implicit branch when semi_implicit_viscosity is set. */
+ * the semi-implicit branch when semi_implicit_viscosity is set.  The fixed
+ * schedule reads phi's ghosts as the constant extrapolation of the current
+ * level set and zeroes each component's wall faces after its update. */
 int fso_sourcing(fso_state* s, double dt) {
     if (!s->P.use_delta_function_formulation) return FS_OK;
-    if (s->P.semi_implicit_viscosity) return semi_implicit_viscosity(s, dt, &s->visc_iters);
+    if (is_fixed(s)) compute_normals(s);
+    if (s->P.semi_implicit_viscosity) {
+        int rc = semi_implicit_viscosity(s, dt, &s->visc_iters);
+        if (is_fixed(s))
+            for (int c = 1; c <= 3; c++) zero_walls(s, c);
+        return rc;
+    }
     double* comps[3] = {s->u, s->v, s->w};
     for (int c = 1; c <= 3; c++) {
         int ni, nj, nk;
@@ -759,6 +805,7 @@ int fso_sourcing(fso_state* s, double dt) {
                 for (int j = 0; j < nj; j++)
                     for (int i = 0; i < ni; i++) A3(fld, i, j, k) = A3(s->vtmp, i, j, k) + visc_face(s, c, i, j, k, dt, ux, vy, wz);
         }
+        if (is_fixed(s)) zero_walls(s, c);
     }
     return FS_OK;
 }
@@ -1062,6 +1109,28 @@ static double gradient_update(fso_state* s, int comp, double dt, int apply) {
     return vmax;
 }
 
+/* Fixed schedule: every face corrected with the pressure matrix's own face
+ * coefficient, u -= dt beta (p_c - p_n) / dx (beta is 0 on wall faces). */
+static double gradient_update_beta(fso_state* s, int comp, double dt) {
+    int ni, nj, nk;
+    extents(s, comp, &ni, &nj, &nk);
+    int di = comp == 1, dj = comp == 2, dk = comp == 3;
+    double* vel = comp == 1 ? s->u : comp == 2 ? s->v : s->w;
+    const double* beta = comp == 1 ? s->fbx : comp == 2 ? s->fby : s->fbz;
+    double od = comp == 1 ? s->cell.x.inv : comp == 2 ? s->cell.y.inv : s->cell.z.inv;
+    double vmax = 0;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                double* vr = &A3(vel, i, j, k);
+                double bf = A3(beta, i, j, k);
+                if (bf != 0) *vr -= dt * bf * (A3(s->p, i, j, k) - A3(s->p, i - di, j - dj, k - dk)) * od;
+                double a = fabs(*vr);
+                vmax = vmax > a ? vmax : a;
+            }
+    return vmax;
+}
+
 static void projection_solve(fso_state* s, double dt) {
     projection_density(s);
     projection_divergence(s, dt);
@@ -1080,9 +1149,12 @@ static void projection_solve(fso_state* s, double dt) {
     /* air/water corrects z-faces only: the x and y updates sit inside the
      * pipe-mode branches (:1188-1232, :1297-1341) while the max |u| is still
      * taken over every visited face */
-    int all = s->P.schedule == FS_SCHEDULE_FIXED;
-    s->max_vel[0] = gradient_update(s, 1, dt, all);
-    s->max_vel[1] = gradient_update(s, 2, dt, all);
+    if (is_fixed(s)) {
+        for (int c = 1; c <= 3; c++) s->max_vel[c - 1] = gradient_update_beta(s, c, dt);
+        return;
+    }
+    s->max_vel[0] = gradient_update(s, 1, dt, 0);
+    s->max_vel[1] = gradient_update(s, 2, dt, 0);
     s->max_vel[2] = gradient_update(s, 3, dt, 1);
 }
 
@@ -1167,8 +1239,10 @@ static int godunov_axis(double sg, double m, double p, double* val) {
     return AX_NONE;
 }
 
-/* ADVECTION_METHOD_3D::WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637. */
+/* ADVECTION_METHOD_3D::WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637
+ * (the fixed schedule takes the Euclidean norm in every Godunov case). */
 static void reinit_sweep(fso_state* s, double dt) {
+    const int exact = is_fixed(s);
     const double eps = s->P.levelset_epsilon;
     const gc_t* gc = &s->cell;
 #pragma omp parallel for num_threads(g_threads) schedule(static)
@@ -1189,7 +1263,7 @@ static void reinit_sweep(fso_state* s, double dt) {
                 int cz = cy ? godunov_axis(sg, mz, pz, &Z) : AX_NONE;
                 if (cx && cy && cz) {
                     double g;
-                    if ((cx == AX_SM || cx == AX_SP) && cy == AX_SP && cz == AX_DROP)
+                    if (!exact && (cx == AX_SM || cx == AX_SP) && cy == AX_SP && cz == AX_DROP)
                         g = sqrt(X * X) + Y * Y - 1; /* :1463, :1605 */
                     else
                         g = sqrt(X * X + Y * Y + Z * Z) - 1;
@@ -1216,7 +1290,10 @@ int fso_reinitialization(fso_state* s, double dt) {
                 A3(s->sgn, i, j, k) = f / sqrt(f * f + dx2);
             }
     for (int it = 0; it < s->P.iteration_number_for_reinitialization; it++) {
-        fill_linear(s, s->sghost, s->phi, s->nx, s->ny, s->nz, 3, 1, 0);
+        if (is_fixed(s))
+            refresh_levelset_ghost(s);
+        else
+            fill_linear(s, s->sghost, s->phi, s->nx, s->ny, s->nz, 3, 1, 0);
         reinit_sweep(s, fictious_dt);
     }
     return FS_OK;
