@@ -182,6 +182,10 @@ def bind(lib, prefix):
         lib.fs_adjoint_sourcing.argtypes = [vp, _D, _PD, _PD, _PD]
         lib.fs_weno5_transport.argtypes = [vp, _D, _PD, _PD, _PD, _PD, _PD]
         lib.fs_adjoint_weno5_transport.argtypes = [vp, _D] + [_PD] * 9
+        lib.fs_adjoint_step.argtypes = [vp, _D, _PD, _PD, _PD, _PD, C.POINTER(FsStepStats)]
+        lib.fs_levelset_loss.argtypes = [vp, _PD, C.POINTER(_D)]
+        lib.fs_adjoint_rollout.argtypes = ([vp, _I, _D, _I, _PD, C.POINTER(_D)] + [_PD] * 4 +
+                                           [C.POINTER(FsStepStats)])
         lib.fs_batch_create.argtypes = [_I, C.POINTER(FsParams), C.POINTER(FsGrid), _I, _I, C.POINTER(vp)]
         lib.fs_batch_destroy.argtypes = [vp]
         lib.fs_batch_destroy.restype = None
@@ -329,6 +333,34 @@ class Solver:
         self._check(self.lib.fs_adjoint_weno5_transport(self.h, dt, _ptr(phi), _ptr(u), _ptr(v), _ptr(w), _ptr(lam),
                                                         *[_ptr(a) for a in out]), "adjoint_weno5_transport")
         return out
+
+    def adjoint_step(self, dt, lphi, lu, lv, lw, allow_not_converged=False):
+        """Adjoint of one fs_step at the current fields: the adjoint of the
+        step's output in, of its input out (the fields are left as they were)."""
+        out = [np.array(a, dtype=np.float64, order="C", copy=True) for a in (lphi, lu, lv, lw)]
+        st = FsStepStats()
+        rc = self.lib.fs_adjoint_step(self.h, dt, *[_ptr(a) for a in out], C.byref(st))
+        self._check(rc, "adjoint_step", allow=(FS_ERR_NOT_CONVERGED,) if allow_not_converged else ())
+        return out, st.as_dict()
+
+    def levelset_loss(self, phi_target):
+        t = np.ascontiguousarray(phi_target, dtype=np.float64)
+        out = _D()
+        self._check(self.lib.fs_levelset_loss(self.h, _ptr(t), C.byref(out)), "levelset_loss")
+        return out.value
+
+    def adjoint_rollout(self, steps, dt, checkpoint_every, phi_target):
+        """Loss of the level set after `steps` steps and its gradient with
+        respect to the initial (phi, u, v, w); the fields are left at the
+        initial state."""
+        t = np.ascontiguousarray(phi_target, dtype=np.float64)
+        f = self.get_fields()
+        grads = [np.empty_like(f[c]) for c in ("phi", "u", "v", "w")]
+        loss = _D()
+        st = FsStepStats()
+        self._check(self.lib.fs_adjoint_rollout(self.h, steps, dt, checkpoint_every, _ptr(t), C.byref(loss),
+                                                *[_ptr(g) for g in grads], C.byref(st)), "adjoint_rollout")
+        return loss.value, grads, st.as_dict()
 
     def apply_operator(self, x):
         x = np.ascontiguousarray(x, dtype=np.float64)

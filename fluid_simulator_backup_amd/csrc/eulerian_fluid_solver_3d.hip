@@ -776,6 +776,256 @@ class EulerianFluidSolver3D {
         sync();
     }
 
+    // ---- reverse pass of one fixed-schedule step ---------------------------
+    // The step is differentiated as the fixed schedule computes it: every
+    // ghost copy is a function of the current fields, the gradient update
+    // uses the matrix's coefficients, the Godunov norm is Euclidean, so the
+    // step is piecewise smooth.  Discrete choices (upwind side, Godunov
+    // case, the +-eps clamp, the phase of a cell) are held at the forward's.
+    void CheckAdjointSupport() const {
+        if (!fixed()) throw FsFail{FS_ERR_UNSUPPORTED, "the adjoint step differentiates the fixed schedule"};
+        if (P.use_delta_function_formulation && P.is_sourcing_active && P.semi_implicit_viscosity)
+            throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the semi-implicit viscosity is not built"};
+        if (P.is_velocity_advection_active && !P.use_5th_weno_v)
+            throw FsFail{FS_ERR_UNSUPPORTED, "adjoint velocity advection needs use_5th_weno_v (ENO3's stencil choice "
+                                             "is discontinuous)"};
+    }
+
+    void EnsureAdjoint() {
+        if (A.ready) return;
+        double** f[] = {&A.lphi, &A.lu, &A.lv, &A.lw, &A.ladv, &A.a0, &A.a1, &A.lsg, &A.ghw, &A.y, &A.phiw, &A.tgt,
+                        &A.aU, &A.aV, &A.aW, &A.src, &A.du, &A.dv, &A.dw};
+        for (double** x : f) *x = dfield();
+        A.part = (double*)dalloc(sizeof(double) * (FS_LOSS_BLOCKS + 8));
+        A.ready = true;
+    }
+
+    void Zero(double* a) { fs_fill_value(st, a, L.total, 0.0); }
+    // dst = ca * a + cb * dst on component comp's box
+    void Blend(double* dst, const double* a, double ca, double cb, int comp) {
+        fs_combine(st, dst, a, ca, cb, L, box(comp));
+    }
+    void VelocityGhostsFrom(const double* su, const double* sv, const double* sw) {
+        const double* s3[3] = {su, sv, sw};
+        double* g3[3] = {gu, gv, gw};
+        for (int c = 0; c < 3; c++) {
+            fs_copy_box(st, g3[c], s3[c], L, box(c + 1));
+            fs_fill_zero_faces(st, g3[c], L, box(c + 1), 3);
+        }
+    }
+
+    // lam <- (dR/dphi)^T lam for the recorded reinitialization call `call`
+    // (identity when reinitialization is off); lam and spare swap roles.
+    void ReverseReinitCall(size_t call, double*& lam, double*& spare) {
+        if (call >= tape.reinit.size()) return;
+        const std::vector<double*>& sweeps = tape.reinit[call];
+        if (sweeps.empty()) return;
+        double fictious_dt = cell.x.d / P.scaling_number_for_reinitialization;
+        double dx2 = cell.x.d * cell.x.d;
+        fs_sign(st, sweeps[0], sgn, L, dx2);
+        Zero(A.lsg);
+        for (int s = (int)sweeps.size() - 1; s >= 0; s--) {
+            fs_copy_box(st, A.ghw, sweeps[s], L, box(0));
+            fs_fill_linear(st, A.ghw, A.ghw, L, box(0), 3, 0);
+            Zero(spare);
+            fs_adj_reinit_sweep(st, lam, sweeps[s], A.ghw, sgn, spare, A.lsg, L, cell, P.levelset_epsilon,
+                                fictious_dt);
+            fs_adj_fold_linear(st, spare, L, 3);
+            std::swap(lam, spare);
+        }
+        fs_adj_sign(st, sweeps[0], A.lsg, lam, L, dx2);
+    }
+
+    // (A.lu, A.lv, A.lw) <- adjoint of one SolveForRK stage's input; the
+    // level-set parts (viscosity and projection coefficients) add to A.ladv.
+    int ReverseRk(size_t r, double dt) {
+        const RkRecord& R = tape.rk.at(r);
+        double* lam[3] = {A.lu, A.lv, A.lw};
+        int rc = FS_OK;
+        // the stage's level set with constant ghosts, as Sourcing and the
+        // projection's assembly read it
+        fs_copy_box(st, A.phiw, R.phi, L, box(0));
+        fs_fill_const(st, A.phiw, A.phiw, L, box(0), 2);
+        if (P.is_projection_active) {
+            double light = P.air_bubble_rising ? P.air_density : P.water_density;
+            double heavy = P.air_bubble_rising ? P.water_density : P.air_density;
+            fs_density(st, A.phiw, rho, irho, L, light, heavy);
+            fs_fill_const(st, irho, irho, L, box(0), 1);
+            Zero(div);
+            fs_assemble(st, A.phiw, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
+            assembled = true;
+            Zero(rhs);
+            fs_adjoint_gradient_beta(st, 1, A.lu, bx, rhs, L, cell, dt);
+            fs_adjoint_gradient_beta(st, 2, A.lv, by, rhs, L, cell, dt);
+            fs_adjoint_gradient_beta(st, 3, A.lw, bz, rhs, L, cell, dt);
+            Zero(A.y);
+            stats.pcg_iterations += run_pcg64(P.stop_rule, P.tolerance, P.rel_tolerance, P.max_iteration, A.y);
+            if (hS->done != 1) {
+                rc = FS_ERR_NOT_CONVERGED;
+                last_error = "adjoint pressure solve reached max_iteration";
+            }
+            fs_adj_projection_phi(st, A.phiw, irho, R.p, A.y, A.lu, A.lv, A.lw, A.ladv, L, cell, dt);
+            fs_remove_mean(st, A.y, scratch, L);
+            fs_adjoint_divergence(st, A.lu, A.lv, A.lw, A.y, L, cell, -1 / dt);
+        }
+        if (P.is_sourcing_active && P.use_delta_function_formulation) {
+            Zero(A.a0);
+            const double* U[3] = {R.u2, R.ux, R.ux};  // what the x, y, z updates read
+            const double* V[3] = {R.v2, R.v2, R.vy};
+            for (int comp = 3; comp >= 1; comp--) {
+                fs_zero_walls(st, comp, lam[comp - 1], L);
+                fs_copy_box(st, A.src, lam[comp - 1], L, box(comp));
+                fs_adjoint_viscosity(st, comp, A.src, A.lu, A.lv, A.lw, A.phiw, L, mat, cell, dt);
+                fs_adj_viscosity_phi(st, comp, A.src, U[comp - 1], V[comp - 1], R.w2, A.phiw, A.a0, L, mat, cell, dt);
+            }
+            for (int c = 1; c <= 3; c++) fs_zero_outside(st, lam[c - 1], L, box(c));
+            fs_adj_fold_const(st, A.a0, L, 2);
+            Blend(A.ladv, A.a0, 1, 1, 0);
+        }
+        if (P.is_external_force_active)
+            for (int c = 1; c <= 3; c++) fs_zero_walls(st, c, lam[c - 1], L);
+        if (P.is_velocity_advection_active) {
+            VelocityGhostsFrom(R.u, R.v, R.w);
+            double* acc[3] = {A.aU, A.aV, A.aW};
+            const double* g3[3] = {gu, gv, gw};
+            const FsGridConst* mg[3] = {&macx, &macy, &macz};
+            for (double* a : acc) Zero(a);
+            for (int comp = 1; comp <= 3; comp++)
+                fs_adj_weno5_stage(st, lam[comp - 1], g3[comp - 1], gu, gv, gw, acc[comp - 1], A.aU, A.aV, A.aW, L,
+                                   comp, *mg[comp - 1], P.velocity_epsilon, dt);
+            for (int c = 1; c <= 3; c++) {
+                fs_zero_outside(st, acc[c - 1], L, box(c));
+                Blend(lam[c - 1], acc[c - 1], 1, 1, c);
+            }
+        }
+        return rc;
+    }
+
+    // lam <- (d phi'/d phi)^T lam for a level-set stage from `phik`; the
+    // sample velocity's part accumulates in A.aU, A.aV, A.aW (gu, gv, gw
+    // hold the velocity ghosts the stage read).
+    void ReverseLevelsetStage(const double* phik, double* lam, double dt) {
+        fs_copy_box(st, A.ghw, phik, L, box(0));
+        fs_fill_linear(st, A.ghw, A.ghw, L, box(0), 3, 0);
+        Zero(A.a0);
+        fs_adj_weno5_stage(st, lam, A.ghw, gu, gv, gw, A.a0, A.aU, A.aV, A.aW, L, 0, cell, P.levelset_epsilon, dt);
+        fs_adj_fold_linear(st, A.a0, L, 3);
+        Blend(lam, A.a0, 1, 1, 0);
+    }
+
+    // Reverse pass over the recorded step: on entry A.lphi, A.lu, A.lv, A.lw
+    // hold the adjoint of the step's output, on return of its input.
+    int ReverseStep(double dt) {
+        const int order = P.order_for_time_advancing;
+        int rc = FS_OK;
+        // reinitialization and its Runge-Kutta blends with tmp = phi_adv
+        double *lam = A.lphi, *spare = A.a1;
+        Zero(A.ladv);
+        if (order == 3) {
+            Blend(A.ladv, lam, (double)1 / 3, 0, 0);
+            Blend(lam, lam, 0, (double)2 / 3, 0);
+            ReverseReinitCall(2, lam, spare);
+            Blend(A.ladv, lam, (double)0.75, 1, 0);
+            Blend(lam, lam, 0, (double)0.25, 0);
+        } else {
+            Blend(A.ladv, lam, (double)0.5, 0, 0);
+            Blend(lam, lam, 0, (double)0.5, 0);
+        }
+        ReverseReinitCall(1, lam, spare);
+        ReverseReinitCall(0, lam, spare);
+        Blend(A.ladv, lam, 1, 1, 0);
+        // velocity stages and their blends with the step's input velocity
+        double* lv3[3] = {A.lu, A.lv, A.lw};
+        double* d3[3] = {A.du, A.dv, A.dw};
+        auto rk = [&](size_t r) {
+            int e = ReverseRk(r, dt);
+            if (e != FS_OK) rc = e;
+        };
+        if (order == 1) {
+            rk(0);
+        } else {
+            for (int c = 0; c < 3; c++) {
+                Blend(d3[c], lv3[c], order == 2 ? (double)0.5 : (double)1 / 3, 0, c + 1);
+                Blend(lv3[c], lv3[c], 0, order == 2 ? (double)0.5 : (double)2 / 3, c + 1);
+            }
+            if (order == 3) {
+                rk(2);
+                for (int c = 0; c < 3; c++) {
+                    Blend(d3[c], lv3[c], (double)0.75, 1, c + 1);
+                    Blend(lv3[c], lv3[c], 0, (double)0.25, c + 1);
+                }
+            }
+            rk(1);
+            rk(0);
+            for (int c = 0; c < 3; c++) Blend(lv3[c], d3[c], 1, 1, c + 1);
+        }
+        // level-set advection with the step's input velocity
+        if (P.is_levelset_advection_active && !tape.ls.empty()) {
+            VelocityGhostsFrom(tape.u, tape.v, tape.w);
+            for (double* a : {A.aU, A.aV, A.aW}) Zero(a);
+            if (order == 3) {
+                ReverseLevelsetStage(tape.ls[0], A.ladv, dt);
+            } else {
+                Blend(A.a1, A.ladv, (double)0.5, 0, 0);
+                Blend(A.ladv, A.ladv, 0, (double)0.5, 0);
+                ReverseLevelsetStage(tape.ls[1], A.ladv, dt);
+                ReverseLevelsetStage(tape.ls[0], A.ladv, dt);
+                Blend(A.ladv, A.a1, 1, 1, 0);
+            }
+            double* acc[3] = {A.aU, A.aV, A.aW};
+            for (int c = 1; c <= 3; c++) {
+                fs_zero_outside(st, acc[c - 1], L, box(c));
+                Blend(lv3[c - 1], acc[c - 1], 1, 1, c);
+            }
+        }
+        fs_copy_box(st, A.lphi, A.ladv, L, box(0));
+        check_launch();
+        return rc;
+    }
+
+    // the recorded step, from the current fields
+    int RecordStep(double dt) {
+        recording = true;
+        int rc;
+        try {
+            rc = AdvanceOneTimeStepThread(dt);
+        } catch (...) {
+            recording = false;
+            throw;
+        }
+        recording = false;
+        return rc;
+    }
+
+    SavedState& Saved(size_t n) {
+        while (saved.size() <= n) {
+            SavedState sv;
+            sv.phi = dfield();
+            sv.u = dfield();
+            sv.v = dfield();
+            sv.w = dfield();
+            sv.p = dfield();
+            saved.push_back(sv);
+        }
+        return saved[n];
+    }
+    void SaveState(SavedState& sv) {
+        fs_copy_box(st, sv.phi, phi, L, box(0));
+        fs_copy_box(st, sv.u, u, L, box(1));
+        fs_copy_box(st, sv.v, v, L, box(2));
+        fs_copy_box(st, sv.w, w, L, box(3));
+        fs_copy_box(st, sv.p, p, L, box(0));
+    }
+    void LoadState(const double* sphi, const double* su, const double* sv, const double* sw, const double* sp) {
+        fs_copy_box(st, phi, sphi, L, box(0));
+        fs_fill_const(st, phi, phi, L, box(0), 2);
+        fs_copy_box(st, u, su, L, box(1));
+        fs_copy_box(st, v, sv, L, box(2));
+        fs_copy_box(st, w, sw, L, box(3));
+        fs_copy_box(st, p, sp, L, box(0));
+    }
+    void LoadState(const SavedState& s_) { LoadState(s_.phi, s_.u, s_.v, s_.w, s_.p); }
+
     // host adjoint fields in, through the spares (their ghosts hold scatter
     // leftovers afterwards and are cleared: the forward relies on zero ghosts)
     template <typename F>
@@ -1224,6 +1474,102 @@ int fs_adjoint_sourcing(fs_solver* s, double dt, double* lu, double* lv, double*
             S.AdjointSourcing(dt, a, b, c);
             return FS_OK;
         });
+    });
+}
+
+int fs_adjoint_step(fs_solver* s, double dt, double* lphi, double* lu, double* lv, double* lw,
+                    fs_step_stats* stats) {
+    if (!(dt > 0) || !std::isfinite(dt) || !lphi || !lu || !lv || !lw) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.CheckAdjointSupport();
+        S.EnsureAdjoint();
+        S.stats = fs_step_stats{};
+        int rc = S.RecordStep(dt);
+        for (double* a : {S.A.lphi, S.A.lu, S.A.lv, S.A.lw}) S.Zero(a);
+        S.put(S.A.lphi, lphi, 0);
+        S.put(S.A.lu, lu, 1);
+        S.put(S.A.lv, lv, 2);
+        S.put(S.A.lw, lw, 3);
+        int rr = S.ReverseStep(dt);
+        S.get(S.A.lphi, lphi, 0);
+        S.get(S.A.lu, lu, 1);
+        S.get(S.A.lv, lv, 2);
+        S.get(S.A.lw, lw, 3);
+        S.LoadState(S.tape.phi, S.tape.u, S.tape.v, S.tape.w, S.tape.p);
+        S.sync();
+        if (stats) *stats = S.stats;
+        return rc != FS_OK ? rc : rr;
+    });
+}
+
+static double read_loss(EulerianFluidSolver3D& S) {
+    double h = 0;
+    HIPCHECK(hipMemcpyAsync(&h, S.A.part + FS_LOSS_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, S.st));
+    S.sync();
+    return h;
+}
+
+int fs_levelset_loss(fs_solver* s, const double* phi_target, double* loss) {
+    if (!phi_target || !loss) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.EnsureAdjoint();
+        S.put(S.A.tgt, phi_target, 0);
+        fs_levelset_loss(S.st, S.phi, S.A.tgt, nullptr, S.A.part, S.L, 1.5 * S.cell.x.d);
+        S.check_launch();
+        *loss = read_loss(S);
+        return FS_OK;
+    });
+}
+
+// Checkpointed reverse sweep: the forward keeps the state at the start of
+// every `checkpoint_every`-th step; the reverse pass walks the segments last
+// to first, re-running a segment from its checkpoint to keep each step's
+// input, then re-records and reverses its steps one by one.
+int fs_adjoint_rollout(fs_solver* s, int steps, double dt, int checkpoint_every, const double* phi_target,
+                       double* loss, double* gphi, double* gu, double* gv, double* gw, fs_step_stats* stats) {
+    if (steps < 1 || checkpoint_every < 1 || !(dt > 0) || !std::isfinite(dt) || !phi_target || !loss)
+        return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.CheckAdjointSupport();
+        S.EnsureAdjoint();
+        S.stats = fs_step_stats{};
+        const int every = checkpoint_every;
+        const int nck = (steps + every - 1) / every;
+        S.Saved((size_t)(nck + std::min(every, steps)) - 1);  // checkpoints, then one segment's states
+        int rc = FS_OK;
+        auto note = [&](int r) {
+            if (r != FS_OK && rc == FS_OK) rc = r;
+        };
+        S.put(S.A.tgt, phi_target, 0);
+        for (int k = 0; k < steps; k++) {
+            if (k % every == 0) S.SaveState(S.saved[k / every]);
+            note(S.AdvanceOneTimeStepThread(dt));
+        }
+        for (double* a : {S.A.lphi, S.A.lu, S.A.lv, S.A.lw}) S.Zero(a);
+        fs_levelset_loss(S.st, S.phi, S.A.tgt, S.A.lphi, S.A.part, S.L, 1.5 * S.cell.x.d);
+        S.check_launch();
+        *loss = read_loss(S);
+        for (int seg = nck - 1; seg >= 0; seg--) {
+            const int k0 = seg * every, k1 = std::min(steps, k0 + every);
+            S.LoadState(S.saved[seg]);
+            for (int k = k0; k < k1; k++) {
+                S.SaveState(S.saved[nck + (k - k0)]);
+                if (k + 1 < k1) note(S.AdvanceOneTimeStepThread(dt));
+            }
+            for (int k = k1 - 1; k >= k0; k--) {
+                S.LoadState(S.saved[nck + (k - k0)]);
+                note(S.RecordStep(dt));
+                note(S.ReverseStep(dt));
+            }
+        }
+        S.LoadState(S.saved[0]);
+        if (gphi) S.get(S.A.lphi, gphi, 0);
+        if (gu) S.get(S.A.lu, gu, 1);
+        if (gv) S.get(S.A.lv, gv, 2);
+        if (gw) S.get(S.A.lw, gw, 3);
+        S.sync();
+        if (stats) *stats = S.stats;
+        return rc;
     });
 }
 

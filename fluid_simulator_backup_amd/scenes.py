@@ -111,7 +111,36 @@ def bubbles(n, seed=2, count=64):
     return phi, zero_velocity(n), params
 
 
-SCENES = {"dam_break": dam_break, "rising_drop": rising_drop, "bubbles": bubbles}
+def slosh(n, seed=4):
+    """Sloshing tank: water below the tilted free surface z = 0.5 + 0.1 (x -
+    0.5) (phi <= 0, water_drop mode as the dam break), air above; densities
+    and viscosities as the dam break."""
+    x, y, z = cell_centres(n)
+    phi = z - (0.5 + 0.1 * (x - 0.5))
+    params = dict(air_bubble_rising=0, water_density=1000.0, air_density=1.0,
+                  water_viscosity=1e-3, air_viscosity=1.8e-5)
+    return phi, zero_velocity(n), params
+
+
+def gaussian_random_velocity(n, seed, sigma_cells=8.0, std=0.05):
+    """MAC velocity whose components are white noise (numpy's PCG64 stream
+    of `seed`) smoothed by a separable Gaussian of sigma_cells cells and
+    rescaled to standard deviation `std`; the normal wall faces are zero."""
+    from scipy.ndimage import gaussian_filter
+    rng = np.random.default_rng(seed)
+    out = []
+    for shape, axis in (((n, n, n + 1), 2), ((n, n + 1, n), 1), ((n + 1, n, n), 0)):
+        f = gaussian_filter(rng.standard_normal(shape), sigma=sigma_cells, mode="reflect")
+        f *= std / f.std()
+        idx = [slice(None)] * 3
+        for wall in (0, -1):
+            idx[axis] = wall
+            f[tuple(idx)] = 0
+        out.append(f)
+    return tuple(out)
+
+
+SCENES = {"dam_break": dam_break, "rising_drop": rising_drop, "bubbles": bubbles, "slosh": slosh}
 
 
 def make(name, n, seed=None, **param_overrides):

@@ -236,6 +236,43 @@ int  fs_weno5_transport(fs_solver* s, double dt, const double* phi, const double
 int  fs_adjoint_weno5_transport(fs_solver* s, double dt, const double* phi, const double* u, const double* v,
                                 const double* w, const double* lam, double* lphi, double* lu, double* lv, double* lw);
 
+/* ---- adjoint time step -------------------------------------------------------
+ * No reference counterpart (the reference has no adjoint): reverse-mode
+ * derivatives of EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread
+ * (EULERIAN_FLUID_SOLVER_3D.h:803-1111) as fs_step computes it with
+ * schedule FS_SCHEDULE_FIXED, which is piecewise smooth: every ghost copy is
+ * a function of the current fields, the pressure gradient uses the matrix's
+ * face coefficients, the Godunov norm is Euclidean.  Every phase is
+ * differentiated, the level set's influence on the GFM coefficients, the
+ * smoothed densities and viscosities, and the reinitialization's sign
+ * function included; the discrete choices (upwind side, Godunov case, +-eps
+ * clamp, a cell's phase) are held at the forward's.  FS_ERR_UNSUPPORTED for
+ * the compat schedule, semi_implicit_viscosity, or ENO3 velocity advection
+ * (use_5th_weno_v = 0).  Orders 1, 2 and 3.  The adjoint pressure solves run
+ * the FP64 Jacobi-PCG with the handle's stop rule.
+ *
+ * fs_adjoint_step: the step from the current fields is recomputed with its
+ * tape; lphi, lu, lv, lw (host, interior layout as fs_set_fields) carry the
+ * adjoint of the step's output in and of its input out.  The fields are left
+ * as they were.
+ *
+ * fs_levelset_loss: sum over cells of (H(phi) - H(phi_target))^2 with H the
+ * projection's smoothed Heaviside (PROJECTION_3D.h:1452-1472) at
+ * eps = 1.5 dx, for the current level set.
+ *
+ * fs_adjoint_rollout: from the current fields, `steps` fs_step calls of
+ * length dt, the loss above at the end, and its gradient with respect to the
+ * initial phi, u, v, w (any output may be NULL), by the checkpointed reverse
+ * sweep: the state at the start of every checkpoint_every-th step is kept on
+ * the device, each segment is re-run from its checkpoint and its steps are
+ * reversed one by one.  The fields are left at the initial state. */
+int  fs_adjoint_step(fs_solver* s, double dt, double* lphi, double* lu, double* lv, double* lw,
+                     fs_step_stats* stats);
+int  fs_levelset_loss(fs_solver* s, const double* phi_target, double* loss);
+int  fs_adjoint_rollout(fs_solver* s, int steps, double dt, int checkpoint_every, const double* phi_target,
+                        double* loss, double* grad_phi, double* grad_u, double* grad_v, double* grad_w,
+                        fs_step_stats* stats);
+
 /* ---- batches of independent scenes ---------------------------------------
  * No reference counterpart: the reference advances one world per process
  * (SIMULATION_WORLD::AdvanceOneFrame, SIMULATION_WORLD.h:320-330).  A batch

@@ -1,0 +1,289 @@
+"""Adjoint of the whole fixed-schedule step (fs_adjoint_step) and of
+checkpointed rollouts (fs_adjoint_rollout), through the C-ABI.
+
+The fixed schedule is piecewise smooth, so central differences of the
+forward converge to the adjoint's directional derivative.  The checks:
+  - one step, each phase alone and all together (orders 1-3): the four
+    blocks (phi -> phi, phi -> velocity, velocity -> phi, velocity ->
+    velocity) of <lam, step(x + h d)> against Richardson-extrapolated
+    central differences, the identity part taken out where a block has it;
+  - a rollout's gradient equals the chain of single-step adjoints (the
+    checkpointing re-runs bit for bit);
+  - rollout gradients against central differences of the GPU forward and of
+    the CPU checker's forward (independent code) along seeded directions.
+"""
+import numpy as np
+import pytest
+
+from fluid_simulator_backup_amd import _abi, scenes
+from helpers import smooth_velocity
+
+pytestmark = pytest.mark.gpu
+
+FIXED = dict(schedule=_abi.FS_SCHEDULE_FIXED, use_5th_weno_v=1, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
+             rel_tolerance=1e-13, max_iteration=20000)
+
+
+def _zero_walls(u, v, w):
+    u[:, :, [0, -1]] = 0
+    v[:, [0, -1], :] = 0
+    w[[0, -1], :, :] = 0
+
+
+def _dot(a, b):
+    return sum(float(np.vdot(x, y)) for x, y in zip(a, b))
+
+
+def _norm(a):
+    return float(np.sqrt(sum(float(np.vdot(x, x)) for x in a)))
+
+
+def _solver(hip_lib, name, n, **over):
+    kw = dict(FIXED)
+    kw.update(over)
+    phi, _, P, g = scenes.make(name, n, **kw)
+    return _abi.Solver(hip_lib, "fs_", P, g, 0), phi
+
+
+def _set(s, x):
+    s.set_fields(x[0], x[1], x[2], x[3], p=np.zeros_like(x[0]))
+
+
+def _step(s, x, dt):
+    _set(s, x)
+    s.step(dt)
+    f = s.get_fields()
+    return [f[c] for c in ("phi", "u", "v", "w")]
+
+
+def _axpy(x, h, d):
+    return [a + h * b for a, b in zip(x, d)]
+
+
+PHASES = {
+    "levelset": dict(is_velocity_advection_active=0, is_external_force_active=0, is_sourcing_active=0,
+                     is_projection_active=0, sussmanstyle_reinitialization=0),
+    "velocity": dict(is_levelset_advection_active=0, is_external_force_active=0, is_sourcing_active=0,
+                     is_projection_active=0, sussmanstyle_reinitialization=0),
+    # a viscous, light pair so the level set's influence through the smoothed
+    # viscosity and density is well above round-off
+    "forces": dict(is_levelset_advection_active=0, is_velocity_advection_active=0, is_projection_active=0,
+                   sussmanstyle_reinitialization=0, water_density=2.0, air_density=1.0, water_viscosity=0.05,
+                   air_viscosity=0.01),
+    "projection": dict(is_levelset_advection_active=0, is_velocity_advection_active=0, is_external_force_active=0,
+                       is_sourcing_active=0, sussmanstyle_reinitialization=0),
+    "reinit": dict(is_levelset_advection_active=0, is_velocity_advection_active=0, is_external_force_active=0,
+                   is_sourcing_active=0, is_projection_active=0),
+    "all": dict(),
+}
+
+
+@pytest.mark.parametrize("name,phase,order", [("dam_break", k, 1) for k in PHASES] +
+                         [("rising_drop", "all", 1), ("dam_break", "all", 2), ("rising_drop", "all", 3)])
+def test_adjoint_step_finite_differences(hip_lib, name, phase, order):
+    n = 16
+    s, phi = _solver(hip_lib, name, n, order_for_time_advancing=order, **PHASES[phase])
+    vel = [a.copy() for a in smooth_velocity(n, seed=3, amp=0.2)]
+    _zero_walls(*vel)
+    x = [phi] + vel
+    dt = 2e-3
+    rng = np.random.default_rng(11)
+    zero = [np.zeros_like(a) for a in x]
+    lam_phi = [rng.standard_normal(phi.shape)] + zero[1:]
+    lam_vel = [zero[0]] + [rng.standard_normal(a.shape) for a in vel]
+    d_phi = [0.05 * rng.standard_normal(phi.shape)] + zero[1:]
+    d_vel = [zero[0]] + [0.05 * rng.standard_normal(a.shape) for a in vel]
+    _zero_walls(*d_vel[1:])
+    adj = {}
+    for key, lam in (("phi", lam_phi), ("vel", lam_vel)):
+        _set(s, x)
+        adj[key], st = s.adjoint_step(dt, *lam)
+    # Richardson over (h, h/2) cancels the h^2 term of central differences.
+    # h = 4e-6 (perturbations of 2e-7 per value) sits between the WENO5
+    # weights' nonlinearity (their epsilon is 1e-6: larger steps bring higher
+    # orders in) and the rounding noise of the differences (smaller steps)
+    h = 4e-6
+    worst = 0.0
+    for lk, lam in (("phi", lam_phi), ("vel", lam_vel)):
+        for dk, d in (("phi", d_phi), ("vel", d_vel)):
+            J = lambda t: _dot(lam, _step(s, _axpy(x, t, d), dt))
+            fd = lambda t: (J(t) - J(-t)) / (2 * t)
+            rich = (4 * fd(h / 2) - fd(h)) / 3
+            ad = _dot(adj[lk], d)
+            ident = _dot(lam, d) if lk == dk else 0.0
+            scale = _norm(lam) * _norm(d)
+            err = abs(rich - ad)
+            print(name, phase, order, f"{dk}->{lk}", "fd", rich, "adjoint", ad, "identity", ident, "err", err)
+            # the differences resolve the whole derivative to ~1e-6 of itself:
+            # every pressure solve stops at 1e-13 relative residual and a
+            # Jacobi-PCG iterate is not linear in its right-hand side, so at
+            # density ratio 1000 the reference carries ~1e-6 of the total
+            assert err <= 1e-5 * abs(ad - ident) + 1e-6 * abs(ad) + 1e-10 * scale, (dk, lk, rich, ad, ident)
+            if abs(ad - ident) > 1e-6 * scale:
+                worst = max(worst, err / abs(ad - ident))
+    print("worst relative error of the non-identity parts", worst)
+
+
+def test_adjoint_step_leaves_fields_and_rejects_unsupported(hip_lib):
+    n = 12
+    s, phi = _solver(hip_lib, "dam_break", n)
+    vel = smooth_velocity(n)
+    s.set_fields(phi, *vel)
+    before = s.get_fields()
+    lam = [np.ones_like(before[c]) for c in ("phi", "u", "v", "w")]
+    s.adjoint_step(1e-3, *lam)
+    after = s.get_fields()
+    for c in before:
+        assert np.array_equal(before[c], after[c]), c
+    for over in (dict(schedule=_abi.FS_SCHEDULE_COMPAT), dict(use_5th_weno_v=0), dict(semi_implicit_viscosity=1)):
+        t, _ = _solver(hip_lib, "dam_break", n, **over)
+        t.set_fields(phi, *vel)
+        with pytest.raises(_abi.FsError) as e:
+            t.adjoint_step(1e-3, *lam)
+        assert e.value.code == _abi.FS_ERR_UNSUPPORTED
+
+
+def _loss_grad_numpy(phi, target, eps):
+    def H(f):
+        return np.where(f < -eps, 0.0, np.where(f > eps, 1.0, 0.5 + f / (2 * eps) + np.sin(np.pi * f / eps) / (2 * np.pi)))
+    dH = np.where(np.abs(phi) > eps, 0.0, (1 + np.cos(np.pi * phi / eps)) / (2 * eps))
+    diff = H(phi) - H(target)
+    return float((diff * diff).sum()), 2 * diff * dH
+
+
+def _slosh(hip_lib, n, seed, **over):
+    s, phi = _solver(hip_lib, "slosh", n, **over)
+    vel = list(scenes.gaussian_random_velocity(n, seed, sigma_cells=max(2.0, n / 16)))
+    return s, phi, vel
+
+
+def _target(s, phi, vel, steps, dt):
+    _set(s, [phi] + vel)
+    for _ in range(steps):
+        s.step(dt)
+    return s.get_fields()["phi"]
+
+
+def test_rollout_equals_chained_step_adjoints(hip_lib):
+    """Five steps with a checkpoint every two (segments of 2, 2, 1): the
+    rollout's gradient is the chain of single-step adjoints from the stored
+    states."""
+    n, steps, dt = 16, 5, 4e-3
+    s, phi, vel = _slosh(hip_lib, n, 40)
+    target = _target(s, phi, vel, steps, dt)
+    _, _, vel1 = _slosh(hip_lib, n, 41)
+    x0 = [phi] + vel1
+    _set(s, x0)
+    loss, grads, _ = s.adjoint_rollout(steps, dt, 2, target)
+    # chain by hand
+    _set(s, x0)
+    states = []
+    for _ in range(steps):
+        f = s.get_fields()
+        states.append(f)
+        s.step(dt)
+    phiT = s.get_fields()["phi"]
+    loss2, lphi = _loss_grad_numpy(phiT, target, 1.5 / n)
+    assert abs(loss - loss2) <= 1e-12 * loss2
+    lam = [lphi] + [np.zeros_like(a) for a in vel1]
+    for f in reversed(states):
+        s.set_fields(f["phi"], f["u"], f["v"], f["w"], p=f["p"])
+        lam, _ = s.adjoint_step(dt, *lam)
+    for g, l in zip(grads, lam):
+        assert np.abs(g - l).max() <= 1e-9 * np.abs(l).max()
+
+
+def _directions(n, count, seed):
+    """Gaussian random velocity directions (smoothed like the initial field,
+    zero normal wall faces) of unit RMS."""
+    return [list(scenes.gaussian_random_velocity(n, seed + q, sigma_cells=max(2.0, n / 16), std=1.0))
+            for q in range(count)]
+
+
+def _rollout_loss(s, x, steps, dt, target):
+    _set(s, x)
+    for _ in range(steps):
+        s.step(dt)
+    return s.levelset_loss(target)
+
+
+def test_rollout_gradient_finite_differences(hip_lib):
+    """Eight steps of the sloshing tank at 32^3, a checkpoint every 4: the
+    gradient with respect to the initial velocity along seeded unit-RMS
+    directions matches central differences with step 5e-7 to 1e-5."""
+    n, steps, dt = 32, 8, 4e-3
+    s, phi, vel = _slosh(hip_lib, n, 40)
+    target = _target(s, phi, vel, steps, dt)
+    _, _, vel1 = _slosh(hip_lib, n, 41)
+    _set(s, [phi] + vel1)
+    loss, grads, st = s.adjoint_rollout(steps, dt, 4, target)
+    print("loss", loss, "stats", st)
+    h = 5e-7
+    for d in _directions(n, 3, 100):
+        fd = (_rollout_loss(s, [phi] + _axpy(vel1, h, d), steps, dt, target) -
+              _rollout_loss(s, [phi] + _axpy(vel1, -h, d), steps, dt, target)) / (2 * h)
+        ad = _dot(grads[1:], d)
+        print("fd", fd, "adjoint", ad, "rel", abs(fd - ad) / abs(ad))
+        assert abs(fd - ad) <= 1e-5 * abs(ad)
+
+
+def test_slosh_128_rollout_gradient(hip_lib):
+    """The sloshing-tank adjoint rollout at full size: 128^3, 16 FP64 steps,
+    a checkpoint every 4, target from the Gaussian random velocity of seed 4,
+    gradient at seed 5, two of the seeded directions (40, 41): Richardson-
+    extrapolated central differences with steps 1e-6 and 5e-7 agree to 1e-5
+    (the loss is curved enough that one central difference at 1e-6 is off by
+    up to ~3e-5 on its own)."""
+    n, steps = 128, 16
+    s, phi = _solver(hip_lib, "slosh", n, rel_tolerance=1e-12, max_iteration=50000)
+    vt = list(scenes.gaussian_random_velocity(n, 4, sigma_cells=8.0))
+    v0 = list(scenes.gaussian_random_velocity(n, 5, sigma_cells=8.0))
+    _set(s, [phi] + v0)
+    dt = s.cfl_dt(0.5, 1)
+    target = _target(s, phi, vt, steps, dt)
+    _set(s, [phi] + v0)
+    loss, grads, st = s.adjoint_rollout(steps, dt, 4, target)
+    print("loss", loss, "dt", dt, "stats", st)
+    h = 1e-6
+    for q in range(2):
+        d = list(scenes.gaussian_random_velocity(n, 40 + q, sigma_cells=8.0, std=1.0))
+        fd = {t: (_rollout_loss(s, [phi] + _axpy(v0, t, d), steps, dt, target) -
+                  _rollout_loss(s, [phi] + _axpy(v0, -t, d), steps, dt, target)) / (2 * t) for t in (h, h / 2)}
+        rich = (4 * fd[h / 2] - fd[h]) / 3
+        ad = _dot(grads[1:], d)
+        print("direction", q, "central", fd, "richardson", rich, "adjoint", ad, "rel", abs(rich - ad) / abs(ad))
+        assert abs(rich - ad) <= 1e-5 * abs(ad)
+
+
+def test_rollout_gradient_against_cpu_checker(oracle_lib, hip_lib):
+    """The GPU adjoint against central differences (step 5e-7, unit-RMS
+    smooth directions) of the CPU checker's own forward (independent code,
+    same fixed schedule): two velocity directions and one level-set
+    direction, three steps at 16^3."""
+    n, steps, dt = 16, 3, 4e-3
+    s, phi, vel = _slosh(hip_lib, n, 40)
+    target = _target(s, phi, vel, steps, dt)
+    _, _, vel1 = _slosh(hip_lib, n, 41)
+    _set(s, [phi] + vel1)
+    loss, grads, _ = s.adjoint_rollout(steps, dt, 2, target)
+    _, _, P, g = scenes.make("slosh", n, **FIXED)
+    cpu = _abi.Solver(oracle_lib, "fso_", P, g)
+    oracle_lib.fso_set_viscosity_inplace(cpu.h, 0)
+
+    def L(x):
+        _set(cpu, x)
+        for _ in range(steps):
+            cpu.step(dt)
+        return _loss_grad_numpy(cpu.get_fields()["phi"], target, 1.5 / n)[0]
+
+    assert abs(L([phi] + vel1) - loss) <= 1e-8 * loss
+    from scipy.ndimage import gaussian_filter
+    dphi = gaussian_filter(np.random.default_rng(5).standard_normal(phi.shape), 2.0, mode="reflect")
+    dirs = [[np.zeros_like(phi)] + d for d in _directions(n, 2, 200)]
+    dirs.append([dphi / dphi.std()] + [np.zeros_like(a) for a in vel1])
+    h = 5e-7
+    for d in dirs:
+        fd = (L(_axpy([phi] + vel1, h, d)) - L(_axpy([phi] + vel1, -h, d))) / (2 * h)
+        ad = _dot(grads, d)
+        print("cpu fd", fd, "gpu adjoint", ad, "rel", abs(fd - ad) / abs(ad))
+        assert abs(fd - ad) <= 1e-5 * abs(ad)
