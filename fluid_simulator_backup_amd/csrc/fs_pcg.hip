@@ -5,7 +5,8 @@
 // cells in x-y marching zc planes in z (Tiling::zc).  Each thread owns V
 // consecutive x-cells (V = 2 in FP64, 4 in FP32: 16-byte loads and stores),
 // keeps p(k-1), p(k), p(k+1) and beta_z(k), beta_z(k+1) in registers, and
-// takes its x-neighbours at the vector ends with two scalar loads that hit L1.
+// reads its y- and x-neighbours from the plane's tile rows staged in LDS (on
+// large grids; small ones take them from L1/L2 with their own loads).
 //
 // One iteration is two kernels: K1 runs the loop tail of the previous
 // iteration (stop test, beta), forms its direction p = p_old*beta + s on the
@@ -123,34 +124,41 @@ __device__ __forceinline__ void vst_s(T* __restrict__ p, const T (&o)[V]) {
 #define NTB(b) ((FS_NT_MASK >> (b)) & 1)
 
 // ---- tiling -------------------------------------------------------------------
+#ifndef FS_MIN_TILES
+#define FS_MIN_TILES 512
+#endif
+// The fused SpMV stages its operand's tile rows in LDS on grids of at least
+// FS_LDS_MIN_CELLS cells (smaller grids are latency-bound and pay for the
+// barriers), with tiles of its own (at least FS_K1_MIN_TILES of them; its
+// partial count then differs from the update's, and each consumer is told
+// its producer's count).
+// (Measured: at 256^3 K1 110 -> 96 us in FP32 and 226 -> 210 us in FP64 with
+// LDS and 1024 tiles; neutral at 128^3, slower at 64^3.)
+#ifndef FS_LDS_MIN_CELLS
+#define FS_LDS_MIN_CELLS (128L * 128 * 128)
+#endif
+#ifndef FS_K1_MIN_TILES
+#define FS_K1_MIN_TILES 1024
+#endif
 struct Tiling {
     int zc;  // z-planes per tile
     int ntx, nty, ntz, ntiles, nb, v;
 };
 
-static Tiling make_tiling(const FsLayout& L, int v) {
+static Tiling make_tiling(const FsLayout& L, int v, int min_tiles = FS_MIN_TILES) {
     Tiling t;
     t.v = v;
     t.ntx = (L.nx + TBX * v - 1) / (TBX * v);
     t.nty = (L.ny + TBY - 1) / TBY;
-    // the deepest z-chunk that still leaves at least 512 tiles: a longer
-    // z-march re-reads fewer halo planes (measured at 256^3: 32 planes beat
-    // 16 and 8), enough tiles keep every CU busy on small grids
+    // the deepest z-chunk that still leaves at least min_tiles tiles: a
+    // longer z-march re-reads fewer halo planes (measured at 256^3: 32 planes
+    // beat 16 and 8 for the plain kernels), enough tiles keep every CU busy
     for (t.zc = 32; t.zc > 4; t.zc >>= 1)
-        if ((long)t.ntx * t.nty * ((L.nz + t.zc - 1) / t.zc) >= 512) break;
+        if ((long)t.ntx * t.nty * ((L.nz + t.zc - 1) / t.zc) >= min_tiles) break;
     t.ntz = (L.nz + t.zc - 1) / t.zc;
     t.ntiles = t.ntx * t.nty * t.ntz;
     int nb = t.ntiles < FS_PCG_MAX_BLOCKS ? t.ntiles : FS_PCG_MAX_BLOCKS;
     if (nb >= 8) nb = nb / 8 * 8;
-    t.nb = nb;
-    return t;
-}
-
-// Tiles of width v walked by the workgroup count of another tiling: kernels
-// of one iteration may use different vector widths, but producer and
-// consumer of the partial sums must agree on the number of workgroups.
-static Tiling make_tiling_nb(const FsLayout& L, int v, int nb) {
-    Tiling t = make_tiling(L, v);
     t.nb = nb;
     return t;
 }
@@ -383,6 +391,123 @@ __device__ __forceinline__ void march_apply(const FsLayout& L, const OpT<T>& op,
     }
 }
 
+// march_apply for a whole workgroup with the operand staged in LDS: each
+// plane's tile rows j0-1 .. j0+TBY (and one cell either side in x) are formed
+// once by the loader and written to LDS, where the y- and x-neighbours are
+// read; the z-neighbours stay in registers.  Two plane buffers alternate, so
+// one barrier per plane separates a plane's writes from its reads (a
+// workgroup cannot run two planes ahead of its slowest thread).  Every
+// thread of the workgroup must call it with the same tile (barriers); cells
+// outside the grid are staged but not computed.  Operand values and the
+// operation order are those of march_apply.
+#define LDS_ROW(V) (8 + TBX * (V))
+template <typename T, int V>
+struct StageLds {
+    T a[2][TBY + 2][LDS_ROW(V)];
+};
+template <typename T, int V, typename LD, typename F>
+__device__ __forceinline__ void march_apply_lds(const FsLayout& L, const OpT<T>& op, const LD& x, int tile_x0, int j0,
+                                                int k0, int zc, T pin_b, T pin_d, StageLds<T, V>& sh, F&& f) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = tile_x0 + tx * V, j = j0 + ty;
+    const bool inside = i0 < L.nx && j < L.ny;
+    const int k1 = min(k0 + zc, L.nz);
+    // halo duties: rows j0-1 and j0+TBY (threads of rows 0 and 1), and the
+    // x-halo cell either side of the 6 rows (12 threads of row 2)
+    const int hrow = ty == 0 ? j0 - 1 : j0 + TBY;
+    const bool hload = ty < 2 && i0 < L.nx;
+    // the right x-halo cell lies past the tile; when the tile overhangs the
+    // grid its right neighbour column is a centre slot (staged as the zero
+    // ghost) and is not loaded
+    const int xr = tile_x0 + TBX * V;
+    const int xrow = tx % (TBY + 2), xside = tx / (TBY + 2);
+    const bool xload = ty == 2 && tx < 2 * (TBY + 2) && (xside == 0 || xr <= L.nx);
+    const int xcol = xside ? 4 + TBX * V : 3;
+    const long xoff = L.idx(xside ? xr : tile_x0 - 1, j0 - 1 + xrow, 0) - L.idx(0, 0, 0);
+    long c = L.idx(i0, j, k0);
+    const long hc = L.idx(i0, hrow, k0);
+    T xm[V], xc[V], bzd[V];
+    auto stage = [&](int b, const T (&ctr)[V], const T (&hal)[V], T xh) {
+#pragma unroll
+        for (int v = 0; v < V; v++) sh.a[b][ty + 1][4 + tx * V + v] = inside ? ctr[v] : (T)0;
+        if (hload) {
+#pragma unroll
+            for (int v = 0; v < V; v++) sh.a[b][ty == 0 ? 0 : TBY + 1][4 + tx * V + v] = hal[v];
+        }
+        if (xload) sh.a[b][xrow][xcol] = xh;
+    };
+    T hal[V] = {}, xh = 0;
+    // the previous tile's last plane may still be read from buffer 0
+    __syncthreads();
+    if (inside) {
+        x.vec(c - L.sy, xm);
+        x.vec(c, xc);
+        vld<T, V>(op.bz + c, bzd);
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) xm[v] = xc[v] = bzd[v] = 0;
+    }
+    if (hload) x.vec(hc, hal);
+    if (xload) xh = x.at(xoff + L.idx(0, 0, k0));
+    stage(0, xc, hal, xh);
+    for (int k = k0; k < k1; k++, c += L.sy) {
+        const int b = (k - k0) & 1;
+        const long kk = (long)(k - k0 + 1) * L.sy;
+        T xp[V], bzu[V], bl[V], bb[V], bt[V], hn[V], xhn = 0;
+#pragma unroll
+        for (int v = 0; v < V; v++) xp[v] = bzu[v] = bl[v] = bb[v] = bt[v] = hn[v] = 0;
+        const bool more = k + 1 < k1;
+        if (inside) {
+            x.vec(c + L.sy, xp);
+            vld<T, V>(op.bz + c + L.sy, bzu);
+            vld<T, V>(op.bx + c, bl);
+            vld<T, V>(op.by + c, bb);
+            vld<T, V>(op.by + c + L.sx, bt);
+        }
+        if (more && hload) x.vec(hc + kk, hn);
+        if (more && xload) xhn = x.at(xoff + L.idx(0, 0, k + 1));
+        __syncthreads();
+        if (inside) {
+            T xb[V], xt[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                xb[v] = sh.a[b][ty][4 + tx * V + v];
+                xt[v] = sh.a[b][ty + 2][4 + tx * V + v];
+            }
+            T xleft = sh.a[b][ty + 1][3 + tx * V], xright = sh.a[b][ty + 1][4 + tx * V + V];
+            T bxr = op.bx[c + V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                Nb<T> n;
+                n.xc = xc[v];
+                n.xl = v == 0 ? xleft : xc[v - 1];
+                n.xr = v == V - 1 ? xright : xc[v + 1];
+                n.xb = xb[v];
+                n.xt = xt[v];
+                n.xd = xm[v];
+                n.xu = xp[v];
+                n.bl = bl[v];
+                n.br = v == V - 1 ? bxr : bl[v + 1];
+                n.bb = bb[v];
+                n.bt = bt[v];
+                n.bd = bzd[v];
+                n.bu = bzu[v];
+                bool pin = (v == 0) && ((i0 | j | k) == 0);
+                f(c, v, k, xc[v], row_val(op, n, pin, pin_b, pin_d));
+            }
+        }
+        if (more) stage(b ^ 1, xp, hn, xhn);
+        if (inside) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                xm[v] = xc[v];
+                xc[v] = xp[v];
+                bzd[v] = bzu[v];
+            }
+        }
+    }
+}
+
 // ---- begin: r = b - A x, s = r/D, p = s; partials of r.s, r.r, b.b -------
 template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_begin(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ b,
@@ -462,22 +587,44 @@ __device__ __forceinline__ bool loop_tail(const double* part, int nb, FsPcgState
 // touches (p(n-1) from pold, written to pnew: the two directions live in
 // separate buffers so no workgroup reads a value another has replaced),
 // q = A p(n) and the partial p.q.
-template <typename T, int V>
+template <typename T, int V, bool LDS>
 __global__ void __launch_bounds__(NT) k_pspmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ s,
                                                   const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
-                                                  double* part, FsPcgState* S, int n) {
+                                                  double* part, FsPcgState* S, int n, int nb_rs) {
     if (S->done) return;
     __shared__ double lds[4];
     T beta = 0;
     if (n > 0) {
         double rz_old = S->rz[(n - 1) & 1];
         double rs;
-        if (loop_tail(part, tl.nb, S, n, rs, lds)) return;
+        if (loop_tail(part, nb_rs, S, n, rs, lds)) return;
         beta = (T)(rs / rz_old);
     }
     OpT<T> op = make_op<T>(o);
     double acc = 0;
     T pin_b = (T)S->pin_b, pin_d = (T)S->pin_d;
+    if constexpr (LDS) {
+    __shared__ StageLds<T, V> sh;
+    for (TileIter it(tl); it.valid(); it.next()) {
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        const int tile_x0 = i0 - threadIdx.x * V, j0 = j - threadIdx.y;
+        T qv[V], pv[V];
+        auto body = [&](long c, int v, int, T pc, T ap) {
+            qv[v] = ap;
+            pv[v] = pc;
+            acc += (double)pc * (double)ap;
+            if (v == V - 1) {
+                vst_s<NTB(8), T, V>(q + c, qv);
+                if (n > 0) vst_s<NTB(9), T, V>(pnew + c, pv);
+            }
+        };
+        if (n > 0)
+            march_apply_lds<T, V>(L, op, LdDir<T, V>{pold, s, beta}, tile_x0, j0, k0, tl.zc, pin_b, pin_d, sh, body);
+        else
+            march_apply_lds<T, V>(L, op, LdPlain<T, V>{pold}, tile_x0, j0, k0, tl.zc, pin_b, pin_d, sh, body);
+    }
+    } else {
     for (TileIter it(tl); it.valid(); it.next()) {
         int i0, j, k0;
         tile_origin<V>(tl, it.t, i0, j, k0);
@@ -496,6 +643,7 @@ __global__ void __launch_bounds__(NT) k_pspmv_dot(FsLayout L, Tiling tl, FsPcgOp
             march_apply<T, V>(L, op, LdDir<T, V>{pold, s, beta}, i0, j, k0, tl.zc, pin_b, pin_d, body);
         else
             march_apply<T, V>(L, op, LdPlain<T, V>{pold}, i0, j, k0, tl.zc, pin_b, pin_d, body);
+    }
     }
     put_partial(part, SLOT_PQ, acc, lds);
 }
@@ -522,10 +670,11 @@ __global__ void __launch_bounds__(NT) k_tail(const double* part, int nb, FsPcgSt
 template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_update(FsLayout L, Tiling tl, const T* __restrict__ D, T* __restrict__ x,
                                                T* __restrict__ r, T* __restrict__ s, const T* __restrict__ p,
-                                               const T* __restrict__ q, double* part, const FsPcgState* S, int n) {
+                                               const T* __restrict__ q, double* part, const FsPcgState* S, int n,
+                                               int nb_pq) {
     if (S->done) return;
     __shared__ double lds[4];
-    double pq = get_total(part, SLOT_PQ, tl.nb, lds);
+    double pq = get_total(part, SLOT_PQ, nb_pq, lds);
     T alpha = (T)(S->rz[n & 1] / pq);
     double a_rs = 0, a_rr = 0;
     for (TileIter it(tl); it.valid(); it.next()) {
@@ -677,15 +826,21 @@ static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, con
     // it may run narrower than the update (fewer registers, more waves)
     constexpr int VW = sizeof(T) == 4 ? FS_SPMV_V32 : FS_SPMV_V64;
     constexpr int VK = VW < V ? VW : V;
+    const bool lds = (long)L.nx * L.ny * L.nz >= FS_LDS_MIN_CELLS;
+    const Tiling tk = make_tiling(L, VK, lds ? FS_K1_MIN_TILES : FS_MIN_TILES);
+    const int nb_pq = op.kind == FS_OP_ROWS ? tl.nb : tk.nb;  // workgroups of this iteration's K1
     if (which == 0 && op.kind == FS_OP_ROWS)
         hipLaunchKernelGGL((k_rows_pspmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)w.s,
                            (const double*)po, (double*)pn, (double*)w.q, w.partials, S, n);
+    else if (which == 0 && lds)
+        hipLaunchKernelGGL((k_pspmv_dot<T, VK, true>), dim3(tk.nb), dim3(TBX, TBY), 0, st, L, tk, op, (const T*)w.s,
+                           po, pn, (T*)w.q, w.partials, S, n, tl.nb);
     else if (which == 0)
-        hipLaunchKernelGGL((k_pspmv_dot<T, VK>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, make_tiling_nb(L, VK, tl.nb),
-                           op, (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n);
+        hipLaunchKernelGGL((k_pspmv_dot<T, VK, false>), dim3(tk.nb), dim3(TBX, TBY), 0, st, L, tk, op,
+                           (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n, tl.nb);
     else if (which == 1)
         hipLaunchKernelGGL((k_update<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, diag, x, (T*)w.r, (T*)w.s,
-                           (const T*)pn, (const T*)w.q, w.partials, S, n);
+                           (const T*)pn, (const T*)w.q, w.partials, S, n, nb_pq);
     else
         hipLaunchKernelGGL(k_tail, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, n);
 }
