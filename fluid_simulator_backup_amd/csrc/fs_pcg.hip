@@ -29,6 +29,8 @@
 // kernel re-reduces the NB partials in the same fixed order, so all of them
 // see the same scalar and results do not depend on scheduling.  Dot products
 // accumulate in FP64 for both precisions.
+#include <hip/hip_ext.h>
+
 #include <cstring>
 
 #include "fs_pcg.h"
@@ -817,8 +819,11 @@ static inline void* dir_buf(FsPcgWork& w, int n) { return (n & 1) ? w.p2 : w.p; 
 // Kernel `which` of iteration n: 0 the fused K1, 1 the update K2, 2 the
 // one-workgroup loop tail of n.
 template <typename T, int V>
+// e0/e1 (may be null) are recorded by the dispatch itself around the
+// kernel (hipExtLaunchKernelGGL), so a probed kernel's time excludes the gaps
+// that separate event packets would add.
 static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
-                       FsPcgState* S, int which, int n) {
+                       FsPcgState* S, int which, int n, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     Tiling tl = make_tiling(L, V);
     T* pn = (T*)dir_buf(w, n);
     const T* po = (const T*)dir_buf(w, n > 0 ? n - 1 : 0);
@@ -830,19 +835,20 @@ static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, con
     const Tiling tk = make_tiling(L, VK, lds ? FS_K1_MIN_TILES : FS_MIN_TILES);
     const int nb_pq = op.kind == FS_OP_ROWS ? tl.nb : tk.nb;  // workgroups of this iteration's K1
     if (which == 0 && op.kind == FS_OP_ROWS)
-        hipLaunchKernelGGL((k_rows_pspmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)w.s,
-                           (const double*)po, (double*)pn, (double*)w.q, w.partials, S, n);
+        hipExtLaunchKernelGGL((k_rows_pspmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tl, op,
+                              (const double*)w.s, (const double*)po, (double*)pn, (double*)w.q, w.partials, S, n);
     else if (which == 0 && lds)
-        hipLaunchKernelGGL((k_pspmv_dot<T, VK, true>), dim3(tk.nb), dim3(TBX, TBY), 0, st, L, tk, op, (const T*)w.s,
-                           po, pn, (T*)w.q, w.partials, S, n, tl.nb);
+        hipExtLaunchKernelGGL((k_pspmv_dot<T, VK, true>), dim3(tk.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tk, op,
+                              (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n, tl.nb);
     else if (which == 0)
-        hipLaunchKernelGGL((k_pspmv_dot<T, VK, false>), dim3(tk.nb), dim3(TBX, TBY), 0, st, L, tk, op,
-                           (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n, tl.nb);
+        hipExtLaunchKernelGGL((k_pspmv_dot<T, VK, false>), dim3(tk.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tk, op,
+                              (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n, tl.nb);
     else if (which == 1)
-        hipLaunchKernelGGL((k_update<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, diag, x, (T*)w.r, (T*)w.s,
-                           (const T*)pn, (const T*)w.q, w.partials, S, n, nb_pq);
+        hipExtLaunchKernelGGL((k_update<T, V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tl, diag, x,
+                              (T*)w.r, (T*)w.s, (const T*)pn, (const T*)w.q, w.partials, S, n, nb_pq);
     else
-        hipLaunchKernelGGL(k_tail, dim3(1), dim3(TBX, TBY), 0, st, w.partials, tl.nb, S, n);
+        hipExtLaunchKernelGGL(k_tail, dim3(1), dim3(TBX, TBY), 0, st, e0, e1, 0, (const double*)w.partials, tl.nb, S,
+                              n);
 }
 
 template <typename T>
@@ -856,11 +862,11 @@ static void begin_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const 
 
 template <typename T>
 static void one_t(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
-                  FsPcgState* S, int which, int n) {
+                  FsPcgState* S, int which, int n, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
     int v = vec_width(L, sizeof(T) == 4);
-    if (v == 4) launch_one<T, (sizeof(T) == 4 ? 4 : 2)>(st, L, op, diag, x, w, S, which, n);
-    else if (v == 2) launch_one<T, 2>(st, L, op, diag, x, w, S, which, n);
-    else launch_one<T, 1>(st, L, op, diag, x, w, S, which, n);
+    if (v == 4) launch_one<T, (sizeof(T) == 4 ? 4 : 2)>(st, L, op, diag, x, w, S, which, n, e0, e1);
+    else if (v == 2) launch_one<T, 2>(st, L, op, diag, x, w, S, which, n, e0, e1);
+    else launch_one<T, 1>(st, L, op, diag, x, w, S, which, n, e0, e1);
 }
 
 // Kernel `which` of iteration n, bracketed by the probe's events when n is
@@ -869,12 +875,11 @@ template <typename T>
 static void one_probed(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
                        FsPcgState* S, int which, int n, FsKernelProbe* probe) {
     bool rec = probe && probe->iter == n;
-    if (rec) (void)hipEventRecord(probe->ev[2 * which], st);
-    one_t<T>(st, L, op, diag, x, w, S, which, n);
-    if (rec) {
-        (void)hipEventRecord(probe->ev[2 * which + 1], st);
-        if (which == FS_PCG_KERNELS - 1) probe->recorded = true;
-    }
+    if (rec)
+        one_t<T>(st, L, op, diag, x, w, S, which, n, probe->ev[2 * which], probe->ev[2 * which + 1]);
+    else
+        one_t<T>(st, L, op, diag, x, w, S, which, n);
+    if (rec && which == FS_PCG_KERNELS - 1) probe->recorded = true;
 }
 
 // Iterations n0 .. n0+count-1 (K1, K2 each) and the loop tail of the last.
