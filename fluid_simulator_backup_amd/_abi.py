@@ -163,6 +163,8 @@ def bind(lib, prefix):
     f("destroy").restype = None
     f("set_fields").argtypes = [vp, _PD, _PD, _PD, _PD, _PD]
     f("get_fields").argtypes = [vp, _PD, _PD, _PD, _PD, _PD]
+    for n in ("write_levelset", "read_levelset", "write_checkpoint", "read_checkpoint"):
+        f(n).argtypes = [vp, C.c_char_p]
     for n in ("advance_frame", "step"):
         f(n).argtypes = [vp, _D, C.POINTER(FsStepStats)]
     f("cfl_dt").argtypes = [vp, _D, _I, C.POINTER(_D)]
@@ -264,6 +266,18 @@ class Solver:
         self._check(self._f("get_fields")(self.h, *[_ptr(out[c]) for c in ("phi", "u", "v", "w", "p")]),
                     "get_fields")
         return out
+
+    def write_levelset(self, path):
+        self._check(self._f("write_levelset")(self.h, os.fsencode(path)), "write_levelset")
+
+    def read_levelset(self, path):
+        self._check(self._f("read_levelset")(self.h, os.fsencode(path)), "read_levelset")
+
+    def write_checkpoint(self, path):
+        self._check(self._f("write_checkpoint")(self.h, os.fsencode(path)), "write_checkpoint")
+
+    def read_checkpoint(self, path):
+        self._check(self._f("read_checkpoint")(self.h, os.fsencode(path)), "read_checkpoint")
 
     def advance_frame(self, dt, allow_not_converged=False):
         st = FsStepStats()

@@ -1191,6 +1191,82 @@ struct fs_solver {
     std::string err;
 };
 
+// ---- on-disk records (LEVELSET_3D::Write / Read, LEVELSET_3D.h:1132-1200) --
+// One record: int i_start, j_start, k_start, i_res, j_res, k_res; double
+// x_min, y_min, z_min, x_max, y_max, z_max; int ghost_width; then the
+// i_res*j_res*k_res values in GRID_ITERATION_3D order (i fastest, k
+// slowest), each field written on its own, native byte order.  A checkpoint
+// is five records back to back: phi and p on the cell grid, u, v, w on their
+// MAC grids (EULERIAN_FLUID_SOLVER_3D.h:405-431); its first record alone is
+// a level-set file.
+namespace {
+
+struct FsRecordGrid {
+    int res[3];
+    double mn[3], mx[3];
+    int ghost;
+};
+
+// grid of component comp (0 cells, 1..3 MAC) as the reference records it,
+// with the ghost width of the reference's field: level set and MAC velocity
+// 2, pressure 1 (EULERIAN_FLUID_SOLVER_3D.h:435-438, PROJECTION_3D.h:362)
+FsRecordGrid record_grid(const EulerianFluidSolver3D& S, int comp, int ghost) {
+    FsRecordGrid g;
+    const FsGridConst* gc[4] = {&S.cell, &S.macx, &S.macy, &S.macz};
+    const double mn[3] = {S.G.x_min, S.G.y_min, S.G.z_min}, mx[3] = {S.G.x_max, S.G.y_max, S.G.z_max};
+    const double d[3] = {gc[0]->x.d, gc[0]->y.d, gc[0]->z.d};
+    const int n[3] = {S.G.nx, S.G.ny, S.G.nz};
+    for (int a = 0; a < 3; a++) {
+        bool face = comp == a + 1;
+        g.res[a] = n[a] + (face ? 1 : 0);
+        g.mn[a] = face ? mn[a] - (double)0.5 * d[a] : mn[a];
+        g.mx[a] = face ? mx[a] + (double)0.5 * d[a] : mx[a];
+    }
+    g.ghost = ghost;
+    return g;
+}
+
+void write_record(FILE* f, const FsRecordGrid& g, const std::vector<double>& v) {
+    const int start = 0;
+    bool ok = true;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&start, sizeof(int), 1, f) == 1;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&g.res[a], sizeof(int), 1, f) == 1;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&g.mn[a], sizeof(double), 1, f) == 1;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&g.mx[a], sizeof(double), 1, f) == 1;
+    ok &= fwrite(&g.ghost, sizeof(int), 1, f) == 1;
+    ok &= fwrite(v.data(), sizeof(double), v.size(), f) == v.size();
+    if (!ok) throw FsFail{FS_ERR_BAD_ARG, "short write"};
+}
+
+// reads one record and checks that its grid is `want` (extents to 1e-12 of
+// the domain size); returns its values
+std::vector<double> read_record(FILE* f, const FsRecordGrid& want, const char* what) {
+    int start[3], res[3], ghost;
+    double mn[3], mx[3];
+    bool ok = fread(start, sizeof(int), 3, f) == 3 && fread(res, sizeof(int), 3, f) == 3 &&
+              fread(mn, sizeof(double), 3, f) == 3 && fread(mx, sizeof(double), 3, f) == 3 &&
+              fread(&ghost, sizeof(int), 1, f) == 1;
+    if (!ok) throw FsFail{FS_ERR_BAD_ARG, std::string(what) + ": truncated header"};
+    for (int a = 0; a < 3; a++) {
+        double tol = 1e-12 * (std::fabs(want.mx[a] - want.mn[a]) + 1.0);
+        if (res[a] != want.res[a] || std::fabs(mn[a] - want.mn[a]) > tol || std::fabs(mx[a] - want.mx[a]) > tol)
+            throw FsFail{FS_ERR_BAD_ARG, std::string(what) + ": grid does not match the solver's"};
+    }
+    std::vector<double> v((size_t)res[0] * res[1] * res[2]);
+    if (fread(v.data(), sizeof(double), v.size(), f) != v.size())
+        throw FsFail{FS_ERR_BAD_ARG, std::string(what) + ": truncated values"};
+    return v;
+}
+
+struct FileCloser {
+    FILE* f;
+    ~FileCloser() {
+        if (f) fclose(f);
+    }
+};
+
+}  // namespace
+
 template <typename F>
 static int guarded(fs_solver* s, F&& f) {
     if (!s || !s->impl) return FS_ERR_BAD_ARG;
@@ -1570,6 +1646,62 @@ int fs_adjoint_rollout(fs_solver* s, int steps, double dt, int checkpoint_every,
         S.sync();
         if (stats) *stats = S.stats;
         return rc;
+    });
+}
+
+static size_t box_size(const EulerianFluidSolver3D& S, int comp) {
+    FsBox b = S.box(comp);
+    return (size_t)b.ni * b.nj * b.nk;
+}
+
+int fs_write_levelset(fs_solver* s, const char* path) {
+    if (!path) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        std::vector<double> v(box_size(S, 0));
+        S.GetFields(v.data(), nullptr, nullptr, nullptr, nullptr);
+        FileCloser fc{fopen(path, "wb")};
+        if (!fc.f) throw FsFail{FS_ERR_BAD_ARG, std::string("cannot open ") + path};
+        write_record(fc.f, record_grid(S, 0, 2), v);
+        return FS_OK;
+    });
+}
+
+int fs_read_levelset(fs_solver* s, const char* path) {
+    if (!path) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        FileCloser fc{fopen(path, "rb")};
+        if (!fc.f) throw FsFail{FS_ERR_BAD_ARG, std::string("cannot open ") + path};
+        std::vector<double> v = read_record(fc.f, record_grid(S, 0, 2), "level set");
+        S.SetFields(v.data(), nullptr, nullptr, nullptr, nullptr);
+        return FS_OK;
+    });
+}
+
+int fs_write_checkpoint(fs_solver* s, const char* path) {
+    if (!path) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        std::vector<double> f[5];
+        const int comp[5] = {0, 1, 2, 3, 0}, ghost[5] = {2, 2, 2, 2, 1};
+        for (int q = 0; q < 5; q++) f[q].resize(box_size(S, comp[q]));
+        S.GetFields(f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data());
+        FileCloser fc{fopen(path, "wb")};
+        if (!fc.f) throw FsFail{FS_ERR_BAD_ARG, std::string("cannot open ") + path};
+        for (int q = 0; q < 5; q++) write_record(fc.f, record_grid(S, comp[q], ghost[q]), f[q]);
+        return FS_OK;
+    });
+}
+
+int fs_read_checkpoint(fs_solver* s, const char* path) {
+    if (!path) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        FileCloser fc{fopen(path, "rb")};
+        if (!fc.f) throw FsFail{FS_ERR_BAD_ARG, std::string("cannot open ") + path};
+        const int comp[5] = {0, 1, 2, 3, 0}, ghost[5] = {2, 2, 2, 2, 1};
+        const char* what[5] = {"phi", "u", "v", "w", "p"};
+        std::vector<double> f[5];
+        for (int q = 0; q < 5; q++) f[q] = read_record(fc.f, record_grid(S, comp[q], ghost[q]), what[q]);
+        S.SetFields(f[0].data(), f[1].data(), f[2].data(), f[3].data(), f[4].data());
+        return FS_OK;
     });
 }
 

@@ -49,7 +49,11 @@ extern "C" {
 
 /* Step schedule. */
 #define FS_SCHEDULE_COMPAT   0   /* reference call order, ghost refresh points and quirks */
-#define FS_SCHEDULE_FIXED    1   /* same kernels; all faces projected, fresh ghosts */
+#define FS_SCHEDULE_FIXED    1   /* same kernels, quirks corrected: ghost copies refreshed
+                                     before use, every face corrected with the
+                                     matrix's coefficients, walls zeroed after
+                                     forces, Euclidean Godunov norm (piecewise
+                                     smooth: fs_adjoint_step differentiates it) */
 
 /* ---- plain-data structs --------------------------------------------------- */
 
@@ -136,6 +140,26 @@ int  fs_set_fields(fs_solver* s, const double* phi, const double* u,
                    const double* v, const double* w, const double* p);
 /* Copy fields out (any pointer may be NULL). */
 int  fs_get_fields(fs_solver* s, double* phi, double* u, double* v, double* w, double* p);
+
+/* ---- files ------------------------------------------------------------------
+ * fs_write_levelset / fs_read_levelset: LEVELSET_3D::Write / Read
+ * (LEVELSET_3D.h:1132-1200), the reference's one binary field format.  A
+ * record is int i_start, j_start, k_start, i_res, j_res, k_res (starts 0);
+ * double x_min, y_min, z_min, x_max, y_max, z_max (first and last sample
+ * positions); int ghost_width; then i_res*j_res*k_res doubles, i fastest, in
+ * native byte order.  Reading checks that the record's grid is the solver's
+ * (FS_ERR_BAD_ARG otherwise) and sets phi as fs_set_fields does.
+ * fs_write_checkpoint / fs_read_checkpoint: the state (phi, u, v, w, p) as
+ * five such records back to back; u, v, w on their MAC grids (faces, the
+ * cell extents widened by dx/2 along their axis,
+ * EULERIAN_FLUID_SOLVER_3D.h:405-431), ghost widths 2, 2, 2, 2, 1.  Its first
+ * record alone is a level-set file the reference can read.  The reference
+ * writes only level sets (FIELD_STRUCTURE_3D::Write is declared, not
+ * defined); the checkpoint extends its layout to the whole state. */
+int  fs_write_levelset(fs_solver* s, const char* path);
+int  fs_read_levelset(fs_solver* s, const char* path);
+int  fs_write_checkpoint(fs_solver* s, const char* path);
+int  fs_read_checkpoint(fs_solver* s, const char* path);
 
 /* ---- the timestep -------------------------------------------------------- */
 

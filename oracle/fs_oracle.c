@@ -1482,3 +1482,107 @@ int fso_get_fields(fso_state* s, double* phi, double* u, double* v, double* w, d
     if (p) get(s, s->p, p, s->nx, s->ny, s->nz);
     return FS_OK;
 }
+
+/* ------------------------------------------------------------------------- */
+/* On-disk records: LEVELSET_3D::Write / Read, LEVELSET_3D.h:1132-1200        */
+/* ------------------------------------------------------------------------- */
+
+/* grid of component comp (0 cells, 1..3 MAC faces, EULERIAN_FLUID_SOLVER_3D.h
+ * :405-431) as the reference records it */
+static void rec_grid(const fso_state* s, int comp, int res[3], double mn[3], double mx[3]) {
+    const int n[3] = {s->nx, s->ny, s->nz};
+    const double lo[3] = {s->gr.x_min, s->gr.y_min, s->gr.z_min}, hi[3] = {s->gr.x_max, s->gr.y_max, s->gr.z_max};
+    const double d[3] = {s->cell.x.d, s->cell.y.d, s->cell.z.d};
+    for (int a = 0; a < 3; a++) {
+        int face = comp == a + 1;
+        res[a] = n[a] + face;
+        mn[a] = face ? lo[a] - (double)0.5 * d[a] : lo[a];
+        mx[a] = face ? hi[a] + (double)0.5 * d[a] : hi[a];
+    }
+}
+
+static int write_rec(FILE* f, const fso_state* s, int comp, int ghost, const double* v) {
+    int res[3], start = 0, ok = 1;
+    double mn[3], mx[3];
+    rec_grid(s, comp, res, mn, mx);
+    for (int a = 0; a < 3; a++) ok &= fwrite(&start, sizeof(int), 1, f) == 1;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&res[a], sizeof(int), 1, f) == 1;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&mn[a], sizeof(double), 1, f) == 1;
+    for (int a = 0; a < 3; a++) ok &= fwrite(&mx[a], sizeof(double), 1, f) == 1;
+    ok &= fwrite(&ghost, sizeof(int), 1, f) == 1;
+    size_t n = (size_t)res[0] * res[1] * res[2];
+    ok &= fwrite(v, sizeof(double), n, f) == n;
+    return ok ? FS_OK : FS_ERR_BAD_ARG;
+}
+
+static int read_rec(FILE* f, const fso_state* s, int comp, double* v) {
+    int res[3], start[3], ghost;
+    double mn[3], mx[3], wmn[3], wmx[3];
+    int want[3];
+    rec_grid(s, comp, want, wmn, wmx);
+    if (fread(start, sizeof(int), 3, f) != 3 || fread(res, sizeof(int), 3, f) != 3 ||
+        fread(mn, sizeof(double), 3, f) != 3 || fread(mx, sizeof(double), 3, f) != 3 ||
+        fread(&ghost, sizeof(int), 1, f) != 1)
+        return FS_ERR_BAD_ARG;
+    for (int a = 0; a < 3; a++) {
+        double tol = 1e-12 * (fabs(wmx[a] - wmn[a]) + 1.0);
+        if (res[a] != want[a] || fabs(mn[a] - wmn[a]) > tol || fabs(mx[a] - wmx[a]) > tol) return FS_ERR_BAD_ARG;
+    }
+    size_t n = (size_t)res[0] * res[1] * res[2];
+    return fread(v, sizeof(double), n, f) == n ? FS_OK : FS_ERR_BAD_ARG;
+}
+
+static size_t comp_size(const fso_state* s, int comp) {
+    return (size_t)(s->nx + (comp == 1)) * (s->ny + (comp == 2)) * (s->nz + (comp == 3));
+}
+
+int fso_write_levelset(fso_state* s, const char* path) {
+    FILE* f = path ? fopen(path, "wb") : NULL;
+    if (!f) return FS_ERR_BAD_ARG;
+    double* v = (double*)malloc(sizeof(double) * comp_size(s, 0));
+    fso_get_fields(s, v, NULL, NULL, NULL, NULL);
+    int rc = write_rec(f, s, 0, 2, v);
+    free(v);
+    fclose(f);
+    return rc;
+}
+
+int fso_read_levelset(fso_state* s, const char* path) {
+    FILE* f = path ? fopen(path, "rb") : NULL;
+    if (!f) return FS_ERR_BAD_ARG;
+    double* v = (double*)malloc(sizeof(double) * comp_size(s, 0));
+    int rc = read_rec(f, s, 0, v);
+    if (rc == FS_OK) rc = fso_set_fields(s, v, NULL, NULL, NULL, NULL);
+    free(v);
+    fclose(f);
+    return rc;
+}
+
+/* phi, u, v, w, p: five records back to back (ghost widths 2, 2, 2, 2, 1) */
+int fso_write_checkpoint(fso_state* s, const char* path) {
+    FILE* f = path ? fopen(path, "wb") : NULL;
+    if (!f) return FS_ERR_BAD_ARG;
+    const int comp[5] = {0, 1, 2, 3, 0}, ghost[5] = {2, 2, 2, 2, 1};
+    double* v[5];
+    for (int q = 0; q < 5; q++) v[q] = (double*)malloc(sizeof(double) * comp_size(s, comp[q]));
+    fso_get_fields(s, v[0], v[1], v[2], v[3], v[4]);
+    int rc = FS_OK;
+    for (int q = 0; q < 5 && rc == FS_OK; q++) rc = write_rec(f, s, comp[q], ghost[q], v[q]);
+    for (int q = 0; q < 5; q++) free(v[q]);
+    fclose(f);
+    return rc;
+}
+
+int fso_read_checkpoint(fso_state* s, const char* path) {
+    FILE* f = path ? fopen(path, "rb") : NULL;
+    if (!f) return FS_ERR_BAD_ARG;
+    const int comp[5] = {0, 1, 2, 3, 0};
+    double* v[5];
+    for (int q = 0; q < 5; q++) v[q] = (double*)malloc(sizeof(double) * comp_size(s, comp[q]));
+    int rc = FS_OK;
+    for (int q = 0; q < 5 && rc == FS_OK; q++) rc = read_rec(f, s, comp[q], v[q]);
+    if (rc == FS_OK) rc = fso_set_fields(s, v[0], v[1], v[2], v[3], v[4]);
+    for (int q = 0; q < 5; q++) free(v[q]);
+    fclose(f);
+    return rc;
+}

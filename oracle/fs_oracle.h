@@ -25,6 +25,12 @@ void fso_destroy(fso_state* s);
 int  fso_set_fields(fso_state* s, const double* phi, const double* u, const double* v,
                     const double* w, const double* p);
 int  fso_get_fields(fso_state* s, double* phi, double* u, double* v, double* w, double* p);
+/* LEVELSET_3D::Write / Read records (LEVELSET_3D.h:1132-1200); a checkpoint
+ * is the records of phi, u, v, w, p back to back */
+int  fso_write_levelset(fso_state* s, const char* path);
+int  fso_read_levelset(fso_state* s, const char* path);
+int  fso_write_checkpoint(fso_state* s, const char* path);
+int  fso_read_checkpoint(fso_state* s, const char* path);
 
 /* 1 = explicit viscosity updated in place in lexicographic order, as the
  * reference's single-thread sweep does (VISCOSITY_3D.h:483-559);
