@@ -169,6 +169,7 @@ def bind(lib, prefix):
         f(n).argtypes = [vp, _D, C.POINTER(FsStepStats)]
     f("cfl_dt").argtypes = [vp, _D, _I, C.POINTER(_D)]
     f("compute_curvatures").argtypes = [vp]
+    f("get_curvature").argtypes = [vp, _PD, _PD]
     for n in ("levelset_advection", "velocity_advection", "apply_gravity", "sourcing", "reinitialization"):
         f(n).argtypes = [vp, _D]
     f("projection").argtypes = [vp, _D, C.POINTER(FsStepStats)]
@@ -298,6 +299,13 @@ class Solver:
 
     def compute_curvatures(self):
         self._check(self._f("compute_curvatures")(self.h), "compute_curvatures")
+
+    def curvature(self):
+        """(curvature (nz, ny, nx), abs_max_curvature) of the current level set."""
+        k = np.empty((self.nz, self.ny, self.nx))
+        m = C.c_double()
+        self._check(self._f("get_curvature")(self.h, _ptr(k), C.byref(m)), "get_curvature")
+        return k, m.value
 
     def phase(self, name, dt):
         self._check(self._f(name)(self.h, dt), name)

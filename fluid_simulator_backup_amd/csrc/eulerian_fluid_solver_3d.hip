@@ -80,6 +80,11 @@ class EulerianFluidSolver3D {
     double* scratch = nullptr;                                   // mean removal partials (nz+1) + pin (2)
     unsigned long long* vmax = nullptr;                          // 4 words
     unsigned long long* hvmax = nullptr;                         // pinned
+    // surface tension (sigma != 0): curvature, jump condition sigma * kappa
+    // (zero ghosts), and [bits of max |kappa| near the interface, count of
+    // cells with a vanishing gradient] of the last curvature pass
+    double *kappa = nullptr, *jc = nullptr;
+    unsigned long long *kstat = nullptr, *hkstat = nullptr;
     double* staging = nullptr;                                   // host <-> device exchange
     long staging_n = 0;
     bool assembled = false;
@@ -156,8 +161,7 @@ class EulerianFluidSolver3D {
         G = grid;
         if (G.nx < 3 || G.ny < 3 || G.nz < 3) throw FsFail{FS_ERR_BAD_ARG, "grid must be at least 3 cells per axis"};
         if ((long)G.nx * G.ny * G.nz > (1L << 31)) throw FsFail{FS_ERR_BAD_ARG, "grid too large"};
-        if (P.surface_tension != 0.0)
-            throw FsFail{FS_ERR_UNSUPPORTED, "surface_tension != 0 (jump terms) is outside this build"};
+        if (!std::isfinite(P.surface_tension)) throw FsFail{FS_ERR_BAD_ARG, "surface_tension must be finite"};
         if (P.order_for_time_advancing < 1 || P.order_for_time_advancing > 3)
             throw FsFail{FS_ERR_BAD_ARG, "order_for_time_advancing must be 1, 2 or 3"};
         if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE) throw FsFail{FS_ERR_BAD_ARG, "precision"};
@@ -232,6 +236,13 @@ class EulerianFluidSolver3D {
         scratch = (double*)dalloc(sizeof(double) * (G.nz + 8));
         vmax = (unsigned long long*)dalloc(sizeof(unsigned long long) * 4);
         HIPCHECK(hipHostMalloc((void**)&hvmax, sizeof(unsigned long long) * 4));
+        if (tension()) {
+            kappa = dfield();
+            jc = dfield();
+            kstat = (unsigned long long*)dalloc(sizeof(unsigned long long) * 2);
+            HIPCHECK(hipHostMalloc((void**)&hkstat, sizeof(unsigned long long) * 2));
+            hkstat[0] = hkstat[1] = 0;
+        }
         staging_n = (long)(G.nx + 1) * (G.ny + 1) * (G.nz + 1);
         staging = (double*)dalloc(sizeof(double) * staging_n);
         HIPCHECK(hipStreamSynchronize(st));
@@ -241,11 +252,11 @@ class EulerianFluidSolver3D {
         if (st) (void)hipStreamSynchronize(st);
         double* f64[] = {phi, sghost, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
                          scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk,
-                         va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials};
+                         va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc};
         for (double* f : f64)
             if (f) (void)hipFree(f);
         void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32,
-                      S64, S32, vmax, wv.r, wv.s, wv.p, wv.q, w64.p2, w32.p2, wv.p2};
+                      S64, S32, vmax, kstat, wv.r, wv.s, wv.p, wv.q, w64.p2, w32.p2, wv.p2};
         for (void* f : fv)
             if (f) (void)hipFree(f);
         for (auto& pl : tape.pool)
@@ -259,6 +270,7 @@ class EulerianFluidSolver3D {
                 if (f) (void)hipFree(f);
         if (hS) (void)hipHostFree(hS);
         if (hvmax) (void)hipHostFree(hvmax);
+        if (hkstat) (void)hipHostFree(hkstat);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : probe.ev)
@@ -309,11 +321,32 @@ class EulerianFluidSolver3D {
     }
 
     // ---- phases -----------------------------------------------------------
-    // LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082: with zero
-    // surface tension the lasting effect is phi's ghost re-extrapolation (:998).
-    void ComputeCurvatures() { fs_fill_linear(st, phi, phi, L, box(0), 2, 1); }
+    // LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082: phi's ghosts
+    // re-extrapolated linearly (:998), then the curvature and its maximum
+    // near the interface (the CFL rule's s_f).  With zero surface tension
+    // nothing reads the curvature and only the ghost refresh is kept.  With_jc:
+    // also jc = sigma * kappa (DetermineJumpConditionField, PROJECTION_3D.h:746-769).
+    void ComputeCurvatures(bool with_jc = false) {
+        fs_fill_linear(st, phi, phi, L, box(0), 2, 1);
+        if (!tension()) return;
+        HIPCHECK(hipMemsetAsync(kstat, 0, sizeof(unsigned long long) * 2, st));
+        fs_curvature(st, phi, kappa, with_jc ? jc : nullptr, P.surface_tension, L, cell, fixed() ? 1 : 0, kstat);
+        HIPCHECK(hipMemcpyAsync(hkstat, kstat, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, st));
+    }
+    // jump condition of the pressure solve, or NULL where it is zero: no
+    // surface tension, or use_delta_function_formulation off (jc_on_solution
+    // then keeps its initial zeros, PROJECTION_3D.h:358, :750)
+    const double* jump() const { return tension() && P.use_delta_function_formulation ? jc : nullptr; }
+    // the curvature pass of a call reached a cell whose gradient vanishes in
+    // both difference forms (the reference stops there, LEVELSET_3D.h:1041-1045);
+    // call after a sync
+    void check_curvature() const {
+        if (tension() && hkstat[1] != 0)
+            throw FsFail{FS_ERR_NONFINITE, "curvature: level-set gradient vanishes in both difference forms"};
+    }
 
     bool fixed() const { return P.schedule == FS_SCHEDULE_FIXED; }
+    bool tension() const { return P.surface_tension != 0.0; }
 
     // copy of field src (component box comp) onto the step tape
     double* Record(const double* src, int comp) {
@@ -484,13 +517,14 @@ class EulerianFluidSolver3D {
         fs_copy_box(st, gw, w, L, bw);
         fs_fill_zero_faces(st, gw, L, bw, 3);
         fs_divergence(st, gu, gv, gw, div, L, cell, 1 / dt);
-        // DetermineJumpConditionField's ComputeCurvatures refresh of phi's
-        // ghosts is overwritten by ComputeNormals below before anything reads
-        // it, so it is not repeated here.
+        // DetermineJumpConditionField: ComputeCurvatures, then jc.  Without
+        // surface tension its refresh of phi's ghosts is overwritten by
+        // ComputeNormals below before anything reads it, so it is skipped.
+        if (tension()) ComputeCurvatures(P.use_delta_function_formulation != 0);
         if (!P.use_previous_pressure) fs_fill_value(st, p, L.total, 0.0);
         fs_fill_const(st, phi, phi, L, box(0), 2);   // ComputeNormals, LEVELSET_3D.h:931
         fs_fill_const(st, irho, irho, L, box(0), 1); // PROJECTION_3D.h:813
-        fs_assemble(st, phi, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
+        fs_assemble(st, phi, irho, div, jump(), bx, by, bz, diag, rhs, S64pin(), L, cell);
         fs_remove_mean(st, rhs, scratch, L);
         assembled = true;
     }
@@ -680,16 +714,17 @@ class EulerianFluidSolver3D {
     void UpdateVelocity(double dt) {
         HIPCHECK(hipMemsetAsync(vmax, 0, sizeof(unsigned long long) * 4, st));
         if (fixed()) {
-            fs_gradient_update_beta(st, 1, u, p, bx, L, cell, dt, vmax + 0);
-            fs_gradient_update_beta(st, 2, v, p, by, L, cell, dt, vmax + 1);
-            fs_gradient_update_beta(st, 3, w, p, bz, L, cell, dt, vmax + 2);
+            fs_gradient_update_beta(st, 1, u, p, bx, phi, jump(), L, cell, dt, vmax + 0);
+            fs_gradient_update_beta(st, 2, v, p, by, phi, jump(), L, cell, dt, vmax + 1);
+            fs_gradient_update_beta(st, 3, w, p, bz, phi, jump(), L, cell, dt, vmax + 2);
         } else {
-            fs_gradient_update(st, 1, 0, u, p, phi, rho, L, mat, cell, dt, vmax + 0);
-            fs_gradient_update(st, 2, 0, v, p, phi, rho, L, mat, cell, dt, vmax + 1);
-            fs_gradient_update(st, 3, 1, w, p, phi, rho, L, mat, cell, dt, vmax + 2);
+            fs_gradient_update(st, 1, 0, u, p, phi, rho, jump(), L, mat, cell, dt, vmax + 0);
+            fs_gradient_update(st, 2, 0, v, p, phi, rho, jump(), L, mat, cell, dt, vmax + 1);
+            fs_gradient_update(st, 3, 1, w, p, phi, rho, jump(), L, mat, cell, dt, vmax + 2);
         }
         HIPCHECK(hipMemcpyAsync(hvmax, vmax, sizeof(unsigned long long) * 3, hipMemcpyDeviceToHost, st));
         sync();
+        check_curvature();
         for (int c = 0; c < 3; c++) {
             double m;
             memcpy(&m, &hvmax[c], sizeof(double));
@@ -734,7 +769,7 @@ class EulerianFluidSolver3D {
         fs_fill_const(st, phi, phi, L, box(0), 2);  // as AssemblePressureSystem
         fs_fill_const(st, irho, irho, L, box(0), 1);
         fs_fill_value(st, div, L.total, 0.0);
-        fs_assemble(st, phi, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
+        fs_assemble(st, phi, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell);
         assembled = true;
         // rhs = G^T lam (the faces the forward corrects)
         fs_fill_value(st, rhs, L.total, 0.0);
@@ -789,6 +824,8 @@ class EulerianFluidSolver3D {
         if (P.is_velocity_advection_active && !P.use_5th_weno_v)
             throw FsFail{FS_ERR_UNSUPPORTED, "adjoint velocity advection needs use_5th_weno_v (ENO3's stencil choice "
                                              "is discontinuous)"};
+        if (tension() && P.use_delta_function_formulation && P.is_projection_active)
+            throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the surface-tension jump (curvature of phi) is not built"};
     }
 
     void EnsureAdjoint() {
@@ -852,7 +889,7 @@ class EulerianFluidSolver3D {
             fs_density(st, A.phiw, rho, irho, L, light, heavy);
             fs_fill_const(st, irho, irho, L, box(0), 1);
             Zero(div);
-            fs_assemble(st, A.phiw, irho, div, bx, by, bz, diag, rhs, S64pin(), L, cell);
+            fs_assemble(st, A.phiw, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell);
             assembled = true;
             Zero(rhs);
             fs_adjoint_gradient_beta(st, 1, A.lu, bx, rhs, L, cell, dt);
@@ -1172,7 +1209,20 @@ class EulerianFluidSolver3D {
                                     : std::fabs(P.gravity[1]);
             g_f = std::sqrt(g * macy.y.inv);
         }
+        // surface tension restriction (:2740-2744, air/water) from the
+        // curvature of the current level set, as SIMULATION_WORLD::AdvanceOneFrame
+        // computes it right before the CFL rule (SIMULATION_WORLD.h:324, :289)
         double s_f = 0;
+        if (tension()) {
+            ComputeCurvatures();
+            sync();
+            check_curvature();
+            double kmax;
+            memcpy(&kmax, &hkstat[0], sizeof(double));
+            double dmin = std::min(std::min(macx.x.d, macy.y.d), macz.z.d);
+            double rmin = P.water_density > P.air_density ? P.air_density : P.water_density;
+            s_f = std::sqrt((P.surface_tension * kmax) / (rmin * (dmin * dmin)));
+        }
         double a = P.water_viscosity / P.water_density, b = P.air_viscosity / P.air_density;
         double v_f = (a > b ? a : b) * ((double)2 * cell.x.inv_d2 + (double)2 * cell.y.inv_d2 + (double)2 * cell.z.inv_d2);
         if (P.semi_implicit_viscosity) v_f = 0;  // :2770-2773
@@ -1444,6 +1494,20 @@ int fs_compute_curvatures(fs_solver* s) {
         S.ComputeCurvatures();
         S.check_launch();
         S.sync();
+        S.check_curvature();
+        return FS_OK;
+    });
+}
+
+int fs_get_curvature(fs_solver* s, double* kappa, double* abs_max_curvature) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        if (!S.tension()) throw FsFail{FS_ERR_UNSUPPORTED, "the curvature is computed only with surface tension"};
+        S.ComputeCurvatures();
+        S.check_launch();
+        S.sync();
+        S.check_curvature();
+        if (kappa) S.get(S.kappa, kappa, 0);
+        if (abs_max_curvature) memcpy(abs_max_curvature, &S.hkstat[0], sizeof(double));
         return FS_OK;
     });
 }
@@ -1716,6 +1780,8 @@ int fs_assemble_pressure_system(fs_solver* s, double dt, double* diag, double* b
         if (byh) S.get(S.by, byh, 2);
         if (bzh) S.get(S.bz, bzh, 3);
         if (rhs) S.get(S.rhs, rhs, 0);
+        S.sync();
+        S.check_curvature();
         return FS_OK;
     });
 }

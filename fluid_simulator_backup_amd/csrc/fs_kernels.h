@@ -54,14 +54,22 @@ void fs_density(hipStream_t st, const double* phi, double* rho, double* irho, co
 void fs_divergence(hipStream_t st, const double* gu, const double* gv, const double* gw, double* div,
                    const FsLayout& L, const FsGridConst& cell, double one_over_dt);
 // beta of every face between two interior cells (0 elsewhere), A(c,c) and
-// b = -div.  pin[0..1]: the first cell's Neumann extras beta_b, beta_d.
-void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, double* bx, double* by,
-                 double* bz, double* diag, double* rhs, double* pin, const FsLayout& L, const FsGridConst& cell);
+// b = -div - F with the jump terms of jc (jc = NULL: b = -div).
+// pin[0..1]: the first cell's Neumann extras beta_b, beta_d.
+void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, const double* jc,
+                 double* bx, double* by, double* bz, double* diag, double* rhs, double* pin, const FsLayout& L,
+                 const FsGridConst& cell);
+// curvature of phi (its ghosts as left), jc = sigma * kappa when jc is given;
+// kstat[0] = bits of max |kappa| near the interface, kstat[1] += cells whose
+// gradient vanishes in both difference forms.  Neither slot is cleared here.
+// exact: the corrected cross term and one-sided z difference.
+void fs_curvature(hipStream_t st, const double* phi, double* kappa, double* jc, double sigma, const FsLayout& L,
+                  const FsGridConst& cell, int exact, unsigned long long* kstat);
 // b -= mean(b) with the per-plane row-order sum; scratch: nz+1 doubles.
 void fs_remove_mean(hipStream_t st, double* rhs, double* scratch, const FsLayout& L);
 void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,
-                        const double* rho, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
-                        double dt, unsigned long long* vmax_bits);
+                        const double* rho, const double* jc, const FsLayout& L, const FsMaterial& m,
+                        const FsGridConst& cell, double dt, unsigned long long* vmax_bits);
 // Adjoint pieces of the projection at fixed phi: g += G^T lam (comp's faces
 // to cells) and lam += scale * D^T z (cells to the faces of u, v, w).
 void fs_adjoint_gradient(hipStream_t st, int comp, const double* lam, const double* phi, const double* rho, double* g,
@@ -72,10 +80,12 @@ void fs_adjoint_divergence(hipStream_t st, double* lu, double* lv, double* lw, c
 void fs_adjoint_gradient_beta(hipStream_t st, int comp, const double* lam, const double* beta, double* g,
                               const FsLayout& L, const FsGridConst& cell, double dt);
 void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, unsigned long long* bits);
-// fixed schedule: u_f -= dt beta_f (p_c - p_n)/dx with the pressure matrix's
-// face coefficients (beta = bx, by or bz of fs_assemble)
+// fixed schedule: u_f -= dt beta_f (p_c - p_n + jump)/dx with the pressure
+// matrix's face coefficients (beta = bx, by or bz of fs_assemble) and its
+// jump terms (jc = NULL: none)
 void fs_gradient_update_beta(hipStream_t st, int comp, double* vel, const double* p, const double* beta,
-                             const FsLayout& L, const FsGridConst& cell, double dt, unsigned long long* vmax_bits);
+                             const double* phi, const double* jc, const FsLayout& L, const FsGridConst& cell,
+                             double dt, unsigned long long* vmax_bits);
 // zero the two normal wall planes of MAC component comp (1..3)
 void fs_zero_walls(hipStream_t st, int comp, double* vel, const FsLayout& L);
 

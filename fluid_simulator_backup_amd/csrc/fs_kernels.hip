@@ -675,25 +675,41 @@ __global__ void k_beta(const double* __restrict__ f, const double* __restrict__ 
 
 // A(c,c) = inv_dxdx * coef (:1119-1213) with the Neumann pin on the first
 // full cell (:1145-1179: + beta_b + beta_d, + beta_u if its top is Neumann)
-// and b = -div (:1215-1328 with zero jump terms).
+// and b = -div - F_X - F_Y - F_Z (:1215-1328): on a face between the two
+// phases F = +-(1/d^2) a beta with a = (jc_c |phi_n| + jc_n |phi_c|) /
+// (|phi_c| + |phi_n|), + when phi_c <= 0; the jc_on_derivative terms are
+// zero.  Faces to ghost cells join cells of one phase (constant ghosts).
+__device__ __forceinline__ bool same_phase(double fc, double fn) { return (fc <= 0 && fn <= 0) || (fc > 0 && fn > 0); }
+__device__ __forceinline__ double jump_a(double fc, double fn, double jcc, double jcn) {
+    return (jcc * fabs(fn) + jcn * fabs(fc)) / (fabs(fc) + fabs(fn));
+}
+__device__ __forceinline__ double jump_face(double fc, double fn, double jcc, double jcn, double od2, double beta) {
+    if (same_phase(fc, fn)) return 0;
+    double a = jump_a(fc, fn, jcc, jcn);
+    if (fc <= 0) return od2 * a * beta;
+    return -od2 * a * beta;
+}
+
 __global__ void k_diag_rhs(const double* __restrict__ f, const double* __restrict__ v, const double* __restrict__ bx,
                            const double* __restrict__ by, const double* __restrict__ bz, const double* __restrict__ div,
-                           double* __restrict__ diag, double* __restrict__ rhs, double* __restrict__ pin, FsLayout L,
-                           double inv_dxdx) {
+                           const double* __restrict__ jc, double* __restrict__ diag, double* __restrict__ rhs,
+                           double* __restrict__ pin, FsLayout L, double inv_dxdx, double odx2, double ody2,
+                           double odz2) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
     long c = L.idx(i, j, k);
+    const long sx = L.sx, sy = L.sy;
     double coef = 0;
     coef += bx[c];
     coef += bx[c + 1];
     coef += by[c];
-    coef += by[c + L.sx];
+    coef += by[c + sx];
     coef += bz[c];
-    coef += bz[c + L.sy];
+    coef += bz[c + sy];
     if (i == 0 && j == 0 && k == 0) {
         double vc = v[c], fc = f[c];
-        double pb = gfm_beta(vc, v[c - L.sx], f[c - L.sx], fc);  // beta_b, face towards j-1
-        double pd = gfm_beta(vc, v[c - L.sy], f[c - L.sy], fc);  // beta_d, face towards k-1
+        double pb = gfm_beta(vc, v[c - sx], f[c - sx], fc);  // beta_b, face towards j-1
+        double pd = gfm_beta(vc, v[c - sy], f[c - sy], fc);  // beta_d, face towards k-1
         coef += pb;
         coef += pd;
         pin[0] = pb;
@@ -701,16 +717,29 @@ __global__ void k_diag_rhs(const double* __restrict__ f, const double* __restric
     }
     if (coef == 0) coef = 1;
     diag[c] = inv_dxdx * coef;
-    rhs[c] = -div[c];
+    if (!jc) {
+        rhs[c] = -div[c];
+        return;
+    }
+    const double fc = f[c], jcc = jc[c];
+    double FL = jump_face(fc, f[c - 1], jcc, jc[c - 1], odx2, bx[c]);
+    double FR = jump_face(fc, f[c + 1], jcc, jc[c + 1], odx2, bx[c + 1]);
+    double FB = jump_face(fc, f[c - sx], jcc, jc[c - sx], ody2, by[c]);
+    double FT = jump_face(fc, f[c + sx], jcc, jc[c + sx], ody2, by[c + sx]);
+    double FD = jump_face(fc, f[c - sy], jcc, jc[c - sy], odz2, bz[c]);
+    double FU = jump_face(fc, f[c + sy], jcc, jc[c + sy], odz2, bz[c + sy]);
+    double FX = FL + FR, FY = FB + FT, FZ = FD + FU;
+    rhs[c] = -div[c] - FX - FY - FZ;
 }
 
-void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, double* bx, double* by,
-                 double* bz, double* diag, double* rhs, double* pin, const FsLayout& L, const FsGridConst& cell) {
+void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, const double* jc,
+                 double* bx, double* by, double* bz, double* diag, double* rhs, double* pin, const FsLayout& L,
+                 const FsGridConst& cell) {
     FsBox bb{L.nx + 1, L.ny + 1, L.nz + 1};
     hipLaunchKernelGGL(k_beta, grid_for(bb), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, L);
     double inv_dxdx = (double)1 / (cell.x.d * cell.x.d);
-    hipLaunchKernelGGL(k_diag_rhs, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, div, diag, rhs,
-                       pin, L, inv_dxdx);
+    hipLaunchKernelGGL(k_diag_rhs, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, div, jc, diag,
+                       rhs, pin, L, inv_dxdx, cell.x.inv_d2, cell.y.inv_d2, cell.z.inv_d2);
 }
 
 // Neumann mean removal, POISSON_SOLVER_3D.h:225-246.  The sum runs in row
@@ -786,6 +815,73 @@ __device__ __forceinline__ void wg_max_atomic(double a, unsigned long long* bits
 
 #define KZ 16  // z-planes per workgroup in the max-reducing kernels
 
+// LEVELSET_3D::ComputeCurvatures, curvature_by_levelset branch
+// (LEVELSET_3D.h:963-1082), on phi with its ghosts as the caller left them:
+// central first, second and cross derivatives, curvature
+// -(px^2 pyy - 2 px py pxy + py^2 pxx + px^2 pzz - 2 px pz pxz + pz^2 pxx +
+// py^2 pzz - c py pz pyz + pz^2 pyy) / |grad phi|^3 with c = 1 as the
+// reference writes it (:1033) or 2 (`exact`), one-sided differences when the
+// central gradient vanishes (:1037, whose z term multiplies by dz unless
+// `exact`; a still vanishing gradient gives 0 and counts in kstat[1], where
+// the reference stops the program, :1040-1044), clamped below at
+// -1/min(dx,dy,dz) (:1061-1068).  kappa[c] = the curvature; jc[c] = sigma * kappa
+// (PROJECTION_3D.h:767) when jc is given; kstat[0] = bits of max |kappa| over
+// cells with |phi| < 0.1 (:1070-1077; NaN skipped as the reference's MAX
+// skips it).
+__global__ void __launch_bounds__(BX* BY) k_curvature(const double* __restrict__ phi, double* __restrict__ kappa,
+                                                      double* __restrict__ jc, double sigma, FsLayout L, FsGridConst g,
+                                                      double tol, int exact, unsigned long long* kstat) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double amax = 0;
+    for (int k = blockIdx.z * KZ; k < min(blockIdx.z * KZ + KZ, L.nz); k++) {
+        if (i >= L.nx || j >= L.ny) continue;
+        const long c = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+        const double ox = g.x.inv2d, oy = g.y.inv2d, oz = g.z.inv2d;
+        double px = (phi[c + 1] - phi[c - 1]) * ox, py = (phi[c + sx] - phi[c - sx]) * oy,
+               pz = (phi[c + sy] - phi[c - sy]) * oz;
+        double pxx = (phi[c + 1] - 2 * phi[c] + phi[c - 1]) * g.x.inv_d2;
+        double pyy = (phi[c + sx] - 2 * phi[c] + phi[c - sx]) * g.y.inv_d2;
+        double pzz = (phi[c + sy] - 2 * phi[c] + phi[c - sy]) * g.z.inv_d2;
+        double pxy = (phi[c + 1 + sx] - phi[c + 1 - sx] - phi[c - 1 + sx] + phi[c - 1 - sx]) * ox * oy;
+        double pxz = (phi[c + 1 + sy] - phi[c + 1 - sy] - phi[c - 1 + sy] + phi[c - 1 - sy]) * ox * oz;
+        double pyz = (phi[c + sx + sy] - phi[c + sx - sy] - phi[c - sx + sy] + phi[c - sx - sy]) * oy * oz;
+        double mag = sqrt(px * px + py * py + pz * pz);
+        double deno = mag * mag * mag;
+        double curv = 0;
+        bool ok = true;
+        if (deno == 0) {
+            px = (phi[c + 1] - phi[c]) * g.x.inv;
+            py = (phi[c + sx] - phi[c]) * g.y.inv;
+            pz = (phi[c + sy] - phi[c]) * (exact ? g.z.inv : g.z.d);
+            mag = sqrt(px * px + py * py + pz * pz);
+            deno = mag * mag * mag;
+            ok = deno != 0;
+            if (!ok) atomicAdd(&kstat[1], 1ull);
+        }
+        if (ok) {
+            double one_over_deno = (double)1 / deno;
+            double cyz = exact ? (double)2 * py * pz * pyz : py * pz * pyz;
+            curv = -(px * px * pyy - (double)2 * px * py * pxy + py * py * pxx + px * px * pzz -
+                     (double)2 * px * pz * pxz + pz * pz * pxx + py * py * pzz - cyz + pz * pz * pyy) *
+                   one_over_deno;
+        }
+        double kv = curv < -tol ? -tol : curv;
+        kappa[c] = kv;
+        if (jc) jc[c] = sigma * kv;
+        if (fabs(phi[c]) < 1e-1 && fabs(kv) > amax) amax = fabs(kv);
+    }
+    wg_max_atomic(amax, kstat);
+}
+
+void fs_curvature(hipStream_t st, const double* phi, double* kappa, double* jc, double sigma, const FsLayout& L,
+                  const FsGridConst& cell, int exact, unsigned long long* kstat) {
+    FsBox b = fs_box(L, 0);
+    double tol = (double)1 / std::min(std::min(cell.x.d, cell.y.d), cell.z.d);
+    dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
+    hipLaunchKernelGGL(k_curvature, g, dim3(BX, BY), 0, st, phi, kappa, jc, sigma, L, cell, tol, exact, kstat);
+}
+
+
 // UpdateVelocityByPressureGradientVariableDensity, PROJECTION_3D.h:1099-1449,
 // one component (comp 1,2,3).  Faces whose + cell is not full are skipped
 // (:1165-1168); the bc product test (:1191) lets a face next to a Neumann
@@ -793,8 +889,9 @@ __device__ __forceinline__ void wg_max_atomic(double a, unsigned long long* bits
 // the projection's ghost copy (linear extrapolation at depth 1, :1115).
 __global__ void __launch_bounds__(BX* BY) k_gradient(int comp, int apply, double* __restrict__ vel,
                                                      const double* __restrict__ p, const double* __restrict__ phi,
-                                                     const double* __restrict__ rho, FsLayout L, FsBox b, FsMaterial m,
-                                                     double eps, double od, double dt, unsigned long long* vmax) {
+                                                     const double* __restrict__ rho, const double* __restrict__ jc,
+                                                     FsLayout L, FsBox b, FsMaterial m, double eps, double od, double dt,
+                                                     unsigned long long* vmax) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double a = 0;
     for (int k = blockIdx.z * KZ; k < min(blockIdx.z * KZ + KZ, b.nk); k++) {
@@ -811,7 +908,9 @@ __global__ void __launch_bounds__(BX* BY) k_gradient(int comp, int apply, double
         double H = fs_heaviside_proj(lface, eps);
         double density_half = m.density(H);
         double one_over_density_ijk = (double)1 / rho[c];
-        double jump_gamma = ((double)0 * fabs(ln) + (double)0 * fabs(lc)) / (fabs(ln) + fabs(lc));
+        // jc's ghosts are zero (PROJECTION_3D.h:358 fills them once)
+        double jcc = jc ? jc[c] : (double)0, jcn = jc ? jc[c - st] : (double)0;
+        double jump_gamma = (jcc * fabs(ln) + jcn * fabs(lc)) / (fabs(ln) + fabs(lc));
         double one_over_density_half = 1 / density_half;
         bool first_cell = (i == 0 && j == 0 && k == 0);
         double pc = p[c], pn = p[c - st];
@@ -831,22 +930,26 @@ __global__ void __launch_bounds__(BX* BY) k_gradient(int comp, int apply, double
 }
 
 void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,
-                        const double* rho, const FsLayout& L, const FsMaterial& m, const FsGridConst& cell,
-                        double dt, unsigned long long* vmax_bits) {
+                        const double* rho, const double* jc, const FsLayout& L, const FsMaterial& m,
+                        const FsGridConst& cell, double dt, unsigned long long* vmax_bits) {
     FsBox b = fs_box(L, comp);
     double dg = comp == 1 ? cell.x.d : comp == 2 ? cell.y.d : cell.z.d;
     double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
     dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
-    hipLaunchKernelGGL(k_gradient, g, dim3(BX, BY), 0, st, comp, apply, vel, p, phi, rho, L, b, m,
+    hipLaunchKernelGGL(k_gradient, g, dim3(BX, BY), 0, st, comp, apply, vel, p, phi, rho, jc, L, b, m,
                        (double)1.5 * dg, od, dt, vmax_bits);
 }
 
 // Gradient update of the fixed schedule: every face takes the pressure
-// matrix's own GFM coefficient, u_f -= dt beta_f (p_c - p_n) / dx, so the
-// corrected velocity is discretely divergence-free (beta is 0 on wall faces,
-// which are left alone).
+// matrix's own GFM coefficient, u_f -= dt beta_f (p_c - p_n + s_c a_f) / dx,
+// with the right-hand side's jump term on faces between the phases (s_c = +1
+// when phi_c <= 0, else -1; a_f as in k_diag_rhs), so the corrected velocity
+// is discretely divergence-free (beta is 0 on wall faces, which are left
+// alone).  jc = NULL: no jump.
 __global__ void __launch_bounds__(BX* BY) k_gradient_beta(double* __restrict__ vel, const double* __restrict__ p,
-                                                          const double* __restrict__ beta, FsLayout L, FsBox b,
+                                                          const double* __restrict__ beta,
+                                                          const double* __restrict__ phi,
+                                                          const double* __restrict__ jc, FsLayout L, FsBox b,
                                                           long st, double od, double dt, unsigned long long* vmax) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
     double a = 0;
@@ -855,7 +958,15 @@ __global__ void __launch_bounds__(BX* BY) k_gradient_beta(double* __restrict__ v
         long c = L.idx(i, j, k);
         double bf = beta[c], vr = vel[c];
         if (bf != 0) {
-            vr -= dt * bf * (p[c] - p[c - st]) * od;
+            double dp = p[c] - p[c - st];
+            if (jc) {
+                double fc = phi[c], fn = phi[c - st];
+                if (!same_phase(fc, fn)) {
+                    double ja = jump_a(fc, fn, jc[c], jc[c - st]);
+                    dp = fc <= 0 ? dp + ja : dp - ja;
+                }
+            }
+            vr -= dt * bf * dp * od;
             vel[c] = vr;
         }
         a = nanmax(a, fabs(vr));
@@ -864,12 +975,13 @@ __global__ void __launch_bounds__(BX* BY) k_gradient_beta(double* __restrict__ v
 }
 
 void fs_gradient_update_beta(hipStream_t st, int comp, double* vel, const double* p, const double* beta,
-                             const FsLayout& L, const FsGridConst& cell, double dt, unsigned long long* vmax_bits) {
+                             const double* phi, const double* jc, const FsLayout& L, const FsGridConst& cell,
+                             double dt, unsigned long long* vmax_bits) {
     FsBox b = fs_box(L, comp);
     double od = comp == 1 ? cell.x.inv : comp == 2 ? cell.y.inv : cell.z.inv;
     long s = comp == 1 ? 1 : comp == 2 ? L.sx : L.sy;
     dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
-    hipLaunchKernelGGL(k_gradient_beta, g, dim3(BX, BY), 0, st, vel, p, beta, L, b, s, od, dt, vmax_bits);
+    hipLaunchKernelGGL(k_gradient_beta, g, dim3(BX, BY), 0, st, vel, p, beta, phi, jc, L, b, s, od, dt, vmax_bits);
 }
 
 // Normal wall faces of component comp set to zero (the fixed schedule's

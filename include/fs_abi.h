@@ -75,7 +75,7 @@ typedef struct fs_params {
     double water_density, air_density;
     double water_viscosity, air_viscosity;
     double gravity[3];
-    double surface_tension;                 /* must be 0 (jump terms vanish) */
+    double surface_tension;                 /* sigma of the jump p_water - p_air = sigma * curvature */
     int    use_delta_function_formulation;  /* explicit viscosity on/off */
     int    order_for_time_advancing;        /* 1 Euler, 2 RK2, 3 TVD-RK3 (EULERIAN_FLUID_SOLVER_3D.h:806-1109) */
     int    sussmanstyle_reinitialization;
@@ -179,13 +179,33 @@ int  fs_step(fs_solver* s, double dt, fs_step_stats* stats);
 int  fs_cfl_dt(fs_solver* s, double cfl, int gravity_norm, double* dt);
 
 /* ---- phases (EULERIAN_FLUID_SOLVER_3D.h:1590-1606, 2076-2237) ------------ */
-int  fs_compute_curvatures(fs_solver* s);           /* LEVELSET_3D::ComputeCurvatures ghost refresh, :998 */
+int  fs_compute_curvatures(fs_solver* s);           /* LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082 */
 int  fs_levelset_advection(fs_solver* s, double dt); /* ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242 */
 int  fs_velocity_advection(fs_solver* s, double dt); /* ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 */
 int  fs_apply_gravity(fs_solver* s, double dt);      /* ApplyGravity, EULERIAN_FLUID_SOLVER_3D.h:2237-2270 */
 int  fs_sourcing(fs_solver* s, double dt);           /* Sourcing -> VISCOSITY_3D::ApplyViscosity, VISCOSITY_3D.h:448-563 */
 int  fs_projection(fs_solver* s, double dt, fs_step_stats* stats); /* PROJECTION_3D::Solve, PROJECTION_3D.h:416-594 */
 int  fs_reinitialization(fs_solver* s, double dt);   /* Reinitialization, EULERIAN_FLUID_SOLVER_3D.h:2198-2225 */
+
+/* ---- surface tension ------------------------------------------------------
+ * With surface_tension != 0 each projection computes the level set's
+ * curvature (LEVELSET_3D::ComputeCurvatures, curvature_by_levelset,
+ * LEVELSET_3D.h:998-1080) and, with use_delta_function_formulation, the jump
+ * condition jc = surface_tension * curvature
+ * (PROJECTION_3D::DetermineJumpConditionField, PROJECTION_3D.h:746-769); the
+ * pressure right-hand side gains the jump terms of
+ * POISSON_SOLVER_3D.h:1215-1328 and the gradient update the jump of
+ * PROJECTION_3D.h:1178-1230, :1287-1339, :1396-1423.  fs_cfl_dt adds the
+ * surface-tension rate s_f (EULERIAN_FLUID_SOLVER_3D.h:2740-2744) from the
+ * curvature of the current level set.  A cell whose gradient vanishes in
+ * central and one-sided differences, where the reference stops the program
+ * (LEVELSET_3D.h:1040-1044), returns FS_ERR_NONFINITE.
+ *
+ * fs_get_curvature: ComputeCurvatures on the current level set (its ghost
+ * refresh included) and the curvature out: kappa nx*ny*nz (may be NULL) and
+ * abs_max_curvature, the maximum |curvature| over cells with |phi| < 0.1
+ * (may be NULL).  FS_ERR_UNSUPPORTED when surface_tension is 0. */
+int  fs_get_curvature(fs_solver* s, double* kappa, double* abs_max_curvature);
 
 /* ---- inspection of the pressure system (per-kernel parity) ------------- */
 

@@ -71,6 +71,12 @@ struct fso_state {
     int last_iters;
     int visc_iters; /* semi-implicit viscosity PCG iterations since the step began */
     double last_res_abs, last_res_rel;
+    /* surface tension: LEVELSET_3D::curvature, PROJECTION_3D::jc_on_solution
+     * (zero ghosts), abs_max_curvature, and whether the last curvature pass
+     * met a cell whose gradient vanishes in both difference forms */
+    double *kappa, *jc;
+    double kmax;
+    int curv_fail;
 };
 
 /* A CSR system with rows numbered plane by plane (per rows in each of
@@ -103,7 +109,6 @@ static double* zalloc(long n) { return (double*)calloc((size_t)n, sizeof(double)
 int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     if (!P || !g || !out || g->nx < 3 || g->ny < 3 || g->nz < 3) return FS_ERR_BAD_ARG;
     if (P->order_for_time_advancing < 1 || P->order_for_time_advancing > 3) return FS_ERR_BAD_ARG;
-    if (P->surface_tension != 0.0) return FS_ERR_UNSUPPORTED;
     fso_state* s = (fso_state*)calloc(1, sizeof(fso_state));
     s->P = *P;
     s->gr = *g;
@@ -132,6 +137,10 @@ int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     s->vtmp = zalloc(n);
     s->urk = zalloc(n); s->vrk = zalloc(n); s->wrk = zalloc(n);
     s->fbx = zalloc(n); s->fby = zalloc(n); s->fbz = zalloc(n);
+    if (P->surface_tension != 0.0) {
+        s->kappa = zalloc(n);
+        s->jc = zalloc(n);
+    }
     s->bc = (int*)calloc((size_t)n, sizeof(int));
     s->plane_part = (double*)calloc((size_t)s->nz + 1, sizeof(double));
     s->visc_inplace = 1;
@@ -149,6 +158,7 @@ void fso_destroy(fso_state* s) {
     if (!s) return;
     free(s->phi); free(s->sghost); free(s->u); free(s->v); free(s->w); free(s->gu); free(s->gv); free(s->gw);
     free(s->p); free(s->div); free(s->rho); free(s->irho); free(s->sgn); free(s->tmp); free(s->pg); free(s->vtmp); free(s->fbx); free(s->fby); free(s->fbz);
+    free(s->kappa); free(s->jc);
     free(s->bc);
     free(s->urk); free(s->vrk); free(s->wrk);
     free(s->plane_part);
@@ -415,12 +425,93 @@ static void eno3_stage(fso_state* s, double* rho, const double* ghost, int comp,
 /* Phases                                                                     */
 /* ------------------------------------------------------------------------- */
 
-/* LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082.  With zero surface
- * tension the curvature never reaches the fields; the call's lasting effect
- * is the linear re-extrapolation of phi's own ghost cells (:998). */
+static int is_fixed_schedule(const fso_state* s) { return s->P.schedule == FS_SCHEDULE_FIXED; }
+
+/* Curvature by level set, LEVELSET_3D::ComputeCurvatures (LEVELSET_3D.h:
+ * 1001-1080), on phi with the ghosts the caller left.  The derivative fields
+ * phi_x .. phi_yz of :1005-1013 are folded into the cell loop (each is read
+ * only at its own cell).  The compatible schedule keeps the reference's
+ * cross term "py pz pyz" (:1033, the mean-curvature formula has 2 py pz pyz)
+ * and its one-sided z difference scaled by dz (:1037, 1/dz intended); the
+ * fixed schedule uses 2 and 1/dz.  Where the one-sided gradient vanishes as
+ * well the reference stops the program (:1040-1044): here the curvature is
+ * 0 and curv_fail is set.  jc (may be NULL) = sigma * curvature, at interior
+ * cells only (PROJECTION_3D.h:752-767). */
+static void curvature(fso_state* s, double* jc) {
+    const gc_t* g = &s->cell;
+    const double* phi = s->phi;
+    double tolerance = (double)1 / fmin(fmin(g->x.d, g->y.d), g->z.d); /* MIN3 of finite spacings */
+    int exact = is_fixed_schedule(s);
+    double abs_max_curvature = 0;
+    s->curv_fail = 0;
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                double px = (A3(phi, i + 1, j, k) - A3(phi, i - 1, j, k)) * g->x.inv2d;
+                double py = (A3(phi, i, j + 1, k) - A3(phi, i, j - 1, k)) * g->y.inv2d;
+                double pz = (A3(phi, i, j, k + 1) - A3(phi, i, j, k - 1)) * g->z.inv2d;
+                double pxx = (A3(phi, i + 1, j, k) - 2 * A3(phi, i, j, k) + A3(phi, i - 1, j, k)) * g->x.inv_d2;
+                double pyy = (A3(phi, i, j + 1, k) - 2 * A3(phi, i, j, k) + A3(phi, i, j - 1, k)) * g->y.inv_d2;
+                double pzz = (A3(phi, i, j, k + 1) - 2 * A3(phi, i, j, k) + A3(phi, i, j, k - 1)) * g->z.inv_d2;
+                double pxy = (A3(phi, i + 1, j + 1, k) - A3(phi, i + 1, j - 1, k) - A3(phi, i - 1, j + 1, k) +
+                              A3(phi, i - 1, j - 1, k)) * g->x.inv2d * g->y.inv2d;
+                double pxz = (A3(phi, i + 1, j, k + 1) - A3(phi, i + 1, j, k - 1) - A3(phi, i - 1, j, k + 1) +
+                              A3(phi, i - 1, j, k - 1)) * g->x.inv2d * g->z.inv2d;
+                double pyz = (A3(phi, i, j + 1, k + 1) - A3(phi, i, j + 1, k - 1) - A3(phi, i, j - 1, k + 1) +
+                              A3(phi, i, j - 1, k - 1)) * g->y.inv2d * g->z.inv2d;
+                double magnitude = sqrt(px * px + py * py + pz * pz);
+                double deno = magnitude * magnitude * magnitude;
+                double curv = 0;
+                if (deno == 0) {
+                    px = (A3(phi, i + 1, j, k) - A3(phi, i, j, k)) * g->x.inv;
+                    py = (A3(phi, i, j + 1, k) - A3(phi, i, j, k)) * g->y.inv;
+                    pz = (A3(phi, i, j, k + 1) - A3(phi, i, j, k)) * (exact ? g->z.inv : g->z.d);
+                    magnitude = sqrt(px * px + py * py + pz * pz);
+                    deno = magnitude * magnitude * magnitude;
+                    if (deno == 0) s->curv_fail = 1;
+                }
+                if (deno != 0) {
+                    double one_over_deno = (double)1 / deno;
+                    double cross_yz = exact ? (double)2 * py * pz * pyz : py * pz * pyz;
+                    curv = -(px * px * pyy - (double)2 * px * py * pxy + py * py * pxx + px * px * pzz -
+                             (double)2 * px * pz * pxz + pz * pz * pxx + py * py * pzz - cross_yz + pz * pz * pyy) *
+                           one_over_deno;
+                }
+                double kv = curv < -tolerance ? -tolerance : curv; /* :1061-1068 */
+                A3(s->kappa, i, j, k) = kv;
+                if (jc) A3(jc, i, j, k) = s->P.surface_tension * kv;
+                if (fabs(A3(phi, i, j, k)) < 1e-1) /* :1070-1077, MAX skips NaN */
+                    abs_max_curvature = fabs(kv) > abs_max_curvature ? fabs(kv) : abs_max_curvature;
+            }
+    s->kmax = abs_max_curvature;
+}
+
+/* LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082: linear
+ * re-extrapolation of phi's own ghost cells (:998), then the curvature.  With
+ * zero surface tension nothing reads the curvature; the ghost refresh is the
+ * call's lasting effect. */
 int fso_compute_curvatures(fso_state* s) {
     fill_linear(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0, 1);
+    if (s->P.surface_tension != 0.0) curvature(s, NULL);
     return FS_OK;
+}
+
+static void get(fso_state* s, const double* src, double* dst, int ni, int nj, int nk);
+
+int fso_get_curvature(fso_state* s, double* kappa, double* abs_max_curvature) {
+    if (s->P.surface_tension == 0.0) return FS_ERR_UNSUPPORTED;
+    fso_compute_curvatures(s);
+    if (s->curv_fail) return FS_ERR_NONFINITE;
+    if (kappa) get(s, s->kappa, kappa, s->nx, s->ny, s->nz);
+    if (abs_max_curvature) *abs_max_curvature = s->kmax;
+    return FS_OK;
+}
+
+/* jc_on_solution as the pressure solve reads it, or NULL where it stays at
+ * its initial zeros (no surface tension, or use_delta_function_formulation
+ * off: PROJECTION_3D.h:750). */
+static const double* jump_field(const fso_state* s) {
+    return s->P.surface_tension != 0.0 && s->P.use_delta_function_formulation ? s->jc : NULL;
 }
 
 /* LEVELSET_3D::ComputeNormals, LEVELSET_3D.h:929-956: lasting effect is the
@@ -887,9 +978,20 @@ static void setup_bc(fso_state* s) {
 #define GFM_BETA(vc, vn, fn, fc) ((vc) * (vn) * (fabs(fn) + fabs(fc)) / ((vc) * fabs(fn) + (vn) * fabs(fc)))
 #define GFM_BETA_R(vc, vn, fn, fc) ((vc) * (vn) * (fabs(fn) + fabs(fc)) / ((vn) * fabs(fc) + (vc) * fabs(fn)))
 
+/* One arm of the jump stencil, POISSON_SOLVER_3D.h:1217-1324: F_L, F_R, ...
+ * for the face between cell c and neighbour n.  jc_on_derivative is zero in
+ * this solver, so the b_* terms vanish and only +-one_over_d2 a beta stays;
+ * + when phi_c <= 0 < phi_n, - when phi_n <= 0 < phi_c, 0 within a phase. */
+static double jump_arm(double fc, double fn, double jcc, double jcn, double one_over_d2, double beta) {
+    double a = (jcc * fabs(fn) + jcn * fabs(fc)) / (fabs(fc) + fabs(fn));
+    if ((fc <= 0 && fn <= 0) || (fc > 0 && fn > 0)) return 0;
+    if (fc <= 0 && fn > 0) return one_over_d2 * a * beta;
+    return -one_over_d2 * a * beta;
+}
+
 /* PROJECTION_3D::DeterminePressure (:794-834) up to the solve:
  * POISSON_SOLVER_3D::BuildLinearSystemNodeJumpConditionVaribleCoefficient
- * (:1030-1358, air/water, zero jump) and the Neumann mean removal (:225-246). */
+ * (:1030-1358, air/water) and the Neumann mean removal (:225-246). */
 static void assemble(fso_state* s) {
     const fs_params* P = &s->P;
     if (!P->use_previous_pressure)
@@ -913,6 +1015,7 @@ static void assemble(fso_state* s) {
     const double inv_dzdz = (double)1 / (s->cell.z.d * s->cell.z.d);
     const double* vv = s->irho;
     const double* f = s->phi;
+    const double* jc = jump_field(s);
     int nz_ix = 0;
     int i0 = 0, j0 = 0, k0 = 0;
     memset(s->fbx, 0, sizeof(double) * (size_t)s->ntot);
@@ -953,7 +1056,19 @@ static void assemble(fso_state* s) {
                 if (coef == 0) coef = 1;
                 s->ci[nz_ix] = row;
                 s->val[nz_ix++] = inv_dxdx * coef;
-                s->b[row] = -A3(s->div, i, j, k);
+                if (jc) { /* :1215-1328 */
+                    double jcc = A3(jc, i, j, k);
+                    double F_L = jump_arm(fc, A3(f, i - 1, j, k), jcc, A3(jc, i - 1, j, k), s->cell.x.inv_d2, bl);
+                    double F_R = jump_arm(fc, A3(f, i + 1, j, k), jcc, A3(jc, i + 1, j, k), s->cell.x.inv_d2, br);
+                    double F_B = jump_arm(fc, A3(f, i, j - 1, k), jcc, A3(jc, i, j - 1, k), s->cell.y.inv_d2, bb);
+                    double F_T = jump_arm(fc, A3(f, i, j + 1, k), jcc, A3(jc, i, j + 1, k), s->cell.y.inv_d2, bt);
+                    double F_D = jump_arm(fc, A3(f, i, j, k - 1), jcc, A3(jc, i, j, k - 1), s->cell.z.inv_d2, bd);
+                    double F_U = jump_arm(fc, A3(f, i, j, k + 1), jcc, A3(jc, i, j, k + 1), s->cell.z.inv_d2, bu);
+                    double F_X = F_L + F_R, F_Y = F_B + F_T, F_Z = F_D + F_U;
+                    s->b[row] = -A3(s->div, i, j, k) - F_X - F_Y - F_Z;
+                } else {
+                    s->b[row] = -A3(s->div, i, j, k);
+                }
                 s->x[row] = A3(s->p, i, j, k);
             }
     s->rp[N] = nz_ix;
@@ -1075,6 +1190,7 @@ static double gradient_update(fso_state* s, int comp, double dt, int apply) {
     double od = comp == 1 ? s->cell.x.inv : comp == 2 ? s->cell.y.inv : s->cell.z.inv;
     double dgrid = comp == 1 ? s->cell.x.d : comp == 2 ? s->cell.y.d : s->cell.z.d;
     double eps = (double)1.5 * dgrid; /* :1126, :1137, :1148 */
+    const double* jc = jump_field(s);
     double vmax = 0;
     for (int k = 0; k < nk; k++)
         for (int j = 0; j < nj; j++)
@@ -1090,7 +1206,9 @@ static double gradient_update(fso_state* s, int comp, double dt, int apply) {
                 double density_ijk = A3(s->rho, i, j, k);
                 double one_over_density_ijk = (double)1 / density_ijk;
                 double lc = A3(s->phi, i, j, k), ln = A3(s->phi, i - di, j - dj, k - dk);
-                double jump_gamma = ((double)0 * fabs(ln) + (double)0 * fabs(lc)) / (fabs(ln) + fabs(lc));
+                /* :1178-1181; jc_on_solution's ghosts stay 0 */
+                double jc_c = jc ? A3(jc, i, j, k) : (double)0, jc_n = jc ? A3(jc, i - di, j - dj, k - dk) : (double)0;
+                double jump_gamma = (jc_c * fabs(ln) + jc_n * fabs(lc)) / (fabs(ln) + fabs(lc));
                 double one_over_density_half = 1 / density_half;
                 double bcd = (double)bcc, bcn = (double)s->bc[IX(s, i - di, j - dj, k - dk)];
                 double* vr = &A3(vel, i, j, k);
@@ -1110,13 +1228,18 @@ static double gradient_update(fso_state* s, int comp, double dt, int apply) {
 }
 
 /* Fixed schedule: every face corrected with the pressure matrix's own face
- * coefficient, u -= dt beta (p_c - p_n) / dx (beta is 0 on wall faces). */
+ * coefficient, u -= dt beta (p_c - p_n + s_c a) / dx (beta is 0 on wall
+ * faces), where a face between the phases carries the right-hand side's jump
+ * a = (jc_c |phi_n| + jc_n |phi_c|) / (|phi_c| + |phi_n|) with s_c = +1 when
+ * phi_c <= 0, else -1: the corrected velocity is then discretely
+ * divergence-free with the jump included. */
 static double gradient_update_beta(fso_state* s, int comp, double dt) {
     int ni, nj, nk;
     extents(s, comp, &ni, &nj, &nk);
     int di = comp == 1, dj = comp == 2, dk = comp == 3;
     double* vel = comp == 1 ? s->u : comp == 2 ? s->v : s->w;
     const double* beta = comp == 1 ? s->fbx : comp == 2 ? s->fby : s->fbz;
+    const double* jc = jump_field(s);
     double od = comp == 1 ? s->cell.x.inv : comp == 2 ? s->cell.y.inv : s->cell.z.inv;
     double vmax = 0;
     for (int k = 0; k < nk; k++)
@@ -1124,19 +1247,36 @@ static double gradient_update_beta(fso_state* s, int comp, double dt) {
             for (int i = 0; i < ni; i++) {
                 double* vr = &A3(vel, i, j, k);
                 double bf = A3(beta, i, j, k);
-                if (bf != 0) *vr -= dt * bf * (A3(s->p, i, j, k) - A3(s->p, i - di, j - dj, k - dk)) * od;
+                if (bf != 0) {
+                    double dp = A3(s->p, i, j, k) - A3(s->p, i - di, j - dj, k - dk);
+                    if (jc) {
+                        double fc = A3(s->phi, i, j, k), fn = A3(s->phi, i - di, j - dj, k - dk);
+                        if (!((fc <= 0 && fn <= 0) || (fc > 0 && fn > 0))) {
+                            double a = (A3(jc, i, j, k) * fabs(fn) + A3(jc, i - di, j - dj, k - dk) * fabs(fc)) /
+                                       (fabs(fc) + fabs(fn));
+                            dp = fc <= 0 ? dp + a : dp - a;
+                        }
+                    }
+                    *vr -= dt * bf * dp * od;
+                }
                 double a = fabs(*vr);
                 vmax = vmax > a ? vmax : a;
             }
     return vmax;
 }
 
+/* PROJECTION_3D::DetermineJumpConditionField, :746-792: ComputeCurvatures,
+ * then jc_on_solution = surface_tension * curvature when
+ * use_delta_function_formulation (its ghosts stay 0). */
+static void determine_jump_condition(fso_state* s) {
+    fill_linear(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0, 1);
+    if (s->P.surface_tension != 0.0) curvature(s, s->P.use_delta_function_formulation ? s->jc : NULL);
+}
+
 static void projection_solve(fso_state* s, double dt) {
     projection_density(s);
     projection_divergence(s, dt);
-    /* DetermineJumpConditionField, :746-792: ComputeCurvatures' ghost refresh;
-     * with zero surface tension jc = 0 */
-    fso_compute_curvatures(s);
+    determine_jump_condition(s);
     assemble(s);
     pcg(s);
     /* POISSON_SOLVER_3D::VectorToGrid, :4060-4078 */
@@ -1161,6 +1301,7 @@ static void projection_solve(fso_state* s, double dt) {
 int fso_projection(fso_state* s, double dt, fs_step_stats* st) {
     if (!(dt > 0)) return FS_ERR_BAD_ARG;
     projection_solve(s, dt);
+    if (s->P.surface_tension != 0.0 && s->curv_fail) return FS_ERR_NONFINITE;
     if (st) {
         st->pcg_iterations = s->last_iters;
         st->refinement_sweeps = 0;
@@ -1177,7 +1318,7 @@ int fso_assemble_pressure_system(fso_state* s, double dt, double* diag, double* 
     if (!(dt > 0)) return FS_ERR_BAD_ARG;
     projection_density(s);
     projection_divergence(s, dt);
-    fso_compute_curvatures(s);
+    determine_jump_condition(s);
     assemble(s);
     int nx = s->nx, ny = s->ny, nz = s->nz;
     for (int k = 0; k < nz; k++)
@@ -1438,7 +1579,17 @@ int fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt) {
                                    : fabs(P->gravity[1]);
         g_f = sqrt(gmag * s->my.y.inv);
     }
+    /* :2740-2744 (air/water), abs_max_curvature of the level set as
+     * SIMULATION_WORLD::AdvanceOneFrame leaves it right before the rule
+     * (SIMULATION_WORLD.h:324, :289) */
     double s_f = 0;
+    if (P->surface_tension != 0.0) {
+        fso_compute_curvatures(s);
+        if (s->curv_fail) return FS_ERR_NONFINITE;
+        double dmin = fmin(fmin(s->mx.x.d, s->my.y.d), s->mz.z.d);
+        double rmin = P->water_density > P->air_density ? P->air_density : P->water_density;
+        s_f = sqrt((P->surface_tension * s->kmax) / (rmin * (dmin * dmin)));
+    }
     double a = P->water_viscosity / P->water_density, b = P->air_viscosity / P->air_density;
     double v_f = (a > b ? a : b) * ((double)2 * s->cell.x.inv_d2 + (double)2 * s->cell.y.inv_d2 + (double)2 * s->cell.z.inv_d2);
     if (P->semi_implicit_viscosity) v_f = 0; /* :2770-2773 */

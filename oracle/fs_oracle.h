@@ -45,6 +45,7 @@ int  fso_step(fso_state* s, double dt, fs_step_stats* st);
 int  fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt);
 
 int  fso_compute_curvatures(fso_state* s);
+int  fso_get_curvature(fso_state* s, double* kappa, double* abs_max_curvature);
 int  fso_levelset_advection(fso_state* s, double dt);
 int  fso_velocity_advection(fso_state* s, double dt);
 int  fso_apply_gravity(fso_state* s, double dt);

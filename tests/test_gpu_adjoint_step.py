@@ -135,7 +135,8 @@ def test_adjoint_step_leaves_fields_and_rejects_unsupported(hip_lib):
     after = s.get_fields()
     for c in before:
         assert np.array_equal(before[c], after[c]), c
-    for over in (dict(schedule=_abi.FS_SCHEDULE_COMPAT), dict(use_5th_weno_v=0), dict(semi_implicit_viscosity=1)):
+    for over in (dict(schedule=_abi.FS_SCHEDULE_COMPAT), dict(use_5th_weno_v=0), dict(semi_implicit_viscosity=1),
+                 dict(surface_tension=0.0728)):
         t, _ = _solver(hip_lib, "dam_break", n, **over)
         t.set_fields(phi, *vel)
         with pytest.raises(_abi.FsError) as e:

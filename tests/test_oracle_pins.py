@@ -362,10 +362,6 @@ def test_error_paths(lib):
     with pytest.raises(_abi.FsError) as e:
         s.step(-1.0)
     assert e.value.code == _abi.FS_ERR_BAD_ARG
-    P = _abi.default_params(surface_tension=0.07)
-    with pytest.raises(_abi.FsError) as e:
-        _abi.Solver(lib, "fso_", P, _abi.unit_cube_grid(8))
-    assert e.value.code == _abi.FS_ERR_UNSUPPORTED
     u = vel[0].copy()
     u[2, 2, 2] = np.nan
     s.set_fields(u=u)
