@@ -23,6 +23,9 @@ FS_STOP_ABS_PRECOND = 0
 FS_STOP_REL_RESIDUAL = 1
 FS_SCHEDULE_COMPAT = 0
 FS_SCHEDULE_FIXED = 1
+FS_SOURCE_NONE = 0
+FS_SOURCE_ADD = 1
+FS_SOURCE_DELETE = 2
 
 ERROR_NAMES = {
     FS_ERR_BAD_ARG: "FS_ERR_BAD_ARG",
@@ -145,6 +148,21 @@ def unit_cube_grid(nx, ny=None, nz=None):
 _PD = C.POINTER(C.c_double)
 
 
+class FsLevelsetSource(C.Structure):
+    _fields_ = [("center", _D * 3), ("radius", _D), ("half_width", _D), ("mode", _I),
+                ("start_frame", _I), ("stop_frame", _I)]
+
+
+class FsWorldOptions(C.Structure):
+    _fields_ = [("precision", _I), ("stop_rule", _I), ("rel_tolerance", _D), ("schedule", _I)]
+
+
+class FsWorldInfo(C.Structure):
+    _fields_ = [("params", FsParams), ("grid", FsGrid),
+                ("dt", _D), ("max_dt", _D), ("cfl", _D), ("frame_rate", _D), ("accu_dt", _D),
+                ("num_substeps", _I), ("frame", _I), ("last_frame", _I), ("n_sources", _I)]
+
+
 def _ptr(a):
     if a is None:
         return None
@@ -169,6 +187,9 @@ def bind(lib, prefix):
         f(n).argtypes = [vp, _D, C.POINTER(FsStepStats)]
     f("cfl_dt").argtypes = [vp, _D, _I, C.POINTER(_D)]
     f("compute_curvatures").argtypes = [vp]
+    f("set_levelset_sources").argtypes = [vp, _I, C.POINTER(FsLevelsetSource)]
+    f("set_frame").argtypes = [vp, _I]
+    f("apply_levelset_sources").argtypes = [vp]
     f("get_curvature").argtypes = [vp, _PD, _PD]
     for n in ("levelset_advection", "velocity_advection", "apply_gravity", "sourcing", "reinitialization"):
         f(n).argtypes = [vp, _D]
@@ -197,6 +218,29 @@ def bind(lib, prefix):
         lib.fs_batch_advance_frame.argtypes = [vp, _PD, C.POINTER(FsStepStats), C.POINTER(_I)]
         lib.fs_batch_last_error.argtypes = [vp]
         lib.fs_batch_last_error.restype = C.c_char_p
+        cs = C.c_char_p
+        lib.fs_script_open.argtypes = [cs, C.POINTER(vp)]
+        lib.fs_script_close.argtypes = [vp]
+        lib.fs_script_close.restype = None
+        lib.fs_script_get_int.argtypes = [vp, cs, cs, _I, C.POINTER(_I)]
+        lib.fs_script_get_float.argtypes = [vp, cs, cs, _D, C.POINTER(_D)]
+        lib.fs_script_get_bool.argtypes = [vp, cs, cs, _I, C.POINTER(_I)]
+        lib.fs_script_get_vector3.argtypes = [vp, cs, cs, _PD, _PD]
+        lib.fs_script_get_int3.argtypes = [vp, cs, cs, C.POINTER(_I), C.POINTER(_I)]
+        lib.fs_script_get_string.argtypes = [vp, cs, cs, cs, C.c_char_p, _I]
+        lib.fs_script_child_count.argtypes = [vp, cs, C.POINTER(_I)]
+        lib.fs_script_child_name.argtypes = [vp, cs, _I, C.c_char_p, _I]
+        lib.fs_world_create.argtypes = [cs, C.POINTER(FsWorldOptions), _I, C.POINTER(vp)]
+        lib.fs_world_destroy.argtypes = [vp]
+        lib.fs_world_destroy.restype = None
+        lib.fs_world_last_error.argtypes = [vp]
+        lib.fs_world_last_error.restype = C.c_char_p
+        lib.fs_world_solver.argtypes = [vp]
+        lib.fs_world_solver.restype = vp
+        lib.fs_world_get_info.argtypes = [vp, C.POINTER(FsWorldInfo)]
+        lib.fs_world_advance_frame.argtypes = [vp, C.POINTER(FsStepStats)]
+        lib.fs_world_parse.argtypes = [cs, C.POINTER(FsWorldOptions), C.POINTER(FsWorldInfo),
+                                       C.POINTER(FsLevelsetSource), _I]
     return lib
 
 
@@ -306,6 +350,24 @@ class Solver:
         m = C.c_double()
         self._check(self._f("get_curvature")(self.h, _ptr(k), C.byref(m)), "get_curvature")
         return k, m.value
+
+    def set_sources(self, sources):
+        """Level-set sources: a list of dicts with center, radius, half_width,
+        mode and optional start_frame / stop_frame."""
+        arr = (FsLevelsetSource * max(1, len(sources)))()
+        for a, d in zip(arr, sources):
+            for i in range(3):
+                a.center[i] = d["center"][i]
+            a.radius, a.half_width, a.mode = d["radius"], d["half_width"], d["mode"]
+            a.start_frame = d.get("start_frame", -(2 ** 31 - 1))
+            a.stop_frame = d.get("stop_frame", 2 ** 31 - 1)
+        self._check(self._f("set_levelset_sources")(self.h, len(sources), arr), "set_levelset_sources")
+
+    def set_frame(self, frame):
+        self._check(self._f("set_frame")(self.h, frame), "set_frame")
+
+    def apply_sources(self):
+        self._check(self._f("apply_levelset_sources")(self.h), "apply_levelset_sources")
 
     def phase(self, name, dt):
         self._check(self._f(name)(self.h, dt), name)
@@ -454,3 +516,114 @@ def load(path, prefix):
     if not os.path.exists(path):
         raise FileNotFoundError(path)
     return bind(C.CDLL(path), prefix)
+
+
+def _opts(precision=FS_FP64, stop_rule=FS_STOP_ABS_PRECOND, rel_tolerance=1e-8, schedule=FS_SCHEDULE_COMPAT):
+    o = FsWorldOptions()
+    o.precision, o.stop_rule, o.rel_tolerance, o.schedule = precision, stop_rule, rel_tolerance, schedule
+    return o
+
+
+class Script:
+    """A parsed .sim script (SCRIPT_READER semantics) of the HIP library."""
+
+    def __init__(self, lib, path):
+        self.lib = lib
+        self.h = C.c_void_p()
+        rc = lib.fs_script_open(os.fsencode(path), C.byref(self.h))
+        if rc != FS_OK:
+            raise FsError(rc, f"cannot read {path}")
+
+    def close(self):
+        if self.h:
+            self.lib.fs_script_close(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def int(self, block, key, default=0):
+        v = _I()
+        self.lib.fs_script_get_int(self.h, block.encode(), key.encode(), default, C.byref(v))
+        return v.value
+
+    def float(self, block, key, default=0.0):
+        v = _D()
+        self.lib.fs_script_get_float(self.h, block.encode(), key.encode(), default, C.byref(v))
+        return v.value
+
+    def bool(self, block, key, default=False):
+        v = _I()
+        self.lib.fs_script_get_bool(self.h, block.encode(), key.encode(), int(default), C.byref(v))
+        return bool(v.value)
+
+    def vector3(self, block, key, default=(0.0, 0.0, 0.0)):
+        d, o = (_D * 3)(*default), (_D * 3)()
+        self.lib.fs_script_get_vector3(self.h, block.encode(), key.encode(), d, o)
+        return tuple(o)
+
+    def int3(self, block, key, default=(0, 0, 0)):
+        d, o = (_I * 3)(*default), (_I * 3)()
+        self.lib.fs_script_get_int3(self.h, block.encode(), key.encode(), d, o)
+        return tuple(o)
+
+    def string(self, block, key, default="Null"):
+        buf = C.create_string_buffer(4096)
+        self.lib.fs_script_get_string(self.h, block.encode(), key.encode(), default.encode(), buf, 4096)
+        return buf.value.decode()
+
+    def children(self, block=""):
+        n = _I()
+        self.lib.fs_script_child_count(self.h, block.encode(), C.byref(n))
+        out = []
+        for i in range(n.value):
+            buf = C.create_string_buffer(256)
+            self.lib.fs_script_child_name(self.h, block.encode(), i, buf, 256)
+            out.append(buf.value.decode())
+        return out
+
+
+def parse_world(lib, path, **opts):
+    """(FsWorldInfo, [FsLevelsetSource]) a script sets up, without a device."""
+    info = FsWorldInfo()
+    src = (FsLevelsetSource * 64)()
+    o = _opts(**opts)
+    rc = lib.fs_world_parse(os.fsencode(path), C.byref(o), C.byref(info), src, 64)
+    if rc != FS_OK:
+        raise FsError(rc, f"fs_world_parse {path}")
+    return info, [src[i] for i in range(min(64, info.n_sources))]
+
+
+class World:
+    """SIMULATION_WORLD from a .sim script on the HIP library."""
+
+    def __init__(self, lib, path, device=0, **opts):
+        self.lib = lib
+        self.h = C.c_void_p()
+        o = _opts(**opts)
+        rc = lib.fs_world_create(os.fsencode(path), C.byref(o), device, C.byref(self.h))
+        if rc != FS_OK:
+            raise FsError(rc, f"fs_world_create {path}")
+        info = self.info()
+        self.solver = Solver(lib, "fs_", info.params, info.grid, device, handle=lib.fs_world_solver(self.h))
+
+    def info(self):
+        i = FsWorldInfo()
+        self.lib.fs_world_get_info(self.h, C.byref(i))
+        return i
+
+    def advance_frame(self, allow_not_converged=False):
+        st = FsStepStats()
+        rc = self.lib.fs_world_advance_frame(self.h, C.byref(st))
+        if rc != FS_OK and not (allow_not_converged and rc == FS_ERR_NOT_CONVERGED):
+            m = self.lib.fs_world_last_error(self.h)
+            raise FsError(rc, f"advance_frame: {m.decode() if m else ''}")
+        return st.as_dict()
+
+    def close(self):
+        if self.h:
+            self.lib.fs_world_destroy(self.h)
+            self.h = C.c_void_p()

@@ -4,7 +4,8 @@
 
 hipcc compiles every csrc/*.hip file for gfx950 with FMA contraction off
 (operation order is part of the parity contract) and packed-FP32 ops
-disabled, then links one shared library in the package directory.
+disabled, g++ the host-only csrc/*.cpp files (the scene layer), then hipcc
+links one shared library in the package directory.
 """
 import concurrent.futures as cf
 import os
@@ -17,6 +18,10 @@ CSRC = os.path.join(PKG, "csrc")
 BUILD = os.path.join(PKG, "_build")
 LIB = os.path.join(PKG, "libfs_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
+
+CXX = os.environ.get("CXX_HOST", "g++")
+# host-only C++ (the scene layer and its script reader): no device code
+CXX_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-ffp-contract=off", "-Wall", "-Wno-unused-function"]
 
 HIP_FLAGS = [
     "--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
@@ -34,14 +39,19 @@ def _run(cmd):
 
 def build_hip(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
-    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip"))
+    srcs = sorted(f for f in os.listdir(CSRC) if f.endswith(".hip") or f.endswith(".cpp"))
     objs = []
     jobs = []
     with cf.ThreadPoolExecutor(max_workers=min(8, len(srcs))) as ex:
         for s in srcs:
-            o = os.path.join(BUILD, s[:-4] + ".o")
+            base, ext = os.path.splitext(s)
+            o = os.path.join(BUILD, base + (".o" if ext == ".hip" else ".host.o"))
             objs.append(o)
-            jobs.append(ex.submit(_run, [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o]))
+            if ext == ".hip":
+                cmd = [HIPCC] + HIP_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o]
+            else:
+                cmd = [CXX] + CXX_FLAGS + ["-c", os.path.join(CSRC, s), "-o", o]
+            jobs.append(ex.submit(_run, cmd))
         for j in jobs:
             r = j.result()
             if verbose and r.stderr:

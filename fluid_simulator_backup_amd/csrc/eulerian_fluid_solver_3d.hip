@@ -85,6 +85,10 @@ class EulerianFluidSolver3D {
     // cells with a vanishing gradient] of the last curvature pass
     double *kappa = nullptr, *jc = nullptr;
     unsigned long long *kstat = nullptr, *hkstat = nullptr;
+    // the scene's objects that source the level set, and the frame number
+    // their windows compare with
+    std::vector<fs_levelset_source> sources;
+    int frame = 0;
     double* staging = nullptr;                                   // host <-> device exchange
     long staging_n = 0;
     bool assembled = false;
@@ -421,7 +425,39 @@ class EulerianFluidSolver3D {
     // fixed schedule reads phi's ghosts as the constant extrapolation of the
     // current level set and zeroes each component's wall faces after its
     // update.
+    // EULERIAN_FLUID_SOLVER_3D::Sourcing, :2076-2100: the viscosity, then the
+    // objects' level-set sourcing when the scene has objects.
     int Sourcing(double dt) {
+        int rc = ApplyViscosity(dt);
+        if (!sources.empty()) SourcingFromObjectList();
+        return rc;
+    }
+
+    // SourcingFromObjectList, :2592-2671, spherical objects: each source in
+    // its frame window over the cells of its box (SIMULATION_OBJECT::
+    // BuildAABBGrid, SIMULATION_OBJECT.h:149-160: GRID_STRUCTURE_3D::ClampedCell
+    // of the box corners, GRID_STRUCTURE_3D.h:310-319).
+    void SourcingFromObjectList() {
+        const FsAxis* ax[3] = {&cell.x, &cell.y, &cell.z};
+        const double origin[3] = {G.x_min, G.y_min, G.z_min};
+        const int res[3] = {G.nx, G.ny, G.nz};
+        for (const fs_levelset_source& o : sources) {
+            if (o.mode == FS_SOURCE_NONE || !(o.start_frame <= frame && frame <= o.stop_frame)) continue;
+            int lo[3], hi[3];
+            for (int a = 0; a < 3; a++) {
+                double fl = std::floor((o.center[a] - o.half_width - origin[a]) * ax[a]->inv);
+                double fh = std::floor((o.center[a] + o.half_width - origin[a]) * ax[a]->inv);
+                lo[a] = (int)std::min(std::max(fl, 0.0), (double)(res[a] - 1));
+                hi[a] = (int)std::min(std::max(fh, 0.0), (double)(res[a] - 1));
+            }
+            fs_source_sphere(st, phi, L, lo, hi, o.center, o.radius, cell, origin, o.mode);
+        }
+        check_launch();
+    }
+
+    // VISCOSITY_3D::ApplyViscosity (air/water explicit, or the semi-implicit
+    // systems), VISCOSITY_3D.h:448-563
+    int ApplyViscosity(double dt) {
         if (!P.use_delta_function_formulation) return FS_OK;
         if (fixed()) fs_fill_const(st, phi, phi, L, box(0), 2);
         if (P.semi_implicit_viscosity) {
@@ -826,6 +862,8 @@ class EulerianFluidSolver3D {
                                              "is discontinuous)"};
         if (tension() && P.use_delta_function_formulation && P.is_projection_active)
             throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the surface-tension jump (curvature of phi) is not built"};
+        if (!sources.empty() && P.is_sourcing_active)
+            throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the level-set sources (a min/max of phi) is not built"};
     }
 
     void EnsureAdjoint() {
@@ -1495,6 +1533,35 @@ int fs_compute_curvatures(fs_solver* s) {
         S.check_launch();
         S.sync();
         S.check_curvature();
+        return FS_OK;
+    });
+}
+
+int fs_set_levelset_sources(fs_solver* s, int n, const fs_levelset_source* src) {
+    if (n < 0 || n > 64 || (n > 0 && !src)) return FS_ERR_BAD_ARG;
+    for (int i = 0; i < n; i++) {
+        const fs_levelset_source& o = src[i];
+        bool finite = std::isfinite(o.radius) && std::isfinite(o.half_width);
+        for (double c : o.center) finite = finite && std::isfinite(c);
+        if (!finite || o.mode < FS_SOURCE_NONE || o.mode > FS_SOURCE_DELETE) return FS_ERR_BAD_ARG;
+    }
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.sources.assign(src, src + n);
+        return FS_OK;
+    });
+}
+
+int fs_set_frame(fs_solver* s, int frame) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.frame = frame;
+        return FS_OK;
+    });
+}
+
+int fs_apply_levelset_sources(fs_solver* s) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.SourcingFromObjectList();
+        S.sync();
         return FS_OK;
     });
 }

@@ -65,6 +65,12 @@ void fs_assemble(hipStream_t st, const double* phi, const double* irho, const do
 // exact: the corrected cross term and one-sided z difference.
 void fs_curvature(hipStream_t st, const double* phi, double* kappa, double* jc, double sigma, const FsLayout& L,
                   const FsGridConst& cell, int exact, unsigned long long* kstat);
+// one spherical level-set source over the cells lo..hi (inclusive, inside
+// the grid): mode 1 phi = min(d, phi), mode 2 phi = max(-d, phi) where
+// d <= 3 dx; origin = (x_min, y_min, z_min) of the cell grid
+void fs_source_sphere(hipStream_t st, double* phi, const FsLayout& L, const int lo[3], const int hi[3],
+                      const double center[3], double radius, const FsGridConst& cell, const double origin[3],
+                      int mode);
 // b -= mean(b) with the per-plane row-order sum; scratch: nz+1 doubles.
 void fs_remove_mean(hipStream_t st, double* rhs, double* scratch, const FsLayout& L);
 void fs_gradient_update(hipStream_t st, int comp, int apply, double* vel, const double* p, const double* phi,

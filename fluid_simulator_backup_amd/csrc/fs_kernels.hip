@@ -742,6 +742,40 @@ void fs_assemble(hipStream_t st, const double* phi, const double* irho, const do
                        rhs, pin, L, inv_dxdx, cell.x.inv_d2, cell.y.inv_d2, cell.z.inv_d2);
 }
 
+// One spherical object of EULERIAN_FLUID_SOLVER_3D::SourcingFromObjectList
+// (EULERIAN_FLUID_SOLVER_3D.h:2592-2671) over the cells [lo, hi] of its box:
+// d = |CellCenter(i,j,k) - center| - radius with CellCenter's half-cell
+// offset (GRID_STRUCTURE_3D.h:269-272); where d <= 3 dx, phi = min(d, phi)
+// (mode 1) or max(-d, phi) (mode 2), as MIN/MAX write them (MACROS.h:17,24).
+__global__ void k_source_sphere(double* __restrict__ phi, FsLayout L, int i0, int j0, int k0, int ni, int nj,
+                                double cx, double cy, double cz, double radius, double x0, double y0, double z0,
+                                double dx, double dy, double dz, double lim, int mode) {
+    int a = blockIdx.x * BX + threadIdx.x, b = blockIdx.y * BY + threadIdx.y;
+    if (a >= ni || b >= nj) return;
+    int i = i0 + a, j = j0 + b, k = k0 + blockIdx.z;
+    double px = x0 + ((double)0.5 + (double)i) * dx - cx;
+    double py = y0 + ((double)0.5 + (double)j) * dy - cy;
+    double pz = z0 + ((double)0.5 + (double)k) * dz - cz;
+    double d = sqrt(px * px + py * py + pz * pz) - radius;
+    if (d <= lim) {
+        long c = L.idx(i, j, k);
+        double f = phi[c];
+        if (mode == 1) phi[c] = d > f ? f : d;
+        else phi[c] = -d > f ? -d : f;
+    }
+}
+
+void fs_source_sphere(hipStream_t st, double* phi, const FsLayout& L, const int lo[3], const int hi[3],
+                      const double center[3], double radius, const FsGridConst& cell, const double origin[3],
+                      int mode) {
+    int ni = hi[0] - lo[0] + 1, nj = hi[1] - lo[1] + 1, nk = hi[2] - lo[2] + 1;
+    if (ni <= 0 || nj <= 0 || nk <= 0) return;
+    dim3 g((ni + BX - 1) / BX, (nj + BY - 1) / BY, nk);
+    hipLaunchKernelGGL(k_source_sphere, g, dim3(BX, BY), 0, st, phi, L, lo[0], lo[1], lo[2], ni, nj, center[0],
+                       center[1], center[2], radius, origin[0], origin[1], origin[2], cell.x.d, cell.y.d, cell.z.d,
+                       cell.x.d * (double)3, mode);
+}
+
 // Neumann mean removal, POISSON_SOLVER_3D.h:225-246.  The sum runs in row
 // order inside each z-plane, then over planes in order (one slab per plane).
 // One workgroup per plane: rows are staged through LDS with coalesced loads

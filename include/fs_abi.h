@@ -187,6 +187,39 @@ int  fs_sourcing(fs_solver* s, double dt);           /* Sourcing -> VISCOSITY_3D
 int  fs_projection(fs_solver* s, double dt, fs_step_stats* stats); /* PROJECTION_3D::Solve, PROJECTION_3D.h:416-594 */
 int  fs_reinitialization(fs_solver* s, double dt);   /* Reinitialization, EULERIAN_FLUID_SOLVER_3D.h:2198-2225 */
 
+/* ---- level-set sources (objects of the scene) -------------------------------
+ * EULERIAN_FLUID_SOLVER_3D::SourcingFromObjectList
+ * (EULERIAN_FLUID_SOLVER_3D.h:2592-2671) for spherical objects
+ * (SIMULATION_WORLD.cpp:355-364, SPHERE.h): at the end of the Sourcing phase
+ * (:2076-2087), and whenever fs_apply_levelset_sources is called, each source
+ * whose frame window holds the current frame visits the cells of its box
+ * (the grid cells from the one holding center - half_width to the one holding
+ * center + half_width per axis, clamped to the grid,
+ * SIMULATION_OBJECT.h:149-160) and, where its signed distance
+ * d = |c - center| - radius at the point c = (x_min + (i + 1/2) dx, ...)
+ * (GRID_STRUCTURE_3D::CellCenter, GRID_STRUCTURE_3D.h:269-272) is at most
+ * 3 dx, sets phi = min(d, phi) (mode FS_SOURCE_ADD, FLUID_CONTROL_OPTIONS
+ * source_levelset) or phi = max(-d, phi) (FS_SOURCE_DELETE,
+ * delete_water_levelset).  Sources act in the order given.  The velocity
+ * sourcing of the reference is commented out (:2644-2657) and none is done. */
+#define FS_SOURCE_NONE    0
+#define FS_SOURCE_ADD     1
+#define FS_SOURCE_DELETE  2
+typedef struct fs_levelset_source {
+    double center[3];
+    double radius;
+    double half_width;      /* box half width: radius + 3 dx of the world grid */
+    int    mode;            /* FS_SOURCE_* */
+    int    start_frame, stop_frame;  /* inclusive window (source_start_frame, source_stop_frame) */
+} fs_levelset_source;
+/* Replaces the solver's object list (n = 0 clears it; at most 64). */
+int  fs_set_levelset_sources(fs_solver* s, int n, const fs_levelset_source* src);
+/* The frame number the sources compare with (the reference's borrowed
+ * frame counter, EULERIAN_FLUID_SOLVER_3D.h:154-162); 0 after fs_create. */
+int  fs_set_frame(fs_solver* s, int frame);
+/* SourcingFromObjectList once, outside a step (SIMULATION_WORLD.h:224-227). */
+int  fs_apply_levelset_sources(fs_solver* s);
+
 /* ---- surface tension ------------------------------------------------------
  * With surface_tension != 0 each projection computes the level set's
  * curvature (LEVELSET_3D::ComputeCurvatures, curvature_by_levelset,

@@ -77,6 +77,9 @@ struct fso_state {
     double *kappa, *jc;
     double kmax;
     int curv_fail;
+    /* the scene's spherical level-set sources and the frame counter */
+    fs_levelset_source src[64];
+    int nsrc, frame;
 };
 
 /* A CSR system with rows numbered plane by plane (per rows in each of
@@ -868,7 +871,70 @@ static int semi_implicit_viscosity(fso_state* s, double dt, int* iterations) {
  * the semi-implicit branch when semi_implicit_viscosity is set.  The fixed
  * schedule reads phi's ghosts as the constant extrapolation of the current
  * level set and zeroes each component's wall faces after its update. */
+/* EULERIAN_FLUID_SOLVER_3D::SourcingFromObjectList, :2592-2671, for
+ * spherical objects (SIMULATION_WORLD.cpp:355-364).  Each object's cells are
+ * those of its AABB grid (SIMULATION_OBJECT::BuildAABBGrid,
+ * SIMULATION_OBJECT.h:149-160: ClampedCell of the box corners,
+ * GRID_STRUCTURE_3D.h:310-319); its signed distance is taken at CellCenter,
+ * which adds half a cell (GRID_STRUCTURE_3D.h:269-272). */
+static void sourcing_from_object_list(fso_state* s) {
+    const ax_t* ax[3] = {&s->cell.x, &s->cell.y, &s->cell.z};
+    const double mn[3] = {s->gr.x_min, s->gr.y_min, s->gr.z_min};
+    const int res[3] = {s->nx, s->ny, s->nz};
+    for (int o = 0; o < s->nsrc; o++) {
+        const fs_levelset_source* b = &s->src[o];
+        if (b->mode == FS_SOURCE_NONE || !(b->start_frame <= s->frame && s->frame <= b->stop_frame)) continue;
+        int lo[3], hi[3];
+        for (int a = 0; a < 3; a++) {
+            double aabb_min = b->center[a] - b->half_width, aabb_max = b->center[a] + b->half_width;
+            double fl = floor((aabb_min - mn[a]) * ax[a]->inv), fh = floor((aabb_max - mn[a]) * ax[a]->inv);
+            lo[a] = fl < 0 ? 0 : fl > res[a] - 1 ? res[a] - 1 : (int)fl;
+            hi[a] = fh < 0 ? 0 : fh > res[a] - 1 ? res[a] - 1 : (int)fh;
+        }
+        for (int k = lo[2]; k <= hi[2]; k++)
+            for (int j = lo[1]; j <= hi[1]; j++)
+                for (int i = lo[0]; i <= hi[0]; i++) {
+                    double cx = mn[0] + ((double)0.5 + (double)i) * s->cell.x.d;
+                    double cy = mn[1] + ((double)0.5 + (double)j) * s->cell.y.d;
+                    double cz = mn[2] + ((double)0.5 + (double)k) * s->cell.z.d;
+                    double dx = cx - b->center[0], dy = cy - b->center[1], dz = cz - b->center[2];
+                    double phi_obj = sqrt(dx * dx + dy * dy + dz * dz) - b->radius; /* SPHERE.h:29-32 */
+                    if (phi_obj <= s->cell.x.d * (double)3) {
+                        double* f = &A3(s->phi, i, j, k);
+                        if (b->mode == FS_SOURCE_ADD) *f = phi_obj > *f ? *f : phi_obj;  /* MIN */
+                        else *f = -phi_obj > *f ? -phi_obj : *f;                         /* MAX */
+                    }
+                }
+    }
+}
+
+int fso_set_levelset_sources(fso_state* s, int n, const fs_levelset_source* src) {
+    if (n < 0 || n > 64 || (n > 0 && !src)) return FS_ERR_BAD_ARG;
+    for (int i = 0; i < n; i++) s->src[i] = src[i];
+    s->nsrc = n;
+    return FS_OK;
+}
+int fso_set_frame(fso_state* s, int frame) {
+    s->frame = frame;
+    return FS_OK;
+}
+int fso_apply_levelset_sources(fso_state* s) {
+    sourcing_from_object_list(s);
+    return FS_OK;
+}
+
+static int apply_viscosity(fso_state* s, double dt);
+
+/* EULERIAN_FLUID_SOLVER_3D::Sourcing, :2076-2100: viscosity, then the objects. */
 int fso_sourcing(fso_state* s, double dt) {
+    int rc = apply_viscosity(s, dt);
+    if (s->nsrc > 0) sourcing_from_object_list(s);
+    return rc;
+}
+
+/* VISCOSITY_3D::ApplyViscosity, VISCOSITY_3D.h:448-563 (and the semi-implicit
+ * systems, :853-989). */
+static int apply_viscosity(fso_state* s, double dt) {
     if (!s->P.use_delta_function_formulation) return FS_OK;
     if (is_fixed(s)) compute_normals(s);
     if (s->P.semi_implicit_viscosity) {

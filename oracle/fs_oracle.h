@@ -44,6 +44,9 @@ int  fso_advance_frame(fso_state* s, double dt, fs_step_stats* st);
 int  fso_step(fso_state* s, double dt, fs_step_stats* st);
 int  fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt);
 
+int  fso_set_levelset_sources(fso_state* s, int n, const fs_levelset_source* src);
+int  fso_set_frame(fso_state* s, int frame);
+int  fso_apply_levelset_sources(fso_state* s);
 int  fso_compute_curvatures(fso_state* s);
 int  fso_get_curvature(fso_state* s, double* kappa, double* abs_max_curvature);
 int  fso_levelset_advection(fso_state* s, double dt);

@@ -37,3 +37,11 @@ def hip_lib():
         pytest.skip("no GPU visible")
     import fluid_simulator_backup_amd as fsa
     return fsa.load_library()  # raises if the HIP build is missing: no fallback
+
+
+@pytest.fixture(scope="session")
+def host_lib():
+    """The HIP library for its host-only entry points (scripts, scene
+    parsing): loading it touches no GPU."""
+    import fluid_simulator_backup_amd as fsa
+    return fsa.load_library()

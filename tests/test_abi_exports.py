@@ -1,5 +1,5 @@
-"""The C-ABI library loads and exports every function include/fs_abi.h
-declares (no GPU work: loading only).  The CPU checker exports its own
+"""The C-ABI library loads and exports every function include/*.h declares
+(no GPU work: loading only).  The CPU checker exports its own
 header's functions too."""
 import ctypes as C
 import os
@@ -29,8 +29,11 @@ def test_hip_library_exports_every_declared_symbol():
     if not os.path.exists(path):
         pytest.fail("libfs_hip.so is not built (run __graft_entry__.build())")
     lib = C.CDLL(path)
-    missing = [n for n in declared("include/fs_abi.h", "fs_") if not hasattr(lib, n)]
-    assert not missing, missing
+    headers = sorted(f for f in os.listdir(os.path.join(ROOT, "include")) if f.endswith(".h"))
+    assert "fs_abi.h" in headers and "fs_world.h" in headers
+    for h in headers:
+        missing = [n for n in declared("include/" + h, "fs_") if not hasattr(lib, n)]
+        assert not missing, (h, missing)
 
 
 def test_oracle_exports_its_header(oracle_lib):

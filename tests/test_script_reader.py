@@ -1,0 +1,154 @@
+"""The .sim script reader (SCRIPT_READER semantics, SCRIPT_READER.cpp) and
+the scene set-up it feeds, through the host-only entry points of the C-ABI
+(no GPU work).
+
+The reference ships no tests for its reader.  Each expectation below is
+read off the reader's rules by hand for the fixture scripts under
+tests/scripts (written for these tests in the reference's format), so a
+slip in a rule (a comment form, the value order, a default, a path) fails
+one of them."""
+import math
+import os
+
+import numpy as np
+import pytest
+
+from fluid_simulator_backup_amd import _abi
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+SCRIPTS = os.path.join(HERE, "scripts")
+REFERENCE_SCRIPT = "/root/reference/3D_script.sim"
+
+
+@pytest.fixture(scope="module")
+def rules(host_lib):
+    s = _abi.Script(host_lib, os.path.join(SCRIPTS, "reader_rules.sim"))
+    yield s
+    s.close()
+
+
+def test_blocks_includes_and_stray_values(rules):
+    # top-level text outside blocks is ignored; an include is inserted once,
+    # a missing one dropped
+    assert rules.children("") == ["RULES", "INCLUDED", "SECOND"]
+    assert rules.int("INCLUDED", "z") == 9
+    assert rules.children("SECOND") == ["OBJ_A", "OBJ_B"]
+    assert rules.string("SECOND.OBJ_B", "shape") == "WATER"
+
+
+def test_comment_forms(rules):
+    # "// NAME {" removes the whole block
+    assert rules.children("RULES") == ["OUTER"]
+    # a line comment after a value
+    assert rules.int("RULES", "line") == 5
+    # a bare "//" at a line end takes the next line with it
+    assert rules.int("RULES", "keep_me") == 1
+    assert rules.int("RULES", "eaten", -1) == -1
+    # "/*" ends at the first '/' after it, not at "*/"
+    assert rules.int("RULES", "gone", -1) == -1
+    assert rules.int("RULES", "kept") == 2
+    assert rules.int("RULES", "after") == 6
+
+
+def test_typed_values_and_defaults(rules):
+    assert rules.int("RULES", "number") == 12
+    assert rules.float("RULES", "float") == -0.25
+    # numbers have no exponent: "1.5e-3" reads as 1.5
+    assert rules.float("RULES", "exponent") == 1.5
+    # first value of a name wins
+    assert rules.int("RULES", "dup") == 1
+    # numbers are collected before booleans: "typed" is the number 3
+    assert rules.int("RULES", "typed") == 3
+    assert rules.bool("RULES", "typed", True) is False
+    assert rules.bool("RULES", "flag") is True and rules.bool("RULES", "off", True) is False
+    assert rules.vector3("RULES", "vec") == (1.0, -2.5, 0.5)
+    assert rules.vector3("RULES", "notvec", (7.0, 7.0, 7.0)) == (0.0, 0.0, 0.0)
+    assert rules.int3("RULES", "ints") == (1, 2, 3)
+    assert rules.string("RULES", "name") == "log.txt"
+    assert rules.string("RULES", "path") == "a/b-c.d"
+    assert rules.string("RULES", "badstr", "none") == "none"  # ',' is outside the string pattern
+    # defaults for absent keys and blocks
+    assert rules.int("RULES", "absent", 42) == 42
+    assert rules.float("NOPE.NADA", "x", 2.5) == 2.5
+    assert rules.vector3("RULES", "absent", (0.0, -9.8, 0.0)) == (0.0, -9.8, 0.0)
+
+
+def test_nested_paths(rules):
+    assert rules.int("RULES.OUTER", "x") == 2
+    assert rules.int("RULES.OUTER.INNER", "x") == 1
+    assert rules.int("RULES.INNER", "x", -1) == -1
+
+
+def test_world_set_up_of_the_bubble_scene(host_lib):
+    info, src = _abi.parse_world(host_lib, os.path.join(SCRIPTS, "air_water_bubble.sim"))
+    P, g = info.params, info.grid
+    assert (g.nx, g.ny, g.nz) == (21, 31, 21)
+    assert (g.x_min, g.y_min, g.z_min, g.x_max, g.y_max, g.z_max) == (-1, -1, -1, 1, 2, 1)
+    assert info.cfl == 0.9 and info.frame_rate == 30 and info.num_substeps == 1 and info.last_frame == 100
+    assert P.surface_tension == 0.0728 and P.water_density == 1000 and P.air_density == 1.226
+    assert P.water_viscosity == 0.001137 and P.air_viscosity == 0.0000178
+    assert tuple(P.gravity) == (0, -9.8, 0)
+    assert P.air_bubble_rising == 1 and P.use_5th_weno_v == 0 and P.order_for_time_advancing == 1
+    assert P.tolerance == 1e-5 and P.max_iteration == 10000 and P.use_previous_pressure == 1
+    assert P.sussmanstyle_reinitialization == 1 and P.iteration_number_for_reinitialization == 25
+    assert P.scaling_number_for_reinitialization == 5 and P.use_delta_function_formulation == 1
+    assert P.levelset_epsilon == 1e-6 and P.velocity_epsilon == 1e-6
+    assert P.semi_implicit_viscosity == 0 and P.stop_rule == _abi.FS_STOP_ABS_PRECOND
+    # the commented-out SOURCE_SPHERE block adds no object
+    assert info.n_sources == 0 and src == []
+
+
+def test_world_set_up_of_the_source_scene(host_lib):
+    info, src = _abi.parse_world(host_lib, os.path.join(SCRIPTS, "air_water_sources.sim"))
+    g = info.grid
+    assert (g.nx, g.ny, g.nz) == (17, 19, 15)
+    assert info.cfl == 0 and info.dt == 0.002 and info.num_substeps == 2 and info.frame_rate == 24
+    assert info.n_sources == 2
+    wdx = 2.0 / 16
+    a, b = src
+    assert tuple(a.center) == (0.5, 0.75, -0.1) and a.radius == 0.2 and a.mode == _abi.FS_SOURCE_ADD
+    assert a.half_width == 0.2 + 3.0 * wdx and (a.start_frame, a.stop_frame) == (0, 1)
+    assert tuple(b.center) == (-0.3, -0.2, 0.25) and b.mode == _abi.FS_SOURCE_DELETE
+    assert (b.start_frame, b.stop_frame) == (-(2 ** 31 - 1), 2 ** 31 - 1)
+
+
+def test_unsupported_scenes_are_named(host_lib, tmp_path):
+    base = open(os.path.join(SCRIPTS, "air_water_bubble.sim")).read()
+    common = open(os.path.join(SCRIPTS, "air_water_common.sim")).read()
+    cases = {
+        "oil": base.replace("oil_water_simulation = false", "oil_water_simulation = true"),
+        "solver": base.replace("#INCLUDE <air_water_common.sim>",
+                               common.replace('"PCG"', '"CG"')),
+        "nowater": base.replace('shape_type = "WATER"', 'shape_type = "NONE"'),
+        "rigid": base.replace('object_type = "FLUID"\n\t\tshape_type = "WATER"',
+                              'object_type = "RIGID"\n\t\tshape_type = "WATER"'),
+    }
+    for name, text in cases.items():
+        assert text != base, name
+        p = tmp_path / (name + ".sim")
+        p.write_text(text)
+        (tmp_path / "air_water_common.sim").write_text(common)
+        with pytest.raises(_abi.FsError) as e:
+            _abi.parse_world(host_lib, str(p))
+        assert e.value.code == _abi.FS_ERR_UNSUPPORTED, name
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_SCRIPT), reason="reference scripts not present")
+def test_reference_script_parses(host_lib):
+    """The reference's own script, read where it lies: its air/water solver
+    block and grid come out as written, and the scene it selects (the
+    oil/water pipe) is reported as outside this build."""
+    s = _abi.Script(host_lib, REFERENCE_SCRIPT)
+    assert s.float("FLUID_SOLVER_UNIFORM_AIR_WATER", "surface_tension") == 0.0728
+    assert s.vector3("FLUID_SOLVER_UNIFORM_AIR_WATER", "gravity") == (0.0, -9.8, 0.0)
+    assert s.string("FLUID_SOLVER_UNIFORM_AIR_WATER.PROJECTION_WATER", "poisson_solver_type") == "PCG"
+    grid = "SIMULATION_WORLD.WORLD_DISCRETIZATION_AIR_WATER.GRID_STRUCTURE_3D_LARGE"
+    assert s.int3(grid, "base_grid_resolution") == (41, 61, 41)
+    assert s.vector3(grid, "base_grid_max") == (1.0, 2.0, 1.0)
+    assert s.bool("SIMULATION_WORLD", "oil_water_simulation") is True
+    # the commented-out SOURCE_SPHERE block is gone; WATER stays
+    assert s.children("OBJECT_LIST") == ["WATER"]
+    s.close()
+    with pytest.raises(_abi.FsError) as e:
+        _abi.parse_world(host_lib, REFERENCE_SCRIPT)
+    assert e.value.code == _abi.FS_ERR_UNSUPPORTED
