@@ -193,6 +193,10 @@ class EulerianFluidSolver3D {
         macz = cell;
         macz.z = fs_make_axis(G.z_min - (double)0.5 * cell.z.d, G.z_max + (double)0.5 * cell.z.d, G.nz + 1);
         mat = FsMaterial{P.air_bubble_rising, P.water_density, P.air_density, P.water_viscosity, P.air_viscosity};
+        if (P.precision == FS_FP32_REFINE && !fixed() && !cubic())
+            throw FsFail{FS_ERR_UNSUPPORTED,
+                         "FP32 refinement on unequal spacings needs the fixed schedule: the compatible schedule's "
+                         "matrix puts 1/dx^2 on the diagonal for every face, so constants are not in its null space"};
 
         double** f64[] = {&phi, &sghost, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
                           &bx, &by, &bz, &diag, &rhs};
@@ -350,6 +354,15 @@ class EulerianFluidSolver3D {
     }
 
     bool fixed() const { return P.schedule == FS_SCHEDULE_FIXED; }
+    // unequal spacings: the reference's diagonal (1/dx^2 for every face,
+    // POISSON_SOLVER_3D.h:1213) differs from the operator's flux form
+    bool cubic() const {
+        double a = (double)1 / (cell.x.d * cell.x.d), b = (double)1 / (cell.y.d * cell.y.d),
+               c = (double)1 / (cell.z.d * cell.z.d);
+        return a == b && a == c;
+    }
+    // the fixed schedule's diagonal: each face's beta over its own d^2
+    int axis_diag() const { return fixed() && !cubic() ? 1 : 0; }
     bool tension() const { return P.surface_tension != 0.0; }
 
     // copy of field src (component box comp) onto the step tape
@@ -560,7 +573,7 @@ class EulerianFluidSolver3D {
         if (!P.use_previous_pressure) fs_fill_value(st, p, L.total, 0.0);
         fs_fill_const(st, phi, phi, L, box(0), 2);   // ComputeNormals, LEVELSET_3D.h:931
         fs_fill_const(st, irho, irho, L, box(0), 1); // PROJECTION_3D.h:813
-        fs_assemble(st, phi, irho, div, jump(), bx, by, bz, diag, rhs, S64pin(), L, cell);
+        fs_assemble(st, phi, irho, div, jump(), bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
         fs_remove_mean(st, rhs, scratch, L);
         assembled = true;
     }
@@ -571,6 +584,7 @@ class EulerianFluidSolver3D {
         o.by = by;
         o.bz = bz;
         o.idx2 = (double)1 / (cell.x.d * cell.x.d);
+        o.axis_diag = fixed() ? 1 : 0;
         o.idy2 = (double)1 / (cell.y.d * cell.y.d);
         o.idz2 = (double)1 / (cell.z.d * cell.z.d);
         return o;
@@ -597,7 +611,7 @@ class EulerianFluidSolver3D {
         FsPcgOp o = op64();
         fs_residual64(st, L, o, rhs, p, (double*)w64.r, r32, w64, S64);
         read_state(S64);
-        double pin = o.idx2 * (hS->pin_b + hS->pin_d);
+        double pin = axis_diag() ? o.idy2 * hS->pin_b + o.idz2 * hS->pin_d : o.idx2 * (hS->pin_b + hS->pin_d);
         if (pin > 0) {
             fs_deflate_constant(st, L, p, (double*)w64.r, r32, S64, pin);
             read_state(S64);
@@ -805,7 +819,7 @@ class EulerianFluidSolver3D {
         fs_fill_const(st, phi, phi, L, box(0), 2);  // as AssemblePressureSystem
         fs_fill_const(st, irho, irho, L, box(0), 1);
         fs_fill_value(st, div, L.total, 0.0);
-        fs_assemble(st, phi, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell);
+        fs_assemble(st, phi, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
         assembled = true;
         // rhs = G^T lam (the faces the forward corrects)
         fs_fill_value(st, rhs, L.total, 0.0);
@@ -927,7 +941,7 @@ class EulerianFluidSolver3D {
             fs_density(st, A.phiw, rho, irho, L, light, heavy);
             fs_fill_const(st, irho, irho, L, box(0), 1);
             Zero(div);
-            fs_assemble(st, A.phiw, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell);
+            fs_assemble(st, A.phiw, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
             assembled = true;
             Zero(rhs);
             fs_adjoint_gradient_beta(st, 1, A.lu, bx, rhs, L, cell, dt);

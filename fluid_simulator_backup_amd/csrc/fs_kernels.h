@@ -55,10 +55,12 @@ void fs_divergence(hipStream_t st, const double* gu, const double* gv, const dou
                    const FsLayout& L, const FsGridConst& cell, double one_over_dt);
 // beta of every face between two interior cells (0 elsewhere), A(c,c) and
 // b = -div - F with the jump terms of jc (jc = NULL: b = -div).
-// pin[0..1]: the first cell's Neumann extras beta_b, beta_d.
+// pin[0..1]: the first cell's Neumann extras beta_b, beta_d.  axis_diag:
+// the diagonal sums each face's beta over its own d^2 instead of the
+// reference's 1/dx^2 for every face.
 void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, const double* jc,
                  double* bx, double* by, double* bz, double* diag, double* rhs, double* pin, const FsLayout& L,
-                 const FsGridConst& cell);
+                 const FsGridConst& cell, int axis_diag);
 // curvature of phi (its ghosts as left), jc = sigma * kappa when jc is given;
 // kstat[0] = bits of max |kappa| near the interface, kstat[1] += cells whose
 // gradient vanishes in both difference forms.  Neither slot is cleared here.

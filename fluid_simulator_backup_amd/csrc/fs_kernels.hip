@@ -694,7 +694,7 @@ __global__ void k_diag_rhs(const double* __restrict__ f, const double* __restric
                            const double* __restrict__ by, const double* __restrict__ bz, const double* __restrict__ div,
                            const double* __restrict__ jc, double* __restrict__ diag, double* __restrict__ rhs,
                            double* __restrict__ pin, FsLayout L, double inv_dxdx, double odx2, double ody2,
-                           double odz2) {
+                           double odz2, int axis) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
     long c = L.idx(i, j, k);
@@ -715,8 +715,25 @@ __global__ void k_diag_rhs(const double* __restrict__ f, const double* __restric
         pin[0] = pb;
         pin[1] = pd;
     }
-    if (coef == 0) coef = 1;
-    diag[c] = inv_dxdx * coef;
+    if (axis) {
+        // each face over its own spacing (the fixed schedule on non-cubic
+        // cells); pin extras on their own axes
+        double cx = 0, cy = 0, cz = 0;
+        cx += bx[c];
+        cx += bx[c + 1];
+        cy += by[c];
+        cy += by[c + sx];
+        cz += bz[c];
+        cz += bz[c + sy];
+        if (i == 0 && j == 0 && k == 0) {
+            cy += pin[0];
+            cz += pin[1];
+        }
+        diag[c] = cx + cy + cz == 0 ? odx2 : odx2 * cx + ody2 * cy + odz2 * cz;
+    } else {
+        if (coef == 0) coef = 1;
+        diag[c] = inv_dxdx * coef;
+    }
     if (!jc) {
         rhs[c] = -div[c];
         return;
@@ -734,12 +751,12 @@ __global__ void k_diag_rhs(const double* __restrict__ f, const double* __restric
 
 void fs_assemble(hipStream_t st, const double* phi, const double* irho, const double* div, const double* jc,
                  double* bx, double* by, double* bz, double* diag, double* rhs, double* pin, const FsLayout& L,
-                 const FsGridConst& cell) {
+                 const FsGridConst& cell, int axis_diag) {
     FsBox bb{L.nx + 1, L.ny + 1, L.nz + 1};
     hipLaunchKernelGGL(k_beta, grid_for(bb), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, L);
     double inv_dxdx = (double)1 / (cell.x.d * cell.x.d);
     hipLaunchKernelGGL(k_diag_rhs, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, irho, bx, by, bz, div, jc, diag,
-                       rhs, pin, L, inv_dxdx, cell.x.inv_d2, cell.y.inv_d2, cell.z.inv_d2);
+                       rhs, pin, L, inv_dxdx, cell.x.inv_d2, cell.y.inv_d2, cell.z.inv_d2, axis_diag);
 }
 
 // One spherical object of EULERIAN_FLUID_SOLVER_3D::SourcingFromObjectList

@@ -34,6 +34,10 @@ struct FsPcgOp {
     const void* by;
     const void* bz;
     double idx2, idy2, idz2;  // 1/dx^2, 1/dy^2, 1/dz^2 (POISSON_SOLVER_3D.h:1076)
+    // 0: the reference's diagonal, 1/dx^2 times the sum of the face betas
+    // (POISSON_SOLVER_3D.h:1213); 1: each face's beta over its own d^2 (the
+    // two agree on cubic cells)
+    int axis_diag;
     int kind;
     const double* a[6];
     const double* dg;

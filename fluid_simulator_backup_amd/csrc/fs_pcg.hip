@@ -239,6 +239,7 @@ struct OpT {
     const T* __restrict__ bz;
     T idx2, idy2, idz2;
     bool cubic;
+    bool axis;  // per-axis diagonal on non-cubic cells
 };
 
 template <typename T>
@@ -251,6 +252,7 @@ __device__ __forceinline__ OpT<T> make_op(const FsPcgOp& o) {
     r.idy2 = (T)o.idy2;
     r.idz2 = (T)o.idz2;
     r.cubic = o.idx2 == o.idy2 && o.idx2 == o.idz2;
+    r.axis = o.axis_diag != 0 && !r.cubic;
     return r;
 }
 
@@ -260,6 +262,28 @@ struct Nb {
     T xc, xl, xr, xb, xt, xd, xu;
     T bl, br, bb, bt, bd, bu;
 };
+
+// Diagonal with each face's beta over its own spacing squared: the betas of
+// x-, y- and z-faces summed per axis (the pin's extras on their own axes,
+// y for beta_b and z for beta_d), then idx2*cx + idy2*cy + idz2*cz; 1/dx^2
+// for a row without faces, as the reference's coef == 0 -> 1.
+__host__ __device__ __forceinline__ double axis_diag(double idx2, double idy2, double idz2, double bl, double br,
+                                                     double bb, double bt, double bd, double bu, bool pin,
+                                                     double pin_b, double pin_d) {
+    double cx = 0, cy = 0, cz = 0;
+    cx += bl;
+    cx += br;
+    cy += bb;
+    cy += bt;
+    cz += bd;
+    cz += bu;
+    if (pin) {
+        cy += pin_b;
+        cz += pin_d;
+    }
+    if (cx + cy + cz == 0) return idx2;
+    return idx2 * cx + idy2 * cy + idz2 * cz;
+}
 
 // FP64 row in the CSR order of the reference.
 __device__ __forceinline__ double row_val(const OpT<double>& op, const Nb<double>& n, bool pin, double pin_b,
@@ -283,7 +307,8 @@ __device__ __forceinline__ double row_val(const OpT<double>& op, const Nb<double
     v += -n.bt * op.idy2 * n.xt;
     v += -n.bd * op.idz2 * n.xd;
     v += -n.bu * op.idz2 * n.xu;
-    v += op.idx2 * coef * n.xc;
+    if (op.axis) v += axis_diag(op.idx2, op.idy2, op.idz2, n.bl, n.br, n.bb, n.bt, n.bd, n.bu, pin, pin_b, pin_d) * n.xc;
+    else v += op.idx2 * coef * n.xc;
     return v;
 }
 
@@ -301,6 +326,11 @@ __device__ __forceinline__ float row_val(const OpT<float>& op, const Nb<float>& 
         float s = op.idx2;
         v = n.bl * (s * (n.xc - n.xl)) + n.br * (s * (n.xc - n.xr)) + n.bb * (s * (n.xc - n.xb)) +
             n.bt * (s * (n.xc - n.xt)) + n.bd * (s * (n.xc - n.xd)) + n.bu * (s * (n.xc - n.xu));
+    } else if (op.axis) {
+        v = n.bl * (op.idx2 * (n.xc - n.xl)) + n.br * (op.idx2 * (n.xc - n.xr)) + n.bb * (op.idy2 * (n.xc - n.xb)) +
+            n.bt * (op.idy2 * (n.xc - n.xt)) + n.bd * (op.idz2 * (n.xc - n.xd)) + n.bu * (op.idz2 * (n.xc - n.xu));
+        if (pin) v += pin_b * (op.idy2 * n.xc) + pin_d * (op.idz2 * n.xc);
+        return v;
     } else {
         float cx = op.idx2 * n.xc;
         v = n.bl * (cx - op.idx2 * n.xl) + n.br * (cx - op.idx2 * n.xr) + n.bb * (cx - op.idy2 * n.xb) +

@@ -1082,6 +1082,8 @@ static void assemble(fso_state* s) {
     const double* vv = s->irho;
     const double* f = s->phi;
     const double* jc = jump_field(s);
+    /* the fixed schedule on unequal spacings corrects the diagonal */
+    const int axis = is_fixed_schedule(s) && !(inv_dxdx == inv_dydy && inv_dxdx == inv_dzdz);
     int nz_ix = 0;
     int i0 = 0, j0 = 0, k0 = 0;
     memset(s->fbx, 0, sizeof(double) * (size_t)s->ntot);
@@ -1119,9 +1121,31 @@ static void assemble(fso_state* s) {
                     if (cd == FS_BC_NEUM) coef += bd;
                     if (cu == FS_BC_NEUM) coef += bu;
                 }
-                if (coef == 0) coef = 1;
+                double dval;
+                if (axis) {
+                    /* fixed schedule, unequal spacings: each face's beta over
+                     * its own d^2, pin extras on their axes (y for beta_b,
+                     * z for beta_d and beta_u) */
+                    double cx = 0, cy = 0, cz = 0;
+                    if (cl > -1) cx += bl;
+                    if (cr > -1) cx += br;
+                    if (cb > -1) cy += bb;
+                    if (ct > -1) cy += bt;
+                    if (cd > -1) cz += bd;
+                    if (cu > -1) cz += bu;
+                    if (i == i0 && j == j0 && k == k0) {
+                        if (cb == FS_BC_NEUM) cy += bb;
+                        if (cd == FS_BC_NEUM) cz += bd;
+                        if (cu == FS_BC_NEUM) cz += bu;
+                    }
+                    dval = cx + cy + cz == 0 ? s->cell.x.inv_d2
+                                             : s->cell.x.inv_d2 * cx + s->cell.y.inv_d2 * cy + s->cell.z.inv_d2 * cz;
+                } else {
+                    if (coef == 0) coef = 1;
+                    dval = inv_dxdx * coef; /* :1213, 1/dx^2 for every face */
+                }
                 s->ci[nz_ix] = row;
-                s->val[nz_ix++] = inv_dxdx * coef;
+                s->val[nz_ix++] = dval;
                 if (jc) { /* :1215-1328 */
                     double jcc = A3(jc, i, j, k);
                     double F_L = jump_arm(fc, A3(f, i - 1, j, k), jcc, A3(jc, i - 1, j, k), s->cell.x.inv_d2, bl);

@@ -38,9 +38,13 @@ def test_pressure_system_bit_exact(hip_lib, oracle_lib, name, n):
     assert np.array_equal(gpu.apply_operator(x), cpu.apply_operator(x))
 
 
-def test_ragged_grid_pressure_system(hip_lib, oracle_lib):
+@pytest.mark.parametrize("sched", [_abi.FS_SCHEDULE_COMPAT, _abi.FS_SCHEDULE_FIXED])
+def test_ragged_grid_pressure_system(hip_lib, oracle_lib, sched):
+    """Unequal cell counts and spacings: the compatible schedule's matrix
+    (1/dx^2 diagonal for every face) and the fixed schedule's (each face over
+    its own d^2) are both reproduced bit for bit."""
     from fluid_simulator_backup_amd import scenes
-    phi, vel, P, _ = scenes.make("rising_drop", 16)
+    phi, vel, P, _ = scenes.make("rising_drop", 16, schedule=sched)
     g = _abi.unit_cube_grid(13, 9, 11)
     z, y, x = np.meshgrid((np.arange(11) + .5) / 11, (np.arange(9) + .5) / 9, (np.arange(13) + .5) / 13,
                           indexing="ij")
@@ -210,6 +214,41 @@ def test_fp32_refined_projection_matches_fp64(hip_lib, oracle_lib, name, n):
         if c == "p":
             a, b = a - a.mean(), b - b.mean()
         assert rel_err(a, b) < 1e-5, (c, rel_err(a, b))
+
+
+def test_fp32_refined_projection_unequal_spacing(hip_lib, oracle_lib):
+    """FP32+refine on unequal spacings (40 x 28 x 34 cells on the unit
+    cube, density ratio 1000) in the fixed schedule, whose flux-form matrix
+    keeps constants in its null space: it reaches 1e-10 relative residual and
+    matches the CPU checker's FP64 solve; the compatible schedule refuses the
+    combination."""
+    nx, ny, nz = 40, 28, 34
+    g = _abi.unit_cube_grid(nx, ny, nz)
+    k, j, i = np.meshgrid((np.arange(nz) + .5) / nz, (np.arange(ny) + .5) / ny, (np.arange(nx) + .5) / nx,
+                          indexing="ij")
+    phi = np.sqrt((i - .5) ** 2 + (j - .45) ** 2 + (k - .4) ** 2) - .2
+    rng = np.random.default_rng(3)
+    vel = (rng.standard_normal((nz, ny, nx + 1)), rng.standard_normal((nz, ny + 1, nx)),
+           rng.standard_normal((nz + 1, ny, nx)))
+    over = dict(stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-10, max_iteration=20000)
+    P32 = _abi.default_params(schedule=_abi.FS_SCHEDULE_FIXED, precision=_abi.FS_FP32_REFINE, **over)
+    P64 = _abi.default_params(schedule=_abi.FS_SCHEDULE_FIXED, **over)
+    gpu = _abi.Solver(hip_lib, "fs_", P32, g, 0)
+    cpu = _abi.Solver(oracle_lib, "fso_", P64, g)
+    for s in (gpu, cpu):
+        s.set_fields(phi, *vel)
+    st = gpu.projection(1e-2)
+    cpu.projection(1e-2)
+    assert st["residual_rel"] <= 1e-10, st
+    fg, fc = gpu.get_fields(), cpu.get_fields()
+    for c in ("p", "u", "v", "w"):
+        a, b = fg[c], fc[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        assert rel_err(a, b) < 1e-6, (c, rel_err(a, b))
+    with pytest.raises(_abi.FsError) as e:
+        _abi.Solver(hip_lib, "fs_", _abi.default_params(precision=_abi.FS_FP32_REFINE, **over), g, 0)
+    assert e.value.code == _abi.FS_ERR_UNSUPPORTED
 
 
 @pytest.mark.parametrize("name,n,frames", [("rising_drop", 24, 4), ("dam_break", 24, 4)])

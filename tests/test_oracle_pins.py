@@ -242,6 +242,51 @@ def test_fixed_schedule_projection_is_divergence_free(lib, light):
     assert np.abs(div).max() < 1e-9 * np.abs(before).max()
 
 
+@pytest.mark.parametrize("light", [1.0, 1e-3])
+def test_fixed_schedule_unequal_spacing_is_divergence_free(lib, light):
+    """On unequal spacings (13 x 9 x 11 cells on the unit cube) the fixed
+    schedule's diagonal takes each face's beta over its own d^2, so the
+    matrix is exactly the divergence of the gradient update: the projection
+    is divergence-free, and every row but the pinned one sums to zero (the
+    reference's 1/dx^2 diagonal breaks both)."""
+    nx, ny, nz = 13, 9, 11
+    g = _abi.unit_cube_grid(nx, ny, nz)
+    k, j, i = np.meshgrid((np.arange(nz) + .5) / nz, (np.arange(ny) + .5) / ny, (np.arange(nx) + .5) / nx,
+                          indexing="ij")
+    phi = np.sqrt((i - .5) ** 2 + (j - .5) ** 2 + (k - .4) ** 2) - .25
+    rng = np.random.default_rng(5)
+    u, v, w = rng.standard_normal((nz, ny, nx + 1)), rng.standard_normal((nz, ny + 1, nx)), \
+        rng.standard_normal((nz + 1, ny, nx))
+    u[:, :, [0, -1]] = 0
+    v[:, [0, -1], :] = 0
+    w[[0, -1], :, :] = 0
+    sums = {}
+    for sched in (_abi.FS_SCHEDULE_FIXED, _abi.FS_SCHEDULE_COMPAT):
+        P = _abi.default_params(air_density=light, water_density=1.0, stop_rule=1, rel_tolerance=1e-13,
+                                schedule=sched)
+        s = _abi.Solver(lib, "fso_", P, g)
+        s.set_fields(phi, u, v, w)
+        s.assemble(1e-2)
+        n, nnz = C.c_int(), C.c_int()
+        lib.fso_csr_info(s.h, C.byref(n), C.byref(nnz))
+        rp = np.zeros(n.value + 1, np.int32)
+        ci = np.zeros(nnz.value, np.int32)
+        val = np.zeros(nnz.value)
+        lib.fso_csr_copy(s.h, rp.ctypes.data_as(C.POINTER(C.c_int)), ci.ctypes.data_as(C.POINTER(C.c_int)),
+                         val.ctypes.data_as(PD), None)
+        rows = np.add.reduceat(val, rp[:-1])
+        diag = np.array([val[rp[r]:rp[r + 1]][ci[rp[r]:rp[r + 1]] == r][0] for r in range(n.value)])
+        sums[sched] = np.abs(rows[1:] / diag[1:]).max()
+        if sched == _abi.FS_SCHEDULE_FIXED:
+            s.projection(1e-2)
+            f = s.get_fields()
+            div = np.diff(f["u"], axis=2) * nx + np.diff(f["v"], axis=1) * ny + np.diff(f["w"], axis=0) * nz
+            before = np.diff(u, axis=2) * nx + np.diff(v, axis=1) * ny + np.diff(w, axis=0) * nz
+            assert np.abs(div).max() < 1e-9 * np.abs(before).max()
+    assert sums[_abi.FS_SCHEDULE_FIXED] < 1e-13
+    assert sums[_abi.FS_SCHEDULE_COMPAT] > 0.1
+
+
 def test_viscosity_shear_wave_is_discrete_diffusion(lib):
     n = 16
     mu, rho = 0.02, 2.0
