@@ -375,11 +375,17 @@ def main():
         res["fp32_refine_vs_fp64_error"] = err
 
     # roofline: algorithmic bytes per launch / measured duration (two kernels
-    # per iteration)
-    # algorithmic words per cell: p-update fused with SpMV + p.q reads s,
-    # p_old, beta_x, beta_y, beta_z and writes p, q; the update reads x, p, r,
-    # q, D and writes x, r, s
-    words = {"pspmv_dot": 7, "update": 8}
+    # per iteration).  Algorithmic words per cell of one Jacobi-PCG iteration
+    # (the workload's fixed accounting): SpMV 5 (read p and the three face
+    # coefficients, write q), update 7 (read x, p, r, q, D^-1, write x, r),
+    # direction 4 (read r, D^-1, p, write p): 16.  k_pspmv_dot does the SpMV
+    # and the direction update (9), k_update the update (7).
+    words = {"pspmv_dot": 9, "update": 7}
+    # words each kernel actually has to move: s, p_old, the level set (or the
+    # three coefficient arrays below 128^3 or with FS_BETA_ARRAYS), p, q; and
+    # x, p, r, q, D, x, r, s
+    phi_coef = a.n ** 3 >= 128 ** 3 and not os.environ.get("FS_BETA_ARRAYS")
+    moved = {"pspmv_dot": 5 if phi_coef else 7, "update": 8}
     names = ["pspmv_dot", "update"]
     kname = {"pspmv_dot": "k_pspmv_dot", "update": "k_update"}
     k32 = k32[:2]
@@ -387,19 +393,25 @@ def main():
     name = names[dom]
     bytes_launch = words[name] * cells * 4
     achieved = bytes_launch / (k32[dom] * 1e-3) / 1e9
-    gbs = lambda ms, i, wb: round(words[names[i]] * cells * wb / (ms * 1e-3) / 1e9, 1)
+    gbs = lambda ms, i, wb, w=words: round(w[names[i]] * cells * wb / (ms * 1e-3) / 1e9, 1)
     roof = {"bound": "hbm", "kernel": f"{kname[name]}<float>", "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
             "traffic": measured_traffic(a.scene, a.n, kname[name]),
             "bytes_per_launch": bytes_launch, "avg_launch_ms": round(float(k32[dom]), 5),
             "timing": f"HIP events on the solver stream around one iteration per 16 of the timed frames "
                       f"({int(n32[dom])} samples)",
+            "algorithmic_words_per_cell": words, "moved_words_per_cell": moved,
+            "moved_gbs": {nm: gbs(k32[i], i, 4, moved) for i, nm in enumerate(names)},
+            "iteration_algorithmic_gbs": round(16 * cells * 4 / (float(sum(k32)) * 1e-3) / 1e9, 1),
             "all_kernels_gbs": {nm: gbs(k32[i], i, 4) for i, nm in enumerate(names)},
             "all_kernels_ms": {nm: round(float(k32[i]), 5) for i, nm in enumerate(names)},
             "back_to_back_repeat_gbs": {nm: gbs(k32_rep[i], i, 4) for i, nm in enumerate(names)}}
     if "fp64" in res:
         k64 = res["fp64"]["kernel_ms"]
         res["fp64"]["kernel_gbs"] = {nm: round(words[nm] * cells * 8 / (k64[nm] * 1e-3) / 1e9, 1) for nm in names}
+        res["fp64"]["moved_gbs"] = {nm: round(moved[nm] * cells * 8 / (k64[nm] * 1e-3) / 1e9, 1) for nm in names}
+        res["fp64"]["iteration_algorithmic_gbs"] = round(
+            16 * cells * 8 / ((k64["pspmv_dot"] + k64["update"]) * 1e-3) / 1e9, 1)
 
     iters32 = float(np.mean([s["pcg_iterations"] for s in st32]))
     cpu = None

@@ -76,6 +76,10 @@ class EulerianFluidSolver3D {
     // PCG workspaces
     FsPcgWork w64{}, w32{};
     float *bx32 = nullptr, *by32 = nullptr, *bz32 = nullptr, *diag32 = nullptr, *e32 = nullptr, *r32 = nullptr;
+    float* phi32 = nullptr;  // the level set of the last assembly, for the FP32 SpMV's coefficients
+    // the projection's operator forms its face coefficients from phi (set by
+    // the forward assembly, whose phi and densities define them)
+    bool op_phi = false;
     FsPcgState *S64 = nullptr, *S32 = nullptr, *hS = nullptr;  // hS: pinned host mirror
     double* scratch = nullptr;                                   // mean removal partials (nz+1) + pin (2)
     unsigned long long* vmax = nullptr;                          // 4 words
@@ -218,6 +222,7 @@ class EulerianFluidSolver3D {
             by32 = ffield();
             bz32 = ffield();
             diag32 = ffield();
+            phi32 = ffield();
             e32 = ffield();
             r32 = ffield();
             w32.r = ffield();
@@ -263,7 +268,7 @@ class EulerianFluidSolver3D {
                          va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc};
         for (double* f : f64)
             if (f) (void)hipFree(f);
-        void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32,
+        void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32, phi32,
                       S64, S32, vmax, kstat, wv.r, wv.s, wv.p, wv.q, w64.p2, w32.p2, wv.p2};
         for (void* f : fv)
             if (f) (void)hipFree(f);
@@ -574,6 +579,7 @@ class EulerianFluidSolver3D {
         fs_fill_const(st, phi, phi, L, box(0), 2);   // ComputeNormals, LEVELSET_3D.h:931
         fs_fill_const(st, irho, irho, L, box(0), 1); // PROJECTION_3D.h:813
         fs_assemble(st, phi, irho, div, jump(), bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
+        op_phi = !std::getenv("FS_BETA_ARRAYS");
         fs_remove_mean(st, rhs, scratch, L);
         assembled = true;
     }
@@ -585,6 +591,14 @@ class EulerianFluidSolver3D {
         o.bz = bz;
         o.idx2 = (double)1 / (cell.x.d * cell.x.d);
         o.axis_diag = fixed() ? 1 : 0;
+        if (op_phi) {
+            // 1/rho as k_density forms it: light where phi <= 0
+            double light = P.air_bubble_rising ? P.air_density : P.water_density;
+            double heavy = P.air_bubble_rising ? P.water_density : P.air_density;
+            o.phi = phi;
+            o.v_neg = 1 / light;
+            o.v_pos = 1 / heavy;
+        }
         o.idy2 = (double)1 / (cell.y.d * cell.y.d);
         o.idz2 = (double)1 / (cell.z.d * cell.z.d);
         return o;
@@ -660,6 +674,7 @@ class EulerianFluidSolver3D {
 
     FsPcgOp op32() const {
         FsPcgOp o = op64();
+        if (op_phi) o.phi = phi32;
         o.bx = bx32;
         o.by = by32;
         o.bz = bz32;
@@ -748,6 +763,7 @@ class EulerianFluidSolver3D {
         // b - A x, the correction update and all dot products stay FP64.
         (void)o;
         fs_make_op32(st, L, bx, by, bz, diag, bx32, by32, bz32, diag32);
+        if (op_phi) fs_to_float(st, phi, phi32, L.total);
         double rr = 0, bb = 0;
         int updates = 0;
         stats.pcg_iterations += run_pcg32_reliable(rr, bb, updates);
@@ -820,6 +836,7 @@ class EulerianFluidSolver3D {
         fs_fill_const(st, irho, irho, L, box(0), 1);
         fs_fill_value(st, div, L.total, 0.0);
         fs_assemble(st, phi, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
+        op_phi = false;
         assembled = true;
         // rhs = G^T lam (the faces the forward corrects)
         fs_fill_value(st, rhs, L.total, 0.0);
@@ -942,6 +959,7 @@ class EulerianFluidSolver3D {
             fs_fill_const(st, irho, irho, L, box(0), 1);
             Zero(div);
             fs_assemble(st, A.phiw, irho, div, nullptr, bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
+            op_phi = false;
             assembled = true;
             Zero(rhs);
             fs_adjoint_gradient_beta(st, 1, A.lu, bx, rhs, L, cell, dt);

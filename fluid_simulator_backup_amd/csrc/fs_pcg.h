@@ -38,6 +38,13 @@ struct FsPcgOp {
     // (POISSON_SOLVER_3D.h:1213); 1: each face's beta over its own d^2 (the
     // two agree on cubic cells)
     int axis_diag;
+    // face coefficients computed in the fused SpMV from the level set phi
+    // (cell field, the operator's precision, ghosts as at assembly) instead
+    // of read from bx/by/bz: beta_f = gfm(v_c, v_n, phi_n, phi_c) with
+    // v = v_neg where phi <= 0, else v_pos (1/rho), 0 on wall faces -- the
+    // values fs_assemble stores, bit for bit in FP64
+    const void* phi;
+    double v_neg, v_pos;
     int kind;
     const double* a[6];
     const double* dg;
@@ -107,6 +114,8 @@ void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r3
 // x += (double) e
 void fs_add_correction(hipStream_t st, const FsLayout& L, double* x, const float* e);
 // FP32 copies of the face coefficients and the FP32 Jacobi diagonal.
+// dst[i] = (float)src[i] for i < n (a whole padded field)
+void fs_to_float(hipStream_t st, const double* src, float* dst, long n);
 void fs_make_op32(hipStream_t st, const FsLayout& L, const double* bx, const double* by, const double* bz,
                   const double* diag, float* bx32, float* by32, float* bz32, float* diag32);
 // One launch of iteration kernel `which` (0 p-update + SpMV + dot, 1

@@ -33,6 +33,7 @@
 
 #include <cstring>
 
+#include "fs_device.h"
 #include "fs_pcg.h"
 
 #define TBX 64
@@ -540,6 +541,163 @@ __device__ __forceinline__ void march_apply_lds(const FsLayout& L, const OpT<T>&
     }
 }
 
+// The GFM face coefficient of fs_assemble (gfm_beta, k_beta) with v = 1/rho
+// from the sign of phi (k_density).  FP64: the same formula, so the same
+// bits.  FP32 (the inner operator of the refinement, which only has to
+// approximate A): a face within one phase takes v itself, which the formula
+// equals up to rounding, and an interface face divides by a hardware
+// reciprocal (about 1 ulp).  Both cells of a face form it from the same
+// operands in the same order, so the operator stays symmetric.
+__device__ __forceinline__ double gfm_t(double vc, double vn, double fn, double fc) { return gfm_beta(vc, vn, fn, fc); }
+__device__ __forceinline__ float gfm_t(float vc, float vn, float fn, float fc) {
+    if (vc == vn) return vc;
+    return vc * vn * (fabsf(fn) + fabsf(fc)) * __builtin_amdgcn_rcpf(vc * fabsf(fn) + vn * fabsf(fc));
+}
+
+// march_apply_lds with the face coefficients formed from the level set:
+// the operand and phi are staged together (same rows, same halo duties),
+// each cell forms its left, bottom, top and up faces (the right face of
+// all but the last of its V cells is the next cell's left face, the down
+// face is the previous plane's up face), wall faces are 0.  Three words per
+// cell fewer than reading bx, by, bz.
+template <typename T, int V>
+struct StageLdsPhi {
+    T a[2][TBY + 2][LDS_ROW(V)];
+    T f[2][TBY + 2][LDS_ROW(V)];
+};
+template <typename T, int V, typename LD, typename F>
+__device__ __forceinline__ void march_apply_lds_phi(const FsLayout& L, const OpT<T>& op, const T* __restrict__ phi,
+                                                    T vneg, T vpos, const LD& x, int tile_x0, int j0, int k0, int zc,
+                                                    T pin_b, T pin_d, StageLdsPhi<T, V>& sh, F&& f) {
+    const int tx = threadIdx.x, ty = threadIdx.y;
+    const int i0 = tile_x0 + tx * V, j = j0 + ty;
+    const bool inside = i0 < L.nx && j < L.ny;
+    const int k1 = min(k0 + zc, L.nz);
+    const int hrow = ty == 0 ? j0 - 1 : j0 + TBY;
+    const bool hload = ty < 2 && i0 < L.nx;
+    const int xr = tile_x0 + TBX * V;
+    const int xrow = tx % (TBY + 2), xside = tx / (TBY + 2);
+    const bool xload = ty == 2 && tx < 2 * (TBY + 2) && (xside == 0 || xr <= L.nx);
+    const int xcol = xside ? 4 + TBX * V : 3;
+    const long xoff = L.idx(xside ? xr : tile_x0 - 1, j0 - 1 + xrow, 0) - L.idx(0, 0, 0);
+    long c = L.idx(i0, j, k0);
+    const long hc = L.idx(i0, hrow, k0);
+    auto vof = [&](T fv) { return fv <= (T)0 ? vneg : vpos; };
+    T xm[V], xc[V], fc[V], bzd[V];
+    auto stage = [&](int b, const T (&ctr)[V], const T (&hal)[V], T xh, const T (&fctr)[V], const T (&fhal)[V],
+                     T fxh) {
+#pragma unroll
+        for (int v = 0; v < V; v++) {
+            sh.a[b][ty + 1][4 + tx * V + v] = inside ? ctr[v] : (T)0;
+            sh.f[b][ty + 1][4 + tx * V + v] = inside ? fctr[v] : (T)0;
+        }
+        if (hload) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                sh.a[b][ty == 0 ? 0 : TBY + 1][4 + tx * V + v] = hal[v];
+                sh.f[b][ty == 0 ? 0 : TBY + 1][4 + tx * V + v] = fhal[v];
+            }
+        }
+        if (xload) {
+            sh.a[b][xrow][xcol] = xh;
+            sh.f[b][xrow][xcol] = fxh;
+        }
+    };
+    T hal[V] = {}, fhal[V] = {}, xh = 0, fxh = 0;
+    __syncthreads();
+    if (inside) {
+        T fm[V];
+        x.vec(c - L.sy, xm);
+        x.vec(c, xc);
+        vld<T, V>(phi + c, fc);
+        vld<T, V>(phi + c - L.sy, fm);
+#pragma unroll
+        for (int v = 0; v < V; v++) bzd[v] = k0 > 0 ? gfm_t(vof(fc[v]), vof(fm[v]), fm[v], fc[v]) : (T)0;
+    } else {
+#pragma unroll
+        for (int v = 0; v < V; v++) xm[v] = xc[v] = fc[v] = bzd[v] = 0;
+    }
+    if (hload) {
+        x.vec(hc, hal);
+        vld<T, V>(phi + hc, fhal);
+    }
+    if (xload) {
+        xh = x.at(xoff + L.idx(0, 0, k0));
+        fxh = phi[xoff + L.idx(0, 0, k0)];
+    }
+    stage(0, xc, hal, xh, fc, fhal, fxh);
+    for (int k = k0; k < k1; k++, c += L.sy) {
+        const int b = (k - k0) & 1;
+        const long kk = (long)(k - k0 + 1) * L.sy;
+        T xp[V], fp[V], hn[V], fhn[V], xhn = 0, fxhn = 0;
+#pragma unroll
+        for (int v = 0; v < V; v++) xp[v] = fp[v] = hn[v] = fhn[v] = 0;
+        const bool more = k + 1 < k1;
+        if (inside) {
+            x.vec(c + L.sy, xp);
+            vld<T, V>(phi + c + L.sy, fp);
+        }
+        if (more && hload) {
+            x.vec(hc + kk, hn);
+            vld<T, V>(phi + hc + kk, fhn);
+        }
+        if (more && xload) {
+            xhn = x.at(xoff + L.idx(0, 0, k + 1));
+            fxhn = phi[xoff + L.idx(0, 0, k + 1)];
+        }
+        __syncthreads();
+        if (inside) {
+            T xb[V], xt[V], fb[V], ft[V], bl[V], bzu[V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                xb[v] = sh.a[b][ty][4 + tx * V + v];
+                xt[v] = sh.a[b][ty + 2][4 + tx * V + v];
+                fb[v] = sh.f[b][ty][4 + tx * V + v];
+                ft[v] = sh.f[b][ty + 2][4 + tx * V + v];
+            }
+            const T xleft = sh.a[b][ty + 1][3 + tx * V], xright = sh.a[b][ty + 1][4 + tx * V + V];
+            const T fleft = sh.f[b][ty + 1][3 + tx * V], fright = sh.f[b][ty + 1][4 + tx * V + V];
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                const T fl = v == 0 ? fleft : fc[v - 1];
+                bl[v] = i0 + v > 0 ? gfm_t(vof(fc[v]), vof(fl), fl, fc[v]) : (T)0;
+                bzu[v] = k + 1 < L.nz ? gfm_t(vof(fp[v]), vof(fc[v]), fc[v], fp[v]) : (T)0;
+            }
+            const T bxr = i0 + V < L.nx ? gfm_t(vof(fright), vof(fc[V - 1]), fc[V - 1], fright) : (T)0;
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                Nb<T> n;
+                n.xc = xc[v];
+                n.xl = v == 0 ? xleft : xc[v - 1];
+                n.xr = v == V - 1 ? xright : xc[v + 1];
+                n.xb = xb[v];
+                n.xt = xt[v];
+                n.xd = xm[v];
+                n.xu = xp[v];
+                n.bl = bl[v];
+                n.br = v == V - 1 ? bxr : bl[v + 1];
+                n.bb = j > 0 ? gfm_t(vof(fc[v]), vof(fb[v]), fb[v], fc[v]) : (T)0;
+                n.bt = j + 1 < L.ny ? gfm_t(vof(ft[v]), vof(fc[v]), fc[v], ft[v]) : (T)0;
+                n.bd = bzd[v];
+                n.bu = bzu[v];
+                bool pin = (v == 0) && ((i0 | j | k) == 0);
+                f(c, v, k, xc[v], row_val(op, n, pin, pin_b, pin_d));
+            }
+#pragma unroll
+            for (int v = 0; v < V; v++) bzd[v] = bzu[v];
+        }
+        if (more) stage(b ^ 1, xp, hn, xhn, fp, fhn, fxhn);
+        if (inside) {
+#pragma unroll
+            for (int v = 0; v < V; v++) {
+                xm[v] = xc[v];
+                xc[v] = xp[v];
+                fc[v] = fp[v];
+            }
+        }
+    }
+}
+
 // ---- begin: r = b - A x, s = r/D, p = s; partials of r.s, r.r, b.b -------
 template <typename T, int V>
 __global__ void __launch_bounds__(NT) k_begin(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ b,
@@ -619,7 +777,7 @@ __device__ __forceinline__ bool loop_tail(const double* part, int nb, FsPcgState
 // touches (p(n-1) from pold, written to pnew: the two directions live in
 // separate buffers so no workgroup reads a value another has replaced),
 // q = A p(n) and the partial p.q.
-template <typename T, int V, bool LDS>
+template <typename T, int V, bool LDS, bool PHI = false>
 __global__ void __launch_bounds__(NT) k_pspmv_dot(FsLayout L, Tiling tl, FsPcgOp o, const T* __restrict__ s,
                                                   const T* __restrict__ pold, T* __restrict__ pnew, T* __restrict__ q,
                                                   double* part, FsPcgState* S, int n, int nb_rs) {
@@ -635,7 +793,32 @@ __global__ void __launch_bounds__(NT) k_pspmv_dot(FsLayout L, Tiling tl, FsPcgOp
     OpT<T> op = make_op<T>(o);
     double acc = 0;
     T pin_b = (T)S->pin_b, pin_d = (T)S->pin_d;
-    if constexpr (LDS) {
+    if constexpr (PHI) {
+    __shared__ StageLdsPhi<T, V> sh;
+    const T* phi = (const T*)o.phi;
+    const T vneg = (T)o.v_neg, vpos = (T)o.v_pos;
+    for (TileIter it(tl); it.valid(); it.next()) {
+        int i0, j, k0;
+        tile_origin<V>(tl, it.t, i0, j, k0);
+        const int tile_x0 = i0 - threadIdx.x * V, j0 = j - threadIdx.y;
+        T qv[V], pv[V];
+        auto body = [&](long c, int v, int, T pc, T ap) {
+            qv[v] = ap;
+            pv[v] = pc;
+            acc += (double)pc * (double)ap;
+            if (v == V - 1) {
+                vst_s<NTB(8), T, V>(q + c, qv);
+                if (n > 0) vst_s<NTB(9), T, V>(pnew + c, pv);
+            }
+        };
+        if (n > 0)
+            march_apply_lds_phi<T, V>(L, op, phi, vneg, vpos, LdDir<T, V>{pold, s, beta}, tile_x0, j0, k0, tl.zc,
+                                      pin_b, pin_d, sh, body);
+        else
+            march_apply_lds_phi<T, V>(L, op, phi, vneg, vpos, LdPlain<T, V>{pold}, tile_x0, j0, k0, tl.zc, pin_b,
+                                      pin_d, sh, body);
+    }
+    } else if constexpr (LDS) {
     __shared__ StageLds<T, V> sh;
     for (TileIter it(tl); it.valid(); it.next()) {
         int i0, j, k0;
@@ -867,6 +1050,9 @@ static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, con
     if (which == 0 && op.kind == FS_OP_ROWS)
         hipExtLaunchKernelGGL((k_rows_pspmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tl, op,
                               (const double*)w.s, (const double*)po, (double*)pn, (double*)w.q, w.partials, S, n);
+    else if (which == 0 && lds && op.phi)
+        hipExtLaunchKernelGGL((k_pspmv_dot<T, VK, true, true>), dim3(tk.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tk,
+                              op, (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n, tl.nb);
     else if (which == 0 && lds)
         hipExtLaunchKernelGGL((k_pspmv_dot<T, VK, true>), dim3(tk.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tk, op,
                               (const T*)w.s, po, pn, (T*)w.q, w.partials, S, n, tl.nb);
@@ -1149,6 +1335,16 @@ __global__ void k_make_op32(FsLayout L, const double* bx, const double* by, cons
     by32[c] = (float)by[c];
     bz32[c] = (float)bz[c];
     d32[c] = (float)diag[c];
+}
+
+__global__ void k_to_float(const double* __restrict__ src, float* __restrict__ dst, long n) {
+    long i = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) dst[i] = (float)src[i];
+}
+
+void fs_to_float(hipStream_t st, const double* src, float* dst, long n) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_to_float, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, src, dst, n);
 }
 
 void fs_make_op32(hipStream_t st, const FsLayout& L, const double* bx, const double* by, const double* bz,

@@ -251,6 +251,43 @@ def test_fp32_refined_projection_unequal_spacing(hip_lib, oracle_lib):
     assert e.value.code == _abi.FS_ERR_UNSUPPORTED
 
 
+@pytest.mark.parametrize("sched", [_abi.FS_SCHEDULE_COMPAT, _abi.FS_SCHEDULE_FIXED])
+def test_spmv_coefficients_from_phi(hip_lib, monkeypatch, sched):
+    """From 128^3 on, the fused SpMV forms the face coefficients from the
+    level set instead of reading the stored arrays.  FP64: the solve is the
+    same bit for bit (same formula, same operands).  FP32+refine: the inner
+    operator rounds differently but reaches 1e-8 and matches the FP64 fields
+    to the solve's accuracy.  (FS_BETA_ARRAYS selects the stored arrays.)"""
+    n = 128
+    over = dict(stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8, schedule=sched)
+    out = {}
+    for prec in (_abi.FS_FP64, _abi.FS_FP32_REFINE):
+        for arrays in (True, False):
+            if arrays:
+                monkeypatch.setenv("FS_BETA_ARRAYS", "1")
+            else:
+                monkeypatch.delenv("FS_BETA_ARRAYS", raising=False)
+            from fluid_simulator_backup_amd import scenes
+            from helpers import smooth_velocity
+            phi, _, P, g = scenes.make("rising_drop", n, precision=prec, **over)
+            gpu = _abi.Solver(hip_lib, "fs_", P, g, 0)
+            gpu.set_fields(phi, *smooth_velocity(n))
+            st = gpu.projection(2e-3)
+            out[prec, arrays] = (st, gpu.get_fields())
+            gpu.close()
+    (s_a, f_a), (s_p, f_p) = out[_abi.FS_FP64, True], out[_abi.FS_FP64, False]
+    assert s_a["pcg_iterations"] == s_p["pcg_iterations"]
+    for c in ("p", "u", "v", "w"):
+        assert np.array_equal(f_a[c], f_p[c]), c
+    s32, f32 = out[_abi.FS_FP32_REFINE, False]
+    assert s32["residual_rel"] <= 1e-8, s32
+    for c in ("p", "w"):
+        a, b = f32[c], f_a[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        assert rel_err(a, b) < 1e-5, (c, rel_err(a, b))
+
+
 @pytest.mark.parametrize("name,n,frames", [("rising_drop", 24, 4), ("dam_break", 24, 4)])
 def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
     """The reference's absolute stop rule at its script tolerance (1e-5):
