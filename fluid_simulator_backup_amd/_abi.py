@@ -19,6 +19,7 @@ FS_ERR_UNSUPPORTED = -6
 
 FS_FP64 = 0
 FS_FP32_REFINE = 1
+FS_FP32_MG = 2
 FS_STOP_ABS_PRECOND = 0
 FS_STOP_REL_RESIDUAL = 1
 FS_SCHEDULE_COMPAT = 0

@@ -23,6 +23,7 @@
 #include "fs_adjoint.h"
 #include "fs_kernels.h"
 #include "fs_layout.h"
+#include "fs_mg.h"
 #include "fs_pcg.h"
 
 namespace {
@@ -77,6 +78,9 @@ class EulerianFluidSolver3D {
     FsPcgWork w64{}, w32{};
     float *bx32 = nullptr, *by32 = nullptr, *bz32 = nullptr, *diag32 = nullptr, *e32 = nullptr, *r32 = nullptr;
     float* phi32 = nullptr;  // the level set of the last assembly, for the FP32 SpMV's coefficients
+    // FS_FP32_MG: the multigrid hierarchy, the CG's partial sums and totals
+    FsMg mg;
+    double *mg_part = nullptr, *mg_tot = nullptr, *hmg_tot = nullptr;
     // the projection's operator forms its face coefficients from phi (set by
     // the forward assembly, whose phi and densities define them)
     bool op_phi = false;
@@ -172,7 +176,8 @@ class EulerianFluidSolver3D {
         if (!std::isfinite(P.surface_tension)) throw FsFail{FS_ERR_BAD_ARG, "surface_tension must be finite"};
         if (P.order_for_time_advancing < 1 || P.order_for_time_advancing > 3)
             throw FsFail{FS_ERR_BAD_ARG, "order_for_time_advancing must be 1, 2 or 3"};
-        if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE) throw FsFail{FS_ERR_BAD_ARG, "precision"};
+        if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE && P.precision != FS_FP32_MG)
+            throw FsFail{FS_ERR_BAD_ARG, "precision"};
         if (P.scaling_number_for_reinitialization <= 0) throw FsFail{FS_ERR_BAD_ARG, "scaling_number"};
         debug = std::getenv("FS_DEBUG") != nullptr;
         // tuning override of the reliable-update trigger (residual^2 drop, default 0.01)
@@ -197,7 +202,7 @@ class EulerianFluidSolver3D {
         macz = cell;
         macz.z = fs_make_axis(G.z_min - (double)0.5 * cell.z.d, G.z_max + (double)0.5 * cell.z.d, G.nz + 1);
         mat = FsMaterial{P.air_bubble_rising, P.water_density, P.air_density, P.water_viscosity, P.air_viscosity};
-        if (P.precision == FS_FP32_REFINE && !fixed() && !cubic())
+        if (P.precision != FS_FP64 && !fixed() && !cubic())
             throw FsFail{FS_ERR_UNSUPPORTED,
                          "FP32 refinement on unequal spacings needs the fixed schedule: the compatible schedule's "
                          "matrix puts 1/dx^2 on the diagonal for every face, so constants are not in its null space"};
@@ -217,7 +222,7 @@ class EulerianFluidSolver3D {
         w64.q = dfield();
         w64.p2 = dfield();
         w64.partials = (double*)dalloc(npart);
-        if (P.precision == FS_FP32_REFINE) {
+        if (P.precision == FS_FP32_REFINE || P.precision == FS_FP32_MG) {
             bx32 = ffield();
             by32 = ffield();
             bz32 = ffield();
@@ -284,6 +289,9 @@ class EulerianFluidSolver3D {
         if (hS) (void)hipHostFree(hS);
         if (hvmax) (void)hipHostFree(hvmax);
         if (hkstat) (void)hipHostFree(hkstat);
+        if (mg_part) (void)hipFree(mg_part);
+        if (mg_tot) (void)hipFree(mg_tot);
+        if (hmg_tot) (void)hipHostFree(hmg_tot);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : probe.ev)
@@ -747,6 +755,81 @@ class EulerianFluidSolver3D {
         return n;
     }
 
+    // ---- FS_FP32_MG ----------------------------------------------------------
+    // CG on the FP32 pin-free operator with one multigrid V-cycle as the
+    // preconditioner (fs_mg.h), from e = 0 for the zero-sum residual r32,
+    // until ||r|| <= rtol ||r32|| or max_it.  Returns the iterations.
+    int run_mgpcg32(double rtol, int max_it) {
+        const FsLayout& Lz = L;
+        const int nb = fs_mg_blocks(Lz);
+        if (!mg_part) {
+            mg_part = (double*)dalloc(sizeof(double) * 2 * (size_t)nb);
+            mg_tot = (double*)dalloc(sizeof(double) * 4);
+            HIPCHECK(hipHostMalloc((void**)&hmg_tot, sizeof(double) * 4));
+        }
+        const float ix = (float)cell.x.inv_d2, iy = (float)cell.y.inv_d2, iz = (float)cell.z.inv_d2;
+        float *x = e32, *r = (float*)w32.r, *z = (float*)w32.s, *p = (float*)w32.p, *q = (float*)w32.q;
+        auto total = [&](int slots) {
+            fs_mg_total(st, mg_part, nb, slots, mg_tot);
+            HIPCHECK(hipMemcpyAsync(hmg_tot, mg_tot, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
+            sync();
+        };
+        HIPCHECK(hipMemsetAsync(x, 0, sizeof(float) * L.total, st));
+        HIPCHECK(hipMemcpyAsync(r, r32, sizeof(float) * L.total, hipMemcpyDeviceToDevice, st));
+        mg.Apply(st, r, z);
+        fs_mg_dot2(st, Lz, r, z, mg_part);
+        total(2);
+        double rz = hmg_tot[0], rr0 = hmg_tot[1];
+        if (!(rr0 > 0)) return 0;
+        HIPCHECK(hipMemcpyAsync(p, z, sizeof(float) * L.total, hipMemcpyDeviceToDevice, st));
+        int it = 0;
+        while (it < max_it) {
+            fs_mg_spmv_dot(st, Lz, bx32, by32, bz32, ix, iy, iz, p, q, mg_part);
+            total(1);
+            double pq = hmg_tot[0];
+            if (!(pq > 0)) break;
+            double alpha = rz / pq;
+            fs_mg_update(st, Lz, x, r, p, q, (float)alpha, mg_part);
+            total(1);
+            it++;
+            if (hmg_tot[0] <= rtol * rtol * rr0) break;
+            mg.Apply(st, r, z);
+            fs_mg_dot2(st, Lz, r, z, mg_part);
+            total(1);
+            double rz_new = hmg_tot[0];
+            fs_mg_direction(st, Lz, z, p, (float)(rz_new / rz));
+            rz = rz_new;
+        }
+        check_launch();
+        return it;
+    }
+
+    // FP32 multigrid-preconditioned CG inside FP64 iterative refinement:
+    // each sweep solves the FP32 correction to 1e-5 of its right-hand side
+    // and adds it to the FP64 pressure; the FP64 residual (its constant mode
+    // settled through the pin, as in the Jacobi refinement) decides the stop.
+    void SolvePressureMg() {
+        mg.Init(L, cell.x.inv_d2, cell.y.inv_d2, cell.z.inv_d2);
+        mg.SetOperator(st, bx32, by32, bz32);
+        init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
+        const double rt2 = P.rel_tolerance * P.rel_tolerance;
+        double bb = 0, rr = true_residual(bb);
+        int sweeps = 0, its = 0;
+        while (bb > 0 && rr > rt2 * bb && sweeps < 40) {
+            // restarted CG: at most 500 iterations per sweep
+            its += run_mgpcg32(1e-5, std::min(P.max_iteration, 500));
+            fs_add_correction(st, L, p, e32);
+            rr = true_residual(bb);
+            sweeps++;
+            if (debug) fprintf(stderr, "[fs] mg sweep %d its %d rel64=%.3e\n", sweeps, its, std::sqrt(rr / bb));
+        }
+        stats.pcg_iterations += its;
+        stats.refinement_sweeps += sweeps;
+        stats.residual_rel = bb > 0 ? std::sqrt(rr / bb) : 0;
+        stats.residual_precond = 0;
+        if (bb > 0 && rr > rt2 * bb) throw FsFail{FS_ERR_NOT_CONVERGED, "multigrid refinement did not reach rel_tolerance"};
+    }
+
     // Pressure solve on p (warm start), FP64 or FP32-inner / FP64-refined.
     void SolvePressure() {
         FsPcgOp o = op64();
@@ -763,6 +846,10 @@ class EulerianFluidSolver3D {
         // b - A x, the correction update and all dot products stay FP64.
         (void)o;
         fs_make_op32(st, L, bx, by, bz, diag, bx32, by32, bz32, diag32);
+        if (P.precision == FS_FP32_MG) {
+            SolvePressureMg();
+            return;
+        }
         if (op_phi) fs_to_float(st, phi, phi32, L.total);
         double rr = 0, bb = 0;
         int updates = 0;

@@ -1,0 +1,447 @@
+// fs_mg.hip -- FP32 multigrid V-cycle and the vector kernels of the
+// FS_FP32_MG inner conjugate gradient (the hierarchy is described in
+// fs_mg.h).
+#include "fs_mg.h"
+
+#include <algorithm>
+#include <stdexcept>
+
+#define MBX 64
+#define MBY 4
+#define MNT (MBX * MBY)
+#ifndef MG_NU
+#define MG_NU 2  // red-black sweeps before and after the coarse correction
+#endif
+#ifndef MG_COARSE_CELLS
+#define MG_COARSE_CELLS 512  // coarsen while a level has more cells
+#endif
+#ifndef MG_COARSE_SWEEPS
+#define MG_COARSE_SWEEPS 24  // red-black sweeps of the coarsest level, each way
+#endif
+
+namespace {
+
+void check(hipError_t e) {
+    if (e != hipSuccess) throw std::runtime_error(hipGetErrorString(e));
+}
+
+dim3 cells_grid(const FsLayout& L) { return dim3((L.nx + MBX - 1) / MBX, (L.ny + MBY - 1) / MBY, L.nz); }
+
+// sum over faces of beta_f (x_c - x_n) / d_f^2 at cell c
+__device__ __forceinline__ float flux_row(const FsLayout& L, const float* __restrict__ bx,
+                                          const float* __restrict__ by, const float* __restrict__ bz, float ix,
+                                          float iy, float iz, const float* __restrict__ x, long c) {
+    const float xc = x[c];
+    return bx[c] * (ix * (xc - x[c - 1])) + bx[c + 1] * (ix * (xc - x[c + 1])) + by[c] * (iy * (xc - x[c - L.sx])) +
+           by[c + L.sx] * (iy * (xc - x[c + L.sx])) + bz[c] * (iz * (xc - x[c - L.sy])) +
+           bz[c + L.sy] * (iz * (xc - x[c + L.sy]));
+}
+
+// block tree sum of one double per thread into part[block]
+__device__ __forceinline__ void block_partial(double v, double* part) {
+    __shared__ double sh[MNT];
+    const int t = threadIdx.y * MBX + threadIdx.x;
+    sh[t] = v;
+    __syncthreads();
+    for (int s = MNT / 2; s > 0; s >>= 1) {
+        if (t < s) sh[t] += sh[t + s];
+        __syncthreads();
+    }
+    if (t == 0) part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
+}
+
+// 1-D prolongation weight of coarse cell C for fine cell f (trilinear,
+// indices clamped at the walls): 3/4 from f/2, 1/4 from its other neighbour
+__device__ __forceinline__ float pweight(int f, int C, int nc) {
+    int c0 = f >> 1;
+    int c1 = (f & 1) ? c0 + 1 : c0 - 1;
+    c1 = c1 < 0 ? 0 : (c1 > nc - 1 ? nc - 1 : c1);
+    float w = 0;
+    if (c0 == C) w += 0.75f;
+    if (c1 == C) w += 0.25f;
+    return w;
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_diag(FsLayout L, const float* __restrict__ bx,
+                                                 const float* __restrict__ by, const float* __restrict__ bz,
+                                                 float ix, float iy, float iz, float* __restrict__ diag) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    diag[c] = ix * (bx[c] + bx[c + 1]) + iy * (by[c] + by[c + L.sx]) + iz * (bz[c] + bz[c + L.sy]);
+}
+
+// conductance of the fine faces f-1, f, f+1 (stride st) in series with
+// weights 1/4, 1/2, 1/4 (the path between two coarse cell centres): 0 if any
+// is 0; faces outside [0, nf] are left out
+__device__ __forceinline__ float series3(const float* __restrict__ b, long c, long st, int f, int nf) {
+    float res = 0, w = 0;
+    for (int d = -1; d <= 1; d++) {
+        int g = f + d;
+        if (g < 0 || g > nf) continue;
+        float bv = b[c + d * st], wd = d == 0 ? 0.5f : 0.25f;
+        if (bv == 0.f) return 0.f;
+        res += wd / bv;
+        w += wd;
+    }
+    return w / res;
+}
+
+// coarse face coefficients: over the fine face lines a coarse face covers,
+// the mean of each line's series conductance across the faces the segment
+// between the two coarse cell centres passes (harmonic along the normal,
+// arithmetic across it, as conductances in series and in parallel combine)
+__global__ void __launch_bounds__(MNT) k_mg_coarsen(FsLayout Lf, const float* __restrict__ fx,
+                                                    const float* __restrict__ fy, const float* __restrict__ fz,
+                                                    FsLayout Lc, float* __restrict__ cx, float* __restrict__ cy,
+                                                    float* __restrict__ cz) {
+    int I = blockIdx.x * MBX + threadIdx.x, J = blockIdx.y * MBY + threadIdx.y, K = blockIdx.z;
+    if (I > Lc.nx || J > Lc.ny) return;
+    long C = Lc.idx(I, J, K);
+    // x-faces: fine face index min(2I, nx), over fine rows 2J, 2J+1 and planes 2K, 2K+1
+    if (J < Lc.ny && K < Lc.nz) {
+        int fi = min(2 * I, Lf.nx);
+        float s = 0;
+        int n = 0;
+        for (int dk = 0; dk < 2; dk++)
+            for (int dj = 0; dj < 2; dj++) {
+                int j = 2 * J + dj, k = 2 * K + dk;
+                if (j < Lf.ny && k < Lf.nz) {
+                    s += series3(fx, Lf.idx(fi, j, k), 1, fi, Lf.nx);
+                    n++;
+                }
+            }
+        cx[C] = n ? s / (float)n : 0.f;
+    }
+    if (I < Lc.nx && K < Lc.nz) {
+        int fj = min(2 * J, Lf.ny);
+        float s = 0;
+        int n = 0;
+        for (int dk = 0; dk < 2; dk++)
+            for (int di = 0; di < 2; di++) {
+                int i = 2 * I + di, k = 2 * K + dk;
+                if (i < Lf.nx && k < Lf.nz) {
+                    s += series3(fy, Lf.idx(i, fj, k), Lf.sx, fj, Lf.ny);
+                    n++;
+                }
+            }
+        cy[C] = n ? s / (float)n : 0.f;
+    }
+    if (I < Lc.nx && J < Lc.ny) {
+        int fk = min(2 * K, Lf.nz);
+        float s = 0;
+        int n = 0;
+        for (int dj = 0; dj < 2; dj++)
+            for (int di = 0; di < 2; di++) {
+                int i = 2 * I + di, j = 2 * J + dj;
+                if (i < Lf.nx && j < Lf.ny) {
+                    s += series3(fz, Lf.idx(i, j, fk), Lf.sy, fk, Lf.nz);
+                    n++;
+                }
+            }
+        cz[C] = n ? s / (float)n : 0.f;
+    }
+}
+
+// red-black Gauss-Seidel half sweep: cells with (i+j+k) % 2 == color
+__global__ void __launch_bounds__(MNT) k_mg_smooth(FsLayout L, const float* __restrict__ bx,
+                                                   const float* __restrict__ by, const float* __restrict__ bz,
+                                                   float ix, float iy, float iz, const float* __restrict__ diag,
+                                                   const float* __restrict__ b, float* __restrict__ x, int color) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny || ((i + j + k) & 1) != color) return;
+    long c = L.idx(i, j, k);
+    float d = diag[c];
+    if (d == 0.f) return;
+    float s = b[c] + bx[c] * (ix * x[c - 1]) + bx[c + 1] * (ix * x[c + 1]) + by[c] * (iy * x[c - L.sx]) +
+              by[c + L.sx] * (iy * x[c + L.sx]) + bz[c] * (iz * x[c - L.sy]) + bz[c + L.sy] * (iz * x[c + L.sy]);
+    x[c] = s / d;
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_residual(FsLayout L, const float* __restrict__ bx,
+                                                     const float* __restrict__ by, const float* __restrict__ bz,
+                                                     float ix, float iy, float iz, const float* __restrict__ b,
+                                                     const float* __restrict__ x, float* __restrict__ r) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    r[c] = b[c] - flux_row(L, bx, by, bz, ix, iy, iz, x, c);
+}
+
+// coarse right-hand side: transpose of the prolongation over the 4x4x4 fine
+// block around the coarse cell, divided by 8; coarse x set to 0
+__global__ void __launch_bounds__(MNT) k_mg_restrict(FsLayout Lf, const float* __restrict__ rf, FsLayout Lc,
+                                                     float* __restrict__ bc, float* __restrict__ xc) {
+    int I = blockIdx.x * MBX + threadIdx.x, J = blockIdx.y * MBY + threadIdx.y, K = blockIdx.z;
+    if (I >= Lc.nx || J >= Lc.ny) return;
+    float s = 0;
+    for (int dk = -1; dk <= 2; dk++) {
+        int k = 2 * K + dk;
+        if (k < 0 || k >= Lf.nz) continue;
+        float wk = pweight(k, K, Lc.nz);
+        if (wk == 0.f) continue;
+        for (int dj = -1; dj <= 2; dj++) {
+            int j = 2 * J + dj;
+            if (j < 0 || j >= Lf.ny) continue;
+            float wj = pweight(j, J, Lc.ny);
+            if (wj == 0.f) continue;
+            float row = 0;
+            for (int di = -1; di <= 2; di++) {
+                int i = 2 * I + di;
+                if (i < 0 || i >= Lf.nx) continue;
+                row += pweight(i, I, Lc.nx) * rf[Lf.idx(i, j, k)];
+            }
+            s += wk * wj * row;
+        }
+    }
+    long C = Lc.idx(I, J, K);
+    bc[C] = 0.125f * s;
+    xc[C] = 0.f;
+}
+
+// x_fine += P x_coarse (trilinear, clamped)
+__global__ void __launch_bounds__(MNT) k_mg_prolong(FsLayout Lc, const float* __restrict__ xc, FsLayout Lf,
+                                                    float* __restrict__ xf) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    if (i >= Lf.nx || j >= Lf.ny) return;
+    auto pair = [](int f, int nc, int& c0, int& c1) {
+        c0 = f >> 1;
+        c1 = (f & 1) ? c0 + 1 : c0 - 1;
+        c1 = c1 < 0 ? 0 : (c1 > nc - 1 ? nc - 1 : c1);
+    };
+    int i0, i1, j0, j1, k0, k1;
+    pair(i, Lc.nx, i0, i1);
+    pair(j, Lc.ny, j0, j1);
+    pair(k, Lc.nz, k0, k1);
+    const int ii[2] = {i0, i1}, jj[2] = {j0, j1}, kk[2] = {k0, k1};
+    const float w[2] = {0.75f, 0.25f};
+    float s = 0;
+    for (int c = 0; c < 2; c++)
+        for (int b = 0; b < 2; b++)
+            for (int a = 0; a < 2; a++) s += w[a] * w[b] * w[c] * xc[Lc.idx(ii[a], jj[b], kk[c])];
+    xf[Lf.idx(i, j, k)] += s;
+}
+
+// coarsest level: red-black sweeps (red first) then the mirrored sweeps, one
+// workgroup, x from 0
+__global__ void __launch_bounds__(MNT) k_mg_coarse(FsLayout L, const float* __restrict__ bx,
+                                                   const float* __restrict__ by, const float* __restrict__ bz,
+                                                   float ix, float iy, float iz, const float* __restrict__ diag,
+                                                   const float* __restrict__ b, float* __restrict__ x, int sweeps) {
+    const int t = threadIdx.y * MBX + threadIdx.x;
+    const int n = L.nx * L.ny * L.nz;
+    for (int m = t; m < n; m += MNT) {
+        int i = m % L.nx, j = (m / L.nx) % L.ny, k = m / (L.nx * L.ny);
+        x[L.idx(i, j, k)] = 0.f;
+    }
+    __syncthreads();
+    for (int s = 0; s < 4 * sweeps; s++) {
+        // forward sweeps red, black; then backward black, red
+        const int half = s < 2 * sweeps ? (s & 1) : 1 - (s & 1);
+        for (int m = t; m < n; m += MNT) {
+            int i = m % L.nx, j = (m / L.nx) % L.ny, k = m / (L.nx * L.ny);
+            if (((i + j + k) & 1) != half) continue;
+            long c = L.idx(i, j, k);
+            float d = diag[c];
+            if (d == 0.f) continue;
+            float v = b[c] + bx[c] * (ix * x[c - 1]) + bx[c + 1] * (ix * x[c + 1]) + by[c] * (iy * x[c - L.sx]) +
+                      by[c + L.sx] * (iy * x[c + L.sx]) + bz[c] * (iz * x[c - L.sy]) + bz[c + L.sy] * (iz * x[c + L.sy]);
+            x[c] = v / d;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- CG vector kernels ------------------------------------------------------
+__global__ void __launch_bounds__(MNT) k_mg_spmv_dot(FsLayout L, const float* __restrict__ bx,
+                                                     const float* __restrict__ by, const float* __restrict__ bz,
+                                                     float ix, float iy, float iz, const float* __restrict__ p,
+                                                     float* __restrict__ q, double* part) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    double acc = 0;
+    if (i < L.nx && j < L.ny) {
+        long c = L.idx(i, j, k);
+        float v = flux_row(L, bx, by, bz, ix, iy, iz, p, c);
+        q[c] = v;
+        acc = (double)p[c] * (double)v;
+    }
+    block_partial(acc, part);
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_update(FsLayout L, float* __restrict__ x, float* __restrict__ r,
+                                                   const float* __restrict__ p, const float* __restrict__ q,
+                                                   float alpha, double* part) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    double acc = 0;
+    if (i < L.nx && j < L.ny) {
+        long c = L.idx(i, j, k);
+        x[c] += alpha * p[c];
+        float rv = r[c] - alpha * q[c];
+        r[c] = rv;
+        acc = (double)rv * (double)rv;
+    }
+    block_partial(acc, part);
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_dot2(FsLayout L, const float* __restrict__ a,
+                                                 const float* __restrict__ b, double* part, int nb) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    double ab = 0, aa = 0;
+    if (i < L.nx && j < L.ny) {
+        long c = L.idx(i, j, k);
+        double av = a[c];
+        ab = av * (double)b[c];
+        aa = av * av;
+    }
+    block_partial(ab, part);
+    __syncthreads();
+    block_partial(aa, part + nb);
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_direction(FsLayout L, const float* __restrict__ z, float* __restrict__ p,
+                                                      float beta) {
+    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    long c = L.idx(i, j, k);
+    p[c] = z[c] + beta * p[c];
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_total(const double* part, int nb, double* out) {
+    __shared__ double sh[MNT];
+    const int t = threadIdx.x;
+    const double* src = part + (long)blockIdx.x * nb;
+    double s = 0;
+    for (int m = t; m < nb; m += MNT) s += src[m];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = MNT / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) out[blockIdx.x] = sh[0];
+}
+
+}  // namespace
+
+// ---- hierarchy ---------------------------------------------------------------
+FsMg::~FsMg() {
+    for (size_t l = 0; l < lv_.size(); l++) {
+        FsMgLevel& v = lv_[l];
+        if (l > 0)
+            for (float* f : {v.bx, v.by, v.bz, v.diag, v.x, v.b})
+                if (f) (void)hipFree(f);
+        if (v.r) (void)hipFree(v.r);
+    }
+    if (diag0_) (void)hipFree(diag0_);
+}
+
+void FsMg::Init(const FsLayout& L0, double idx2, double idy2, double idz2) {
+    if (!lv_.empty() && lv_[0].L.nx == L0.nx && lv_[0].L.ny == L0.ny && lv_[0].L.nz == L0.nz) return;
+    if (!lv_.empty()) throw std::runtime_error("multigrid hierarchy built for another grid");
+    auto field = [](const FsLayout& L) {
+        float* p = nullptr;
+        check(hipMalloc(&p, sizeof(float) * (size_t)L.total));
+        check(hipMemset(p, 0, sizeof(float) * (size_t)L.total));
+        return p;
+    };
+    FsMgLevel v{};
+    v.L = L0;
+    v.idx2 = (float)idx2;
+    v.idy2 = (float)idy2;
+    v.idz2 = (float)idz2;
+    v.r = field(L0);
+    diag0_ = field(L0);
+    v.diag = diag0_;
+    lv_.push_back(v);
+    while ((long)lv_.back().L.nx * lv_.back().L.ny * lv_.back().L.nz > MG_COARSE_CELLS) {
+        const FsMgLevel& f = lv_.back();
+        FsMgLevel c{};
+        c.L = fs_make_layout((f.L.nx + 1) / 2, (f.L.ny + 1) / 2, (f.L.nz + 1) / 2);
+        // the spacing doubles along an axis that halves; an axis of one cell stays
+        c.idx2 = f.L.nx > 1 ? f.idx2 * 0.25f : f.idx2;
+        c.idy2 = f.L.ny > 1 ? f.idy2 * 0.25f : f.idy2;
+        c.idz2 = f.L.nz > 1 ? f.idz2 * 0.25f : f.idz2;
+        c.bx = field(c.L);
+        c.by = field(c.L);
+        c.bz = field(c.L);
+        c.diag = field(c.L);
+        c.x = field(c.L);
+        c.b = field(c.L);
+        c.r = field(c.L);
+        lv_.push_back(c);
+    }
+}
+
+void FsMg::SetOperator(hipStream_t st, const float* bx, const float* by, const float* bz) {
+    lv_[0].bx = (float*)bx;
+    lv_[0].by = (float*)by;
+    lv_[0].bz = (float*)bz;
+    for (size_t l = 0; l < lv_.size(); l++) {
+        FsMgLevel& v = lv_[l];
+        if (l > 0) {
+            const FsMgLevel& f = lv_[l - 1];
+            dim3 g((v.L.nx + 1 + MBX - 1) / MBX, (v.L.ny + 1 + MBY - 1) / MBY, v.L.nz + 1);
+            hipLaunchKernelGGL(k_mg_coarsen, g, dim3(MBX, MBY), 0, st, f.L, f.bx, f.by, f.bz, v.L, v.bx, v.by, v.bz);
+        }
+        hipLaunchKernelGGL(k_mg_diag, cells_grid(v.L), dim3(MBX, MBY), 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2,
+                           v.idz2, v.diag);
+    }
+}
+
+void FsMg::Cycle(hipStream_t st, int l) {
+    FsMgLevel& v = lv_[l];
+    const dim3 g = cells_grid(v.L), t(MBX, MBY);
+    if (l + 1 == (int)lv_.size()) {
+        hipLaunchKernelGGL(k_mg_coarse, dim3(1), t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag,
+                           v.b, v.x, MG_COARSE_SWEEPS);
+        return;
+    }
+    for (int s = 0; s < 2 * MG_NU; s++)  // red, black, ...
+        hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b, v.x,
+                           s & 1);
+    hipLaunchKernelGGL(k_mg_residual, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x, v.r);
+    FsMgLevel& c = lv_[l + 1];
+    hipLaunchKernelGGL(k_mg_restrict, cells_grid(c.L), t, 0, st, v.L, v.r, c.L, c.b, c.x);
+    Cycle(st, l + 1);
+    hipLaunchKernelGGL(k_mg_prolong, g, t, 0, st, c.L, c.x, v.L, v.x);
+    for (int s = 0; s < 2 * MG_NU; s++)  // ..., black, red
+        hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b, v.x,
+                           1 - (s & 1));
+}
+
+void FsMg::Apply(hipStream_t st, const float* r, float* z) {
+    FsMgLevel& v = lv_[0];
+    v.b = (float*)r;
+    v.x = z;
+    check(hipMemsetAsync(z, 0, sizeof(float) * (size_t)v.L.total, st));
+    Cycle(st, 0);
+}
+
+// ---- CG vector kernels ------------------------------------------------------
+int fs_mg_blocks(const FsLayout& L) {
+    dim3 g = cells_grid(L);
+    return (int)(g.x * g.y * g.z);
+}
+
+void fs_mg_spmv_dot(hipStream_t st, const FsLayout& L, const float* bx, const float* by, const float* bz, float idx2,
+                    float idy2, float idz2, const float* p, float* q, double* part) {
+    hipLaunchKernelGGL(k_mg_spmv_dot, cells_grid(L), dim3(MBX, MBY), 0, st, L, bx, by, bz, idx2, idy2, idz2, p, q,
+                       part);
+}
+
+void fs_mg_update(hipStream_t st, const FsLayout& L, float* x, float* r, const float* p, const float* q, float alpha,
+                  double* part) {
+    hipLaunchKernelGGL(k_mg_update, cells_grid(L), dim3(MBX, MBY), 0, st, L, x, r, p, q, alpha, part);
+}
+
+void fs_mg_dot2(hipStream_t st, const FsLayout& L, const float* a, const float* b, double* part) {
+    hipLaunchKernelGGL(k_mg_dot2, cells_grid(L), dim3(MBX, MBY), 0, st, L, a, b, part, fs_mg_blocks(L));
+}
+
+void fs_mg_direction(hipStream_t st, const FsLayout& L, const float* z, float* p, float beta) {
+    hipLaunchKernelGGL(k_mg_direction, cells_grid(L), dim3(MBX, MBY), 0, st, L, z, p, beta);
+}
+
+void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* out) {
+    hipLaunchKernelGGL(k_mg_total, dim3(slots), dim3(MNT), 0, st, part, nb, out);
+}

@@ -42,6 +42,9 @@ extern "C" {
 /* Pressure solve precision. */
 #define FS_FP64          0   /* Jacobi-PCG in FP64 (PCG_METHOD.cpp:4-126) */
 #define FS_FP32_REFINE   1   /* FP32 Jacobi-PCG inside FP64 iterative refinement */
+#define FS_FP32_MG       2   /* FP32 CG preconditioned by one geometric-multigrid V-cycle, inside FP64
+                                iterative refinement (no counterpart in the reference, whose multigrid
+                                options are declared but not built, PROJECTION_3D.h:207-228) */
 
 /* PCG stop rule. */
 #define FS_STOP_ABS_PRECOND  0   /* r^T D^-1 r < tol^2, the reference rule (PCG_METHOD.cpp:85-90) */
@@ -90,7 +93,7 @@ typedef struct fs_params {
     int    use_previous_pressure;
     int    stop_rule;          /* FS_STOP_* */
     double rel_tolerance;      /* for FS_STOP_REL_RESIDUAL */
-    int    precision;          /* FS_FP64 | FS_FP32_REFINE */
+    int    precision;          /* FS_FP64 | FS_FP32_REFINE | FS_FP32_MG */
     int    schedule;           /* FS_SCHEDULE_* */
     /* debugging switches, EULERIAN_FLUID_SOLVER_3D.h:103-107 */
     int    is_levelset_advection_active;

@@ -288,6 +288,65 @@ def test_spmv_coefficients_from_phi(hip_lib, monkeypatch, sched):
         assert rel_err(a, b) < 1e-5, (c, rel_err(a, b))
 
 
+@pytest.mark.parametrize("name,n", [("rising_drop", 32), ("dam_break", 32), ("bubbles", 40)])
+def test_multigrid_refined_projection(hip_lib, oracle_lib, name, n):
+    """FS_FP32_MG: CG preconditioned by a multigrid V-cycle in FP32, inside
+    the same FP64 refinement.  It reaches 1e-10 relative residual (the FP64
+    residual decides), matches the CPU checker's FP64 solve to the solve's
+    accuracy, and needs far fewer inner iterations than Jacobi."""
+    over = dict(stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-10)
+    gmg, cpu = make_pair(hip_lib, oracle_lib, name, n=n, precision=_abi.FS_FP32_MG, **over)
+    gj, _ = make_pair(hip_lib, oracle_lib, name, n=n, precision=_abi.FS_FP32_REFINE, **over)
+    smg, sj = gmg.projection(2e-3), gj.projection(2e-3)
+    cpu.projection(2e-3)
+    print(name, n, "mg", smg["pcg_iterations"], smg["refinement_sweeps"], "jacobi", sj["pcg_iterations"])
+    assert smg["residual_rel"] <= 1e-10, smg
+    assert 3 * smg["pcg_iterations"] < sj["pcg_iterations"], (smg, sj)
+    fg, fc = gmg.get_fields(), cpu.get_fields()
+    for c in ("p", "u", "v", "w"):
+        a, b = fg[c], fc[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        assert rel_err(a, b) < 1e-7, (c, rel_err(a, b))
+
+
+def test_multigrid_odd_and_unequal_grids(hip_lib, oracle_lib):
+    """Odd cell counts (ceil coarsening, clamped transfer at the walls) and
+    unequal spacings (fixed schedule): the multigrid refinement reaches
+    1e-10 and matches the FP64 checker."""
+    nx, ny, nz = 37, 29, 41
+    g = _abi.unit_cube_grid(nx, ny, nz)
+    k, j, i = np.meshgrid((np.arange(nz) + .5) / nz, (np.arange(ny) + .5) / ny, (np.arange(nx) + .5) / nx,
+                          indexing="ij")
+    phi = np.minimum(np.sqrt((i - .4) ** 2 + (j - .55) ** 2 + (k - .45) ** 2) - .2, 0.8 - k)
+    rng = np.random.default_rng(8)
+    vel = (rng.standard_normal((nz, ny, nx + 1)), rng.standard_normal((nz, ny + 1, nx)),
+           rng.standard_normal((nz + 1, ny, nx)))
+    over = dict(stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-10, schedule=_abi.FS_SCHEDULE_FIXED)
+    gpu = _abi.Solver(hip_lib, "fs_", _abi.default_params(precision=_abi.FS_FP32_MG, **over), g, 0)
+    cpu = _abi.Solver(oracle_lib, "fso_", _abi.default_params(**over), g)
+    for s in (gpu, cpu):
+        s.set_fields(phi, *vel)
+    st = gpu.projection(1e-2)
+    cpu.projection(1e-2)
+    assert st["residual_rel"] <= 1e-10, st
+    fg, fc = gpu.get_fields(), cpu.get_fields()
+    for c in ("p", "u", "v", "w"):
+        a, b = fg[c], fc[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        assert rel_err(a, b) < 1e-7, (c, rel_err(a, b))
+
+
+def test_multigrid_frames(hip_lib, oracle_lib):
+    """Two frames of the dam break with the multigrid refinement agree with
+    the CPU checker's FP64 frames to 1e-8 when both solve to 1e-11."""
+    its, errs = _frames(hip_lib, oracle_lib, "dam_break", 24, 2, velocity_scale=True, precision=_abi.FS_FP32_MG,
+                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11)
+    for c, e in errs.items():
+        assert e < 1e-8, (c, e)
+
+
 @pytest.mark.parametrize("name,n,frames", [("rising_drop", 24, 4), ("dam_break", 24, 4)])
 def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
     """The reference's absolute stop rule at its script tolerance (1e-5):
