@@ -6,9 +6,12 @@ A step is one frame of SIMULATION_WORLD::AdvanceOneFrame on the oil/water
 rising-drop scene at n^3 (default 256^3): level-set WENO5 stages, ENO3 MAC
 advection, gravity, explicit viscosity, the variable-density projection
 solved to 1e-8 relative residual, and 2 x 25 reinitialization sweeps.  The
-headline value is ms/step with FP32 Jacobi-PCG inside FP64 iterative
-refinement; the plain FP64 path runs the same frames for comparison.  Inputs
-are resident on the GPU before the timed region; every step is synchronous.
+headline value is ms/step with the FP32 inner solve inside FP64 iterative
+refinement, the inner solve being CG preconditioned by a geometric multigrid
+V-cycle; FP32 Jacobi-PCG inside the same refinement and the plain FP64
+Jacobi-PCG path run the same frames from the same state for comparison
+(`--paths`).  Inputs are resident on the GPU before the timed region; every
+step is synchronous.
 
 One JSON line is printed.  `roofline` is for the dominant kernel (timed with
 HIP events on the solver's stream); `cpu_baseline` times the CPU checker on
@@ -36,7 +39,11 @@ def parse():
     ap.add_argument("--n", type=int, default=None, help="grid edge (default 256; 128 for the adjoint workload)")
     ap.add_argument("--scene", default="rising_drop")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-fp64", action="store_true")
+    ap.add_argument("--paths", default="mg,refine,fp64",
+                    help="projection paths run on the same frames; the first is the headline: mg (FP32 CG with a "
+                         "multigrid preconditioner inside FP64 refinement), refine (FP32 Jacobi-PCG inside FP64 "
+                         "refinement), fp64 (FP64 Jacobi-PCG)")
+    ap.add_argument("--no-fp64", action="store_true", help="drop fp64 from --paths")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (profiler byte-counter calibration)")
@@ -300,6 +307,96 @@ def adjoint_main(a):
     print(json.dumps(out))
 
 
+PATHS = {"mg": "FS_FP32_MG", "refine": "FS_FP32_REFINE", "fp64": "FS_FP64"}
+LABEL = {"mg": "fp32+multigrid+refine", "refine": "fp32+jacobi+refine", "fp64": "fp64"}
+
+
+def run_path(lib, scene, n, path, warmup, steps, sync):
+    """Frames of one projection path from the same initial state; returns
+    the timing, the solver statistics, the in-loop kernel samples and the
+    final fields."""
+    from fluid_simulator_backup_amd import _abi, scenes
+    prec = getattr(_abi, PATHS[path])
+    phi, vel, P, g = scenes.make(scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
+                                 precision=prec, max_iteration=20000)
+    s = _abi.Solver(lib, "fs_", P, g, 0)
+    s.set_fields(phi, *vel)
+    dt = s.cfl_dt(0.5, 1)
+    print(f"[bench] {scene} {n}^3 dt={dt:.4e} {LABEL[path]} warmup={warmup} steps={steps}", file=sys.stderr,
+          flush=True)
+    ms, st = run_frames(s, dt, warmup, steps, sync)
+    print(f"[bench] {LABEL[path]} {ms:.2f} ms/step; last {st[-1]}", file=sys.stderr, flush=True)
+    out = {"ms_per_step": ms, "stats": st, "dt": dt,
+           "frame": {k: _abi.frame_kernel_samples(s, getattr(_abi, k)) for k in ("FS_KS_REINIT", "FS_KS_MG_SMOOTH")}}
+    if path != "mg":
+        out["pcg"] = _abi.kernel_samples(s, prec)
+        if path == "refine":
+            out["pcg_repeat"] = _abi.time_pcg_kernels(s, prec, 20)
+    out["fields"] = s.get_fields()
+    s.close()
+    return out
+
+
+def summary(r):
+    st = r["stats"]
+    return {"ms_per_step": round(r["ms_per_step"], 3),
+            "pcg_iterations_per_step": float(np.mean([s["pcg_iterations"] for s in st])),
+            "refinement_sweeps_per_step": float(np.mean([s["refinement_sweeps"] for s in st])),
+            "projection_ms": round(float(np.mean([s["projection_ms"] for s in st])), 3),
+            "residual_rel": st[-1]["residual_rel"]}
+
+
+def field_error(fa, fb):
+    """Per-field relative L2 and Linf of fa against fb (pressure with its
+    mean removed: the additive constant is held only by the weak pin)."""
+    err = {}
+    for c in ("phi", "u", "v", "w", "p"):
+        a, b = fa[c], fb[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        d = a - b
+        err[c] = {"rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(b), 1e-300)),
+                  "rel_linf": float(np.abs(d).max() / max(np.abs(b).max(), 1e-300))}
+    return err
+
+
+def pcg_roofline(k, cnt, k_rep, cells, n, wb):
+    """Jacobi-PCG iteration kernels.  Algorithmic words per cell of one
+    iteration (the workload's fixed accounting): SpMV 5 (read p and the
+    three face coefficients, write q), update 7 (read x, p, r, q, D^-1,
+    write x, r), direction 4 (read r, D^-1, p, write p): 16.  k_pspmv_dot
+    does the SpMV and the direction update (9), k_update the update (7).
+    Moved words: s, p_old, the level set (or the three coefficient arrays
+    below 128^3 or with FS_BETA_ARRAYS), p, q; and x, p, r, q, D, x, r, s."""
+    words = {"pspmv_dot": 9, "update": 7}
+    phi_coef = n ** 3 >= 128 ** 3 and not os.environ.get("FS_BETA_ARRAYS")
+    moved = {"pspmv_dot": 5 if phi_coef else 7, "update": 8}
+    names = ["pspmv_dot", "update"]
+    gbs = lambda ms, nm, w=words: round(w[nm] * cells * wb / (ms * 1e-3) / 1e9, 1)
+    out = {"kernel_ms": {nm: round(float(k[i]), 5) for i, nm in enumerate(names)},
+           "samples": {nm: int(cnt[i]) for i, nm in enumerate(names)},
+           "algorithmic_words_per_cell": words, "moved_words_per_cell": moved,
+           "algorithmic_gbs": {nm: gbs(k[i], nm) for i, nm in enumerate(names)},
+           "moved_gbs": {nm: gbs(k[i], nm, moved) for i, nm in enumerate(names)},
+           "iteration_algorithmic_gbs": round(16 * cells * wb / (float(k[0] + k[1]) * 1e-3) / 1e9, 1)}
+    if k_rep is not None:
+        out["back_to_back_repeat_gbs"] = {nm: gbs(k_rep[i], nm) for i, nm in enumerate(names)}
+    return out
+
+
+# Frame kernels sampled in the loop: algorithmic and moved bytes per cell.
+# k_reinit, one Sussman sweep in FP64: read phi^n (its 19-point stencil) and
+# the sign field, write phi^{n+1} (3 words); it reads the ghost copy and phi
+# itself (4 words moved).  k_mg_smooth, one red-black half sweep on the
+# finest level in FP32: read every face coefficient (each face has one cell
+# of the colour being updated: 3 words per cell), and for the half of the
+# cells of that colour b, the diagonal and the new x written, plus x of the
+# other half read (2 words per cell): 5; moved, whole cache lines of the
+# three coefficient fields, D, b, x read and x written: 7.
+FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 3 * 8, 4 * 8),
+                 "k_mg_smooth": ("FS_KS_MG_SMOOTH", 5 * 4, 7 * 4)}
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -316,7 +413,6 @@ def main():
     import torch
 
     import fluid_simulator_backup_amd as fsa
-    from fluid_simulator_backup_amd import _abi, scenes
 
     torch.cuda.set_device(0)
     lib = fsa.load_library()
@@ -328,127 +424,118 @@ def main():
             dst.copy_(src)
         sync()
         del src, dst
-    phi, vel, P32, g = scenes.make(a.scene, a.n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
-                                   precision=_abi.FS_FP32_REFINE, max_iteration=20000)
+    paths = [p for p in a.paths.split(",") if p]
+    if a.no_fp64:
+        paths = [p for p in paths if p != "fp64"]
+    if not paths or any(p not in PATHS for p in paths):
+        sys.exit(f"--paths: a comma-separated subset of {','.join(PATHS)}")
+    head = paths[0]
     cells = a.n ** 3
-    s32 = _abi.Solver(lib, "fs_", P32, g, 0)
-    s32.set_fields(phi, *vel)
-    dt = s32.cfl_dt(0.5, 1)
-    print(f"[bench] {a.scene} {a.n}^3 dt={dt:.4e} fp32+refine warmup={a.warmup} steps={a.steps}", file=sys.stderr,
-          flush=True)
-    ms32, st32 = run_frames(s32, dt, a.warmup, a.steps, sync)
-    print(f"[bench] fp32+refine {ms32:.2f} ms/step; last {st32[-1]}", file=sys.stderr, flush=True)
-    # FP32 inner-loop kernels: in-loop event samples from the timed frames,
-    # and back-to-back repeats of each kernel for comparison
-    k32, n32 = _abi.kernel_samples(s32, _abi.FS_FP32_REFINE)
-    k32_rep = _abi.time_pcg_kernels(s32, _abi.FS_FP32_REFINE, 20)
-    f32 = s32.get_fields()
-    s32.close()
+    runs = {p: run_path(lib, a.scene, a.n, p, a.warmup, a.steps, sync) for p in paths}
+    R = runs[head]
+    ms = R["ms_per_step"]
+    dt = R["dt"]
+
+    # roofline for the headline path's dominant sampled kernel (largest
+    # in-loop time per frame)
+    if head == "mg":
+        share = {}
+        for kn, (ks, _, _) in FRAME_KERNELS.items():
+            avg, n_s, launches = R["frame"][ks]
+            share[kn] = avg * launches / a.steps
+        kn = max(share, key=share.get)
+        ks, alg, mov = FRAME_KERNELS[kn]
+        avg, n_s, launches = R["frame"][ks]
+        bytes_launch = alg * cells
+        achieved = bytes_launch / (avg * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(a.scene, a.n, kn),
+                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg, 5),
+                "timing": f"HIP events on the solver stream around one launch per call of the timed frames "
+                          f"({n_s} samples of {launches} launches)",
+                "frame_kernels": {k: {"avg_launch_ms": round(R["frame"][FRAME_KERNELS[k][0]][0], 5),
+                                      "launches_per_step": R["frame"][FRAME_KERNELS[k][0]][2] / a.steps,
+                                      "ms_per_step": round(share[k], 3),
+                                      "algorithmic_bytes_per_cell": FRAME_KERNELS[k][1],
+                                      "moved_bytes_per_cell": FRAME_KERNELS[k][2],
+                                      "algorithmic_gbs": round(FRAME_KERNELS[k][1] * cells /
+                                                               (max(R["frame"][FRAME_KERNELS[k][0]][0], 1e-9) * 1e-3)
+                                                               / 1e9, 1),
+                                      "moved_gbs": round(FRAME_KERNELS[k][2] * cells /
+                                                         (max(R["frame"][FRAME_KERNELS[k][0]][0], 1e-9) * 1e-3) / 1e9,
+                                                         1)}
+                                  for k in FRAME_KERNELS}}
+    else:
+        k, cnt = R["pcg"]
+        k = k[:2]
+        wb = 4 if head == "refine" else 8
+        pr = pcg_roofline(k, cnt, R.get("pcg_repeat"), cells, a.n, wb)
+        names = ["pspmv_dot", "update"]
+        dom = int(np.argmax(k))
+        kn = {"pspmv_dot": "k_pspmv_dot", "update": "k_update"}[names[dom]]
+        bytes_launch = pr["algorithmic_words_per_cell"][names[dom]] * cells * wb
+        achieved = bytes_launch / (k[dom] * 1e-3) / 1e9
+        roof = {"bound": "hbm", "kernel": f"{kn}<{'float' if wb == 4 else 'double'}>", "achieved": round(achieved, 1),
+                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+                "traffic": measured_traffic(a.scene, a.n, kn) if wb == 4 else None,
+                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(float(k[dom]), 5),
+                "timing": f"HIP events on the solver stream around one iteration per 16 of the timed frames "
+                          f"({int(cnt[dom])} samples)"}
+        roof.update(pr)
 
     res = {}
-    ms64 = None
-    if not a.no_fp64:
-        P64 = scenes.make(a.scene, a.n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
-                          precision=_abi.FS_FP64, max_iteration=20000)[2]
-        s64 = _abi.Solver(lib, "fs_", P64, g, 0)
-        s64.set_fields(phi, *vel)
-        ms64, st64 = run_frames(s64, dt, a.warmup, a.steps, sync)
-        print(f"[bench] fp64 {ms64:.2f} ms/step; last {st64[-1]}", file=sys.stderr, flush=True)
-        k64, _ = _abi.kernel_samples(s64, _abi.FS_FP64)
-        f64 = s64.get_fields()
-        s64.close()
-        res["fp64"] = {"ms_per_step": ms64, "pcg_iterations_per_step": float(np.mean([s["pcg_iterations"] for s in st64])),
-                       "projection_ms": float(np.mean([s["projection_ms"] for s in st64])),
-                       "residual_rel": st64[-1]["residual_rel"],
-                       "kernel_ms": {"pspmv_dot": float(k64[0]), "update": float(k64[1])}}
-        # both paths ran the same frames from the same state: per-field error
-        # of FP32+refine against FP64 (pressure compared with its mean
-        # removed: the additive constant is held only by the weak pin)
-        err = {}
-        for c in ("phi", "u", "v", "w", "p"):
-            a32, a64 = f32[c], f64[c]
-            if c == "p":
-                a32, a64 = a32 - a32.mean(), a64 - a64.mean()
-            d = a32 - a64
-            err[c] = {"rel_l2": float(np.linalg.norm(d) / max(np.linalg.norm(a64), 1e-300)),
-                      "rel_linf": float(np.abs(d).max() / max(np.abs(a64).max(), 1e-300))}
-        res["fp32_refine_vs_fp64_error"] = err
+    for p in paths:
+        res[p] = summary(runs[p])
+        if "pcg" in runs[p]:
+            k, cnt = runs[p]["pcg"]
+            res[p]["pcg_kernels"] = pcg_roofline(k[:2], cnt, runs[p].get("pcg_repeat"), cells, a.n,
+                                                 8 if p == "fp64" else 4)
+    if "fp64" in runs:
+        for p in paths:
+            if p != "fp64":
+                res[p]["error_vs_fp64"] = field_error(runs[p]["fields"], runs["fp64"]["fields"])
+                res[p]["speedup_vs_fp64"] = round(runs["fp64"]["ms_per_step"] / runs[p]["ms_per_step"], 3)
 
-    # roofline: algorithmic bytes per launch / measured duration (two kernels
-    # per iteration).  Algorithmic words per cell of one Jacobi-PCG iteration
-    # (the workload's fixed accounting): SpMV 5 (read p and the three face
-    # coefficients, write q), update 7 (read x, p, r, q, D^-1, write x, r),
-    # direction 4 (read r, D^-1, p, write p): 16.  k_pspmv_dot does the SpMV
-    # and the direction update (9), k_update the update (7).
-    words = {"pspmv_dot": 9, "update": 7}
-    # words each kernel actually has to move: s, p_old, the level set (or the
-    # three coefficient arrays below 128^3 or with FS_BETA_ARRAYS), p, q; and
-    # x, p, r, q, D, x, r, s
-    phi_coef = a.n ** 3 >= 128 ** 3 and not os.environ.get("FS_BETA_ARRAYS")
-    moved = {"pspmv_dot": 5 if phi_coef else 7, "update": 8}
-    names = ["pspmv_dot", "update"]
-    kname = {"pspmv_dot": "k_pspmv_dot", "update": "k_update"}
-    k32 = k32[:2]
-    dom = int(np.argmax(k32))
-    name = names[dom]
-    bytes_launch = words[name] * cells * 4
-    achieved = bytes_launch / (k32[dom] * 1e-3) / 1e9
-    gbs = lambda ms, i, wb, w=words: round(w[names[i]] * cells * wb / (ms * 1e-3) / 1e9, 1)
-    roof = {"bound": "hbm", "kernel": f"{kname[name]}<float>", "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": measured_traffic(a.scene, a.n, kname[name]),
-            "bytes_per_launch": bytes_launch, "avg_launch_ms": round(float(k32[dom]), 5),
-            "timing": f"HIP events on the solver stream around one iteration per 16 of the timed frames "
-                      f"({int(n32[dom])} samples)",
-            "algorithmic_words_per_cell": words, "moved_words_per_cell": moved,
-            "moved_gbs": {nm: gbs(k32[i], i, 4, moved) for i, nm in enumerate(names)},
-            "iteration_algorithmic_gbs": round(16 * cells * 4 / (float(sum(k32)) * 1e-3) / 1e9, 1),
-            "all_kernels_gbs": {nm: gbs(k32[i], i, 4) for i, nm in enumerate(names)},
-            "all_kernels_ms": {nm: round(float(k32[i]), 5) for i, nm in enumerate(names)},
-            "back_to_back_repeat_gbs": {nm: gbs(k32_rep[i], i, 4) for i, nm in enumerate(names)}}
-    if "fp64" in res:
-        k64 = res["fp64"]["kernel_ms"]
-        res["fp64"]["kernel_gbs"] = {nm: round(words[nm] * cells * 8 / (k64[nm] * 1e-3) / 1e9, 1) for nm in names}
-        res["fp64"]["moved_gbs"] = {nm: round(moved[nm] * cells * 8 / (k64[nm] * 1e-3) / 1e9, 1) for nm in names}
-        res["fp64"]["iteration_algorithmic_gbs"] = round(
-            16 * cells * 8 / ((k64["pspmv_dot"] + k64["update"]) * 1e-3) / 1e9, 1)
-
-    iters32 = float(np.mean([s["pcg_iterations"] for s in st32]))
     cpu = None
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        iters = res["fp64"]["pcg_iterations_per_step"] if "fp64" in res else iters32
+        iters = res["fp64"]["pcg_iterations_per_step"] if "fp64" in res else None
         try:
+            if iters is None:
+                raise RuntimeError("the CPU baseline scales the FP64 path's iteration count: run it (--paths ...,fp64)")
             cpu = cpu_baseline(a.scene, a.n, iters, dt, threads)
         except Exception as e:  # the baseline must not sink the GPU line
             cpu = {"error": str(e)}
+    h = res[head]
     out = {
         "metric": "ms/step at 256^3 two-phase, % HBM peak; FP32+refine vs FP64 at 1e-8 resid",
-        "value": round(ms32, 3),
+        "value": round(ms, 3),
         "unit": "ms/step",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms32, 3),
+        "ms_per_step": round(ms, 3),
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere",
+        "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere" if head != "fp64" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{a.scene} {a.n}^3, air_bubble_rising (oil 800/0.05 in water 1000/1e-3), "
-                               f"dt={dt:.4e} (CFL 0.5, |g|), projection to 1e-8 relative residual",
+                               f"dt={dt:.4e} (CFL 0.5, |g|), projection to 1e-8 relative residual, "
+                               f"{LABEL[head]} ({PATHS[head]})",
                    "cells": cells},
-        "cells_per_s": round(cells / (ms32 * 1e-3), 1),
-        "pcg_iterations_per_step": iters32,
-        "refinement_sweeps_per_step": float(np.mean([s["refinement_sweeps"] for s in st32])),
-        "residual_rel": st32[-1]["residual_rel"],
-        "projection_ms": float(np.mean([s["projection_ms"] for s in st32])),
-        "fp64_ms_per_step": ms64,
-        "speedup_fp32refine_vs_fp64": (round(ms64 / ms32, 3) if ms64 else None),
+        "path": LABEL[head],
+        "cells_per_s": round(cells / (ms * 1e-3), 1),
+        "pcg_iterations_per_step": h["pcg_iterations_per_step"],
+        "refinement_sweeps_per_step": h["refinement_sweeps_per_step"],
+        "residual_rel": h["residual_rel"],
+        "projection_ms": h["projection_ms"],
+        "fp64_ms_per_step": res["fp64"]["ms_per_step"] if "fp64" in res else None,
+        "speedup_vs_fp64": h.get("speedup_vs_fp64"),
         "roofline": roof,
         "cpu_baseline": cpu,
+        "paths": res,
     }
-    out.update({k: v for k, v in res.items()})
     print(json.dumps(out))
 
 

@@ -27,6 +27,8 @@ FS_SCHEDULE_FIXED = 1
 FS_SOURCE_NONE = 0
 FS_SOURCE_ADD = 1
 FS_SOURCE_DELETE = 2
+FS_KS_REINIT = 0
+FS_KS_MG_SMOOTH = 1
 
 ERROR_NAMES = {
     FS_ERR_BAD_ARG: "FS_ERR_BAD_ARG",
@@ -201,6 +203,7 @@ def bind(lib, prefix):
         lib.fs_time_pcg_kernels.argtypes = [vp, _I, _I, _PD]
         lib.fs_set_kernel_sampling.argtypes = [vp, _I]
         lib.fs_get_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(_I)]
+        lib.fs_get_frame_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
         lib.fs_last_error.argtypes = [vp]
         lib.fs_last_error.restype = C.c_char_p
         lib.fs_adjoint_projection.argtypes = [vp, _D, _PD, _PD, _PD, C.POINTER(FsStepStats)]
@@ -511,6 +514,16 @@ def kernel_samples(solver, precision):
     cnt = (_I * 3)()
     solver._check(solver.lib.fs_get_kernel_samples(solver.h, precision, _ptr(avg), cnt), "get_kernel_samples")
     return avg, np.array(list(cnt))
+
+
+def frame_kernel_samples(solver, kernel):
+    """Mean in-loop duration (ms), timed launches and all launches of a
+    frame kernel (FS_KS_REINIT, FS_KS_MG_SMOOTH) since sampling was enabled."""
+    avg = C.c_double()
+    n, launches = C.c_longlong(), C.c_longlong()
+    solver._check(solver.lib.fs_get_frame_kernel_samples(solver.h, kernel, C.byref(avg), C.byref(n),
+                                                         C.byref(launches)), "get_frame_kernel_samples")
+    return avg.value, n.value, launches.value
 
 
 def load(path, prefix):

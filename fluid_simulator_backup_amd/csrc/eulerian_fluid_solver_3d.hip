@@ -24,6 +24,7 @@
 #include "fs_kernels.h"
 #include "fs_layout.h"
 #include "fs_mg.h"
+#include "fs_probe.h"
 #include "fs_pcg.h"
 
 namespace {
@@ -58,7 +59,7 @@ class EulerianFluidSolver3D {
     hipEvent_t ev[4] = {};
 
     // persistent fields (padded layout, FP64)
-    double *phi = nullptr, *sghost = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
+    double *phi = nullptr, *sghost = nullptr, *sghost2 = nullptr, *u = nullptr, *v = nullptr, *w = nullptr;
     // one spare per component: each keeps its component's extents, so ghost
     // cells of every velocity buffer stay zero across the swaps
     double *spare_u = nullptr, *spare_v = nullptr, *spare_w = nullptr;
@@ -116,6 +117,10 @@ class EulerianFluidSolver3D {
     bool sampling = false;
     double ksum[2][3] = {};
     int kcount[2][3] = {};
+    // frame kernels sampled the same way: FS_KS_REINIT (the first sweep of
+    // each reinitialization call), FS_KS_MG_SMOOTH (the first level-0
+    // smoother launch of each V-cycle)
+    FsLaunchProbe kprobe[FS_KS_COUNT];
 
     // ---- tape of one fixed-schedule step (what its reverse pass reads) ----
     // Fields are drawn from one pool per component box (0 cells, 1..3 MAC)
@@ -190,6 +195,7 @@ class EulerianFluidSolver3D {
         HIPCHECK(hipStreamCreate(&st));
         for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
         for (auto& e : probe.ev) HIPCHECK(hipEventCreate(&e));
+        for (auto& k : kprobe) k.create();
         L = fs_make_layout(G.nx, G.ny, G.nz);
         cell.x = fs_make_axis(G.x_min, G.x_max, G.nx);
         cell.y = fs_make_axis(G.y_min, G.y_max, G.ny);
@@ -207,7 +213,7 @@ class EulerianFluidSolver3D {
                          "FP32 refinement on unequal spacings needs the fixed schedule: the compatible schedule's "
                          "matrix puts 1/dx^2 on the diagonal for every face, so constants are not in its null space"};
 
-        double** f64[] = {&phi, &sghost, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
+        double** f64[] = {&phi, &sghost, &sghost2, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
                           &bx, &by, &bz, &diag, &rhs};
         for (double** f : f64) *f = dfield();
         if (P.order_for_time_advancing > 1) {
@@ -268,7 +274,7 @@ class EulerianFluidSolver3D {
 
     ~EulerianFluidSolver3D() {
         if (st) (void)hipStreamSynchronize(st);
-        double* f64[] = {phi, sghost, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
+        double* f64[] = {phi, sghost, sghost2, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
                          scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk,
                          va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc};
         for (double* f : f64)
@@ -296,6 +302,7 @@ class EulerianFluidSolver3D {
             if (e) (void)hipEventDestroy(e);
         for (auto& e : probe.ev)
             if (e) (void)hipEventDestroy(e);
+        for (auto& k : kprobe) k.destroy();
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -764,13 +771,13 @@ class EulerianFluidSolver3D {
         const int nb = fs_mg_blocks(Lz);
         if (!mg_part) {
             mg_part = (double*)dalloc(sizeof(double) * 2 * (size_t)nb);
-            mg_tot = (double*)dalloc(sizeof(double) * 4);
+            mg_tot = (double*)dalloc(sizeof(double) * (4 + 2 * FS_MG_MID));
             HIPCHECK(hipHostMalloc((void**)&hmg_tot, sizeof(double) * 4));
         }
         const float ix = (float)cell.x.inv_d2, iy = (float)cell.y.inv_d2, iz = (float)cell.z.inv_d2;
         float *x = e32, *r = (float*)w32.r, *z = (float*)w32.s, *p = (float*)w32.p, *q = (float*)w32.q;
         auto total = [&](int slots) {
-            fs_mg_total(st, mg_part, nb, slots, mg_tot);
+            fs_mg_total(st, mg_part, nb, slots, mg_tot + 4, mg_tot);
             HIPCHECK(hipMemcpyAsync(hmg_tot, mg_tot, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
             sync();
         };
@@ -810,6 +817,7 @@ class EulerianFluidSolver3D {
     // settled through the pin, as in the Jacobi refinement) decides the stop.
     void SolvePressureMg() {
         mg.Init(L, cell.x.inv_d2, cell.y.inv_d2, cell.z.inv_d2);
+        mg.probe = sampling ? &kprobe[FS_KS_MG_SMOOTH] : nullptr;
         mg.SetOperator(st, bx32, by32, bz32);
         init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
         const double rt2 = P.rel_tolerance * P.rel_tolerance;
@@ -1244,17 +1252,43 @@ class EulerianFluidSolver3D {
         double fictious_dt = cell.x.d / P.scaling_number_for_reinitialization;
         fs_sign(st, phi, sgn, L, cell.x.d * cell.x.d);
         if (recording) tape.reinit.emplace_back();
-        for (int it = 0; it < P.iteration_number_for_reinitialization; it++) {
+        const int n = P.iteration_number_for_reinitialization;
+        // The sweeps ping-pong between two ghost copies: sweep `it` reads
+        // `src` and writes the next one's box (phi's on the last sweep), whose
+        // ghost layers are then filled; what each sweep reads is what the
+        // reference's copy-then-fill gives (ADVECTION_3D::ReinitializationBySussman,
+        // ADVECTION_3D.h:360-378).  The compatible schedule's fill reads the
+        // deeper layers' sources from phi's own ghost layers, as the
+        // reference's FillGhostCellsContinuousDerivativesFrom(phi) does.
+        // A recorded step keeps phi current after every sweep for the tape.
+        double *src = sghost, *nxt = sghost2;
+        for (int it = 0; it < n; it++) {
             if (recording) tape.reinit.back().push_back(Record(phi, 0));
-            // ADVECTION_3D::ReinitializationBySussman, ADVECTION_3D.h:360-378
-            if (fixed()) {
-                RefreshLevelsetGhost();
-            } else {
-                fs_copy_box(st, sghost, phi, L, box(0));
-                fs_fill_linear(st, sghost, phi, L, box(0), 3, 0);
+            if (it == 0 || recording) {
+                if (fixed()) {
+                    RefreshLevelsetGhost();
+                } else {
+                    fs_copy_box(st, sghost, phi, L, box(0));
+                    fs_fill_linear(st, sghost, phi, L, box(0), 3, 0);
+                }
             }
-            fs_reinit_sweep(st, phi, sghost, sgn, L, cell, P.levelset_epsilon, fictious_dt, fixed());
+            const bool last = it == n - 1 || recording;
+            const bool timed = sampling && it == 0;
+            if (timed) kprobe[FS_KS_REINIT].begin(st);
+            fs_reinit_sweep(st, last ? phi : nxt, src, sgn, L, cell, P.levelset_epsilon, fictious_dt, fixed());
+            if (timed) kprobe[FS_KS_REINIT].end(st);
+            if (!last) {
+                if (fixed())
+                    fs_fill_linear(st, nxt, nxt, L, box(0), 3, 0);
+                else
+                    fs_fill_linear_split(st, nxt, nxt, phi, L, box(0), 3);
+                std::swap(src, nxt);
+            }
         }
+        // the compatible schedule's level-set advection reads the ghost copy
+        // the last sweep read: keep it in sghost
+        if (src != sghost) std::swap(sghost, sghost2);
+        if (sampling) kprobe[FS_KS_REINIT].launches += n;
     }
 
     // EULERIAN_FLUID_SOLVER_3D::SolveForRK, :1453-1476; a non-converged
@@ -2035,11 +2069,26 @@ int fs_set_kernel_sampling(fs_solver* s, int enable) {
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.sampling = enable != 0;
         S.probe.recorded = false;
+        HIPCHECK(hipStreamSynchronize(S.st));
+        for (auto& k : S.kprobe) k.reset();
         for (int p = 0; p < 2; p++)
             for (int k = 0; k < 3; k++) {
                 S.ksum[p][k] = 0;
                 S.kcount[p][k] = 0;
             }
+        return FS_OK;
+    });
+}
+
+int fs_get_frame_kernel_samples(fs_solver* s, int kernel, double* avg_ms, long long* samples,
+                                long long* launches) {
+    if (!avg_ms || !samples || !launches || kernel < 0 || kernel >= FS_KS_COUNT) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        FsLaunchProbe& k = S.kprobe[kernel];
+        k.harvest();
+        *avg_ms = k.samples ? k.sum_ms / k.samples : 0;
+        *samples = k.samples;
+        *launches = k.launches;
         return FS_OK;
     });
 }

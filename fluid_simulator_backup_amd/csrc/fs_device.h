@@ -10,8 +10,38 @@
 
 #define FS_PI 3.14159265358979323846  // MACROS.h:13
 
+// Correctly rounded quotients without a division instruction sequence per
+// quotient.  With y = RN(1/d), q0 = RN(x y), the remainder x - d q0 taken by
+// one fma and q = RN(q0 + r y): Markstein's theorem (IBM J. Res. Dev. 34(1),
+// 1990; Muller et al., Handbook of Floating-Point Arithmetic, ch. on
+// division) gives q = RN(x/d) whenever q0 is within one ulp of x/d and
+// nothing underflows or overflows.
+//
+// fs_div3/fs_div6: d = 3 or 6, whose RN(1/d) has relative error 2^-54, so
+// q0 is within one ulp.  copysign restores the sign of a zero quotient
+// (the correction of -0 gives +0).  For finite x of magnitude above 2^-1000.
+__device__ __forceinline__ double fs_div_const(double x, double d, double y) {
+    double q0 = x * y;
+    double r = fma(-d, q0, x);
+    return copysign(fma(r, y, q0), x);
+}
+__device__ __forceinline__ double fs_div3(double x) { return fs_div_const(x, (double)3, (double)1 / 3); }
+__device__ __forceinline__ double fs_div6(double x) { return fs_div_const(x, (double)6, (double)1 / 6); }
+// a / b for a >= 0, b > 0 with y = RN(1/b) (an IEEE division): RN(a y) may
+// be off by up to 1.5 ulps, so one correction brings it within half an ulp
+// plus a term of order u^2 and a second one, now under Markstein's
+// hypothesis, rounds it correctly.
+__device__ __forceinline__ double fs_div_recip(double a, double b, double y) {
+    double q0 = a * y;
+    double q1 = fma(fma(-b, q0, a), y, q0);
+    return fma(fma(-b, q1, a), y, q1);
+}
+
 // WENO5-JS one-sided derivatives at c[0] from c[-3..3]
-// (ADVECTION_METHOD_3D.h:55-183).
+// (ADVECTION_METHOD_3D.h:55-183).  Bit for bit the reference's arithmetic:
+// its `m * (T)7 / 6` is (m * 7) / 6 and its `m * (T)1 / 3` is m / 3, both
+// correctly rounded divisions, taken here by fs_div3/fs_div6; the weights
+// a_k / sum share one reciprocal of the sum (fs_div_recip).
 __device__ __forceinline__ void fs_weno5(const double* c, double inv, double eps, double& dmo, double& dpo) {
     double m1 = inv * (c[-2] - c[-3]), m2 = inv * (c[-1] - c[-2]), m3 = inv * (c[0] - c[-1]);
     double m4 = inv * (c[1] - c[0]), m5 = inv * (c[2] - c[1]);
@@ -28,19 +58,21 @@ __device__ __forceinline__ void fs_weno5(const double* c, double inv, double eps
     double am1 = (double)1 / 10 * (double)1 / ((eps + sm1) * (eps + sm1));
     double am2 = (double)6 / 10 * (double)1 / ((eps + sm2) * (eps + sm2));
     double am3 = (double)3 / 10 * (double)1 / ((eps + sm3) * (eps + sm3));
-    double summ = am1 + am2 + am3;
-    double wm1 = am1 / summ, wm2 = am2 / summ, wm3 = am3 / summ;
+    double summ = am1 + am2 + am3, rsumm = (double)1 / summ;
+    double wm1 = fs_div_recip(am1, summ, rsumm), wm2 = fs_div_recip(am2, summ, rsumm);
+    double wm3 = fs_div_recip(am3, summ, rsumm);
     double ap1 = (double)1 / 10 * (double)1 / ((eps + sp1) * (eps + sp1));
     double ap2 = (double)6 / 10 * (double)1 / ((eps + sp2) * (eps + sp2));
     double ap3 = (double)3 / 10 * (double)1 / ((eps + sp3) * (eps + sp3));
-    double sump = ap1 + ap2 + ap3;
-    double wp1 = ap1 / sump, wp2 = ap2 / sump, wp3 = ap3 / sump;
-    dmo = wm1 * (m1 * (double)1 / 3 - m2 * (double)7 / 6 + m3 * (double)11 / 6) +
-          wm2 * (m2 * (-(double)1 / 6) + m3 * (double)5 / 6 + m4 * (double)1 / 3) +
-          wm3 * (m3 * (double)1 / 3 + m4 * (double)5 / 6 - m5 * (double)1 / 6);
-    dpo = wp1 * (p1 * (double)1 / 3 - p2 * (double)7 / 6 + p3 * (double)11 / 6) +
-          wp2 * (p2 * (-(double)1 / 6) + p3 * (double)5 / 6 + p4 * (double)1 / 3) +
-          wp3 * (p3 * (double)1 / 3 + p4 * (double)5 / 6 - p5 * (double)1 / 6);
+    double sump = ap1 + ap2 + ap3, rsump = (double)1 / sump;
+    double wp1 = fs_div_recip(ap1, sump, rsump), wp2 = fs_div_recip(ap2, sump, rsump);
+    double wp3 = fs_div_recip(ap3, sump, rsump);
+    dmo = wm1 * (fs_div3(m1) - fs_div6(m2 * 7) + fs_div6(m3 * 11)) +
+          wm2 * (m2 * (-(double)1 / 6) + fs_div6(m3 * 5) + fs_div3(m4)) +
+          wm3 * (fs_div3(m3) + fs_div6(m4 * 5) - fs_div6(m5));
+    dpo = wp1 * (fs_div3(p1) - fs_div6(p2 * 7) + fs_div6(p3 * 11)) +
+          wp2 * (p2 * (-(double)1 / 6) + fs_div6(p3 * 5) + fs_div3(p4)) +
+          wp3 * (fs_div3(p3) + fs_div6(p4 * 5) - fs_div6(p5));
 }
 
 // ENO3 one-sided derivatives (ADVECTION_METHOD_3D.h:1653-1718).

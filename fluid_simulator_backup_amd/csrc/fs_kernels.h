@@ -18,6 +18,10 @@ void fs_copy_box(hipStream_t st, double* dst, const double* src, const FsLayout&
 void fs_fill_zero_faces(hipStream_t st, double* dst, const FsLayout& L, FsBox b, int depth);
 void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth);
 void fs_fill_linear(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth, int edges);
+// the face layers of fs_fill_linear with the box's values read from src and
+// the ghost layers' from gsrc (fs_fill_linear: gsrc == src)
+void fs_fill_linear_split(hipStream_t st, double* dst, const double* src, const double* gsrc, const FsLayout& L,
+                          FsBox b, int depth);
 
 // ---- advection ------------------------------------------------------------
 void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
@@ -101,7 +105,9 @@ void fs_zero_walls(hipStream_t st, int comp, double* vel, const FsLayout& L);
 void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, double dx2);
 // exact: the Euclidean gradient norm in every Godunov case (the reference's
 // misplaced parenthesis of ADVECTION_METHOD_3D.h:1605 otherwise)
-void fs_reinit_sweep(hipStream_t st, double* phi, const double* ghost, const double* sgn, const FsLayout& L,
+// One sweep from the ghost copy `ghost` (its box holds phi) into out's box;
+// out may be phi or another field, not ghost.
+void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const double* sgn, const FsLayout& L,
                      const FsGridConst& cell, double eps, double dt, int exact);
 
 // ---- misc -----------------------------------------------------------------

@@ -82,23 +82,25 @@ void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayou
 // FillGhostCellsContinuousDerivativesFrom face blocks,
 // FIELD_STRUCTURE_3D.h:1806-1894: 2a-b outwards, deeper layers reading the
 // layers just written when src == dst (one thread walks its own row).
-__global__ void k_linear_faces(double* dst, const double* src, FsLayout L, FsBox b, int d, int axis) {
+__global__ void k_linear_faces(double* dst, const double* src, const double* gsrc, FsLayout L, FsBox b, int d,
+                               int axis) {
+    // S(x): src inside the box along the axis, gsrc in its ghost layers
     int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
     if (axis == 0) {
         if (a >= b.nj) return;
-        for (int i = -1; i >= -d; i--) dst[L.idx(i, a, c)] = (double)2 * src[L.idx(i + 1, a, c)] - src[L.idx(i + 2, a, c)];
-        for (int i = b.ni; i <= b.ni - 1 + d; i++)
-            dst[L.idx(i, a, c)] = (double)2 * src[L.idx(i - 1, a, c)] - src[L.idx(i - 2, a, c)];
+        auto S = [&](int i) { return (i >= 0 && i < b.ni ? src : gsrc)[L.idx(i, a, c)]; };
+        for (int i = -1; i >= -d; i--) dst[L.idx(i, a, c)] = (double)2 * S(i + 1) - S(i + 2);
+        for (int i = b.ni; i <= b.ni - 1 + d; i++) dst[L.idx(i, a, c)] = (double)2 * S(i - 1) - S(i - 2);
     } else if (axis == 1) {
         if (a >= b.ni) return;
-        for (int j = -1; j >= -d; j--) dst[L.idx(a, j, c)] = (double)2 * src[L.idx(a, j + 1, c)] - src[L.idx(a, j + 2, c)];
-        for (int j = b.nj; j <= b.nj - 1 + d; j++)
-            dst[L.idx(a, j, c)] = (double)2 * src[L.idx(a, j - 1, c)] - src[L.idx(a, j - 2, c)];
+        auto S = [&](int j) { return (j >= 0 && j < b.nj ? src : gsrc)[L.idx(a, j, c)]; };
+        for (int j = -1; j >= -d; j--) dst[L.idx(a, j, c)] = (double)2 * S(j + 1) - S(j + 2);
+        for (int j = b.nj; j <= b.nj - 1 + d; j++) dst[L.idx(a, j, c)] = (double)2 * S(j - 1) - S(j - 2);
     } else {
         if (a >= b.ni) return;
-        for (int k = -1; k >= -d; k--) dst[L.idx(a, c, k)] = (double)2 * src[L.idx(a, c, k + 1)] - src[L.idx(a, c, k + 2)];
-        for (int k = b.nk; k <= b.nk - 1 + d; k++)
-            dst[L.idx(a, c, k)] = (double)2 * src[L.idx(a, c, k - 1)] - src[L.idx(a, c, k - 2)];
+        auto S = [&](int k) { return (k >= 0 && k < b.nk ? src : gsrc)[L.idx(a, c, k)]; };
+        for (int k = -1; k >= -d; k--) dst[L.idx(a, c, k)] = (double)2 * S(k + 1) - S(k + 2);
+        for (int k = b.nk; k <= b.nk - 1 + d; k++) dst[L.idx(a, c, k)] = (double)2 * S(k - 1) - S(k - 2);
     }
 }
 
@@ -123,10 +125,15 @@ __global__ void k_linear_edges(double* dst, const double* src, FsLayout L, FsBox
 
 void fs_fill_linear(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth,
                     int edges) {
-    hipLaunchKernelGGL(k_linear_faces, dim3((b.nj + 63) / 64, b.nk), dim3(64), 0, st, dst, src, L, b, depth, 0);
-    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nk), dim3(64), 0, st, dst, src, L, b, depth, 1);
-    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nj), dim3(64), 0, st, dst, src, L, b, depth, 2);
+    fs_fill_linear_split(st, dst, src, src, L, b, depth);
     if (edges) hipLaunchKernelGGL(k_linear_edges, dim3((b.nk + 63) / 64), dim3(64), 0, st, dst, src, L, b, depth);
+}
+
+void fs_fill_linear_split(hipStream_t st, double* dst, const double* src, const double* gsrc, const FsLayout& L,
+                          FsBox b, int depth) {
+    hipLaunchKernelGGL(k_linear_faces, dim3((b.nj + 63) / 64, b.nk), dim3(64), 0, st, dst, src, gsrc, L, b, depth, 0);
+    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nk), dim3(64), 0, st, dst, src, gsrc, L, b, depth, 1);
+    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nj), dim3(64), 0, st, dst, src, gsrc, L, b, depth, 2);
 }
 
 // ---------------------------------------------------------------------------
@@ -1202,7 +1209,7 @@ void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, 
 // parenthesis of :1463/:1605 (x shock, y shock towards +, z expansion; the
 // Euclidean norm instead when `exact` is set), and the sign-preserving clamp
 // (:1624-1634).
-__global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, const double* __restrict__ gh,
+__global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ out, const double* __restrict__ gh,
                                                    const double* __restrict__ sgn, FsLayout L, double ix, double iy,
                                                    double iz, double eps, double dt, int exact) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
@@ -1216,7 +1223,7 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, con
     gather7(gh, c, L.sy, f);
     fs_weno5(f + 3, iz, eps, mz, pz);
     double sg = sgn[c];
-    double rho = phi[c];
+    double rho = gh[c];  // the ghost copy's box is phi
     int cx = fs_godunov_axis(sg, mx, px, X);
     int cy = cx ? fs_godunov_axis(sg, my, py, Y) : FS_AX_NONE;
     int cz = cy ? fs_godunov_axis(sg, mz, pz, Z) : FS_AX_NONE;
@@ -1232,12 +1239,12 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ phi, con
         if (sg < 0) rho = -eps;
         if (sg > 0) rho = eps;
     }
-    phi[c] = rho;
+    out[c] = rho;
 }
 
-void fs_reinit_sweep(hipStream_t st, double* phi, const double* ghost, const double* sgn, const FsLayout& L,
+void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const double* sgn, const FsLayout& L,
                      const FsGridConst& cell, double eps, double dt, int exact) {
-    hipLaunchKernelGGL(k_reinit, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, ghost, sgn, L, cell.x.inv,
+    hipLaunchKernelGGL(k_reinit, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, out, ghost, sgn, L, cell.x.inv,
                        cell.y.inv, cell.z.inv, eps, dt, exact);
 }
 

@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "fs_layout.h"
+#include "fs_probe.h"
 
 struct FsMgLevel {
     FsLayout L;
@@ -45,6 +46,9 @@ class FsMg {
     // (zero ghosts).
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
+    // When set, the first level-0 smoother launch of every V-cycle is timed
+    // and every level-0 smoother launch counted (measurement only).
+    FsLaunchProbe* probe = nullptr;
 
    private:
     void Cycle(hipStream_t st, int l);
@@ -66,5 +70,7 @@ void fs_mg_update(hipStream_t st, const FsLayout& L, float* x, float* r, const f
 void fs_mg_dot2(hipStream_t st, const FsLayout& L, const float* a, const float* b, double* part);
 // p = z + beta p
 void fs_mg_direction(hipStream_t st, const FsLayout& L, const float* z, float* p, float beta);
-// out[s] = sum of part[s*nb .. (s+1)*nb) in order, for s < slots
-void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* out);
+// out[s] = sum of part[s*nb .. (s+1)*nb) in a fixed order, for s < slots;
+// mid holds slots * FS_MG_MID doubles of intermediate sums
+#define FS_MG_MID 128
+void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* mid, double* out);

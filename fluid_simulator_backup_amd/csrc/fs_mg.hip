@@ -306,6 +306,26 @@ __global__ void __launch_bounds__(MNT) k_mg_direction(FsLayout L, const float* _
     p[c] = z[c] + beta * p[c];
 }
 
+// Sums of `slots` runs of nb partials in two launches, each in a fixed
+// order: workgroup (b, s) sums its contiguous share of run s into
+// mid[s * FS_MG_MID + b]; then one workgroup per run sums those.
+__global__ void __launch_bounds__(MNT) k_mg_total_mid(const double* part, int nb, double* mid) {
+    __shared__ double sh[MNT];
+    const int t = threadIdx.x, b = blockIdx.x;
+    const int share = (nb + FS_MG_MID - 1) / FS_MG_MID;
+    const double* src = part + (long)blockIdx.y * nb;
+    const int m0 = b * share, m1 = min(nb, m0 + share);
+    double s = 0;
+    for (int m = m0 + t; m < m1; m += MNT) s += src[m];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = MNT / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    if (t == 0) mid[blockIdx.y * FS_MG_MID + b] = sh[0];
+}
+
 __global__ void __launch_bounds__(MNT) k_mg_total(const double* part, int nb, double* out) {
     __shared__ double sh[MNT];
     const int t = threadIdx.x;
@@ -396,9 +416,14 @@ void FsMg::Cycle(hipStream_t st, int l) {
                            v.b, v.x, MG_COARSE_SWEEPS);
         return;
     }
-    for (int s = 0; s < 2 * MG_NU; s++)  // red, black, ...
+    for (int s = 0; s < 2 * MG_NU; s++) {  // red, black, ...
+        const bool timed = probe && l == 0 && s == 0;
+        if (timed) probe->begin(st);
         hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b, v.x,
                            s & 1);
+        if (timed) probe->end(st);
+    }
+    if (probe && l == 0) probe->launches += 4 * MG_NU;
     hipLaunchKernelGGL(k_mg_residual, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x, v.r);
     FsMgLevel& c = lv_[l + 1];
     hipLaunchKernelGGL(k_mg_restrict, cells_grid(c.L), t, 0, st, v.L, v.r, c.L, c.b, c.x);
@@ -442,6 +467,7 @@ void fs_mg_direction(hipStream_t st, const FsLayout& L, const float* z, float* p
     hipLaunchKernelGGL(k_mg_direction, cells_grid(L), dim3(MBX, MBY), 0, st, L, z, p, beta);
 }
 
-void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* out) {
-    hipLaunchKernelGGL(k_mg_total, dim3(slots), dim3(MNT), 0, st, part, nb, out);
+void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* mid, double* out) {
+    hipLaunchKernelGGL(k_mg_total_mid, dim3(FS_MG_MID, slots), dim3(MNT), 0, st, part, nb, mid);
+    hipLaunchKernelGGL(k_mg_total, dim3(slots), dim3(MNT), 0, st, mid, FS_MG_MID, out);
 }

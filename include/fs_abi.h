@@ -281,6 +281,19 @@ int  fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms);
 int  fs_set_kernel_sampling(fs_solver* s, int enable);
 int  fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* count);
 
+/* Frame kernels sampled while fs_set_kernel_sampling is on: FS_KS_REINIT
+ * times the first sweep of each reinitialization call (k_reinit) and counts
+ * every sweep; FS_KS_MG_SMOOTH times the first finest-level smoother launch
+ * of each V-cycle of the FS_FP32_MG solve (k_mg_smooth) and counts every
+ * finest-level smoother launch.  Returns the mean duration (ms), the number
+ * of timed launches and the number of launches since sampling was enabled.
+ * No reference counterpart: measurement hooks beside fs_time_pcg_kernels. */
+#define FS_KS_REINIT 0
+#define FS_KS_MG_SMOOTH 1
+#define FS_KS_COUNT 2
+int  fs_get_frame_kernel_samples(fs_solver* s, int kernel, double* avg_ms, long long* samples,
+                                 long long* launches);
+
 /* ---- adjoints (no reference counterpart) ---------------------------------
  * Reverse-mode pieces of the timestep for gradients through the GPU path.
  *

@@ -347,6 +347,32 @@ def test_multigrid_frames(hip_lib, oracle_lib):
         assert e < 1e-8, (c, e)
 
 
+def test_frame_kernel_sampling_leaves_frames_unchanged(hip_lib):
+    """The in-loop probes of the reinitialization sweep and the finest-level
+    smoother time one launch per call, count every launch, and change no
+    result: the sampled frames equal unsampled ones bit for bit."""
+    from fluid_simulator_backup_amd import scenes
+    phi, vel, P, g = scenes.make("rising_drop", 20, precision=_abi.FS_FP32_MG, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
+                                 rel_tolerance=1e-8)
+    out = []
+    for sampling in (False, True):
+        s = _abi.Solver(hip_lib, "fs_", P, g, 0)
+        s.set_fields(phi, *vel)
+        dt = s.cfl_dt(0.5, 1)
+        _abi.set_kernel_sampling(s, sampling)
+        for _ in range(2):
+            s.advance_frame(dt)
+        if sampling:
+            avg, n, launches = _abi.frame_kernel_samples(s, _abi.FS_KS_REINIT)
+            assert avg > 0 and n >= 2 and launches == n * P.iteration_number_for_reinitialization
+            avg, n, launches = _abi.frame_kernel_samples(s, _abi.FS_KS_MG_SMOOTH)
+            assert avg > 0 and n >= 2 and launches % 4 == 0 and launches >= 4 * n
+        out.append(s.get_fields())
+        s.close()
+    for c in out[0]:
+        assert np.array_equal(out[0][c], out[1][c]), c
+
+
 @pytest.mark.parametrize("name,n,frames", [("rising_drop", 24, 4), ("dam_break", 24, 4)])
 def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
     """The reference's absolute stop rule at its script tolerance (1e-5):
