@@ -13,8 +13,10 @@
 // prolongation is trilinear with clamped (zero-gradient) indices at the walls.
 // Smoother: red-black Gauss-Seidel, red then black before the coarse
 // correction and black then red after it, so the V-cycle is a symmetric
-// operator; the coarsest level (every axis below 8 cells) is smoothed by one
-// workgroup in a single launch.
+// operator.  Each sweep is one launch that updates both colours (the first
+// colour's values held in LDS for three planes), out of place between x and
+// the level's residual buffer.  The coarsest level (at most 512 cells) is
+// swept in LDS by one workgroup in a single launch.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -46,13 +48,16 @@ class FsMg {
     // (zero ghosts).
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
-    // When set, the first level-0 smoother launch of every V-cycle is timed
-    // and every level-0 smoother launch counted (measurement only).
+    // When set, the first finest-level smoother launch (one red-black sweep)
+    // of every V-cycle is timed and every finest-level smoother launch
+    // counted (measurement only).
     FsLaunchProbe* probe = nullptr;
 
    private:
     void Cycle(hipStream_t st, int l);
+    void Smooth(hipStream_t st, int l, int c1);
     std::vector<FsMgLevel> lv_;
+    bool split_ = false;
     float* diag0_ = nullptr;
 };
 

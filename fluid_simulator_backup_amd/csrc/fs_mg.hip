@@ -4,6 +4,7 @@
 #include "fs_mg.h"
 
 #include <algorithm>
+#include <cstdlib>
 #include <stdexcept>
 
 #define MBX 64
@@ -222,31 +223,140 @@ __global__ void __launch_bounds__(MNT) k_mg_prolong(FsLayout Lc, const float* __
     xf[Lf.idx(i, j, k)] += s;
 }
 
-// coarsest level: red-black sweeps (red first) then the mirrored sweeps, one
-// workgroup, x from 0
+// coarsest level (at most MG_COARSE_CELLS cells): red-black sweeps (red
+// first) then the mirrored sweeps, one workgroup, x from 0.  The level is
+// copied into LDS in a compact layout with one ghost layer (ghost x zero,
+// wall faces' coefficients zero), swept there and x written back.
+#define MG_CPAD 2400  // (nx+2)(ny+2)(nz+2) of the coarsest level at most (Init coarsens until it fits)
 __global__ void __launch_bounds__(MNT) k_mg_coarse(FsLayout L, const float* __restrict__ bx,
                                                    const float* __restrict__ by, const float* __restrict__ bz,
                                                    float ix, float iy, float iz, const float* __restrict__ diag,
                                                    const float* __restrict__ b, float* __restrict__ x, int sweeps) {
+    __shared__ float sx[MG_CPAD], sbx[MG_CPAD], sby[MG_CPAD], sbz[MG_CPAD], sb[MG_CPAD], sd[MG_CPAD];
     const int t = threadIdx.y * MBX + threadIdx.x;
+    const int px = L.nx + 2, py = L.ny + 2, pxy = px * py, ptot = pxy * (L.nz + 2);
+    for (int m = t; m < ptot; m += MNT) {
+        sx[m] = 0.f;
+        sbx[m] = sby[m] = sbz[m] = sb[m] = sd[m] = 0.f;
+    }
+    __syncthreads();
     const int n = L.nx * L.ny * L.nz;
     for (int m = t; m < n; m += MNT) {
         int i = m % L.nx, j = (m / L.nx) % L.ny, k = m / (L.nx * L.ny);
-        x[L.idx(i, j, k)] = 0.f;
+        long c = L.idx(i, j, k);
+        int q = (i + 1) + px * (j + 1) + pxy * (k + 1);
+        sbx[q] = bx[c];
+        sby[q] = by[c];
+        sbz[q] = bz[c];
+        sb[q] = b[c];
+        sd[q] = diag[c];
+        // the far wall faces belong to the ghost cell beyond
+        if (i == L.nx - 1) sbx[q + 1] = bx[c + 1];
+        if (j == L.ny - 1) sby[q + px] = by[c + L.sx];
+        if (k == L.nz - 1) sbz[q + pxy] = bz[c + L.sy];
     }
     __syncthreads();
+    // thread t owns the t-th cell of each colour in index order (at most
+    // MNT cells of a colour: n <= MG_COARSE_CELLS = 2 MNT); -1 if none
+    __shared__ int qcol[2][MNT];
+    for (int m = t; m < 2 * MNT; m += MNT) qcol[m / MNT][m % MNT] = -1;
+    __syncthreads();
+    if (t == 0) {
+        int cnt[2] = {0, 0};
+        for (int k = 0; k < L.nz; k++)
+            for (int j = 0; j < L.ny; j++)
+                for (int i = 0; i < L.nx; i++) {
+                    int h = (i + j + k) & 1;
+                    qcol[h][cnt[h]++] = (i + 1) + px * (j + 1) + pxy * (k + 1);
+                }
+    }
+    __syncthreads();
+    const int qr = qcol[0][t], qb = qcol[1][t];
     for (int s = 0; s < 4 * sweeps; s++) {
         // forward sweeps red, black; then backward black, red
         const int half = s < 2 * sweeps ? (s & 1) : 1 - (s & 1);
-        for (int m = t; m < n; m += MNT) {
-            int i = m % L.nx, j = (m / L.nx) % L.ny, k = m / (L.nx * L.ny);
-            if (((i + j + k) & 1) != half) continue;
-            long c = L.idx(i, j, k);
-            float d = diag[c];
-            if (d == 0.f) continue;
-            float v = b[c] + bx[c] * (ix * x[c - 1]) + bx[c + 1] * (ix * x[c + 1]) + by[c] * (iy * x[c - L.sx]) +
-                      by[c + L.sx] * (iy * x[c + L.sx]) + bz[c] * (iz * x[c - L.sy]) + bz[c + L.sy] * (iz * x[c + L.sy]);
-            x[c] = v / d;
+        const int q = half ? qb : qr;
+        if (q >= 0) {
+            float d = sd[q];
+            if (d != 0.f) {
+                float v = sb[q] + sbx[q] * (ix * sx[q - 1]) + sbx[q + 1] * (ix * sx[q + 1]) +
+                          sby[q] * (iy * sx[q - px]) + sby[q + px] * (iy * sx[q + px]) + sbz[q] * (iz * sx[q - pxy]) +
+                          sbz[q + pxy] * (iz * sx[q + pxy]);
+                sx[q] = v / d;
+            }
+        }
+        __syncthreads();
+    }
+    for (int m = t; m < n; m += MNT) {
+        int i = m % L.nx, j = (m / L.nx) % L.ny, k = m / (L.nx * L.ny);
+        x[L.idx(i, j, k)] = sx[(i + 1) + px * (j + 1) + pxy * (k + 1)];
+    }
+}
+
+// One symmetric-order red-black Gauss-Seidel sweep, out of place: the cells
+// of colour c1 ((i+j+k) % 2 == c1) from x_in, then the other colour from
+// those, written with them to x_out; the same arithmetic per cell as two
+// k_mg_smooth launches, the diagonal formed from the coefficients as
+// k_mg_diag forms it.  A workgroup owns a 64 x 16 column tile and marches
+// up SKC planes; the first colour's new values on the tile and a ring of one
+// cell around it go to LDS for three planes (k-1, k, k+1), so x is read
+// once and the coefficients serve both colours.
+#define SBX 64
+#define SBY 16
+#define SRX (SBX + 2)
+#define SRY (SBY + 2)
+__global__ void __launch_bounds__(SBX* SBY) k_mg_rb_sweep(FsLayout L, const float* __restrict__ bx,
+                                                       const float* __restrict__ by, const float* __restrict__ bz,
+                                                       float ix, float iy, float iz, const float* __restrict__ b,
+                                                       const float* __restrict__ x_in, float* __restrict__ x_out,
+                                                       int c1, int kc) {
+    __shared__ float R[3][SRY * SRX];
+    const int tx = threadIdx.x, ty = threadIdx.y, t = ty * SBX + tx;
+    const int i0 = blockIdx.x * SBX, j0 = blockIdx.y * SBY, k0 = blockIdx.z * kc;
+    const int k1 = min(L.nz, k0 + kc);
+    // first colour at plane kk on the tile and its ring (0 outside the box)
+    // first colour at plane kk on the tile and its ring (0 outside the box)
+    auto stage = [&](int kk) {
+        float* Rp = R[(kk + 3) % 3];
+        for (int m = t; m < SRX * SRY; m += SBX * SBY) {
+            int i = i0 - 1 + m % SRX, j = j0 - 1 + m / SRX;
+            float val = 0.f;
+            if (i >= 0 && i < L.nx && j >= 0 && j < L.ny && kk >= 0 && kk < L.nz) {
+                long c = L.idx(i, j, kk);
+                val = x_in[c];
+                if (((i + j + kk) & 1) == c1) {
+                    float d = ix * (bx[c] + bx[c + 1]) + iy * (by[c] + by[c + L.sx]) + iz * (bz[c] + bz[c + L.sy]);
+                    if (d != 0.f) {
+                        float v = b[c] + bx[c] * (ix * x_in[c - 1]) + bx[c + 1] * (ix * x_in[c + 1]) +
+                                  by[c] * (iy * x_in[c - L.sx]) + by[c + L.sx] * (iy * x_in[c + L.sx]) +
+                                  bz[c] * (iz * x_in[c - L.sy]) + bz[c + L.sy] * (iz * x_in[c + L.sy]);
+                        val = v / d;
+                    }
+                }
+            }
+            Rp[m] = val;
+        }
+    };
+    stage(k0 - 1);
+    stage(k0);
+    const int i = i0 + tx, j = j0 + ty;
+    const bool own = i < L.nx && j < L.ny;
+    const int q = (ty + 1) * SRX + tx + 1;
+    for (int k = k0; k < k1; k++) {
+        // the second colour's operands at plane k, loaded before the staging
+        // of plane k+1 so both sets of loads are in flight together
+        const long c = L.idx(min(i, L.nx - 1), min(j, L.ny - 1), k);
+        const float x0 = x_in[c], bc = b[c];
+        const float cw = bx[c], ce = bx[c + 1], cs = by[c], cn = by[c + L.sx], cb = bz[c], ct = bz[c + L.sy];
+        stage(k + 1);
+        __syncthreads();
+        if (own) {
+            const float *Rm = R[(k + 2) % 3], *R0 = R[k % 3], *Rp = R[(k + 1) % 3];
+            const float d = ix * (cw + ce) + iy * (cs + cn) + iz * (cb + ct);
+            const float v = bc + cw * (ix * R0[q - 1]) + ce * (ix * R0[q + 1]) + cs * (iy * R0[q - SRX]) +
+                            cn * (iy * R0[q + SRX]) + cb * (iz * Rm[q]) + ct * (iz * Rp[q]);
+            const float r = v / (d != 0.f ? d : 1.f);
+            x_out[c] = ((i + j + k) & 1) == c1 ? R0[q] : (d != 0.f ? r : x0);
         }
         __syncthreads();
     }
@@ -373,7 +483,11 @@ void FsMg::Init(const FsLayout& L0, double idx2, double idy2, double idz2) {
     diag0_ = field(L0);
     v.diag = diag0_;
     lv_.push_back(v);
-    while ((long)lv_.back().L.nx * lv_.back().L.ny * lv_.back().L.nz > MG_COARSE_CELLS) {
+    auto too_big = [](const FsLayout& L) {
+        return (long)L.nx * L.ny * L.nz > MG_COARSE_CELLS || (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2) > MG_CPAD;
+    };
+    split_ = std::getenv("FS_MG_SPLIT_SMOOTH") != nullptr;
+    while (too_big(lv_.back().L)) {
         const FsMgLevel& f = lv_.back();
         FsMgLevel c{};
         c.L = fs_make_layout((f.L.nx + 1) / 2, (f.L.ny + 1) / 2, (f.L.nz + 1) / 2);
@@ -408,6 +522,43 @@ void FsMg::SetOperator(hipStream_t st, const float* bx, const float* by, const f
     }
 }
 
+// MG_NU red-black sweeps on level l, colour c1 first in each: fused sweeps
+// ping-pong between x and the level's residual buffer (free until the
+// residual is formed, and again after the restriction), ending in x; with
+// FS_MG_SPLIT_SMOOTH set, two in-place half sweeps per sweep instead (the
+// same values).
+void FsMg::Smooth(hipStream_t st, int l, int c1) {
+    FsMgLevel& v = lv_[l];
+    const bool timed = probe && l == 0 && c1 == 0;
+    if (split_) {
+        const dim3 g = cells_grid(v.L), t(MBX, MBY);
+        for (int s = 0; s < 2 * MG_NU; s++) {
+            if (timed && s == 0) probe->begin(st);
+            hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b,
+                               v.x, (s & 1) ^ c1);
+            if (timed && s == 0) probe->end(st);
+        }
+        if (probe && l == 0) probe->launches += 2 * MG_NU;
+        return;
+    }
+    static_assert(MG_NU % 2 == 0, "fused sweeps ping-pong and must end in x");
+    const int gx = (v.L.nx + SBX - 1) / SBX, gy = (v.L.ny + SBY - 1) / SBY;
+    // planes per workgroup: 16 on large levels, fewer where that leaves
+    // under 1024 workgroups, at least 4 (or nz)
+    int kc = (int)std::min<long>(16, std::max<long>(4, (long)v.L.nz * gx * gy / 1024));
+    kc = std::min(kc, v.L.nz);
+    const dim3 g(gx, gy, (v.L.nz + kc - 1) / kc), t(SBX, SBY);
+    float *in = v.x, *out = v.r;
+    for (int s = 0; s < MG_NU; s++) {
+        if (timed && s == 0) probe->begin(st);
+        hipLaunchKernelGGL(k_mg_rb_sweep, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, in, out, c1,
+                           kc);
+        if (timed && s == 0) probe->end(st);
+        std::swap(in, out);
+    }
+    if (probe && l == 0) probe->launches += MG_NU;
+}
+
 void FsMg::Cycle(hipStream_t st, int l) {
     FsMgLevel& v = lv_[l];
     const dim3 g = cells_grid(v.L), t(MBX, MBY);
@@ -416,22 +567,13 @@ void FsMg::Cycle(hipStream_t st, int l) {
                            v.b, v.x, MG_COARSE_SWEEPS);
         return;
     }
-    for (int s = 0; s < 2 * MG_NU; s++) {  // red, black, ...
-        const bool timed = probe && l == 0 && s == 0;
-        if (timed) probe->begin(st);
-        hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b, v.x,
-                           s & 1);
-        if (timed) probe->end(st);
-    }
-    if (probe && l == 0) probe->launches += 4 * MG_NU;
+    Smooth(st, l, 0);  // red, black, ...
     hipLaunchKernelGGL(k_mg_residual, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x, v.r);
     FsMgLevel& c = lv_[l + 1];
     hipLaunchKernelGGL(k_mg_restrict, cells_grid(c.L), t, 0, st, v.L, v.r, c.L, c.b, c.x);
     Cycle(st, l + 1);
     hipLaunchKernelGGL(k_mg_prolong, g, t, 0, st, c.L, c.x, v.L, v.x);
-    for (int s = 0; s < 2 * MG_NU; s++)  // ..., black, red
-        hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b, v.x,
-                           1 - (s & 1));
+    Smooth(st, l, 1);  // ..., black, red
 }
 
 void FsMg::Apply(hipStream_t st, const float* r, float* z) {
