@@ -80,13 +80,34 @@ def measured_traffic(scene, n, kernel):
     dispatch from rocprofv3 PMC runs of this bench, scaled by the counters'
     reading of a 1 GiB copy), when a profile of this scene and size is
     committed under profiles/; else None."""
-    path = os.path.join(ROOT, "profiles", f"pcg_traffic_{scene}_{n}.json")
+    path = os.path.join(ROOT, "profiles", f"traffic_{scene}_{n}.json")
     try:
         with open(path) as f:
             k = json.load(f)["kernels"][kernel]
         return int(k["read_bytes"] + k["write_bytes"])
     except (OSError, KeyError, ValueError):
         return None
+
+
+# VALU issue peak of the MI355X: 256 CUs x 4 SIMDs, one 64-lane FP32 or
+# FP64 instruction per SIMD every 4 cycles at 2.4 GHz
+VALU_PEAK_WAVE_INSTR_PER_S = 256 * 4 * 2.4e9 / 4
+
+
+def measured_valu(scene, n, kernel, avg_ms):
+    """VALU issue rate of `kernel` from its SQ_INSTS_VALU / SQ_WAVES counts
+    per dispatch (rocprofv3 PMC run of this bench, committed under
+    profiles/) over the live average launch time; None without a profile."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{scene}_{n}.json")
+    try:
+        with open(path) as f:
+            k = json.load(f)["kernels"][kernel]
+        ipw, waves = float(k["valu_instructions_per_wave"]), float(k["waves_per_dispatch"])
+    except (OSError, KeyError, ValueError):
+        return None
+    rate = ipw * waves / (avg_ms * 1e-3)
+    return {"instructions_per_wave": ipw, "waves": waves, "achieved_wave_instr_per_s": round(rate, -6),
+            "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
 
 
 def divergent_velocity(n):
@@ -386,15 +407,20 @@ def pcg_roofline(k, cnt, k_rep, cells, n, wb):
 
 # Frame kernels sampled in the loop: algorithmic and moved bytes per cell.
 # k_reinit, one Sussman sweep in FP64: read phi^n (its 19-point stencil) and
-# the sign field, write phi^{n+1} (3 words); it reads the ghost copy and phi
-# itself (4 words moved).  k_mg_smooth, one red-black half sweep on the
-# finest level in FP32: read every face coefficient (each face has one cell
-# of the colour being updated: 3 words per cell), and for the half of the
-# cells of that colour b, the diagonal and the new x written, plus x of the
-# other half read (2 words per cell): 5; moved, whole cache lines of the
-# three coefficient fields, D, b, x read and x written: 7.
-FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 3 * 8, 4 * 8),
-                 "k_mg_smooth": ("FS_KS_MG_SMOOTH", 5 * 4, 7 * 4)}
+# the sign field, write phi^{n+1} (3 words); it reads the ghost copy, the
+# sign field and writes the next copy's box (3 words moved), then a small
+# kernel fills the copy's ghost layers.  k_mg_rb_sweep, one red-black
+# Gauss-Seidel sweep of the finest multigrid level in FP32: read x, b and
+# the three face-coefficient fields, write x (6 words; the diagonal is
+# formed from the coefficients); the same 6 words moved, plus the one-cell
+# ring of the first colour's operands that neighbouring tiles also read.
+FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 3 * 8, 3 * 8),
+                 "k_mg_rb_sweep": ("FS_KS_MG_SMOOTH", 6 * 4, 6 * 4)}
+# what bounds each, for the roofline note
+LIMITER = {"k_reinit": "FP64 VALU issue: the WENO5 smoothness weights take 24 IEEE divisions per cell and every "
+                       "operation keeps the reference's order, so the kernel runs at its instruction rate, far "
+                       "below the HBM bound (see the VALU counter summary under profiles/)",
+           "k_mg_rb_sweep": "HBM and load latency of the staged first colour"}
 
 
 def main():
@@ -450,6 +476,7 @@ def main():
         achieved = bytes_launch / (avg * 1e-3) / 1e9
         roof = {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(a.scene, a.n, kn),
+                "limiter": LIMITER[kn], "valu": measured_valu(a.scene, a.n, kn, avg),
                 "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg, 5),
                 "timing": f"HIP events on the solver stream around one launch per call of the timed frames "
                           f"({n_s} samples of {launches} launches)",

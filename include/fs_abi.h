@@ -43,8 +43,10 @@ extern "C" {
 #define FS_FP64          0   /* Jacobi-PCG in FP64 (PCG_METHOD.cpp:4-126) */
 #define FS_FP32_REFINE   1   /* FP32 Jacobi-PCG inside FP64 iterative refinement */
 #define FS_FP32_MG       2   /* FP32 CG preconditioned by one geometric-multigrid V-cycle, inside FP64
-                                iterative refinement (no counterpart in the reference, whose multigrid
-                                options are declared but not built, PROJECTION_3D.h:207-228) */
+                                iterative refinement.  No counterpart in the reference: MACROS.h:3
+                                declares MULTIGRID and PROJECTION_3D.h:55 a multigrid smoother, but its
+                                script parser never selects it (PROJECTION_3D.h:207-226).  Both FP32
+                                paths stop on rel_tolerance, whatever stop_rule says. */
 
 /* PCG stop rule. */
 #define FS_STOP_ABS_PRECOND  0   /* r^T D^-1 r < tol^2, the reference rule (PCG_METHOD.cpp:85-90) */
@@ -92,7 +94,7 @@ typedef struct fs_params {
     int    max_iteration;
     int    use_previous_pressure;
     int    stop_rule;          /* FS_STOP_* */
-    double rel_tolerance;      /* for FS_STOP_REL_RESIDUAL */
+    double rel_tolerance;      /* for FS_STOP_REL_RESIDUAL and the FP32 precisions */
     int    precision;          /* FS_FP64 | FS_FP32_REFINE | FS_FP32_MG */
     int    schedule;           /* FS_SCHEDULE_* */
     /* debugging switches, EULERIAN_FLUID_SOLVER_3D.h:103-107 */
@@ -113,7 +115,7 @@ typedef struct fs_params {
 /* Per-step (or per-phase) statistics. */
 typedef struct fs_step_stats {
     int    pcg_iterations;       /* inner PCG iterations, summed over the step's projections */
-    int    refinement_sweeps;    /* outer FP64 sweeps (FS_FP32_REFINE), else 0 */
+    int    refinement_sweeps;    /* outer FP64 sweeps (FS_FP32_REFINE, FS_FP32_MG), else 0 */
     double residual_precond;     /* sqrt(r^T D^-1 r) at exit, PCG_METHOD.cpp:111 */
     double residual_rel;         /* ||b - A x||_2 / ||b||_2 at exit (recomputed in FP64) */
     double max_velocity[3];      /* max |u|,|v|,|w| after the gradient update, PROJECTION_3D.h:1236,1345,1427 */
@@ -284,8 +286,8 @@ int  fs_get_kernel_samples(fs_solver* s, int precision, double* avg_ms, int* cou
 /* Frame kernels sampled while fs_set_kernel_sampling is on: FS_KS_REINIT
  * times the first sweep of each reinitialization call (k_reinit) and counts
  * every sweep; FS_KS_MG_SMOOTH times the first finest-level smoother launch
- * of each V-cycle of the FS_FP32_MG solve (k_mg_smooth) and counts every
- * finest-level smoother launch.  Returns the mean duration (ms), the number
+ * of each V-cycle of the FS_FP32_MG solve (k_mg_rb_sweep, one red-black sweep)
+ * and counts every finest-level smoother launch.  Returns the mean duration (ms), the number
  * of timed launches and the number of launches since sampling was enabled.
  * No reference counterpart: measurement hooks beside fs_time_pcg_kernels. */
 #define FS_KS_REINIT 0
