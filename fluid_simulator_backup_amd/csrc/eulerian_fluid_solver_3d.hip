@@ -25,6 +25,10 @@
 #include "fs_layout.h"
 #include "fs_mg.h"
 #include "fs_probe.h"
+
+#ifndef FS_MG_INNER_RTOL
+#define FS_MG_INNER_RTOL 1e-5  // FS_FP32_MG: relative residual of each sweep's FP32 solve
+#endif
 #include "fs_pcg.h"
 
 namespace {
@@ -812,7 +816,8 @@ class EulerianFluidSolver3D {
     }
 
     // FP32 multigrid-preconditioned CG inside FP64 iterative refinement:
-    // each sweep solves the FP32 correction to 1e-5 of its right-hand side
+    // each sweep solves the FP32 correction to FS_MG_INNER_RTOL of its
+    // right-hand side
     // and adds it to the FP64 pressure; the FP64 residual (its constant mode
     // settled through the pin, as in the Jacobi refinement) decides the stop.
     void SolvePressureMg() {
@@ -825,7 +830,7 @@ class EulerianFluidSolver3D {
         int sweeps = 0, its = 0;
         while (bb > 0 && rr > rt2 * bb && sweeps < 40) {
             // restarted CG: at most 500 iterations per sweep
-            its += run_mgpcg32(1e-5, std::min(P.max_iteration, 500));
+            its += run_mgpcg32(FS_MG_INNER_RTOL, std::min(P.max_iteration, 500));
             fs_add_correction(st, L, p, e32);
             rr = true_residual(bb);
             sweeps++;
