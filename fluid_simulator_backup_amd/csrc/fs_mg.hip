@@ -144,7 +144,10 @@ __global__ void __launch_bounds__(MNT) k_mg_coarsen(FsLayout Lf, const float* __
     }
 }
 
-// red-black Gauss-Seidel half sweep: cells with (i+j+k) % 2 == color
+// red-black Gauss-Seidel half sweep: cells with (i+j+k) % 2 == color.
+// Every smoother divides by the diagonal through the hardware reciprocal
+// (within 1 ulp): the V-cycle is a preconditioner, and all its smoothers use
+// the same form, so the fused and split sweeps still agree bit for bit.
 __global__ void __launch_bounds__(MNT) k_mg_smooth(FsLayout L, const float* __restrict__ bx,
                                                    const float* __restrict__ by, const float* __restrict__ bz,
                                                    float ix, float iy, float iz, const float* __restrict__ diag,
@@ -156,7 +159,7 @@ __global__ void __launch_bounds__(MNT) k_mg_smooth(FsLayout L, const float* __re
     if (d == 0.f) return;
     float s = b[c] + bx[c] * (ix * x[c - 1]) + bx[c + 1] * (ix * x[c + 1]) + by[c] * (iy * x[c - L.sx]) +
               by[c + L.sx] * (iy * x[c + L.sx]) + bz[c] * (iz * x[c - L.sy]) + bz[c + L.sy] * (iz * x[c + L.sy]);
-    x[c] = s / d;
+    x[c] = s * __builtin_amdgcn_rcpf(d);
 }
 
 __global__ void __launch_bounds__(MNT) k_mg_residual(FsLayout L, const float* __restrict__ bx,
@@ -282,7 +285,7 @@ __global__ void __launch_bounds__(MNT) k_mg_coarse(FsLayout L, const float* __re
                 float v = sb[q] + sbx[q] * (ix * sx[q - 1]) + sbx[q + 1] * (ix * sx[q + 1]) +
                           sby[q] * (iy * sx[q - px]) + sby[q + px] * (iy * sx[q + px]) + sbz[q] * (iz * sx[q - pxy]) +
                           sbz[q + pxy] * (iz * sx[q + pxy]);
-                sx[q] = v / d;
+                sx[q] = v * __builtin_amdgcn_rcpf(d);
             }
         }
         __syncthreads();
@@ -298,9 +301,9 @@ __global__ void __launch_bounds__(MNT) k_mg_coarse(FsLayout L, const float* __re
 // those, written with them to x_out; the same arithmetic per cell as two
 // k_mg_smooth launches, the diagonal formed from the coefficients as
 // k_mg_diag forms it.  A workgroup owns a 64 x 16 column tile and marches
-// up SKC planes; the first colour's new values on the tile and a ring of one
-// cell around it go to LDS for three planes (k-1, k, k+1), so x is read
-// once and the coefficients serve both colours.
+// up kc planes, two per step; the first colour's new values on the tile and
+// a ring of one cell around it go to LDS (planes k-1 .. k+2), so the
+// coefficients serve both colours.
 #define SBX 64
 #define SBY 16
 #define SRX (SBX + 2)
@@ -309,54 +312,77 @@ __global__ void __launch_bounds__(SBX* SBY) k_mg_rb_sweep(FsLayout L, const floa
                                                        const float* __restrict__ by, const float* __restrict__ bz,
                                                        float ix, float iy, float iz, const float* __restrict__ b,
                                                        const float* __restrict__ x_in, float* __restrict__ x_out,
-                                                       int c1, int kc) {
-    __shared__ float R[3][SRY * SRX];
+                                                       int c1, int kc, int gx, int gy) {
+    __shared__ float R[4][SRY * SRX];
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * SBX + tx;
-    const int i0 = blockIdx.x * SBX, j0 = blockIdx.y * SBY, k0 = blockIdx.z * kc;
+    // workgroup b runs on XCD b % 8; with a multiple of 8 tiles, each XCD
+    // takes a contiguous range of tiles (x fastest), so the rings that
+    // neighbouring tiles share are read through one L2
+    const int nt = gridDim.x, wg = blockIdx.x;
+    const int tile = (nt & 7) == 0 ? (wg & 7) * (nt >> 3) + (wg >> 3) : wg;
+    const int i0 = (tile % gx) * SBX, j0 = ((tile / gx) % gy) * SBY, k0 = (tile / (gx * gy)) * kc;
     const int k1 = min(L.nz, k0 + kc);
-    // first colour at plane kk on the tile and its ring (0 outside the box)
-    // first colour at plane kk on the tile and its ring (0 outside the box)
+    // first colour at plane kk on the tile and its ring: one thread per cell
+    // of that colour (half of each ring row, SRX even), every operand loaded
+    // before any is used (addresses clamped into the box, 0 outside it).
+    // The second colour's update reads R only at first-colour cells.
+    static_assert(SRX % 2 == 0 && (SRX / 2) * SRY <= SBX * SBY, "one first-colour ring cell per thread");
     auto stage = [&](int kk) {
-        float* Rp = R[(kk + 3) % 3];
-        for (int m = t; m < SRX * SRY; m += SBX * SBY) {
-            int i = i0 - 1 + m % SRX, j = j0 - 1 + m / SRX;
-            float val = 0.f;
-            if (i >= 0 && i < L.nx && j >= 0 && j < L.ny && kk >= 0 && kk < L.nz) {
-                long c = L.idx(i, j, kk);
-                val = x_in[c];
-                if (((i + j + kk) & 1) == c1) {
-                    float d = ix * (bx[c] + bx[c + 1]) + iy * (by[c] + by[c + L.sx]) + iz * (bz[c] + bz[c + L.sy]);
-                    if (d != 0.f) {
-                        float v = b[c] + bx[c] * (ix * x_in[c - 1]) + bx[c + 1] * (ix * x_in[c + 1]) +
-                                  by[c] * (iy * x_in[c - L.sx]) + by[c + L.sx] * (iy * x_in[c + L.sx]) +
-                                  bz[c] * (iz * x_in[c - L.sy]) + bz[c + L.sy] * (iz * x_in[c + L.sy]);
-                        val = v / d;
-                    }
-                }
-            }
-            Rp[m] = val;
-        }
+        if (t >= (SRX / 2) * SRY) return;
+        float* Rp = R[(kk + 4) % 4];
+        const int lj = t / (SRX / 2), j = j0 - 1 + lj;
+        const int li = ((c1 - (i0 - 1 + j + kk)) & 1) + 2 * (t % (SRX / 2)), i = i0 - 1 + li;
+        const bool in = i >= 0 && i < L.nx && j >= 0 && j < L.ny && kk >= 0 && kk < L.nz;
+        const long c = L.idx(min(max(i, 0), L.nx - 1), min(max(j, 0), L.ny - 1), min(max(kk, 0), L.nz - 1));
+        const float x0 = x_in[c], xw = x_in[c - 1], xe = x_in[c + 1], xs = x_in[c - L.sx], xn = x_in[c + L.sx],
+                    xb = x_in[c - L.sy], xt = x_in[c + L.sy];
+        const float cw = bx[c], ce = bx[c + 1], cs = by[c], cn = by[c + L.sx], cb = bz[c], ct = bz[c + L.sy];
+        const float bc = b[c];
+        const float d = ix * (cw + ce) + iy * (cs + cn) + iz * (cb + ct);
+        const float v = bc + cw * (ix * xw) + ce * (ix * xe) + cs * (iy * xs) + cn * (iy * xn) + cb * (iz * xb) +
+                        ct * (iz * xt);
+        const float r = v * __builtin_amdgcn_rcpf(d);
+        Rp[lj * SRX + li] = in ? (d != 0.f ? r : x0) : 0.f;
     };
     stage(k0 - 1);
     stage(k0);
     const int i = i0 + tx, j = j0 + ty;
     const bool own = i < L.nx && j < L.ny;
     const int q = (ty + 1) * SRX + tx + 1;
-    for (int k = k0; k < k1; k++) {
-        // the second colour's operands at plane k, loaded before the staging
-        // of plane k+1 so both sets of loads are in flight together
-        const long c = L.idx(min(i, L.nx - 1), min(j, L.ny - 1), k);
-        const float x0 = x_in[c], bc = b[c];
-        const float cw = bx[c], ce = bx[c + 1], cs = by[c], cn = by[c + L.sx], cb = bz[c], ct = bz[c + L.sy];
+    // two planes per step: the first colour staged for planes k+1 and k+2,
+    // then the second colour of planes k and k+1 (R holds planes k-1..k+2)
+    for (int k = k0; k < k1; k += 2) {
+        // the second colour's operands, loaded before the staging so all the
+        // step's loads are in flight together
+        float x0[2], bc[2], cw[2], ce[2], cs[2], cn[2], cb[2], ct[2];
+        long c[2];
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            c[h] = L.idx(min(i, L.nx - 1), min(j, L.ny - 1), min(k + h, L.nz - 1));
+            x0[h] = x_in[c[h]];
+            bc[h] = b[c[h]];
+            cw[h] = bx[c[h]];
+            ce[h] = bx[c[h] + 1];
+            cs[h] = by[c[h]];
+            cn[h] = by[c[h] + L.sx];
+            cb[h] = bz[c[h]];
+            ct[h] = bz[c[h] + L.sy];
+        }
         stage(k + 1);
+        stage(k + 2);
         __syncthreads();
-        if (own) {
-            const float *Rm = R[(k + 2) % 3], *R0 = R[k % 3], *Rp = R[(k + 1) % 3];
-            const float d = ix * (cw + ce) + iy * (cs + cn) + iz * (cb + ct);
-            const float v = bc + cw * (ix * R0[q - 1]) + ce * (ix * R0[q + 1]) + cs * (iy * R0[q - SRX]) +
-                            cn * (iy * R0[q + SRX]) + cb * (iz * Rm[q]) + ct * (iz * Rp[q]);
-            const float r = v / (d != 0.f ? d : 1.f);
-            x_out[c] = ((i + j + k) & 1) == c1 ? R0[q] : (d != 0.f ? r : x0);
+#pragma unroll
+        for (int h = 0; h < 2; h++) {
+            const int kk = k + h;
+            if (own && kk < k1) {
+                const float *Rm = R[(kk + 3) % 4], *R0 = R[kk % 4], *Rp = R[(kk + 1) % 4];
+                const float d = ix * (cw[h] + ce[h]) + iy * (cs[h] + cn[h]) + iz * (cb[h] + ct[h]);
+                const float v = bc[h] + cw[h] * (ix * R0[q - 1]) + ce[h] * (ix * R0[q + 1]) +
+                                cs[h] * (iy * R0[q - SRX]) + cn[h] * (iy * R0[q + SRX]) + cb[h] * (iz * Rm[q]) +
+                                ct[h] * (iz * Rp[q]);
+                const float r = v * __builtin_amdgcn_rcpf(d);
+                x_out[c[h]] = ((i + j + kk) & 1) == c1 ? R0[q] : (d != 0.f ? r : x0[h]);
+            }
         }
         __syncthreads();
     }
@@ -547,12 +573,12 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
     // under 1024 workgroups, at least 4 (or nz)
     int kc = (int)std::min<long>(16, std::max<long>(4, (long)v.L.nz * gx * gy / 1024));
     kc = std::min(kc, v.L.nz);
-    const dim3 g(gx, gy, (v.L.nz + kc - 1) / kc), t(SBX, SBY);
+    const dim3 g(gx * gy * ((v.L.nz + kc - 1) / kc)), t(SBX, SBY);
     float *in = v.x, *out = v.r;
     for (int s = 0; s < MG_NU; s++) {
         if (timed && s == 0) probe->begin(st);
         hipLaunchKernelGGL(k_mg_rb_sweep, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, in, out, c1,
-                           kc);
+                           kc, gx, gy);
         if (timed && s == 0) probe->end(st);
         std::swap(in, out);
     }
