@@ -11,15 +11,16 @@
 // spacing doubled, this keeps the finite-volume fluxes), the residual is
 // restricted with the transpose of the prolongation (divided by 8), and the
 // prolongation is trilinear with clamped (zero-gradient) indices at the walls.
-// Smoother: red-black Gauss-Seidel, red then black before the coarse
-// correction and black then red after it, so the V-cycle is a symmetric
-// operator.  Each sweep is one launch that updates both colours (the first
+// Smoother: red-black Gauss-Seidel (two or four sweeps each way), red first
+// before the coarse correction and black first after it, so the V-cycle is
+// a symmetric operator.  Each sweep is one launch that updates both colours (the first
 // colour's values held in LDS for three planes), out of place between x and
 // the level's residual buffer.  The coarsest level (at most 512 cells) is
 // swept in LDS by one workgroup in a single launch.
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <stdexcept>
 #include <vector>
 
 #include "fs_layout.h"
@@ -48,6 +49,12 @@ class FsMg {
     // (zero ghosts).
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
+    // red-black sweeps before and after the coarse correction; even, since
+    // the fused sweeps ping-pong and must end in x
+    void SetSweeps(int nu) {
+        if (nu < 2 || nu % 2) throw std::invalid_argument("multigrid sweeps must be even and at least 2");
+        nu_ = nu;
+    }
     // When set, the first finest-level smoother launch (one red-black sweep)
     // of every V-cycle is timed and every finest-level smoother launch
     // counted (measurement only).
@@ -58,6 +65,7 @@ class FsMg {
     void Smooth(hipStream_t st, int l, int c1);
     std::vector<FsMgLevel> lv_;
     bool split_ = false;
+    int nu_ = 2;
     float* diag0_ = nullptr;
 };
 

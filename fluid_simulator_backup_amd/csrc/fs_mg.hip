@@ -10,9 +10,6 @@
 #define MBX 64
 #define MBY 4
 #define MNT (MBX * MBY)
-#ifndef MG_NU
-#define MG_NU 2  // red-black sweeps before and after the coarse correction
-#endif
 #ifndef MG_COARSE_CELLS
 #define MG_COARSE_CELLS 512  // coarsen while a level has more cells
 #endif
@@ -548,7 +545,7 @@ void FsMg::SetOperator(hipStream_t st, const float* bx, const float* by, const f
     }
 }
 
-// MG_NU red-black sweeps on level l, colour c1 first in each: fused sweeps
+// nu_ red-black sweeps on level l, colour c1 first in each: fused sweeps
 // ping-pong between x and the level's residual buffer (free until the
 // residual is formed, and again after the restriction), ending in x; with
 // FS_MG_SPLIT_SMOOTH set, two in-place half sweeps per sweep instead (the
@@ -558,16 +555,15 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
     const bool timed = probe && l == 0 && c1 == 0;
     if (split_) {
         const dim3 g = cells_grid(v.L), t(MBX, MBY);
-        for (int s = 0; s < 2 * MG_NU; s++) {
+        for (int s = 0; s < 2 * nu_; s++) {
             if (timed && s == 0) probe->begin(st);
             hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b,
                                v.x, (s & 1) ^ c1);
             if (timed && s == 0) probe->end(st);
         }
-        if (probe && l == 0) probe->launches += 2 * MG_NU;
+        if (probe && l == 0) probe->launches += 2 * nu_;
         return;
     }
-    static_assert(MG_NU % 2 == 0, "fused sweeps ping-pong and must end in x");
     const int gx = (v.L.nx + SBX - 1) / SBX, gy = (v.L.ny + SBY - 1) / SBY;
     // planes per workgroup: 16 on large levels, fewer where that leaves
     // under 1024 workgroups, at least 4 (or nz)
@@ -575,14 +571,14 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
     kc = std::min(kc, v.L.nz);
     const dim3 g(gx * gy * ((v.L.nz + kc - 1) / kc)), t(SBX, SBY);
     float *in = v.x, *out = v.r;
-    for (int s = 0; s < MG_NU; s++) {
+    for (int s = 0; s < nu_; s++) {
         if (timed && s == 0) probe->begin(st);
         hipLaunchKernelGGL(k_mg_rb_sweep, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, in, out, c1,
                            kc, gx, gy);
         if (timed && s == 0) probe->end(st);
         std::swap(in, out);
     }
-    if (probe && l == 0) probe->launches += MG_NU;
+    if (probe && l == 0) probe->launches += nu_;
 }
 
 void FsMg::Cycle(hipStream_t st, int l) {
