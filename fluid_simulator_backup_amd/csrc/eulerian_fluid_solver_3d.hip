@@ -26,6 +26,9 @@
 #include "fs_mg.h"
 #include "fs_probe.h"
 
+#ifndef FS_FP32_RESTART
+#define FS_FP32_RESTART 4000  // FS_FP32_REFINE: restart a recursion that dropped under 100x in this many iterations
+#endif
 #ifndef FS_MG_INNER_RTOL
 #define FS_MG_INNER_RTOL 1e-5  // FS_FP32_MG: relative residual of each sweep's FP32 solve
 #endif
@@ -731,7 +734,7 @@ class EulerianFluidSolver3D {
         hbb = bb;
         HIPCHECK(hipMemcpyAsync(&S32->bb, &hbb, sizeof(double), hipMemcpyHostToDevice, st));
         double rr_ref = rr64;
-        int n = 0;
+        int n = 0, done_before = 0, n_update = 0;
         while (true) {
             fs_pcg32_chunk(st, L, o32, diag32, e32, w32, S32, n, chunk, sampling ? &probe : nullptr);
             check_launch();
@@ -740,6 +743,29 @@ class EulerianFluidSolver3D {
             if (hS->done == 2) {
                 n = hS->iter;
                 break;
+            }
+            // a recursion that has not dropped the residual 100x in
+            // FS_FP32_RESTART iterations has stagnated at FP32's attainable
+            // accuracy: fold e into the pressure and restart it from the
+            // FP64 residual (restarted CG; the 64-bubble pool at 512^3 needs it)
+            if (hS->done == 0 && n + chunk - n_update >= FS_FP32_RESTART && !(hS->rr < update_drop * rr_ref)) {
+                n += chunk;
+                fs_add_correction(st, L, p, e32);
+                HIPCHECK(hipMemsetAsync(e32, 0, sizeof(float) * L.total, st));
+                rr64 = true_residual(bb);
+                updates++;
+                done_before += n;
+                if (debug) fprintf(stderr, "[fs] fp32 restart after %d rel64=%.3e\n", done_before, std::sqrt(rr64 / bb));
+                if (rr64 <= rt2 * bb || updates > 400 || done_before >= P.max_iteration) {
+                    n = 0;
+                    break;
+                }
+                init_state(S32, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, P.max_iteration - done_before, false);
+                fs_pcg32_begin(st, L, o32, r32, diag32, e32, w32, S32);
+                HIPCHECK(hipMemcpyAsync(&S32->bb, &hbb, sizeof(double), hipMemcpyHostToDevice, st));
+                rr_ref = rr64;
+                n = n_update = 0;
+                continue;
             }
             bool stopped = hS->done == 1;
             if (debug && dbg_lines++ < 120)
@@ -761,9 +787,10 @@ class EulerianFluidSolver3D {
                 if (rr64 <= rt2 * bb || updates > 400) break;
                 fs_pcg32_reliable_update(st, L, r32, diag32, w32, S32);
                 rr_ref = rr64;
+                n_update = n;
             }
         }
-        return n;
+        return done_before + n;
     }
 
     // ---- FS_FP32_MG ----------------------------------------------------------
