@@ -302,7 +302,9 @@ __global__ void __launch_bounds__(MNT) k_mg_coarse(FsLayout L, const float* __re
 // a ring of one cell around it go to LDS (planes k-1 .. k+2), so the
 // coefficients serve both colours.
 #define SBX 64
+#ifndef SBY
 #define SBY 16
+#endif
 #define SRX (SBX + 2)
 #define SRY (SBY + 2)
 __global__ void __launch_bounds__(SBX* SBY) k_mg_rb_sweep(FsLayout L, const float* __restrict__ bx,
