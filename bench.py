@@ -52,6 +52,8 @@ def parse():
                          "rollout gradient checked against central differences; projection: one projection of "
                          "the 64-bubble pool (default 512^3) per path (each its own JSON line)")
     ap.add_argument("--max-iteration", type=int, default=40000, help="projection workload: PCG iteration cap")
+    ap.add_argument("--batch-path", default="mg", choices=["mg", "refine"],
+                    help="batch workload: the FP32 projection inside FP64 refinement (multigrid or Jacobi)")
     ap.add_argument("--directions", type=int, default=32)
     ap.add_argument("--rollout-steps", type=int, default=16)
     ap.add_argument("--checkpoint-every", type=int, default=4)
@@ -180,14 +182,16 @@ def cpu_baseline(scene, n, iters_per_step, dt, threads):
 def batch_main(a):
     """Uneven scenes (edges 32..128, dam break or rising drop, density ratio
     1.25..1000) advanced together through fs_batch_advance_frame, against
-    the same scenes advanced one at a time; FP32+refine projection to 1e-8."""
+    the same scenes advanced one at a time; the FP32 projection inside FP64
+    refinement (--batch-path: multigrid or Jacobi) to 1e-8."""
     import torch
 
     import fluid_simulator_backup_amd as fsa
     from fluid_simulator_backup_amd import _abi, scenes
     torch.cuda.set_device(0)
     lib = fsa.load_library()
-    sc = scenes.batch_scenes(a.scenes, 3, precision=_abi.FS_FP32_REFINE, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
+    prec = getattr(_abi, PATHS[a.batch_path])
+    sc = scenes.batch_scenes(a.scenes, 3, precision=prec, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
                              rel_tolerance=1e-8, max_iteration=20000)
     cells = sum(g.nx * g.ny * g.nz for _, _, _, _, g in sc)
     batch = _abi.Batch(lib, [s[3] for s in sc], [s[4] for s in sc], 0, a.threads)
@@ -229,7 +233,7 @@ def batch_main(a):
         "ms_per_step": round(t_batch * 1e3 / a.steps, 3), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere", "data": "synthetic",
         "config": {"workload": f"{len(sc)} scenes, edges 32-128, dam break / rising drop, density ratio 1.25-1000, "
-                               f"FP32+refine to 1e-8, host threads {a.threads or min(len(sc), 16)}",
+                               f"{LABEL[a.batch_path]} to 1e-8, host threads {a.threads or min(len(sc), 16)}",
                    "cells": cells},
         "one_at_a_time_cells_per_s": round(cells * a.steps / t_seq, 1),
         "speedup_vs_one_at_a_time": round(t_seq / t_batch, 3),

@@ -416,12 +416,14 @@ def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
         assert e < 1e-4, (c, e)
 
 
-def test_batch_matches_scenes_advanced_alone(hip_lib):
-    """A batch of uneven scenes (FP32+refine projection, one host thread per
-    scene stream) gives, bit for bit, what each scene gives advanced alone:
-    every kernel and reduction is deterministic and no state is shared."""
+@pytest.mark.parametrize("precision", [_abi.FS_FP32_REFINE, _abi.FS_FP32_MG])
+def test_batch_matches_scenes_advanced_alone(hip_lib, precision):
+    """A batch of uneven scenes (FP32 projection inside FP64 refinement, one
+    host thread per scene stream) gives, bit for bit, what each scene gives
+    advanced alone: every kernel and reduction is deterministic and no state
+    is shared."""
     from fluid_simulator_backup_amd import scenes
-    sc = scenes.batch_scenes(6, seed=5, edges=(12, 16, 20, 24), precision=_abi.FS_FP32_REFINE,
+    sc = scenes.batch_scenes(6, seed=5, edges=(12, 16, 20, 24), precision=precision,
                              stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8, max_iteration=5000)
     batch = _abi.Batch(hip_lib, [s[3] for s in sc], [s[4] for s in sc], 0, 4)
     alone = []

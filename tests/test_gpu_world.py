@@ -109,11 +109,13 @@ def test_world_sources_follow_their_frame_window(hip_lib):
     world.close()
 
 
-def test_world_fp32_refine_and_fixed_schedule(hip_lib):
-    """The script's scene with the FP32+refine projection and the fixed
-    schedule advances and keeps every solve at its relative residual."""
+@pytest.mark.parametrize("precision", [_abi.FS_FP32_REFINE, _abi.FS_FP32_MG])
+def test_world_fp32_refine_and_fixed_schedule(hip_lib, precision):
+    """The script's scene with an FP32 projection inside FP64 refinement
+    (Jacobi or multigrid) and the fixed schedule advances and keeps every
+    solve at its relative residual."""
     path = os.path.join(SCRIPTS, "air_water_bubble.sim")
-    world = _abi.World(hip_lib, path, 0, precision=_abi.FS_FP32_REFINE, schedule=_abi.FS_SCHEDULE_FIXED,
+    world = _abi.World(hip_lib, path, 0, precision=precision, schedule=_abi.FS_SCHEDULE_FIXED,
                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8)
     for _ in range(2):
         st = world.advance_frame()
