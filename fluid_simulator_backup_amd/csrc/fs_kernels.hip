@@ -82,10 +82,13 @@ void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayou
 // FillGhostCellsContinuousDerivativesFrom face blocks,
 // FIELD_STRUCTURE_3D.h:1806-1894: 2a-b outwards, deeper layers reading the
 // layers just written when src == dst (one thread walks its own row).
-__global__ void k_linear_faces(double* dst, const double* src, const double* gsrc, FsLayout L, FsBox b, int d,
-                               int axis) {
+// The three axes in one launch (blockIdx.z = axis): each writes only its own
+// face layers and reads only its own row, so they are independent.
+__global__ void k_linear_faces(double* dst, const double* src, const double* gsrc, FsLayout L, FsBox b, int d) {
     // S(x): src inside the box along the axis, gsrc in its ghost layers
+    const int axis = blockIdx.z;
     int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
+    if (c >= (axis == 2 ? b.nj : b.nk)) return;
     if (axis == 0) {
         if (a >= b.nj) return;
         auto S = [&](int i) { return (i >= 0 && i < b.ni ? src : gsrc)[L.idx(i, a, c)]; };
@@ -131,9 +134,8 @@ void fs_fill_linear(hipStream_t st, double* dst, const double* src, const FsLayo
 
 void fs_fill_linear_split(hipStream_t st, double* dst, const double* src, const double* gsrc, const FsLayout& L,
                           FsBox b, int depth) {
-    hipLaunchKernelGGL(k_linear_faces, dim3((b.nj + 63) / 64, b.nk), dim3(64), 0, st, dst, src, gsrc, L, b, depth, 0);
-    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nk), dim3(64), 0, st, dst, src, gsrc, L, b, depth, 1);
-    hipLaunchKernelGGL(k_linear_faces, dim3((b.ni + 63) / 64, b.nj), dim3(64), 0, st, dst, src, gsrc, L, b, depth, 2);
+    const int na = b.ni > b.nj ? b.ni : b.nj, nc = b.nj > b.nk ? b.nj : b.nk;
+    hipLaunchKernelGGL(k_linear_faces, dim3((na + 63) / 64, nc, 3), dim3(64), 0, st, dst, src, gsrc, L, b, depth);
 }
 
 // ---------------------------------------------------------------------------
@@ -823,7 +825,16 @@ __global__ void __launch_bounds__(256) k_plane_sums(const double* __restrict__ r
         __syncthreads();
         if (threadIdx.x == 0) {
             if (L.nx <= PS_CHUNK) {
-                for (int t = 0; t < n; t++) s += buf[t];
+                // in order; eight LDS reads issued ahead of their adds
+                int t = 0;
+                for (; t + 8 <= n; t += 8) {
+                    double v[8];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) v[u] = buf[t + u];
+#pragma unroll
+                    for (int u = 0; u < 8; u++) s += v[u];
+                }
+                for (; t < n; t++) s += buf[t];
             } else {  // rows wider than the stage: read straight from memory
                 for (int jj = 0; jj < nrow; jj++) {
                     const double* row = rhs + L.idx(0, j0 + jj, k);
