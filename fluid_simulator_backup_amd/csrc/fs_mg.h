@@ -31,6 +31,7 @@ struct FsMgLevel {
     float idx2, idy2, idz2;         // 1/d^2 per axis on this level
     float *bx, *by, *bz, *diag;     // level 0: coefficients borrowed from the FP32 operator
     float *x, *b, *r;               // level 0: x and b are the caller's vectors
+    int kc;                         // planes per workgroup of the fused sweep (0 until tuned)
 };
 
 class FsMg {
@@ -46,7 +47,9 @@ class FsMg {
     // and forms every level's coefficients and diagonals.
     void SetOperator(hipStream_t st, const float* bx, const float* by, const float* bz);
     // z = M r: one V-cycle from z = 0.  r and z are padded FP32 cell fields
-    // (zero ghosts).
+    // (zero ghosts).  The first call also tunes each level's sweep: the
+    // planes per workgroup (2, 4, 8 or 16) that time fastest on this GPU,
+    // which changes no value.
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
     // red-black sweeps before and after the coarse correction; even, since
@@ -63,6 +66,8 @@ class FsMg {
    private:
     void Cycle(hipStream_t st, int l);
     void Smooth(hipStream_t st, int l, int c1);
+    void Tune(hipStream_t st);
+    bool tuned_ = false;
     std::vector<FsMgLevel> lv_;
     bool split_ = false;
     int nu_ = 2;

@@ -4,6 +4,7 @@
 #include "fs_mg.h"
 
 #include <algorithm>
+#include <cstdio>
 #include <cstdlib>
 #include <stdexcept>
 
@@ -567,10 +568,13 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
         return;
     }
     const int gx = (v.L.nx + SBX - 1) / SBX, gy = (v.L.ny + SBY - 1) / SBY;
-    // planes per workgroup: 16 on large levels, fewer where that leaves
-    // under 1024 workgroups, at least 4 (or nz)
-    int kc = (int)std::min<long>(16, std::max<long>(4, (long)v.L.nz * gx * gy / 1024));
-    kc = std::min(kc, v.L.nz);
+    // planes per workgroup: as tuned, else 16 on large levels, fewer where
+    // that leaves under 1024 workgroups, at least 4 (or nz)
+    int kc = v.kc;
+    if (kc <= 0) {
+        kc = (int)std::min<long>(16, std::max<long>(4, (long)v.L.nz * gx * gy / 1024));
+        kc = std::min(kc, v.L.nz);
+    }
     const dim3 g(gx * gy * ((v.L.nz + kc - 1) / kc)), t(SBX, SBY);
     float *in = v.x, *out = v.r;
     for (int s = 0; s < nu_; s++) {
@@ -600,10 +604,54 @@ void FsMg::Cycle(hipStream_t st, int l) {
     Smooth(st, l, 1);  // ..., black, red
 }
 
+// Autotuner of the fused sweep: on each level but the coarsest, time one
+// sweep (best of three after a warm-up) for 2, 4, 8 and 16 planes per
+// workgroup and keep the fastest.  The sweeps read x and b and write the
+// level's residual buffer, which holds nothing yet; each cell's arithmetic
+// does not depend on the chunking, so the choice changes no value.
+void FsMg::Tune(hipStream_t st) {
+    hipEvent_t e0, e1;
+    check(hipEventCreate(&e0));
+    check(hipEventCreate(&e1));
+    for (size_t l = 0; l + 1 < lv_.size(); l++) {
+        FsMgLevel& v = lv_[l];
+        const int gx = (v.L.nx + SBX - 1) / SBX, gy = (v.L.ny + SBY - 1) / SBY;
+        float best = 0;
+        int best_kc = 0;
+        for (int kc : {2, 4, 8, 16}) {
+            if (kc > v.L.nz && kc != 2) break;
+            const int k = std::min(kc, v.L.nz);
+            const dim3 g(gx * gy * ((v.L.nz + k - 1) / k)), t(SBX, SBY);
+            float ms_min = 0;
+            for (int rep = 0; rep < 4; rep++) {
+                check(hipEventRecord(e0, st));
+                hipLaunchKernelGGL(k_mg_rb_sweep, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x,
+                                   v.r, 0, k, gx, gy);
+                check(hipEventRecord(e1, st));
+                check(hipEventSynchronize(e1));
+                float ms = 0;
+                check(hipEventElapsedTime(&ms, e0, e1));
+                if (rep == 1 || (rep > 1 && ms < ms_min)) ms_min = ms;  // rep 0 warms up
+            }
+            if (best_kc == 0 || ms_min < best) {
+                best = ms_min;
+                best_kc = k;
+            }
+        }
+        v.kc = best_kc;
+        if (std::getenv("FS_DEBUG")) fprintf(stderr, "[fs] mg level %zu (%d x %d x %d): %d planes per workgroup, %.1f us\n",
+                                             l, v.L.nx, v.L.ny, v.L.nz, best_kc, best * 1e3f);
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tuned_ = true;
+}
+
 void FsMg::Apply(hipStream_t st, const float* r, float* z) {
     FsMgLevel& v = lv_[0];
     v.b = (float*)r;
     v.x = z;
+    if (!tuned_ && !split_) Tune(st);
     check(hipMemsetAsync(z, 0, sizeof(float) * (size_t)v.L.total, st));
     Cycle(st, 0);
 }
