@@ -16,6 +16,12 @@ step is synchronous.
 One JSON line is printed.  `roofline` is for the dominant kernel (timed with
 HIP events on the solver's stream); `cpu_baseline` times the CPU checker on
 the host on a bounded sample of the same workload (see `sample`).
+
+Other workloads (each its own JSON line): `--workload batch` (48 uneven
+scenes advanced together against one at a time), `--workload adjoint` (the
+checkpointed rollout gradient against central differences), `--workload
+projection` (one projection of the 64-bubble pool, density ratio 1000, at
+512^3 per path).
 """
 import argparse
 import json
