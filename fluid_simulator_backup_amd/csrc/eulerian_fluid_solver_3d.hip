@@ -63,6 +63,11 @@ class EulerianFluidSolver3D {
     FsMaterial mat;
     int device = 0;
     hipStream_t st = nullptr;
+    // second stream: the step's reinitialization runs on it, on a copy of the
+    // advected level set, while the projection runs on st (ConcurrentReinit)
+    hipStream_t st2 = nullptr;
+    hipEvent_t ev_ls = nullptr, ev_rn = nullptr;
+    double* phi_r = nullptr;
     hipEvent_t ev[4] = {};
 
     // persistent fields (padded layout, FP64)
@@ -200,6 +205,9 @@ class EulerianFluidSolver3D {
         if (dev >= 0) HIPCHECK(hipSetDevice(dev));
         HIPCHECK(hipGetDevice(&device));
         HIPCHECK(hipStreamCreate(&st));
+        HIPCHECK(hipStreamCreate(&st2));
+        HIPCHECK(hipEventCreateWithFlags(&ev_ls, hipEventDisableTiming));
+        HIPCHECK(hipEventCreateWithFlags(&ev_rn, hipEventDisableTiming));
         for (auto& e : ev) HIPCHECK(hipEventCreate(&e));
         for (auto& e : probe.ev) HIPCHECK(hipEventCreate(&e));
         for (auto& k : kprobe) k.create();
@@ -220,7 +228,7 @@ class EulerianFluidSolver3D {
                          "FP32 refinement on unequal spacings needs the fixed schedule: the compatible schedule's "
                          "matrix puts 1/dx^2 on the diagonal for every face, so constants are not in its null space"};
 
-        double** f64[] = {&phi, &sghost, &sghost2, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
+        double** f64[] = {&phi, &phi_r, &sghost, &sghost2, &u, &v, &w, &spare_u, &spare_v, &spare_w, &gu, &gv, &gw, &p, &div, &rho, &irho, &sgn, &tmp,
                           &bx, &by, &bz, &diag, &rhs};
         for (double** f : f64) *f = dfield();
         if (P.order_for_time_advancing > 1) {
@@ -281,7 +289,7 @@ class EulerianFluidSolver3D {
 
     ~EulerianFluidSolver3D() {
         if (st) (void)hipStreamSynchronize(st);
-        double* f64[] = {phi, sghost, sghost2, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
+        double* f64[] = {phi, phi_r, sghost, sghost2, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
                          scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk,
                          va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc};
         for (double* f : f64)
@@ -310,6 +318,10 @@ class EulerianFluidSolver3D {
         for (auto& e : probe.ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& k : kprobe) k.destroy();
+        if (st2) (void)hipStreamSynchronize(st2);
+        if (ev_ls) (void)hipEventDestroy(ev_ls);
+        if (ev_rn) (void)hipEventDestroy(ev_rn);
+        if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -1389,6 +1401,31 @@ class EulerianFluidSolver3D {
                 fs_average(st, phi, tmp, L);
             }
         }
+        // The reinitialization reads only the advected level set (and, in the
+        // compatible schedule, its ghost layers).  SolveForRK changes neither
+        // the box nor, beyond the constant fill replayed below, the ghosts,
+        // unless surface tension refreshes them or level-set sources change
+        // the box.  Then the reinitialization is enqueued first, on the
+        // second stream and a full copy of phi, and overlaps the projection;
+        // phi takes the result after SolveForRK.  The values are those of
+        // the sequential order.
+        const bool concurrent = ConcurrentReinit();
+        if (concurrent) {
+            HIPCHECK(hipEventRecord(ev_ls, st));
+            HIPCHECK(hipStreamWaitEvent(st2, ev_ls, 0));
+            std::swap(st, st2);
+            HIPCHECK(hipMemcpyAsync(phi_r, phi, sizeof(double) * (size_t)L.total, hipMemcpyDeviceToDevice, st));
+            std::swap(phi, phi_r);
+            // the ghost layers SolveForRK will leave on phi: the constant
+            // fill of ComputeNormals (assembly) and of the fixed schedule's
+            // viscosity, a function of the box, which SolveForRK leaves as is
+            if (P.is_projection_active || (fixed() && P.is_sourcing_active && P.use_delta_function_formulation))
+                fs_fill_const(st, phi, phi, L, box(0), 2);
+            ReinitializationStage(order);
+            std::swap(phi, phi_r);
+            HIPCHECK(hipEventRecord(ev_rn, st));
+            std::swap(st, st2);
+        }
         if (order == 1) {
             SolveForRK(dt, rc);  // :834
         } else {
@@ -1403,8 +1440,19 @@ class EulerianFluidSolver3D {
                 BlendVelocity((double)1 / (double)3, (double)2 / (double)3);  // :1059-1075
             }
         }
-        // reinitialization, RK2 (:836-858, :958-980) or RK3 (:1080-1109)
-        if (recording) tape.phi_adv = Record(phi, 0);
+        if (concurrent) {
+            HIPCHECK(hipStreamWaitEvent(st, ev_rn, 0));
+            std::swap(phi, phi_r);
+        } else {
+            if (recording) tape.phi_adv = Record(phi, 0);
+            ReinitializationStage(order);
+        }
+        check_launch();
+        return rc;
+    }
+
+    // reinitialization, RK2 (:836-858, :958-980) or RK3 (:1080-1109)
+    void ReinitializationStage(int order) {
         fs_copy_box(st, tmp, phi, L, box(0));
         Reinitialization();
         Reinitialization();
@@ -1415,8 +1463,12 @@ class EulerianFluidSolver3D {
         } else {
             fs_average(st, phi, tmp, L);
         }
-        check_launch();
-        return rc;
+    }
+
+    // whether the step's reinitialization may overlap SolveForRK (above)
+    bool ConcurrentReinit() const {
+        static const bool serial = std::getenv("FS_SERIAL_REINIT") != nullptr;
+        return !serial && !recording && P.sussmanstyle_reinitialization && !tension() && sources.empty();
     }
 
     // EULERIAN_FLUID_SOLVER_3D::CFLOneTimeStep, :2673-2785
