@@ -116,8 +116,26 @@ def measured_valu(scene, n, kernel, avg_ms):
     except (OSError, KeyError, ValueError):
         return None
     rate = ipw * waves / (avg_ms * 1e-3)
-    return {"instructions_per_wave": ipw, "waves": waves, "achieved_wave_instr_per_s": round(rate, -6),
-            "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
+    out = {"instructions_per_wave": ipw, "waves": waves, "achieved_wave_instr_per_s": round(rate, -6),
+           "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
+    if kernel == "k_reinit":
+        f = reinit_cu_fraction()
+        out["reinit_cu_fraction"] = f
+        out["frac_of_its_cus"] = round(rate / (VALU_PEAK_WAVE_INSTR_PER_S * f), 4)
+    return out
+
+
+def reinit_cu_fraction():
+    """Fraction of the CUs the step's reinitialization stream may use (the
+    solver's default 3/4, FS_REINIT_CU_FRAC; all of them when the step runs
+    in sequence, FS_SERIAL_REINIT)."""
+    if os.environ.get("FS_SERIAL_REINIT"):
+        return 1.0
+    try:
+        f = float(os.environ.get("FS_REINIT_CU_FRAC", "0.75"))
+    except ValueError:
+        return 1.0
+    return f if 0 < f < 1 else 1.0
 
 
 def divergent_velocity(n):
@@ -515,7 +533,8 @@ FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 3 * 8, 3 * 8),
 # what bounds each, for the roofline note
 LIMITER = {"k_reinit": "FP64 VALU issue: the WENO5 smoothness weights take 24 IEEE divisions per cell and every "
                        "operation keeps the reference's order, so the kernel runs at its instruction rate, far "
-                       "below the HBM bound (see the VALU counter summary under profiles/)",
+                       "below the HBM bound (see the VALU counter summary under profiles/); in the frame it runs "
+                       "on a second stream confined to `reinit_cu_fraction` of the CUs, beside the projection",
            "k_mg_rb_sweep": "HBM and load latency of the staged first colour"}
 
 
