@@ -1468,7 +1468,7 @@ class EulerianFluidSolver3D {
     // batch's scenes already fill the GPU from their own streams, so they
     // keep the sequential order.
     bool ConcurrentReinit() const {
-        static const bool serial = std::getenv("FS_SERIAL_REINIT") != nullptr;
+        const bool serial = std::getenv("FS_SERIAL_REINIT") != nullptr;
         return !serial && !batch_member && !recording && P.sussmanstyle_reinitialization && !tension() &&
                sources.empty();
     }
@@ -1489,10 +1489,13 @@ class EulerianFluidSolver3D {
         if (f > 0 && f < 1 && keep > 0 && n >= 8) {
             std::vector<uint32_t> mask((n + 31) / 32, 0u);
             for (int c = 0; c < keep; c++) mask[c / 32] |= 1u << (c % 32);
-            HIPCHECK(hipExtStreamCreateWithCUMask(&st2, (uint32_t)mask.size(), mask.data()));
-        } else {
-            HIPCHECK(hipStreamCreate(&st2));
+            // a device that refuses the mask gets an unmasked stream
+            if (hipExtStreamCreateWithCUMask(&st2, (uint32_t)mask.size(), mask.data()) != hipSuccess) {
+                (void)hipGetLastError();
+                st2 = nullptr;
+            }
         }
+        if (!st2) HIPCHECK(hipStreamCreate(&st2));
         HIPCHECK(hipEventCreateWithFlags(&ev_ls, hipEventDisableTiming));
         HIPCHECK(hipEventCreateWithFlags(&ev_rn, hipEventDisableTiming));
         phi_r = dfield();

@@ -379,6 +379,31 @@ def test_multigrid_fused_sweeps_match_half_sweeps(hip_lib, monkeypatch):
         assert np.array_equal(out[0][0][c], out[1][0][c]), c
 
 
+@pytest.mark.parametrize("name,order", [("rising_drop", 1), ("dam_break", 2)])
+def test_concurrent_reinitialization_matches_sequential_order(hip_lib, monkeypatch, name, order):
+    """The step's reinitialization on the second stream (on a copy of the
+    advected level set, with the assembly's ghost fill replayed) gives the
+    frames of the sequential order bit for bit (FS_SERIAL_REINIT)."""
+    from fluid_simulator_backup_amd import scenes
+    phi, vel, P, g = scenes.make(name, 24, precision=_abi.FS_FP32_MG, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
+                                 rel_tolerance=1e-8, order_for_time_advancing=order)
+    out = []
+    for serial in (True, False):
+        if serial:
+            monkeypatch.setenv("FS_SERIAL_REINIT", "1")
+        else:
+            monkeypatch.delenv("FS_SERIAL_REINIT", raising=False)
+        s = _abi.Solver(hip_lib, "fs_", P, g, 0)
+        s.set_fields(phi, *vel)
+        dt = s.cfl_dt(0.5, 1)
+        for _ in range(3):
+            s.advance_frame(dt)
+        out.append(s.get_fields())
+        s.close()
+    for c in out[0]:
+        assert np.array_equal(out[0][c], out[1][c]), c
+
+
 def test_frame_kernel_sampling_leaves_frames_unchanged(hip_lib):
     """The in-loop probes of the reinitialization sweep and the finest-level
     smoother time one launch per call, count every launch, and change no
