@@ -29,6 +29,12 @@
 #ifndef FS_FP32_RESTART
 #define FS_FP32_RESTART 4000  // FS_FP32_REFINE: restart a recursion that dropped under 100x in this many iterations
 #endif
+#ifndef FS_MG_SWEEPS_LOW
+#define FS_MG_SWEEPS_LOW 2   // FS_FP32_MG smoothing sweeps each way below 10:1 density contrast
+#endif
+#ifndef FS_MG_SWEEPS_HIGH
+#define FS_MG_SWEEPS_HIGH 4  // and from 10:1
+#endif
 #ifndef FS_MG_INNER_RTOL
 #define FS_MG_INNER_RTOL 1e-5  // FS_FP32_MG: relative residual of each sweep's FP32 solve
 #endif
@@ -864,7 +870,7 @@ class EulerianFluidSolver3D {
         // bubbles 263 -> 133 per frame at 256^3) for twice the smoothing
         {
             const double hi = std::max(P.water_density, P.air_density), lo = std::min(P.water_density, P.air_density);
-            mg.SetSweeps(lo > 0 && hi / lo < 10 ? 2 : 4);
+            mg.SetSweeps(lo > 0 && hi / lo < 10 ? FS_MG_SWEEPS_LOW : FS_MG_SWEEPS_HIGH);
         }
         mg.probe = sampling ? &kprobe[FS_KS_MG_SMOOTH] : nullptr;
         mg.SetOperator(st, bx32, by32, bz32);

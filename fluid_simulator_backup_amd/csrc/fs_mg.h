@@ -52,10 +52,10 @@ class FsMg {
     // which changes no value.
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
-    // red-black sweeps before and after the coarse correction; even, since
-    // the fused sweeps ping-pong and must end in x
+    // red-black sweeps before and after the coarse correction (an odd count
+    // ends the fused sweeps' ping-pong in the residual buffer, copied back)
     void SetSweeps(int nu) {
-        if (nu < 2 || nu % 2) throw std::invalid_argument("multigrid sweeps must be even and at least 2");
+        if (nu < 1) throw std::invalid_argument("multigrid sweeps must be at least 1");
         nu_ = nu;
     }
     // When set, the first finest-level smoother launch (one red-black sweep)

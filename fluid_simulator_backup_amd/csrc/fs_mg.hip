@@ -584,6 +584,7 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
         if (timed && s == 0) probe->end(st);
         std::swap(in, out);
     }
+    if (in != v.x) check(hipMemcpyAsync(v.x, in, sizeof(float) * (size_t)v.L.total, hipMemcpyDeviceToDevice, st));
     if (probe && l == 0) probe->launches += nu_;
 }
 
