@@ -67,7 +67,8 @@ class FsMg {
     void Cycle(hipStream_t st, int l);
     void Smooth(hipStream_t st, int l, int c1);
     void Tune(hipStream_t st);
-    bool tuned_ = false;
+    void Release();
+    bool tuned_ = false, built_ = false;
     std::vector<FsMgLevel> lv_;
     bool split_ = false;
     int nu_ = 2;

@@ -480,7 +480,9 @@ __global__ void __launch_bounds__(MNT) k_mg_total(const double* part, int nb, do
 }  // namespace
 
 // ---- hierarchy ---------------------------------------------------------------
-FsMg::~FsMg() {
+FsMg::~FsMg() { Release(); }
+
+void FsMg::Release() {
     for (size_t l = 0; l < lv_.size(); l++) {
         FsMgLevel& v = lv_[l];
         if (l > 0)
@@ -488,27 +490,33 @@ FsMg::~FsMg() {
                 if (f) (void)hipFree(f);
         if (v.r) (void)hipFree(v.r);
     }
+    lv_.clear();
     if (diag0_) (void)hipFree(diag0_);
+    diag0_ = nullptr;
+    built_ = tuned_ = false;
 }
 
 void FsMg::Init(const FsLayout& L0, double idx2, double idy2, double idz2) {
-    if (!lv_.empty() && lv_[0].L.nx == L0.nx && lv_[0].L.ny == L0.ny && lv_[0].L.nz == L0.nz) return;
-    if (!lv_.empty()) throw std::runtime_error("multigrid hierarchy built for another grid");
+    if (built_ && lv_[0].L.nx == L0.nx && lv_[0].L.ny == L0.ny && lv_[0].L.nz == L0.nz) return;
+    if (built_) throw std::runtime_error("multigrid hierarchy built for another grid");
+    Release();  // what an earlier, failed Init left
     auto field = [](const FsLayout& L) {
         float* p = nullptr;
         check(hipMalloc(&p, sizeof(float) * (size_t)L.total));
         check(hipMemset(p, 0, sizeof(float) * (size_t)L.total));
         return p;
     };
+    // every level is pushed before its buffers are allocated, so a failed
+    // allocation leaves nothing the destructor does not free
     FsMgLevel v{};
     v.L = L0;
     v.idx2 = (float)idx2;
     v.idy2 = (float)idy2;
     v.idz2 = (float)idz2;
-    v.r = field(L0);
-    diag0_ = field(L0);
-    v.diag = diag0_;
     lv_.push_back(v);
+    lv_[0].r = field(L0);
+    diag0_ = field(L0);
+    lv_[0].diag = diag0_;
     auto too_big = [](const FsLayout& L) {
         return (long)L.nx * L.ny * L.nz > MG_COARSE_CELLS || (long)(L.nx + 2) * (L.ny + 2) * (L.nz + 2) > MG_CPAD;
     };
@@ -521,15 +529,11 @@ void FsMg::Init(const FsLayout& L0, double idx2, double idy2, double idz2) {
         c.idx2 = f.L.nx > 1 ? f.idx2 * 0.25f : f.idx2;
         c.idy2 = f.L.ny > 1 ? f.idy2 * 0.25f : f.idy2;
         c.idz2 = f.L.nz > 1 ? f.idz2 * 0.25f : f.idz2;
-        c.bx = field(c.L);
-        c.by = field(c.L);
-        c.bz = field(c.L);
-        c.diag = field(c.L);
-        c.x = field(c.L);
-        c.b = field(c.L);
-        c.r = field(c.L);
         lv_.push_back(c);
+        FsMgLevel& n = lv_.back();
+        for (float** buf : {&n.bx, &n.by, &n.bz, &n.diag, &n.x, &n.b, &n.r}) *buf = field(n.L);
     }
+    built_ = true;
 }
 
 void FsMg::SetOperator(hipStream_t st, const float* bx, const float* by, const float* bz) {
