@@ -1481,6 +1481,10 @@ class EulerianFluidSolver3D {
     // the second stream, its events and the level-set copy, on first use
     void EnsureSecondStream() {
         if (st2) return;
+        // events and the copy first: the stream, made last, marks completion
+        if (!ev_ls) HIPCHECK(hipEventCreateWithFlags(&ev_ls, hipEventDisableTiming));
+        if (!ev_rn) HIPCHECK(hipEventCreateWithFlags(&ev_rn, hipEventDisableTiming));
+        if (!phi_r) phi_r = dfield();
         // The reinitialization stream runs on the lowest-indexed 3/4 of
         // the CUs (FS_REINIT_CU_FRAC overrides the fraction; 0 or 1
         // lifts the mask).  The projection beside it is latency-bound and
@@ -1502,9 +1506,6 @@ class EulerianFluidSolver3D {
             }
         }
         if (!st2) HIPCHECK(hipStreamCreate(&st2));
-        HIPCHECK(hipEventCreateWithFlags(&ev_ls, hipEventDisableTiming));
-        HIPCHECK(hipEventCreateWithFlags(&ev_rn, hipEventDisableTiming));
-        phi_r = dfield();
     }
 
     // EULERIAN_FLUID_SOLVER_3D::CFLOneTimeStep, :2673-2785
