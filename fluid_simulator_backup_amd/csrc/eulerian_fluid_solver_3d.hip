@@ -874,6 +874,12 @@ class EulerianFluidSolver3D {
         }
         mg.probe = sampling ? &kprobe[FS_KS_MG_SMOOTH] : nullptr;
         mg.SetOperator(st, bx32, by32, bz32);
+        if (!mg.Tuned()) {
+            // time the sweeps with the GPU to themselves: let a concurrent
+            // reinitialization finish first (once per handle)
+            if (st2) HIPCHECK(hipStreamSynchronize(st2));
+            mg.Tune(st, (float*)w32.r, (float*)w32.s);
+        }
         init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
         const double rt2 = P.rel_tolerance * P.rel_tolerance;
         double bb = 0, rr = true_residual(bb);

@@ -52,6 +52,12 @@ class FsMg {
     // which changes no value.
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
+    bool Tuned() const { return tuned_ || split_; }
+    // tunes the fused sweep now, on the finest level with b = r and x = z
+    // (read, and the level's own residual buffer written); Apply tunes on its
+    // first call otherwise.  A caller with other streams drains them first so
+    // the timings are the sweep's.
+    void Tune(hipStream_t st, const float* r, float* z);
     // red-black sweeps before and after the coarse correction (an odd count
     // ends the fused sweeps' ping-pong in the residual buffer, copied back)
     void SetSweeps(int nu) {
@@ -66,7 +72,6 @@ class FsMg {
    private:
     void Cycle(hipStream_t st, int l);
     void Smooth(hipStream_t st, int l, int c1);
-    void Tune(hipStream_t st);
     void Release();
     bool tuned_ = false, built_ = false;
     std::vector<FsMgLevel> lv_;

@@ -614,7 +614,9 @@ void FsMg::Cycle(hipStream_t st, int l) {
 // workgroup and keep the fastest.  The sweeps read x and b and write the
 // level's residual buffer, which holds nothing yet; each cell's arithmetic
 // does not depend on the chunking, so the choice changes no value.
-void FsMg::Tune(hipStream_t st) {
+void FsMg::Tune(hipStream_t st, const float* r, float* z) {
+    lv_[0].b = (float*)r;
+    lv_[0].x = z;
     hipEvent_t e0, e1;
     check(hipEventCreate(&e0));
     check(hipEventCreate(&e1));
@@ -656,7 +658,7 @@ void FsMg::Apply(hipStream_t st, const float* r, float* z) {
     FsMgLevel& v = lv_[0];
     v.b = (float*)r;
     v.x = z;
-    if (!tuned_ && !split_) Tune(st);
+    if (!tuned_ && !split_) Tune(st, r, z);
     check(hipMemsetAsync(z, 0, sizeof(float) * (size_t)v.L.total, st));
     Cycle(st, 0);
 }
