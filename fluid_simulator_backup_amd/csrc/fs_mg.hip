@@ -15,7 +15,7 @@
 #define MG_COARSE_CELLS 512  // coarsen while a level has more cells
 #endif
 #ifndef MG_COARSE_SWEEPS
-#define MG_COARSE_SWEEPS 24  // red-black sweeps of the coarsest level, each way
+#define MG_COARSE_SWEEPS 12  // red-black sweeps of the coarsest level, each way (24: no fewer iterations; 6: more)
 #endif
 
 namespace {
