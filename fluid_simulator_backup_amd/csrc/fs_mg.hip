@@ -36,17 +36,21 @@ __device__ __forceinline__ float flux_row(const FsLayout& L, const float* __rest
            bz[c + L.sy] * (iz * (xc - x[c + L.sy]));
 }
 
-// block tree sum of one double per thread into part[block]
+// workgroup sum of one double per thread into part[block], in a fixed order:
+// a butterfly within each 64-lane wave (MBX = 64, one wave per row of the
+// block), then the MBY wave sums in row order
 __device__ __forceinline__ void block_partial(double v, double* part) {
-    __shared__ double sh[MNT];
-    const int t = threadIdx.y * MBX + threadIdx.x;
-    sh[t] = v;
+    static_assert(MBX == 64, "one wave per block row");
+    __shared__ double sh[MBY];
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o);
+    if (threadIdx.x == 0) sh[threadIdx.y] = v;
     __syncthreads();
-    for (int s = MNT / 2; s > 0; s >>= 1) {
-        if (t < s) sh[t] += sh[t + s];
-        __syncthreads();
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        double s = sh[0];
+        for (int w = 1; w < MBY; w++) s += sh[w];
+        part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = s;
     }
-    if (t == 0) part[((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = sh[0];
+    __syncthreads();  // sh is reused by a second call in the same kernel
 }
 
 // 1-D prolongation weight of coarse cell C for fine cell f (trilinear,
