@@ -3,19 +3,25 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--n 256]
 
 A step is one frame of SIMULATION_WORLD::AdvanceOneFrame on the oil/water
-rising-drop scene at n^3 (default 256^3): level-set WENO5 stages, ENO3 MAC
-advection, gravity, explicit viscosity, the variable-density projection
-solved to 1e-8 relative residual, and 2 x 25 reinitialization sweeps.  The
-headline value is ms/step with the FP32 inner solve inside FP64 iterative
-refinement, the inner solve being CG preconditioned by a geometric multigrid
-V-cycle; FP32 Jacobi-PCG inside the same refinement and the plain FP64
-Jacobi-PCG path run the same frames from the same state for comparison
-(`--paths`).  Inputs are resident on the GPU before the timed region; every
-step is synchronous.
+rising-drop scene at n^3 (default 256^3): the CFL rule's dt for the current
+velocity (recomputed before every frame, SIMULATION_WORLD.h:288-312),
+level-set WENO5 stages, ENO3 MAC advection, gravity, explicit viscosity, the
+variable-density projection solved to 1e-8 relative residual, and 2 x 25
+reinitialization sweeps.  The default run is the workload's 50 frames from
+rest (5 untimed, 45 timed).  The headline value is ms/step with the FP32
+inner solve inside FP64 iterative refinement, the inner solve being CG
+preconditioned by a geometric multigrid V-cycle; FP32 Jacobi-PCG inside the
+same refinement and the plain FP64 Jacobi-PCG path run the same frames from
+the same state for comparison (`--paths`), each on its own, so that one
+path's failure (named by frame, dt and residual) leaves the others'
+figures.  Inputs are resident on the GPU before the timed region; every step
+is synchronous.
 
 One JSON line is printed.  `roofline` is for the dominant kernel (timed with
 HIP events on the solver's stream); `cpu_baseline` times the CPU checker on
-the host on a bounded sample of the same workload (see `sample`).
+the host: two complete frames of the same workload with the PCG capped
+(see `sample`), and the 64^3 dam break's 20 frames end to end on the CPU
+and on the GPU (`c0_end_to_end`).
 
 Other workloads (each its own JSON line): `--workload batch` (48 uneven
 scenes advanced together against one at a time), `--workload adjoint` (the
@@ -40,11 +46,13 @@ HBM_PEAK_GBS = 8000.0
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=45)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=None, help="grid edge (default 256; 128 for the adjoint workload)")
     ap.add_argument("--scene", default="rising_drop")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-c0", action="store_true",
+                    help="skip the end-to-end CPU/GPU comparison on the 64^3 dam break (20 frames)")
     ap.add_argument("--paths", default="mg,refine,fp64",
                     help="projection paths run on the same frames; the first is the headline: mg (FP32 CG with a "
                          "multigrid preconditioner inside FP64 refinement), refine (FP32 Jacobi-PCG inside FP64 "
@@ -68,21 +76,6 @@ def parse():
     ap.add_argument("--scenes", type=int, default=48)
     ap.add_argument("--threads", type=int, default=0)
     return ap.parse_args()
-
-
-def run_frames(solver, dt, warmup, steps, sync):
-    from fluid_simulator_backup_amd import _abi
-    for _ in range(warmup):
-        solver.advance_frame(dt)
-    sync()
-    _abi.set_kernel_sampling(solver, True)
-    stats = []
-    t0 = time.perf_counter()
-    for _ in range(steps):
-        stats.append(solver.advance_frame(dt))
-    sync()
-    t1 = time.perf_counter()
-    return (t1 - t0) * 1e3 / steps, stats
 
 
 def measured_traffic(scene, n, kernel):
@@ -149,58 +142,97 @@ def divergent_velocity(n):
     return u, v, w
 
 
-def cpu_baseline(scene, n, iters_per_step, dt, threads):
-    """Per-step time of the CPU checker at n^3, composed from a bounded
-    sample of the same frame: the projection timed with PCG capped at 4 and
-    at 24 iterations (assembly + per-iteration cost), and each other phase of
-    the frame timed once with the frame's own parameters."""
-    from fluid_simulator_backup_amd import _abi, scenes
+def cpu_frames_timed(lib, scene, n, frames, max_iteration=20000, allow_cap=False):
+    """Frames of the CPU checker from the scene's initial state through the
+    shared frame loop (dt recomputed per frame, CFL 0.5, |g|), each timed end
+    to end; with allow_cap, a projection stopped by max_iteration is kept.
+    Returns (seconds per frame, records)."""
+    from fluid_simulator_backup_amd import _abi, frame_loop, scenes
+    phi, vel, P, g = scenes.make(scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
+                                 max_iteration=max_iteration)
+    s = _abi.Solver(lib, "fso_", P, g)
+    lib.fso_set_viscosity_inplace(s.h, 0)  # the HIP path's out-of-place order
+    s.set_fields(phi, *vel)
+    secs, recs = [], []
+    for f in range(frames):
+        dt = s.cfl_dt(0.5, 1)
+        t = time.perf_counter()
+        rc, st = frame_loop.advance_frame_rc(s, dt)
+        secs.append(time.perf_counter() - t)
+        if rc != _abi.FS_OK and not (allow_cap and rc == _abi.FS_ERR_NOT_CONVERGED):
+            s.close()
+            raise frame_loop.FrameError(f, dt, rc, "CPU checker", st)
+        recs.append(dict(st, dt=dt))
+    s.close()
+    return secs, recs
+
+
+def cpu_baseline(scene, n, iters_per_step, threads, cap=(100, 300)):
+    """Per-step time of the CPU checker (C, OpenMP, FP64, CSR Jacobi-PCG as
+    the reference) on the headline workload, from two complete frames timed
+    end to end: the scene's first frame with the PCG capped at cap[0] and
+    again at cap[1] iterations.  Their difference is the per-iteration cost;
+    the per-step value adds the FP64 path's remaining iterations at it (a
+    full 256^3 CPU frame to 1e-8 runs for minutes)."""
+    from fluid_simulator_backup_amd import _abi
     lib = _abi.load(os.path.join(ROOT, "oracle", "libfs_oracle.so"), "fso_")
     lib.fso_set_threads(threads)
-    phi, _, P, g = scenes.make(scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8)
-    vel = divergent_velocity(n)
-
-    def timed(fn):
-        t = time.perf_counter()
-        fn()
-        return time.perf_counter() - t
-
-    def fresh(max_it):
-        P.max_iteration = max_it
-        s = _abi.Solver(lib, "fso_", P, g)
-        lib.fso_set_viscosity_inplace(s.h, 0)  # the HIP path's out-of-place order
-        s.set_fields(phi, *vel)
-        return s
-
-    t_proj = {}
-    for cap in (4, 24):
-        s = fresh(cap)
-        s.projection(dt, allow_not_converged=True)  # first touch of the system's pages
-        s.set_fields(phi, *vel)
-        t_proj[cap] = timed(lambda: s.projection(dt, allow_not_converged=True))
-        if cap == 4:
-            s.close()
-    t_iter = (t_proj[24] - t_proj[4]) / 20
-    t_fixed = t_proj[4] - 4 * t_iter
-    t_curv = timed(s.compute_curvatures)
-    t_ls = timed(lambda: s.phase("levelset_advection", dt))
-    t_vel = timed(lambda: s.phase("velocity_advection", dt))
-    t_grav = timed(lambda: s.phase("apply_gravity", dt))
-    t_visc = timed(lambda: s.phase("sourcing", dt))
-    t_reinit = timed(lambda: s.phase("reinitialization", dt))
-    s.close()
-    # a frame: curvature ghost refresh, 2 level-set stages, ENO3 MAC stage,
-    # gravity, viscosity, projection, 2 reinitialization calls of 25 sweeps
-    per_step = t_curv + 2 * t_ls + t_vel + t_grav + t_visc + t_fixed + iters_per_step * t_iter + 2 * t_reinit
-    timed_s = t_proj[4] + t_proj[24] + t_curv + t_ls + t_vel + t_grav + t_visc + t_reinit
+    t = {}
+    for c in cap:
+        secs, _ = cpu_frames_timed(lib, scene, n, 1, max_iteration=c, allow_cap=True)
+        t[c] = secs[0]
+        print(f"[bench] cpu baseline: one {n}^3 frame, PCG capped at {c}: {t[c]:.2f} s", file=sys.stderr, flush=True)
+    t_iter = (t[cap[1]] - t[cap[0]]) / (cap[1] - cap[0])
+    per_step = t[cap[0]] + max(iters_per_step - cap[0], 0) * t_iter
     return {
         "value": round(per_step * 1e3, 1), "unit": "ms/step", "cores": threads, "kind": "port",
-        "sample": (f"CPU checker (C, OpenMP, FP64) at {n}^3 with {threads} threads on a divergent velocity field: "
-                   f"projection capped at 4 and 24 PCG iterations -> {t_fixed:.2f} s assembly + "
-                   f"{t_iter * 1e3:.1f} ms/iteration, scaled to the FP64 path's {iters_per_step:.0f} iterations; "
-                   f"one 25-sweep reinitialization call (x2), level-set stage (x2), ENO3 MAC stage, gravity, "
-                   f"viscosity, curvature ghost refresh, each timed once; {timed_s:.1f} s timed"),
+        "sample": (f"CPU checker (C, OpenMP, FP64, CSR Jacobi-PCG) with {threads} threads: the {scene} {n}^3 "
+                   f"first frame timed end to end twice, PCG capped at {cap[0]} ({t[cap[0]]:.2f} s) and at "
+                   f"{cap[1]} ({t[cap[1]]:.2f} s) iterations -> {t_iter * 1e3:.1f} ms/iteration; value = the "
+                   f"{cap[0]}-iteration frame + the FP64 GPU path's remaining {max(iters_per_step - cap[0], 0):.0f} "
+                   f"iterations per frame at that cost (a model of the full frame from measured frames); "
+                   f"{t[cap[0]] + t[cap[1]]:.1f} s timed"),
+        "frame_capped_s": {str(c): round(t[c], 3) for c in cap},
+        "ms_per_pcg_iteration": round(t_iter * 1e3, 3),
     }
+
+
+def c0_comparison(lib_gpu, threads, frames=20, n=64):
+    """The reference's CPU-runnable case measured end to end on both sides:
+    the air/water dam break at 64^3, `frames` frames of the plain FP64
+    method (WENO5 level set with reinitialization, ENO3 MAC advection,
+    explicit viscosity, Jacobi-PCG to 1e-8 relative residual) on the CPU
+    checker, and the same frames on the GPU with FP64 and with the headline
+    multigrid path."""
+    from fluid_simulator_backup_amd import _abi, frame_loop, scenes
+    lib = _abi.load(os.path.join(ROOT, "oracle", "libfs_oracle.so"), "fso_")
+    lib.fso_set_threads(threads)
+    secs, recs = cpu_frames_timed(lib, "dam_break", n, frames)
+    out = {"workload": f"dam_break {n}^3 (air/water 1/1000), {frames} frames from rest, dt from the CFL rule each "
+                       f"frame (0.5, |g|), projection to 1e-8 relative residual",
+           "cpu": {"ms_per_step": round(1e3 * sum(secs) / frames, 2), "cores": threads,
+                   "pcg_iterations_per_step": float(np.mean([r["pcg_iterations"] for r in recs]))}}
+    print(f"[bench] c0 dam break {n}^3 CPU checker: {out['cpu']['ms_per_step']:.1f} ms/step", file=sys.stderr,
+          flush=True)
+    import torch
+    for path in ("fp64", "mg"):
+        phi, vel, P, g = scenes.make("dam_break", n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
+                                     max_iteration=20000, precision=getattr(_abi, PATHS[path]))
+        s = _abi.Solver(lib_gpu, "fs_", P, g, 0)
+        s.set_fields(phi, *vel)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        grecs = frame_loop.advance_frames(s, frames)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3 / frames
+        f = s.get_fields()
+        s.close()
+        out[path] = {"ms_per_step": round(ms, 3), "speedup_vs_cpu": round(out["cpu"]["ms_per_step"] / ms, 2),
+                     "pcg_iterations_per_step": float(np.mean([r["pcg_iterations"] for r in grecs])),
+                     "max_residual_rel": max(r["residual_rel"] for r in grecs)}
+        print(f"[bench] c0 dam break {n}^3 GPU {LABEL[path]}: {ms:.2f} ms/step", file=sys.stderr, flush=True)
+        del f
+    return out
 
 
 def batch_main(a):
@@ -211,7 +243,7 @@ def batch_main(a):
     import torch
 
     import fluid_simulator_backup_amd as fsa
-    from fluid_simulator_backup_amd import _abi, scenes
+    from fluid_simulator_backup_amd import _abi, frame_loop, scenes
     torch.cuda.set_device(0)
     lib = fsa.load_library()
     prec = getattr(_abi, PATHS[a.batch_path])
@@ -221,16 +253,21 @@ def batch_main(a):
     batch = _abi.Batch(lib, [s[3] for s in sc], [s[4] for s in sc], 0, a.threads)
     for (_, phi, vel, _, _), bs in zip(sc, batch.scenes):
         bs.set_fields(phi, *vel)
-    dts = [bs.cfl_dt(0.5, 1) for bs in batch.scenes]
     print(f"[bench] batch of {len(sc)} scenes, {cells / 1e6:.1f} M cells, warmup={a.warmup} steps={a.steps}",
           file=sys.stderr, flush=True)
+
+    def frame_batch():
+        # every scene's dt from the CFL rule on its own velocity, each frame
+        dts = [bs.cfl_dt(0.5, 1) for bs in batch.scenes]
+        return sum(st["pcg_iterations"] for st in batch.advance_frame(dts))
+
     for _ in range(a.warmup):
-        batch.advance_frame(dts)
+        frame_batch()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     its = 0
     for _ in range(a.steps):
-        its += sum(st["pcg_iterations"] for st in batch.advance_frame(dts))
+        its += frame_batch()
     torch.cuda.synchronize()
     t_batch = time.perf_counter() - t0
     batch.close()
@@ -238,15 +275,13 @@ def batch_main(a):
     # the same scenes, one at a time
     t_seq = 0.0
     its_seq = 0
-    for (_, phi, vel, P, g), dt in zip(sc, dts):
+    for (_, phi, vel, P, g) in sc:
         s = _abi.Solver(lib, "fs_", P, g, 0)
         s.set_fields(phi, *vel)
-        for _ in range(a.warmup):
-            s.advance_frame(dt)
+        frame_loop.advance_frames(s, a.warmup)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        for _ in range(a.steps):
-            its_seq += s.advance_frame(dt)["pcg_iterations"]
+        its_seq += sum(r["pcg_iterations"] for r in frame_loop.advance_frames(s, a.steps))
         torch.cuda.synchronize()
         t_seq += time.perf_counter() - t0
         s.close()
@@ -447,38 +482,68 @@ LABEL = {"mg": "fp32+multigrid+refine", "refine": "fp32+jacobi+refine", "fp64": 
 
 
 def run_path(lib, scene, n, path, warmup, steps, sync):
-    """Frames of one projection path from the same initial state; returns
-    the timing, the solver statistics, the in-loop kernel samples and the
-    final fields."""
-    from fluid_simulator_backup_amd import _abi, scenes
+    """Frames of one projection path from the scene's initial state, each
+    with dt recomputed by the CFL rule (SIMULATION_WORLD.h:288-312): `warmup`
+    untimed frames, then `steps` timed ones (the CFL reduction and the host's
+    read of dt inside the timed region, as in the reference's frame).
+    Returns the timing, every frame's record, the in-loop kernel samples and
+    the final fields."""
+    from fluid_simulator_backup_amd import _abi, frame_loop, scenes
     prec = getattr(_abi, PATHS[path])
     phi, vel, P, g = scenes.make(scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
                                  precision=prec, max_iteration=20000)
     s = _abi.Solver(lib, "fs_", P, g, 0)
-    s.set_fields(phi, *vel)
-    dt = s.cfl_dt(0.5, 1)
-    print(f"[bench] {scene} {n}^3 dt={dt:.4e} {LABEL[path]} warmup={warmup} steps={steps}", file=sys.stderr,
-          flush=True)
-    ms, st = run_frames(s, dt, warmup, steps, sync)
-    print(f"[bench] {LABEL[path]} {ms:.2f} ms/step; last {st[-1]}", file=sys.stderr, flush=True)
-    out = {"ms_per_step": ms, "stats": st, "dt": dt,
-           "frame": {k: _abi.frame_kernel_samples(s, getattr(_abi, k)) for k in ("FS_KS_REINIT", "FS_KS_MG_SMOOTH")}}
-    if path != "mg":
-        out["pcg"] = _abi.kernel_samples(s, prec)
-        if path == "refine":
-            out["pcg_repeat"] = _abi.time_pcg_kernels(s, prec, 20)
-    out["fields"] = s.get_fields()
-    s.close()
-    return out
+    try:
+        s.set_fields(phi, *vel)
+        del phi, vel
+        print(f"[bench] {scene} {n}^3 {LABEL[path]} warmup={warmup} steps={steps}", file=sys.stderr, flush=True)
+
+        def progress(tag):
+            def cb(r):
+                if r["frame"] % 10 == 9 or r["frame"] == 0:
+                    print(f"[bench] {LABEL[path]} {tag} frame {r['frame']}: dt {r['dt']:.4e}, "
+                          f"{r['pcg_iterations']} iterations, residual {r['residual_rel']:.2e}, "
+                          f"step {r['step_ms']:.1f} ms", file=sys.stderr, flush=True)
+            return cb
+
+        warm = frame_loop.advance_frames(s, warmup, on_frame=progress("warmup"))
+        sync()
+        _abi.set_kernel_sampling(s, True)
+        t0 = time.perf_counter()
+        recs = frame_loop.advance_frames(s, steps, on_frame=progress("timed"))
+        sync()
+        ms = (time.perf_counter() - t0) * 1e3 / steps
+        print(f"[bench] {LABEL[path]} {ms:.2f} ms/step", file=sys.stderr, flush=True)
+        out = {"ms_per_step": ms, "stats": recs, "warmup_stats": warm,
+               "frame": {k: _abi.frame_kernel_samples(s, getattr(_abi, k))
+                         for k in ("FS_KS_REINIT", "FS_KS_MG_SMOOTH")}}
+        if path != "mg":
+            out["pcg"] = _abi.kernel_samples(s, prec)
+            if path == "refine":
+                out["pcg_repeat"] = _abi.time_pcg_kernels(s, prec, 20)
+        out["fields"] = s.get_fields()
+        return out
+    finally:
+        s.close()
 
 
 def summary(r):
     st = r["stats"]
+    allf = r["warmup_stats"] + st
+    dts = [s["dt"] for s in allf]
     return {"ms_per_step": round(r["ms_per_step"], 3),
             "pcg_iterations_per_step": float(np.mean([s["pcg_iterations"] for s in st])),
             "refinement_sweeps_per_step": float(np.mean([s["refinement_sweeps"] for s in st])),
             "projection_ms": round(float(np.mean([s["projection_ms"] for s in st])), 3),
-            "residual_rel": st[-1]["residual_rel"]}
+            "residual_rel": st[-1]["residual_rel"],
+            "max_residual_rel": max(s["residual_rel"] for s in allf),
+            "frames": len(allf), "dt_min": min(dts), "dt_max": max(dts),
+            "max_cfl_start": max(s["cfl_start"] for s in allf),
+            "per_frame": {"dt": [float(f"{d:.6e}") for d in dts],
+                          "pcg_iterations": [s["pcg_iterations"] for s in allf],
+                          "refinement_sweeps": [s["refinement_sweeps"] for s in allf],
+                          "residual_rel": [float(f"{s['residual_rel']:.3e}") for s in allf],
+                          "step_ms": [round(s["step_ms"], 2) for s in allf]}}
 
 
 def field_error(fa, fb):
@@ -520,15 +585,16 @@ def pcg_roofline(k, cnt, k_rep, cells, n, wb):
 
 
 # Frame kernels sampled in the loop: algorithmic and moved bytes per cell.
-# k_reinit, one Sussman sweep in FP64: read phi^n (its 19-point stencil) and
-# the sign field, write phi^{n+1} (3 words); it reads the ghost copy, the
-# sign field and writes the next copy's box (3 words moved), then a small
-# kernel fills the copy's ghost layers.  k_mg_rb_sweep, one red-black
-# Gauss-Seidel sweep of the finest multigrid level in FP32: read x, b and
-# the three face-coefficient fields, write x (6 words; the diagonal is
-# formed from the coefficients); the same 6 words moved, plus the one-cell
-# ring of the first colour's operands that neighbouring tiles also read.
-FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 3 * 8, 3 * 8),
+# k_reinit, one Sussman sweep in FP64, in the workload's fixed accounting: 6
+# words (the ghost copy of phi written and read, the sign field, phi^n read,
+# phi^{n+1} written); this build's sweep moves 3 of them (it reads the ghost
+# copy and the sign field and writes the next copy's box; a small kernel
+# then fills that copy's ghost layers).  k_mg_rb_sweep, one red-black
+# Gauss-Seidel sweep of the finest multigrid level in FP32: read x, b and the
+# three face-coefficient fields, write x (6 words; the diagonal is formed
+# from the coefficients); the same 6 words moved, plus the one-cell ring of
+# the first colour's operands that neighbouring tiles also read.
+FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 6 * 8, 3 * 8),
                  "k_mg_rb_sweep": ("FS_KS_MG_SMOOTH", 6 * 4, 6 * 4)}
 # what bounds each, for the roofline note
 LIMITER = {"k_reinit": "FP64 VALU issue: the WENO5 smoothness weights take 24 IEEE divisions per cell and every "
@@ -536,6 +602,61 @@ LIMITER = {"k_reinit": "FP64 VALU issue: the WENO5 smoothness weights take 24 IE
                        "below the HBM bound (see the VALU counter summary under profiles/); in the frame it runs "
                        "on a second stream confined to `reinit_cu_fraction` of the CUs, beside the projection",
            "k_mg_rb_sweep": "HBM and load latency of the staged first colour"}
+
+
+def frame_roofline(R, scene, n, steps):
+    """Roofline of the multigrid path's dominant sampled frame kernel (the
+    larger in-loop time per frame)."""
+    cells = n ** 3
+    share = {}
+    for kn, (ks, _, _) in FRAME_KERNELS.items():
+        avg, n_s, launches = R["frame"][ks]
+        share[kn] = avg * launches / steps
+    kn = max(share, key=share.get)
+    ks, alg, mov = FRAME_KERNELS[kn]
+    avg, n_s, launches = R["frame"][ks]
+    bytes_launch = alg * cells
+    achieved = bytes_launch / (avg * 1e-3) / 1e9
+
+    def rates(k):
+        a = max(R["frame"][FRAME_KERNELS[k][0]][0], 1e-9)
+        return {"avg_launch_ms": round(R["frame"][FRAME_KERNELS[k][0]][0], 5),
+                "launches_per_step": R["frame"][FRAME_KERNELS[k][0]][2] / steps,
+                "ms_per_step": round(share[k], 3),
+                "algorithmic_bytes_per_cell": FRAME_KERNELS[k][1], "moved_bytes_per_cell": FRAME_KERNELS[k][2],
+                "algorithmic_gbs": round(FRAME_KERNELS[k][1] * cells / (a * 1e-3) / 1e9, 1),
+                "moved_gbs": round(FRAME_KERNELS[k][2] * cells / (a * 1e-3) / 1e9, 1)}
+
+    return {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(scene, n, kn),
+            "limiter": LIMITER[kn], "valu": measured_valu(scene, n, kn, avg),
+            "bytes_per_launch": bytes_launch, "moved_bytes_per_launch": mov * cells,
+            "moved_frac": round(mov * cells / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "avg_launch_ms": round(avg, 5),
+            "timing": f"HIP events on the solver stream around one launch per call of the timed frames "
+                      f"({n_s} samples of {launches} launches)",
+            "frame_kernels": {k: rates(k) for k in FRAME_KERNELS}}
+
+
+def jacobi_roofline(R, head, scene, n):
+    cells = n ** 3
+    k, cnt = R["pcg"]
+    k = k[:2]
+    wb = 4 if head == "refine" else 8
+    pr = pcg_roofline(k, cnt, R.get("pcg_repeat"), cells, n, wb)
+    names = ["pspmv_dot", "update"]
+    dom = int(np.argmax(k))
+    kn = {"pspmv_dot": "k_pspmv_dot", "update": "k_update"}[names[dom]]
+    bytes_launch = pr["algorithmic_words_per_cell"][names[dom]] * cells * wb
+    achieved = bytes_launch / (k[dom] * 1e-3) / 1e9
+    roof = {"bound": "hbm", "kernel": f"{kn}<{'float' if wb == 4 else 'double'}>", "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": measured_traffic(scene, n, kn) if wb == 4 else None,
+            "bytes_per_launch": bytes_launch, "avg_launch_ms": round(float(k[dom]), 5),
+            "timing": f"HIP events on the solver stream around one iteration per 16 of the timed frames "
+                      f"({int(cnt[dom])} samples)"}
+    roof.update(pr)
+    return roof
 
 
 def main():
@@ -575,113 +696,94 @@ def main():
         sys.exit(f"--paths: a comma-separated subset of {','.join(PATHS)}")
     head = paths[0]
     cells = a.n ** 3
-    runs = {p: run_path(lib, a.scene, a.n, p, a.warmup, a.steps, sync) for p in paths}
-    R = runs[head]
-    ms = R["ms_per_step"]
-    dt = R["dt"]
-
-    # roofline for the headline path's dominant sampled kernel (largest
-    # in-loop time per frame)
-    if head == "mg":
-        share = {}
-        for kn, (ks, _, _) in FRAME_KERNELS.items():
-            avg, n_s, launches = R["frame"][ks]
-            share[kn] = avg * launches / a.steps
-        kn = max(share, key=share.get)
-        ks, alg, mov = FRAME_KERNELS[kn]
-        avg, n_s, launches = R["frame"][ks]
-        bytes_launch = alg * cells
-        achieved = bytes_launch / (avg * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(a.scene, a.n, kn),
-                "limiter": LIMITER[kn], "valu": measured_valu(a.scene, a.n, kn, avg),
-                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(avg, 5),
-                "timing": f"HIP events on the solver stream around one launch per call of the timed frames "
-                          f"({n_s} samples of {launches} launches)",
-                "frame_kernels": {k: {"avg_launch_ms": round(R["frame"][FRAME_KERNELS[k][0]][0], 5),
-                                      "launches_per_step": R["frame"][FRAME_KERNELS[k][0]][2] / a.steps,
-                                      "ms_per_step": round(share[k], 3),
-                                      "algorithmic_bytes_per_cell": FRAME_KERNELS[k][1],
-                                      "moved_bytes_per_cell": FRAME_KERNELS[k][2],
-                                      "algorithmic_gbs": round(FRAME_KERNELS[k][1] * cells /
-                                                               (max(R["frame"][FRAME_KERNELS[k][0]][0], 1e-9) * 1e-3)
-                                                               / 1e9, 1),
-                                      "moved_gbs": round(FRAME_KERNELS[k][2] * cells /
-                                                         (max(R["frame"][FRAME_KERNELS[k][0]][0], 1e-9) * 1e-3) / 1e9,
-                                                         1)}
-                                  for k in FRAME_KERNELS}}
-    else:
-        k, cnt = R["pcg"]
-        k = k[:2]
-        wb = 4 if head == "refine" else 8
-        pr = pcg_roofline(k, cnt, R.get("pcg_repeat"), cells, a.n, wb)
-        names = ["pspmv_dot", "update"]
-        dom = int(np.argmax(k))
-        kn = {"pspmv_dot": "k_pspmv_dot", "update": "k_update"}[names[dom]]
-        bytes_launch = pr["algorithmic_words_per_cell"][names[dom]] * cells * wb
-        achieved = bytes_launch / (k[dom] * 1e-3) / 1e9
-        roof = {"bound": "hbm", "kernel": f"{kn}<{'float' if wb == 4 else 'double'}>", "achieved": round(achieved, 1),
-                "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                "traffic": measured_traffic(a.scene, a.n, kn) if wb == 4 else None,
-                "bytes_per_launch": bytes_launch, "avg_launch_ms": round(float(k[dom]), 5),
-                "timing": f"HIP events on the solver stream around one iteration per 16 of the timed frames "
-                          f"({int(cnt[dom])} samples)"}
-        roof.update(pr)
-
+    # each path on its own: a failure is recorded with its frame and residual
+    # and does not stop the others
+    runs, errors = {}, {}
+    for p in paths:
+        try:
+            runs[p] = run_path(lib, a.scene, a.n, p, a.warmup, a.steps, sync)
+        except Exception as e:
+            errors[p] = {"error": str(e)}
+            if hasattr(e, "frame"):
+                errors[p].update({"frame": e.frame, "dt": e.dt,
+                                  "residual_rel": e.stats["residual_rel"] if e.stats else None})
+            print(f"[bench] {LABEL[p]} failed: {e}", file=sys.stderr, flush=True)
+    R = runs.get(head)
+    roof = None
+    if R is not None:
+        try:
+            roof = frame_roofline(R, a.scene, a.n, a.steps) if head == "mg" else jacobi_roofline(R, head, a.scene, a.n)
+        except Exception as e:
+            roof = {"error": str(e)}
     res = {}
     for p in paths:
+        if p in errors:
+            res[p] = errors[p]
+            continue
         res[p] = summary(runs[p])
         if "pcg" in runs[p]:
             k, cnt = runs[p]["pcg"]
             res[p]["pcg_kernels"] = pcg_roofline(k[:2], cnt, runs[p].get("pcg_repeat"), cells, a.n,
                                                  8 if p == "fp64" else 4)
     if "fp64" in runs:
-        for p in paths:
+        for p in runs:
             if p != "fp64":
                 res[p]["error_vs_fp64"] = field_error(runs[p]["fields"], runs["fp64"]["fields"])
                 res[p]["speedup_vs_fp64"] = round(runs["fp64"]["ms_per_step"] / runs[p]["ms_per_step"], 3)
+    for r in runs.values():
+        r.pop("fields", None)
 
     cpu = None
     if not a.no_cpu_baseline:
         threads = a.cpu_threads or min(16, os.cpu_count() or 1)
-        iters = res["fp64"]["pcg_iterations_per_step"] if "fp64" in res else None
+        iters = res["fp64"].get("pcg_iterations_per_step") if "fp64" in res else None
         try:
             if iters is None:
                 raise RuntimeError("the CPU baseline scales the FP64 path's iteration count: run it (--paths ...,fp64)")
-            cpu = cpu_baseline(a.scene, a.n, iters, dt, threads)
+            cpu = cpu_baseline(a.scene, a.n, iters, threads)
         except Exception as e:  # the baseline must not sink the GPU line
             cpu = {"error": str(e)}
-    h = res[head]
+        if not a.no_c0:
+            try:
+                cpu["c0_end_to_end"] = c0_comparison(lib, threads)
+            except Exception as e:
+                cpu["c0_end_to_end"] = {"error": str(e)}
+    h = res.get(head, {})
+    ms = R["ms_per_step"] if R is not None else None
     out = {
         "metric": "ms/step at 256^3 two-phase, % HBM peak; FP32+refine vs FP64 at 1e-8 resid",
-        "value": round(ms, 3),
+        "value": round(ms, 3) if ms is not None else None,
         "unit": "ms/step",
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": round(ms, 3),
+        "ms_per_step": round(ms, 3) if ms is not None else None,
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere" if head != "fp64" else "f64",
         "data": "synthetic",
         "config": {"workload": f"{a.scene} {a.n}^3, air_bubble_rising (oil 800/0.05 in water 1000/1e-3), "
-                               f"dt={dt:.4e} (CFL 0.5, |g|), projection to 1e-8 relative residual, "
-                               f"{LABEL[head]} ({PATHS[head]})",
+                               f"{a.warmup} + {a.steps} frames from rest, dt from the CFL rule before every frame "
+                               f"(CFL 0.5, |g|), projection to 1e-8 relative residual, {LABEL[head]} ({PATHS[head]})",
                    "cells": cells},
         "path": LABEL[head],
-        "cells_per_s": round(cells / (ms * 1e-3), 1),
-        "pcg_iterations_per_step": h["pcg_iterations_per_step"],
-        "refinement_sweeps_per_step": h["refinement_sweeps_per_step"],
-        "residual_rel": h["residual_rel"],
-        "projection_ms": h["projection_ms"],
-        "fp64_ms_per_step": res["fp64"]["ms_per_step"] if "fp64" in res else None,
+        "cells_per_s": round(cells / (ms * 1e-3), 1) if ms else None,
+        "pcg_iterations_per_step": h.get("pcg_iterations_per_step"),
+        "refinement_sweeps_per_step": h.get("refinement_sweeps_per_step"),
+        "residual_rel": h.get("residual_rel"),
+        "max_residual_rel": h.get("max_residual_rel"),
+        "dt_range": [h.get("dt_min"), h.get("dt_max")],
+        "projection_ms": h.get("projection_ms"),
+        "fp64_ms_per_step": res["fp64"].get("ms_per_step") if "fp64" in res else None,
         "speedup_vs_fp64": h.get("speedup_vs_fp64"),
         "roofline": roof,
         "cpu_baseline": cpu,
         "paths": res,
     }
     print(json.dumps(out))
+    if errors:
+        sys.exit(1)
 
 
 if __name__ == "__main__":

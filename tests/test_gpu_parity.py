@@ -102,9 +102,11 @@ def test_phase_sequence(hip_lib, oracle_lib):
 
 def _frames(hip_lib, oracle_lib, name, n, frames, velocity_scale=False, **over):
     gpu, cpu = make_pair(hip_lib, oracle_lib, name, n=n, moving=False, **over)
-    dt = cpu.cfl_dt(0.5, 1)
     its = []
     for f in range(frames):
+        # the CFL rule before every frame (SIMULATION_WORLD.h:288-312), from
+        # the checker's velocity, and the same dt on both sides
+        dt = cpu.cfl_dt(0.5, 1)
         sg = gpu.advance_frame(dt)
         sc = cpu.advance_frame(dt)
         its.append((sg["pcg_iterations"], sc["pcg_iterations"]))
