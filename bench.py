@@ -332,7 +332,9 @@ def adjoint_main(a):
     def forward(vel):
         set_state(vel)
         for _ in range(a.rollout_steps):
-            s.step(dt)
+            # 1e-12 is below FP64's attainable residual at ratio 1000: the
+            # solve stops there and reports it, the step goes on
+            s.step(dt, allow_not_converged=True)
 
     sig = 8.0 * n / 128
     v_target = scenes.gaussian_random_velocity(n, 4, sigma_cells=sig)
@@ -349,7 +351,8 @@ def adjoint_main(a):
     loss0 = s.levelset_loss(target)
     set_state(v0)
     t0 = time.perf_counter()
-    loss, grads, st = s.adjoint_rollout(a.rollout_steps, dt, a.checkpoint_every, target)
+    loss, grads, st = s.adjoint_rollout(a.rollout_steps, dt, a.checkpoint_every, target,
+                                        allow_not_converged=True)
     torch.cuda.synchronize()
     t_adj = time.perf_counter() - t0
     print(f"[bench] adjoint {n}^3 steps={a.rollout_steps} dt={dt:.4e} loss={loss:.6e} forward {t_fwd:.2f} s, "

@@ -437,7 +437,7 @@ class Solver:
         self._check(self.lib.fs_levelset_loss(self.h, _ptr(t), C.byref(out)), "levelset_loss")
         return out.value
 
-    def adjoint_rollout(self, steps, dt, checkpoint_every, phi_target):
+    def adjoint_rollout(self, steps, dt, checkpoint_every, phi_target, allow_not_converged=False):
         """Loss of the level set after `steps` steps and its gradient with
         respect to the initial (phi, u, v, w); the fields are left at the
         initial state."""
@@ -447,7 +447,8 @@ class Solver:
         loss = _D()
         st = FsStepStats()
         self._check(self.lib.fs_adjoint_rollout(self.h, steps, dt, checkpoint_every, _ptr(t), C.byref(loss),
-                                                *[_ptr(g) for g in grads], C.byref(st)), "adjoint_rollout")
+                                                *[_ptr(g) for g in grads], C.byref(st)), "adjoint_rollout",
+                    (FS_ERR_NOT_CONVERGED,) if allow_not_converged else ())
         return loss.value, grads, st.as_dict()
 
     def apply_operator(self, x):

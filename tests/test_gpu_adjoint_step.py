@@ -20,8 +20,14 @@ from helpers import smooth_velocity
 
 pytestmark = pytest.mark.gpu
 
+# Solves stop at a relative residual of 1e-13, below FP64's attainable
+# accuracy for the density ratio 1000 systems (their true residual stays near
+# 1e-11): the FP64 path then reports FS_ERR_NOT_CONVERGED with the step
+# completed, which these tests accept (ALLOW), so that every solve is as
+# tight as FP64 allows and the finite differences see a smooth map.
 FIXED = dict(schedule=_abi.FS_SCHEDULE_FIXED, use_5th_weno_v=1, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
              rel_tolerance=1e-13, max_iteration=20000)
+ALLOW = dict(allow_not_converged=True)
 
 
 def _zero_walls(u, v, w):
@@ -51,7 +57,7 @@ def _set(s, x):
 
 def _step(s, x, dt):
     _set(s, x)
-    s.step(dt)
+    s.step(dt, **ALLOW)
     f = s.get_fields()
     return [f[c] for c in ("phi", "u", "v", "w")]
 
@@ -97,7 +103,7 @@ def test_adjoint_step_finite_differences(hip_lib, name, phase, order):
     adj = {}
     for key, lam in (("phi", lam_phi), ("vel", lam_vel)):
         _set(s, x)
-        adj[key], st = s.adjoint_step(dt, *lam)
+        adj[key], st = s.adjoint_step(dt, *lam, **ALLOW)
     # Richardson over (h, h/2) cancels the h^2 term of central differences.
     # h = 4e-6 (perturbations of 2e-7 per value) sits between the WENO5
     # weights' nonlinearity (their epsilon is 1e-6: larger steps bring higher
@@ -131,7 +137,7 @@ def test_adjoint_step_leaves_fields_and_rejects_unsupported(hip_lib):
     s.set_fields(phi, *vel)
     before = s.get_fields()
     lam = [np.ones_like(before[c]) for c in ("phi", "u", "v", "w")]
-    s.adjoint_step(1e-3, *lam)
+    s.adjoint_step(1e-3, *lam, **ALLOW)
     after = s.get_fields()
     for c in before:
         assert np.array_equal(before[c], after[c]), c
@@ -140,7 +146,7 @@ def test_adjoint_step_leaves_fields_and_rejects_unsupported(hip_lib):
         t, _ = _solver(hip_lib, "dam_break", n, **over)
         t.set_fields(phi, *vel)
         with pytest.raises(_abi.FsError) as e:
-            t.adjoint_step(1e-3, *lam)
+            t.adjoint_step(1e-3, *lam, **ALLOW)
         assert e.value.code == _abi.FS_ERR_UNSUPPORTED
 
 
@@ -161,7 +167,7 @@ def _slosh(hip_lib, n, seed, **over):
 def _target(s, phi, vel, steps, dt):
     _set(s, [phi] + vel)
     for _ in range(steps):
-        s.step(dt)
+        s.step(dt, **ALLOW)
     return s.get_fields()["phi"]
 
 
@@ -175,21 +181,21 @@ def test_rollout_equals_chained_step_adjoints(hip_lib):
     _, _, vel1 = _slosh(hip_lib, n, 41)
     x0 = [phi] + vel1
     _set(s, x0)
-    loss, grads, _ = s.adjoint_rollout(steps, dt, 2, target)
+    loss, grads, _ = s.adjoint_rollout(steps, dt, 2, target, **ALLOW)
     # chain by hand
     _set(s, x0)
     states = []
     for _ in range(steps):
         f = s.get_fields()
         states.append(f)
-        s.step(dt)
+        s.step(dt, **ALLOW)
     phiT = s.get_fields()["phi"]
     loss2, lphi = _loss_grad_numpy(phiT, target, 1.5 / n)
     assert abs(loss - loss2) <= 1e-12 * loss2
     lam = [lphi] + [np.zeros_like(a) for a in vel1]
     for f in reversed(states):
         s.set_fields(f["phi"], f["u"], f["v"], f["w"], p=f["p"])
-        lam, _ = s.adjoint_step(dt, *lam)
+        lam, _ = s.adjoint_step(dt, *lam, **ALLOW)
     for g, l in zip(grads, lam):
         assert np.abs(g - l).max() <= 1e-9 * np.abs(l).max()
 
@@ -204,7 +210,7 @@ def _directions(n, count, seed):
 def _rollout_loss(s, x, steps, dt, target):
     _set(s, x)
     for _ in range(steps):
-        s.step(dt)
+        s.step(dt, **ALLOW)
     return s.levelset_loss(target)
 
 
@@ -217,7 +223,7 @@ def test_rollout_gradient_finite_differences(hip_lib):
     target = _target(s, phi, vel, steps, dt)
     _, _, vel1 = _slosh(hip_lib, n, 41)
     _set(s, [phi] + vel1)
-    loss, grads, st = s.adjoint_rollout(steps, dt, 4, target)
+    loss, grads, st = s.adjoint_rollout(steps, dt, 4, target, **ALLOW)
     print("loss", loss, "stats", st)
     h = 5e-7
     for d in _directions(n, 3, 100):
@@ -243,7 +249,7 @@ def test_slosh_128_rollout_gradient(hip_lib):
     dt = s.cfl_dt(0.5, 1)
     target = _target(s, phi, vt, steps, dt)
     _set(s, [phi] + v0)
-    loss, grads, st = s.adjoint_rollout(steps, dt, 4, target)
+    loss, grads, st = s.adjoint_rollout(steps, dt, 4, target, **ALLOW)
     print("loss", loss, "dt", dt, "stats", st)
     h = 1e-6
     for q in range(2):
@@ -266,7 +272,7 @@ def test_rollout_gradient_against_cpu_checker(oracle_lib, hip_lib):
     target = _target(s, phi, vel, steps, dt)
     _, _, vel1 = _slosh(hip_lib, n, 41)
     _set(s, [phi] + vel1)
-    loss, grads, _ = s.adjoint_rollout(steps, dt, 2, target)
+    loss, grads, _ = s.adjoint_rollout(steps, dt, 2, target, **ALLOW)
     _, _, P, g = scenes.make("slosh", n, **FIXED)
     cpu = _abi.Solver(oracle_lib, "fso_", P, g)
     oracle_lib.fso_set_viscosity_inplace(cpu.h, 0)

@@ -1,0 +1,226 @@
+"""The benchmark's scenes at their frame counts, HIP path against the CPU
+checker, through the C-ABI.
+
+Each scene runs the reference's frame loop (SIMULATION_WORLD.h:288-312): the
+CFL rule's dt (0.5, |g|) from the current velocity before every frame, then
+one step with the projection solved to 1e-8 relative residual.  The
+checker's dt sequence is replayed on the GPU, so both sides advance the same
+inputs; the GPU's own CFL dt is checked against it at every frame.
+
+  - dam break (air/water 1/1000), 64^3, 20 frames, FP64;
+  - rising drop (oil/water 800/1000), 48^3, 50 frames, every projection
+    path (FP64, FP32 Jacobi-PCG inside FP64 refinement, FP32 multigrid CG
+    inside FP64 refinement) against the checker's FP64;
+  - 64-bubble pool (air/water 1/1000), 64^3, 10 frames, FP64 and multigrid,
+    with the iteration counts and, in the fixed schedule, the divergence
+    left by one projection;
+  - a batch of 48 uneven scenes (edges 32, 48, 64; dam break or rising drop;
+    density ratio 1.25-1000) advanced together for 2 frames on the multigrid
+    path, every scene against the checker run alone.
+
+Per-field tolerances (relative L2 and Linf of the difference over the
+checker field's own norm; pressure with its mean removed, since the weak
+Neumann pin holds its constant) are the TOL tables below.  Both sides stop
+at 1e-8, so the fields differ by the solves' tolerance carried through the
+frames; the velocity components that the air/water projection does not
+correct (u and v: only z faces are corrected, PROJECTION_3D.h:1188-1424) are
+three orders smaller than w, so they are compared against the largest
+component's norm.
+"""
+import numpy as np
+import pytest
+
+from fluid_simulator_backup_amd import _abi, frame_loop, scenes
+
+pytestmark = pytest.mark.gpu
+
+REL = dict(stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8, max_iteration=40000)
+PREC = {"fp64": _abi.FS_FP64, "refine": _abi.FS_FP32_REFINE, "mg": _abi.FS_FP32_MG}
+
+
+def _errors(fg, fc):
+    out = {}
+    vscale = max(np.linalg.norm(fc[c]) for c in ("u", "v", "w"))
+    for c in ("phi", "u", "v", "w", "p"):
+        a, b = fg[c], fc[c]
+        if c == "p":
+            a, b = a - a.mean(), b - b.mean()
+        d = a - b
+        ref_l2 = vscale if c in ("u", "v", "w") else np.linalg.norm(b)
+        ref_inf = max(np.abs(fc[k]).max() for k in ("u", "v", "w")) if c in ("u", "v", "w") else np.abs(b).max()
+        out[c] = (float(np.linalg.norm(d) / max(ref_l2, 1e-300)), float(np.abs(d).max() / max(ref_inf, 1e-300)))
+    return out
+
+
+def _oracle_run(oracle_lib, name, n, frames, **kw):
+    phi, vel, P, g = scenes.make(name, n, **dict(REL, **kw))
+    s = _abi.Solver(oracle_lib, "fso_", P, g)
+    oracle_lib.fso_set_viscosity_inplace(s.h, 0)  # the GPU's out-of-place order
+    s.set_fields(phi, *vel)
+    recs = frame_loop.advance_frames(s, frames)
+    f = s.get_fields()
+    s.close()
+    return recs, f
+
+
+def _gpu_replay(hip_lib, name, n, dts, precision, **kw):
+    """GPU frames with the checker's dt sequence; returns (records, fields,
+    the GPU's own CFL dt before each frame)."""
+    phi, vel, P, g = scenes.make(name, n, **dict(REL, precision=precision, **kw))
+    s = _abi.Solver(hip_lib, "fs_", P, g, 0)
+    s.set_fields(phi, *vel)
+    recs, own = [], []
+    for f, dt in enumerate(dts):
+        own.append(s.cfl_dt(0.5, 1))
+        rc, st = frame_loop.advance_frame_rc(s, dt)
+        if rc != _abi.FS_OK:
+            raise frame_loop.FrameError(f, dt, rc, s.lib.fs_last_error(s.h).decode(), st)
+        recs.append(st)
+    fields = s.get_fields()
+    s.close()
+    return recs, fields, own
+
+
+def _check(tag, errs, tol):
+    print(tag, {c: f"L2 {e[0]:.2e} Linf {e[1]:.2e}" for c, e in errs.items()})
+    for c, (l2, linf) in errs.items():
+        assert l2 <= tol[c][0] and linf <= tol[c][1], (tag, c, l2, linf, tol[c])
+
+
+def _check_dt(tag, own, dts):
+    rel = max(abs(a - b) / b for a, b in zip(own, dts))
+    print(tag, "max relative difference of the GPU's CFL dt", rel)
+    assert rel <= 1e-6, (tag, rel)
+
+
+# (relative L2, relative Linf) per field
+TOL_DAM_FP64 = {"phi": (1e-6, 1e-5), "u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "p": (1e-6, 1e-5)}
+TOL_DROP = {
+    "fp64": {"phi": (1e-6, 1e-5), "u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "p": (1e-6, 1e-5)},
+    "refine": {"phi": (1e-6, 1e-5), "u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "p": (1e-6, 1e-5)},
+    "mg": {"phi": (1e-6, 1e-5), "u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "p": (1e-6, 1e-5)},
+}
+TOL_BUBBLES = {"phi": (1e-6, 1e-5), "u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "p": (1e-6, 1e-5)}
+TOL_BATCH = {"phi": (1e-6, 1e-5), "u": (1e-5, 1e-4), "v": (1e-5, 1e-4), "w": (1e-5, 1e-4), "p": (1e-6, 1e-5)}
+
+
+def test_dam_break_64_20_frames_fp64(hip_lib, oracle_lib):
+    recs, fc = _oracle_run(oracle_lib, "dam_break", 64, 20)
+    dts = [r["dt"] for r in recs]
+    grecs, fg, own = _gpu_replay(hip_lib, "dam_break", 64, dts, _abi.FS_FP64)
+    _check_dt("dam break 64 fp64", own, dts)
+    for a, b in zip(grecs, recs):
+        assert a["residual_rel"] <= 1e-8 and b["residual_rel"] <= 1e-8
+        assert abs(a["pcg_iterations"] - b["pcg_iterations"]) <= 0.02 * b["pcg_iterations"] + 3
+    _check("dam break 64 fp64", _errors(fg, fc), TOL_DAM_FP64)
+
+
+@pytest.fixture(scope="module")
+def drop48(oracle_lib):
+    return _oracle_run(oracle_lib, "rising_drop", 48, 50)
+
+
+@pytest.mark.parametrize("path", ["fp64", "refine", "mg"])
+def test_rising_drop_48_50_frames(hip_lib, drop48, path):
+    recs, fc = drop48
+    dts = [r["dt"] for r in recs]
+    grecs, fg, own = _gpu_replay(hip_lib, "rising_drop", 48, dts, PREC[path])
+    _check_dt(f"rising drop 48 {path}", own, dts)
+    for a in grecs:
+        assert a["residual_rel"] <= 1e-8, a
+    print(path, "iterations per frame", [a["pcg_iterations"] for a in grecs][:10], "...")
+    _check(f"rising drop 48 {path}", _errors(fg, fc), TOL_DROP[path])
+
+
+@pytest.mark.parametrize("path", ["fp64", "mg"])
+def test_bubble_pool_64_10_frames(hip_lib, oracle_lib, path):
+    recs, fc = _oracle_run(oracle_lib, "bubbles", 64, 10)
+    dts = [r["dt"] for r in recs]
+    grecs, fg, own = _gpu_replay(hip_lib, "bubbles", 64, dts, PREC[path])
+    _check_dt(f"bubbles 64 {path}", own, dts)
+    print(path, "iterations gpu", [a["pcg_iterations"] for a in grecs], "checker", [r["pcg_iterations"] for r in recs],
+          "sweeps", [a["refinement_sweeps"] for a in grecs])
+    for a in grecs:
+        assert a["residual_rel"] <= 1e-8, a
+    _check(f"bubbles 64 {path}", _errors(fg, fc), TOL_BUBBLES)
+
+
+def _div_max(n, f):
+    d = (f["u"][:, :, 1:] - f["u"][:, :, :-1]) * n
+    d += (f["v"][:, 1:, :] - f["v"][:, :-1, :]) * n
+    d += (f["w"][1:] - f["w"][:-1]) * n
+    return float(np.abs(d).max())
+
+
+@pytest.mark.parametrize("path", ["fp64", "refine", "mg"])
+def test_bubble_pool_projection_divergence(hip_lib, path):
+    """One projection of a smooth divergent velocity in the fixed schedule
+    (every face corrected with the matrix's own coefficients): the residual
+    reported is the true one and meets 1e-8, and the divergence left is the
+    residual's (max |div| after / before below 1e-6)."""
+    n = 64
+    phi, _, P, g = scenes.make("bubbles", n, **dict(REL, precision=PREC[path], schedule=_abi.FS_SCHEDULE_FIXED))
+    c = (np.arange(n) + 0.5) / n
+    fc = np.arange(n + 1) / n
+    u = 0.1 * np.sin(np.pi * fc)[None, None, :] * np.cos(np.pi * c)[None, :, None] * np.ones((n, 1, 1))
+    v = 0.1 * np.sin(2 * np.pi * fc)[None, :, None] * np.cos(np.pi * c)[:, None, None] * np.ones((1, 1, n))
+    w = 0.1 * np.sin(3 * np.pi * fc)[:, None, None] * np.cos(np.pi * c)[None, None, :] * np.ones((1, n, 1))
+    before = _div_max(n, {"u": u, "v": v, "w": w})
+    s = _abi.Solver(hip_lib, "fs_", P, g, 0)
+    s.set_fields(phi, u, v, w)
+    st = s.projection(1e-3)
+    after = _div_max(n, s.get_fields())
+    s.close()
+    print(path, "iterations", st["pcg_iterations"], "sweeps", st["refinement_sweeps"], "residual", st["residual_rel"],
+          "div before", before, "after", after)
+    assert st["residual_rel"] <= 1e-8
+    assert after <= 1e-6 * before
+
+
+@pytest.mark.parametrize("path", ["fp64", "refine", "mg"])
+def test_returned_residual_meets_rel_tolerance(hip_lib, path):
+    """Every path's reported relative residual is the true FP64 one and a
+    projection that returns FS_OK meets rel_tolerance (the dam break, ratio
+    1000, where the FP64 recursion drifts furthest from the true residual)."""
+    for tol in (1e-6, 1e-8, 1e-10):
+        phi, _, P, g = scenes.make("dam_break", 40, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=tol,
+                                   max_iteration=40000, precision=PREC[path])
+        s = _abi.Solver(hip_lib, "fs_", P, g, 0)
+        from helpers import smooth_velocity
+        s.set_fields(phi, *smooth_velocity(40))
+        st = s.projection(1e-3)
+        s.close()
+        print(path, tol, st["pcg_iterations"], st["refinement_sweeps"], st["residual_rel"])
+        assert 0 < st["residual_rel"] <= tol, (path, tol, st)
+
+
+def test_batch_48_scenes_against_checker(hip_lib, oracle_lib):
+    sc = scenes.batch_scenes(48, 3, edges=(32, 48, 64), precision=_abi.FS_FP32_MG, **REL)
+    batch = _abi.Batch(hip_lib, [s[3] for s in sc], [s[4] for s in sc], 0, 0)
+    for (_, phi, vel, _, _), bs in zip(sc, batch.scenes):
+        bs.set_fields(phi, *vel)
+    cpu = []
+    for name, phi, vel, P, g in sc:
+        Pc = type(P).from_buffer_copy(P)
+        Pc.precision = _abi.FS_FP64
+        s = _abi.Solver(oracle_lib, "fso_", Pc, g)
+        oracle_lib.fso_set_viscosity_inplace(s.h, 0)
+        s.set_fields(phi, *vel)
+        cpu.append(s)
+    worst = {c: [0.0, 0.0] for c in ("phi", "u", "v", "w", "p")}
+    for f in range(2):
+        dts = [s.cfl_dt(0.5, 1) for s in cpu]
+        own = [bs.cfl_dt(0.5, 1) for bs in batch.scenes]
+        _check_dt(f"batch frame {f}", own, dts)
+        stats = batch.advance_frame(dts)
+        for s, dt in zip(cpu, dts):
+            s.advance_frame(dt)
+        for st in stats:
+            assert st["residual_rel"] <= 1e-8, st
+    for i, (s, bs) in enumerate(zip(cpu, batch.scenes)):
+        e = _errors(bs.get_fields(), s.get_fields())
+        for c in e:
+            worst[c] = [max(worst[c][0], e[c][0]), max(worst[c][1], e[c][1])]
+        s.close()
+    batch.close()
+    _check("batch of 48 (worst scene per field)", {c: tuple(v) for c, v in worst.items()}, TOL_BATCH)
