@@ -509,7 +509,17 @@ def run_path(lib, scene, n, path, warmup, steps, sync):
                           f"step {r['step_ms']:.1f} ms", file=sys.stderr, flush=True)
             return cb
 
-        warm = frame_loop.advance_frames(s, warmup, on_frame=progress("warmup"))
+        snaps = {}
+        cb_w = progress("warmup")
+
+        def warm_cb(r):
+            cb_w(r)
+            # fields after the first frame and at the end of the warmup, for
+            # the paths' agreement over time (untimed)
+            if r["frame"] + 1 in (1, warmup):
+                snaps[r["frame"] + 1] = s.get_fields()
+
+        warm = frame_loop.advance_frames(s, warmup, on_frame=warm_cb)
         sync()
         _abi.set_kernel_sampling(s, True)
         t0 = time.perf_counter()
@@ -524,7 +534,8 @@ def run_path(lib, scene, n, path, warmup, steps, sync):
             out["pcg"] = _abi.kernel_samples(s, prec)
             if path == "refine":
                 out["pcg_repeat"] = _abi.time_pcg_kernels(s, prec, 20)
-        out["fields"] = s.get_fields()
+        snaps[warmup + steps] = s.get_fields()
+        out["snapshots"] = snaps
         return out
     finally:
         s.close()
@@ -551,8 +562,10 @@ def summary(r):
 
 def field_error(fa, fb):
     """Per-field relative L2 and Linf of fa against fb (pressure with its
-    mean removed: the additive constant is held only by the weak pin)."""
-    err = {}
+    mean removed: the additive constant is held only by the weak pin), and
+    the fraction of cells whose phase (phi <= 0, the cell classification)
+    agrees."""
+    err = {"phase_agreement": float(np.mean((fa["phi"] <= 0) == (fb["phi"] <= 0)))}
     for c in ("phi", "u", "v", "w", "p"):
         a, b = fa[c], fb[c]
         if c == "p":
@@ -731,10 +744,12 @@ def main():
     if "fp64" in runs:
         for p in runs:
             if p != "fp64":
-                res[p]["error_vs_fp64"] = field_error(runs[p]["fields"], runs["fp64"]["fields"])
+                res[p]["error_vs_fp64"] = {
+                    f"after_frame_{f}": field_error(runs[p]["snapshots"][f], runs["fp64"]["snapshots"][f])
+                    for f in sorted(runs[p]["snapshots"]) if f in runs["fp64"]["snapshots"]}
                 res[p]["speedup_vs_fp64"] = round(runs["fp64"]["ms_per_step"] / runs[p]["ms_per_step"], 3)
     for r in runs.values():
-        r.pop("fields", None)
+        r.pop("snapshots", None)
 
     cpu = None
     if not a.no_cpu_baseline:

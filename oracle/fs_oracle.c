@@ -80,6 +80,8 @@ struct fso_state {
     /* the scene's spherical level-set sources and the frame counter */
     fs_levelset_source src[64];
     int nsrc, frame;
+    /* test hook (fso_solve_poisson): a jump field that replaces sigma*kappa */
+    const double* jc_override;
 };
 
 /* A CSR system with rows numbered plane by plane (per rows in each of
@@ -514,6 +516,7 @@ int fso_get_curvature(fso_state* s, double* kappa, double* abs_max_curvature) {
  * its initial zeros (no surface tension, or use_delta_function_formulation
  * off: PROJECTION_3D.h:750). */
 static const double* jump_field(const fso_state* s) {
+    if (s->jc_override) return s->jc_override;
     return s->P.surface_tension != 0.0 && s->P.use_delta_function_formulation ? s->jc : NULL;
 }
 
@@ -1470,6 +1473,30 @@ static int godunov_axis(double sg, double m, double p, double* val) {
     return AX_NONE;
 }
 
+/* |grad phi| - 1 of the Godunov case tree (ADVECTION_METHOD_3D.h:908-1622)
+ * from the one-sided derivatives of the three axes; 0 when a case is not
+ * met (only NaN operands).  The compatible form keeps the two leaves whose
+ * norm the reference writes as sqrt(px^2) + py^2 (:1463, :1605). */
+static int godunov_norm(double sg, double mx, double px, double my, double py, double mz, double pz, int exact,
+                        double* g) {
+    double X, Y, Z;
+    int cx = godunov_axis(sg, mx, px, &X);
+    int cy = cx ? godunov_axis(sg, my, py, &Y) : AX_NONE;
+    int cz = cy ? godunov_axis(sg, mz, pz, &Z) : AX_NONE;
+    if (!(cx && cy && cz)) return 0;
+    if (!exact && (cx == AX_SM || cx == AX_SP) && cy == AX_SP && cz == AX_DROP)
+        *g = sqrt(X * X) + Y * Y - 1; /* :1463, :1605 */
+    else
+        *g = sqrt(X * X + Y * Y + Z * Z) - 1;
+    return 1;
+}
+
+int fso_godunov_axis(double sg, double m, double p, double* val) { return godunov_axis(sg, m, p, val); }
+int fso_godunov_norm(double sg, double mx, double px, double my, double py, double mz, double pz, int exact,
+                     double* g) {
+    return godunov_norm(sg, mx, px, my, py, mz, pz, exact, g);
+}
+
 /* ADVECTION_METHOD_3D::WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637
  * (the fixed schedule takes the Euclidean norm in every Godunov case). */
 static void reinit_sweep(fso_state* s, double dt) {
@@ -1480,7 +1507,7 @@ static void reinit_sweep(fso_state* s, double dt) {
     for (int k = 0; k < s->nz; k++)
         for (int j = 0; j < s->ny; j++)
             for (int i = 0; i < s->nx; i++) {
-                double f[7], mx, px, my, py, mz, pz, X, Y, Z;
+                double f[7], mx, px, my, py, mz, pz;
                 gather7(s, s->sghost, i, j, k, 0, f);
                 fso_weno5_1d(f, gc->x.inv, eps, &mx, &px);
                 gather7(s, s->sghost, i, j, k, 1, f);
@@ -1489,17 +1516,8 @@ static void reinit_sweep(fso_state* s, double dt) {
                 fso_weno5_1d(f, gc->z.inv, eps, &mz, &pz);
                 double sg = A3(s->sgn, i, j, k);
                 double* rho = &A3(s->phi, i, j, k);
-                int cx = godunov_axis(sg, mx, px, &X);
-                int cy = cx ? godunov_axis(sg, my, py, &Y) : AX_NONE;
-                int cz = cy ? godunov_axis(sg, mz, pz, &Z) : AX_NONE;
-                if (cx && cy && cz) {
-                    double g;
-                    if (!exact && (cx == AX_SM || cx == AX_SP) && cy == AX_SP && cz == AX_DROP)
-                        g = sqrt(X * X) + Y * Y - 1; /* :1463, :1605 */
-                    else
-                        g = sqrt(X * X + Y * Y + Z * Z) - 1;
-                    *rho = *rho - dt * sg * g;
-                }
+                double g;
+                if (godunov_norm(sg, mx, px, my, py, mz, pz, exact, &g)) *rho = *rho - dt * sg * g;
                 if (sg * *rho < (double)0) { /* :1624-1634 */
                     if (sg < 0) *rho = -eps;
                     if (sg > 0) *rho = eps;
@@ -1685,6 +1703,66 @@ int fso_cfl_dt(fso_state* s, double cfl, int gravity_norm, double* dt) {
     if (P->semi_implicit_viscosity) v_f = 0; /* :2770-2773 */
     double c = (double)0.5 * (c_f + v_f + sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) + (double)4 * (s_f * s_f)));
     *dt = cfl * ((double)1 / c);
+    return FS_OK;
+}
+
+/* ------------------------------------------------------------------------- */
+/* Test hooks                                                                 */
+/* ------------------------------------------------------------------------- */
+
+static void put(fso_state* s, double* dst, const double* src, int ni, int nj, int nk);
+
+/* The projection's linear system for a given source and jump field at the
+ * current level set (DeterminePressure, PROJECTION_3D.h:794-834, with the
+ * divergence field replaced by `source` -- div u / dt, so that the solve is
+ * div(beta grad p) = source -- and jc_on_solution by `jump`, the jump
+ * p(phi > 0 side) - p(phi <= 0 side); NULL = no jump), solved from p = 0;
+ * p receives the solution (interior, x fastest).  This is the reference's
+ * own Poisson test hook (PROJECTION_3D.h:564-593, 727-740, 771-774) with the
+ * flux jump [beta dp/dn] held at zero, as this solver's assembly has it. */
+int fso_solve_poisson(fso_state* s, const double* source, const double* jump, double* p, fs_step_stats* st) {
+    if (!source || !p) return FS_ERR_BAD_ARG;
+    double* jz = NULL;
+    if (jump) {
+        jz = zalloc(s->ntot); /* zero ghosts, as jc_on_solution's */
+        put(s, jz, jump, s->nx, s->ny, s->nz);
+    }
+    projection_density(s);
+    for (long n = 0; n < s->ntot; n++) s->div[n] = 0;
+    put(s, s->div, source, s->nx, s->ny, s->nz);
+    for (long n = 0; n < s->ntot; n++) s->p[n] = 0;
+    s->jc_override = jz;
+    assemble(s);
+    pcg(s);
+    s->jc_override = NULL;
+    free(jz);
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) A3(s->p, i, j, k) = s->x[s->bc[IX(s, i, j, k)]];
+    get(s, s->p, p, s->nx, s->ny, s->nz);
+    if (st) {
+        memset(st, 0, sizeof(*st));
+        st->pcg_iterations = s->last_iters;
+        st->residual_precond = s->last_res_abs;
+        st->residual_rel = s->last_res_rel;
+    }
+    return s->last_iters >= s->P.max_iteration ? FS_ERR_NOT_CONVERGED : FS_OK;
+}
+
+/* The persistent ghost copies, each over its box grown by the ghost width
+ * (x fastest): the shared scalar copy (sghost, (nx+6)(ny+6)(nz+6)) and the
+ * MAC copies gu, gv, gw ((nx+7)(ny+6)(nz+6), ...).  NULL skips one. */
+static void get_ghosted(fso_state* s, const double* src, double* dst, int ni, int nj, int nk) {
+    const int gi = ni + 2 * G, gj = nj + 2 * G, gk = nk + 2 * G;
+    for (int k = 0; k < gk; k++)
+        for (int j = 0; j < gj; j++)
+            for (int i = 0; i < gi; i++) dst[i + (long)gi * (j + (long)gj * k)] = A3(src, i - G, j - G, k - G);
+}
+int fso_get_ghost_copies(fso_state* s, double* sghost, double* gu, double* gv, double* gw) {
+    if (sghost) get_ghosted(s, s->sghost, sghost, s->nx, s->ny, s->nz);
+    if (gu) get_ghosted(s, s->gu, gu, s->nx + 1, s->ny, s->nz);
+    if (gv) get_ghosted(s, s->gv, gv, s->nx, s->ny + 1, s->nz);
+    if (gw) get_ghosted(s, s->gw, gw, s->nx, s->ny, s->nz + 1);
     return FS_OK;
 }
 

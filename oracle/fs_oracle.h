@@ -75,6 +75,23 @@ void fso_weno5_1d(const double* v, double one_over_dx, double epsilon,
 void fso_eno3_1d(const double* v, double dx, double one_over_dx, double one_over_2dx,
                  double one_over_3dx, double* d_minus, double* d_plus);
 
+/* Test hooks for closed-form pins. */
+/* One axis of the Godunov case tree (ADVECTION_METHOD_3D.h:908-1622): the
+ * selected one-sided derivative in *val; returns the case (1 upwind m, 2
+ * upwind p, 3 expansion, 4/5 shock taking m/p, 0 none). */
+int  fso_godunov_axis(double sg, double m, double p, double* val);
+/* |grad phi| - 1 from the three axes' one-sided derivatives (exact = 1: the
+ * Euclidean norm in every case; 0: the reference's two leaves as written);
+ * returns 0 when no case is met. */
+int  fso_godunov_norm(double sg, double mx, double px, double my, double py, double mz, double pz,
+                      int exact, double* g);
+/* The projection's system div(beta grad p) = source with the jump
+ * p(phi > 0) - p(phi <= 0) = jump (NULL: none) at the current level set,
+ * solved from zero; p receives the solution. */
+int  fso_solve_poisson(fso_state* s, const double* source, const double* jump, double* p, fs_step_stats* st);
+/* The persistent ghost copies over their boxes grown by 3 (NULL skips). */
+int  fso_get_ghost_copies(fso_state* s, double* sghost, double* gu, double* gv, double* gw);
+
 #ifdef __cplusplus
 }
 #endif

@@ -128,9 +128,10 @@ def _frames(hip_lib, oracle_lib, name, n, frames, velocity_scale=False, **over):
 
 @pytest.mark.parametrize("name,n,frames", [("rising_drop", 24, 3), ("dam_break", 24, 3)])
 def test_frames_fp64_tight(hip_lib, oracle_lib, name, n, frames):
-    """Solve to 1e-11 relative residual: the frames agree to 1e-8."""
+    """Solve to 4e-11 relative residual (just above the attainable FP64
+    residual of the density ratio 1000 scenes): the frames agree to 1e-8."""
     its, errs = _frames(hip_lib, oracle_lib, name, n, frames, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
-                        rel_tolerance=1e-11)
+                        rel_tolerance=4e-11)
     for c, e in errs.items():
         assert e < 1e-8, (c, e)
 
@@ -159,9 +160,9 @@ def test_semi_implicit_viscosity_phase(hip_lib, oracle_lib, name):
 def test_frames_semi_implicit_rk3(hip_lib, oracle_lib, name):
     """TVD-RK3 frames with semi-implicit viscosity (three viscosity solves
     and three projections per step) agree to 1e-8 after 2 frames when every
-    solve reaches 1e-11 relative residual."""
+    solve reaches 4e-11 relative residual."""
     its, errs = _frames(hip_lib, oracle_lib, name, 20, 2, order_for_time_advancing=3, semi_implicit_viscosity=1,
-                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11)
+                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11)
     for c, e in errs.items():
         assert e < 1e-8, (c, e)
 
@@ -170,9 +171,9 @@ def test_frames_semi_implicit_rk3(hip_lib, oracle_lib, name):
 def test_frames_runge_kutta_orders(hip_lib, oracle_lib, name, order):
     """Orders 2 and 3 of the step schedule (two / three projections per
     step, stage blends, RK reinitialization) agree to 1e-8 after 2 frames
-    when each solve reaches 1e-11 relative residual."""
+    when each solve reaches 4e-11 relative residual."""
     its, errs = _frames(hip_lib, oracle_lib, name, 20, 2, order_for_time_advancing=order,
-                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11)
+                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11)
     for c, e in errs.items():
         assert e < 1e-8, (c, e)
 
@@ -188,10 +189,10 @@ def test_frames_fixed_schedule(hip_lib, oracle_lib, name, order, over):
     every face corrected with the matrix's coefficients, wall faces zeroed
     after forces, each WENO5 velocity component on its own ghost copy, the
     Euclidean Godunov norm) agrees with the CPU checker's to 1e-8 after 2
-    frames when each solve reaches 1e-11 relative residual (velocity
+    frames when each solve reaches 4e-11 relative residual (velocity
     components measured against the whole velocity's norm)."""
     its, errs = _frames(hip_lib, oracle_lib, name, 20, 2, velocity_scale=True, order_for_time_advancing=order,
-                        schedule=_abi.FS_SCHEDULE_FIXED, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11,
+                        schedule=_abi.FS_SCHEDULE_FIXED, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11,
                         **over)
     for c, e in errs.items():
         assert e < 1e-8, (c, e)
@@ -342,9 +343,9 @@ def test_multigrid_odd_and_unequal_grids(hip_lib, oracle_lib):
 
 def test_multigrid_frames(hip_lib, oracle_lib):
     """Two frames of the dam break with the multigrid refinement agree with
-    the CPU checker's FP64 frames to 1e-8 when both solve to 1e-11."""
+    the CPU checker's FP64 frames to 1e-8 when both solve to 4e-11."""
     its, errs = _frames(hip_lib, oracle_lib, "dam_break", 24, 2, velocity_scale=True, precision=_abi.FS_FP32_MG,
-                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11)
+                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11)
     for c, e in errs.items():
         assert e < 1e-8, (c, e)
 

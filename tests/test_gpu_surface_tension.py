@@ -91,8 +91,8 @@ def test_frames_with_surface_tension(hip_lib, oracle_lib, sched, name):
     """Two CFL frames with sigma = 0.0728: the first CFL step agrees bit for
     bit (velocity maxima and curvature maximum are exact on equal fields),
     the second to the solve tolerance; the fields to 1e-8 when each solve
-    reaches 1e-11 relative residual."""
-    over = dict(surface_tension=SIGMA, schedule=sched, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-11)
+    reaches 4e-11 relative residual."""
+    over = dict(surface_tension=SIGMA, schedule=sched, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11)
     gpu, cpu = make_pair(hip_lib, oracle_lib, name, n=20, **over)
     for f in range(2):
         dg, dc = gpu.cfl_dt(0.5, 1), cpu.cfl_dt(0.5, 1)
