@@ -25,6 +25,7 @@
 #include "fs_layout.h"
 #include "fs_mg.h"
 #include "fs_probe.h"
+#include "fs_cfl.h"
 
 #ifndef FS_FP32_RESTART
 #define FS_FP32_RESTART 4000  // FS_FP32_REFINE: restart a recursion that dropped under 100x in this many iterations
@@ -1566,18 +1567,10 @@ class EulerianFluidSolver3D {
         for (int c = 0; c < 3; c++) memcpy(&m[c], &hvmax[c], sizeof(double));
         for (double x : m)
             if (!std::isfinite(x)) throw FsFail{FS_ERR_NONFINITE, "non-finite velocity"};
-        double c_f = m[0] * macx.x.inv + m[1] * macy.y.inv + m[2] * macz.z.inv;
-        double g_f = 0;
-        if (P.is_external_force_active) {
-            double g = gravity_norm ? std::sqrt(P.gravity[0] * P.gravity[0] + P.gravity[1] * P.gravity[1] +
-                                                P.gravity[2] * P.gravity[2])
-                                    : std::fabs(P.gravity[1]);
-            g_f = std::sqrt(g * macy.y.inv);
-        }
+        FsCflTerms t = fs_cfl_terms(P, G, gravity_norm);
         // surface tension restriction (:2740-2744, air/water) from the
         // curvature of the current level set, as SIMULATION_WORLD::AdvanceOneFrame
         // computes it right before the CFL rule (SIMULATION_WORLD.h:324, :289)
-        double s_f = 0;
         if (tension()) {
             ComputeCurvatures();
             sync();
@@ -1586,14 +1579,9 @@ class EulerianFluidSolver3D {
             memcpy(&kmax, &hkstat[0], sizeof(double));
             double dmin = std::min(std::min(macx.x.d, macy.y.d), macz.z.d);
             double rmin = P.water_density > P.air_density ? P.air_density : P.water_density;
-            s_f = std::sqrt((P.surface_tension * kmax) / (rmin * (dmin * dmin)));
+            t.s_f = std::sqrt((P.surface_tension * kmax) / (rmin * (dmin * dmin)));
         }
-        double a = P.water_viscosity / P.water_density, b = P.air_viscosity / P.air_density;
-        double v_f = (a > b ? a : b) * ((double)2 * cell.x.inv_d2 + (double)2 * cell.y.inv_d2 + (double)2 * cell.z.inv_d2);
-        if (P.semi_implicit_viscosity) v_f = 0;  // :2770-2773
-        double cfl = (double)0.5 * (c_f + v_f + std::sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) +
-                                                          (double)4 * (s_f * s_f)));
-        return (double)1 / cfl;
+        return (double)1 / fs_cfl_rate(t, m[0], m[1], m[2]);
     }
 };
 

@@ -104,10 +104,13 @@ def _check(tag, errs, tol):
 
 def _check_snaps(tag, gs, cs, tol_by_frame):
     """Fields after each snapshot frame against the checker's, with the
-    tolerance of the first row whose frame bound covers it."""
+    tolerance of the first row whose frame bound covers it; every cell's
+    phase (the classification's sign test) must agree."""
     for f in sorted(cs):
         errs = _errors(gs[f], cs[f])
-        print(f"{tag} frame {f}: phase agreement {_sign_agreement(gs[f], cs[f]):.8f}")
+        agree = _sign_agreement(gs[f], cs[f])
+        print(f"{tag} frame {f}: phase agreement {agree:.8f}")
+        assert agree == 1.0, (tag, f, agree)
         tol = next(t for bound, t in tol_by_frame if f <= bound)
         _check(f"{tag} frame {f}", errs, tol)
 
@@ -119,13 +122,30 @@ def _check_dt(tag, own, dts, frames=None):
     assert max(rel[:upto]) <= 1e-6, (tag, rel)
 
 
-def _tol(l2, linf):
-    return {c: (l2, linf) for c in ("phi", "u", "v", "w", "p")}
+def _tol(phi, vel, p):
+    """(relative L2, relative Linf) bounds for phi, the three velocity
+    components and the pressure."""
+    return {"phi": phi, "u": vel, "v": vel, "w": vel, "p": p}
 
 
-# (relative L2, relative Linf) per field, by the frame count a row covers
-TOL_FP64 = [(50, _tol(1e-5, 1e-4))]
-TOL_FP32 = [(5, _tol(1e-5, 1e-4)), (50, _tol(1.0, 10.0))]
+# Per-field tolerances by the frame count a row covers, about 10x the
+# differences measured on an MI355X (the GPU's reductions are fixed-order,
+# so a rerun gives the same fields).  In these scenes a difference between
+# two runs grows about 10x every 4 frames, whatever its origin: the FP64
+# path against the checker starts from the order of the dot products
+# (1e-16) and the FP32 paths from their inner solves' 1e-8, and both grow at
+# that rate (the far-field level set and the uncorrected wall faces carry
+# it), while the sign of phi -- the cell classification -- agrees in every
+# cell at every frame.
+TOL_FP64 = [(10, _tol((1e-10, 1e-9), (1e-9, 1e-8), (1e-8, 2e-8))),
+            (20, _tol((1e-10, 1e-9), (1e-9, 1e-7), (1e-8, 2e-8))),
+            (50, _tol((1e-7, 3e-6), (3e-7, 4e-6), (1e-6, 5e-6)))]
+# two frames of 48 scenes, the worst scene per field
+TOL_BATCH = _tol((1e-12, 1e-12), (1e-9, 2e-8), (4e-8, 3e-7))
+TOL_FP32 = [(5, _tol((1e-11, 3e-11), (3e-9, 1e-7), (6e-8, 9e-8))),
+            (10, _tol((1e-10, 5e-10), (1e-8, 4e-7), (1e-7, 7e-7))),
+            (20, _tol((3e-7, 3e-5), (5e-7, 4e-5), (5e-6, 3e-5))),
+            (50, _tol((5e-2, 2e-1), (3e-1, 3.0), (1.0, 3.0)))]
 
 
 def test_dam_break_64_20_frames_fp64(hip_lib, oracle_lib):
@@ -247,4 +267,4 @@ def test_batch_48_scenes_against_checker(hip_lib, oracle_lib):
             worst[c] = [max(worst[c][0], e[c][0]), max(worst[c][1], e[c][1])]
         s.close()
     batch.close()
-    _check("batch of 48 (worst scene per field)", {c: tuple(v) for c, v in worst.items()}, TOL_FP32[0][1])
+    _check("batch of 48 (worst scene per field)", {c: tuple(v) for c, v in worst.items()}, TOL_BATCH)
