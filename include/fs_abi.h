@@ -183,6 +183,21 @@ int  fs_step(fs_solver* s, double dt, fs_step_stats* stats);
  * |g_y| as the reference does (:2717); 1 uses |g|.  Returns in *dt. */
 int  fs_cfl_dt(fs_solver* s, double cfl, int gravity_norm, double* dt);
 
+/* fs_cfl_dt without a handle and without any allocation.  No counterpart in
+ * the reference: for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc).
+ * u, v, w are device pointers to interior MAC fields in the layout above;
+ * dt is a device pointer that receives cfl / rate (NaN when a velocity is
+ * not finite), the value fs_cfl_dt gives for the same fields.  The work is
+ * enqueued on `stream` (a hipStream_t; NULL is the null stream) and the call
+ * returns without waiting.  Partial maxima live in a fixed device array of
+ * 3 x 2048 doubles per GPU, so each velocity component may have at most
+ * 2048 * 2^17 faces (FS_ERR_OOM beyond); surface tension is not supported
+ * (its term needs the curvature: FS_ERR_UNSUPPORTED).  Calls that overlap
+ * on one GPU share that array and race. */
+int  fs_cfl_dt_noalloc(const fs_params* params, const fs_grid* grid, const double* u, const double* v,
+                       const double* w, double cfl, int gravity_norm, double* dt, void* stream);
+
 /* ---- phases (EULERIAN_FLUID_SOLVER_3D.h:1590-1606, 2076-2237) ------------ */
 int  fs_compute_curvatures(fs_solver* s);           /* LEVELSET_3D::ComputeCurvatures, LEVELSET_3D.h:963-1082 */
 int  fs_levelset_advection(fs_solver* s, double dt); /* ADVECTION_3D::Solve_Levelset, ADVECTION_3D.h:204-242 */
