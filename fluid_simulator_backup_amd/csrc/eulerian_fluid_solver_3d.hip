@@ -75,6 +75,16 @@ class EulerianFluidSolver3D {
     hipStream_t st2 = nullptr;
     hipEvent_t ev_ls = nullptr, ev_rn = nullptr;
     double* phi_r = nullptr;
+    // The reinitialization's sweeps past `reinit_switch` run on st3, a
+    // stream on every CU (after an event on st2): once SolveForRK is done,
+    // the CUs st2's mask leaves to it would idle.  The switch point follows
+    // the last concurrent step's timings: SolveForRK's device time over the
+    // masked sweep's (ev_t: reinitialization start and switch on st2,
+    // SolveForRK start and end on st).
+    hipStream_t st3 = nullptr;
+    hipEvent_t ev_sw = nullptr, ev_t[4] = {};
+    int reinit_switch = -1, reinit_count = 0, last_switch = -1;
+    bool timed_concurrent = false, switched = false;
     bool batch_member = false;  // set by fs_batch_create
     hipEvent_t ev[4] = {};
 
@@ -293,7 +303,9 @@ class EulerianFluidSolver3D {
     }
 
     ~EulerianFluidSolver3D() {
-        if (st) (void)hipStreamSynchronize(st);
+        // every stream drained before any buffer is freed
+        for (hipStream_t s_ : {st, st2, st3})
+            if (s_) (void)hipStreamSynchronize(s_);
         double* f64[] = {phi, phi_r, sghost, sghost2, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
                          scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk,
                          va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc};
@@ -323,9 +335,9 @@ class EulerianFluidSolver3D {
         for (auto& e : probe.ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& k : kprobe) k.destroy();
-        if (st2) (void)hipStreamSynchronize(st2);
-        if (ev_ls) (void)hipEventDestroy(ev_ls);
-        if (ev_rn) (void)hipEventDestroy(ev_rn);
+        for (hipEvent_t e : {ev_ls, ev_rn, ev_sw, ev_t[0], ev_t[1], ev_t[2], ev_t[3]})
+            if (e) (void)hipEventDestroy(e);
+        if (st3) (void)hipStreamDestroy(st3);
         if (st2) (void)hipStreamDestroy(st2);
         if (st) (void)hipStreamDestroy(st);
     }
@@ -879,6 +891,7 @@ class EulerianFluidSolver3D {
             // time the sweeps with the GPU to themselves: let a concurrent
             // reinitialization finish first (once per handle)
             if (st2) HIPCHECK(hipStreamSynchronize(st2));
+            if (st3) HIPCHECK(hipStreamSynchronize(st3));
             mg.Tune(st, (float*)w32.r, (float*)w32.s);
         }
         init_state(S64, FS_STOP_REL_RESIDUAL, 0, P.rel_tolerance, 0);
@@ -1365,6 +1378,7 @@ class EulerianFluidSolver3D {
         // A recorded step keeps phi current after every sweep for the tape.
         double *src = sghost, *nxt = sghost2;
         for (int it = 0; it < n; it++) {
+            if (reinit_switch >= 0 && reinit_count++ == reinit_switch) SwitchReinitStream();
             if (recording) tape.reinit.back().push_back(Record(phi, 0));
             if (it == 0 || recording) {
                 if (fixed()) {
@@ -1463,34 +1477,61 @@ class EulerianFluidSolver3D {
         const bool concurrent = ConcurrentReinit();
         if (concurrent) {
             EnsureSecondStream();
+            const int sweeps = P.iteration_number_for_reinitialization * (order == 3 ? 3 : 2);
+            const int at = ReinitSwitchPoint(sweeps);
             HIPCHECK(hipEventRecord(ev_ls, st));
             HIPCHECK(hipStreamWaitEvent(st2, ev_ls, 0));
-            std::swap(st, st2);
+            // enqueued on st2 (then st3) with phi and phi_r exchanged; the
+            // guard gives st and phi back however the stage leaves
+            ReinitScope scope(*this);
+            st = st2;
+            HIPCHECK(hipEventRecord(ev_t[0], st));
             HIPCHECK(hipMemcpyAsync(phi_r, phi, sizeof(double) * (size_t)L.total, hipMemcpyDeviceToDevice, st));
             std::swap(phi, phi_r);
+            scope.swapped = true;
             // the ghost layers SolveForRK will leave on phi: the constant
             // fill of ComputeNormals (assembly) and of the fixed schedule's
             // viscosity, a function of the box, which SolveForRK leaves as is
             if (P.is_projection_active || (fixed() && P.is_sourcing_active && P.use_delta_function_formulation))
                 fs_fill_const(st, phi, phi, L, box(0), 2);
+            reinit_switch = at < sweeps ? at : -1;
+            reinit_count = 0;
+            switched = false;
             ReinitializationStage(order);
-            std::swap(phi, phi_r);
+            reinit_switch = -1;
+            if (!switched) HIPCHECK(hipEventRecord(ev_t[1], st));  // every sweep ran masked
+            last_switch = switched ? at : sweeps;
             HIPCHECK(hipEventRecord(ev_rn, st));
-            std::swap(st, st2);
+            scope.release();
         }
-        if (order == 1) {
-            SolveForRK(dt, rc);  // :834
-        } else {
-            SaveVelocity();  // :913-932, :1015-1034
-            SolveForRK(dt, rc);
-            SolveForRK(dt, rc);
-            if (order == 2) {
-                BlendVelocity((double)0.5, (double)0.5);  // :937-953
+        try {
+            if (concurrent) HIPCHECK(hipEventRecord(ev_t[2], st));
+            if (order == 1) {
+                SolveForRK(dt, rc);  // :834
             } else {
-                BlendVelocity((double)0.75, (double)0.25);  // :1039-1055
+                SaveVelocity();  // :913-932, :1015-1034
                 SolveForRK(dt, rc);
-                BlendVelocity((double)1 / (double)3, (double)2 / (double)3);  // :1059-1075
+                SolveForRK(dt, rc);
+                if (order == 2) {
+                    BlendVelocity((double)0.5, (double)0.5);  // :937-953
+                } else {
+                    BlendVelocity((double)0.75, (double)0.25);  // :1039-1055
+                    SolveForRK(dt, rc);
+                    BlendVelocity((double)1 / (double)3, (double)2 / (double)3);  // :1059-1075
+                }
             }
+            if (concurrent) {
+                HIPCHECK(hipEventRecord(ev_t[3], st));
+                timed_concurrent = true;
+            }
+        } catch (...) {
+            // the reinitialization still writes phi_r: wait for it and take
+            // its result, so later calls see one consistent state
+            if (concurrent) {
+                (void)hipStreamWaitEvent(st, ev_rn, 0);
+                std::swap(phi, phi_r);
+            }
+            throw;
         }
         if (concurrent) {
             HIPCHECK(hipStreamWaitEvent(st, ev_rn, 0));
@@ -1525,9 +1566,71 @@ class EulerianFluidSolver3D {
         return !serial && !batch_member && !recording && P.sussmanstyle_reinitialization && !tension() &&
                sources.empty();
     }
+    // Restores the main stream and phi after the concurrent stage's
+    // enqueue, also when an enqueue throws (phi then waits for what st2
+    // already holds).
+    struct ReinitScope {
+        EulerianFluidSolver3D& S;
+        hipStream_t main;
+        bool swapped = false, active = true;
+        explicit ReinitScope(EulerianFluidSolver3D& s) : S(s), main(s.st) {}
+        void release() {
+            if (swapped) std::swap(S.phi, S.phi_r);
+            swapped = false;
+            S.st = main;
+            active = false;
+        }
+        ~ReinitScope() {
+            if (!active) return;
+            hipStream_t cur = S.st;
+            S.st = main;
+            S.reinit_switch = -1;
+            if (swapped) {
+                // what was enqueued on phi (now phi_r's buffer) finishes
+                // before main touches it
+                if (hipEventRecord(S.ev_rn, cur) == hipSuccess) (void)hipStreamWaitEvent(main, S.ev_rn, 0);
+                std::swap(S.phi, S.phi_r);
+            }
+        }
+    };
+
+    // continue the reinitialization on the all-CU stream st3
+    void SwitchReinitStream() {
+        switched = true;
+        HIPCHECK(hipEventRecord(ev_sw, st));
+        HIPCHECK(hipEventRecord(ev_t[1], st));
+        HIPCHECK(hipStreamWaitEvent(st3, ev_sw, 0));
+        st = st3;
+    }
+
+    // Sweeps of this step's reinitialization to run on the masked stream:
+    // as many as the last concurrent step's SolveForRK lasted in masked
+    // sweeps (half of them at the first step); FS_REINIT_MASKED_SWEEPS
+    // fixes the count (negative: all of them, the single-stream order).
+    int ReinitSwitchPoint(int sweeps) {
+        if (const char* e = std::getenv("FS_REINIT_MASKED_SWEEPS")) {
+            int k = std::atoi(e);
+            return k < 0 ? sweeps : std::min(k, sweeps);
+        }
+        int at = sweeps / 2;
+        if (timed_concurrent && last_switch > 0) {
+            float masked = 0, solve = 0;
+            if (hipEventElapsedTime(&masked, ev_t[0], ev_t[1]) == hipSuccess &&
+                hipEventElapsedTime(&solve, ev_t[2], ev_t[3]) == hipSuccess && masked > 0) {
+                const double per = (double)masked / last_switch;
+                at = (int)std::lround((double)solve / per);
+            }
+        }
+        return std::max(0, std::min(at, sweeps));
+    }
+
     // the second stream, its events and the level-set copy, on first use
     void EnsureSecondStream() {
         if (st2) return;
+        if (!ev_sw) HIPCHECK(hipEventCreateWithFlags(&ev_sw, hipEventDisableTiming));
+        for (auto& e : ev_t)
+            if (!e) HIPCHECK(hipEventCreate(&e));
+        if (!st3) HIPCHECK(hipStreamCreate(&st3));
         // events and the copy first: the stream, made last, marks completion
         if (!ev_ls) HIPCHECK(hipEventCreateWithFlags(&ev_ls, hipEventDisableTiming));
         if (!ev_rn) HIPCHECK(hipEventCreateWithFlags(&ev_rn, hipEventDisableTiming));

@@ -382,15 +382,23 @@ def test_multigrid_fused_sweeps_match_half_sweeps(hip_lib, monkeypatch):
         assert np.array_equal(out[0][0][c], out[1][0][c]), c
 
 
-@pytest.mark.parametrize("name,order", [("rising_drop", 1), ("dam_break", 2)])
-def test_concurrent_reinitialization_matches_sequential_order(hip_lib, monkeypatch, name, order):
+@pytest.mark.parametrize("name,order,masked", [("rising_drop", 1, None), ("dam_break", 2, None), ("rising_drop", 1, "7"),
+                                               ("dam_break", 3, "0"), ("rising_drop", 2, "-1")])
+def test_concurrent_reinitialization_matches_sequential_order(hip_lib, monkeypatch, name, order, masked):
     """The step's reinitialization on the second stream (on a copy of the
-    advected level set, with the assembly's ghost fill replayed) gives the
-    frames of the sequential order bit for bit (FS_SERIAL_REINIT)."""
+    advected level set, with the assembly's ghost fill replayed), its later
+    sweeps on the all-CU stream from the switch point on (adaptive, or fixed
+    by FS_REINIT_MASKED_SWEEPS: 7 within the first call, 0 at once, -1
+    never), gives the frames of the sequential order bit for bit
+    (FS_SERIAL_REINIT)."""
     from fluid_simulator_backup_amd import scenes
     phi, vel, P, g = scenes.make(name, 24, precision=_abi.FS_FP32_MG, stop_rule=_abi.FS_STOP_REL_RESIDUAL,
                                  rel_tolerance=1e-8, order_for_time_advancing=order)
     out = []
+    if masked is None:
+        monkeypatch.delenv("FS_REINIT_MASKED_SWEEPS", raising=False)
+    else:
+        monkeypatch.setenv("FS_REINIT_MASKED_SWEEPS", masked)
     for serial in (True, False):
         if serial:
             monkeypatch.setenv("FS_SERIAL_REINIT", "1")
