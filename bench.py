@@ -10,9 +10,12 @@ variable-density projection solved to 1e-8 relative residual, and 2 x 25
 reinitialization sweeps.  The default run is the workload's 50 frames from
 rest (5 untimed, 45 timed).  The headline value is ms/step with the FP32
 inner solve inside FP64 iterative refinement, the inner solve being CG
-preconditioned by a geometric multigrid V-cycle; FP32 Jacobi-PCG inside the
-same refinement and the plain FP64 Jacobi-PCG path run the same frames from
-the same state for comparison (`--paths`), each on its own, so that one
+preconditioned by a geometric multigrid V-cycle, and the WENO5 derivatives
+of the advection and reinitialization in the fewer-operation form
+(fast_weno, FP64; within rounding of the reference's order); the same path
+in the reference's operation order, FP32 Jacobi-PCG inside the same
+refinement and the plain FP64 Jacobi-PCG path run the same frames from the
+same state for comparison (`--paths`), each on its own, so that one
 path's failure (named by frame, dt and residual) leaves the others'
 figures.  Inputs are resident on the GPU before the timed region; every step
 is synchronous.
@@ -53,10 +56,11 @@ def parse():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-c0", action="store_true",
                     help="skip the end-to-end CPU/GPU comparison on the 64^3 dam break (20 frames)")
-    ap.add_argument("--paths", default="mg,refine,fp64",
+    ap.add_argument("--paths", default="mg-fast,mg,refine,fp64",
                     help="projection paths run on the same frames; the first is the headline: mg (FP32 CG with a "
                          "multigrid preconditioner inside FP64 refinement), refine (FP32 Jacobi-PCG inside FP64 "
-                         "refinement), fp64 (FP64 Jacobi-PCG)")
+                         "refinement), fp64 (FP64 Jacobi-PCG); a -fast suffix takes the WENO5 derivatives in "
+                         "fewer operations (fast_weno)")
     ap.add_argument("--no-fp64", action="store_true", help="drop fp64 from --paths")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
@@ -482,6 +486,11 @@ def projection_main(a):
 
 PATHS = {"mg": "FS_FP32_MG", "refine": "FS_FP32_REFINE", "fp64": "FS_FP64"}
 LABEL = {"mg": "fp32+multigrid+refine", "refine": "fp32+jacobi+refine", "fp64": "fp64"}
+# "<path>-fast": the same projection path with the WENO5 derivatives in fewer
+# operations (fs_params.fast_weno: product-form weights, fused multiply-adds)
+for _p in list(PATHS):
+    PATHS[_p + "-fast"] = PATHS[_p]
+    LABEL[_p + "-fast"] = LABEL[_p] + ", fast WENO5"
 
 
 def run_path(lib, scene, n, path, warmup, steps, sync):
@@ -494,7 +503,7 @@ def run_path(lib, scene, n, path, warmup, steps, sync):
     from fluid_simulator_backup_amd import _abi, frame_loop, scenes
     prec = getattr(_abi, PATHS[path])
     phi, vel, P, g = scenes.make(scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
-                                 precision=prec, max_iteration=20000)
+                                 precision=prec, max_iteration=20000, fast_weno=int(path.endswith("-fast")))
     s = _abi.Solver(lib, "fs_", P, g, 0)
     try:
         s.set_fields(phi, *vel)
@@ -530,9 +539,9 @@ def run_path(lib, scene, n, path, warmup, steps, sync):
         out = {"ms_per_step": ms, "stats": recs, "warmup_stats": warm,
                "frame": {k: _abi.frame_kernel_samples(s, getattr(_abi, k))
                          for k in ("FS_KS_REINIT", "FS_KS_MG_SMOOTH")}}
-        if path != "mg":
+        if not path.startswith("mg"):
             out["pcg"] = _abi.kernel_samples(s, prec)
-            if path == "refine":
+            if path.startswith("refine"):
                 out["pcg_repeat"] = _abi.time_pcg_kernels(s, prec, 20)
         snaps[warmup + steps] = s.get_fields()
         out["snapshots"] = snaps
@@ -728,7 +737,8 @@ def main():
     roof = None
     if R is not None:
         try:
-            roof = frame_roofline(R, a.scene, a.n, a.steps) if head == "mg" else jacobi_roofline(R, head, a.scene, a.n)
+            roof = (frame_roofline(R, a.scene, a.n, a.steps) if head.startswith("mg")
+                    else jacobi_roofline(R, head.split("-")[0], a.scene, a.n))
         except Exception as e:
             roof = {"error": str(e)}
     res = {}
@@ -740,7 +750,7 @@ def main():
         if "pcg" in runs[p]:
             k, cnt = runs[p]["pcg"]
             res[p]["pcg_kernels"] = pcg_roofline(k[:2], cnt, runs[p].get("pcg_repeat"), cells, a.n,
-                                                 8 if p == "fp64" else 4)
+                                                 8 if p.startswith("fp64") else 4)
     if "fp64" in runs:
         for p in runs:
             if p != "fp64":
@@ -779,7 +789,7 @@ def main():
         "higher_is_better": False,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere" if head != "fp64" else "f64",
+        "dtype": "f32-inner/f64-refined (pressure solve), f64 elsewhere" if not head.startswith("fp64") else "f64",
         "data": "synthetic",
         "config": {"workload": f"{a.scene} {a.n}^3, air_bubble_rising (oil 800/0.05 in water 1000/1e-3), "
                                f"{a.warmup} + {a.steps} frames from rest, dt from the CFL rule before every frame "

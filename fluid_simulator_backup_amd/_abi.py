@@ -77,6 +77,7 @@ class FsParams(C.Structure):
         ("is_sourcing_active", _I),
         ("is_projection_active", _I),
         ("semi_implicit_viscosity", _I),
+        ("fast_weno", _I),
     ]
 
 
@@ -127,6 +128,7 @@ def default_params(**kw):
     p.is_sourcing_active = 1
     p.is_projection_active = 1
     p.semi_implicit_viscosity = 0
+    p.fast_weno = 0
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):

@@ -420,6 +420,8 @@ class EulerianFluidSolver3D {
     // the fixed schedule's diagonal: each face's beta over its own d^2
     int axis_diag() const { return fixed() && !cubic() ? 1 : 0; }
     bool tension() const { return P.surface_tension != 0.0; }
+    // WENO5 derivatives in the fewer-operation form (not bit for bit)
+    bool fast() const { return P.fast_weno != 0; }
 
     // copy of field src (component box comp) onto the step tape
     double* Record(const double* src, int comp) {
@@ -462,7 +464,7 @@ class EulerianFluidSolver3D {
             RefreshLevelsetGhost();
             RefreshVelocityGhosts();
         }
-        fs_weno5_stage(st, phi, sghost, gu, gv, gw, L, 0, cell, P.levelset_epsilon, dt);
+        fs_weno5_stage(st, phi, sghost, gu, gv, gw, L, 0, cell, P.levelset_epsilon, dt, fast());
     }
 
     // ADVECTION_3D::Solve_Velocity, ADVECTION_3D.h:244-358 (air/water).  The
@@ -472,9 +474,9 @@ class EulerianFluidSolver3D {
         if (fixed()) RefreshVelocityGhosts();
         if (P.use_5th_weno_v) {
             bool own = fixed();
-            fs_weno5_stage(st, u, gu, gu, gv, gw, L, 1, macx, P.velocity_epsilon, dt);
-            fs_weno5_stage(st, v, own ? gv : gu, gu, gv, gw, L, 2, macy, P.velocity_epsilon, dt);
-            fs_weno5_stage(st, w, own ? gw : gu, gu, gv, gw, L, 3, macz, P.velocity_epsilon, dt);
+            fs_weno5_stage(st, u, gu, gu, gv, gw, L, 1, macx, P.velocity_epsilon, dt, fast());
+            fs_weno5_stage(st, v, own ? gv : gu, gu, gv, gw, L, 2, macy, P.velocity_epsilon, dt, fast());
+            fs_weno5_stage(st, w, own ? gw : gu, gu, gv, gw, L, 3, macz, P.velocity_epsilon, dt, fast());
         } else {
             fs_eno3_stage(st, u, gu, gu, gv, gw, L, 1, macx, dt);
             fs_eno3_stage(st, v, gv, gu, gv, gw, L, 2, macy, dt);
@@ -1391,7 +1393,8 @@ class EulerianFluidSolver3D {
             const bool last = it == n - 1 || recording;
             const bool timed = sampling && it == 0;
             if (timed) kprobe[FS_KS_REINIT].begin(st);
-            fs_reinit_sweep(st, last ? phi : nxt, src, sgn, L, cell, P.levelset_epsilon, fictious_dt, fixed());
+            fs_reinit_sweep(st, last ? phi : nxt, src, sgn, L, cell, P.levelset_epsilon, fictious_dt, fixed(),
+                            fast());
             if (timed) kprobe[FS_KS_REINIT].end(st);
             if (!last) {
                 if (fixed())

@@ -24,8 +24,10 @@ void fs_fill_linear_split(hipStream_t st, double* dst, const double* src, const 
                           FsBox b, int depth);
 
 // ---- advection ------------------------------------------------------------
+// fast: the WENO5 derivatives in fs_weno5_fast's operation order (fast_weno)
 void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
-                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt);
+                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt,
+                    bool fast = false);
 void fs_eno3_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double dt);
 
@@ -108,7 +110,7 @@ void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, 
 // One sweep from the ghost copy `ghost` (its box holds phi) into out's box;
 // out may be phi or another field, not ghost.
 void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const double* sgn, const FsLayout& L,
-                     const FsGridConst& cell, double eps, double dt, int exact);
+                     const FsGridConst& cell, double eps, double dt, int exact, bool fast = false);
 
 // ---- misc -----------------------------------------------------------------
 void fs_average(hipStream_t st, double* phi, const double* tmp, const FsLayout& L);  // phi = 0.5*(tmp+phi)

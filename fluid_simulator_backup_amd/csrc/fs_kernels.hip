@@ -152,6 +152,17 @@ __device__ __forceinline__ double upwind(double rho, double dt, double uu, doubl
     return rho - dt * (uu * X + vv * Y + ww * Z);
 }
 
+// WENO5 one-sided derivatives in the reference's operation order (FAST =
+// false, bit for bit) or in fs_weno5_fast's fewer operations
+template <bool FAST>
+__device__ __forceinline__ void weno5(const double* c, double inv, double eps, double& dmo, double& dpo) {
+    if constexpr (FAST)
+        fs_weno5_fast(c, inv, eps, dmo, dpo);
+    else
+        fs_weno5(c, inv, eps, dmo, dpo);
+}
+
+template <bool FAST>
 __global__ void __launch_bounds__(BX* BY) k_weno5_stage(double* __restrict__ rho, const double* __restrict__ gh,
                                                         const double* __restrict__ gu, const double* __restrict__ gv,
                                                         const double* __restrict__ gw, FsLayout L, FsBox b, int comp,
@@ -161,11 +172,11 @@ __global__ void __launch_bounds__(BX* BY) k_weno5_stage(double* __restrict__ rho
     long c = L.idx(i, j, k);
     double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
     gather7(gh, c, 1, f);
-    fs_weno5(f + 3, ix, eps, mx, px);
+    weno5<FAST>(f + 3, ix, eps, mx, px);
     gather7(gh, c, L.sx, f);
-    fs_weno5(f + 3, iy, eps, my, py);
+    weno5<FAST>(f + 3, iy, eps, my, py);
     gather7(gh, c, L.sy, f);
-    fs_weno5(f + 3, iz, eps, mz, pz);
+    weno5<FAST>(f + 3, iz, eps, mz, pz);
     vel_at(gu, gv, gw, L, comp, i, j, k, uu, vv, ww);
     rho[c] = upwind(rho[c], dt, uu, vv, ww, mx, px, my, py, mz, pz);
 }
@@ -280,10 +291,15 @@ void fs_adjoint_weno5_transport(hipStream_t st, const double* phi, const double*
 }
 
 void fs_weno5_stage(hipStream_t st, double* rho, const double* ghost, const double* gu, const double* gv,
-                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt) {
+                    const double* gw, const FsLayout& L, int comp, const FsGridConst& gc, double eps, double dt,
+                    bool fast) {
     FsBox b = fs_box(L, comp);
-    hipLaunchKernelGGL(k_weno5_stage, grid_for(b), dim3(BX, BY), 0, st, rho, ghost, gu, gv, gw, L, b, comp, gc.x.inv,
-                       gc.y.inv, gc.z.inv, eps, dt);
+    if (fast)
+        hipLaunchKernelGGL(k_weno5_stage<true>, grid_for(b), dim3(BX, BY), 0, st, rho, ghost, gu, gv, gw, L, b, comp,
+                           gc.x.inv, gc.y.inv, gc.z.inv, eps, dt);
+    else
+        hipLaunchKernelGGL(k_weno5_stage<false>, grid_for(b), dim3(BX, BY), 0, st, rho, ghost, gu, gv, gw, L, b, comp,
+                           gc.x.inv, gc.y.inv, gc.z.inv, eps, dt);
     (void)gc_dummy;
 }
 
@@ -1220,6 +1236,7 @@ void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, 
 // parenthesis of :1463/:1605 (x shock, y shock towards +, z expansion; the
 // Euclidean norm instead when `exact` is set), and the sign-preserving clamp
 // (:1624-1634).
+template <bool FAST>
 __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ out, const double* __restrict__ gh,
                                                    const double* __restrict__ sgn, FsLayout L, double ix, double iy,
                                                    double iz, double eps, double dt, int exact) {
@@ -1228,11 +1245,11 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ out, con
     long c = L.idx(i, j, k);
     double f[7], mx, px, my, py, mz, pz, X, Y, Z;
     gather7(gh, c, 1, f);
-    fs_weno5(f + 3, ix, eps, mx, px);
+    weno5<FAST>(f + 3, ix, eps, mx, px);
     gather7(gh, c, L.sx, f);
-    fs_weno5(f + 3, iy, eps, my, py);
+    weno5<FAST>(f + 3, iy, eps, my, py);
     gather7(gh, c, L.sy, f);
-    fs_weno5(f + 3, iz, eps, mz, pz);
+    weno5<FAST>(f + 3, iz, eps, mz, pz);
     double sg = sgn[c];
     double rho = gh[c];  // the ghost copy's box is phi
     int cx = fs_godunov_axis(sg, mx, px, X);
@@ -1254,9 +1271,13 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ out, con
 }
 
 void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const double* sgn, const FsLayout& L,
-                     const FsGridConst& cell, double eps, double dt, int exact) {
-    hipLaunchKernelGGL(k_reinit, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, out, ghost, sgn, L, cell.x.inv,
-                       cell.y.inv, cell.z.inv, eps, dt, exact);
+                     const FsGridConst& cell, double eps, double dt, int exact, bool fast) {
+    if (fast)
+        hipLaunchKernelGGL(k_reinit<true>, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, out, ghost, sgn, L,
+                           cell.x.inv, cell.y.inv, cell.z.inv, eps, dt, exact);
+    else
+        hipLaunchKernelGGL(k_reinit<false>, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, out, ghost, sgn, L,
+                           cell.x.inv, cell.y.inv, cell.z.inv, eps, dt, exact);
 }
 
 // ---------------------------------------------------------------------------

@@ -110,6 +110,14 @@ typedef struct fs_params {
      * (EULERIAN_FLUID_SOLVER_3D.h:2770-2773).  Needs
      * use_delta_function_formulation like the explicit path. */
     int    semi_implicit_viscosity;
+    /* No counterpart in the reference: 1 takes the WENO5 one-sided
+     * derivatives of the level-set stages, the WENO5 velocity stages and the
+     * reinitialization sweeps in fewer operations (fused multiply-adds, the
+     * weights in an algebraically equal product form, one division per cell
+     * and axis), FP64 throughout; the results then differ from the
+     * reference's operation order in the last bits of each derivative.
+     * 0 keeps that order, bit for bit. */
+    int    fast_weno;
 } fs_params;
 
 /* Per-step (or per-phase) statistics. */

@@ -148,15 +148,22 @@ TOL_FP32 = [(5, _tol((1e-11, 3e-11), (3e-9, 1e-7), (6e-8, 9e-8))),
             (50, _tol((5e-2, 2e-1), (3e-1, 3.0), (1.0, 3.0)))]
 
 
-def test_dam_break_64_20_frames_fp64(hip_lib, oracle_lib):
-    recs, cs = _oracle_run(oracle_lib, "dam_break", 64, 20)
+@pytest.fixture(scope="module")
+def dam64(oracle_lib):
+    return _oracle_run(oracle_lib, "dam_break", 64, 20)
+
+
+@pytest.mark.parametrize("fast", [0, 1])
+def test_dam_break_64_20_frames_fp64(hip_lib, dam64, fast):
+    recs, cs = dam64
     dts = [r["dt"] for r in recs]
-    grecs, gs, own = _gpu_replay(hip_lib, "dam_break", 64, dts, _abi.FS_FP64)
-    _check_dt("dam break 64 fp64", own, dts)
+    grecs, gs, own = _gpu_replay(hip_lib, "dam_break", 64, dts, _abi.FS_FP64, fast_weno=fast)
+    tag = "dam break 64 fp64" + (" fast" if fast else "")
+    _check_dt(tag, own, dts)
     for a, b in zip(grecs, recs):
         assert a["residual_rel"] <= 1e-8 and b["residual_rel"] <= 1e-8
         assert abs(a["pcg_iterations"] - b["pcg_iterations"]) <= 0.02 * b["pcg_iterations"] + 3
-    _check_snaps("dam break 64 fp64", gs, cs, TOL_FP64)
+    _check_snaps(tag, gs, cs, TOL_FP64)
 
 
 @pytest.fixture(scope="module")
@@ -164,16 +171,20 @@ def drop48(oracle_lib):
     return _oracle_run(oracle_lib, "rising_drop", 48, 50)
 
 
-@pytest.mark.parametrize("path", ["fp64", "refine", "mg"])
+@pytest.mark.parametrize("path", ["fp64", "refine", "mg", "fp64-fast", "mg-fast"])
 def test_rising_drop_48_50_frames(hip_lib, drop48, path):
+    """Every projection path against the checker's FP64 frames; "-fast" with
+    the WENO5 derivatives in fewer operations (fast_weno), which keeps the
+    FP64 path in the FP64 tolerance class."""
     recs, cs = drop48
     dts = [r["dt"] for r in recs]
-    grecs, gs, own = _gpu_replay(hip_lib, "rising_drop", 48, dts, PREC[path])
-    _check_dt(f"rising drop 48 {path}", own, dts, None if path == "fp64" else 10)
+    base = path.split("-")[0]
+    grecs, gs, own = _gpu_replay(hip_lib, "rising_drop", 48, dts, PREC[base], fast_weno=int(path.endswith("fast")))
+    _check_dt(f"rising drop 48 {path}", own, dts, None if base == "fp64" else 10)
     for a in grecs:
         assert a["residual_rel"] <= 1e-8, a
     print(path, "iterations per frame", [a["pcg_iterations"] for a in grecs][:10], "...")
-    _check_snaps(f"rising drop 48 {path}", gs, cs, TOL_FP64 if path == "fp64" else TOL_FP32)
+    _check_snaps(f"rising drop 48 {path}", gs, cs, TOL_FP64 if base == "fp64" else TOL_FP32)
 
 
 @pytest.mark.parametrize("path", ["fp64", "mg"])
