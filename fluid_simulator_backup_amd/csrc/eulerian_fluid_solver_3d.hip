@@ -111,7 +111,8 @@ class EulerianFluidSolver3D {
     float* phi32 = nullptr;  // the level set of the last assembly, for the FP32 SpMV's coefficients
     // FS_FP32_MG: the multigrid hierarchy, the CG's partial sums and totals
     FsMg mg;
-    double *mg_part = nullptr, *mg_tot = nullptr, *hmg_tot = nullptr;
+    double *mg_part = nullptr, *mg_mid = nullptr;
+    FsMgCg *mg_cg = nullptr, *hmg_cg = nullptr;  // the CG's device scalars and their pinned host copy
     // the projection's operator forms its face coefficients from phi (set by
     // the forward assembly, whose phi and densities define them)
     bool op_phi = false;
@@ -328,8 +329,9 @@ class EulerianFluidSolver3D {
         if (hvmax) (void)hipHostFree(hvmax);
         if (hkstat) (void)hipHostFree(hkstat);
         if (mg_part) (void)hipFree(mg_part);
-        if (mg_tot) (void)hipFree(mg_tot);
-        if (hmg_tot) (void)hipHostFree(hmg_tot);
+        if (mg_mid) (void)hipFree(mg_mid);
+        if (mg_cg) (void)hipFree(mg_cg);
+        if (hmg_cg) (void)hipHostFree(hmg_cg);
         for (auto& e : ev)
             if (e) (void)hipEventDestroy(e);
         for (auto& e : probe.ev)
@@ -830,47 +832,72 @@ class EulerianFluidSolver3D {
     // until ||r|| <= rtol ||r32|| or max_it.  Returns the iterations.
     int run_mgpcg32(double rtol, int max_it) {
         const FsLayout& Lz = L;
-        const int nb = fs_mg_blocks(Lz);
+        const int nb = fs_mgc_blocks(Lz);
         if (!mg_part) {
             mg_part = (double*)dalloc(sizeof(double) * 2 * (size_t)nb);
-            mg_tot = (double*)dalloc(sizeof(double) * (4 + 2 * FS_MG_MID));
-            HIPCHECK(hipHostMalloc((void**)&hmg_tot, sizeof(double) * 4));
+            mg_mid = (double*)dalloc(sizeof(double) * 2 * FS_MG_MID);
+            mg_cg = (FsMgCg*)dalloc(sizeof(FsMgCg));
+            HIPCHECK(hipHostMalloc((void**)&hmg_cg, sizeof(FsMgCg)));
         }
         const float ix = (float)cell.x.inv_d2, iy = (float)cell.y.inv_d2, iz = (float)cell.z.inv_d2;
         float *x = e32, *r = (float*)w32.r, *z = (float*)w32.s, *p = (float*)w32.p, *q = (float*)w32.q;
-        auto total = [&](int slots) {
-            fs_mg_total(st, mg_part, nb, slots, mg_tot + 4, mg_tot);
-            HIPCHECK(hipMemcpyAsync(hmg_tot, mg_tot, sizeof(double) * slots, hipMemcpyDeviceToHost, st));
-            sync();
-        };
-        HIPCHECK(hipMemsetAsync(x, 0, sizeof(float) * L.total, st));
+        fs_mg_zero(st, x, L.total, nullptr);
         HIPCHECK(hipMemcpyAsync(r, r32, sizeof(float) * L.total, hipMemcpyDeviceToDevice, st));
+        fs_mgc_start(st, mg_cg, rtol * rtol, max_it);
+        mg.skip = nullptr;
+        // the in-loop probe samples this V-cycle, which always runs; the
+        // loop's finest-level launches are counted below
+        FsLaunchProbe* kp = mg.probe;
         mg.Apply(st, r, z);
-        fs_mg_dot2(st, Lz, r, z, mg_part);
-        total(2);
-        double rz = hmg_tot[0], rr0 = hmg_tot[1];
-        if (!(rr0 > 0)) return 0;
+        mg.probe = nullptr;
+        fs_mgc_dot2(st, Lz, r, z, mg_part, mg_cg);
+        fs_mgc_total(st, mg_part, nb, 2, mg_mid, mg_cg, FS_MGC_INIT);
         HIPCHECK(hipMemcpyAsync(p, z, sizeof(float) * L.total, hipMemcpyDeviceToDevice, st));
-        int it = 0;
-        while (it < max_it) {
-            fs_mg_spmv_dot(st, Lz, bx32, by32, bz32, ix, iy, iz, p, q, mg_part);
-            total(1);
-            double pq = hmg_tot[0];
-            if (!(pq > 0)) break;
-            double alpha = rz / pq;
-            fs_mg_update(st, Lz, x, r, p, q, (float)alpha, mg_part);
-            total(1);
-            it++;
-            if (hmg_tot[0] <= rtol * rtol * rr0) break;
-            mg.Apply(st, r, z);
-            fs_mg_dot2(st, Lz, r, z, mg_part);
-            total(1);
-            double rz_new = hmg_tot[0];
-            fs_mg_direction(st, Lz, z, p, (float)(rz_new / rz));
-            rz = rz_new;
+        // Iterations are enqueued in batches with the scalars on the device;
+        // the host reads the state after each batch.  A batch is as long as
+        // the convergence rate seen so far says is left (2 at first, at most
+        // 8), and iterations past the stop return at once (`done`, which
+        // the V-cycle reads through mg.skip).
+        mg.skip = &mg_cg->done;
+        int enq = 0, batch = 2;
+        hmg_cg->done = 0;
+        hmg_cg->it = 0;
+        while (enq < max_it) {
+            const int k = std::min(batch, max_it - enq);
+            for (int n = 0; n < k; n++) {
+                fs_mgc_spmv_dot(st, Lz, bx32, by32, bz32, ix, iy, iz, p, q, mg_part, mg_cg);
+                fs_mgc_total(st, mg_part, nb, 1, mg_mid, mg_cg, FS_MGC_PQ);
+                fs_mgc_update(st, Lz, x, r, p, q, mg_part, mg_cg);
+                fs_mgc_total(st, mg_part, nb, 1, mg_mid, mg_cg, FS_MGC_RR);
+                mg.Apply(st, r, z);
+                fs_mgc_dot2(st, Lz, r, z, mg_part, mg_cg);
+                fs_mgc_total(st, mg_part, nb, 2, mg_mid, mg_cg, FS_MGC_RZ);
+                fs_mgc_direction(st, Lz, z, p, mg_cg);
+            }
+            enq += k;
+            check_launch();
+            HIPCHECK(hipMemcpyAsync(hmg_cg, mg_cg, sizeof(FsMgCg), hipMemcpyDeviceToHost, st));
+            sync();
+            if (hmg_cg->done) break;
+            // iterations left at the mean rate so far: rr falls by rate^it
+            const double target = hmg_cg->rtol2 * hmg_cg->rr0;
+            if (hmg_cg->it > 0 && hmg_cg->rr > 0 && hmg_cg->rr < hmg_cg->rr0 && target > 0) {
+                const double per = std::log(hmg_cg->rr / hmg_cg->rr0) / hmg_cg->it;
+                const double left = std::log(target / hmg_cg->rr) / per;
+                batch = (int)std::min(8.0, std::max(1.0, std::ceil(left)));
+            } else {
+                batch = 2;
+            }
         }
-        check_launch();
-        return it;
+        mg.skip = nullptr;
+        mg.probe = kp;
+        if (kp) {
+            // V-cycles the loop ran: one per update, but for the last when
+            // the stop came right after it
+            const int d = hmg_cg->done;
+            kp->launches += (long)std::max(0, hmg_cg->it - (d == 1 || d == 3 ? 1 : 0)) * mg.FinestLaunchesPerCycle();
+        }
+        return hmg_cg->it;
     }
 
     // FP32 multigrid-preconditioned CG inside FP64 iterative refinement:

@@ -52,6 +52,8 @@ class FsMg {
     // which changes no value.
     void Apply(hipStream_t st, const float* r, float* z);
     int Levels() const { return (int)lv_.size(); }
+    // finest-level smoother launches of one V-cycle
+    int FinestLaunchesPerCycle() const { return (split_ ? 4 : 2) * nu_; }
     bool Tuned() const { return tuned_ || split_; }
     // tunes the fused sweep now, on the finest level with b = r and x = z
     // (read, and the level's own residual buffer written); Apply tunes on its
@@ -68,6 +70,10 @@ class FsMg {
     // of every V-cycle is timed and every finest-level smoother launch
     // counted (measurement only).
     FsLaunchProbe* probe = nullptr;
+    // When set, every V-cycle launch returns at once while *skip != 0 (the
+    // CG's `done` flag, so iterations enqueued past convergence cost only
+    // their launches).
+    const int* skip = nullptr;
 
    private:
     void Cycle(hipStream_t st, int l);
@@ -80,21 +86,45 @@ class FsMg {
     float* diag0_ = nullptr;
 };
 
-// ---- vector kernels of the FS_FP32_MG conjugate gradient ---------------------
-// Partial sums: one double per workgroup of the cell grid (fs_mg_blocks), each
-// kernel writes its own; fs_mg_total sums them in a fixed order into *out.
-int fs_mg_blocks(const FsLayout& L);
-// q = A p (flux form, pin-free); partial p.q
-void fs_mg_spmv_dot(hipStream_t st, const FsLayout& L, const float* bx, const float* by, const float* bz, float idx2,
-                    float idy2, float idz2, const float* p, float* q, double* part);
-// x += alpha p, r -= alpha q; partial r.r
-void fs_mg_update(hipStream_t st, const FsLayout& L, float* x, float* r, const float* p, const float* q, float alpha,
-                  double* part);
-// partials of a.b and b.b (two slots: part[0..nb), part[nb..2nb))
-void fs_mg_dot2(hipStream_t st, const FsLayout& L, const float* a, const float* b, double* part);
-// p = z + beta p
-void fs_mg_direction(hipStream_t st, const FsLayout& L, const float* z, float* p, float beta);
-// out[s] = sum of part[s*nb .. (s+1)*nb) in a fixed order, for s < slots;
-// mid holds slots * FS_MG_MID doubles of intermediate sums
+// ---- the FS_FP32_MG conjugate gradient, scalars on the device -----------------
+// The CG's scalars live in device memory (FsMgCg): every vector kernel reads
+// alpha, beta and `done` there, and single-workgroup kernels form them from
+// the partial sums, so a whole iteration (and its V-cycle, through
+// FsMg::skip) is enqueued without the host; after `done` is set the
+// remaining launches return at once.  The host reads the state every few
+// iterations.  Partial sums: one double per workgroup of a z-marching
+// 64 x 4 x FS_MGC_KZ tiling (fs_mgc_blocks), summed in a fixed order.
+struct FsMgCg {
+    double rz, rr0, rtol2;  // r.z of the current iteration, r.r at the start, rtol^2
+    double pq, rr;          // p.q and r.r of the last iteration
+    float alpha, beta;
+    int it, max_it;
+    int done;  // 0 running; 1 converged; 2 p.q <= 0; 3 max_it reached; 4 zero right-hand side
+};
+#define FS_MGC_KZ 8
 #define FS_MG_MID 128
-void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* mid, double* out);
+int fs_mgc_blocks(const FsLayout& L);
+// state := start of a solve (it 0, done 0, rtol^2, max_it)
+void fs_mgc_start(hipStream_t st, FsMgCg* S, double rtol2, int max_it);
+// partials of a.b and a.a (two runs of nb); skipped once done
+void fs_mgc_dot2(hipStream_t st, const FsLayout& L, const float* a, const float* b, double* part, const FsMgCg* S);
+// q = A p (flux form, pin-free), partial p.q; skipped once done
+void fs_mgc_spmv_dot(hipStream_t st, const FsLayout& L, const float* bx, const float* by, const float* bz, float idx2,
+                     float idy2, float idz2, const float* p, float* q, double* part, const FsMgCg* S);
+// x += alpha p, r -= alpha q, partial r.r; skipped once done
+void fs_mgc_update(hipStream_t st, const FsLayout& L, float* x, float* r, const float* p, const float* q, double* part,
+                   const FsMgCg* S);
+// p = z + beta p; skipped once done
+void fs_mgc_direction(hipStream_t st, const FsLayout& L, const float* z, float* p, const FsMgCg* S);
+// Sum `slots` runs of nb partials in a fixed order (mid: slots * FS_MG_MID
+// doubles) and update the state: FS_MGC_INIT (rz, rr0 from a.b, a.a; done 4
+// if rr0 is not positive), FS_MGC_PQ (alpha = rz / p.q; done 2 if p.q is not
+// positive), FS_MGC_RR (it += 1; done 1 at r.r <= rtol^2 rr0, else 3 at
+// max_it), FS_MGC_RZ (beta = rz_new / rz, rz = rz_new).
+#define FS_MGC_INIT 0
+#define FS_MGC_PQ 1
+#define FS_MGC_RR 2
+#define FS_MGC_RZ 3
+void fs_mgc_total(hipStream_t st, const double* part, int nb, int slots, double* mid, FsMgCg* S, int kind);
+// v = 0 over a whole padded field (n elements); skipped while *skip != 0
+void fs_mg_zero(hipStream_t st, float* v, long n, const int* skip);

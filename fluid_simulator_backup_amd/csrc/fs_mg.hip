@@ -153,7 +153,9 @@ __global__ void __launch_bounds__(MNT) k_mg_coarsen(FsLayout Lf, const float* __
 __global__ void __launch_bounds__(MNT) k_mg_smooth(FsLayout L, const float* __restrict__ bx,
                                                    const float* __restrict__ by, const float* __restrict__ bz,
                                                    float ix, float iy, float iz, const float* __restrict__ diag,
-                                                   const float* __restrict__ b, float* __restrict__ x, int color) {
+                                                   const float* __restrict__ b, float* __restrict__ x, int color,
+                                                   const int* __restrict__ skip) {
+    if (skip && *skip) return;
     int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny || ((i + j + k) & 1) != color) return;
     long c = L.idx(i, j, k);
@@ -167,7 +169,9 @@ __global__ void __launch_bounds__(MNT) k_mg_smooth(FsLayout L, const float* __re
 __global__ void __launch_bounds__(MNT) k_mg_residual(FsLayout L, const float* __restrict__ bx,
                                                      const float* __restrict__ by, const float* __restrict__ bz,
                                                      float ix, float iy, float iz, const float* __restrict__ b,
-                                                     const float* __restrict__ x, float* __restrict__ r) {
+                                                     const float* __restrict__ x, float* __restrict__ r,
+                                                     const int* __restrict__ skip) {
+    if (skip && *skip) return;
     int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
     long c = L.idx(i, j, k);
@@ -177,7 +181,9 @@ __global__ void __launch_bounds__(MNT) k_mg_residual(FsLayout L, const float* __
 // coarse right-hand side: transpose of the prolongation over the 4x4x4 fine
 // block around the coarse cell, divided by 8; coarse x set to 0
 __global__ void __launch_bounds__(MNT) k_mg_restrict(FsLayout Lf, const float* __restrict__ rf, FsLayout Lc,
-                                                     float* __restrict__ bc, float* __restrict__ xc) {
+                                                     float* __restrict__ bc, float* __restrict__ xc,
+                                                     const int* __restrict__ skip) {
+    if (skip && *skip) return;
     int I = blockIdx.x * MBX + threadIdx.x, J = blockIdx.y * MBY + threadIdx.y, K = blockIdx.z;
     if (I >= Lc.nx || J >= Lc.ny) return;
     float s = 0;
@@ -207,7 +213,8 @@ __global__ void __launch_bounds__(MNT) k_mg_restrict(FsLayout Lf, const float* _
 
 // x_fine += P x_coarse (trilinear, clamped)
 __global__ void __launch_bounds__(MNT) k_mg_prolong(FsLayout Lc, const float* __restrict__ xc, FsLayout Lf,
-                                                    float* __restrict__ xf) {
+                                                    float* __restrict__ xf, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
     if (i >= Lf.nx || j >= Lf.ny) return;
     auto pair = [](int f, int nc, int& c0, int& c1) {
@@ -236,7 +243,9 @@ __global__ void __launch_bounds__(MNT) k_mg_prolong(FsLayout Lc, const float* __
 __global__ void __launch_bounds__(MNT) k_mg_coarse(FsLayout L, const float* __restrict__ bx,
                                                    const float* __restrict__ by, const float* __restrict__ bz,
                                                    float ix, float iy, float iz, const float* __restrict__ diag,
-                                                   const float* __restrict__ b, float* __restrict__ x, int sweeps) {
+                                                   const float* __restrict__ b, float* __restrict__ x, int sweeps,
+                                                   const int* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ float sx[MG_CPAD], sbx[MG_CPAD], sby[MG_CPAD], sbz[MG_CPAD], sb[MG_CPAD], sd[MG_CPAD];
     const int t = threadIdx.y * MBX + threadIdx.x;
     const int px = L.nx + 2, py = L.ny + 2, pxy = px * py, ptot = pxy * (L.nz + 2);
@@ -316,7 +325,8 @@ __global__ void __launch_bounds__(SBX* SBY) k_mg_rb_sweep(FsLayout L, const floa
                                                        const float* __restrict__ by, const float* __restrict__ bz,
                                                        float ix, float iy, float iz, const float* __restrict__ b,
                                                        const float* __restrict__ x_in, float* __restrict__ x_out,
-                                                       int c1, int kc, int gx, int gy) {
+                                                       int c1, int kc, int gx, int gy, const int* __restrict__ skip) {
+    if (skip && *skip) return;
     __shared__ float R[4][SRY * SRX];
     const int tx = threadIdx.x, ty = threadIdx.y, t = ty * SBX + tx;
     // workgroup b runs on XCD b % 8; with a multiple of 8 tiles, each XCD
@@ -392,64 +402,104 @@ __global__ void __launch_bounds__(SBX* SBY) k_mg_rb_sweep(FsLayout L, const floa
     }
 }
 
-// ---- CG vector kernels ------------------------------------------------------
-__global__ void __launch_bounds__(MNT) k_mg_spmv_dot(FsLayout L, const float* __restrict__ bx,
-                                                     const float* __restrict__ by, const float* __restrict__ bz,
-                                                     float ix, float iy, float iz, const float* __restrict__ p,
-                                                     float* __restrict__ q, double* part) {
-    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+// ---- CG vector kernels (scalars in FsMgCg) ----------------------------------
+// A workgroup owns a 64 x 4 column tile and marches up FS_MGC_KZ planes,
+// each thread keeping its column's running partial; one partial per
+// workgroup.
+dim3 cg_grid(const FsLayout& L) {
+    return dim3((L.nx + MBX - 1) / MBX, (L.ny + MBY - 1) / MBY, (L.nz + FS_MGC_KZ - 1) / FS_MGC_KZ);
+}
+
+__global__ void __launch_bounds__(MNT) k_mgc_spmv_dot(FsLayout L, const float* __restrict__ bx,
+                                                      const float* __restrict__ by, const float* __restrict__ bz,
+                                                      float ix, float iy, float iz, const float* __restrict__ p,
+                                                      float* __restrict__ q, double* part, const FsMgCg* S) {
+    if (S->done) return;
+    const int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k0 = blockIdx.z * FS_MGC_KZ;
+    const int k1 = min(L.nz, k0 + FS_MGC_KZ);
     double acc = 0;
     if (i < L.nx && j < L.ny) {
-        long c = L.idx(i, j, k);
-        float v = flux_row(L, bx, by, bz, ix, iy, iz, p, c);
-        q[c] = v;
-        acc = (double)p[c] * (double)v;
+        long c = L.idx(i, j, k0);
+        float pm = p[c - L.sy], pc = p[c];
+        for (int k = k0; k < k1; k++, c += L.sy) {
+            const float pp = p[c + L.sy];
+            // flux_row's terms in its order, the z neighbours from the march
+            const float v = bx[c] * (ix * (pc - p[c - 1])) + bx[c + 1] * (ix * (pc - p[c + 1])) +
+                            by[c] * (iy * (pc - p[c - L.sx])) + by[c + L.sx] * (iy * (pc - p[c + L.sx])) +
+                            bz[c] * (iz * (pc - pm)) + bz[c + L.sy] * (iz * (pc - pp));
+            q[c] = v;
+            acc += (double)pc * (double)v;
+            pm = pc;
+            pc = pp;
+        }
     }
     block_partial(acc, part);
 }
 
-__global__ void __launch_bounds__(MNT) k_mg_update(FsLayout L, float* __restrict__ x, float* __restrict__ r,
-                                                   const float* __restrict__ p, const float* __restrict__ q,
-                                                   float alpha, double* part) {
-    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+__global__ void __launch_bounds__(MNT) k_mgc_update(FsLayout L, float* __restrict__ x, float* __restrict__ r,
+                                                    const float* __restrict__ p, const float* __restrict__ q,
+                                                    double* part, const FsMgCg* S) {
+    if (S->done) return;
+    const float alpha = S->alpha;
+    const int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k0 = blockIdx.z * FS_MGC_KZ;
+    const int k1 = min(L.nz, k0 + FS_MGC_KZ);
     double acc = 0;
     if (i < L.nx && j < L.ny) {
-        long c = L.idx(i, j, k);
-        x[c] += alpha * p[c];
-        float rv = r[c] - alpha * q[c];
-        r[c] = rv;
-        acc = (double)rv * (double)rv;
+        long c = L.idx(i, j, k0);
+        for (int k = k0; k < k1; k++, c += L.sy) {
+            x[c] += alpha * p[c];
+            const float rv = r[c] - alpha * q[c];
+            r[c] = rv;
+            acc += (double)rv * (double)rv;
+        }
     }
     block_partial(acc, part);
 }
 
-__global__ void __launch_bounds__(MNT) k_mg_dot2(FsLayout L, const float* __restrict__ a,
-                                                 const float* __restrict__ b, double* part, int nb) {
-    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+__global__ void __launch_bounds__(MNT) k_mgc_dot2(FsLayout L, const float* __restrict__ a,
+                                                  const float* __restrict__ b, double* part, int nb,
+                                                  const FsMgCg* S) {
+    if (S->done) return;
+    const int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k0 = blockIdx.z * FS_MGC_KZ;
+    const int k1 = min(L.nz, k0 + FS_MGC_KZ);
     double ab = 0, aa = 0;
     if (i < L.nx && j < L.ny) {
-        long c = L.idx(i, j, k);
-        double av = a[c];
-        ab = av * (double)b[c];
-        aa = av * av;
+        long c = L.idx(i, j, k0);
+        for (int k = k0; k < k1; k++, c += L.sy) {
+            const double av = a[c];
+            ab += av * (double)b[c];
+            aa += av * av;
+        }
     }
     block_partial(ab, part);
-    __syncthreads();
     block_partial(aa, part + nb);
 }
 
-__global__ void __launch_bounds__(MNT) k_mg_direction(FsLayout L, const float* __restrict__ z, float* __restrict__ p,
-                                                      float beta) {
-    int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k = blockIdx.z;
+__global__ void __launch_bounds__(MNT) k_mgc_direction(FsLayout L, const float* __restrict__ z, float* __restrict__ p,
+                                                       const FsMgCg* S) {
+    if (S->done) return;
+    const float beta = S->beta;
+    const int i = blockIdx.x * MBX + threadIdx.x, j = blockIdx.y * MBY + threadIdx.y, k0 = blockIdx.z * FS_MGC_KZ;
     if (i >= L.nx || j >= L.ny) return;
-    long c = L.idx(i, j, k);
-    p[c] = z[c] + beta * p[c];
+    const int k1 = min(L.nz, k0 + FS_MGC_KZ);
+    long c = L.idx(i, j, k0);
+    for (int k = k0; k < k1; k++, c += L.sy) p[c] = z[c] + beta * p[c];
+}
+
+__global__ void k_mgc_start(FsMgCg* S, double rtol2, int max_it) {
+    FsMgCg s{};
+    s.rtol2 = rtol2;
+    s.max_it = max_it;
+    *S = s;
 }
 
 // Sums of `slots` runs of nb partials in two launches, each in a fixed
 // order: workgroup (b, s) sums its contiguous share of run s into
-// mid[s * FS_MG_MID + b]; then one workgroup per run sums those.
-__global__ void __launch_bounds__(MNT) k_mg_total_mid(const double* part, int nb, double* mid) {
+// mid[s * FS_MG_MID + b]; then one workgroup sums each run's mids and
+// updates the state.
+__global__ void __launch_bounds__(MNT) k_mgc_total_mid(const double* part, int nb, double* mid, const FsMgCg* S,
+                                                       int always) {
+    if (!always && S->done) return;
     __shared__ double sh[MNT];
     const int t = threadIdx.x, b = blockIdx.x;
     const int share = (nb + FS_MG_MID - 1) / FS_MG_MID;
@@ -466,19 +516,51 @@ __global__ void __launch_bounds__(MNT) k_mg_total_mid(const double* part, int nb
     if (t == 0) mid[blockIdx.y * FS_MG_MID + b] = sh[0];
 }
 
-__global__ void __launch_bounds__(MNT) k_mg_total(const double* part, int nb, double* out) {
+__global__ void __launch_bounds__(MNT) k_mgc_finish(const double* mid, int slots, FsMgCg* S, int kind) {
+    if (kind != FS_MGC_INIT && S->done) return;
     __shared__ double sh[MNT];
+    __shared__ double tot[2];
     const int t = threadIdx.x;
-    const double* src = part + (long)blockIdx.x * nb;
-    double s = 0;
-    for (int m = t; m < nb; m += MNT) s += src[m];
-    sh[t] = s;
-    __syncthreads();
-    for (int w = MNT / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] += sh[t + w];
+    for (int s = 0; s < slots; s++) {
+        sh[t] = t < FS_MG_MID ? mid[s * FS_MG_MID + t] : 0.0;
+        __syncthreads();
+        for (int w = MNT / 2; w > 0; w >>= 1) {
+            if (t < w) sh[t] += sh[t + w];
+            __syncthreads();
+        }
+        if (t == 0) tot[s] = sh[0];
         __syncthreads();
     }
-    if (t == 0) out[blockIdx.x] = sh[0];
+    if (t != 0) return;
+    FsMgCg s = *S;
+    const double v = tot[0];
+    if (kind == FS_MGC_INIT) {
+        s.rz = v;
+        s.rr0 = tot[1];
+        s.done = s.rr0 > 0 ? 0 : 4;
+    } else if (kind == FS_MGC_PQ) {
+        s.pq = v;
+        if (!(v > 0))
+            s.done = 2;
+        else
+            s.alpha = (float)(s.rz / v);
+    } else if (kind == FS_MGC_RR) {
+        s.rr = v;
+        s.it++;
+        if (v <= s.rtol2 * s.rr0)
+            s.done = 1;
+        else if (s.it >= s.max_it)
+            s.done = 3;
+    } else {
+        s.beta = (float)(v / s.rz);
+        s.rz = v;
+    }
+    *S = s;
+}
+
+__global__ void __launch_bounds__(MNT) k_mg_zero(float* __restrict__ v, long n, const int* skip) {
+    if (skip && *skip) return;
+    for (long m = (long)blockIdx.x * MNT + threadIdx.x; m < n; m += (long)gridDim.x * MNT) v[m] = 0.f;
 }
 
 }  // namespace
@@ -569,7 +651,7 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
         for (int s = 0; s < 2 * nu_; s++) {
             if (timed && s == 0) probe->begin(st);
             hipLaunchKernelGGL(k_mg_smooth, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag, v.b,
-                               v.x, (s & 1) ^ c1);
+                               v.x, (s & 1) ^ c1, skip);
             if (timed && s == 0) probe->end(st);
         }
         if (probe && l == 0) probe->launches += 2 * nu_;
@@ -588,7 +670,7 @@ void FsMg::Smooth(hipStream_t st, int l, int c1) {
     for (int s = 0; s < nu_; s++) {
         if (timed && s == 0) probe->begin(st);
         hipLaunchKernelGGL(k_mg_rb_sweep, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, in, out, c1,
-                           kc, gx, gy);
+                           kc, gx, gy, skip);
         if (timed && s == 0) probe->end(st);
         std::swap(in, out);
     }
@@ -601,15 +683,15 @@ void FsMg::Cycle(hipStream_t st, int l) {
     const dim3 g = cells_grid(v.L), t(MBX, MBY);
     if (l + 1 == (int)lv_.size()) {
         hipLaunchKernelGGL(k_mg_coarse, dim3(1), t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.diag,
-                           v.b, v.x, MG_COARSE_SWEEPS);
+                           v.b, v.x, MG_COARSE_SWEEPS, skip);
         return;
     }
     Smooth(st, l, 0);  // red, black, ...
-    hipLaunchKernelGGL(k_mg_residual, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x, v.r);
+    hipLaunchKernelGGL(k_mg_residual, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x, v.r, skip);
     FsMgLevel& c = lv_[l + 1];
-    hipLaunchKernelGGL(k_mg_restrict, cells_grid(c.L), t, 0, st, v.L, v.r, c.L, c.b, c.x);
+    hipLaunchKernelGGL(k_mg_restrict, cells_grid(c.L), t, 0, st, v.L, v.r, c.L, c.b, c.x, skip);
     Cycle(st, l + 1);
-    hipLaunchKernelGGL(k_mg_prolong, g, t, 0, st, c.L, c.x, v.L, v.x);
+    hipLaunchKernelGGL(k_mg_prolong, g, t, 0, st, c.L, c.x, v.L, v.x, skip);
     Smooth(st, l, 1);  // ..., black, red
 }
 
@@ -637,7 +719,7 @@ void FsMg::Tune(hipStream_t st, const float* r, float* z) {
             for (int rep = 0; rep < 4; rep++) {
                 check(hipEventRecord(e0, st));
                 hipLaunchKernelGGL(k_mg_rb_sweep, g, t, 0, st, v.L, v.bx, v.by, v.bz, v.idx2, v.idy2, v.idz2, v.b, v.x,
-                                   v.r, 0, k, gx, gy);
+                                   v.r, 0, k, gx, gy, (const int*)nullptr);
                 check(hipEventRecord(e1, st));
                 check(hipEventSynchronize(e1));
                 float ms = 0;
@@ -663,36 +745,46 @@ void FsMg::Apply(hipStream_t st, const float* r, float* z) {
     v.b = (float*)r;
     v.x = z;
     if (!tuned_ && !split_) Tune(st, r, z);
-    check(hipMemsetAsync(z, 0, sizeof(float) * (size_t)v.L.total, st));
+    fs_mg_zero(st, z, v.L.total, skip);
     Cycle(st, 0);
 }
 
-// ---- CG vector kernels ------------------------------------------------------
-int fs_mg_blocks(const FsLayout& L) {
-    dim3 g = cells_grid(L);
+// ---- CG vector kernels -------------------------------------------------------
+int fs_mgc_blocks(const FsLayout& L) {
+    dim3 g = cg_grid(L);
     return (int)(g.x * g.y * g.z);
 }
 
-void fs_mg_spmv_dot(hipStream_t st, const FsLayout& L, const float* bx, const float* by, const float* bz, float idx2,
-                    float idy2, float idz2, const float* p, float* q, double* part) {
-    hipLaunchKernelGGL(k_mg_spmv_dot, cells_grid(L), dim3(MBX, MBY), 0, st, L, bx, by, bz, idx2, idy2, idz2, p, q,
-                       part);
+void fs_mgc_start(hipStream_t st, FsMgCg* S, double rtol2, int max_it) {
+    hipLaunchKernelGGL(k_mgc_start, dim3(1), dim3(1), 0, st, S, rtol2, max_it);
 }
 
-void fs_mg_update(hipStream_t st, const FsLayout& L, float* x, float* r, const float* p, const float* q, float alpha,
-                  double* part) {
-    hipLaunchKernelGGL(k_mg_update, cells_grid(L), dim3(MBX, MBY), 0, st, L, x, r, p, q, alpha, part);
+void fs_mgc_dot2(hipStream_t st, const FsLayout& L, const float* a, const float* b, double* part, const FsMgCg* S) {
+    hipLaunchKernelGGL(k_mgc_dot2, cg_grid(L), dim3(MBX, MBY), 0, st, L, a, b, part, fs_mgc_blocks(L), S);
 }
 
-void fs_mg_dot2(hipStream_t st, const FsLayout& L, const float* a, const float* b, double* part) {
-    hipLaunchKernelGGL(k_mg_dot2, cells_grid(L), dim3(MBX, MBY), 0, st, L, a, b, part, fs_mg_blocks(L));
+void fs_mgc_spmv_dot(hipStream_t st, const FsLayout& L, const float* bx, const float* by, const float* bz, float idx2,
+                     float idy2, float idz2, const float* p, float* q, double* part, const FsMgCg* S) {
+    hipLaunchKernelGGL(k_mgc_spmv_dot, cg_grid(L), dim3(MBX, MBY), 0, st, L, bx, by, bz, idx2, idy2, idz2, p, q,
+                       part, S);
 }
 
-void fs_mg_direction(hipStream_t st, const FsLayout& L, const float* z, float* p, float beta) {
-    hipLaunchKernelGGL(k_mg_direction, cells_grid(L), dim3(MBX, MBY), 0, st, L, z, p, beta);
+void fs_mgc_update(hipStream_t st, const FsLayout& L, float* x, float* r, const float* p, const float* q, double* part,
+                   const FsMgCg* S) {
+    hipLaunchKernelGGL(k_mgc_update, cg_grid(L), dim3(MBX, MBY), 0, st, L, x, r, p, q, part, S);
 }
 
-void fs_mg_total(hipStream_t st, const double* part, int nb, int slots, double* mid, double* out) {
-    hipLaunchKernelGGL(k_mg_total_mid, dim3(FS_MG_MID, slots), dim3(MNT), 0, st, part, nb, mid);
-    hipLaunchKernelGGL(k_mg_total, dim3(slots), dim3(MNT), 0, st, mid, FS_MG_MID, out);
+void fs_mgc_direction(hipStream_t st, const FsLayout& L, const float* z, float* p, const FsMgCg* S) {
+    hipLaunchKernelGGL(k_mgc_direction, cg_grid(L), dim3(MBX, MBY), 0, st, L, z, p, S);
+}
+
+void fs_mgc_total(hipStream_t st, const double* part, int nb, int slots, double* mid, FsMgCg* S, int kind) {
+    hipLaunchKernelGGL(k_mgc_total_mid, dim3(FS_MG_MID, slots), dim3(MNT), 0, st, part, nb, mid, S,
+                       kind == FS_MGC_INIT ? 1 : 0);
+    hipLaunchKernelGGL(k_mgc_finish, dim3(1), dim3(MNT), 0, st, mid, slots, S, kind);
+}
+
+void fs_mg_zero(hipStream_t st, float* v, long n, const int* skip) {
+    const long nb = std::min<long>((n + MNT - 1) / MNT, 4096);
+    hipLaunchKernelGGL(k_mg_zero, dim3((unsigned)nb), dim3(MNT), 0, st, v, n, skip);
 }
