@@ -927,9 +927,20 @@ class EulerianFluidSolver3D {
         const double rt2 = P.rel_tolerance * P.rel_tolerance;
         double bb = 0, rr = true_residual(bb);
         int sweeps = 0, its = 0;
+        const bool fixed_inner = std::getenv("FS_MG_FIXED_INNER") != nullptr;
         while (bb > 0 && rr > rt2 * bb && sweeps < 40) {
+            // Each sweep's FP32 solve asks for the reduction still missing
+            // (half of it again, as a margin), between FS_MG_INNER_RTOL --
+            // about what FP32 attains -- and 0.1: a sweep that starts close
+            // to the target then takes one or two iterations, not a full
+            // FS_MG_INNER_RTOL reduction past it.
+            double inner = FS_MG_INNER_RTOL;
+            if (!fixed_inner) {
+                const double need = 0.5 * P.rel_tolerance / std::sqrt(rr / bb);
+                inner = std::min(0.1, std::max((double)FS_MG_INNER_RTOL, need));
+            }
             // restarted CG: at most 500 iterations per sweep
-            its += run_mgpcg32(FS_MG_INNER_RTOL, std::min(P.max_iteration, 500));
+            its += run_mgpcg32(inner, std::min(P.max_iteration, 500));
             fs_add_correction(st, L, p, e32);
             rr = true_residual(bb);
             sweeps++;
