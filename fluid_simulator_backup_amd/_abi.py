@@ -29,6 +29,10 @@ FS_SOURCE_ADD = 1
 FS_SOURCE_DELETE = 2
 FS_KS_REINIT = 0
 FS_KS_MG_SMOOTH = 1
+FS_LAYOUT_INTERIOR = 0
+FS_LAYOUT_GHOSTED = 1
+FS_MEM_HOST = 0
+FS_MEM_DEVICE = 1
 
 ERROR_NAMES = {
     FS_ERR_BAD_ARG: "FS_ERR_BAD_ARG",
@@ -206,6 +210,11 @@ def bind(lib, prefix):
         lib.fs_set_kernel_sampling.argtypes = [vp, _I]
         lib.fs_get_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(_I)]
         lib.fs_get_frame_kernel_samples.argtypes = [vp, _I, _PD, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
+        lib.fs_set_fields_ex.argtypes = [vp, vp, vp, vp, vp, vp, _I, _I]
+        lib.fs_get_fields_ex.argtypes = [vp, vp, vp, vp, vp, vp, _I, _I]
+        lib.fs_set_async.argtypes = [vp, _I]
+        lib.fs_synchronize.argtypes = [vp]
+        lib.fs_get_stream.argtypes = [vp, C.POINTER(vp)]
         lib.fs_last_error.argtypes = [vp]
         lib.fs_last_error.restype = C.c_char_p
         lib.fs_adjoint_projection.argtypes = [vp, _D, _PD, _PD, _PD, C.POINTER(FsStepStats)]

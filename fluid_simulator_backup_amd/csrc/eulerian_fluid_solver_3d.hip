@@ -86,6 +86,7 @@ class EulerianFluidSolver3D {
     int reinit_switch = -1, reinit_count = 0, last_switch = -1;
     bool timed_concurrent = false, switched = false;
     bool batch_member = false;  // set by fs_batch_create
+    bool async = false;         // fs_set_async: steps return without the final wait
     hipEvent_t ev[4] = {};
 
     // persistent fields (padded layout, FP64)
@@ -350,40 +351,71 @@ class EulerianFluidSolver3D {
     void check_launch() { HIPCHECK(hipGetLastError()); }
 
     // ---- field exchange ---------------------------------------------------
-    void put(double* dst, const double* host, int comp) {
+    // An x-fastest array over component comp's box grown by g ghost layers,
+    // in host memory (through the staging buffer) or device memory.
+    void put(double* dst, const double* src, int comp, int g = 0, bool device = false) {
         FsBox b = box(comp);
-        size_t n = (size_t)b.ni * b.nj * b.nk;
-        HIPCHECK(hipMemcpyAsync(staging, host, n * sizeof(double), hipMemcpyHostToDevice, st));
-        fs_scatter(st, dst, staging, L, b);
+        const double* lin = src;
+        if (!device) {
+            size_t n = (size_t)(b.ni + 2 * g) * (b.nj + 2 * g) * (b.nk + 2 * g);
+            EnsureStaging((long)n);
+            HIPCHECK(hipMemcpyAsync(staging, src, n * sizeof(double), hipMemcpyHostToDevice, st));
+            lin = staging;
+        }
+        fs_scatter(st, dst, lin, L, b, g);
         sync();
     }
-    void get(const double* src, double* host, int comp) {
+    void get(const double* src, double* out, int comp, int g = 0, bool device = false) {
         FsBox b = box(comp);
-        size_t n = (size_t)b.ni * b.nj * b.nk;
-        fs_gather(st, staging, src, L, b);
-        HIPCHECK(hipMemcpyAsync(host, staging, n * sizeof(double), hipMemcpyDeviceToHost, st));
+        if (device) {
+            fs_gather(st, out, src, L, b, g);
+            sync();
+            return;
+        }
+        size_t n = (size_t)(b.ni + 2 * g) * (b.nj + 2 * g) * (b.nk + 2 * g);
+        EnsureStaging((long)n);
+        fs_gather(st, staging, src, L, b, g);
+        HIPCHECK(hipMemcpyAsync(out, staging, n * sizeof(double), hipMemcpyDeviceToHost, st));
         sync();
     }
+    void EnsureStaging(long n) {
+        if (n <= staging_n) return;
+        sync();
+        HIPCHECK(hipFree(staging));
+        staging = nullptr;
+        staging = (double*)dalloc(sizeof(double) * (size_t)n);
+        staging_n = n;
+    }
+    // ghost layers of each field in the reference's arrays (fs_set_fields_ex
+    // layout FS_LAYOUT_GHOSTED): phi and the MAC velocities 2
+    // (EULERIAN_FLUID_SOLVER_3D.h:407-459), the pressure 1 (PROJECTION_3D.h:362)
+    static int ref_ghost(int field) { return field == 4 ? 1 : 2; }
 
-    void SetFields(const double* hphi, const double* hu, const double* hv, const double* hw, const double* hp) {
+    void SetFields(const double* hphi, const double* hu, const double* hv, const double* hw, const double* hp,
+                   int layout = FS_LAYOUT_INTERIOR, int memory = FS_MEM_HOST) {
+        const bool dev = memory == FS_MEM_DEVICE;
+        auto g = [&](int f) { return layout == FS_LAYOUT_GHOSTED ? ref_ghost(f) : 0; };
         if (hphi) {
-            put(phi, hphi, 0);
+            put(phi, hphi, 0, g(0), dev);
             // level-set ghosts by constant extrapolation, EULERIAN_FLUID_SOLVER_3D.h:757
             fs_fill_const(st, phi, phi, L, box(0), 2);
         }
-        if (hu) put(u, hu, 1);
-        if (hv) put(v, hv, 2);
-        if (hw) put(w, hw, 3);
-        if (hp) put(p, hp, 0);
+        if (hu) put(u, hu, 1, g(1), dev);
+        if (hv) put(v, hv, 2, g(2), dev);
+        if (hw) put(w, hw, 3, g(3), dev);
+        if (hp) put(p, hp, 0, g(4), dev);
         check_launch();
         sync();
     }
-    void GetFields(double* hphi, double* hu, double* hv, double* hw, double* hp) {
-        if (hphi) get(phi, hphi, 0);
-        if (hu) get(u, hu, 1);
-        if (hv) get(v, hv, 2);
-        if (hw) get(w, hw, 3);
-        if (hp) get(p, hp, 0);
+    void GetFields(double* hphi, double* hu, double* hv, double* hw, double* hp, int layout = FS_LAYOUT_INTERIOR,
+                   int memory = FS_MEM_HOST) {
+        const bool dev = memory == FS_MEM_DEVICE;
+        auto g = [&](int f) { return layout == FS_LAYOUT_GHOSTED ? ref_ghost(f) : 0; };
+        if (hphi) get(phi, hphi, 0, g(0), dev);
+        if (hu) get(u, hu, 1, g(1), dev);
+        if (hv) get(v, hv, 2, g(2), dev);
+        if (hw) get(w, hw, 3, g(3), dev);
+        if (hp) get(p, hp, 0, g(4), dev);
     }
 
     // ---- phases -----------------------------------------------------------
@@ -1868,6 +1900,48 @@ int fs_set_fields(fs_solver* s, const double* phi, const double* u, const double
     });
 }
 
+int fs_set_fields_ex(fs_solver* s, const double* phi, const double* u, const double* v, const double* w,
+                     const double* p, int layout, int memory) {
+    if ((layout != FS_LAYOUT_INTERIOR && layout != FS_LAYOUT_GHOSTED) || (memory != FS_MEM_HOST && memory != FS_MEM_DEVICE))
+        return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.SetFields(phi, u, v, w, p, layout, memory);
+        return FS_OK;
+    });
+}
+
+int fs_get_fields_ex(fs_solver* s, double* phi, double* u, double* v, double* w, double* p, int layout, int memory) {
+    if ((layout != FS_LAYOUT_INTERIOR && layout != FS_LAYOUT_GHOSTED) || (memory != FS_MEM_HOST && memory != FS_MEM_DEVICE))
+        return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.GetFields(phi, u, v, w, p, layout, memory);
+        return FS_OK;
+    });
+}
+
+int fs_set_async(fs_solver* s, int async) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.async = async != 0;
+        return FS_OK;
+    });
+}
+
+int fs_synchronize(fs_solver* s) {
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        S.sync();
+        S.check_launch();
+        return FS_OK;
+    });
+}
+
+int fs_get_stream(fs_solver* s, void** stream) {
+    if (!stream) return FS_ERR_BAD_ARG;
+    return guarded(s, [&](EulerianFluidSolver3D& S) {
+        *stream = (void*)S.st;
+        return FS_OK;
+    });
+}
+
 int fs_get_fields(fs_solver* s, double* phi, double* u, double* v, double* w, double* p) {
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.GetFields(phi, u, v, w, p);
@@ -1882,10 +1956,12 @@ static int step_impl(EulerianFluidSolver3D& S, double dt, fs_step_stats* stats, 
     if (frame) S.ComputeCurvatures();
     int rc = S.AdvanceOneTimeStepThread(dt);
     HIPCHECK(hipEventRecord(S.ev[1], S.st));
-    S.sync();
-    float ms = 0;
-    HIPCHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
-    S.stats.step_ms = ms;
+    if (!S.async) {
+        S.sync();
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+        S.stats.step_ms = ms;
+    }
     if (stats) *stats = S.stats;
     return rc;
 }

@@ -116,6 +116,8 @@ void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const dou
 void fs_average(hipStream_t st, double* phi, const double* tmp, const FsLayout& L);  // phi = 0.5*(tmp+phi)
 // dst = ca*a + cb*dst on box b
 void fs_combine(hipStream_t st, double* dst, const double* a, double ca, double cb, const FsLayout& L, FsBox b);
-void fs_scatter(hipStream_t st, double* dst, const double* lin, const FsLayout& L, FsBox b);
-void fs_gather(hipStream_t st, double* lin, const double* src, const FsLayout& L, FsBox b);
+// box interior <- / -> an x-fastest array over the box grown by g ghost
+// layers (g <= FS_G; gather includes the layers, scatter skips them)
+void fs_scatter(hipStream_t st, double* dst, const double* lin, const FsLayout& L, FsBox b, int g = 0);
+void fs_gather(hipStream_t st, double* lin, const double* src, const FsLayout& L, FsBox b, int g = 0);
 void fs_fill_value(hipStream_t st, double* dst, long n, double v);

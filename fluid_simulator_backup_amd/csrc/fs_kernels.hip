@@ -1313,22 +1313,28 @@ void fs_combine(hipStream_t st, double* dst, const double* a, double ca, double 
     hipLaunchKernelGGL(k_combine, grid_for(b), dim3(BX, BY), 0, st, dst, a, ca, cb, L, b);
 }
 
-__global__ void k_scatter(double* __restrict__ dst, const double* __restrict__ lin, FsLayout L, FsBox b) {
+// lin is x-fastest over the box grown by g layers on every side (g = 0: the
+// box itself); scatter writes the box's interior from it, gather reads the
+// grown box (ghost layers included) into it
+__global__ void k_scatter(double* __restrict__ dst, const double* __restrict__ lin, FsLayout L, FsBox b, int g) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= b.ni || j >= b.nj) return;
-    dst[L.idx(i, j, k)] = lin[i + (long)b.ni * (j + (long)b.nj * k)];
+    const long gi = b.ni + 2 * g, gj = b.nj + 2 * g;
+    dst[L.idx(i, j, k)] = lin[(i + g) + gi * ((j + g) + gj * (k + g))];
 }
-__global__ void k_gather(double* __restrict__ lin, const double* __restrict__ src, FsLayout L, FsBox b) {
+__global__ void k_gather(double* __restrict__ lin, const double* __restrict__ src, FsLayout L, FsBox b, int g) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
-    if (i >= b.ni || j >= b.nj) return;
-    lin[i + (long)b.ni * (j + (long)b.nj * k)] = src[L.idx(i, j, k)];
+    const int gi = b.ni + 2 * g, gj = b.nj + 2 * g;
+    if (i >= gi || j >= gj) return;
+    lin[i + (long)gi * (j + (long)gj * k)] = src[L.idx(i - g, j - g, k - g)];
 }
 
-void fs_scatter(hipStream_t st, double* dst, const double* lin, const FsLayout& L, FsBox b) {
-    hipLaunchKernelGGL(k_scatter, grid_for(b), dim3(BX, BY), 0, st, dst, lin, L, b);
+void fs_scatter(hipStream_t st, double* dst, const double* lin, const FsLayout& L, FsBox b, int g) {
+    hipLaunchKernelGGL(k_scatter, grid_for(b), dim3(BX, BY), 0, st, dst, lin, L, b, g);
 }
-void fs_gather(hipStream_t st, double* lin, const double* src, const FsLayout& L, FsBox b) {
-    hipLaunchKernelGGL(k_gather, grid_for(b), dim3(BX, BY), 0, st, lin, src, L, b);
+void fs_gather(hipStream_t st, double* lin, const double* src, const FsLayout& L, FsBox b, int g) {
+    FsBox gb{b.ni + 2 * g, b.nj + 2 * g, b.nk + 2 * g};
+    hipLaunchKernelGGL(k_gather, grid_for(gb), dim3(BX, BY), 0, st, lin, src, L, b, g);
 }
 
 __global__ void k_fill_value(double* dst, long n, double v) {

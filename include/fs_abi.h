@@ -154,6 +154,37 @@ int  fs_set_fields(fs_solver* s, const double* phi, const double* u,
 /* Copy fields out (any pointer may be NULL). */
 int  fs_get_fields(fs_solver* s, double* phi, double* u, double* v, double* w, double* p);
 
+/* fs_set_fields / fs_get_fields with a choice of layout and memory.
+ * FS_LAYOUT_INTERIOR is the interior-only layout above.  FS_LAYOUT_GHOSTED
+ * is the reference's own arrays, ARRAY_3D over the box grown by its ghost
+ * layers, x fastest: 2 for phi and u, v, w (EULERIAN_FLUID_SOLVER_3D.h:
+ * 407-459), 1 for p (PROJECTION_3D.h:362); on input the ghost values are
+ * ignored (phi's are extrapolated as for fs_set_fields), on output they are
+ * what the solver's ghost cells hold.  FS_MEM_DEVICE: the pointers are
+ * device memory of the handle's GPU (a caller that keeps its state on the
+ * GPU), copied with kernels on the handle's stream; FS_MEM_HOST: host
+ * memory, as fs_set_fields. */
+#define FS_LAYOUT_INTERIOR 0
+#define FS_LAYOUT_GHOSTED  1
+#define FS_MEM_HOST        0
+#define FS_MEM_DEVICE      1
+int  fs_set_fields_ex(fs_solver* s, const double* phi, const double* u, const double* v, const double* w,
+                      const double* p, int layout, int memory);
+int  fs_get_fields_ex(fs_solver* s, double* phi, double* u, double* v, double* w, double* p, int layout,
+                      int memory);
+
+/* Asynchronous steps (no counterpart in the reference, whose steps block
+ * on their thread barriers).  With async set, fs_step and fs_advance_frame
+ * return once the step's last host decision is taken (the projection's stop
+ * tests still wait for the device), leaving the remaining kernels -- the
+ * reinitialization, the last blends -- queued on the handle's stream, and
+ * report step_ms as 0.  fs_synchronize waits for the stream; fs_get_stream
+ * gives it (a hipStream_t) so a caller can order its own work after the
+ * step.  Every other call still waits before it returns. */
+int  fs_set_async(fs_solver* s, int async);
+int  fs_synchronize(fs_solver* s);
+int  fs_get_stream(fs_solver* s, void** stream);
+
 /* ---- files ------------------------------------------------------------------
  * fs_write_levelset / fs_read_levelset: LEVELSET_3D::Write / Read
  * (LEVELSET_3D.h:1132-1200), the reference's one binary field format.  A
