@@ -21,6 +21,7 @@
 
 #include "../../include/fs_abi.h"
 #include "fs_adjoint.h"
+#include "fs_batch.h"
 #include "fs_kernels.h"
 #include "fs_layout.h"
 #include "fs_mg.h"
@@ -1529,16 +1530,7 @@ class EulerianFluidSolver3D {
             tape.w = Record(w, 3);
             tape.p = Record(p, 0);
         }
-        if (P.is_levelset_advection_active) {
-            if (order == 3) {
-                Levelset_Advection(dt);  // :1010-1013, one Euler stage
-            } else {
-                fs_copy_box(st, tmp, phi, L, box(0));  // :810-832, :888-911
-                Levelset_Advection(dt);
-                Levelset_Advection(dt);
-                fs_average(st, phi, tmp, L);
-            }
-        }
+        LevelsetStage(dt);
         // The reinitialization reads only the advected level set (and, in the
         // compatible schedule, its ghost layers).  SolveForRK changes neither
         // the box nor, beyond the constant fill replayed below, the ghosts,
@@ -1565,8 +1557,7 @@ class EulerianFluidSolver3D {
             // the ghost layers SolveForRK will leave on phi: the constant
             // fill of ComputeNormals (assembly) and of the fixed schedule's
             // viscosity, a function of the box, which SolveForRK leaves as is
-            if (P.is_projection_active || (fixed() && P.is_sourcing_active && P.use_delta_function_formulation))
-                fs_fill_const(st, phi, phi, L, box(0), 2);
+            if (ReplaysConstFill()) fs_fill_const(st, phi, phi, L, box(0), 2);
             reinit_switch = at < sweeps ? at : -1;
             reinit_count = 0;
             switched = false;
@@ -1579,20 +1570,7 @@ class EulerianFluidSolver3D {
         }
         try {
             if (concurrent) HIPCHECK(hipEventRecord(ev_t[2], st));
-            if (order == 1) {
-                SolveForRK(dt, rc);  // :834
-            } else {
-                SaveVelocity();  // :913-932, :1015-1034
-                SolveForRK(dt, rc);
-                SolveForRK(dt, rc);
-                if (order == 2) {
-                    BlendVelocity((double)0.5, (double)0.5);  // :937-953
-                } else {
-                    BlendVelocity((double)0.75, (double)0.25);  // :1039-1055
-                    SolveForRK(dt, rc);
-                    BlendVelocity((double)1 / (double)3, (double)2 / (double)3);  // :1059-1075
-                }
-            }
+            VelocityStages(dt, rc);
             if (concurrent) {
                 HIPCHECK(hipEventRecord(ev_t[3], st));
                 timed_concurrent = true;
@@ -1615,6 +1593,69 @@ class EulerianFluidSolver3D {
         }
         check_launch();
         return rc;
+    }
+
+    // the level-set stages: RK2 (:810-832, :888-911) or, at order 3, one
+    // Euler stage (:1010-1013)
+    void LevelsetStage(double dt) {
+        if (!P.is_levelset_advection_active) return;
+        if (P.order_for_time_advancing == 3) {
+            Levelset_Advection(dt);
+        } else {
+            fs_copy_box(st, tmp, phi, L, box(0));
+            Levelset_Advection(dt);
+            Levelset_Advection(dt);
+            fs_average(st, phi, tmp, L);
+        }
+    }
+
+    // SolveForRK and the velocity blends of the step's order
+    void VelocityStages(double dt, int& rc) {
+        const int order = P.order_for_time_advancing;
+        if (order == 1) {
+            SolveForRK(dt, rc);  // :834
+            return;
+        }
+        SaveVelocity();  // :913-932, :1015-1034
+        SolveForRK(dt, rc);
+        SolveForRK(dt, rc);
+        if (order == 2) {
+            BlendVelocity((double)0.5, (double)0.5);  // :937-953
+        } else {
+            BlendVelocity((double)0.75, (double)0.25);  // :1039-1055
+            SolveForRK(dt, rc);
+            BlendVelocity((double)1 / (double)3, (double)2 / (double)3);  // :1059-1075
+        }
+    }
+
+    // whether this scene's step may take its reinitialization from a batch's
+    // scene-table stage: the conditions of the concurrent order, and the
+    // constant fill the stage replays
+    bool BatchReinitEligible() const {
+        return P.sussmanstyle_reinitialization && !tension() && sources.empty() && !recording &&
+               (P.iteration_number_for_reinitialization & 1) == 1;
+    }
+    bool ReplaysConstFill() const {
+        return P.is_projection_active || (fixed() && P.is_sourcing_active && P.use_delta_function_formulation);
+    }
+    // the scene's entry of a batch's scene table (buffers 0/1 phi and its
+    // copy, in the order phi, phi_r at the time of the call)
+    FsReinitScene ReinitSceneEntry() {
+        if (!phi_r) phi_r = dfield();
+        FsReinitScene e{};
+        e.L = L;
+        e.ix = cell.x.inv;
+        e.iy = cell.y.inv;
+        e.iz = cell.z.inv;
+        e.eps = P.levelset_epsilon;
+        e.dt = cell.x.d / P.scaling_number_for_reinitialization;
+        e.dx2 = cell.x.d * cell.x.d;
+        e.exact = fixed() ? 1 : 0;
+        e.fill_const = ReplaysConstFill() ? 1 : 0;
+        e.fast = fast() ? 1 : 0;
+        double* b[6] = {phi, phi_r, tmp, sgn, sghost, sghost2};
+        for (int q = 0; q < 6; q++) e.buf[q] = b[q];
+        return e;
     }
 
     // reinitialization, RK2 (:836-858, :958-980) or RK3 (:1080-1109)
@@ -1974,7 +2015,152 @@ struct fs_batch {
     int device = 0;
     int threads = 1;
     std::string err;
+    // the scene-table reinitialization (fs_batch.h): built at the first frame
+    // when every scene qualifies, run on its own stream
+    FsBatchReinit table;
+    int table_state = 0;  // 0 not tried, 1 built, -1 scenes do not qualify
+    hipStream_t bst = nullptr;
+    hipEvent_t ev_brn = nullptr;
+    std::vector<hipEvent_t> ev_ls;
+    std::vector<double*> buf0, buf1;  // each scene's table buffers 0 and 1 (its phi and phi_r at the build)
+    ~fs_batch() {
+        if (bst) (void)hipStreamSynchronize(bst);
+        for (hipEvent_t e : ev_ls)
+            if (e) (void)hipEventDestroy(e);
+        if (ev_brn) (void)hipEventDestroy(ev_brn);
+        if (bst) (void)hipStreamDestroy(bst);
+    }
 };
+
+// Whether the batch's scenes take their reinitialization from the scene
+// table, building it on first use: FS_BATCH_TABLE is set, and every scene
+// qualifies (the concurrent order's conditions, an odd sweep count) with the
+// same sweep count and calls per stage.  Opt-in: on 48 uneven scenes of
+// 12^3..64^3 the table stage (193 ms/frame) trails the per-scene stages on
+// their own streams (182 ms/frame), whose reinitialization overlaps the
+// other scenes' projections where the table has to wait for every scene's
+// advected level set.
+static bool batch_table(fs_batch* b) {
+    if (!std::getenv("FS_BATCH_TABLE")) return false;
+    if (b->table_state != 0) {
+        if (b->table_state < 0) return false;
+        // sources added since, or a scene's phi outside the two table buffers
+        for (size_t i = 0; i < b->scenes.size(); i++) {
+            EulerianFluidSolver3D& S = *b->scenes[i]->impl;
+            if (!S.BatchReinitEligible()) return false;
+        }
+        return true;
+    }
+    b->table_state = -1;
+    const EulerianFluidSolver3D& S0 = *b->scenes[0]->impl;
+    const int sweeps = S0.P.iteration_number_for_reinitialization, calls = S0.P.order_for_time_advancing == 3 ? 3 : 2;
+    for (fs_solver* s : b->scenes) {
+        const EulerianFluidSolver3D& S = *s->impl;
+        if (!S.BatchReinitEligible() || S.P.iteration_number_for_reinitialization != sweeps ||
+            (S.P.order_for_time_advancing == 3 ? 3 : 2) != calls)
+            return false;
+    }
+    std::vector<FsReinitScene> entries;
+    for (fs_solver* s : b->scenes) {
+        EulerianFluidSolver3D& S = *s->impl;
+        entries.push_back(S.ReinitSceneEntry());
+        b->buf0.push_back(S.phi);
+        b->buf1.push_back(S.phi_r);
+        hipEvent_t e = nullptr;
+        HIPCHECK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+        b->ev_ls.push_back(e);
+        S.sync();
+    }
+    HIPCHECK(hipEventCreateWithFlags(&b->ev_brn, hipEventDisableTiming));
+    HIPCHECK(hipStreamCreate(&b->bst));
+    b->table.Build(entries, sweeps, calls);
+    b->table_state = 1;
+    return true;
+}
+
+// A frame of every scene with the reinitialization from the scene table:
+// each scene's level-set stages on its stream; the table's stage for all
+// scenes on the batch stream once every scene's advected level set is there
+// (it works on the copies); each scene's SolveForRK on its stream beside it;
+// then each scene takes its reinitialized copy as phi.
+static int batch_frame_table(fs_batch* b, const double* dt, fs_step_stats* stats, std::vector<int>& rc) {
+    const int n = (int)b->scenes.size();
+    int parity = -1;
+    for (int i = 0; i < n; i++) {
+        const EulerianFluidSolver3D& S = *b->scenes[i]->impl;
+        const int q = S.phi == b->buf0[i] && S.phi_r == b->buf1[i] ? 0 : (S.phi == b->buf1[i] && S.phi_r == b->buf0[i] ? 1 : -1);
+        if (q < 0 || (parity >= 0 && q != parity)) {
+            b->err = "batched reinitialization: scene " + std::to_string(i) + " is out of step with the table";
+            return FS_ERR_UNSUPPORTED;
+        }
+        parity = q;
+    }
+    std::atomic<int> next{0};
+    auto pool = [&](auto&& body) {
+        next = 0;
+        auto work = [&]() {
+            if (hipSetDevice(b->device) != hipSuccess) return;
+            for (int q = next++; q < n; q = next++) {
+                int i = (int)b->order[q];
+                if (rc[i] != FS_OK) continue;
+                rc[i] = guarded(b->scenes[i], [&](EulerianFluidSolver3D& S) { return body(S, i); });
+            }
+        };
+        std::vector<std::thread> th;
+        for (int t = 1; t < b->threads; t++) th.emplace_back(work);
+        work();
+        for (auto& t : th) t.join();
+    };
+    // 1. level-set stages
+    pool([&](EulerianFluidSolver3D& S, int i) {
+        if (!(dt[i] > 0) || !std::isfinite(dt[i])) throw FsFail{FS_ERR_BAD_ARG, "dt must be positive and finite"};
+        S.stats = fs_step_stats{};
+        HIPCHECK(hipEventRecord(S.ev[0], S.st));
+        S.ComputeCurvatures();
+        S.LevelsetStage(dt[i]);
+        HIPCHECK(hipEventRecord(b->ev_ls[i], S.st));
+        return FS_OK;
+    });
+    for (int i = 0; i < n; i++)
+        if (rc[i] != FS_OK) {
+            for (fs_solver* s : b->scenes) (void)hipStreamSynchronize(s->impl->st);
+            return rc[i];
+        }
+    // 2. the reinitialization of every scene's copy
+    try {
+        for (int i = 0; i < n; i++) HIPCHECK(hipStreamWaitEvent(b->bst, b->ev_ls[i], 0));
+        b->table.Run(b->bst, parity);
+        HIPCHECK(hipEventRecord(b->ev_brn, b->bst));
+    } catch (const std::exception& e) {
+        b->err = std::string("batched reinitialization: ") + e.what();
+        return FS_ERR_HIP;
+    } catch (const HipError& e) {
+        b->err = std::string("batched reinitialization: ") + hipGetErrorString(e.e);
+        return FS_ERR_HIP;
+    }
+    // 3. SolveForRK beside it, then phi takes the copy
+    pool([&](EulerianFluidSolver3D& S, int i) {
+        int r = FS_OK;
+        try {
+            S.VelocityStages(dt[i], r);
+        } catch (...) {
+            (void)hipStreamWaitEvent(S.st, b->ev_brn, 0);
+            std::swap(S.phi, S.phi_r);
+            throw;
+        }
+        HIPCHECK(hipStreamWaitEvent(S.st, b->ev_brn, 0));
+        std::swap(S.phi, S.phi_r);
+        S.check_launch();
+        HIPCHECK(hipEventRecord(S.ev[1], S.st));
+        S.sync();
+        float ms = 0;
+        HIPCHECK(hipEventElapsedTime(&ms, S.ev[0], S.ev[1]));
+        S.stats.step_ms = ms;
+        if (stats) stats[i] = S.stats;
+        return r;
+    });
+    return FS_OK;
+}
 
 int fs_batch_create(int n, const fs_params* params, const fs_grid* grids, int device, int threads, fs_batch** out) {
     if (n <= 0 || !params || !grids || !out) return FS_ERR_BAD_ARG;
@@ -2022,6 +2208,37 @@ int fs_batch_advance_frame(fs_batch* b, const double* dt, fs_step_stats* stats, 
     if (!b || !dt) return FS_ERR_BAD_ARG;
     const int n = (int)b->scenes.size();
     std::vector<int> rc(n, FS_OK);
+    bool table = false;
+    try {
+        table = batch_table(b);
+    } catch (const HipError& e) {
+        b->err = std::string("batch table: ") + hipGetErrorString(e.e);
+        return FS_ERR_HIP;
+    } catch (const FsFail& f) {
+        b->err = "batch table: " + f.msg;
+        return f.code;
+    } catch (const std::exception& e) {
+        b->err = std::string("batch table: ") + e.what();
+        return FS_ERR_HIP;
+    }
+    if (table) {
+        int r = batch_frame_table(b, dt, stats, rc);
+        if (r != FS_OK) {
+            if (b->err.empty()) b->err = "batched frame failed";
+            for (int i = 0; i < n; i++)
+                if (rcs) rcs[i] = rc[i];
+            return r;
+        }
+        int first = FS_OK;
+        for (int i = 0; i < n; i++) {
+            if (rcs) rcs[i] = rc[i];
+            if (rc[i] != FS_OK && first == FS_OK) {
+                first = rc[i];
+                b->err = "scene " + std::to_string(i) + ": " + b->scenes[i]->err;
+            }
+        }
+        return first;
+    }
     std::atomic<int> next{0};
     auto work = [&]() {
         if (hipSetDevice(b->device) != hipSuccess) return;

@@ -2,16 +2,9 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include "fs_bodies.h"  // FsBox, fs_box: box extents of a component (0 cell, 1..3 x-, y-, z-faces)
 #include "fs_device.h"
 #include "fs_layout.h"
-
-// Box extents of a component: 0 cell, 1 x-faces, 2 y-faces, 3 z-faces.
-struct FsBox {
-    int ni, nj, nk;
-};
-inline FsBox fs_box(const FsLayout& L, int comp) {
-    return FsBox{L.nx + (comp == 1), L.ny + (comp == 2), L.nz + (comp == 3)};
-}
 
 // ---- ghost fills (FIELD_STRUCTURE_3D.h) ---------------------------------
 void fs_copy_box(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b);

@@ -55,23 +55,13 @@ void fs_fill_zero_faces(hipStream_t st, double* dst, const FsLayout& L, FsBox b,
     hipLaunchKernelGGL(k_zero_faces, dim3((b.ni + 63) / 64, b.nj), dim3(64), 0, st, dst, L, b, depth, 2);
 }
 
-__device__ __forceinline__ int clampi(int v, int lo, int hi) { return v < lo ? lo : (v > hi ? hi : v); }
+__device__ __forceinline__ int clampi(int v, int lo, int hi) { return fs_clampi(v, lo, hi); }
 
-// FillGhostCellsFrom, FIELD_STRUCTURE_3D.h:663-1086: every ghost cell of the
-// depth-d shell takes its clamped interior value (reads interior only).
+// FillGhostCellsFrom (fs_fill_const_row): one thread per row of the grown box.
 __global__ void k_fill_const(double* dst, const double* src, FsLayout L, FsBox b, int d) {
     int jj = blockIdx.x * blockDim.x + threadIdx.x - d, kk = blockIdx.y - d;
     if (jj >= b.nj + d) return;
-    bool row_inside = jj >= 0 && jj < b.nj && kk >= 0 && kk < b.nk;
-    int jc = clampi(jj, 0, b.nj - 1), kc = clampi(kk, 0, b.nk - 1);
-    if (row_inside) {
-        for (int g = 1; g <= d; g++) {
-            dst[L.idx(-g, jj, kk)] = src[L.idx(0, jc, kc)];
-            dst[L.idx(b.ni - 1 + g, jj, kk)] = src[L.idx(b.ni - 1, jc, kc)];
-        }
-    } else {
-        for (int ii = -d; ii < b.ni + d; ii++) dst[L.idx(ii, jj, kk)] = src[L.idx(clampi(ii, 0, b.ni - 1), jc, kc)];
-    }
+    fs_fill_const_row(dst, src, L, b, d, jj, kk);
 }
 
 void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayout& L, FsBox b, int depth) {
@@ -79,32 +69,11 @@ void fs_fill_const(hipStream_t st, double* dst, const double* src, const FsLayou
                        L, b, depth);
 }
 
-// FillGhostCellsContinuousDerivativesFrom face blocks,
-// FIELD_STRUCTURE_3D.h:1806-1894: 2a-b outwards, deeper layers reading the
-// layers just written when src == dst (one thread walks its own row).
+// FillGhostCellsContinuousDerivativesFrom face blocks (fs_linear_faces_row).
 // The three axes in one launch (blockIdx.z = axis): each writes only its own
 // face layers and reads only its own row, so they are independent.
 __global__ void k_linear_faces(double* dst, const double* src, const double* gsrc, FsLayout L, FsBox b, int d) {
-    // S(x): src inside the box along the axis, gsrc in its ghost layers
-    const int axis = blockIdx.z;
-    int a = blockIdx.x * blockDim.x + threadIdx.x, c = blockIdx.y;
-    if (c >= (axis == 2 ? b.nj : b.nk)) return;
-    if (axis == 0) {
-        if (a >= b.nj) return;
-        auto S = [&](int i) { return (i >= 0 && i < b.ni ? src : gsrc)[L.idx(i, a, c)]; };
-        for (int i = -1; i >= -d; i--) dst[L.idx(i, a, c)] = (double)2 * S(i + 1) - S(i + 2);
-        for (int i = b.ni; i <= b.ni - 1 + d; i++) dst[L.idx(i, a, c)] = (double)2 * S(i - 1) - S(i - 2);
-    } else if (axis == 1) {
-        if (a >= b.ni) return;
-        auto S = [&](int j) { return (j >= 0 && j < b.nj ? src : gsrc)[L.idx(a, j, c)]; };
-        for (int j = -1; j >= -d; j--) dst[L.idx(a, j, c)] = (double)2 * S(j + 1) - S(j + 2);
-        for (int j = b.nj; j <= b.nj - 1 + d; j++) dst[L.idx(a, j, c)] = (double)2 * S(j - 1) - S(j - 2);
-    } else {
-        if (a >= b.ni) return;
-        auto S = [&](int k) { return (k >= 0 && k < b.nk ? src : gsrc)[L.idx(a, c, k)]; };
-        for (int k = -1; k >= -d; k--) dst[L.idx(a, c, k)] = (double)2 * S(k + 1) - S(k + 2);
-        for (int k = b.nk; k <= b.nk - 1 + d; k++) dst[L.idx(a, c, k)] = (double)2 * S(k - 1) - S(k - 2);
-    }
+    fs_linear_faces_row(dst, src, gsrc, L, b, d, blockIdx.z, blockIdx.x * blockDim.x + threadIdx.x, blockIdx.y);
 }
 
 // The three filled edge blocks normal to z (FIELD_STRUCTURE_3D.h:1896-1946),
@@ -152,14 +121,9 @@ __device__ __forceinline__ double upwind(double rho, double dt, double uu, doubl
     return rho - dt * (uu * X + vv * Y + ww * Z);
 }
 
-// WENO5 one-sided derivatives in the reference's operation order (FAST =
-// false, bit for bit) or in fs_weno5_fast's fewer operations
 template <bool FAST>
 __device__ __forceinline__ void weno5(const double* c, double inv, double eps, double& dmo, double& dpo) {
-    if constexpr (FAST)
-        fs_weno5_fast(c, inv, eps, dmo, dpo);
-    else
-        fs_weno5(c, inv, eps, dmo, dpo);
+    fs_weno5_of<FAST>(c, inv, eps, dmo, dpo);
 }
 
 template <bool FAST>
@@ -1222,52 +1186,21 @@ void fs_abs_max(hipStream_t st, const double* a, const FsLayout& L, FsBox b, uns
 __global__ void k_sign(const double* __restrict__ phi, double* __restrict__ sgn, FsLayout L, double dx2) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
-    long c = L.idx(i, j, k);
-    double f = phi[c];
-    sgn[c] = f / sqrt(f * f + dx2);
+    fs_sign_cell(phi, sgn, L.idx(i, j, k), dx2);
 }
 
 void fs_sign(hipStream_t st, const double* phi, double* sgn, const FsLayout& L, double dx2) {
     hipLaunchKernelGGL(k_sign, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, sgn, L, dx2);
 }
 
-// WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637: WENO5 one-sided
-// derivatives of the ghost copy, Godunov case per axis, the misplaced
-// parenthesis of :1463/:1605 (x shock, y shock towards +, z expansion; the
-// Euclidean norm instead when `exact` is set), and the sign-preserving clamp
-// (:1624-1634).
+// WENO5thReinitialization, ADVECTION_METHOD_3D.h:745-1637 (fs_reinit_cell).
 template <bool FAST>
 __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ out, const double* __restrict__ gh,
                                                    const double* __restrict__ sgn, FsLayout L, double ix, double iy,
                                                    double iz, double eps, double dt, int exact) {
     int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
     if (i >= L.nx || j >= L.ny) return;
-    long c = L.idx(i, j, k);
-    double f[7], mx, px, my, py, mz, pz, X, Y, Z;
-    gather7(gh, c, 1, f);
-    weno5<FAST>(f + 3, ix, eps, mx, px);
-    gather7(gh, c, L.sx, f);
-    weno5<FAST>(f + 3, iy, eps, my, py);
-    gather7(gh, c, L.sy, f);
-    weno5<FAST>(f + 3, iz, eps, mz, pz);
-    double sg = sgn[c];
-    double rho = gh[c];  // the ghost copy's box is phi
-    int cx = fs_godunov_axis(sg, mx, px, X);
-    int cy = cx ? fs_godunov_axis(sg, my, py, Y) : FS_AX_NONE;
-    int cz = cy ? fs_godunov_axis(sg, mz, pz, Z) : FS_AX_NONE;
-    if (cx && cy && cz) {
-        double g;
-        if (!exact && (cx == FS_AX_SM || cx == FS_AX_SP) && cy == FS_AX_SP && cz == FS_AX_DROP)
-            g = sqrt(X * X) + Y * Y - 1;
-        else
-            g = sqrt(X * X + Y * Y + Z * Z) - 1;
-        rho = rho - dt * sg * g;
-    }
-    if (sg * rho < (double)0) {
-        if (sg < 0) rho = -eps;
-        if (sg > 0) rho = eps;
-    }
-    out[c] = rho;
+    fs_reinit_cell<FAST>(out, gh, sgn, L, ix, iy, iz, eps, dt, exact, L.idx(i, j, k));
 }
 
 void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const double* sgn, const FsLayout& L,

@@ -486,15 +486,36 @@ def test_frames_fp64_reference_stop_rule(hip_lib, oracle_lib, name, n, frames):
         assert e < 1e-4, (c, e)
 
 
-@pytest.mark.parametrize("precision", [_abi.FS_FP32_REFINE, _abi.FS_FP32_MG])
-def test_batch_matches_scenes_advanced_alone(hip_lib, precision):
+@pytest.mark.parametrize("precision,mode,mixed", [(_abi.FS_FP32_REFINE, "graph", False),
+                                                   (_abi.FS_FP32_MG, "graph", False),
+                                                   (_abi.FS_FP32_MG, "graph", True),
+                                                   (_abi.FS_FP32_MG, "eager", True),
+                                                   (_abi.FS_FP32_MG, "per-scene", False),
+                                                   (_abi.FS_FP32_MG, "per-scene", True)])
+def test_batch_matches_scenes_advanced_alone(hip_lib, monkeypatch, precision, mode, mixed):
     """A batch of uneven scenes (FP32 projection inside FP64 refinement, one
-    host thread per scene stream) gives, bit for bit, what each scene gives
-    advanced alone: every kernel and reduction is deterministic and no state
-    is shared."""
+    host thread per scene stream) gives, bit for bit and after every frame,
+    what each scene gives advanced alone: every kernel and reduction is
+    deterministic and no state is shared.  The batch's reinitialization is
+    each scene's own stage on its stream ("per-scene", the default), or the
+    opt-in scene-table stage (FS_BATCH_TABLE: one launch per kernel for all
+    scenes) replayed from its two captured graphs frame after frame
+    ("graph") or the same launches without the graphs ("eager"); `mixed` gives every second scene the fixed schedule and
+    the fast WENO5 arithmetic."""
     from fluid_simulator_backup_amd import scenes
+    monkeypatch.delenv("FS_BATCH_NO_GRAPH", raising=False)
+    monkeypatch.delenv("FS_BATCH_TABLE", raising=False)
+    if mode != "per-scene":
+        monkeypatch.setenv("FS_BATCH_TABLE", "1")
+    if mode == "eager":
+        monkeypatch.setenv("FS_BATCH_NO_GRAPH", "1")
     sc = scenes.batch_scenes(6, seed=5, edges=(12, 16, 20, 24), precision=precision,
                              stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8, max_iteration=5000)
+    if mixed:
+        for q, s in enumerate(sc):
+            if q % 2:
+                s[3].schedule = _abi.FS_SCHEDULE_FIXED
+                s[3].fast_weno = 1
     batch = _abi.Batch(hip_lib, [s[3] for s in sc], [s[4] for s in sc], 0, 4)
     alone = []
     for (name, phi, vel, P, g), bs in zip(sc, batch.scenes):
@@ -502,15 +523,14 @@ def test_batch_matches_scenes_advanced_alone(hip_lib, precision):
         s = _abi.Solver(hip_lib, "fs_", P, g, 0)
         s.set_fields(phi, *vel)
         alone.append(s)
-    dts = [s.cfl_dt(0.5, 1) for s in alone]
-    for _ in range(2):
+    for f in range(4):
+        dts = [s.cfl_dt(0.5, 1) for s in alone]
         st_b = batch.advance_frame(dts)
         st_a = [s.advance_frame(dt) for s, dt in zip(alone, dts)]
         for a, b in zip(st_a, st_b):
             assert a["pcg_iterations"] == b["pcg_iterations"]
-    for s, bs in zip(alone, batch.scenes):
-        fa, fb = s.get_fields(), bs.get_fields()
-        for c in ("phi", "u", "v", "w", "p"):
-            assert np.array_equal(fa[c], fb[c]), c
-        s.close()
+        for q, (s, bs) in enumerate(zip(alone, batch.scenes)):
+            fa, fb = s.get_fields(), bs.get_fields()
+            for c in ("phi", "u", "v", "w", "p"):
+                assert np.array_equal(fa[c], fb[c]), (f, q, c)
     batch.close()
