@@ -56,6 +56,9 @@ struct FsPcgWork {
     void *r, *s, *p, *q;    // padded-layout vectors (ghosts stay zero)
     double* partials;       // FS_PCG_SLOTS * FS_PCG_MAX_BLOCKS per-workgroup partial sums
     void* p2;
+    // tiling of the fused SpMV (0) and the update (1): z-planes per tile and
+    // the cap on workgroups; 0 takes the fixed rule (fs_pcg_tune sets them)
+    int zc[2], nbmax[2];
 };
 
 // Kernels per PCG iteration: the fused p-update + SpMV + p.q, and the
@@ -122,3 +125,15 @@ void fs_make_op32(hipStream_t st, const FsLayout& L, const double* bx, const dou
 // update) as iteration n, for timing.
 void fs_pcg_time_kernels(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, void* x,
                          FsPcgWork& w, FsPcgState* dstate, int precision, int which, int n);
+
+// Tiling of the two iteration kernels for the workspace's shape (GFM
+// operator): on the first call for a (grid, precision) in the process, each
+// kernel runs every candidate of z-planes per tile {8, 16, 32, 64} and
+// workgroup cap {256, 512, 1024, 2048} on the solve's own buffers, with a
+// scratch solution and a state that never stops, and the fastest of each is
+// kept; later calls (any handle of that shape) reuse the choice.  The values
+// of a solve then depend on that choice through the order of its partial
+// sums (deterministic within the process).  FS_PCG_TUNE=0 keeps the fixed
+// rule; FS_PCG_TILING="zc0,nb0,zc1,nb1" sets one.  Call before begin: it leaves r, s, p, q and the partials undefined.
+void fs_pcg_tune(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, FsPcgWork& w,
+                 FsPcgState* dstate, int precision);

@@ -31,7 +31,12 @@
 // accumulate in FP64 for both precisions.
 #include <hip/hip_ext.h>
 
+#include <algorithm>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 
 #include "fs_device.h"
 #include "fs_pcg.h"
@@ -148,22 +153,36 @@ struct Tiling {
     int ntx, nty, ntz, ntiles, nb, v;
 };
 
-static Tiling make_tiling(const FsLayout& L, int v, int min_tiles = FS_MIN_TILES) {
+// zc > 0 forces the z-planes per tile and nbmax > 0 caps the workgroups (the
+// tuned tiling); otherwise the fixed rule below.
+static Tiling make_tiling(const FsLayout& L, int v, int min_tiles = FS_MIN_TILES, int zc = 0, int nbmax = 0) {
     Tiling t;
     t.v = v;
     t.ntx = (L.nx + TBX * v - 1) / (TBX * v);
     t.nty = (L.ny + TBY - 1) / TBY;
-    // the deepest z-chunk that still leaves at least min_tiles tiles: a
-    // longer z-march re-reads fewer halo planes (measured at 256^3: 32 planes
-    // beat 16 and 8 for the plain kernels), enough tiles keep every CU busy
-    for (t.zc = 32; t.zc > 4; t.zc >>= 1)
-        if ((long)t.ntx * t.nty * ((L.nz + t.zc - 1) / t.zc) >= min_tiles) break;
+    if (zc > 0) {
+        t.zc = zc;
+    } else {
+        // the deepest z-chunk that still leaves at least min_tiles tiles: a
+        // longer z-march re-reads fewer halo planes (measured at 256^3: 32
+        // planes beat 16 and 8 for the plain kernels), enough tiles keep
+        // every CU busy
+        for (t.zc = 32; t.zc > 4; t.zc >>= 1)
+            if ((long)t.ntx * t.nty * ((L.nz + t.zc - 1) / t.zc) >= min_tiles) break;
+    }
     t.ntz = (L.nz + t.zc - 1) / t.zc;
     t.ntiles = t.ntx * t.nty * t.ntz;
-    int nb = t.ntiles < FS_PCG_MAX_BLOCKS ? t.ntiles : FS_PCG_MAX_BLOCKS;
+    const int cap = nbmax > 0 && nbmax < FS_PCG_MAX_BLOCKS ? nbmax : FS_PCG_MAX_BLOCKS;
+    int nb = t.ntiles < cap ? t.ntiles : cap;
     if (nb >= 8) nb = nb / 8 * 8;
     t.nb = nb;
     return t;
+}
+
+// The tiling of iteration kernel k (0 the fused SpMV, 1 the update and the
+// kernels that write partials the SpMV reads) of workspace w.
+static Tiling work_tiling(const FsLayout& L, int v, const FsPcgWork& w, int k, int min_tiles = FS_MIN_TILES) {
+    return make_tiling(L, v, min_tiles, w.zc[k], w.nbmax[k]);
 }
 
 // Vector width for a precision: the widest that divides nx.
@@ -1015,7 +1034,7 @@ __global__ void __launch_bounds__(NT) k_rows_pspmv_dot(FsLayout L, Tiling tl, Fs
 template <typename T, int V>
 static void begin_v(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* rhs, const T* diag, T* x,
                     FsPcgWork& w, FsPcgState* S) {
-    Tiling tl = make_tiling(L, V);
+    Tiling tl = work_tiling(L, V, w, 1);
     if (op.kind == FS_OP_ROWS)
         hipLaunchKernelGGL((k_rows_begin<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, op, (const double*)rhs,
                            (const double*)diag, (const double*)x, (double*)w.r, (double*)w.s, (double*)w.p,
@@ -1037,7 +1056,7 @@ template <typename T, int V>
 // that separate event packets would add.
 static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const T* diag, T* x, FsPcgWork& w,
                        FsPcgState* S, int which, int n, hipEvent_t e0 = nullptr, hipEvent_t e1 = nullptr) {
-    Tiling tl = make_tiling(L, V);
+    Tiling tl = work_tiling(L, V, w, 1);
     T* pn = (T*)dir_buf(w, n);
     const T* po = (const T*)dir_buf(w, n > 0 ? n - 1 : 0);
     // the fused SpMV keeps three planes and six coefficient vectors live:
@@ -1045,7 +1064,7 @@ static void launch_one(hipStream_t st, const FsLayout& L, const FsPcgOp& op, con
     constexpr int VW = sizeof(T) == 4 ? FS_SPMV_V32 : FS_SPMV_V64;
     constexpr int VK = VW < V ? VW : V;
     const bool lds = (long)L.nx * L.ny * L.nz >= FS_LDS_MIN_CELLS;
-    const Tiling tk = make_tiling(L, VK, lds ? FS_K1_MIN_TILES : FS_MIN_TILES);
+    const Tiling tk = work_tiling(L, VK, w, 0, lds ? FS_K1_MIN_TILES : FS_MIN_TILES);
     const int nb_pq = op.kind == FS_OP_ROWS ? tl.nb : tk.nb;  // workgroups of this iteration's K1
     if (which == 0 && op.kind == FS_OP_ROWS)
         hipExtLaunchKernelGGL((k_rows_pspmv_dot<V>), dim3(tl.nb), dim3(TBX, TBY), 0, st, e0, e1, 0, L, tl, op,
@@ -1135,6 +1154,137 @@ void fs_pcg_time_kernels(hipStream_t st, const FsLayout& L, const FsPcgOp& op, c
         one_t<double>(st, L, op, (const double*)diag, (double*)x, w, S, which, n);
     else
         one_t<float>(st, L, op, (const float*)diag, (float*)x, w, S, which, n);
+}
+
+// ---- tiling autotuner ----------------------------------------------------
+// The state the timed launches run with: never stops (relative rule against
+// a zero target, a cap out of reach), r.z of one.
+__global__ void k_tune_state(FsPcgState* S) {
+    if (threadIdx.x == 0 && threadIdx.y == 0) {
+        S->rz[0] = 1;
+        S->rz[1] = 1;
+        S->done = 0;
+        S->max_iter = 1 << 30;
+        S->stop_rule = 1;
+        S->rtol2 = 0;
+        S->bb = 1;
+    }
+}
+
+__global__ void k_tune_zero(double* a, long n) {
+    for (long i = (long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long)gridDim.x * blockDim.x) a[i] = 0;
+}
+
+namespace {
+struct TuneKey {
+    int dev, nx, ny, nz, prec;
+    bool operator<(const TuneKey& o) const {
+        const int a[5] = {dev, nx, ny, nz, prec}, b[5] = {o.dev, o.nx, o.ny, o.nz, o.prec};
+        return std::lexicographical_compare(a, a + 5, b, b + 5);
+    }
+};
+struct TuneVal {
+    int zc[2], nbmax[2];
+};
+std::mutex tune_mu;
+std::map<TuneKey, TuneVal> tune_cache;
+}  // namespace
+
+void fs_pcg_tune(hipStream_t st, const FsLayout& L, const FsPcgOp& op, const void* diag, FsPcgWork& w,
+                 FsPcgState* S, int precision) {
+    if (op.kind != FS_OP_GFM) return;
+    const char* env = std::getenv("FS_PCG_TUNE");
+    if (env && std::strcmp(env, "0") == 0) {
+        w.zc[0] = w.zc[1] = w.nbmax[0] = w.nbmax[1] = 0;
+        return;
+    }
+    // FS_PCG_TILING="zc0,nb0,zc1,nb1" sets the tiling instead (measuring the
+    // spread that the partial sums' order leaves in the results)
+    if (const char* fix = std::getenv("FS_PCG_TILING")) {
+        int a[4] = {0, 0, 0, 0};
+        if (std::sscanf(fix, "%d,%d,%d,%d", &a[0], &a[1], &a[2], &a[3]) == 4 && a[0] > 0 && a[2] > 0 && a[1] > 0 &&
+            a[3] > 0) {
+            w.zc[0] = a[0];
+            w.nbmax[0] = a[1];
+            w.zc[1] = a[2];
+            w.nbmax[1] = a[3];
+            return;
+        }
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return;
+    const TuneKey key{dev, L.nx, L.ny, L.nz, precision};
+    // one tuning per shape at a time; other handles of the shape wait for it
+    std::lock_guard<std::mutex> lk(tune_mu);
+    auto hit = tune_cache.find(key);
+    if (hit != tune_cache.end()) {
+        for (int k = 0; k < 2; k++) {
+            w.zc[k] = hit->second.zc[k];
+            w.nbmax[k] = hit->second.nbmax[k];
+        }
+        return;
+    }
+    const size_t bytes = (size_t)L.total * (precision == 0 ? sizeof(double) : sizeof(float));
+    void* x = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto fail = [&](hipError_t e) {
+        if (x) (void)hipFree(x);
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipGetLastError();
+        w.zc[0] = w.zc[1] = w.nbmax[0] = w.nbmax[1] = 0;
+        (void)e;
+    };
+    hipError_t e;
+    if ((e = hipStreamSynchronize(st)) != hipSuccess || (e = hipMalloc(&x, bytes)) != hipSuccess ||
+        (e = hipMemsetAsync(x, 0, bytes, st)) != hipSuccess || (e = hipEventCreate(&e0)) != hipSuccess ||
+        (e = hipEventCreate(&e1)) != hipSuccess)
+        return fail(e);
+    hipLaunchKernelGGL(k_tune_zero, dim3(64), dim3(256), 0, st, w.partials, (long)FS_PCG_SLOTS * FS_PCG_MAX_BLOCKS);
+    const int v = vec_width(L, precision != 0);
+    const bool debug = std::getenv("FS_DEBUG") != nullptr;
+    TuneVal best{{0, 0}, {0, 0}};
+    for (int k = 0; k < FS_PCG_KERNELS; k++) {
+        float best_ms = 0;
+        for (int zc : {8, 16, 32, 64}) {
+            if (zc > 8 && (zc >> 1) >= L.nz) break;
+            for (int nbmax : {256, 512, 1024, 2048}) {
+                const Tiling t = make_tiling(L, v, FS_MIN_TILES, zc, nbmax);
+                if (nbmax > 256 && t.ntiles <= (nbmax >> 1)) break;  // same grid as the smaller cap
+                w.zc[k] = zc;
+                w.nbmax[k] = nbmax;
+                float ms_min = 0;
+                for (int rep = 0; rep < 4; rep++) {
+                    hipLaunchKernelGGL(k_tune_state, dim3(1), dim3(64), 0, st, S);
+                    if ((e = hipEventRecord(e0, st)) != hipSuccess) return fail(e);
+                    for (int q = 0; q < 3; q++) fs_pcg_time_kernels(st, L, op, diag, x, w, S, precision, k, 1);
+                    if ((e = hipEventRecord(e1, st)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess)
+                        return fail(e);
+                    float ms = 0;
+                    if ((e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) return fail(e);
+                    if (rep == 1 || (rep > 1 && ms < ms_min)) ms_min = ms;  // rep 0 warms up
+                }
+                if (debug)
+                    fprintf(stderr, "[fs] pcg tune %dx%dx%d prec %d kernel %d: zc %d nb %d (%d tiles) %.1f us\n",
+                            L.nx, L.ny, L.nz, precision, k, zc, t.nb, t.ntiles, ms_min * 1e3f / 3);
+                if (best.zc[k] == 0 || ms_min < best_ms) {
+                    best_ms = ms_min;
+                    best.zc[k] = zc;
+                    best.nbmax[k] = nbmax;
+                }
+            }
+        }
+        w.zc[k] = best.zc[k];
+        w.nbmax[k] = best.nbmax[k];
+    }
+    if ((e = hipStreamSynchronize(st)) != hipSuccess) return fail(e);
+    (void)hipFree(x);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    tune_cache[key] = best;
+    if (debug)
+        fprintf(stderr, "[fs] pcg tiling %dx%dx%d prec %d: spmv zc %d nb<=%d, update zc %d nb<=%d\n", L.nx, L.ny, L.nz,
+                precision, best.zc[0], best.nbmax[0], best.zc[1], best.nbmax[1]);
 }
 
 // ---- plain operator application (FP64) ----------------------------------
@@ -1289,7 +1439,7 @@ __global__ void __launch_bounds__(NT) k_peek(const double* part, int nb, FsPcgSt
 void fs_pcg32_reliable_update(hipStream_t st, const FsLayout& L, const float* r32, const float* diag, FsPcgWork& w,
                               FsPcgState* S) {
     int v = vec_width(L, 1);
-    Tiling tl = make_tiling(L, v);
+    Tiling tl = work_tiling(L, v, w, 1);
     if (v == 4)
         hipLaunchKernelGGL(k_reprecond<4>, dim3(tl.nb), dim3(TBX, TBY), 0, st, L, tl, r32, diag, (float*)w.s,
                            (float*)w.r, w.partials);

@@ -129,8 +129,12 @@ def _tol(phi, vel, p):
 
 
 # Per-field tolerances by the frame count a row covers, about 10x the
-# differences measured on an MI355X (the GPU's reductions are fixed-order,
-# so a rerun gives the same fields).  In these scenes a difference between
+# largest difference measured on an MI355X over several tilings of the
+# Jacobi-PCG kernels (FS_PCG_TILING; the autotuner's choice sets the order
+# of the FP64 partial sums, e.g. w's Linf after 20 dam-break frames is
+# 5e-9 .. 1.5e-7 across tilings; the FP32 paths round alpha and beta to
+# FP32 and do not see it).  The GPU's reductions are fixed-order, so with
+# one tiling a rerun gives the same fields.  In these scenes a difference between
 # two runs grows about 10x every 4 frames, whatever its origin: the FP64
 # path against the checker starts from the order of the dot products
 # (1e-16) and the FP32 paths from their inner solves' 1e-8, and both grow at
@@ -138,7 +142,7 @@ def _tol(phi, vel, p):
 # it), while the sign of phi -- the cell classification -- agrees in every
 # cell at every frame.
 TOL_FP64 = [(10, _tol((1e-10, 1e-9), (1e-9, 1e-8), (1e-8, 2e-8))),
-            (20, _tol((1e-10, 1e-9), (1e-9, 1e-7), (1e-8, 2e-8))),
+            (20, _tol((1e-10, 1e-9), (1e-8, 1.5e-6), (1e-8, 3e-8))),
             (50, _tol((1e-7, 3e-6), (3e-7, 4e-6), (1e-6, 5e-6)))]
 # two frames of 48 scenes, the worst scene per field
 TOL_BATCH = _tol((1e-12, 1e-12), (1e-9, 2e-8), (4e-8, 3e-7))
