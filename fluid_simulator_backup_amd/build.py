@@ -63,7 +63,7 @@ def build_hip(verbose=False):
 
 
 def build_oracle():
-    _run(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+    _run(["make", "-s", "-B", "-C", os.path.join(ROOT, "oracle")])
     return os.path.join(ROOT, "oracle", "libfs_oracle.so")
 
 

@@ -39,6 +39,13 @@
 #include <omp.h>
 #endif
 
+/* A statistic read out of a field value: the value itself here; the
+ * reverse-mode build (fs_oracle_ad.cpp) defines it as the value part of its
+ * active type. */
+#ifndef FSO_VALUE
+#define FSO_VALUE(x) (x)
+#endif
+
 #define G 3
 #define FS_BC_FULL 0
 #define FS_BC_NEUM (-4) /* MACROS.h:10 */
@@ -1398,11 +1405,11 @@ int fso_projection(fso_state* s, double dt, fs_step_stats* st) {
     if (st) {
         st->pcg_iterations = s->last_iters;
         st->refinement_sweeps = 0;
-        st->residual_precond = s->last_res_abs;
-        st->residual_rel = s->last_res_rel;
-        st->max_velocity[0] = s->max_vel[0];
-        st->max_velocity[1] = s->max_vel[1];
-        st->max_velocity[2] = s->max_vel[2];
+        st->residual_precond = FSO_VALUE(s->last_res_abs);
+        st->residual_rel = FSO_VALUE(s->last_res_rel);
+        st->max_velocity[0] = FSO_VALUE(s->max_vel[0]);
+        st->max_velocity[1] = FSO_VALUE(s->max_vel[1]);
+        st->max_velocity[2] = FSO_VALUE(s->max_vel[2]);
     }
     return s->last_iters >= s->P.max_iteration ? FS_ERR_NOT_CONVERGED : FS_OK;
 }
@@ -1743,8 +1750,8 @@ int fso_solve_poisson(fso_state* s, const double* source, const double* jump, do
     if (st) {
         memset(st, 0, sizeof(*st));
         st->pcg_iterations = s->last_iters;
-        st->residual_precond = s->last_res_abs;
-        st->residual_rel = s->last_res_rel;
+        st->residual_precond = FSO_VALUE(s->last_res_abs);
+        st->residual_rel = FSO_VALUE(s->last_res_rel);
     }
     return s->last_iters >= s->P.max_iteration ? FS_ERR_NOT_CONVERGED : FS_OK;
 }

@@ -36,6 +36,24 @@ def oracle_lib():
 
 
 @pytest.fixture(scope="session")
+def oracle_ad():
+    """The CPU checker's reverse-mode build (oracle/fs_oracle_ad.cpp)."""
+    import ctypes as C
+    path = os.path.join(ROOT, "oracle", "libfs_oracle_ad.so")
+    if not os.path.exists(path):
+        import subprocess
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
+    lib = C.CDLL(path)
+    vp, pd, i = C.c_void_p, C.POINTER(C.c_double), C.c_int
+    lib.fsa_rollout_gradient.argtypes = [vp, vp, pd, pd, pd, pd, pd, i, i, C.c_double, pd, pd, pd, pd, pd, pd,
+                                         C.POINTER(C.c_long)]
+    lib.fsa_rollout_gradient.restype = i
+    lib.fsa_rollout_values.argtypes = [vp, vp, pd, pd, pd, pd, pd, i, i, C.c_double]
+    lib.fsa_rollout_values.restype = i
+    return lib
+
+
+@pytest.fixture(scope="session")
 def hip_lib():
     if not _gpu_available():
         pytest.skip("no GPU visible")

@@ -42,3 +42,25 @@ def make_pair(hip_lib, oracle_lib, name="rising_drop", n=24, moving=True, inplac
 def rel_err(a, b):
     scale = max(float(np.abs(b).max()), 1e-300)
     return float(np.abs(a - b).max()) / scale
+
+
+def ad_gradient(ad, P, g, x, steps, dt, target, p=None):
+    """The FP64 reference gradient (the CPU checker in reverse mode,
+    oracle/fs_oracle_ad.cpp): (loss, [d/dphi, d/du, d/dv, d/dw], tape nodes)
+    of `steps` fixed-dt steps from x = (phi, u, v, w) with the out-of-place
+    viscosity and the level-set loss at eps = 1.5 dx."""
+    import ctypes as C
+    pd = C.POINTER(C.c_double)
+
+    def ptr(a):
+        return a.ctypes.data_as(pd)
+
+    x = [np.ascontiguousarray(a, dtype=np.float64) for a in x]
+    out = [np.zeros_like(a) for a in x]
+    loss, nodes = C.c_double(), C.c_long()
+    pw = None if p is None else ptr(np.ascontiguousarray(p, dtype=np.float64))
+    tgt = np.ascontiguousarray(target, dtype=np.float64)
+    rc = ad.fsa_rollout_gradient(C.byref(P), C.byref(g), *[ptr(a) for a in x], pw, 0, steps, dt, ptr(tgt),
+                                 C.byref(loss), *[ptr(a) for a in out], C.byref(nodes))
+    assert rc in (_abi.FS_OK, _abi.FS_ERR_NOT_CONVERGED), rc
+    return loss.value, out, nodes.value
