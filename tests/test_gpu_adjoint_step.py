@@ -312,9 +312,10 @@ def test_rollout_gradient_against_cpu_checker(oracle_lib, hip_lib):
 def test_rollout_gradient_against_reference_gradient(hip_lib, oracle_ad, ratio, order, steps):
     """The GPU adjoint rollout (checkpoint every 2) against the FP64
     reference gradient at 16^3: each of d loss / d (phi, u, v, w) within
-    1e-7 of the reference's norm in relative L2, and within 1e-6 of it in
-    Linf.  Both are the derivative at the point (branches as taken), so the
-    density ratio 1000 case is compared as tightly as the smooth one; the
+    1e-9 of the reference's norm in relative L2 and in Linf (measured on an
+    MI355X: at most 1.5e-11 at every ratio and order).  Both are the
+    derivative at the point (branches as taken), so the density ratio 1000
+    case is compared as tightly as the smooth one; the
     differences left are the two solves' tolerance and reduction order (the
     checker differentiates through its PCG iterations, the GPU solves the
     adjoint system)."""
@@ -336,7 +337,7 @@ def test_rollout_gradient_against_reference_gradient(hip_lib, oracle_ad, ratio, 
         l2 = float(np.linalg.norm(a - b) / np.linalg.norm(b))
         linf = float(np.abs(a - b).max() / np.abs(b).max())
         print(c, "relative L2", l2, "Linf", linf)
-        assert l2 <= 1e-7 and linf <= 1e-6, (c, l2, linf)
+        assert l2 <= 1e-9 and linf <= 1e-9, (c, l2, linf)
 
 
 def test_rollout_as_autograd_function(hip_lib, oracle_ad):
@@ -369,4 +370,4 @@ def test_rollout_as_autograd_function(hip_lib, oracle_ad):
     _, _, P, g = scenes.make("slosh", n, **FIXED)
     _, ref, _ = ad_gradient(oracle_ad, P, g, [phi] + v0, steps, dt, target)
     for a, b in zip(got, ref):
-        assert float(np.linalg.norm(a.numpy() - b) / np.linalg.norm(b)) <= 1e-7
+        assert float(np.linalg.norm(a.numpy() - b) / np.linalg.norm(b)) <= 1e-9
