@@ -115,7 +115,7 @@ def measured_valu(scene, n, kernel, avg_ms):
     rate = ipw * waves / (avg_ms * 1e-3)
     out = {"instructions_per_wave": ipw, "waves": waves, "achieved_wave_instr_per_s": round(rate, -6),
            "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
-    if kernel == "k_reinit":
+    if kernel.startswith("k_reinit"):
         f = reinit_cu_fraction()
         out["reinit_cu_fraction"] = f
         out["frac_of_its_cus"] = round(rate / (VALU_PEAK_WAVE_INSTR_PER_S * f), 4)
@@ -622,16 +622,23 @@ def pcg_roofline(k, cnt, k_rep, cells, n, wb):
 FRAME_KERNELS = {"k_reinit": ("FS_KS_REINIT", 6 * 8, 3 * 8),
                  "k_mg_rb_sweep": ("FS_KS_MG_SMOOTH", 6 * 4, 6 * 4)}
 # what bounds each, for the roofline note
-LIMITER = {"k_reinit": "FP64 VALU issue: the WENO5 smoothness weights take 24 IEEE divisions per cell and every "
-                       "operation keeps the reference's order, so the kernel runs at its instruction rate, far "
-                       "below the HBM bound (see the VALU counter summary under profiles/); in the frame it runs "
-                       "on a second stream confined to `reinit_cu_fraction` of the CUs, beside the projection",
+LIMITER = {"k_reinit<false>": "FP64 VALU issue: the WENO5 smoothness weights take 24 IEEE divisions per cell and "
+                              "every operation keeps the reference's order, so the kernel runs at its instruction "
+                              "rate, far below the HBM bound (see the VALU counter summary under profiles/); in the "
+                              "frame it runs on a second stream confined to `reinit_cu_fraction` of the CUs, beside "
+                              "the projection",
+           "k_reinit<true>": "FP64 VALU issue: WENO5 in product-form weights with fused multiply-adds and one "
+                             "division per cell and axis (fast_weno) still issues several hundred FP64 instructions "
+                             "per cell for 48 bytes, far below the HBM bound (see the VALU counter summary under "
+                             "profiles/); in the frame it runs on a second stream confined to `reinit_cu_fraction` "
+                             "of the CUs, beside the projection",
            "k_mg_rb_sweep": "HBM and load latency of the staged first colour"}
 
 
-def frame_roofline(R, scene, n, steps):
+def frame_roofline(R, scene, n, steps, fast=False):
     """Roofline of the multigrid path's dominant sampled frame kernel (the
-    larger in-loop time per frame)."""
+    larger in-loop time per frame); `fast`: the reinitialization runs the
+    fast_weno instantiation k_reinit<true>."""
     cells = n ** 3
     share = {}
     for kn, (ks, _, _) in FRAME_KERNELS.items():
@@ -652,9 +659,10 @@ def frame_roofline(R, scene, n, steps):
                 "algorithmic_gbs": round(FRAME_KERNELS[k][1] * cells / (a * 1e-3) / 1e9, 1),
                 "moved_gbs": round(FRAME_KERNELS[k][2] * cells / (a * 1e-3) / 1e9, 1)}
 
-    return {"bound": "hbm", "kernel": kn, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(scene, n, kn),
-            "limiter": LIMITER[kn], "valu": measured_valu(scene, n, kn, avg),
+    name = f"k_reinit<{'true' if fast else 'false'}>" if kn == "k_reinit" else kn
+    return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(scene, n, name),
+            "limiter": LIMITER[name], "valu": measured_valu(scene, n, name, avg),
             "bytes_per_launch": bytes_launch, "moved_bytes_per_launch": mov * cells,
             "moved_frac": round(mov * cells / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "avg_launch_ms": round(avg, 5),
@@ -737,7 +745,7 @@ def main():
     roof = None
     if R is not None:
         try:
-            roof = (frame_roofline(R, a.scene, a.n, a.steps) if head.startswith("mg")
+            roof = (frame_roofline(R, a.scene, a.n, a.steps, head.endswith("-fast")) if head.startswith("mg")
                     else jacobi_roofline(R, head.split("-")[0], a.scene, a.n))
         except Exception as e:
             roof = {"error": str(e)}
