@@ -82,6 +82,7 @@ class FsParams(C.Structure):
         ("is_projection_active", _I),
         ("semi_implicit_viscosity", _I),
         ("fast_weno", _I),
+        ("csf_model", _I),
     ]
 
 
@@ -133,6 +134,7 @@ def default_params(**kw):
     p.is_projection_active = 1
     p.semi_implicit_viscosity = 0
     p.fast_weno = 0
+    p.csf_model = 0
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):

@@ -435,7 +435,10 @@ class EulerianFluidSolver3D {
     // jump condition of the pressure solve, or NULL where it is zero: no
     // surface tension, or use_delta_function_formulation off (jc_on_solution
     // then keeps its initial zeros, PROJECTION_3D.h:358, :750)
-    const double* jump() const { return tension() && P.use_delta_function_formulation ? jc : nullptr; }
+    const double* jump() const { return tension() && P.use_delta_function_formulation && !csf_on ? jc : nullptr; }
+    // the projection's CSF_model flag, set by the first Sourcing with the
+    // CSF model on and kept for the handle's life (as water_projection's)
+    bool csf_on = false;
     // the curvature pass of a call reached a cell whose gradient vanishes in
     // both difference forms (the reference stops there, LEVELSET_3D.h:1041-1045);
     // call after a sync
@@ -536,6 +539,14 @@ class EulerianFluidSolver3D {
     int Sourcing(double dt) {
         int rc = ApplyViscosity(dt);
         if (!sources.empty()) SourcingFromObjectList();
+        // CSF_model, :2089-2093: ApplySurfaceTension's curvature (and its
+        // ghost refresh of phi); its force is oil/water-only (:2106-2195).
+        // The projection's CSF flag it sets zeroes the jump condition from
+        // the next projection on (PROJECTION_3D.h:754-757).
+        if (P.csf_model) {
+            ComputeCurvatures();
+            csf_on = true;
+        }
         return rc;
     }
 
@@ -662,7 +673,7 @@ class EulerianFluidSolver3D {
         // DetermineJumpConditionField: ComputeCurvatures, then jc.  Without
         // surface tension its refresh of phi's ghosts is overwritten by
         // ComputeNormals below before anything reads it, so it is skipped.
-        if (tension()) ComputeCurvatures(P.use_delta_function_formulation != 0);
+        if (tension()) ComputeCurvatures(P.use_delta_function_formulation != 0 && !csf_on);
         if (!P.use_previous_pressure) fs_fill_value(st, p, L.total, 0.0);
         fs_fill_const(st, phi, phi, L, box(0), 2);   // ComputeNormals, LEVELSET_3D.h:931
         fs_fill_const(st, irho, irho, L, box(0), 1); // PROJECTION_3D.h:813
@@ -1193,6 +1204,8 @@ class EulerianFluidSolver3D {
             throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the surface-tension jump (curvature of phi) is not built"};
         if (!sources.empty() && P.is_sourcing_active)
             throw FsFail{FS_ERR_UNSUPPORTED, "adjoint of the level-set sources (a min/max of phi) is not built"};
+        if (P.csf_model && P.is_sourcing_active)
+            throw FsFail{FS_ERR_UNSUPPORTED, "adjoint with the CSF model is not built"};
     }
 
     void EnsureAdjoint() {
@@ -1647,7 +1660,7 @@ class EulerianFluidSolver3D {
     // scene-table stage: the conditions of the concurrent order, and the
     // constant fill the stage replays
     bool BatchReinitEligible() const {
-        return P.sussmanstyle_reinitialization && !tension() && sources.empty() && !recording &&
+        return P.sussmanstyle_reinitialization && !tension() && !P.csf_model && sources.empty() && !recording &&
                (P.iteration_number_for_reinitialization & 1) == 1;
     }
     bool ReplaysConstFill() const {
@@ -1692,8 +1705,10 @@ class EulerianFluidSolver3D {
     // keep the sequential order.
     bool ConcurrentReinit() const {
         const bool serial = std::getenv("FS_SERIAL_REINIT") != nullptr;
+        // (the CSF model refreshes phi's ghosts inside SolveForRK, as surface
+        // tension does)
         return !serial && !batch_member && !recording && P.sussmanstyle_reinitialization && !tension() &&
-               sources.empty();
+               !P.csf_model && sources.empty();
     }
     // Restores the main stream and phi after the concurrent stage's
     // enqueue, also when an enqueue throws (phi then waits for what st2

@@ -77,53 +77,69 @@ __device__ __forceinline__ void fs_weno5(const double* c, double inv, double eps
 
 // The same WENO5-JS one-sided derivatives in fewer operations, for the
 // fast_weno option: not the reference's operation order, so the results
-// differ from fs_weno5 in the last bits.  Multiply-adds are fused, the
-// divisions by 3 and 6 are products with the constants' reciprocals, and
-// the weights w_k = (d_k / X_k) / sum_j (d_j / X_j), X_k = (eps + s_k)^2,
-// take the algebraically equal form n_k / sum_j n_j with n_1 = d_1 X_2 X_3,
-// n_2 = d_2 X_1 X_3, n_3 = d_3 X_1 X_2 (all terms positive, so each weight
-// is within a few ulps), and both sides share one division:
-// D- = N-/S-, D+ = N+/S+ = (N- S+, N+ S-) / (S- S+).  X_k lies in
-// [eps^2, ~1e22] for a level set on any grid, so the products stay far
-// inside the FP64 range.
-__device__ __forceinline__ double fs_weno5_s(double a, double b, double c, bool mid, bool left) {
-#pragma clang fp contract(fast)
-    const double k13 = (double)13 / 12;
-    const double e = a - 2 * b + c;
-    const double f = mid ? a - c : (left ? a - 4 * b + 3 * c : 3 * a - 4 * b + c);
-    return k13 * (e * e) + (double)0.25 * (f * f);
+// differ from fs_weno5 in the last bits.  Every rounding is written out
+// (explicit fma, the build contracts nothing), so the per-cell form below
+// and the z-marching reinitialization sweep, which computes the same
+// quantities once per triple and slides them along z, give the same bits.
+//
+// With first differences d_j = (c[j+1] - c[j]) / dx, j = -3..2, the six
+// candidate stencils of the two sides are the triples T_j = (d_j, d_j+1,
+// d_j+2), j = -3..0, read forwards (side -) or backwards (side +).  Each
+// triple has one smoothness part A = 13/12 (a - 2b + c)^2 and three ways of
+// closing it (Jiang & Shu 1996): left (a - 4b + 3c), middle (a - c) and
+// right (3a - 4b + c), the right one of a triple being the left one of the
+// triple read backwards.  X = (eps + A + f^2/4)^2.  Side - takes
+// XL(T-3), XM(T-2), XR(T-1); side + takes XR(T0), XM(T-1), XL(T-2).  The
+// weights w_k = (d_k / X_k) / sum_j (d_j / X_j) take the algebraically equal
+// form n_k / sum_j n_j with n_1 = d_1 X_2 X_3, n_2 = d_2 X_1 X_3, n_3 = d_3
+// X_1 X_2 (all terms positive, so each weight is within a few ulps), and
+// both sides share one division: D- = N-/S-, D+ = N+/S+ = (N- S+, N+ S-) /
+// (S- S+).  X_k lies in [eps^2, ~1e22] for a level set on any grid, so the
+// products stay far inside the FP64 range.
+struct FsW5Tri {
+    double xl, xm, xr;
+};
+__device__ __forceinline__ FsW5Tri fs_w5_tri(double a, double b, double c, double eps) {
+    const double e = fma(-2.0, b, a) + c;
+    const double A = ((double)13 / 12) * (e * e);
+    const double fl = fma(3.0, c, fma(-4.0, b, a)), fm = a - c, fr = fma(-4.0, b, fma(3.0, a, c));
+    FsW5Tri t;
+    t.xl = eps + fma(0.25 * fl, fl, A);
+    t.xm = eps + fma(0.25 * fm, fm, A);
+    t.xr = eps + fma(0.25 * fr, fr, A);
+    t.xl *= t.xl;
+    t.xm *= t.xm;
+    t.xr *= t.xr;
+    return t;
 }
-__device__ __forceinline__ void fs_weno5_fast(const double* c, double inv, double eps, double& dmo, double& dpo) {
-#pragma clang fp contract(fast)
-    const double m1 = inv * (c[-2] - c[-3]), m2 = inv * (c[-1] - c[-2]), m3 = inv * (c[0] - c[-1]);
-    const double m4 = inv * (c[1] - c[0]), m5 = inv * (c[2] - c[1]), p1 = inv * (c[3] - c[2]);
-    const double p2 = m5, p3 = m4, p4 = m3, p5 = m2;
-    const double r3 = (double)1 / 3, r6 = (double)1 / 6;
-    // side -: stencils (m1 m2 m3), (m2 m3 m4), (m3 m4 m5)
-    double X1 = eps + fs_weno5_s(m1, m2, m3, false, true), X2 = eps + fs_weno5_s(m2, m3, m4, true, true),
-           X3 = eps + fs_weno5_s(m3, m4, m5, false, false);
-    X1 *= X1;
-    X2 *= X2;
-    X3 *= X3;
-    const double nm1 = (double)0.1 * (X2 * X3), nm2 = (double)0.6 * (X1 * X3), nm3 = (double)0.3 * (X1 * X2);
-    const double qm1 = m1 * r3 - m2 * (7 * r6) + m3 * (11 * r6);
-    const double qm2 = -m2 * r6 + m3 * (5 * r6) + m4 * r3;
-    const double qm3 = m3 * r3 + m4 * (5 * r6) - m5 * r6;
-    const double Nm = nm1 * qm1 + nm2 * qm2 + nm3 * qm3, Sm = nm1 + nm2 + nm3;
-    // side +: the mirrored stencils of p1..p5
-    double Y1 = eps + fs_weno5_s(p1, p2, p3, false, true), Y2 = eps + fs_weno5_s(p2, p3, p4, true, true),
-           Y3 = eps + fs_weno5_s(p3, p4, p5, false, false);
-    Y1 *= Y1;
-    Y2 *= Y2;
-    Y3 *= Y3;
-    const double np1 = (double)0.1 * (Y2 * Y3), np2 = (double)0.6 * (Y1 * Y3), np3 = (double)0.3 * (Y1 * Y2);
-    const double qp1 = p1 * r3 - p2 * (7 * r6) + p3 * (11 * r6);
-    const double qp2 = -p2 * r6 + p3 * (5 * r6) + p4 * r3;
-    const double qp3 = p3 * r3 + p4 * (5 * r6) - p5 * r6;
-    const double Np = np1 * qp1 + np2 * qp2 + np3 * qp3, Sp = np1 + np2 + np3;
+// d[0..5] = d_-3 .. d_2; t[0..3] = the triples T-3 .. T0
+__device__ __forceinline__ void fs_w5_combine(const double* d, const FsW5Tri* t, double& dmo, double& dpo) {
+    const double r3 = (double)1 / 3, r6 = (double)1 / 6, r56 = (double)5 / 6, r76 = (double)7 / 6,
+                 r116 = (double)11 / 6;
+    const double X1 = t[0].xl, X2 = t[1].xm, X3 = t[2].xr;
+    const double nm1 = 0.1 * (X2 * X3), nm2 = 0.6 * (X1 * X3), nm3 = 0.3 * (X1 * X2);
+    const double qm1 = fma(r116, d[2], fma(-r76, d[1], r3 * d[0]));
+    const double qm2 = fma(r3, d[3], fma(r56, d[2], -r6 * d[1]));
+    const double qm3 = fma(-r6, d[4], fma(r56, d[3], r3 * d[2]));
+    const double Nm = fma(nm3, qm3, fma(nm2, qm2, nm1 * qm1)), Sm = (nm1 + nm2) + nm3;
+    const double Y1 = t[3].xr, Y2 = t[2].xm, Y3 = t[1].xl;
+    const double np1 = 0.1 * (Y2 * Y3), np2 = 0.6 * (Y1 * Y3), np3 = 0.3 * (Y1 * Y2);
+    const double qp1 = fma(r116, d[3], fma(-r76, d[4], r3 * d[5]));
+    const double qp2 = fma(r3, d[2], fma(r56, d[3], -r6 * d[4]));
+    const double qp3 = fma(-r6, d[1], fma(r56, d[2], r3 * d[3]));
+    const double Np = fma(np3, qp3, fma(np2, qp2, np1 * qp1)), Sp = (np1 + np2) + np3;
     const double r = (double)1 / (Sm * Sp);
     dmo = (Nm * Sp) * r;
     dpo = (Np * Sm) * r;
+}
+__device__ __forceinline__ void fs_weno5_fast(const double* c, double inv, double eps, double& dmo, double& dpo) {
+    double d[6];
+#pragma unroll
+    for (int q = 0; q < 6; q++) d[q] = inv * (c[q - 2] - c[q - 3]);
+    FsW5Tri t[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) t[q] = fs_w5_tri(d[q], d[q + 1], d[q + 2], eps);
+    fs_w5_combine(d, t, dmo, dpo);
 }
 
 // ENO3 one-sided derivatives (ADVECTION_METHOD_3D.h:1653-1718).

@@ -111,7 +111,9 @@ WorldSetup parse(const std::string& path, const fs_world_options* opts) {
     P.scaling_number_for_reinitialization = fb.GetInteger("scaling_number_for_reinitialization", 5);
     P.iteration_number_for_reinitialization = fb.GetInteger("iteration_number_for_reinitialization", 10);
     P.use_delta_function_formulation = fb.GetBoolean("use_delta_function_formulation", false);
-    if (fb.GetBoolean("CSF_model", false)) unsupported("CSF_model (continuum surface force)");
+    // EULERIAN_FLUID_SOLVER_3D.h:210; in air/water mode it zeroes the jump
+    // condition and applies no force (fs_params.csf_model)
+    P.csf_model = fb.GetBoolean("CSF_model", false) ? 1 : 0;
     // viscosities and densities: the solver's and VISCOSITY_3D's reads of
     // the same keys (VISCOSITY_3D.h:186-204) differ only in the air
     // viscosity's default (0 and 1), so that key must be given

@@ -118,6 +118,14 @@ typedef struct fs_params {
      * reference's operation order in the last bits of each derivative.
      * 0 keeps that order, bit for bit. */
     int    fast_weno;
+    /* CSF_model (EULERIAN_FLUID_SOLVER_3D.h:210): Sourcing's
+     * ApplySurfaceTension (:2089-2093, :2102-2196) computes the curvature
+     * (with its ghost refresh of phi) and sets the projection's CSF flag,
+     * which zeroes the jump condition from then on (PROJECTION_3D.h:754-757);
+     * its force is applied only in oil/water mode (:2106), so in this
+     * air/water path the model removes the surface-tension jump and adds no
+     * force, as the reference does.  The CFL's surface-tension term stays. */
+    int    csf_model;
 } fs_params;
 
 /* Per-step (or per-phase) statistics. */

@@ -89,6 +89,8 @@ struct fso_state {
     int nsrc, frame;
     /* test hook (fso_solve_poisson): a jump field that replaces sigma*kappa */
     const double* jc_override;
+    /* PROJECTION_3D::CSF_model, set by the first Sourcing with the CSF model */
+    int csf_on;
 };
 
 /* A CSR system with rows numbered plane by plane (per rows in each of
@@ -524,7 +526,7 @@ int fso_get_curvature(fso_state* s, double* kappa, double* abs_max_curvature) {
  * off: PROJECTION_3D.h:750). */
 static const double* jump_field(const fso_state* s) {
     if (s->jc_override) return s->jc_override;
-    return s->P.surface_tension != 0.0 && s->P.use_delta_function_formulation ? s->jc : NULL;
+    return s->P.surface_tension != 0.0 && s->P.use_delta_function_formulation && !s->csf_on ? s->jc : NULL;
 }
 
 /* LEVELSET_3D::ComputeNormals, LEVELSET_3D.h:929-956: lasting effect is the
@@ -939,6 +941,15 @@ static int apply_viscosity(fso_state* s, double dt);
 int fso_sourcing(fso_state* s, double dt) {
     int rc = apply_viscosity(s, dt);
     if (s->nsrc > 0) sourcing_from_object_list(s);
+    /* CSF_model, EULERIAN_FLUID_SOLVER_3D.h:2089-2093: ApplySurfaceTension
+     * computes the curvature (LEVELSET_3D::ComputeCurvatures, with its ghost
+     * refresh of phi, :2104); its force is oil/water-only (:2106-2195).  It
+     * sets the projection's CSF flag, which zeroes jc_on_solution from the
+     * next projection on (PROJECTION_3D.h:754-757). */
+    if (s->P.csf_model) {
+        fso_compute_curvatures(s);
+        s->csf_on = 1;
+    }
     return rc;
 }
 
@@ -1370,7 +1381,7 @@ static double gradient_update_beta(fso_state* s, int comp, double dt) {
  * use_delta_function_formulation (its ghosts stay 0). */
 static void determine_jump_condition(fso_state* s) {
     fill_linear(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0, 1);
-    if (s->P.surface_tension != 0.0) curvature(s, s->P.use_delta_function_formulation ? s->jc : NULL);
+    if (s->P.surface_tension != 0.0) curvature(s, s->P.use_delta_function_formulation && !s->csf_on ? s->jc : NULL);
 }
 
 static void projection_solve(fso_state* s, double dt) {

@@ -133,3 +133,31 @@ def test_flat_level_set_reports_nonfinite(hip_lib):
     with pytest.raises(_abi.FsError) as e:
         s.cfl_dt(0.5, 1)
     assert e.value.code == _abi.FS_ERR_NONFINITE
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_frames_with_csf_model(hip_lib, oracle_lib, sched):
+    """CSF_model with sigma = 0.0728 in the air/water path (no force, jump
+    condition zeroed from the first Sourcing on): two CFL frames against the
+    checker to 1e-8 (the CFL keeps its surface-tension term: the first dt bit
+    for bit), and the GPU's fields equal its own frames with sigma = 0 and the
+    same dt sequence bit for bit."""
+    over = dict(schedule=sched, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11)
+    gpu, cpu = make_pair(hip_lib, oracle_lib, "rising_drop", n=20, surface_tension=SIGMA, csf_model=1, **over)
+    plain, _ = make_pair(hip_lib, oracle_lib, "rising_drop", n=20, surface_tension=0.0, **over)
+    for f in range(2):
+        dg, dc = gpu.cfl_dt(0.5, 1), cpu.cfl_dt(0.5, 1)
+        assert dg == dc if f == 0 else abs(dg - dc) < 1e-8 * dc, (f, dg, dc)
+        gpu.advance_frame(dc)
+        cpu.advance_frame(dc)
+        plain.advance_frame(dc)
+    fg, fc, fp = gpu.get_fields(), cpu.get_fields(), plain.get_fields()
+    for c in ("phi", "u", "v", "w", "p"):
+        assert np.array_equal(fg[c], fp[c]), c
+    fg["p"] -= fg["p"].mean()
+    fc["p"] -= fc["p"].mean()
+    scale = max(np.linalg.norm(fc[c]) for c in ("u", "v", "w"))
+    for c in ("phi", "p"):
+        assert rel_err(fg[c], fc[c]) < 1e-8, (c, rel_err(fg[c], fc[c]))
+    for c in ("u", "v", "w"):
+        assert np.linalg.norm(fg[c] - fc[c]) / scale < 1e-8, c

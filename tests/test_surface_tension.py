@@ -173,3 +173,40 @@ def test_zero_gradient_reports_nonfinite(oracle_lib):
     with pytest.raises(_abi.FsError) as e:
         s.curvature()
     assert e.value.code == _abi.FS_ERR_NONFINITE
+
+
+@pytest.mark.parametrize("sched", [_abi.FS_SCHEDULE_FIXED, _abi.FS_SCHEDULE_COMPAT])
+def test_csf_model_in_air_water_mode_removes_the_jump(oracle_lib, sched):
+    """CSF_model in the air/water path: ApplySurfaceTension's force is
+    oil/water-only (EULERIAN_FLUID_SOLVER_3D.h:2106-2195) and the flag it sets
+    zeroes the projection's jump condition (PROJECTION_3D.h:754-757), so a
+    step with sigma != 0 and the CSF model equals, bit for bit, the step with
+    sigma = 0; without the CSF model the jump changes the step; with
+    Sourcing off the flag is never set and the jump stays."""
+    from helpers import smooth_velocity
+    n = 20
+    phi, _, _, g = scenes.make("rising_drop", n)
+    vel = smooth_velocity(n)
+
+    def run(**over):
+        kw = dict(schedule=sched, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-12)
+        kw.update(over)
+        _, _, P, g = scenes.make("rising_drop", n, **kw)
+        s = _abi.Solver(oracle_lib, "fso_", P, g)
+        s.set_fields(phi, *vel)
+        for _ in range(2):
+            s.step(2e-3)
+        f = s.get_fields()
+        s.close()
+        return f
+
+    csf = run(surface_tension=0.0728, csf_model=1)
+    plain = run(surface_tension=0.0)
+    jump = run(surface_tension=0.0728)
+    for c in ("phi", "u", "v", "w", "p"):
+        assert np.array_equal(csf[c], plain[c]), c
+    assert not np.array_equal(jump["p"], plain["p"])
+    no_sourcing = run(surface_tension=0.0728, csf_model=1, is_sourcing_active=0)
+    with_jump_no_sourcing = run(surface_tension=0.0728, is_sourcing_active=0)
+    for c in ("phi", "u", "v", "w", "p"):
+        assert np.array_equal(no_sourcing[c], with_jump_no_sourcing[c]), c
