@@ -77,22 +77,16 @@ __device__ __forceinline__ void fs_weno5_of(const double* c, double inv, double 
         fs_weno5(c, inv, eps, dmo, dpo);
 }
 
-// One Sussman sweep at cell c (WENO5thReinitialization,
-// ADVECTION_METHOD_3D.h:745-1637): WENO5 one-sided derivatives of the ghost
-// copy gh, Godunov case per axis, the misplaced parenthesis of :1463/:1605
+// The Godunov update and clamp of one Sussman sweep at cell c from its six
+// one-sided derivatives (WENO5thReinitialization, ADVECTION_METHOD_3D.h:
+// 745-1637): Godunov case per axis, the misplaced parenthesis of :1463/:1605
 // (x shock, y shock towards +, z expansion; the Euclidean norm instead when
 // `exact` is set), and the sign-preserving clamp (:1624-1634).
-template <bool FAST>
-__device__ __forceinline__ void fs_reinit_cell(double* __restrict__ out, const double* __restrict__ gh,
-                                               const double* __restrict__ sgn, const FsLayout& L, double ix, double iy,
-                                               double iz, double eps, double dt, int exact, long c) {
-    double f[7], mx, px, my, py, mz, pz, X, Y, Z;
-    gather7(gh, c, 1, f);
-    fs_weno5_of<FAST>(f + 3, ix, eps, mx, px);
-    gather7(gh, c, L.sx, f);
-    fs_weno5_of<FAST>(f + 3, iy, eps, my, py);
-    gather7(gh, c, L.sy, f);
-    fs_weno5_of<FAST>(f + 3, iz, eps, mz, pz);
+__device__ __forceinline__ void fs_reinit_finish(double* __restrict__ out, const double* __restrict__ gh,
+                                                 const double* __restrict__ sgn, double mx, double px, double my,
+                                                 double py, double mz, double pz, double eps, double dt, int exact,
+                                                 long c) {
+    double X, Y, Z;
     double sg = sgn[c];
     double rho = gh[c];  // the ghost copy's box is phi
     int cx = fs_godunov_axis(sg, mx, px, X);
@@ -111,4 +105,20 @@ __device__ __forceinline__ void fs_reinit_cell(double* __restrict__ out, const d
         if (sg > 0) rho = eps;
     }
     out[c] = rho;
+}
+
+// One Sussman sweep at cell c: WENO5 one-sided derivatives of the ghost copy
+// gh along each axis, then fs_reinit_finish.
+template <bool FAST>
+__device__ __forceinline__ void fs_reinit_cell(double* __restrict__ out, const double* __restrict__ gh,
+                                               const double* __restrict__ sgn, const FsLayout& L, double ix, double iy,
+                                               double iz, double eps, double dt, int exact, long c) {
+    double f[7], mx, px, my, py, mz, pz;
+    gather7(gh, c, 1, f);
+    fs_weno5_of<FAST>(f + 3, ix, eps, mx, px);
+    gather7(gh, c, L.sx, f);
+    fs_weno5_of<FAST>(f + 3, iy, eps, my, py);
+    gather7(gh, c, L.sy, f);
+    fs_weno5_of<FAST>(f + 3, iz, eps, mz, pz);
+    fs_reinit_finish(out, gh, sgn, mx, px, my, py, mz, pz, eps, dt, exact, c);
 }

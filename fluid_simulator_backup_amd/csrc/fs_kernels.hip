@@ -3,6 +3,8 @@
 // One thread per cell/face, 64-wide x rows (one wavefront per x row segment,
 // coalesced loads), 4 rows per workgroup.  Stencils read the padded layout
 // directly; neighbours along y and z come through L2.
+#include <cstdlib>
+
 #include "fs_kernels.h"
 
 #define BX 64
@@ -1203,9 +1205,60 @@ __global__ void __launch_bounds__(BX* BY) k_reinit(double* __restrict__ out, con
     fs_reinit_cell<FAST>(out, gh, sgn, L, ix, iy, iz, eps, dt, exact, L.idx(i, j, k));
 }
 
+// The fast-WENO5 sweep marching z: each thread owns an (i, j) column and KC
+// consecutive planes.  Along z the seven values of the stencil, their six
+// first differences and the four triples' smoothness terms (fs_w5_tri) slide
+// with k, so each cell loads one new value and forms one new triple along z
+// instead of seven loads and four triples; x and y take the per-cell form.
+// The same operations on the same values as fs_reinit_cell<true>, so the
+// same bits.
+#define FS_REINIT_KC 8
+__global__ void __launch_bounds__(BX* BY) k_reinit_zm(double* __restrict__ out, const double* __restrict__ gh,
+                                                      const double* __restrict__ sgn, FsLayout L, double ix,
+                                                      double iy, double iz, double eps, double dt, int exact) {
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k0 = blockIdx.z * FS_REINIT_KC;
+    if (i >= L.nx || j >= L.ny) return;
+    const int k1 = min(k0 + FS_REINIT_KC, L.nz);
+    const long sy = L.sy;
+    long c = L.idx(i, j, k0);
+    // z window at cell k: phi(k+2) (next difference's left end), d_{k-3}..d_{k+1}, T_{k-3}..T_{k-1}
+    double d[6];
+    FsW5Tri t[4];
+    double prev = gh[c - 3 * sy];
+#pragma unroll
+    for (int q = 0; q < 5; q++) {
+        const double nx_ = gh[c + (q - 2) * sy];
+        d[q] = iz * (nx_ - prev);
+        prev = nx_;
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) t[q] = fs_w5_tri(d[q], d[q + 1], d[q + 2], eps);
+    for (int k = k0; k < k1; k++, c += sy) {
+        const double nx_ = gh[c + 3 * sy];
+        d[5] = iz * (nx_ - prev);
+        prev = nx_;
+        t[3] = fs_w5_tri(d[3], d[4], d[5], eps);
+        double f[7], mx, px, my, py, mz, pz;
+        fs_w5_combine(d, t, mz, pz);
+        gather7(gh, c, 1, f);
+        fs_weno5_fast(f + 3, ix, eps, mx, px);
+        gather7(gh, c, L.sx, f);
+        fs_weno5_fast(f + 3, iy, eps, my, py);
+        fs_reinit_finish(out, gh, sgn, mx, px, my, py, mz, pz, eps, dt, exact, c);
+#pragma unroll
+        for (int q = 0; q < 5; q++) d[q] = d[q + 1];
+#pragma unroll
+        for (int q = 0; q < 3; q++) t[q] = t[q + 1];
+    }
+}
+
 void fs_reinit_sweep(hipStream_t st, double* out, const double* ghost, const double* sgn, const FsLayout& L,
                      const FsGridConst& cell, double eps, double dt, int exact, bool fast) {
-    if (fast)
+    const FsBox b = fs_box(L, 0);
+    if (fast && !std::getenv("FS_REINIT_PER_CELL"))
+        hipLaunchKernelGGL(k_reinit_zm, dim3((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + FS_REINIT_KC - 1) / FS_REINIT_KC),
+                           dim3(BX, BY), 0, st, out, ghost, sgn, L, cell.x.inv, cell.y.inv, cell.z.inv, eps, dt, exact);
+    else if (fast)
         hipLaunchKernelGGL(k_reinit<true>, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, out, ghost, sgn, L,
                            cell.x.inv, cell.y.inv, cell.z.inv, eps, dt, exact);
     else

@@ -134,6 +134,25 @@ def test_fast_weno_phases_within_rounding(hip_lib, oracle_lib, schedule):
         assert e < 1e-11, c
 
 
+@pytest.mark.parametrize("schedule", [_abi.FS_SCHEDULE_COMPAT, _abi.FS_SCHEDULE_FIXED])
+def test_fast_reinit_z_march_equals_per_cell(hip_lib, oracle_lib, monkeypatch, schedule):
+    """The fast sweep marching z (each column's WENO5 triples sliding with k,
+    8 planes per thread) gives the per-cell fast sweep's bits, also in a
+    last chunk shorter than 8 planes (21 = 2 x 8 + 5)."""
+    out = []
+    for per_cell in (False, True):
+        if per_cell:
+            monkeypatch.setenv("FS_REINIT_PER_CELL", "1")
+        else:
+            monkeypatch.delenv("FS_REINIT_PER_CELL", raising=False)
+        gpu, _ = make_pair(hip_lib, oracle_lib, "rising_drop", n=21, schedule=schedule, fast_weno=1)
+        gpu.compute_curvatures()
+        gpu.phase("reinitialization", 2e-3)
+        out.append(gpu.get_fields()["phi"])
+        gpu.close()
+    assert np.array_equal(out[0], out[1])
+
+
 def _frames(hip_lib, oracle_lib, name, n, frames, velocity_scale=False, **over):
     gpu, cpu = make_pair(hip_lib, oracle_lib, name, n=n, moving=False, **over)
     its = []
