@@ -116,20 +116,20 @@ def measured_valu(scene, n, kernel, avg_ms):
     out = {"instructions_per_wave": ipw, "waves": waves, "achieved_wave_instr_per_s": round(rate, -6),
            "peak_wave_instr_per_s": VALU_PEAK_WAVE_INSTR_PER_S, "frac": round(rate / VALU_PEAK_WAVE_INSTR_PER_S, 4)}
     if kernel.startswith("k_reinit"):
-        f = reinit_cu_fraction()
+        f = reinit_cu_fraction(kernel == "k_reinit_zm")
         out["reinit_cu_fraction"] = f
         out["frac_of_its_cus"] = round(rate / (VALU_PEAK_WAVE_INSTR_PER_S * f), 4)
     return out
 
 
-def reinit_cu_fraction():
+def reinit_cu_fraction(fast=False):
     """Fraction of the CUs the step's reinitialization stream may use (the
-    solver's default 3/4, FS_REINIT_CU_FRAC; all of them when the step runs
-    in sequence, FS_SERIAL_REINIT)."""
+    solver's default 3/4, 5/8 with fast WENO5, FS_REINIT_CU_FRAC; all of them
+    when the step runs in sequence, FS_SERIAL_REINIT)."""
     if os.environ.get("FS_SERIAL_REINIT"):
         return 1.0
     try:
-        f = float(os.environ.get("FS_REINIT_CU_FRAC", "0.75"))
+        f = float(os.environ.get("FS_REINIT_CU_FRAC", "0.625" if fast else "0.75"))
     except ValueError:
         return 1.0
     return f if 0 < f < 1 else 1.0
@@ -627,18 +627,18 @@ LIMITER = {"k_reinit<false>": "FP64 VALU issue: the WENO5 smoothness weights tak
                               "rate, far below the HBM bound (see the VALU counter summary under profiles/); in the "
                               "frame it runs on a second stream confined to `reinit_cu_fraction` of the CUs, beside "
                               "the projection",
-           "k_reinit<true>": "FP64 VALU issue: WENO5 in product-form weights with fused multiply-adds and one "
-                             "division per cell and axis (fast_weno) still issues several hundred FP64 instructions "
-                             "per cell for 48 bytes, far below the HBM bound (see the VALU counter summary under "
-                             "profiles/); in the frame it runs on a second stream confined to `reinit_cu_fraction` "
-                             "of the CUs, beside the projection",
+           "k_reinit_zm": "FP64 VALU issue: WENO5 in product-form weights (fast_weno), its smoothness terms "
+                          "shared between the two sides and slid along z, still issues several hundred FP64 "
+                          "instructions per cell for 48 bytes, far below the HBM bound (see the VALU counter summary "
+                          "under profiles/); in the frame it runs on a second stream confined to "
+                          "`reinit_cu_fraction` of the CUs, beside the projection",
            "k_mg_rb_sweep": "HBM and load latency of the staged first colour"}
 
 
 def frame_roofline(R, scene, n, steps, fast=False):
     """Roofline of the multigrid path's dominant sampled frame kernel (the
     larger in-loop time per frame); `fast`: the reinitialization runs the
-    fast_weno instantiation k_reinit<true>."""
+    fast_weno sweep marching z, k_reinit_zm."""
     cells = n ** 3
     share = {}
     for kn, (ks, _, _) in FRAME_KERNELS.items():
@@ -659,7 +659,7 @@ def frame_roofline(R, scene, n, steps, fast=False):
                 "algorithmic_gbs": round(FRAME_KERNELS[k][1] * cells / (a * 1e-3) / 1e9, 1),
                 "moved_gbs": round(FRAME_KERNELS[k][2] * cells / (a * 1e-3) / 1e9, 1)}
 
-    name = f"k_reinit<{'true' if fast else 'false'}>" if kn == "k_reinit" else kn
+    name = ("k_reinit_zm" if fast else "k_reinit<false>") if kn == "k_reinit" else kn
     return {"bound": "hbm", "kernel": name, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": measured_traffic(scene, n, name),
             "limiter": LIMITER[name], "valu": measured_valu(scene, n, name, avg),

@@ -1780,13 +1780,15 @@ class EulerianFluidSolver3D {
         if (!ev_rn) HIPCHECK(hipEventCreateWithFlags(&ev_rn, hipEventDisableTiming));
         if (!phi_r) phi_r = dfield();
         // The reinitialization stream runs on the lowest-indexed 3/4 of
-        // the CUs (FS_REINIT_CU_FRAC overrides the fraction; 0 or 1
-        // lifts the mask).  The projection beside it is latency-bound and
-        // keeps the remaining CUs to itself: at 256^3 on an MI355X the
-        // frame takes 42.3 ms against 45.4 ms unmasked (5/8 of the CUs:
-        // 45.4, 7/8: 44.1).
+        // the CUs, 5/8 with fast WENO5 (FS_REINIT_CU_FRAC overrides the
+        // fraction; 0 or 1 lifts the mask).  The projection beside it is
+        // latency-bound and keeps the remaining CUs to itself.  Measured at
+        // 256^3 on an MI355X: reference-order sweeps 42.3 ms per frame
+        // against 45.4 ms unmasked (5/8 of the CUs: 45.4, 7/8: 44.1); the
+        // fast sweeps, half the instructions, 28.1 ms at 5/8 against 28.4
+        // (1/2), 28.7 (3/4) and 28.7 (7/8).
         const char* fr = std::getenv("FS_REINIT_CU_FRAC");
-        const double f = fr ? std::atof(fr) : 0.75;
+        const double f = fr ? std::atof(fr) : (fast() ? 0.625 : 0.75);
         hipDeviceProp_t prop;
         HIPCHECK(hipGetDeviceProperties(&prop, device));
         const int n = prop.multiProcessorCount, keep = (int)std::lround(n * f);
