@@ -65,7 +65,8 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (profiler byte-counter calibration)")
-    ap.add_argument("--workload", default="timestep", choices=["timestep", "batch", "adjoint", "projection"],
+    ap.add_argument("--workload", default="timestep",
+                    choices=["timestep", "batch", "adjoint", "projection", "sensitivity"],
                     help="batch: 48 uneven scenes advanced together vs one at a time; adjoint: checkpointed "
                          "rollout gradient checked against central differences; projection: one projection of "
                          "the 64-bubble pool (default 512^3) per path (each its own JSON line)")
@@ -400,6 +401,51 @@ def adjoint_main(a):
     print(json.dumps(out))
 
 
+def sensitivity_main(a):
+    """The headline scene's sensitivity to rounding: the same 50 rising-drop
+    frames run twice on the plain FP64 path, with two tilings of the
+    Jacobi-PCG kernels (FS_PCG_TILING), which differ only in the order of the
+    FP64 partial sums; per-field differences after selected frames, in the
+    measure the FP32 paths' error against FP64 is reported in."""
+    import torch
+
+    import fluid_simulator_backup_amd as fsa
+    from fluid_simulator_backup_amd import _abi, frame_loop, scenes
+    torch.cuda.set_device(0)
+    lib = fsa.load_library()
+    n = a.n or 256
+    frames = a.warmup + a.steps
+    snap_at = sorted({f for f in (1, 5, 10, 20, 30, 40, 50, frames) if f <= frames})
+    tilings = ("8,256,8,256", "64,2048,16,1024")
+    runs = []
+    for tiling in tilings:
+        os.environ["FS_PCG_TILING"] = tiling
+        phi, vel, P, g = scenes.make(a.scene, n, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-8,
+                                     precision=_abi.FS_FP64, max_iteration=20000)
+        s = _abi.Solver(lib, "fs_", P, g, 0)
+        s.set_fields(phi, *vel)
+        snaps = {}
+
+        def keep(r):
+            if r["frame"] + 1 in snap_at:
+                snaps[r["frame"] + 1] = s.get_fields()
+                print(f"[bench] tiling {tiling}: frame {r['frame'] + 1}", file=sys.stderr, flush=True)
+
+        frame_loop.advance_frames(s, frames, on_frame=keep)
+        s.close()
+        runs.append(snaps)
+    os.environ.pop("FS_PCG_TILING", None)
+    diff = {f"after_frame_{f}": field_error(runs[1][f], runs[0][f]) for f in snap_at}
+    print(json.dumps({
+        "metric": "FP64 run-to-run difference from the order of the PCG partial sums alone", "value":
+            diff[f"after_frame_{snap_at[-1]}"]["phi"]["rel_l2"], "unit": "relative L2 of phi after the last frame",
+        "n_gpus": 1, "steps": frames, "warmup": 0, "higher_is_better": False, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"{a.scene} {n}^3, {frames} frames from rest, CFL dt per frame, FP64 Jacobi-PCG to "
+                               f"1e-8, Jacobi tilings {tilings[0]} and {tilings[1]} (zc,nb of the SpMV, of the "
+                               f"update)", "cells": n ** 3},
+        "difference": diff}))
+
+
 def projection_main(a):
     """Ill-conditioned projection: the 64-bubble pool (water z < 0.7 with 64
     seeded air bubbles, radii U(0.02, 0.06), density ratio 1000) at n^3
@@ -706,6 +752,9 @@ def main():
         return
     if a.workload == "projection":
         projection_main(a)
+        return
+    if a.workload == "sensitivity":
+        sensitivity_main(a)
         return
     a.n = a.n or 256
     import torch
