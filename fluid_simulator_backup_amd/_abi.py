@@ -24,6 +24,7 @@ FS_STOP_ABS_PRECOND = 0
 FS_STOP_REL_RESIDUAL = 1
 FS_SCHEDULE_COMPAT = 0
 FS_SCHEDULE_FIXED = 1
+FS_CURV_LEVELSET, FS_CURV_NORMALS, FS_CURV_BOTH, FS_CURV_NONE = 0, 1, 2, 3
 FS_SOURCE_NONE = 0
 FS_SOURCE_ADD = 1
 FS_SOURCE_DELETE = 2
@@ -83,6 +84,7 @@ class FsParams(C.Structure):
         ("semi_implicit_viscosity", _I),
         ("fast_weno", _I),
         ("csf_model", _I),
+        ("curvature_method", _I),
     ]
 
 
@@ -135,6 +137,7 @@ def default_params(**kw):
     p.semi_implicit_viscosity = 0
     p.fast_weno = 0
     p.csf_model = 0
+    p.curvature_method = FS_CURV_LEVELSET
     for k, v in kw.items():
         if k == "gravity":
             for i in range(3):

@@ -217,6 +217,8 @@ class EulerianFluidSolver3D {
         if (P.precision != FS_FP64 && P.precision != FS_FP32_REFINE && P.precision != FS_FP32_MG)
             throw FsFail{FS_ERR_BAD_ARG, "precision"};
         if (P.scaling_number_for_reinitialization <= 0) throw FsFail{FS_ERR_BAD_ARG, "scaling_number"};
+        if (P.curvature_method < FS_CURV_LEVELSET || P.curvature_method > FS_CURV_NONE)
+            throw FsFail{FS_ERR_BAD_ARG, "curvature_method"};
         debug = std::getenv("FS_DEBUG") != nullptr;
         // tuning override of the reliable-update trigger (residual^2 drop, default 0.01)
         if (const char* e = std::getenv("FS_FP32_UPDATE_DROP")) {
@@ -311,7 +313,8 @@ class EulerianFluidSolver3D {
             if (s_) (void)hipStreamSynchronize(s_);
         double* f64[] = {phi, phi_r, sghost, sghost2, u, v, w, spare_u, spare_v, spare_w, gu, gv, gw, p, div, rho, irho, sgn, tmp, bx, by, bz, diag, rhs,
                          scratch, staging, w64.partials, w32.partials, u_rk, v_rk, w_rk,
-                         va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc};
+                         va[0], va[1], va[2], va[3], va[4], va[5], vdiag, vrhs, wv.partials, kappa, jc,
+                         nrm[0], nrm[1], nrm[2]};
         for (double* f : f64)
             if (f) (void)hipFree(f);
         void* fv[] = {w64.r, w64.s, w64.p, w64.q, w32.r, w32.s, w32.p, w32.q, bx32, by32, bz32, diag32, e32, r32, phi32,
@@ -425,17 +428,43 @@ class EulerianFluidSolver3D {
     // near the interface (the CFL rule's s_f).  With zero surface tension
     // nothing reads the curvature and only the ghost refresh is kept.  With_jc:
     // also jc = sigma * kappa (DetermineJumpConditionField, PROJECTION_3D.h:746-769).
+    // curvature_method: by the normals first (fixed schedule: formed from
+    // the current phi; compatible: those the last projection left), then by
+    // the level set, which overwrites kappa; the maximum spans both passes.
     void ComputeCurvatures(bool with_jc = false) {
         fs_fill_linear(st, phi, phi, L, box(0), 2, 1);
         if (!tension()) return;
         HIPCHECK(hipMemsetAsync(kstat, 0, sizeof(unsigned long long) * 2, st));
-        fs_curvature(st, phi, kappa, with_jc ? jc : nullptr, P.surface_tension, L, cell, fixed() ? 1 : 0, kstat);
+        if (curv_normals()) {
+            EnsureNormals();
+            if (fixed()) fs_normals(st, phi, nrm, L, cell);
+            fs_curvature_normals(st, nrm, kappa, with_jc && !curv_levelset() ? jc : nullptr, P.surface_tension, L,
+                                 cell, kstat);
+        }
+        if (curv_levelset())
+            fs_curvature(st, phi, kappa, with_jc ? jc : nullptr, P.surface_tension, L, cell, fixed() ? 1 : 0,
+                         kstat);
         HIPCHECK(hipMemcpyAsync(hkstat, kstat, sizeof(unsigned long long) * 2, hipMemcpyDeviceToHost, st));
+    }
+    bool curv_normals() const {
+        return P.curvature_method == FS_CURV_NORMALS || P.curvature_method == FS_CURV_BOTH;
+    }
+    bool curv_levelset() const {
+        return P.curvature_method == FS_CURV_LEVELSET || P.curvature_method == FS_CURV_BOTH;
+    }
+    // LEVELSET_3D::normal, zero until the first ComputeNormals
+    double* nrm[3] = {nullptr, nullptr, nullptr};
+    void EnsureNormals() {
+        for (double*& f : nrm)
+            if (!f) f = dfield();
     }
     // jump condition of the pressure solve, or NULL where it is zero: no
     // surface tension, or use_delta_function_formulation off (jc_on_solution
     // then keeps its initial zeros, PROJECTION_3D.h:358, :750)
-    const double* jump() const { return tension() && P.use_delta_function_formulation && !csf_on ? jc : nullptr; }
+    const double* jump() const {
+        return tension() && P.use_delta_function_formulation && !csf_on && P.curvature_method != FS_CURV_NONE ? jc
+                                                                                                             : nullptr;
+    }
     // the projection's CSF_model flag, set by the first Sourcing with the
     // CSF model on and kept for the handle's life (as water_projection's)
     bool csf_on = false;
@@ -676,6 +705,12 @@ class EulerianFluidSolver3D {
         if (tension()) ComputeCurvatures(P.use_delta_function_formulation != 0 && !csf_on);
         if (!P.use_previous_pressure) fs_fill_value(st, p, L.total, 0.0);
         fs_fill_const(st, phi, phi, L, box(0), 2);   // ComputeNormals, LEVELSET_3D.h:931
+        // ... and its normals, which the compatible schedule's next curvature
+        // by normals reads (PROJECTION_3D.h:803)
+        if (tension() && curv_normals() && !fixed()) {
+            EnsureNormals();
+            fs_normals(st, phi, nrm, L, cell);
+        }
         fs_fill_const(st, irho, irho, L, box(0), 1); // PROJECTION_3D.h:813
         fs_assemble(st, phi, irho, div, jump(), bx, by, bz, diag, rhs, S64pin(), L, cell, axis_diag());
         op_phi = !std::getenv("FS_BETA_ARRAYS");

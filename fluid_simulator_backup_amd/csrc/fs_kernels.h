@@ -66,6 +66,12 @@ void fs_assemble(hipStream_t st, const double* phi, const double* irho, const do
 // exact: the corrected cross term and one-sided z difference.
 void fs_curvature(hipStream_t st, const double* phi, double* kappa, double* jc, double sigma, const FsLayout& L,
                   const FsGridConst& cell, int exact, unsigned long long* kstat);
+// unit normals of phi (ComputeNormals), nrm[0..2] = x, y, z components
+void fs_normals(hipStream_t st, const double* phi, double* const* nrm, const FsLayout& L, const FsGridConst& cell);
+// curvature by normals: kappa (jc = sigma * kappa when given), kstat[0] =
+// bits of max |div n| over every cell (not cleared here)
+void fs_curvature_normals(hipStream_t st, double* const* nrm, double* kappa, double* jc, double sigma,
+                          const FsLayout& L, const FsGridConst& cell, unsigned long long* kstat);
 // one spherical level-set source over the cells lo..hi (inclusive, inside
 // the grid): mode 1 phi = min(d, phi), mode 2 phi = max(-d, phi) where
 // d <= 3 dx; origin = (x_min, y_min, z_min) of the cell grid

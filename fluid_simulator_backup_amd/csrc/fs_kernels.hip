@@ -924,6 +924,81 @@ __global__ void __launch_bounds__(BX* BY) k_curvature(const double* __restrict__
     wg_max_atomic(amax, kstat);
 }
 
+// LEVELSET_3D::ComputeNormals (LEVELSET_3D.h:933-955): unit normals of phi,
+// its ghosts read as their constant fill (:931): central differences,
+// forward ones where the central gradient vanishes, each vector times the
+// reciprocal of its magnitude unless that is zero (VECTOR_3D.h:153-165).
+__global__ void __launch_bounds__(BX* BY) k_normals(const double* __restrict__ phi, double* __restrict__ n0,
+                                                    double* __restrict__ n1, double* __restrict__ n2, FsLayout L,
+                                                    FsGridConst g) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    auto P = [&](int a, int b, int c) {
+        return phi[L.idx(fs_clampi(a, 0, L.nx - 1), fs_clampi(b, 0, L.ny - 1), fs_clampi(c, 0, L.nz - 1))];
+    };
+    double x = (P(i + 1, j, k) - P(i - 1, j, k)) * g.x.inv2d;
+    double y = (P(i, j + 1, k) - P(i, j - 1, k)) * g.y.inv2d;
+    double z = (P(i, j, k + 1) - P(i, j, k - 1)) * g.z.inv2d;
+    double mag = sqrt(x * x + y * y + z * z);
+    if (!(mag != 0)) {
+        x = (P(i + 1, j, k) - P(i, j, k)) * g.x.inv;
+        y = (P(i, j + 1, k) - P(i, j, k)) * g.y.inv;
+        z = (P(i, j, k + 1) - P(i, j, k)) * g.z.inv;
+    }
+    double m = sqrt(x * x + y * y + z * z);
+    if (m != 0) {
+        double r = (double)1 / m;
+        x *= r;
+        y *= r;
+        z *= r;
+    }
+    const long c = L.idx(i, j, k);
+    n0[c] = x;
+    n1[c] = y;
+    n2[c] = z;
+}
+
+void fs_normals(hipStream_t st, const double* phi, double* const* nrm, const FsLayout& L, const FsGridConst& cell) {
+    hipLaunchKernelGGL(k_normals, grid_for(fs_box(L, 0)), dim3(BX, BY), 0, st, phi, nrm[0], nrm[1], nrm[2], L, cell);
+}
+
+// Curvature by normals (LEVELSET_3D.h:974-996): the central divergence of
+// the normal field (its one ghost layer the constant fill, :976), |curv|
+// above tol taken as -tol; kstat[0] = bits of max |curv| (unclamped) over
+// every cell.  jc[c] = sigma * kappa when jc is given.
+__global__ void __launch_bounds__(BX* BY) k_curvature_normals(const double* __restrict__ n0,
+                                                              const double* __restrict__ n1,
+                                                              const double* __restrict__ n2, double* __restrict__ kappa,
+                                                              double* __restrict__ jc, double sigma, FsLayout L,
+                                                              FsGridConst g, double tol, unsigned long long* kstat) {
+    int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    double amax = 0;
+    for (int k = blockIdx.z * KZ; k < min(blockIdx.z * KZ + KZ, L.nz); k++) {
+        if (i >= L.nx || j >= L.ny) continue;
+        auto N = [&](const double* f, int a, int b, int c) {
+            return f[L.idx(fs_clampi(a, 0, L.nx - 1), fs_clampi(b, 0, L.ny - 1), fs_clampi(c, 0, L.nz - 1))];
+        };
+        double curv = (N(n0, i + 1, j, k) - N(n0, i - 1, j, k)) * g.x.inv2d;
+        curv += (N(n1, i, j + 1, k) - N(n1, i, j - 1, k)) * g.y.inv2d;
+        curv += (N(n2, i, j, k + 1) - N(n2, i, j, k - 1)) * g.z.inv2d;
+        const double kv = fabs(curv) <= tol ? curv : -tol;
+        const long c = L.idx(i, j, k);
+        kappa[c] = kv;
+        if (jc) jc[c] = sigma * kv;
+        if (fabs(curv) > amax) amax = fabs(curv);
+    }
+    wg_max_atomic(amax, kstat);
+}
+
+void fs_curvature_normals(hipStream_t st, double* const* nrm, double* kappa, double* jc, double sigma,
+                          const FsLayout& L, const FsGridConst& cell, unsigned long long* kstat) {
+    FsBox b = fs_box(L, 0);
+    double tol = (double)1 / std::min(std::min(cell.x.d, cell.y.d), cell.z.d);
+    dim3 g((b.ni + BX - 1) / BX, (b.nj + BY - 1) / BY, (b.nk + KZ - 1) / KZ);
+    hipLaunchKernelGGL(k_curvature_normals, g, dim3(BX, BY), 0, st, nrm[0], nrm[1], nrm[2], kappa, jc, sigma, L, cell,
+                       tol, kstat);
+}
+
 void fs_curvature(hipStream_t st, const double* phi, double* kappa, double* jc, double sigma, const FsLayout& L,
                   const FsGridConst& cell, int exact, unsigned long long* kstat) {
     FsBox b = fs_box(L, 0);

@@ -133,10 +133,14 @@ WorldSetup parse(const std::string& path, const fs_world_options* opts) {
     // the semi-implicit viscosity runs only in oil/water mode
     // (VISCOSITY_3D.h:565-989), so an air/water script never reaches it
     P.semi_implicit_viscosity = 0;
-    if (P.surface_tension != 0 && !fb.GetBoolean("curvature_by_levelset", false))
-        unsupported("surface tension with curvature_by_levelset = false (curvature by normals)");
-    if (fb.GetBoolean("curvature_by_normal_vector", false) && P.surface_tension != 0)
-        unsupported("curvature_by_normal_vector");
+    // the level set's two curvature flags (EULERIAN_FLUID_SOLVER_3D.h:758-759,
+    // both false by default: no curvature at all)
+    {
+        const bool by_normals = fb.GetBoolean("curvature_by_normal_vector", false),
+                   by_levelset = fb.GetBoolean("curvature_by_levelset", false);
+        P.curvature_method = by_normals ? (by_levelset ? FS_CURV_BOTH : FS_CURV_NORMALS)
+                                        : (by_levelset ? FS_CURV_LEVELSET : FS_CURV_NONE);
+    }
 
     // ADVECTION_3D::InitializeFromBlock, ADVECTION_3D.h:100-113
     const fs::ScriptBlock& ls = fb.Find("LEVELSET_ADVECTION");

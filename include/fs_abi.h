@@ -73,6 +73,12 @@ typedef struct fs_grid {
 
 /* The script keys of the hot path (EULERIAN_FLUID_SOLVER_3D.h:192-260,
  * ADVECTION_3D.h:102-113, PROJECTION_3D.h:178-227), air/water mode. */
+/* fs_params.curvature_method */
+#define FS_CURV_LEVELSET 0
+#define FS_CURV_NORMALS  1
+#define FS_CURV_BOTH     2
+#define FS_CURV_NONE     3
+
 typedef struct fs_params {
     /* phase labels: 1 = air_bubble_rising (phi <= 0 is the light phase),
      * 0 = water_drop (phi <= 0 is water); PROJECTION_3D.h:605-639 */
@@ -126,6 +132,19 @@ typedef struct fs_params {
      * air/water path the model removes the surface-tension jump and adds no
      * force, as the reference does.  The CFL's surface-tension term stays. */
     int    csf_model;
+    /* How LEVELSET_3D::ComputeCurvatures forms the curvature
+     * (curvature_by_normal_vector / curvature_by_levelset,
+     * EULERIAN_FLUID_SOLVER_3D.h:758-759, LEVELSET_3D.h:963-1082):
+     * FS_CURV_LEVELSET (0, the shipped scripts' choice) from the level set's
+     * derivatives; FS_CURV_NORMALS the divergence of the unit normals, clamped
+     * to -1/min(dx,dy,dz) where |kappa| exceeds 1/min(dx,dy,dz) (its maximum
+     * over all cells, unclamped, feeds the CFL); FS_CURV_BOTH the normals'
+     * pass, then the level set's, which overwrites kappa (the maximum is over
+     * both); FS_CURV_NONE neither (kappa and the jump stay zero).  In the
+     * compatible schedule the normals are the ones the last projection's
+     * ComputeNormals left (PROJECTION_3D.h:803; zero before the first); the
+     * fixed schedule forms them from the current level set first. */
+    int    curvature_method;
 } fs_params;
 
 /* Per-step (or per-phase) statistics. */

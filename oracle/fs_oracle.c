@@ -91,6 +91,8 @@ struct fso_state {
     const double* jc_override;
     /* PROJECTION_3D::CSF_model, set by the first Sourcing with the CSF model */
     int csf_on;
+    /* LEVELSET_3D::normal (unit normals, zero until the first ComputeNormals) */
+    double* nrm[3];
 };
 
 /* A CSR system with rows numbered plane by plane (per rows in each of
@@ -154,6 +156,7 @@ int fso_create(const fs_params* P, const fs_grid* g, fso_state** out) {
     if (P->surface_tension != 0.0) {
         s->kappa = zalloc(n);
         s->jc = zalloc(n);
+        s->nrm[0] = zalloc(n); s->nrm[1] = zalloc(n); s->nrm[2] = zalloc(n);
     }
     s->bc = (int*)calloc((size_t)n, sizeof(int));
     s->plane_part = (double*)calloc((size_t)s->nz + 1, sizeof(double));
@@ -173,6 +176,7 @@ void fso_destroy(fso_state* s) {
     free(s->phi); free(s->sghost); free(s->u); free(s->v); free(s->w); free(s->gu); free(s->gv); free(s->gw);
     free(s->p); free(s->div); free(s->rho); free(s->irho); free(s->sgn); free(s->tmp); free(s->pg); free(s->vtmp); free(s->fbx); free(s->fby); free(s->fbz);
     free(s->kappa); free(s->jc);
+    free(s->nrm[0]); free(s->nrm[1]); free(s->nrm[2]);
     free(s->bc);
     free(s->urk); free(s->vrk); free(s->wrk);
     free(s->plane_part);
@@ -441,8 +445,50 @@ static void eno3_stage(fso_state* s, double* rho, const double* ghost, int comp,
 
 static int is_fixed_schedule(const fso_state* s) { return s->P.schedule == FS_SCHEDULE_FIXED; }
 
-/* Curvature by level set, LEVELSET_3D::ComputeCurvatures (LEVELSET_3D.h:
- * 1001-1080), on phi with the ghosts the caller left.  The derivative fields
+static int curv_normals(const fso_state* s) {
+    return s->P.curvature_method == FS_CURV_NORMALS || s->P.curvature_method == FS_CURV_BOTH;
+}
+static int curv_levelset(const fso_state* s) {
+    return s->P.curvature_method == FS_CURV_LEVELSET || s->P.curvature_method == FS_CURV_BOTH;
+}
+
+/* The unit normals of ComputeNormals, LEVELSET_3D.h:933-955: central
+ * differences of phi, forward ones where those vanish, each vector scaled by
+ * the reciprocal of its magnitude unless that is zero (VECTOR_3D::Normalize,
+ * VECTOR_3D.h:153-165).  phi's ghosts are read as its constant fill (:931). */
+static void normals_from_phi(fso_state* s) {
+    const gc_t* g = &s->cell;
+    const double* phi = s->phi;
+#define PC(a, b, c) A3(phi, clampi(a, 0, s->nx - 1), clampi(b, 0, s->ny - 1), clampi(c, 0, s->nz - 1))
+    for (int k = 0; k < s->nz; k++)
+        for (int j = 0; j < s->ny; j++)
+            for (int i = 0; i < s->nx; i++) {
+                double x = (PC(i + 1, j, k) - PC(i - 1, j, k)) * g->x.inv2d;
+                double y = (PC(i, j + 1, k) - PC(i, j - 1, k)) * g->y.inv2d;
+                double z = (PC(i, j, k + 1) - PC(i, j, k - 1)) * g->z.inv2d;
+                double mag = sqrt(x * x + y * y + z * z);
+                if (!(mag != 0)) {
+                    x = (PC(i + 1, j, k) - PC(i, j, k)) * g->x.inv;
+                    y = (PC(i, j + 1, k) - PC(i, j, k)) * g->y.inv;
+                    z = (PC(i, j, k + 1) - PC(i, j, k)) * g->z.inv;
+                }
+                double m = sqrt(x * x + y * y + z * z);
+                if (m != 0) {
+                    double r = (double)1 / m;
+                    x *= r;
+                    y *= r;
+                    z *= r;
+                }
+                A3(s->nrm[0], i, j, k) = x;
+                A3(s->nrm[1], i, j, k) = y;
+                A3(s->nrm[2], i, j, k) = z;
+            }
+#undef PC
+}
+
+/* LEVELSET_3D::ComputeCurvatures (LEVELSET_3D.h:963-1082): by the normals
+ * (:974-996) when curvature_method asks for them, then by the level set
+ * (:1001-1080), on phi with the ghosts the caller left.  The derivative fields
  * phi_x .. phi_yz of :1005-1013 are folded into the cell loop (each is read
  * only at its own cell).  The compatible schedule keeps the reference's
  * cross term "py pz pyz" (:1033, the mean-curvature formula has 2 py pz pyz)
@@ -458,6 +504,30 @@ static void curvature(fso_state* s, double* jc) {
     int exact = is_fixed_schedule(s);
     double abs_max_curvature = 0;
     s->curv_fail = 0;
+    if (curv_normals(s)) {
+        /* LEVELSET_3D.h:974-996: the divergence of the normals (their ghost
+         * layer the constant fill, :976), |curv| above the tolerance taken as
+         * -tolerance, the maximum of |curv| over every cell */
+        if (exact) normals_from_phi(s);
+        double* const* n = s->nrm;
+#define NC(f, a, b, c) A3(f, clampi(a, 0, s->nx - 1), clampi(b, 0, s->ny - 1), clampi(c, 0, s->nz - 1))
+        for (int k = 0; k < s->nz; k++)
+            for (int j = 0; j < s->ny; j++)
+                for (int i = 0; i < s->nx; i++) {
+                    double curv = (NC(n[0], i + 1, j, k) - NC(n[0], i - 1, j, k)) * g->x.inv2d;
+                    curv += (NC(n[1], i, j + 1, k) - NC(n[1], i, j - 1, k)) * g->y.inv2d;
+                    curv += (NC(n[2], i, j, k + 1) - NC(n[2], i, j, k - 1)) * g->z.inv2d;
+                    double kv = fabs(curv) <= tolerance ? curv : -tolerance;
+                    A3(s->kappa, i, j, k) = kv;
+                    if (jc) A3(jc, i, j, k) = s->P.surface_tension * kv;
+                    abs_max_curvature = fabs(curv) > abs_max_curvature ? fabs(curv) : abs_max_curvature;
+                }
+#undef NC
+    }
+    if (!curv_levelset(s)) {
+        s->kmax = abs_max_curvature;
+        return;
+    }
     for (int k = 0; k < s->nz; k++)
         for (int j = 0; j < s->ny; j++)
             for (int i = 0; i < s->nx; i++) {
@@ -526,13 +596,23 @@ int fso_get_curvature(fso_state* s, double* kappa, double* abs_max_curvature) {
  * off: PROJECTION_3D.h:750). */
 static const double* jump_field(const fso_state* s) {
     if (s->jc_override) return s->jc_override;
-    return s->P.surface_tension != 0.0 && s->P.use_delta_function_formulation && !s->csf_on ? s->jc : NULL;
+    return s->P.surface_tension != 0.0 && s->P.use_delta_function_formulation && !s->csf_on &&
+                   s->P.curvature_method != FS_CURV_NONE
+               ? s->jc
+               : NULL;
 }
 
 /* LEVELSET_3D::ComputeNormals, LEVELSET_3D.h:929-956: lasting effect is the
  * constant re-extrapolation of phi's ghost cells (:931); normals only feed
  * jump terms that vanish with zero surface tension. */
-static void compute_normals(fso_state* s) { fill_const(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0); }
+/* LEVELSET_3D::ComputeNormals, LEVELSET_3D.h:929-956: the lasting effect on
+ * phi is its constant ghost fill; the normals themselves are kept only where
+ * the curvature is taken from them (compatible schedule: read by the next
+ * ComputeCurvatures). */
+static void compute_normals(fso_state* s) {
+    fill_const(s, s->phi, s->phi, s->nx, s->ny, s->nz, 2, 0);
+    if (s->P.surface_tension != 0.0 && curv_normals(s) && s->P.schedule != FS_SCHEDULE_FIXED) normals_from_phi(s);
+}
 
 static int is_fixed(const fso_state* s) { return s->P.schedule == FS_SCHEDULE_FIXED; }
 

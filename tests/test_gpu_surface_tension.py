@@ -161,3 +161,50 @@ def test_frames_with_csf_model(hip_lib, oracle_lib, sched):
         assert rel_err(fg[c], fc[c]) < 1e-8, (c, rel_err(fg[c], fc[c]))
     for c in ("u", "v", "w"):
         assert np.linalg.norm(fg[c] - fc[c]) / scale < 1e-8, c
+
+
+@pytest.mark.parametrize("method", [_abi.FS_CURV_NORMALS, _abi.FS_CURV_BOTH, _abi.FS_CURV_NONE])
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_curvature_methods_bit_exact(hip_lib, oracle_lib, sched, method):
+    """curvature_method: by normals, both passes, or none, bit for bit with the
+    checker -- before any projection (compatible schedule: zero normals) and
+    after one (its ComputeNormals), on a ragged grid that reaches the clamp."""
+    g = _abi.unit_cube_grid(23, 19, 21)
+    k, j, i = np.meshgrid(np.arange(21), np.arange(19), np.arange(23), indexing="ij")
+    x, y, z = (i + .5) / 23, (j + .5) / 19, (k + .5) / 21
+    phi = np.minimum(np.sqrt((x - .5) ** 2 + (y - .45) ** 2 + (z - .55) ** 2) - 0.27,
+                     np.sqrt((x - .2) ** 2 + (y - .7) ** 2 + (z - .3) ** 2) - 0.6 / 21)
+    gpu, cpu = _pair_on(hip_lib, oracle_lib, phi, g, schedule=sched, curvature_method=method,
+                        stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=1e-10)
+    for when in ("before", "after"):
+        (kg, mg), (kc, mc) = gpu.curvature(), cpu.curvature()
+        assert np.array_equal(kg, kc), (when, rel_err(kg, kc))
+        assert mg == mc, (when, mg, mc)
+        if when == "before":
+            for s in (gpu, cpu):
+                s.projection(1e-3, allow_not_converged=True)
+    if method == _abi.FS_CURV_NONE:
+        assert not kg.any()
+
+
+@pytest.mark.parametrize("sched", SCHEDULES)
+def test_frames_with_curvature_by_normals(hip_lib, oracle_lib, sched):
+    """Two CFL frames with sigma = 0.0728 and the curvature by normals: the
+    first CFL step bit for bit, the fields to 1e-8 of the checker when each
+    solve reaches 4e-11 relative residual."""
+    over = dict(surface_tension=SIGMA, schedule=sched, stop_rule=_abi.FS_STOP_REL_RESIDUAL, rel_tolerance=4e-11,
+                curvature_method=_abi.FS_CURV_NORMALS)
+    gpu, cpu = make_pair(hip_lib, oracle_lib, "rising_drop", n=20, **over)
+    for f in range(2):
+        dg, dc = gpu.cfl_dt(0.5, 1), cpu.cfl_dt(0.5, 1)
+        assert dg == dc if f == 0 else abs(dg - dc) < 1e-8 * dc, (f, dg, dc)
+        gpu.advance_frame(dc)
+        cpu.advance_frame(dc)
+    fg, fc = gpu.get_fields(), cpu.get_fields()
+    fg["p"] -= fg["p"].mean()
+    fc["p"] -= fc["p"].mean()
+    scale = max(np.linalg.norm(fc[c]) for c in ("u", "v", "w"))
+    for c in ("phi", "p"):
+        assert rel_err(fg[c], fc[c]) < 1e-8, (c, rel_err(fg[c], fc[c]))
+    for c in ("u", "v", "w"):
+        assert np.linalg.norm(fg[c] - fc[c]) / scale < 1e-8, c

@@ -210,3 +210,66 @@ def test_csf_model_in_air_water_mode_removes_the_jump(oracle_lib, sched):
     with_jump_no_sourcing = run(surface_tension=0.0728, is_sourcing_active=0)
     for c in ("phi", "u", "v", "w", "p"):
         assert np.array_equal(no_sourcing[c], with_jump_no_sourcing[c]), c
+
+
+def test_curvature_by_normals_on_a_sphere(oracle_lib):
+    """Curvature by normals (LEVELSET_3D.h:974-996) on a sphere's signed
+    distance, normals formed from it (the fixed schedule): the divergence of
+    the unit normal converges to +2/r at second order -- the opposite sign
+    of the curvature by level set (-2/r), as the reference's two forms
+    stand; its maximum is over every cell and unclamped; a plane gives 0
+    exactly."""
+    errs = []
+    for n in (24, 48):
+        x, y, z = scenes.cell_centres(n)
+        r = np.sqrt((x - .5) ** 2 + (y - .5) ** 2 + (z - .5) ** 2)
+        s = _solver(oracle_lib, n, r - 0.25, curvature_method=_abi.FS_CURV_NORMALS)
+        kap, kmax = s.curvature()
+        band = _interior(n) & (r > 0.15) & (r < 0.35)
+        e = np.abs(kap[band] - 2 / r[band]) * r[band] / 2
+        errs.append(e.mean())
+        assert e.max() < 40 / n ** 2, (n, e.max())
+        assert kmax >= np.abs(kap).max()
+        lev, _ = _solver(oracle_lib, n, r - 0.25).curvature()
+        assert (np.abs(kap[band] + lev[band]) * r[band] / 2).max() < 60 / n ** 2
+    assert errs[0] / errs[1] > 3.5, errs
+    n = 16
+    x, y, z = scenes.cell_centres(n)
+    kap, kmax = _solver(oracle_lib, n, z - 0.3, curvature_method=_abi.FS_CURV_NORMALS).curvature()
+    assert not kap.any() and kmax == 0
+
+
+def test_curvature_by_normals_clamp_and_methods(oracle_lib):
+    """|div n| above 1/min(dx,dy,dz) becomes -1/min(...) whatever its sign
+    (:985-992); FS_CURV_BOTH leaves the level set's curvature, with the
+    maximum over both passes; FS_CURV_NONE computes none (kappa 0, max 0)."""
+    n = 16
+    x, y, z = scenes.cell_centres(n)
+    phi = np.sqrt((x - 0.53) ** 2 + (y - 0.47) ** 2 + (z - 0.5) ** 2) - 0.6 / n
+    kap, kmax = _solver(oracle_lib, n, phi, curvature_method=_abi.FS_CURV_NORMALS).curvature()
+    assert kap.min() == -float(n) and kap.max() <= n
+    assert kmax > n  # the unclamped divergence next to the tiny sphere
+    lev, lmax = _solver(oracle_lib, n, phi).curvature()
+    both, bmax = _solver(oracle_lib, n, phi, curvature_method=_abi.FS_CURV_BOTH).curvature()
+    assert np.array_equal(both, lev)
+    assert bmax == max(kmax, lmax)
+    none, nmax = _solver(oracle_lib, n, phi, curvature_method=_abi.FS_CURV_NONE).curvature()
+    assert not none.any() and nmax == 0
+
+
+def test_curvature_by_normals_compat_reads_the_last_projections_normals(oracle_lib):
+    """The compatible schedule reads the normals the last projection's
+    ComputeNormals left (PROJECTION_3D.h:803): zero before any projection
+    (curvature 0), and after one projection the divergence of the normals
+    of the level set that projection saw -- bit for bit the fixed
+    schedule's curvature of that level set."""
+    n = 20
+    x, y, z = scenes.cell_centres(n)
+    phi = np.sqrt((x - .5) ** 2 + (y - .45) ** 2 + (z - .55) ** 2) - 0.27
+    s = _solver(oracle_lib, n, phi, schedule=_abi.FS_SCHEDULE_COMPAT, curvature_method=_abi.FS_CURV_NORMALS)
+    kap0, kmax0 = s.curvature()
+    assert not kap0.any() and kmax0 == 0
+    s.projection(1e-3, allow_not_converged=True)  # only its ComputeNormals matters here
+    kap1, kmax1 = s.curvature()
+    ref, rmax = _solver(oracle_lib, n, phi, curvature_method=_abi.FS_CURV_NORMALS).curvature()
+    assert np.array_equal(kap1, ref) and kmax1 == rmax
