@@ -1006,7 +1006,9 @@ class EulerianFluidSolver3D {
         // bubbles 263 -> 133 per frame at 256^3) for twice the smoothing
         {
             const double hi = std::max(P.water_density, P.air_density), lo = std::min(P.water_density, P.air_density);
-            mg.SetSweeps(lo > 0 && hi / lo < 10 ? FS_MG_SWEEPS_LOW : FS_MG_SWEEPS_HIGH);
+            int nu = lo > 0 && hi / lo < 10 ? FS_MG_SWEEPS_LOW : FS_MG_SWEEPS_HIGH;
+            if (const char* e = std::getenv("FS_MG_SWEEPS")) nu = std::max(1, std::atoi(e));  // tuning override
+            mg.SetSweeps(nu);
         }
         mg.probe = sampling ? &kprobe[FS_KS_MG_SMOOTH] : nullptr;
         mg.SetOperator(st, bx32, by32, bz32);

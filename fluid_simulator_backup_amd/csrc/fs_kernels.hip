@@ -784,46 +784,30 @@ void fs_source_sphere(hipStream_t st, double* phi, const FsLayout& L, const int 
                        cell.x.d * (double)3, mode);
 }
 
-// Neumann mean removal, POISSON_SOLVER_3D.h:225-246.  The sum runs in row
-// order inside each z-plane, then over planes in order (one slab per plane).
-// One workgroup per plane: rows are staged through LDS with coalesced loads
-// and one thread adds them in row order.
-#define PS_CHUNK 4096
+// Neumann mean removal, POISSON_SOLVER_3D.h:225-246.
+// One workgroup per plane: each thread adds whole rows in x order (its row's
+// cache lines stream through L1), the row sums go to LDS, and one thread adds
+// them in row order.  The order (rows, then the plane's rows, then planes) is
+// the oracle's; the reference's own order depends on its thread count
+// (per-thread slabs, END_GRID_ITERATION_SUM).
+#define PS_ROWS 4096
 __global__ void __launch_bounds__(256) k_plane_sums(const double* __restrict__ rhs, double* __restrict__ part,
                                                     FsLayout L) {
-    __shared__ double buf[PS_CHUNK];
-    int k = blockIdx.x;
-    int rows = PS_CHUNK / L.nx;
-    if (rows < 1) rows = 1;
+    __shared__ double rows[PS_ROWS];
+    const int k = blockIdx.x;
     double s = 0;
-    for (int j0 = 0; j0 < L.ny; j0 += rows) {
-        int nrow = min(rows, L.ny - j0);
-        int n = nrow * L.nx;
+    for (int j0 = 0; j0 < L.ny; j0 += PS_ROWS) {
+        const int nr = min(PS_ROWS, L.ny - j0);
         __syncthreads();
-        for (int t = threadIdx.x; t < n && t < PS_CHUNK; t += blockDim.x) {
-            int jj = t / L.nx, ii = t - jj * L.nx;
-            buf[t] = rhs[L.idx(ii, j0 + jj, k)];
+        for (int jj = threadIdx.x; jj < nr; jj += blockDim.x) {
+            const double* row = rhs + L.idx(0, j0 + jj, k);
+            double r = 0;
+            for (int i = 0; i < L.nx; i++) r += row[i];
+            rows[jj] = r;
         }
         __syncthreads();
-        if (threadIdx.x == 0) {
-            if (L.nx <= PS_CHUNK) {
-                // in order; eight LDS reads issued ahead of their adds
-                int t = 0;
-                for (; t + 8 <= n; t += 8) {
-                    double v[8];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) v[u] = buf[t + u];
-#pragma unroll
-                    for (int u = 0; u < 8; u++) s += v[u];
-                }
-                for (; t < n; t++) s += buf[t];
-            } else {  // rows wider than the stage: read straight from memory
-                for (int jj = 0; jj < nrow; jj++) {
-                    const double* row = rhs + L.idx(0, j0 + jj, k);
-                    for (int i = 0; i < L.nx; i++) s += row[i];
-                }
-            }
-        }
+        if (threadIdx.x == 0)
+            for (int jj = 0; jj < nr; jj++) s += rows[jj];
     }
     if (threadIdx.x == 0) part[k] = s;
 }

@@ -1264,12 +1264,18 @@ static void assemble(fso_state* s) {
             }
     s->rp[N] = nz_ix;
     s->nnz = nz_ix;
-    /* Neumann compatibility: b -= mean(b), POISSON_SOLVER_3D.h:225-246 */
+    /* Neumann compatibility: b -= mean(b), POISSON_SOLVER_3D.h:225-246.  The
+     * reference adds per-thread slabs (its order depends on the thread
+     * count); here each row in x order, the rows of a plane in order, the
+     * planes in order. */
     double sum = 0;
-    int per = s->nx * s->ny;
     for (int k = 0; k < s->nz; k++) {
         double part = 0;
-        for (int r = k * per; r < (k + 1) * per; r++) part += s->b[r];
+        for (int j = 0; j < s->ny; j++) {
+            double rowsum = 0;
+            for (int r = (k * s->ny + j) * s->nx; r < (k * s->ny + j + 1) * s->nx; r++) rowsum += s->b[r];
+            part += rowsum;
+        }
         sum += part;
     }
     double one_over_b_dimension = 1 / (double)N;
