@@ -658,3 +658,186 @@ class World:
         if self.h:
             self.lib.fs_world_destroy(self.h)
             self.h = C.c_void_p()
+
+
+# ---- oil/water pipe mode ---------------------------------------------------
+
+FS_PIPE_LEVELSET_ADVECTION, FS_PIPE_VELOCITY_ADVECTION, FS_PIPE_GRAVITY = 0, 1, 2
+FS_PIPE_SOURCING, FS_PIPE_PROJECTION, FS_PIPE_REINITIALIZATION = 3, 4, 5
+PIPE_ARRAYS = ["phi", "scalar_ghost", "ghost_u", "ghost_v", "ghost_w",
+               "adv_u_y", "adv_u_z", "adv_v_y", "adv_v_z", "adv_w_y", "adv_w_z",
+               "vis_u_y", "vis_u_z", "vis_v_y", "vis_v_z", "vis_w_y", "vis_w_z",
+               "boundary", "divergence", "jump", "pressure"]
+_PIPE_ARRAY_COMP = {"ghost_u": 1, "ghost_v": 2, "ghost_w": 3, "adv_u_y": 1, "adv_u_z": 1, "adv_v_y": 2,
+                    "adv_v_z": 2, "adv_w_y": 3, "adv_w_z": 3, "vis_u_y": 1, "vis_u_z": 1, "vis_v_y": 2,
+                    "vis_v_z": 2, "vis_w_y": 3, "vis_w_z": 3}
+
+
+class FsPipeParams(C.Structure):
+    _fields_ = [("is_parallel", _I), ("a", _D), ("R2", _D), ("A_0", _D), ("alpha", _D),
+                ("m", _D), ("eta", _D), ("K", _D), ("g", _D),
+                ("oil_density", _D), ("oil_viscosity", _D),
+                ("dimensionless_form", _I), ("cylindrical_boundary", _I), ("boundary_order_velocity", _I)]
+
+
+def pipe_params(**kw):
+    """fs_params and fs_pipe_params of the reference's oil/water script
+    (3D_script.sim:32-40, 195-290): core-annular flow, dimensionless form."""
+    p = default_params(water_density=0.995, water_viscosity=0.01, surface_tension=8.54,
+                       gravity=(0.0, -980.0, 0.0), order_for_time_advancing=2,
+                       scaling_number_for_reinitialization=2, iteration_number_for_reinitialization=10,
+                       tolerance=1e-6)
+    q = FsPipeParams()
+    q.is_parallel = 1
+    q.a, q.R2, q.A_0, q.alpha = 1.28, 0.47625, 0.001, 2.0
+    q.m, q.eta, q.K, q.g = 0.00166, 1.1, -0.4552, -980.0
+    q.oil_density, q.oil_viscosity = 0.905, 6.01
+    q.dimensionless_form, q.cylindrical_boundary, q.boundary_order_velocity = 1, 1, 1
+    for k, v in kw.items():
+        if hasattr(q, k) and k not in ("water_density",):
+            setattr(q, k, v)
+        elif k == "gravity":
+            for i in range(3):
+                p.gravity[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p, q
+
+
+def pipe_grid(n=51):
+    """GRID_STRUCTURE_3D_PARALLEL of the script (3D_script.sim:86-93) at n^3."""
+    g = FsGrid()
+    g.nx = g.ny = g.nz = n
+    g.x_min, g.y_min, g.z_min = 0.0, -1.570796, -1.570796
+    g.x_max, g.y_max, g.z_max = 3.141592, 1.570796, 1.570796
+    return g
+
+
+def bind_pipe(lib, prefix):
+    vp = C.c_void_p
+    if prefix == "fs_":
+        lib.fs_create_pipe.argtypes = [C.POINTER(FsParams), C.POINTER(FsPipeParams), C.POINTER(FsGrid), _I,
+                                       C.POINTER(vp)]
+        lib.fs_last_error.argtypes = [vp]
+        lib.fs_last_error.restype = C.c_char_p
+        names = {"init": "fs_pipe_initial_fields", "constants": "fs_pipe_constants", "phase": "fs_pipe_phase",
+                 "array": "fs_pipe_get_array", "step": "fs_step", "frame": "fs_advance_frame",
+                 "set": "fs_set_fields", "get": "fs_get_fields", "destroy": "fs_destroy"}
+        getattr(lib, names["init"]).argtypes = [vp]
+        getattr(lib, names["constants"]).argtypes = [vp, _PD]
+        lib.fs_cfl_dt.argtypes = [vp, _D, _I, C.POINTER(_D)]
+    else:
+        lib.fsop_create.argtypes = [C.POINTER(FsParams), C.POINTER(FsPipeParams), C.POINTER(FsGrid), C.POINTER(vp)]
+        names = {"init": "fsop_initial_fields", "constants": "fsop_constants", "phase": "fsop_phase",
+                 "array": "fsop_get_array", "step": "fsop_step", "frame": "fsop_advance_frame",
+                 "set": "fsop_set_fields", "get": "fsop_get_fields", "destroy": "fsop_destroy"}
+        lib.fsop_initial_fields.argtypes = [vp]
+        lib.fsop_constants.argtypes = [vp, _PD]
+        lib.fsop_cfl_dt.argtypes = [vp, _D, C.POINTER(_D)]
+        lib.fsop_compute_curvatures.argtypes = [vp]
+    getattr(lib, names["phase"]).argtypes = [vp, _I, _D, C.POINTER(FsStepStats)]
+    getattr(lib, names["array"]).argtypes = [vp, _I, _PD]
+    getattr(lib, names["step"]).argtypes = [vp, _D, C.POINTER(FsStepStats)]
+    getattr(lib, names["frame"]).argtypes = [vp, _D, C.POINTER(FsStepStats)]
+    getattr(lib, names["set"]).argtypes = [vp, _PD, _PD, _PD, _PD, _PD]
+    getattr(lib, names["get"]).argtypes = [vp, _PD, _PD, _PD, _PD, _PD]
+    getattr(lib, names["destroy"]).argtypes = [vp]
+    getattr(lib, names["destroy"]).restype = None
+    return names
+
+
+class Pipe:
+    """One oil/water pipe scene on either library: the HIP library's
+    fs_create_pipe handle (prefix "fs_") or the CPU checker's (prefix "fsop_")."""
+
+    PHASES = {"levelset_advection": FS_PIPE_LEVELSET_ADVECTION, "velocity_advection": FS_PIPE_VELOCITY_ADVECTION,
+              "gravity": FS_PIPE_GRAVITY, "sourcing": FS_PIPE_SOURCING, "projection": FS_PIPE_PROJECTION,
+              "reinitialization": FS_PIPE_REINITIALIZATION}
+
+    def __init__(self, lib, prefix, params, pipe, grid, device=0):
+        self.lib, self.prefix, self.params, self.pipe, self.grid = lib, prefix, params, pipe, grid
+        self.nx, self.ny, self.nz = grid.nx, grid.ny, grid.nz
+        self.names = bind_pipe(lib, prefix)
+        h = C.c_void_p()
+        if prefix == "fs_":
+            rc = lib.fs_create_pipe(C.byref(params), C.byref(pipe), C.byref(grid), device, C.byref(h))
+        else:
+            rc = lib.fsop_create(C.byref(params), C.byref(pipe), C.byref(grid), C.byref(h))
+        if rc != FS_OK:
+            raise FsError(rc, "create_pipe failed")
+        self.h = h
+
+    def _call(self, key, *args, allow=()):
+        rc = getattr(self.lib, self.names[key])(self.h, *args)
+        if rc != FS_OK and rc not in allow:
+            msg = key
+            if self.prefix == "fs_":
+                m = self.lib.fs_last_error(self.h)
+                msg = f"{key}: {m.decode() if m else ''}"
+            raise FsError(rc, msg)
+        return rc
+
+    def close(self):
+        if getattr(self, "h", None):
+            getattr(self.lib, self.names["destroy"])(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, comp):
+        return Solver.shape(self, comp)
+
+    def initial_fields(self):
+        self._call("init")
+
+    def constants(self):
+        out = np.empty(7)
+        self._call("constants", _ptr(out))
+        return dict(zip(("R1", "f", "A", "V0", "J", "Re", "We"), out.tolist()))
+
+    def set_fields(self, phi=None, u=None, v=None, w=None, p=None):
+        conv = lambda a: None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+        arrs = list(map(conv, (phi, u, v, w, p)))
+        self._call("set", *[_ptr(a) for a in arrs])
+
+    def get_fields(self):
+        out = {c: np.empty(self.shape(c)) for c in ("phi", "u", "v", "w", "p")}
+        self._call("get", *[_ptr(out[c]) for c in ("phi", "u", "v", "w", "p")])
+        return out
+
+    def phase(self, name, dt):
+        st = FsStepStats()
+        self._call("phase", self.PHASES[name], dt, C.byref(st))
+        return st.as_dict()
+
+    def step(self, dt):
+        st = FsStepStats()
+        self._call("step", dt, C.byref(st))
+        return st.as_dict()
+
+    def advance_frame(self, dt):
+        st = FsStepStats()
+        self._call("frame", dt, C.byref(st))
+        return st.as_dict()
+
+    def cfl_dt(self, cfl):
+        d = C.c_double()
+        if self.prefix == "fs_":
+            rc = self.lib.fs_cfl_dt(self.h, cfl, 0, C.byref(d))
+        else:
+            rc = self.lib.fsop_cfl_dt(self.h, cfl, C.byref(d))
+        if rc != FS_OK:
+            raise FsError(rc, "cfl_dt")
+        return d.value
+
+    def array(self, name):
+        """A persistent array over its box grown by 3 ghost layers, [k, j, i]."""
+        c = _PIPE_ARRAY_COMP.get(name, 0)
+        ni, nj, nk = self.nx + (c == 1), self.ny + (c == 2), self.nz + (c == 3)
+        out = np.empty((nk + 6, nj + 6, ni + 6))
+        self._call("array", PIPE_ARRAYS.index(name), _ptr(out))
+        return out

@@ -27,6 +27,7 @@
 #include "fs_mg.h"
 #include "fs_probe.h"
 #include "fs_cfl.h"
+#include "fs_pipe.h"
 
 #ifndef FS_FP32_RESTART
 #define FS_FP32_RESTART 4000  // FS_FP32_REFINE: restart a recursion that dropped under 100x in this many iterations
@@ -1881,6 +1882,7 @@ class EulerianFluidSolver3D {
 
 struct fs_solver {
     EulerianFluidSolver3D* impl = nullptr;
+    FsPipe* pipe = nullptr;  // the oil/water pipe mode (fs_create_pipe)
     std::string err;
 };
 
@@ -1962,6 +1964,10 @@ struct FileCloser {
 
 template <typename F>
 static int guarded(fs_solver* s, F&& f) {
+    if (s && s->pipe) {
+        s->err = "not available for the oil/water pipe mode";
+        return FS_ERR_UNSUPPORTED;
+    }
     if (!s || !s->impl) return FS_ERR_BAD_ARG;
     try {
         int rc = f(*s->impl);
@@ -1976,7 +1982,64 @@ static int guarded(fs_solver* s, F&& f) {
     }
 }
 
+// a call on a pipe-mode handle: FsPipeError and HIP failures to codes
+template <typename F>
+static int pipe_guarded(fs_solver* s, F&& f) {
+    try {
+        return f(*s->pipe);
+    } catch (FsPipeError& e) {
+        s->err = e.msg;
+        return e.code;
+    } catch (HipError& e) {
+        s->err = std::string(e.what) + ": " + hipGetErrorString(e.e);
+        return FS_ERR_HIP;
+    }
+}
+
 extern "C" {
+
+int fs_create_pipe(const fs_params* params, const fs_pipe_params* pipe, const fs_grid* grid, int device,
+                   fs_solver** out) {
+    if (!params || !pipe || !grid || !out) return FS_ERR_BAD_ARG;
+    *out = nullptr;
+    try {
+        FsPipe* p = fs_pipe_new(*params, *pipe, *grid, device);
+        auto* s = new fs_solver();
+        s->pipe = p;
+        *out = s;
+        return FS_OK;
+    } catch (FsPipeError& e) {
+        return e.code;
+    }
+}
+
+int fs_pipe_initial_fields(fs_solver* s) {
+    if (!s || !s->pipe) return FS_ERR_BAD_ARG;
+    return pipe_guarded(s, [&](FsPipe& P) {
+        fs_pipe_initial(&P);
+        return FS_OK;
+    });
+}
+
+int fs_pipe_constants(fs_solver* s, double* out) {
+    if (!s || !s->pipe || !out) return FS_ERR_BAD_ARG;
+    fs_pipe_consts(s->pipe, out);
+    return FS_OK;
+}
+
+int fs_pipe_phase(fs_solver* s, int phase, double dt, fs_step_stats* stats) {
+    if (!s || !s->pipe) return FS_ERR_BAD_ARG;
+    if (!(dt > 0) || !std::isfinite(dt)) return FS_ERR_BAD_ARG;
+    return pipe_guarded(s, [&](FsPipe& P) { return fs_pipe_run_phase(&P, phase, dt, stats); });
+}
+
+int fs_pipe_get_array(fs_solver* s, int which, double* dst) {
+    if (!s || !s->pipe || !dst) return FS_ERR_BAD_ARG;
+    return pipe_guarded(s, [&](FsPipe& P) {
+        fs_pipe_array(&P, which, dst);
+        return FS_OK;
+    });
+}
 
 int fs_create(const fs_params* params, const fs_grid* grid, int device, fs_solver** out) {
     if (!params || !grid || !out) return FS_ERR_BAD_ARG;
@@ -2000,6 +2063,7 @@ int fs_create(const fs_params* params, const fs_grid* grid, int device, fs_solve
 
 void fs_destroy(fs_solver* s) {
     if (!s) return;
+    if (s->pipe) fs_pipe_delete(s->pipe);
     delete s->impl;
     delete s;
 }
@@ -2008,6 +2072,11 @@ const char* fs_last_error(const fs_solver* s) { return s ? s->err.c_str() : "nul
 
 int fs_set_fields(fs_solver* s, const double* phi, const double* u, const double* v, const double* w,
                   const double* p) {
+    if (s && s->pipe)
+        return pipe_guarded(s, [&](FsPipe& P) {
+            fs_pipe_set(&P, phi, u, v, w, p);
+            return FS_OK;
+        });
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.SetFields(phi, u, v, w, p);
         return FS_OK;
@@ -2034,6 +2103,10 @@ int fs_get_fields_ex(fs_solver* s, double* phi, double* u, double* v, double* w,
 }
 
 int fs_set_async(fs_solver* s, int async) {
+    if (s && s->pipe) {
+        fs_pipe_async(s->pipe, async != 0);
+        return FS_OK;
+    }
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.async = async != 0;
         return FS_OK;
@@ -2041,6 +2114,11 @@ int fs_set_async(fs_solver* s, int async) {
 }
 
 int fs_synchronize(fs_solver* s) {
+    if (s && s->pipe)
+        return pipe_guarded(s, [&](FsPipe& P) {
+            fs_pipe_sync(&P);
+            return FS_OK;
+        });
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.sync();
         S.check_launch();
@@ -2050,6 +2128,10 @@ int fs_synchronize(fs_solver* s) {
 
 int fs_get_stream(fs_solver* s, void** stream) {
     if (!stream) return FS_ERR_BAD_ARG;
+    if (s && s->pipe) {
+        *stream = (void*)fs_pipe_stream(s->pipe);
+        return FS_OK;
+    }
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         *stream = (void*)S.st;
         return FS_OK;
@@ -2057,6 +2139,11 @@ int fs_get_stream(fs_solver* s, void** stream) {
 }
 
 int fs_get_fields(fs_solver* s, double* phi, double* u, double* v, double* w, double* p) {
+    if (s && s->pipe)
+        return pipe_guarded(s, [&](FsPipe& P) {
+            fs_pipe_get(&P, phi, u, v, w, p);
+            return FS_OK;
+        });
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.GetFields(phi, u, v, w, p);
         return FS_OK;
@@ -2338,15 +2425,24 @@ int fs_batch_advance_frame(fs_batch* b, const double* dt, fs_step_stats* stats, 
 }
 
 int fs_advance_frame(fs_solver* s, double dt, fs_step_stats* stats) {
+    if (s && s->pipe) return pipe_guarded(s, [&](FsPipe& P) { return fs_pipe_advance(&P, dt, stats, true); });
     return guarded(s, [&](EulerianFluidSolver3D& S) { return step_impl(S, dt, stats, true); });
 }
 
 int fs_step(fs_solver* s, double dt, fs_step_stats* stats) {
+    if (s && s->pipe) return pipe_guarded(s, [&](FsPipe& P) { return fs_pipe_advance(&P, dt, stats, false); });
     return guarded(s, [&](EulerianFluidSolver3D& S) { return step_impl(S, dt, stats, false); });
 }
 
 int fs_cfl_dt(fs_solver* s, double cfl, int gravity_norm, double* dt) {
     if (!dt || !(cfl > 0)) return FS_ERR_BAD_ARG;
+    // the pipe mode's CFL uses g_y and its dimensionless forcing as the
+    // reference does (EULERIAN_FLUID_SOLVER_3D.h:2719-2733); gravity_norm is ignored
+    if (s && s->pipe)
+        return pipe_guarded(s, [&](FsPipe& P) {
+            *dt = fs_pipe_cfl(&P, cfl);
+            return FS_OK;
+        });
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         *dt = cfl * S.CFLOneTimeStep(gravity_norm);
         return FS_OK;

@@ -1,0 +1,1715 @@
+// fs_pipe.hip -- the oil/water pipe mode on gfx950: core-annular flow in a
+// cylinder along x (EULERIAN_FLUID_SOLVER_3D in oil_water_simulation mode,
+// is_parallel, cylindrical_boundary, dimensionless_form,
+// semi_implicit_approach; the configuration of the reference's shipped
+// script, 3D_script.sim:32-99, 195-290).
+//
+// The reference's modules keep persistent ghosted arrays: the solver's
+// scalar_field_ghost and vector_field_mac_ghost_x/y/z are written by
+// ADVECTION_3D, VISCOSITY_3D and PROJECTION_3D in turn, and each module has
+// its own per-axis copies velocity_field_mac_ghost_<c>_<axis>.  The cells
+// outside the pipe and the ghost layers keep what the last writer left, and
+// later stencils read them.  Every such array lives here in HBM with the same
+// refresh points, in the padded layout of fs_layout.h (ghost width 3), zeroed
+// at creation as ARRAY_3D::Initialize(..., true) zeroes them.  The arrays the
+// reference computes only on their interior (boundary_phi_<c>, the pipe's
+// level set sqrt(y^2 + z^2) - a at the faces) are static and computed once;
+// their ghost cells read 0, as the reference's do.
+//
+// Every kernel keeps the reference's operation order (-ffp-contract=off).
+// The linear systems (pressure and the three semi-implicit viscosity
+// systems) have a static pattern: the pipe wall fixes which cells are
+// unknowns, so their bc fields and row numbering (k, j, i order over the FULL
+// cells, POISSON_SOLVER_3D.h:527-545) are built once on the host.  Each solve
+// assembles its values row by row on the device into an ELL matrix whose
+// entries keep the reference's CSR order, and runs the Jacobi-PCG of
+// PCG_METHOD.cpp:4-160 with device-resident scalars; dot products use one
+// fixed order (blocks of 256 rows summed by a binary tree, block sums in
+// order), which the CPU checker uses too.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "fs_kernels.h"
+#include "fs_layout.h"
+#include "fs_pipe.h"
+
+#define PBX 64
+#define PBY 4
+#define RBLK 256  // rows per reduction block (the fixed order of the pipe's dot products)
+
+#define BC_FULL 0    // MACROS.h:6
+#define BC_DIR (-1)  // :7
+#define BC_NEUM (-4) // :10
+#define BC_PER (-5)  // :11
+#define BC_OTHER (-6)
+
+namespace {
+
+#define PCHECK(x)                                                                                   \
+    do {                                                                                            \
+        hipError_t _e = (x);                                                                        \
+        if (_e != hipSuccess) throw FsPipeError{FS_ERR_HIP, std::string(#x) + ": " + hipGetErrorString(_e)}; \
+    } while (0)
+
+inline dim3 box_grid(FsBox b, int lo = 0) {
+    return dim3((b.ni + 2 * lo + PBX - 1) / PBX, (b.nj + PBY - 1) / PBY, b.nk);
+}
+
+// the centreline faces ADVECTION_3D::Solve_Velocity holds fixed, is_parallel
+// (ADVECTION_3D.h:279-307); set from creation: every phase that reads them
+// runs after that call in a step
+__host__ __device__ inline bool fixed_face(int c, int j, int k, int ny, int nz) {
+    if (c == 1) return j == (ny - 1) / 2 && k == (nz - 1) / 2;
+    if (c == 2) return (j == ny / 2 || j == ny / 2 + 1) && k == (nz - 1) / 2;
+    if (c == 3) return j == (ny - 1) / 2 && (k == nz / 2 || k == nz / 2 + 1);
+    return false;
+}
+
+__device__ __forceinline__ double theta_of(double nearv, double far) { return fabs(nearv) / (fabs(nearv) + fabs(far)); }
+
+// ---- x-periodic ghost copies, FIELD_STRUCTURE_3D.h:1088-1327 ---------------
+// One thread per cell of the row segment i in [-gw, ni-1+gw].  The first
+// cell (cell fields) or the two end faces (x faces) and the ghosts beyond
+// take the node-periodic image; with a mask (boundary_phi of the
+// component) only where the mask at the written cell is < 1e-8, and the
+// interior copy only where it is <= 0.  The reads and the writes never meet
+// for nx >= 8, so dst == src works in one pass.
+__global__ void k_periodic_x(double* dst, const double* src, const double* __restrict__ mask, FsLayout L, FsBox b,
+                             int c, int gw, int copy) {
+    int i = blockIdx.x * PBX + threadIdx.x - gw, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i > b.ni - 1 + gw || j >= b.nj) return;
+    const int ie = b.ni - 1;
+    const long o = L.idx(i, j, k);
+    const bool left = i <= 0, right = c == 1 ? i >= ie : i >= ie + 1;
+    if (left || right) {
+        if (mask && !(mask[o] < (double)1e-8)) return;
+        int si = left ? (c == 1 ? ie + (i - 1) : ie + i) : (c == 1 ? 0 + (i - (ie - 1)) : 0 + (i - ie));
+        dst[o] = src[L.idx(si, j, k)];
+    } else if (copy) {
+        if (mask && !(mask[o] <= 0)) return;
+        dst[o] = src[o];
+    }
+}
+
+// ---- wall conditions of the velocity ghost copies --------------------------
+
+// ADVECTION_3D::SetupBoundaryConditionForVelocity, level-set branch,
+// is_parallel, y faces (ADVECTION_3D.h:510-531): outside the pipe v = 0 and
+// the ghost copy is zeroed (all four y/z neighbours outside) or extrapolated;
+// the second test repeats `phi(j-1) < 0` and reads the j+1 face, whose value
+// is the one before the loop (gin: a copy taken before it).
+__global__ void k_lsbc_y(double* __restrict__ v, double* __restrict__ g, const double* __restrict__ gin,
+                         const double* __restrict__ b, FsLayout L, FsBox bx) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double b0 = b[o];
+    if (!(b0 > 0)) return;
+    v[o] = 0;
+    double val = 0;
+    bool wr = false;
+    if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0) val = 0, wr = true;
+    if (b[o - sx] < 0) {
+        double th = theta_of(b[o - sx], b0);
+        val = (th - 1) / th * gin[o - sx];
+        wr = true;
+    }
+    if (b[o - sx] < 0) {
+        double th = theta_of(b[o + sx], b0);
+        val = (th - 1) / th * gin[o + sx];
+        wr = true;
+    }
+    if (wr) g[o] = val;
+}
+
+// the same for z faces (ADVECTION_3D.h:533-550): w = 0 outside, the copy
+// zeroed or extrapolated from the inside neighbour below or above (inside
+// faces are never written here, so the loop reads no value it wrote)
+__global__ void k_lsbc_z(double* __restrict__ w, double* __restrict__ g, const double* __restrict__ b, FsLayout L,
+                         FsBox bx) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double b0 = b[o];
+    if (!(b0 > 0)) return;
+    w[o] = 0;
+    if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0) g[o] = 0;
+    if (b[o - sy] < 0) {
+        double th = theta_of(b[o - sy], b0);
+        g[o] = (th - 1) / th * g[o - sy];
+    }
+    if (b[o + sy] < 0) {
+        double th = theta_of(b[o + sy], b0);
+        g[o] = (th - 1) / th * g[o + sy];
+    }
+}
+
+// ADVECTION_3D::SetupBoundaryConditionForVelocity, velocity branch,
+// is_parallel, boundary_order_velocity 1 (ADVECTION_3D.h:716-851): at every
+// crossing inside -> outside along y or z the first outside value of the
+// axis' copy is (theta - 1)/theta times the inside value and the next three
+// are pow(10 t, t) = 400, 27000, 2560000.  Each written cell belongs to one
+// crossing (a y or z line meets the convex pipe in one interval).
+__global__ void k_velbc_cross(double* __restrict__ ay, double* __restrict__ az, const double* __restrict__ g,
+                              const double* __restrict__ b, FsLayout L, FsBox bx) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double b0 = b[o];
+    if (!(b0 < 0)) return;
+    const double sent[3] = {400.0, 27000.0, 2560000.0};
+    const double gv = g[o];
+    if (b[o + sx] > 0) {
+        double th = theta_of(b0, b[o + sx]);
+        ay[o + sx] = (th - 1) / th * gv;
+        for (int t = 2; t <= 4; t++) ay[o + t * sx] = sent[t - 2];
+    }
+    if (b[o - sx] > 0) {
+        double th = theta_of(b0, b[o - sx]);
+        ay[o - sx] = (th - 1) / th * gv;
+        for (int t = 2; t <= 4; t++) ay[o - t * sx] = sent[t - 2];
+    }
+    if (b[o + sy] > 0) {
+        double th = theta_of(b0, b[o + sy]);
+        az[o + sy] = (th - 1) / th * gv;
+        for (int t = 2; t <= 4; t++) az[o + t * sy] = sent[t - 2];
+    }
+    if (b[o - sy] > 0) {
+        double th = theta_of(b0, b[o - sy]);
+        az[o - sy] = (th - 1) / th * gv;
+        for (int t = 2; t <= 4; t++) az[o - t * sy] = sent[t - 2];
+    }
+}
+
+// VISCOSITY_3D::SetupBoundaryConditionsForVelocity(thread_id), is_parallel,
+// boundary_order_velocity 1 (VISCOSITY_3D.h:2255-2352): each face outside
+// the pipe takes, in the reference's else-if order, zero in both copies
+// (all four neighbours outside), the extrapolation from the first inside
+// neighbour j-1, j+1 (y copy), k-1, k+1 (z copy), or zero in both.
+__global__ void k_viscbc(double* __restrict__ ay, double* __restrict__ az, const double* __restrict__ g,
+                         const double* __restrict__ b, FsLayout L, FsBox bx) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double b0 = b[o];
+    if (!(b0 > 0)) return;
+    if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0) {
+        ay[o] = 0;
+        az[o] = 0;
+    } else if (b[o - sx] < -0) {
+        double th = theta_of(b[o - sx], b0);
+        ay[o] = (th - 1) / th * g[o - sx];
+    } else if (b[o + sx] < -0) {
+        double th = theta_of(b[o + sx], b0);
+        ay[o] = (th - 1) / th * g[o + sx];
+    } else if (b[o - sy] < -0) {
+        double th = theta_of(b[o - sy], b0);
+        az[o] = (th - 1) / th * g[o - sy];
+    } else if (b[o + sy] < -0) {
+        double th = theta_of(b[o + sy], b0);
+        az[o] = (th - 1) / th * g[o + sy];
+    } else {
+        ay[o] = 0;
+        az[o] = 0;
+    }
+}
+
+// PROJECTION_3D::SetupBoundaryConditionsForVelocityDivergence, is_parallel,
+// y (st = sx) or z (st = sy) faces (PROJECTION_3D.h:1629-1685): zero, the
+// extrapolation from the inside neighbour along the face's axis, or zero.
+// Inside faces are not written, so reading them in place is safe.
+__global__ void k_divbc(double* __restrict__ g, const double* __restrict__ b, FsLayout L, FsBox bx, long st) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double b0 = b[o];
+    if (!(b0 > 0)) return;
+    if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0)
+        g[o] = 0;
+    else if (b[o - st] < 0) {
+        double th = theta_of(b[o - st], b0);
+        g[o] = (th - (double)1) / th * g[o - st];
+    } else if (b[o + st] < 0) {
+        double th = theta_of(b[o + st], b0);
+        g[o] = (th - (double)1) / th * g[o + st];
+    } else
+        g[o] = 0;
+}
+
+// ---- advection ----------------------------------------------------------------
+
+__device__ __forceinline__ double upwind_p(double rho, double dt, double uu, double vv, double ww, double mx,
+                                           double px, double my, double py, double mz, double pz) {
+    double X = uu > 0 ? mx : px, Y = vv > 0 ? my : py, Z = ww > 0 ? mz : pz;
+    return rho - dt * (uu * X + vv * Y + ww * Z);
+}
+
+// ADVECTION_METHOD_3D::WENO5th with the pipe's level set
+// (ADVECTION_METHOD_3D.h:281-523): cells outside the pipe skipped
+__global__ void __launch_bounds__(PBX* PBY) k_weno5_pipe(double* __restrict__ phi, const double* __restrict__ gh,
+                                                         const double* __restrict__ gu, const double* __restrict__ gv,
+                                                         const double* __restrict__ gw,
+                                                         const double* __restrict__ bls, FsLayout L, double ix,
+                                                         double iy, double iz, double eps, double dt) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long c = L.idx(i, j, k);
+    if (bls[c] > 0) return;
+    double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+    gather7(gh, c, 1, f);
+    fs_weno5(f + 3, ix, eps, mx, px);
+    gather7(gh, c, L.sx, f);
+    fs_weno5(f + 3, iy, eps, my, py);
+    gather7(gh, c, L.sy, f);
+    fs_weno5(f + 3, iz, eps, mz, pz);
+    vel_at(gu, gv, gw, L, 0, i, j, k, uu, vv, ww);
+    phi[c] = upwind_p(phi[c], dt, uu, vv, ww, mx, px, my, py, mz, pz);
+}
+
+// ADVECTION_METHOD_3D::ENO3rd with the pipe (ADVECTION_METHOD_3D.h:
+// 1936-2243): fixed faces and faces with boundary_phi >= 0 keep their
+// value; x derivatives from sxa, y from sya, z from sza
+__global__ void __launch_bounds__(PBX* PBY) k_eno3_pipe(double* __restrict__ rho, const double* __restrict__ sxa,
+                                                        const double* __restrict__ sya, const double* __restrict__ sza,
+                                                        const double* __restrict__ gu, const double* __restrict__ gv,
+                                                        const double* __restrict__ gw, const double* __restrict__ b,
+                                                        FsLayout L, FsBox bx, int comp, FsGridConst gc, double dt) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    if (fixed_face(comp, j, k, L.ny, L.nz)) return;
+    const long c = L.idx(i, j, k);
+    if (!(b[c] < (double)0)) return;
+    const double inv3x = gc.x.inv * (double)1 / 3, inv3y = gc.y.inv * (double)1 / 3, inv3z = gc.z.inv * (double)1 / 3;
+    double f[7], mx, px, my, py, mz, pz, uu, vv, ww;
+    gather7(sxa, c, 1, f);
+    fs_eno3(f + 3, gc.x.d, gc.x.inv, gc.x.inv2d, inv3x, mx, px);
+    gather7(sya, c, L.sx, f);
+    fs_eno3(f + 3, gc.y.d, gc.y.inv, gc.y.inv2d, inv3y, my, py);
+    gather7(sza, c, L.sy, f);
+    fs_eno3(f + 3, gc.z.d, gc.z.inv, gc.z.inv2d, inv3z, mz, pz);
+    vel_at(gu, gv, gw, L, comp, i, j, k, uu, vv, ww);
+    rho[c] = upwind_p(rho[c], dt, uu, vv, ww, mx, px, my, py, mz, pz);
+}
+
+// ---- forces -----------------------------------------------------------------
+
+// sin(x) for |x| <= pi (a hair beyond after rounding): the smoothed
+// Heaviside's one transcendental.  Host and device libm differ in the last
+// bit on about 3% of its arguments, so the pipe mode takes this polynomial
+// on both sides instead: reduction to [0, pi/2] by sin(x) = sin(pi - x) with
+// a two-part pi (the subtraction from the leading part exact by Sterbenz'
+// lemma), then the Taylor series of sin to x^23 (truncation below 1e-20 on
+// [0, pi/2]) by Horner's rule, coefficients (-1)^k/(2k+1)! rounded to
+// double.  Within 2 ulps of sin; no FMA, so every build gives the same bits.
+__device__ __forceinline__ double pipe_sin(double x) {
+    const double c[11] = {-0x1.5555555555555p-3, 0x1.1111111111111p-7, -0x1.a01a01a01a01ap-13, 0x1.71de3a556c734p-19, -0x1.ae64567f544e4p-26, 0x1.6124613a86d09p-33, -0x1.ae7f3e733b81fp-41, 0x1.952c77030ad4ap-49, -0x1.2f49b46814157p-57, 0x1.71b8ef6dcf572p-66, -0x1.761b41316381ap-75};
+    const double pi_hi = 0x1.921fb54442d18p+1, pi_lo = 0x1.1a62633145c07p-53;
+    const bool neg = x < 0;
+    double a = neg ? -x : x;
+    if (a > pi_hi * 0.5) a = (pi_hi - a) + pi_lo;
+    double a2 = a * a, t = c[10];
+#pragma unroll
+    for (int k = 9; k >= 0; k--) t = t * a2 + c[k];
+    double r = a + a * (a2 * t);
+    return neg ? -r : r;
+}
+
+// EULERIAN_FLUID_SOLVER_3D::HeavisideFunction (:2459-2479), also
+// PROJECTION_3D's (:1452-1472)
+__device__ __forceinline__ double heav_solver(double phi, double eps) {
+    double one_over_epsilon = (double)1 / eps, one_over_pi = (double)1 / FS_PI;
+    if (phi < -eps) return 0;
+    if (phi > eps) return 1;
+    return (double)0.5 + phi * one_over_epsilon * (double)0.5 +
+           (double)0.5 * one_over_pi * pipe_sin((double)FS_PI * phi * one_over_epsilon);
+}
+// VISCOSITY_3D::HeavisideFunction (VISCOSITY_3D.h:1234-1256)
+__device__ __forceinline__ double heav_visc(double phi, double eps) {
+    double one_over_epsilon = (double)1 / eps, one_over_pi = (double)1 / FS_PI;
+    if (phi < -eps) return (double)0;
+    if (phi > eps) return (double)1;
+    return (double)0.5 * ((double)1 + phi * one_over_epsilon + one_over_pi * pipe_sin(FS_PI * phi * one_over_epsilon));
+}
+
+// EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless, is_parallel
+// (:2340-2393): inside the pipe, fixed faces excepted, u += R1/V0^2 (dt g_x
+// + 1/(rho_half rho_oil) dt f) with rho_half = 1 + (eta - 1) H of the face
+// level set (eps 1.5 dx, :2522-2561); v, w += R1/V0^2 dt g
+struct FsGravPipe {
+    double gd[3], drive, eta, ro, R1, V0, eps;
+};
+__global__ void k_gravity_pipe(double* __restrict__ vel, const double* __restrict__ sfg, const double* __restrict__ b,
+                               FsLayout L, FsBox bx, int comp, FsGravPipe G) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    if (fixed_face(comp, j, k, L.ny, L.nz)) return;
+    const long o = L.idx(i, j, k);
+    if (!(b[o] < 0)) return;
+    if (comp == 1) {
+        double lc = (double)0.5 * (sfg[o] + sfg[o - 1]);
+        double dh = (double)1 + (G.eta - (double)1) * heav_solver(lc, G.eps);
+        vel[o] += G.R1 / (G.V0 * G.V0) * (G.gd[0] + ((double)1 / (dh * G.ro)) * G.drive);
+    } else {
+        vel[o] += G.R1 / (G.V0 * G.V0) * G.gd[comp - 1];
+    }
+}
+
+// ---- semi-implicit viscosity: coefficients and explicit terms ----------------
+
+// VISCOSITY_3D::ApplyViscosity, dimensionless, is_parallel
+// (VISCOSITY_3D.h:579-852), one component: the level set at the face, edge
+// centres and the two cells about it from the shared scalar ghost copy,
+// density 1 + (eta - 1) H, viscosities 1 + (m - 1) H (eps 1.5 dx,
+// :1258-1347), coefficients 2 dt/(Re rho) mu for the normal pair and
+// dt/(Re rho) mu for the tangential ones, and the explicit term from the
+// per-axis ghost copies (Y/Z: the two cross-component copies the component
+// reads).
+struct FsViscPipe {
+    double dt, Re, eta, m, eps;
+};
+__global__ void k_visc_coef(double* __restrict__ c0, double* __restrict__ c1, double* __restrict__ c2,
+                            double* __restrict__ c3, double* __restrict__ c4, double* __restrict__ c5,
+                            double* __restrict__ expl, const double* __restrict__ W, const double* __restrict__ self,
+                            const double* __restrict__ A, const double* __restrict__ B, FsLayout L, FsBox bx, int comp,
+                            FsGridConst g, FsViscPipe V) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const double ox = g.x.inv, oy = g.y.inv, oz = g.z.inv;
+    double half, l[6];
+    if (comp == 1) {
+        half = (double)0.5 * (W[o - 1] + W[o]);
+        l[0] = W[o];
+        l[1] = W[o - 1];
+        l[2] = (double)0.25 * (W[o - 1] + W[o] + W[o - 1 + sx] + W[o + sx]);
+        l[3] = (double)0.25 * (W[o - 1] + W[o] + W[o - 1 - sx] + W[o - sx]);
+        l[4] = (double)0.25 * (W[o - 1] + W[o] + W[o - 1 + sy] + W[o + sy]);
+        l[5] = (double)0.25 * (W[o - 1] + W[o] + W[o - 1 - sy] + W[o - sy]);
+    } else if (comp == 2) {
+        half = (double)0.5 * (W[o - sx] + W[o]);
+        l[0] = (double)0.25 * (W[o - sx] + W[o + 1 - sx] + W[o] + W[o + 1]);
+        l[1] = (double)0.25 * (W[o - 1 - sx] + W[o - sx] + W[o - 1] + W[o]);
+        l[2] = W[o];
+        l[3] = W[o - sx];
+        l[4] = (double)0.25 * (W[o - sx + sy] + W[o - sx] + W[o + sy] + W[o]);
+        l[5] = (double)0.25 * (W[o - sx - sy] + W[o - sx] + W[o - sy] + W[o]);
+    } else {
+        half = (double)0.5 * (W[o - sy] + W[o]);
+        l[0] = (double)0.25 * (W[o - sy] + W[o + 1 - sy] + W[o] + W[o + 1]);
+        l[1] = (double)0.25 * (W[o - sy] + W[o - 1 - sy] + W[o] + W[o - 1]);
+        l[2] = (double)0.25 * (W[o - sy] + W[o + sx - sy] + W[o] + W[o + sx]);
+        l[3] = (double)0.25 * (W[o - sy] + W[o - sx - sy] + W[o] + W[o - sx]);
+        l[4] = W[o];
+        l[5] = W[o - sy];
+    }
+    const double rho_h = (double)1 + (V.eta - (double)1) * heav_visc(half, V.eps);
+    double mu[6];
+    for (int e = 0; e < 6; e++) mu[e] = (double)1 + (V.m - (double)1) * heav_visc(l[e], V.eps);
+    const int n0 = comp == 1 ? 0 : comp == 2 ? 2 : 4;
+    double* cf[6] = {c0, c1, c2, c3, c4, c5};
+    for (int e = 0; e < 6; e++)
+        cf[e][o] = (e == n0 || e == n0 + 1) ? (double)2 * V.dt / (V.Re * rho_h) * mu[e] : V.dt / (V.Re * rho_h) * mu[e];
+    const double dt = V.dt, Re = V.Re;
+    double ex;
+    if (comp == 1) {
+        ex = self[o] +
+             dt / (Re * rho_h) * (oy * (mu[2] * ox * (A[o + sx] - A[o - 1 + sx]) - mu[3] * ox * (A[o] - A[o - 1]))) +
+             dt / (Re * rho_h) * (oz * (mu[4] * ox * (B[o + sy] - B[o - 1 + sy]) - mu[5] * ox * (B[o] - B[o - 1])));
+    } else if (comp == 2) {
+        ex = self[o] +
+             dt / (Re * rho_h) * (ox * (mu[0] * oy * (A[o + 1] - A[o + 1 - sx]) - mu[1] * oy * (A[o] - A[o - sx]))) +
+             dt / (Re * rho_h) * (oz * (mu[4] * oy * (B[o + sy] - B[o - sx + sy]) - mu[5] * ox * (B[o] - B[o - sx])));
+    } else {
+        ex = self[o] +
+             dt / (Re * rho_h) * (ox * (oz * (mu[0] * (A[o + 1] - A[o + 1 - sy])) - oz * (mu[1] * (A[o] - A[o - sy])))) +
+             dt / (Re * rho_h) * (oy * (oz * (mu[2] * (B[o + sx] - B[o + sx - sy])) - ox * (mu[3] * (B[o] - B[o - sy]))));
+    }
+    expl[o] = ex;
+}
+
+// ---- linear systems ------------------------------------------------------------
+
+// ELL matrix with up to 7 entries per row in the reference's CSR order.
+#define PSLOTS 7
+
+// POISSON_SOLVER_3D::BuildLinearSystemNodeForSemiImplicitViscosity with
+// per-axis ghost arrays (:3641-3828), one row per thread.  bc: the
+// component's bc field after row numbering (FULL cells hold their row).
+struct FsViscSys {
+    double inv_dxdx, inv_dydy, inv_dzdz;
+    int comp, ie;
+};
+__global__ void k_visc_assemble(const long* __restrict__ rows, int N, const int* __restrict__ bc,
+                                const double* __restrict__ c0, const double* __restrict__ c1,
+                                const double* __restrict__ c2, const double* __restrict__ c3,
+                                const double* __restrict__ c4, const double* __restrict__ c5,
+                                const double* __restrict__ expl, const double* __restrict__ vx,
+                                const double* __restrict__ vy, const double* __restrict__ vz,
+                                const double* __restrict__ vel, double* __restrict__ val, int* __restrict__ col,
+                                int* __restrict__ cnt, double* __restrict__ dg, double* __restrict__ rhs,
+                                double* __restrict__ x, FsLayout L, FsViscSys S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const long o = rows[r], sx = L.sx, sy = L.sy;
+    const double cf[6] = {c0[o], c1[o], c2[o], c3[o], c4[o], c5[o]};
+    // neighbours i-1, i+1, j-1, j+1, k-1, k+1 with coef_2, coef_1, coef_4,
+    // coef_3, coef_6, coef_5 and 1/dx^2, 1/dx^2, 1/dy^2, ...
+    const long off[6] = {-1, 1, -sx, sx, -sy, sy};
+    const int ci[6] = {1, 0, 3, 2, 5, 4};
+    const double inv[6] = {S.inv_dxdx, S.inv_dxdx, S.inv_dydy, S.inv_dydy, S.inv_dzdz, S.inv_dzdz};
+    int nb[6];
+    for (int e = 0; e < 6; e++) nb[e] = bc[o + off[e]];
+    int n = 0;
+    double coef_ijk = 0;
+    for (int e = 0; e < 6; e++)
+        if (nb[e] > -1) {
+            coef_ijk += cf[ci[e]];
+            val[n * N + r] = -cf[ci[e]] * inv[e];
+            col[n * N + r] = nb[e];
+            n++;
+        }
+    // x-periodic neighbours (:3721-3746)
+    const int i = (int)((o - L.off) % sx) - FS_G;
+    if (nb[0] == BC_PER) {
+        coef_ijk += cf[1];
+        int pi = S.comp == 1 ? S.ie + (i - 1 - (0 + 1)) : S.ie + (i - 1 - 0);
+        val[n * N + r] = -cf[1] * S.inv_dxdx;
+        col[n * N + r] = bc[o + (pi - i)];
+        n++;
+    }
+    if (nb[1] == BC_PER) {
+        coef_ijk += cf[0];
+        int pi = S.comp == 1 ? 0 + (i + 1 - (S.ie - 1)) : 0 + (i + 1 - S.ie);
+        val[n * N + r] = -cf[0] * S.inv_dxdx;
+        col[n * N + r] = bc[o + (pi - i)];
+        n++;
+    }
+    for (int e = 0; e < 6; e++)
+        if (nb[e] == BC_DIR) coef_ijk += cf[ci[e]];
+    if (coef_ijk == 0) coef_ijk = 1;
+    const double d = (double)1 + S.inv_dxdx * (double)coef_ijk;
+    val[n * N + r] = d;
+    col[n * N + r] = r;
+    n++;
+    cnt[r] = n;
+    dg[r] = d;
+    double bv = expl[o];
+    const double* dir[6] = {vx, vx, vy, vy, vz, vz};
+    for (int e = 0; e < 6; e++)
+        if (nb[e] == BC_DIR) bv += cf[ci[e]] * inv[e] * dir[e][o + off[e]];
+    rhs[r] = bv;
+    x[r] = vel[o];
+}
+
+// POISSON_SOLVER_3D::BuildLinearSystemNodeJumpConditionVaribleCoefficient,
+// oil/water (:1360-1815), with jc_on_derivative = 0: GFM betas from the
+// level set L and 1/rho V, the off-diagonals to FULL neighbours, the pin of
+// the first full cell (beta_d when its lower neighbour is NEUM), the jump
+// terms of the right-hand side, x0 = p.
+struct FsPresSys {
+    double inv_dxdx, inv_dydy, inv_dzdz, odx, ody, odz, odx2, ody2, odz2;
+};
+__global__ void k_pres_assemble(const long* __restrict__ rows, int N, const int* __restrict__ bc,
+                                const double* __restrict__ Lv, const double* __restrict__ V,
+                                const double* __restrict__ JC, const double* __restrict__ div,
+                                const double* __restrict__ p, double* __restrict__ val, int* __restrict__ col,
+                                int* __restrict__ cnt, double* __restrict__ dg, double* __restrict__ rhs,
+                                double* __restrict__ x, FsLayout L, FsPresSys S) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const long o = rows[r], sx = L.sx, sy = L.sy;
+    const long off[6] = {-1, 1, -sx, sx, -sy, sy};
+    const double lc = Lv[o], vc = V[o];
+    double ln[6], vn[6], bt[6];
+    for (int e = 0; e < 6; e++) ln[e] = Lv[o + off[e]], vn[e] = V[o + off[e]];
+    for (int e = 0; e < 6; e++)
+        bt[e] = (e % 2 == 0) ? vc * vn[e] * (fabs(ln[e]) + fabs(lc)) / (vc * fabs(ln[e]) + vn[e] * fabs(lc))
+                             : vc * vn[e] * (fabs(ln[e]) + fabs(lc)) / (vn[e] * fabs(lc) + vc * fabs(ln[e]));
+    const double inv[6] = {S.inv_dxdx, S.inv_dxdx, S.inv_dydy, S.inv_dydy, S.inv_dzdz, S.inv_dzdz};
+    int n = 0;
+    double coef_ijk = 0;
+    for (int e = 0; e < 6; e++) {
+        int nb = bc[o + off[e]];
+        if (nb > -1) {
+            coef_ijk += bt[e];
+            val[n * N + r] = -bt[e] * inv[e];
+            col[n * N + r] = nb;
+            n++;
+        }
+    }
+    if (r == 0 && bc[o - sy] == BC_NEUM) coef_ijk += bt[4];
+    if (coef_ijk == 0) coef_ijk = 1;
+    const double d = S.inv_dxdx * (double)coef_ijk;
+    val[n * N + r] = d;
+    col[n * N + r] = r;
+    n++;
+    cnt[r] = n;
+    dg[r] = d;
+    const double o2[6] = {S.odx2, S.odx2, S.ody2, S.ody2, S.odz2, S.odz2}, o1[6] = {S.odx, S.odx, S.ody, S.ody, S.odz, S.odz};
+    const double jcc = JC[o];
+    double F[6];
+    for (int e = 0; e < 6; e++) {
+        const double jn = JC[o + off[e]];
+        double subcell = fabs(ln[e]) / (fabs(lc) + fabs(ln[e]));
+        double aa = (jcc * fabs(ln[e]) + jn * fabs(lc)) / (fabs(lc) + fabs(ln[e]));
+        double bb = (0 * fabs(ln[e]) + 0 * fabs(lc)) / (fabs(lc) + fabs(ln[e]));
+        const bool minus = e % 2 == 0;
+        if ((lc <= 0 && ln[e] <= 0) || (lc > 0 && ln[e] > 0))
+            F[e] = 0;
+        else if (lc <= 0 && ln[e] > 0)
+            F[e] = minus ? o2[e] * aa * bt[e] - o1[e] * bt[e] * bb * subcell / vn[e]
+                         : o2[e] * aa * bt[e] + o1[e] * bt[e] * bb * subcell / vn[e];
+        else
+            F[e] = minus ? -o2[e] * aa * bt[e] + o1[e] * bt[e] * bb * subcell / vn[e]
+                         : -o2[e] * aa * bt[e] - o1[e] * bt[e] * bb * subcell / vn[e];
+    }
+    const double FX = F[0] + F[1], FY = F[2] + F[3], FZ = F[4] + F[5];
+    rhs[r] = -div[o] - FX - FY - FZ;
+    x[r] = p[o];
+}
+
+// ---- reductions in the fixed order -------------------------------------------
+
+// one block of RBLK rows: the binary tree of the values (rows past N are +0)
+__device__ __forceinline__ double block_tree(double v, double* sh) {
+    sh[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = RBLK / 2; h >= 1; h /= 2) {
+        if ((int)threadIdx.x < h) sh[threadIdx.x] += sh[threadIdx.x + h];
+        __syncthreads();
+    }
+    return sh[0];
+}
+
+// PCG scalars on the device.  done: 1 converged, 2 max_iteration reached.
+struct FsPipePcg {
+    double rz, rz_old, pq, alpha, beta, rr, bb, sum;
+    int iter, done, max_iter, rel;
+    double tol2, rtol;
+};
+
+// partial[blk] = tree over the block of a*b (b == NULL: a)
+__global__ void __launch_bounds__(RBLK) k_dot_part(const double* __restrict__ a, const double* __restrict__ b, int N,
+                                                   double* __restrict__ part) {
+    __shared__ double sh[RBLK];
+    int r = blockIdx.x * RBLK + threadIdx.x;
+    double v = r < N ? (b ? a[r] * b[r] : a[r]) : (double)0;
+    double t = block_tree(v, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t;
+}
+// the block sums in block order into *out
+__global__ void k_sum_parts(const double* __restrict__ part, int nb, double* out) {
+    if (threadIdx.x != 0) return;
+    double total = 0;
+    for (int q = 0; q < nb; q++) total += part[q];
+    *out = total;
+}
+
+// b -= (1/N) sum(b), POISSON_SOLVER_3D.h:221-246
+__global__ void k_remove_mean(double* __restrict__ b, int N, const double* __restrict__ sum) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    double one_over_b_dimension = 1 / (double)N;
+    b[r] -= one_over_b_dimension * *sum;
+}
+
+// y = A x in each row's entry order (CSR_MATRIX::Multiply, CSR_MATRIX.h:
+// 236-260); with `part`, the block trees of x.y too
+__global__ void __launch_bounds__(RBLK) k_spmv(const double* __restrict__ val, const int* __restrict__ col,
+                                               const int* __restrict__ cnt, int N, const double* __restrict__ x,
+                                               double* __restrict__ y, double* __restrict__ part,
+                                               const FsPipePcg* __restrict__ S) {
+    __shared__ double sh[RBLK];
+    if (S && S->done) return;
+    int r = blockIdx.x * RBLK + threadIdx.x;
+    double v = 0;
+    if (r < N) {
+        const int n = cnt[r];
+        for (int e = 0; e < n; e++) v += val[e * N + r] * x[col[e * N + r]];
+        y[r] = v;
+    }
+    if (part) {
+        double t = block_tree(r < N ? x[r] * v : (double)0, sh);
+        if (threadIdx.x == 0) part[blockIdx.x] = t;
+    }
+}
+
+// r = b - q, z = r / D, p = z; partials of r.z, b.b, r.r
+__global__ void __launch_bounds__(RBLK) k_pcg_begin(const double* __restrict__ b, const double* __restrict__ q,
+                                                    const double* __restrict__ dg, double* __restrict__ r,
+                                                    double* __restrict__ z, double* __restrict__ p, int N,
+                                                    double* __restrict__ part, int nb) {
+    __shared__ double sh[RBLK];
+    int i = blockIdx.x * RBLK + threadIdx.x;
+    double rv = 0, zv = 0, bv = 0;
+    if (i < N) {
+        bv = b[i];
+        rv = bv - q[i];
+        zv = rv / dg[i];
+        r[i] = rv;
+        z[i] = zv;
+        p[i] = zv;
+    }
+    double t0 = block_tree(i < N ? rv * zv : (double)0, sh);
+    __syncthreads();
+    double t1 = block_tree(i < N ? bv * bv : (double)0, sh);
+    __syncthreads();
+    double t2 = block_tree(i < N ? rv * rv : (double)0, sh);
+    if (threadIdx.x == 0) {
+        part[blockIdx.x] = t0;
+        part[nb + blockIdx.x] = t1;
+        part[2 * nb + blockIdx.x] = t2;
+    }
+}
+__global__ void k_pcg_begin_scalars(const double* __restrict__ part, int nb, FsPipePcg* S) {
+    if (threadIdx.x != 0) return;
+    double a = 0, b = 0, c = 0;
+    for (int q = 0; q < nb; q++) a += part[q];
+    for (int q = 0; q < nb; q++) b += part[nb + q];
+    for (int q = 0; q < nb; q++) c += part[2 * nb + q];
+    S->rz = a;
+    S->bb = b;
+    S->rr = c;
+    S->iter = 0;
+    S->done = 0;
+    double bnorm = sqrt(b);
+    if (S->rel && (bnorm == 0 || sqrt(c) <= S->rtol * bnorm)) S->done = 1;
+    if (S->max_iter <= 0) S->done = S->done ? S->done : 2;
+}
+// alpha = r.z / p.q
+__global__ void k_pcg_alpha(const double* __restrict__ part, int nb, FsPipePcg* S) {
+    if (threadIdx.x != 0 || S->done) return;
+    double t = 0;
+    for (int q = 0; q < nb; q++) t += part[q];
+    S->pq = t;
+    S->alpha = S->rz / t;
+}
+// x += alpha p, r -= alpha q, z = r / D; partials of r.z (and r.r)
+__global__ void __launch_bounds__(RBLK) k_pcg_update(double* __restrict__ x, double* __restrict__ r,
+                                                     double* __restrict__ z, const double* __restrict__ p,
+                                                     const double* __restrict__ q, const double* __restrict__ dg, int N,
+                                                     double* __restrict__ part, int nb, const FsPipePcg* S) {
+    __shared__ double sh[RBLK];
+    if (S->done) return;
+    const double alpha = S->alpha;
+    int i = blockIdx.x * RBLK + threadIdx.x;
+    double rv = 0, zv = 0;
+    if (i < N) {
+        x[i] += alpha * p[i];
+        rv = r[i] - alpha * q[i];
+        r[i] = rv;
+        zv = rv / dg[i];
+        z[i] = zv;
+    }
+    double t0 = block_tree(i < N ? rv * zv : (double)0, sh);
+    if (threadIdx.x == 0) part[blockIdx.x] = t0;
+    if (S->rel) {
+        __syncthreads();
+        double t1 = block_tree(i < N ? rv * rv : (double)0, sh);
+        if (threadIdx.x == 0) part[nb + blockIdx.x] = t1;
+    }
+}
+// the stop test and beta (PCG_METHOD.cpp:81-101)
+__global__ void k_pcg_tail(const double* __restrict__ part, int nb, FsPipePcg* S) {
+    if (threadIdx.x != 0 || S->done) return;
+    double t = 0;
+    for (int q = 0; q < nb; q++) t += part[q];
+    S->rz_old = S->rz;
+    S->rz = t;
+    if (S->rel) {
+        double rr = 0;
+        for (int q = 0; q < nb; q++) rr += part[nb + q];
+        S->rr = rr;
+        if (sqrt(rr) < S->rtol * sqrt(S->bb)) {
+            S->done = 1;
+            return;
+        }
+    } else if (t < S->tol2) {
+        S->done = 1;
+        return;
+    }
+    S->beta = S->rz / S->rz_old;
+    S->iter++;
+    if (S->iter >= S->max_iter) S->done = 2;
+}
+// p = beta p + z (PCG_METHOD.cpp:104-108: p *= beta; p += z)
+__global__ void __launch_bounds__(RBLK) k_pcg_direction(double* __restrict__ p, const double* __restrict__ z, int N,
+                                                        const FsPipePcg* S) {
+    // stopped (converged, or max_iteration reached: the reference's last p
+    // update is never read)
+    if (S->done) return;
+    int i = blockIdx.x * RBLK + threadIdx.x;
+    if (i >= N) return;
+    double v = p[i];
+    v *= S->beta;
+    v += z[i];
+    p[i] = v;
+}
+// r = b - q (the true residual after the solve)
+__global__ void k_residual(const double* __restrict__ b, const double* __restrict__ q, double* __restrict__ r, int N) {
+    int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < N) r[i] = b[i] - q[i];
+}
+
+// VectorToGrid (POISSON_SOLVER_3D.h:4060-4078): FULL cells take x, fixed
+// faces excepted
+__global__ void k_scatter(double* __restrict__ dst, const double* __restrict__ x, const long* __restrict__ rows, int N,
+                          FsLayout L, int comp) {
+    int r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= N) return;
+    const long o = rows[r];
+    if (comp) {
+        long q = o - L.off;
+        int j = (int)((q / L.sx) % (L.sy / L.sx)) - FS_G, k = (int)(q / L.sy) - FS_G;
+        if (fixed_face(comp, j, k, L.ny, L.nz)) return;
+    }
+    dst[o] = x[r];
+}
+
+// u(ie, j, k) = u(1, j, k) after the x solve (POISSON_SOLVER_3D.h:366-376)
+__global__ void k_xfix_last(double* __restrict__ u, FsLayout L, FsBox bx) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (j >= bx.nj) return;
+    u[L.idx(bx.ni - 1, j, k)] = u[L.idx(0 + 1, j, k)];
+}
+
+// ---- projection -----------------------------------------------------------------
+
+// DetermineProjectionDensity, dimensionless (PROJECTION_3D.h:642-668)
+__global__ void k_pdensity(const double* __restrict__ phi, double* __restrict__ rho, double* __restrict__ irho,
+                           FsLayout L, double rr) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long o = L.idx(i, j, k);
+    double d = phi[o] <= 0 ? 1 : rr;
+    rho[o] = d;
+    irho[o] = 1 / d;
+}
+
+// DetermineDivergence, oil/water (PROJECTION_3D.h:697-725): 0 outside the
+// pipe, else the difference of the ghost copies; times 1/dt
+__global__ void k_pdiv(const double* __restrict__ gu, const double* __restrict__ gv, const double* __restrict__ gw,
+                       const double* __restrict__ bls, double* __restrict__ div, FsLayout L, FsGridConst g,
+                       double one_over_dt) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= L.nx || j >= L.ny) return;
+    const long o = L.idx(i, j, k);
+    double dv;
+    if (bls[o] > 0)
+        dv = (double)0;
+    else {
+        dv = g.x.inv * (gu[o + 1] - gu[o]);
+        dv += g.y.inv * (gv[o + L.sx] - gv[o]);
+        dv += g.z.inv * (gw[o + L.sy] - gw[o]);
+    }
+    div[o] = dv * one_over_dt;
+}
+
+// UpdateVelocityByPressureGradientVariableDensity, oil/water, is_parallel
+// (PROJECTION_3D.h:1117-1427), one component; vmax: bits of max |u| over the
+// faces updated
+struct FsPresUpd {
+    double dt, od, eps, rr;
+};
+__global__ void k_pupdate(double* __restrict__ vel, const double* __restrict__ psfg, const double* __restrict__ phi,
+                          const double* __restrict__ bls, const double* __restrict__ jc,
+                          const double* __restrict__ rho, const double* __restrict__ p, const int* __restrict__ bc,
+                          FsLayout L, FsBox bx, int comp, FsPresUpd U, unsigned long long* vmax) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    double mine = 0;
+    if (i < bx.ni && j < bx.nj && !fixed_face(comp, j, k, L.ny, L.nz)) {
+        const long o = L.idx(i, j, k);
+        const long n = o - (comp == 1 ? 1 : comp == 2 ? L.sx : L.sy);
+        if (bc[o] >= 0) {
+            const double lcf = (double)0.5 * (psfg[o] + psfg[n]);
+            const double dh = 1 + (U.rr - (double)1) * heav_solver(lcf, U.eps);
+            double vp = vel[o];
+            const double one_over_density_ijk = (double)1 / rho[o];
+            const double lv = phi[o], ln = phi[n];
+            const double gamma = (jc[o] * fabs(ln) + jc[n] * fabs(lv)) / (fabs(ln) + fabs(lv));
+            const double one_over_half = 1 / dh;
+            const double bcv = (double)bc[o], bcn = (double)bc[n];
+            const double dp = p[o] - p[n];
+            if (comp == 1) {
+                if (bls[o] * bls[n] >= 0) {
+                    if (bcv * bcn > 0)
+                        vp -= U.dt * dp * one_over_density_ijk * U.od;
+                    else if (lv <= 0)
+                        vp -= U.dt * (dp + gamma) * one_over_half * U.od;
+                    else if (lv > 0)
+                        vp -= U.dt * (dp - gamma) * one_over_half * U.od;
+                }
+            } else if (comp == 2) {
+                if (bcv * bcn >= 0) {
+                    if (lv * ln > 0)
+                        vp -= U.dt * dp * one_over_density_ijk * U.od;
+                    else if (lv < 0)
+                        vp -= U.dt * (dp + gamma) * one_over_half * U.od;
+                    else if (lv > 0)
+                        vp -= U.dt * (dp - gamma) * one_over_half * U.od;
+                }
+            } else {
+                if (bcv * bcn >= 0) {
+                    if (lv * ln > 0)
+                        vp -= U.dt * dp * one_over_density_ijk * U.od;
+                    else if (lv <= 0)
+                        vp -= U.dt * (dp + gamma) * one_over_half * U.od;
+                    else if (lv > 0)
+                        vp -= U.dt * (dp - gamma) * one_over_half * U.od;
+                }
+            }
+            vel[o] = vp;
+            mine = fabs(vp);
+        }
+    }
+    // block max, then one atomic (values >= 0 order as their bit patterns)
+    __shared__ double sh[PBX * PBY];
+    const int t = threadIdx.y * PBX + threadIdx.x;
+    sh[t] = mine;
+    __syncthreads();
+    for (int h = PBX * PBY / 2; h >= 1; h /= 2) {
+        if (t < h) sh[t] = fmax(sh[t], sh[t + h]);
+        __syncthreads();
+    }
+    if (t == 0) atomicMax(vmax, (unsigned long long)__double_as_longlong(sh[0]));
+}
+
+// the x-periodic copies of the end faces after the update (PROJECTION_3D.h:
+// 1248-1256, 1355-1363, 1436-1444): x faces u(ie) = u(1), u(0) = u(ie-1);
+// y and z faces v(ie) = v(0)
+__global__ void k_pfix(double* __restrict__ vel, FsLayout L, FsBox bx, int comp) {
+    int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (j >= bx.nj) return;
+    const int ie = bx.ni - 1;
+    if (comp == 1) {
+        double a = vel[L.idx(0 + 1, j, k)], b = vel[L.idx(ie - 1, j, k)];
+        vel[L.idx(ie, j, k)] = a;
+        vel[L.idx(0, j, k)] = b;
+    } else
+        vel[L.idx(ie, j, k)] = vel[L.idx(0, j, k)];
+}
+
+// max |u| of a face box (bits of the FP64 value)
+__global__ void k_absmax(const double* __restrict__ a, FsLayout L, FsBox bx, unsigned long long* out) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    double mine = 0;
+    if (i < bx.ni && j < bx.nj) mine = fabs(a[L.idx(i, j, k)]);
+    __shared__ double sh[PBX * PBY];
+    const int t = threadIdx.y * PBX + threadIdx.x;
+    sh[t] = mine;
+    __syncthreads();
+    for (int h = PBX * PBY / 2; h >= 1; h /= 2) {
+        if (t < h) sh[t] = fmax(sh[t], sh[t + h]);
+        __syncthreads();
+    }
+    if (t == 0) atomicMax(out, (unsigned long long)__double_as_longlong(sh[0]));
+}
+
+// ---- host data of one linear system ---------------------------------------------
+
+struct PipeSys {
+    int N = 0;
+    int* bc = nullptr;     // device: the bc field after numbering (padded layout)
+    long* rows = nullptr;  // device: row -> padded index
+};
+
+}  // namespace
+
+// =====================================================================================
+// the solver
+// =====================================================================================
+
+struct FsPipe {
+    fs_params P;
+    fs_pipe_params Q;
+    fs_grid G;
+    FsLayout L;
+    FsGridConst cell, mac[4];
+    double y0f[4], z0f[4];
+    double R1, f, A, V0, J, Re, We, A0;
+    int device = 0;
+    hipStream_t st = nullptr;
+    bool async = false;
+    // fields
+    double *phi = nullptr, *bls = nullptr, *u = nullptr, *v = nullptr, *w = nullptr, *p = nullptr;
+    double *sfg = nullptr, *g[4] = {nullptr, nullptr, nullptr, nullptr};
+    double* ag[4][2] = {};
+    double* vg[4][2] = {};
+    double* zero = nullptr;  // VISCOSITY_3D's y_x and z_x copies, never written in this mode
+    double* bp[4] = {};
+    double *psfg = nullptr, *rho = nullptr, *irho = nullptr, *div = nullptr, *jc = nullptr, *kappa = nullptr;
+    double *sgn = nullptr, *tmp = nullptr, *scr = nullptr, *rk[4] = {};
+    double* coef[6] = {};
+    double* expl = nullptr;
+    // systems: 0 pressure, 1..3 viscosity x, y, z
+    PipeSys sys[4];
+    int Nmax = 0;
+    double *val = nullptr, *dg = nullptr, *rhs = nullptr, *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv = nullptr,
+           *qv = nullptr, *part = nullptr;
+    int *col = nullptr, *cnt = nullptr;
+    FsPipePcg* pcg = nullptr;
+    double* dscal = nullptr;             // device scalars: [0] a sum of the fixed-order reductions
+    unsigned long long* bits = nullptr;  // [0] kappa max, [1] curvature failures, [2..4] max |u|, |v|, |w|
+    FsPipePcg* host_pcg = nullptr;
+    double* host_bits = nullptr;
+    std::vector<double*> owned;
+    double kmax = 0;
+    int last_iters = 0, visc_iters = 0;
+    double last_res_abs = 0, last_res_rel = 0;
+    bool not_converged = false;
+
+    FsBox box(int c) const { return fs_box(L, c); }
+    double* field(int c) { return c == 1 ? u : c == 2 ? v : w; }
+
+    double* alloc_field() {
+        double* d = nullptr;
+        PCHECK(hipMalloc(&d, sizeof(double) * L.total));
+        owned.push_back(d);
+        PCHECK(hipMemsetAsync(d, 0, sizeof(double) * L.total, st));
+        return d;
+    }
+
+    ~FsPipe() {
+        if (st) (void)hipStreamSynchronize(st);
+        for (double* d : owned) (void)hipFree(d);
+        for (int s = 0; s < 4; s++) {
+            (void)hipFree(sys[s].bc);
+            (void)hipFree(sys[s].rows);
+        }
+        for (void* d : {(void*)val, (void*)dg, (void*)rhs, (void*)xv, (void*)rv, (void*)zv, (void*)pv, (void*)qv,
+                        (void*)part, (void*)col, (void*)cnt, (void*)pcg, (void*)bits, (void*)dscal})
+            (void)hipFree(d);
+        if (host_pcg) (void)hipHostFree(host_pcg);
+        if (host_bits) (void)hipHostFree(host_bits);
+        if (st) (void)hipStreamDestroy(st);
+    }
+
+    void sync() { PCHECK(hipStreamSynchronize(st)); }
+    void check_launch() { PCHECK(hipGetLastError()); }
+
+    // ---- set-up --------------------------------------------------------------
+    void init(const fs_params& P_, const fs_pipe_params& Q_, const fs_grid& G_, int dev) {
+        P = P_, Q = Q_, G = G_;
+        if (!Q.is_parallel || !Q.dimensionless_form || !Q.cylindrical_boundary || Q.boundary_order_velocity != 1)
+            throw FsPipeError{FS_ERR_UNSUPPORTED,
+                              "pipe mode: only the parallel pipe with a cylindrical boundary, the dimensionless form "
+                              "and first-order wall extrapolation are implemented"};
+        if (P.precision != FS_FP64 || P.schedule != FS_SCHEDULE_COMPAT)
+            throw FsPipeError{FS_ERR_UNSUPPORTED, "pipe mode: FS_FP64 and FS_SCHEDULE_COMPAT only"};
+        if (P.csf_model) throw FsPipeError{FS_ERR_UNSUPPORTED, "pipe mode: CSF_model is not implemented"};
+        if (P.curvature_method != FS_CURV_LEVELSET)
+            throw FsPipeError{FS_ERR_UNSUPPORTED, "pipe mode: curvature by level set only"};
+        if (P.fast_weno) throw FsPipeError{FS_ERR_UNSUPPORTED, "pipe mode: fast_weno is not implemented"};
+        if (G.nx < 8 || G.ny < 8 || G.nz < 8) throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: every extent must be >= 8"};
+        if (P.order_for_time_advancing < 1 || P.order_for_time_advancing > 3)
+            throw FsPipeError{FS_ERR_BAD_ARG, "order_for_time_advancing must be 1, 2 or 3"};
+        if (!(Q.a > 0) || !(Q.R2 > 0) || !(Q.oil_viscosity > 0) || !(Q.oil_density > 0) ||
+            !(P.water_viscosity > 0) || !(P.water_density > 0))
+            throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: a, R2, densities and viscosities must be positive"};
+        if ((long)G.nx * G.ny * G.nz > (1L << 28)) throw FsPipeError{FS_ERR_OOM, "grid too large"};
+        if (dev >= 0) PCHECK(hipSetDevice(dev));
+        PCHECK(hipGetDevice(&device));
+        PCHECK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        L = fs_make_layout(G.nx, G.ny, G.nz);
+        cell.x = fs_make_axis(G.x_min, G.x_max, G.nx);
+        cell.y = fs_make_axis(G.y_min, G.y_max, G.ny);
+        cell.z = fs_make_axis(G.z_min, G.z_max, G.nz);
+        const double dx = cell.x.d, dy = cell.y.d, dz = cell.z.d;
+        for (int c = 0; c < 4; c++) mac[c] = cell, y0f[c] = G.y_min, z0f[c] = G.z_min;
+        // MAC grids, EULERIAN_FLUID_SOLVER_3D.h:407,415,423
+        mac[1].x = fs_make_axis(G.x_min - (double)0.5 * dx, G.x_max + (double)0.5 * dx, G.nx + 1);
+        mac[2].y = fs_make_axis(G.y_min - (double)0.5 * dy, G.y_max + (double)0.5 * dy, G.ny + 1);
+        mac[3].z = fs_make_axis(G.z_min - (double)0.5 * dz, G.z_max + (double)0.5 * dz, G.nz + 1);
+        y0f[2] = G.y_min - (double)0.5 * dy;
+        z0f[3] = G.z_min - (double)0.5 * dz;
+        // EULERIAN_FLUID_SOLVER_3D.h:496-520
+        const double a = Q.a, m = Q.m, K = Q.K, gg = Q.g, rw = P.water_density, ro = Q.oil_density;
+        R1 = Q.R2 / a;
+        f = (double)1 / (K - (double)1) * gg * (ro - K * rw);
+        A = (a * a - (double)1) + (m * K + (double)2 * (K - 1) * std::log(a));
+        V0 = (f + rw * gg) * (R1 * R1) / ((double)4 * P.water_viscosity) * A;
+        J = P.surface_tension * R1 * ro / (Q.oil_viscosity * Q.oil_viscosity);
+        Re = (ro * R1 * std::fabs(V0)) / Q.oil_viscosity;
+        We = ((double)1 / J) * (Re * Re);
+        A0 = Q.A_0 / (Q.R2 / a);  // SIMULATION_WORLD.h:126-127
+        if (!std::isfinite(Re) || !std::isfinite(We) || !(V0 != 0))
+            throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: the derived constants V0, Re, We are not finite"};
+        // fields
+        phi = alloc_field(); bls = alloc_field(); u = alloc_field(); v = alloc_field(); w = alloc_field();
+        p = alloc_field(); sfg = alloc_field();
+        for (int c = 1; c < 4; c++) {
+            g[c] = alloc_field();
+            ag[c][0] = alloc_field();
+            ag[c][1] = alloc_field();
+            vg[c][0] = alloc_field();
+            vg[c][1] = alloc_field();
+            bp[c] = alloc_field();
+            rk[c] = alloc_field();
+        }
+        zero = alloc_field();
+        psfg = alloc_field(); rho = alloc_field(); irho = alloc_field(); div = alloc_field(); jc = alloc_field();
+        kappa = alloc_field(); sgn = alloc_field(); tmp = alloc_field(); scr = alloc_field();
+        for (int e = 0; e < 6; e++) coef[e] = alloc_field();
+        expl = alloc_field();
+        PCHECK(hipMalloc(&pcg, sizeof(FsPipePcg)));
+        PCHECK(hipMalloc(&bits, 8 * sizeof(unsigned long long)));
+        PCHECK(hipMalloc(&dscal, 8 * sizeof(double)));
+        PCHECK(hipHostMalloc(&host_pcg, sizeof(FsPipePcg)));
+        PCHECK(hipHostMalloc(&host_bits, 8 * sizeof(double)));
+        static_geometry();
+        sync();
+    }
+
+    // boundary_phi_<c> and the pipe's level set (computed on the host, as the
+    // reference computes them on each module's interior), and the static bc
+    // fields and row numberings of the four systems
+    void static_geometry() {
+        const long n = L.total;
+        std::vector<double> h(n);
+        // boundary_phi_<c> on the interior of each face grid (ADVECTION_3D.h:
+        // 412-432 and its copies in the other modules)
+        std::vector<std::vector<double>> hb(4);
+        for (int c = 1; c < 4; c++) {
+            std::fill(h.begin(), h.end(), 0.0);
+            FsBox b = box(c);
+            const double ddy = mac[c].y.d, ddz = mac[c].z.d;
+            for (int k = 0; k < b.nk; k++)
+                for (int j = 0; j < b.nj; j++)
+                    for (int i = 0; i < b.ni; i++) {
+                        double y = y0f[c] + j * ddy, z = z0f[c] + k * ddz;
+                        h[L.idx(i, j, k)] = std::sqrt(y * y + z * z) - Q.a;
+                    }
+            PCHECK(hipMemcpy(bp[c], h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+            hb[c] = h;
+        }
+        // the pipe inside the box: the first and last two cells and faces of
+        // every y and z line outside the wall
+        for (int c = 0; c < 4; c++) {
+            FsBox b = box(c);
+            const double ddy = mac[c].y.d, ddz = mac[c].z.d;
+            for (int k = 0; k < b.nk; k++)
+                for (int j = 0; j < b.nj; j++) {
+                    if (!(j < 2 || j >= b.nj - 2 || k < 2 || k >= b.nk - 2)) continue;
+                    double y = y0f[c] + j * ddy, z = z0f[c] + k * ddz;
+                    if (!(std::sqrt(y * y + z * z) - Q.a > 0))
+                        throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: the pipe wall must lie inside the box, two cells "
+                                                          "from every y and z face"};
+                }
+        }
+        // the pipe's level set, SIMULATION_WORLD.cpp:305-319: sqrt(y^2 + z^2)
+        // - a on the cells, ghosts by linear extrapolation
+        std::fill(h.begin(), h.end(), 0.0);
+        for (int k = 0; k < G.nz; k++)
+            for (int j = 0; j < G.ny; j++)
+                for (int i = 0; i < G.nx; i++) {
+                    double y = G.y_min + j * cell.y.d, z = G.z_min + k * cell.z.d;
+                    h[L.idx(i, j, k)] = std::sqrt(y * y + z * z) - Q.a;
+                }
+        std::vector<double> hbls = linear_ghosts(h);
+        PCHECK(hipMemcpy(bls, hbls.data(), sizeof(double) * n, hipMemcpyHostToDevice));
+        // pressure bc, PROJECTION_3D::SetupBoundaryConditions, Neumann,
+        // cylindrical, is_parallel (:1017-1065)
+        std::vector<int> bc(n, 0);
+        auto B = [&](int i, int j, int k) { return hbls[L.idx(i, j, k)]; };
+        for (int k = -1; k <= G.nz; k++)
+            for (int j = -1; j <= G.ny; j++)
+                for (int i = -1; i <= G.nx; i++) {
+                    int val;
+                    if (B(i, j, k) > 0)
+                        val = (B(i, j - 1, k) < 0 || B(i, j + 1, k) < 0 || B(i, j, k - 1) < 0 || B(i, j, k + 1) < 0)
+                                  ? BC_NEUM : BC_OTHER;
+                    else if (i == -1 || i == G.nx)
+                        val = BC_NEUM;
+                    else
+                        val = BC_FULL;
+                    bc[L.idx(i, j, k)] = val;
+                }
+        upload_system(0, bc, box(0));
+        // viscosity bc, VISCOSITY_3D::SetupBoundaryConditionsForVelocity,
+        // cylindrical, is_parallel (:1482-1539)
+        for (int c = 1; c < 4; c++) {
+            std::fill(bc.begin(), bc.end(), 0);
+            FsBox b = box(c);
+            const std::vector<double>& hbp = hb[c];
+            for (int k = -1; k <= b.nk; k++)
+                for (int j = -1; j <= b.nj; j++)
+                    for (int i = -1; i <= b.ni; i++) {
+                        int val;
+                        if (hbp[L.idx(i, j, k)] > 0)
+                            val = BC_DIR;
+                        else if (i < 0)
+                            val = BC_PER;
+                        else if (c == 1 ? i >= b.ni - 1 : i > b.ni - 1)
+                            val = BC_PER;
+                        else
+                            val = BC_FULL;
+                        bc[L.idx(i, j, k)] = val;
+                    }
+            upload_system(c, bc, b);
+        }
+        // ELL storage for the largest system
+        const size_t N = (size_t)std::max(Nmax, 1);
+        const int nb = (int)((N + RBLK - 1) / RBLK);
+        PCHECK(hipMalloc(&val, sizeof(double) * N * PSLOTS));
+        PCHECK(hipMalloc(&col, sizeof(int) * N * PSLOTS));
+        PCHECK(hipMalloc(&cnt, sizeof(int) * N));
+        for (double** d : {&dg, &rhs, &xv, &rv, &zv, &pv, &qv}) PCHECK(hipMalloc(d, sizeof(double) * N));
+        PCHECK(hipMalloc(&part, sizeof(double) * 3 * (nb + 1)));
+    }
+
+    // FillGhostCellsContinuousDerivativesFrom of a cell field onto itself,
+    // depth 2, faces then the three edge blocks (FIELD_STRUCTURE_3D.h:
+    // 1776-2199), on the host
+    std::vector<double> linear_ghosts(const std::vector<double>& in) {
+        std::vector<double> a = in;
+        const int ni = G.nx, nj = G.ny, nk = G.nz, d = 2;
+        auto A_ = [&](int i, int j, int k) -> double& { return a[L.idx(i, j, k)]; };
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++) {
+                for (int i = -1; i >= -d; i--) A_(i, j, k) = (double)2 * A_(i + 1, j, k) - A_(i + 2, j, k);
+                for (int i = ni; i <= ni - 1 + d; i++) A_(i, j, k) = (double)2 * A_(i - 1, j, k) - A_(i - 2, j, k);
+            }
+        for (int k = 0; k < nk; k++)
+            for (int i = 0; i < ni; i++) {
+                for (int j = -1; j >= -d; j--) A_(i, j, k) = (double)2 * A_(i, j + 1, k) - A_(i, j + 2, k);
+                for (int j = nj; j <= nj - 1 + d; j++) A_(i, j, k) = (double)2 * A_(i, j - 1, k) - A_(i, j - 2, k);
+            }
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++) {
+                for (int k = -1; k >= -d; k--) A_(i, j, k) = (double)2 * A_(i, j, k + 1) - A_(i, j, k + 2);
+                for (int k = nk; k <= nk - 1 + d; k++) A_(i, j, k) = (double)2 * A_(i, j, k - 1) - A_(i, j, k - 2);
+            }
+        for (int k = 0; k < nk; k++) {
+            for (int j = -1; j >= -d; j--)
+                for (int i = -1; i >= -d; i--)
+                    A_(i, j, k) = (double)0.5 * (((double)2 * A_(i + 1, j, k) - A_(i + 2, j, k)) +
+                                                 ((double)2 * A_(i, j + 1, k) - A_(i, j + 2, k)));
+            for (int j = nj; j <= nj - 1 + d; j++)
+                for (int i = ni; i <= ni - 1 + d; i++)
+                    A_(i, j, k) = (double)0.5 * (((double)2 * A_(i - 1, j, k) - A_(i - 2, j, k)) +
+                                                 ((double)2 * A_(i, j - 1, k) - A_(i, j - 2, k)));
+            for (int j = nj; j <= nj - 1 + d; j++)
+                for (int i = -1; i >= -d; i--)
+                    A_(i, j, k) = (double)0.5 * (((double)2 * A_(i + 1, j, k) - A_(i + 2, j, k)) +
+                                                 ((double)2 * A_(i, j - 1, k) - A_(i, j - 2, k)));
+        }
+        return a;
+    }
+
+    // AssignSequentialindicesToFullCells (POISSON_SOLVER_3D.h:527-545): FULL
+    // cells of the interior renumbered in k, j, i order
+    void upload_system(int s, std::vector<int>& bc, FsBox b) {
+        std::vector<long> rows;
+        for (int k = 0; k < b.nk; k++)
+            for (int j = 0; j < b.nj; j++)
+                for (int i = 0; i < b.ni; i++) {
+                    long o = L.idx(i, j, k);
+                    if (bc[o] > -1) {
+                        bc[o] = (int)rows.size();
+                        rows.push_back(o);
+                    }
+                }
+        sys[s].N = (int)rows.size();
+        if (sys[s].N == 0) throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: no cell inside the pipe"};
+        Nmax = std::max(Nmax, sys[s].N);
+        PCHECK(hipMalloc(&sys[s].bc, sizeof(int) * L.total));
+        PCHECK(hipMemcpy(sys[s].bc, bc.data(), sizeof(int) * L.total, hipMemcpyHostToDevice));
+        PCHECK(hipMalloc(&sys[s].rows, sizeof(long) * rows.size()));
+        PCHECK(hipMemcpy(sys[s].rows, rows.data(), sizeof(long) * rows.size(), hipMemcpyHostToDevice));
+    }
+
+    // ---- the scene's initial state --------------------------------------------
+    void initial() {
+        const long n = L.total;
+        std::vector<double> hu(n, 0.0), hphi(n, 0.0);
+        const double a = Q.a, K = Q.K, m = Q.m;
+        // EULERIAN_FLUID_SOLVER_3D.h:536-563: x faces, ghost layers of width 2
+        for (int k = -2; k <= G.nz - 1 + 2; k++)
+            for (int j = -2; j <= G.ny - 1 + 2; j++)
+                for (int i = -2; i <= G.nx + 2; i++) {
+                    double y = G.y_min + j * mac[1].y.d, z = G.z_min + k * mac[1].z.d;
+                    double magnitude = std::sqrt(y * y + z * z), val;
+                    if ((magnitude >= 1) && (magnitude <= a))
+                        val = (double)1 / A * (a * a - (y * y + z * z) - (double)2 * (K - 1) * std::log(std::sqrt(y * y + z * z) / a));
+                    else if (magnitude < 1)
+                        val = (double)1 - (double)1 / A * (m * (y * y + z * z) * K);
+                    else
+                        val = 0;
+                    hu[L.idx(i, j, k)] = val;
+                }
+        // SIMULATION_WORLD.cpp:290-303: the interface; its ghosts from the pipe
+        // level set's values at that moment, all 3 dx (:755-757)
+        const double three_dx = cell.x.d * (double)3;
+        std::fill(hphi.begin(), hphi.end(), three_dx);
+        for (int k = 0; k < G.nz; k++)
+            for (int j = 0; j < G.ny; j++)
+                for (int i = 0; i < G.nx; i++) {
+                    double x = G.x_min + i * cell.x.d, y = G.y_min + j * cell.y.d, z = G.z_min + k * cell.z.d;
+                    hphi[L.idx(i, j, k)] = std::sqrt(y * y + z * z) - (A0 * std::cos(Q.alpha * x) + (double)1);
+                }
+        PCHECK(hipMemcpyAsync(u, hu.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        PCHECK(hipMemcpyAsync(phi, hphi.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        PCHECK(hipMemsetAsync(v, 0, sizeof(double) * n, st));
+        PCHECK(hipMemsetAsync(w, 0, sizeof(double) * n, st));
+        PCHECK(hipMemsetAsync(p, 0, sizeof(double) * n, st));
+        sync();
+    }
+
+    // interior-only host arrays in and out
+    void put(double* dst, const double* src, int c) {
+        FsBox b = box(c);
+        std::vector<double> h(L.total, 0.0);
+        PCHECK(hipMemcpy(h.data(), dst, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        long q = 0;
+        for (int k = 0; k < b.nk; k++)
+            for (int j = 0; j < b.nj; j++)
+                for (int i = 0; i < b.ni; i++) h[L.idx(i, j, k)] = src[q++];
+        PCHECK(hipMemcpy(dst, h.data(), sizeof(double) * L.total, hipMemcpyHostToDevice));
+    }
+    void take(double* dst, const double* src, int c) {
+        FsBox b = box(c);
+        std::vector<double> h(L.total);
+        PCHECK(hipMemcpy(h.data(), src, sizeof(double) * L.total, hipMemcpyDeviceToHost));
+        long q = 0;
+        for (int k = 0; k < b.nk; k++)
+            for (int j = 0; j < b.nj; j++)
+                for (int i = 0; i < b.ni; i++) dst[q++] = h[L.idx(i, j, k)];
+    }
+
+    // ---- launch helpers ----------------------------------------------------------
+    void periodic(double* dst, const double* src, int c, int gw, int copy, const double* mask) {
+        FsBox b = box(c);
+        hipLaunchKernelGGL(k_periodic_x, box_grid(b, gw), dim3(PBX, PBY), 0, st, dst, src, mask, L, b, c, gw, copy);
+    }
+    void masked_copies(double* const* dsts3) {  // dsts3[1..3] <- u, v, w masked by their boundary_phi
+        for (int c = 1; c < 4; c++) periodic(dsts3[c], field(c), c, 3, 1, bp[c]);
+    }
+
+    // ---- phases ------------------------------------------------------------------
+    // ADVECTION_3D::Solve_Levelset, oil/water (ADVECTION_3D.h:204-242)
+    void LevelsetAdvection(double dt) {
+        masked_copies(g);
+        PCHECK(hipMemcpyAsync(scr, g[2], sizeof(double) * L.total, hipMemcpyDeviceToDevice, st));
+        hipLaunchKernelGGL(k_lsbc_y, box_grid(box(2)), dim3(PBX, PBY), 0, st, v, g[2], scr, bp[2], L, box(2));
+        hipLaunchKernelGGL(k_lsbc_z, box_grid(box(3)), dim3(PBX, PBY), 0, st, w, g[3], bp[3], L, box(3));
+        periodic(sfg, phi, 0, 3, 1, nullptr);
+        hipLaunchKernelGGL(k_weno5_pipe, box_grid(box(0)), dim3(PBX, PBY), 0, st, phi, sfg, g[1], g[2], g[3], bls, L,
+                           cell.x.inv, cell.y.inv, cell.z.inv, P.levelset_epsilon, dt);
+    }
+
+    // ADVECTION_3D::Solve_Velocity, ENO3 with the pipe (ADVECTION_3D.h:244-357)
+    void VelocityAdvection(double dt) {
+        masked_copies(g);
+        for (int c = 1; c < 4; c++) {
+            periodic(ag[c][0], field(c), c, 3, 1, bp[c]);
+            periodic(ag[c][1], field(c), c, 3, 1, bp[c]);
+        }
+        const int order[3] = {1, 3, 2};
+        for (int oc = 0; oc < 3; oc++) {
+            int c = order[oc];
+            hipLaunchKernelGGL(k_velbc_cross, box_grid(box(c)), dim3(PBX, PBY), 0, st, ag[c][0], ag[c][1], g[c], bp[c],
+                               L, box(c));
+        }
+        for (int c = 1; c < 4; c++)
+            hipLaunchKernelGGL(k_eno3_pipe, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), g[c], ag[c][0],
+                               ag[c][1], g[1], g[2], g[3], bp[c], L, box(c), c, mac[c], dt);
+    }
+
+    // EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless, is_parallel
+    void ApplyGravity(double dt) {
+        FsGravPipe Gp;
+        Gp.gd[0] = dt * P.gravity[0];
+        Gp.gd[1] = dt * P.gravity[1];
+        Gp.gd[2] = dt * P.gravity[2];
+        Gp.drive = dt * f;
+        Gp.eta = Q.eta, Gp.ro = Q.oil_density, Gp.R1 = R1, Gp.V0 = V0, Gp.eps = (double)1.5 * cell.x.d;
+        periodic(sfg, phi, 0, 3, 1, nullptr);
+        for (int c = 1; c < 4; c++)
+            hipLaunchKernelGGL(k_gravity_pipe, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), sfg, bp[c], L, box(c),
+                               c, Gp);
+    }
+
+    // the Jacobi-PCG of PCG_METHOD.cpp:4-160 on the assembled ELL system of
+    // N rows (x holds x0); returns the iterations
+    int Pcg(int N) {
+        const int nb = (N + RBLK - 1) / RBLK;
+        *host_pcg = FsPipePcg{};
+        host_pcg->max_iter = P.max_iteration;
+        host_pcg->rel = P.stop_rule == FS_STOP_REL_RESIDUAL;
+        host_pcg->tol2 = P.tolerance * P.tolerance;
+        host_pcg->rtol = P.rel_tolerance;
+        PCHECK(hipMemcpyAsync(pcg, host_pcg, sizeof(FsPipePcg), hipMemcpyHostToDevice, st));
+        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv, nullptr, nullptr);
+        hipLaunchKernelGGL(k_pcg_begin, dim3(nb), dim3(RBLK), 0, st, rhs, qv, dg, rv, zv, pv, N, part, nb);
+        hipLaunchKernelGGL(k_pcg_begin_scalars, dim3(1), dim3(64), 0, st, part, nb, pcg);
+        int batch = 8, enq = 0;
+        while (true) {
+            for (int q = 0; q < batch; q++) {
+                hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, pv, qv, part, pcg);
+                hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, st, part, nb, pcg);
+                hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(RBLK), 0, st, xv, rv, zv, pv, qv, dg, N, part, nb, pcg);
+                hipLaunchKernelGGL(k_pcg_tail, dim3(1), dim3(64), 0, st, part, nb, pcg);
+                hipLaunchKernelGGL(k_pcg_direction, dim3(nb), dim3(RBLK), 0, st, pv, zv, N, pcg);
+            }
+            enq += batch;
+            PCHECK(hipMemcpyAsync(host_pcg, pcg, sizeof(FsPipePcg), hipMemcpyDeviceToHost, st));
+            sync();
+            if (host_pcg->done) break;
+            if (enq > P.max_iteration + 2) throw FsPipeError{FS_ERR_HIP, "pipe PCG: iteration count out of step"};
+            batch = std::min(64, batch * 2);
+        }
+        check_launch();
+        const int it = host_pcg->iter;
+        last_res_abs = std::sqrt(host_pcg->rz);
+        // the true residual for the report
+        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv, nullptr, nullptr);
+        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(RBLK), 0, st, rhs, qv, rv, N);
+        hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rv, rv, N, part);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part, nb, dscal);
+        PCHECK(hipMemcpyAsync(host_bits, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
+        sync();
+        const double bnorm = std::sqrt(host_pcg->bb);
+        last_res_rel = bnorm > 0 ? std::sqrt(host_bits[0]) / bnorm : 0;
+        if (host_pcg->done == 2) not_converged = true;
+        return it;
+    }
+
+    // VISCOSITY_3D::ApplyViscosity, semi-implicit, dimensionless, is_parallel
+    // (VISCOSITY_3D.h:565-852): the boundary conditions (VISCOSITY_3D.h:
+    // 2048-2106, 2235-2356) once, then each component's coefficients,
+    // explicit term, system and solve
+    void Viscosity(double dt) {
+        periodic(sfg, phi, 0, 3, 1, nullptr);
+        masked_copies(g);
+        for (int c = 1; c < 4; c++) {
+            periodic(vg[c][0], field(c), c, 3, 1, bp[c]);
+            periodic(vg[c][1], field(c), c, 3, 1, bp[c]);
+        }
+        const int order[3] = {1, 3, 2};
+        for (int oc = 0; oc < 3; oc++) {
+            int c = order[oc];
+            hipLaunchKernelGGL(k_viscbc, box_grid(box(c)), dim3(PBX, PBY), 0, st, vg[c][0], vg[c][1], g[c], bp[c], L,
+                               box(c));
+        }
+        FsViscPipe Vp{dt, Re, Q.eta, Q.m, (double)1.5 * cell.x.d};
+        for (int c = 1; c < 4; c++) {
+            // cross-component copies each explicit term reads: x reads y_x,
+            // z_x (zero here), y reads x_y and z_y, z reads x_z and y_z
+            const double* A_ = c == 1 ? zero : c == 2 ? vg[1][0] : vg[1][1];
+            const double* B_ = c == 1 ? zero : c == 2 ? vg[3][0] : vg[2][1];
+            hipLaunchKernelGGL(k_visc_coef, box_grid(box(c)), dim3(PBX, PBY), 0, st, coef[0], coef[1], coef[2], coef[3],
+                               coef[4], coef[5], expl, sfg, g[c], A_, B_, L, box(c), c, mac[c], Vp);
+            FsViscSys S;
+            S.inv_dxdx = (double)1 / (mac[c].x.d * mac[c].x.d);
+            S.inv_dydy = (double)1 / (mac[c].y.d * mac[c].y.d);
+            S.inv_dzdz = (double)1 / (mac[c].z.d * mac[c].z.d);
+            S.comp = c;
+            S.ie = box(c).ni - 1;
+            const int N = sys[c].N, nb = (N + RBLK - 1) / RBLK;
+            hipLaunchKernelGGL(k_visc_assemble, dim3(nb), dim3(RBLK), 0, st, sys[c].rows, N, sys[c].bc, coef[0],
+                               coef[1], coef[2], coef[3], coef[4], coef[5], expl, g[c], vg[c][0], vg[c][1], field(c),
+                               val, col, cnt, dg, rhs, xv, L, S);
+            visc_iters += Pcg(N);
+            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, field(c), xv, sys[c].rows, N, L, c);
+            if (c == 1)
+                hipLaunchKernelGGL(k_xfix_last, dim3((box(1).nj + 63) / 64, box(1).nk), dim3(64), 0, st, u, L, box(1));
+        }
+    }
+
+    // LEVELSET_3D::ComputeCurvatures by level set (LEVELSET_3D.h:963-1082):
+    // phi's ghosts by linear extrapolation, kappa and its maximum
+    void Curvatures(double* ph) {
+        fs_fill_linear(st, ph, ph, L, box(0), 2, 1);
+        PCHECK(hipMemsetAsync(bits, 0, 2 * sizeof(unsigned long long), st));
+        fs_curvature(st, ph, kappa, jc, (double)1 / We, L, cell, 0, bits);
+        PCHECK(hipMemcpyAsync(host_bits, bits, sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        sync();
+        kmax = host_bits[0];
+    }
+
+    // PROJECTION_3D::Solve, oil/water, is_parallel (PROJECTION_3D.h:414-432)
+    void Projection(double dt, fs_step_stats* stt) {
+        // phi's own x ghosts and first cell from the periodic image (:424-427)
+        periodic(phi, phi, 0, 2, 0, nullptr);
+        const double rr = P.water_density / Q.oil_density;
+        hipLaunchKernelGGL(k_pdensity, box_grid(box(0)), dim3(PBX, PBY), 0, st, phi, rho, irho, L, rr);
+        // DetermineDivergence (:697-725)
+        masked_copies(g);
+        hipLaunchKernelGGL(k_divbc, box_grid(box(3)), dim3(PBX, PBY), 0, st, g[3], bp[3], L, box(3), (long)L.sy);
+        hipLaunchKernelGGL(k_divbc, box_grid(box(2)), dim3(PBX, PBY), 0, st, g[2], bp[2], L, box(2), (long)L.sx);
+        hipLaunchKernelGGL(k_pdiv, box_grid(box(0)), dim3(PBX, PBY), 0, st, g[1], g[2], g[3], bls, div, L, cell, 1 / dt);
+        // DetermineJumpConditionField (:746-792): jc = kappa/We, x-periodic
+        Curvatures(phi);
+        periodic(jc, jc, 0, 1, 1, nullptr);
+        // DeterminePressure (:794-834)
+        fs_fill_const(st, phi, phi, L, box(0), 2);  // water_levelset.ComputeNormals, LEVELSET_3D.h:931
+        periodic(irho, irho, 0, 2, 0, nullptr);
+        fs_fill_const(st, bls, bls, L, box(0), 2);  // boundary_levelset.ComputeNormals
+        FsPresSys S;
+        S.inv_dxdx = (double)1 / (cell.x.d * cell.x.d);
+        S.inv_dydy = (double)1 / (cell.y.d * cell.y.d);
+        S.inv_dzdz = (double)1 / (cell.z.d * cell.z.d);
+        S.odx = cell.x.inv, S.ody = cell.y.inv, S.odz = cell.z.inv;
+        S.odx2 = cell.x.inv_d2, S.ody2 = cell.y.inv_d2, S.odz2 = cell.z.inv_d2;
+        const int N = sys[0].N, nb = (N + RBLK - 1) / RBLK;
+        hipLaunchKernelGGL(k_pres_assemble, dim3(nb), dim3(RBLK), 0, st, sys[0].rows, N, sys[0].bc, phi, irho, jc, div,
+                           p, val, col, cnt, dg, rhs, xv, L, S);
+        // Neumann compatibility, POISSON_SOLVER_3D.h:221-246
+        hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rhs, nullptr, N, part);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part, nb, dscal);
+        hipLaunchKernelGGL(k_remove_mean, dim3(nb), dim3(RBLK), 0, st, rhs, N, dscal);
+        last_iters = Pcg(N);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, p, xv, sys[0].rows, N, L, 0);
+        // UpdateVelocity (:1101-1449)
+        periodic(psfg, phi, 0, 2, 1, nullptr);
+        PCHECK(hipMemsetAsync(bits + 2, 0, 3 * sizeof(unsigned long long), st));
+        for (int c = 1; c < 4; c++) {
+            FsPresUpd U;
+            U.dt = dt;
+            U.od = c == 1 ? mac[1].x.inv : c == 2 ? mac[2].y.inv : mac[3].z.inv;
+            U.eps = (double)1.5 * (c == 1 ? cell.x.d : c == 2 ? cell.y.d : cell.z.d);
+            U.rr = rr;
+            hipLaunchKernelGGL(k_pupdate, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), psfg, phi, bls, jc, rho, p,
+                               sys[0].bc, L, box(c), c, U, bits + 1 + c);
+            hipLaunchKernelGGL(k_pfix, dim3((box(c).nj + 63) / 64, box(c).nk), dim3(64), 0, st, field(c), L, box(c), c);
+        }
+        PCHECK(hipMemcpyAsync(host_bits + 2, bits + 2, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        sync();
+        if (stt) {
+            stt->pcg_iterations += last_iters;
+            stt->residual_precond = last_res_abs;
+            stt->residual_rel = last_res_rel;
+            for (int c = 0; c < 3; c++) stt->max_velocity[c] = host_bits[2 + c];
+        }
+    }
+
+    // EULERIAN_FLUID_SOLVER_3D::Reinitialization with ReinitializationBySussman,
+    // oil/water (ADVECTION_3D.h:360-378): each sweep copies phi into the
+    // shared scalar ghost copy, x-periodic, and takes one WENO5 Sussman step
+    void Reinitialization() {
+        if (!P.sussmanstyle_reinitialization) return;
+        const double fictious_dt = cell.x.d / P.scaling_number_for_reinitialization;
+        fs_sign(st, phi, sgn, L, cell.x.d * cell.x.d);
+        for (int it = 0; it < P.iteration_number_for_reinitialization; it++) {
+            periodic(sfg, phi, 0, 3, 1, nullptr);
+            fs_reinit_sweep(st, phi, sfg, sgn, L, cell, P.levelset_epsilon, fictious_dt, 0, false);
+        }
+    }
+
+    // SolveForRK, EULERIAN_FLUID_SOLVER_3D.h:1453-1476
+    void SolveForRK(double dt, fs_step_stats* stt) {
+        if (P.is_velocity_advection_active) VelocityAdvection(dt);
+        if (P.is_external_force_active) ApplyGravity(dt);
+        if (P.is_sourcing_active) {
+            visc_iters = 0;
+            Viscosity(dt);
+            if (stt) stt->viscosity_iterations += visc_iters;
+        }
+        if (P.is_projection_active) Projection(dt, stt);
+    }
+
+    // AdvanceOneTimeStepThread, orders 1-3 (EULERIAN_FLUID_SOLVER_3D.h:803-1111)
+    void Step(double dt, fs_step_stats* stt) {
+        const int order = P.order_for_time_advancing;
+        if (order == 1 || order == 2) {
+            if (P.is_levelset_advection_active) {
+                fs_copy_box(st, tmp, phi, L, box(0));
+                LevelsetAdvection(dt);
+                LevelsetAdvection(dt);
+                fs_average(st, phi, tmp, L);
+            }
+            if (order == 1)
+                SolveForRK(dt, stt);
+            else {
+                for (int c = 1; c < 4; c++) fs_copy_box(st, rk[c], field(c), L, box(c));
+                SolveForRK(dt, stt);
+                SolveForRK(dt, stt);
+                for (int c = 1; c < 4; c++) fs_combine(st, field(c), rk[c], (double)0.5, (double)0.5, L, box(c));
+            }
+            fs_copy_box(st, tmp, phi, L, box(0));
+            Reinitialization();
+            Reinitialization();
+            fs_average(st, phi, tmp, L);
+        } else {
+            if (P.is_levelset_advection_active) LevelsetAdvection(dt);
+            for (int c = 1; c < 4; c++) fs_copy_box(st, rk[c], field(c), L, box(c));
+            SolveForRK(dt, stt);
+            SolveForRK(dt, stt);
+            for (int c = 1; c < 4; c++) fs_combine(st, field(c), rk[c], (double)0.75, (double)0.25, L, box(c));
+            SolveForRK(dt, stt);
+            for (int c = 1; c < 4; c++)
+                fs_combine(st, field(c), rk[c], (double)1 / (double)3, (double)2 / (double)3, L, box(c));
+            fs_copy_box(st, tmp, phi, L, box(0));
+            Reinitialization();
+            Reinitialization();
+            fs_combine(st, phi, tmp, (double)0.75, (double)0.25, L, box(0));
+            Reinitialization();
+            fs_combine(st, phi, tmp, (double)1 / (double)3, (double)2 / (double)3, L, box(0));
+        }
+    }
+
+    // CFLOneTimeStep, oil/water, dimensionless, semi-implicit
+    // (EULERIAN_FLUID_SOLVER_3D.h:2673-2785), the curvature maximum of the
+    // current level set (the frame's ComputeCurvatures, SIMULATION_WORLD.h:324)
+    double Cfl(double cfl) {
+        PCHECK(hipMemsetAsync(bits + 2, 0, 3 * sizeof(unsigned long long), st));
+        for (int c = 1; c < 4; c++)
+            hipLaunchKernelGGL(k_absmax, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), L, box(c), bits + 1 + c);
+        PCHECK(hipMemcpyAsync(scr, phi, sizeof(double) * L.total, hipMemcpyDeviceToDevice, st));
+        // the curvature pass writes kappa and jc: keep the projection's copies
+        PCHECK(hipMemcpyAsync(tmp, jc, sizeof(double) * L.total, hipMemcpyDeviceToDevice, st));
+        Curvatures(scr);
+        PCHECK(hipMemcpyAsync(jc, tmp, sizeof(double) * L.total, hipMemcpyDeviceToDevice, st));
+        PCHECK(hipMemcpyAsync(host_bits + 2, bits + 2, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
+        sync();
+        const double c_f = host_bits[2] * mac[1].x.inv + host_bits[3] * mac[2].y.inv + host_bits[4] * mac[3].z.inv;
+        double g_f = 0;
+        if (P.is_external_force_active) {
+            double mn = std::min(Q.oil_density, P.water_density);
+            g_f = std::sqrt(R1 / (V0 * V0) * (std::fabs(P.gravity[1]) + std::fabs((double)1 / mn * f)) * mac[2].y.inv);
+        }
+        const double rr = P.water_density / Q.oil_density;
+        const double mind = std::min(std::min(mac[1].x.d, mac[2].y.d), mac[3].z.d);
+        const double s_f = std::sqrt(((double)1 / We * kmax) / ((rr < 1 ? rr : 1) * (mind * mind)));
+        const double v_f = 0;
+        const double rate =
+            (double)0.5 * (c_f + v_f + std::sqrt((c_f + v_f) * (c_f + v_f) + (double)4 * (g_f * g_f) + (double)4 * (s_f * s_f)));
+        return cfl * ((double)1 / rate);
+    }
+};
+
+// ---- entry points used by the C-ABI ------------------------------------------------
+
+FsPipe* fs_pipe_new(const fs_params& P, const fs_pipe_params& Q, const fs_grid& g, int device) {
+    FsPipe* s = new FsPipe();
+    try {
+        s->init(P, Q, g, device);
+    } catch (...) {
+        delete s;
+        throw;
+    }
+    return s;
+}
+
+void fs_pipe_delete(FsPipe* s) { delete s; }
+
+void fs_pipe_initial(FsPipe* s) { s->initial(); }
+
+void fs_pipe_consts(FsPipe* s, double* out) {
+    out[0] = s->R1, out[1] = s->f, out[2] = s->A, out[3] = s->V0, out[4] = s->J, out[5] = s->Re, out[6] = s->We;
+}
+
+void fs_pipe_set(FsPipe* s, const double* phi, const double* u, const double* v, const double* w, const double* p) {
+    s->sync();
+    if (phi) s->put(s->phi, phi, 0);
+    if (u) s->put(s->u, u, 1);
+    if (v) s->put(s->v, v, 2);
+    if (w) s->put(s->w, w, 3);
+    if (p) s->put(s->p, p, 0);
+}
+
+void fs_pipe_get(FsPipe* s, double* phi, double* u, double* v, double* w, double* p) {
+    s->sync();
+    if (phi) s->take(phi, s->phi, 0);
+    if (u) s->take(u, s->u, 1);
+    if (v) s->take(v, s->v, 2);
+    if (w) s->take(w, s->w, 3);
+    if (p) s->take(p, s->p, 0);
+}
+
+int fs_pipe_advance(FsPipe* s, double dt, fs_step_stats* stt, bool frame) {
+    if (!(dt > 0) || !std::isfinite(dt)) throw FsPipeError{FS_ERR_BAD_ARG, "dt must be positive and finite"};
+    fs_step_stats local{};
+    hipEvent_t e0, e1;
+    PCHECK(hipEventCreate(&e0));
+    PCHECK(hipEventCreate(&e1));
+    PCHECK(hipEventRecord(e0, s->st));
+    s->not_converged = false;
+    if (frame) s->Curvatures(s->phi);
+    s->Step(dt, &local);
+    PCHECK(hipEventRecord(e1, s->st));
+    s->check_launch();
+    float ms = 0;
+    if (!s->async) {
+        s->sync();
+        PCHECK(hipEventElapsedTime(&ms, e0, e1));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    local.step_ms = ms;
+    if (stt) *stt = local;
+    return s->not_converged ? FS_ERR_NOT_CONVERGED : FS_OK;
+}
+
+int fs_pipe_run_phase(FsPipe* s, int phase, double dt, fs_step_stats* stt) {
+    fs_step_stats local{};
+    s->not_converged = false;
+    switch (phase) {
+        case FS_PIPE_LEVELSET_ADVECTION: s->LevelsetAdvection(dt); break;
+        case FS_PIPE_VELOCITY_ADVECTION: s->VelocityAdvection(dt); break;
+        case FS_PIPE_GRAVITY: s->ApplyGravity(dt); break;
+        case FS_PIPE_SOURCING:
+            s->visc_iters = 0;
+            s->Viscosity(dt);
+            local.viscosity_iterations = s->visc_iters;
+            break;
+        case FS_PIPE_PROJECTION: s->Projection(dt, &local); break;
+        case FS_PIPE_REINITIALIZATION: s->Reinitialization(); break;
+        default: throw FsPipeError{FS_ERR_BAD_ARG, "unknown phase"};
+    }
+    s->check_launch();
+    s->sync();
+    if (stt) *stt = local;
+    return s->not_converged ? FS_ERR_NOT_CONVERGED : FS_OK;
+}
+
+double fs_pipe_cfl(FsPipe* s, double cfl) { return s->Cfl(cfl); }
+
+void fs_pipe_array(FsPipe* s, int which, double* dst) {
+    double* src = nullptr;
+    int c = 0;
+    switch (which) {
+        case FS_PIPE_ARR_PHI: src = s->phi; break;
+        case FS_PIPE_ARR_SCALAR_GHOST: src = s->sfg; break;
+        case FS_PIPE_ARR_GHOST_U: src = s->g[1], c = 1; break;
+        case FS_PIPE_ARR_GHOST_V: src = s->g[2], c = 2; break;
+        case FS_PIPE_ARR_GHOST_W: src = s->g[3], c = 3; break;
+        case FS_PIPE_ARR_ADV_U_Y: src = s->ag[1][0], c = 1; break;
+        case FS_PIPE_ARR_ADV_U_Z: src = s->ag[1][1], c = 1; break;
+        case FS_PIPE_ARR_ADV_V_Y: src = s->ag[2][0], c = 2; break;
+        case FS_PIPE_ARR_ADV_V_Z: src = s->ag[2][1], c = 2; break;
+        case FS_PIPE_ARR_ADV_W_Y: src = s->ag[3][0], c = 3; break;
+        case FS_PIPE_ARR_ADV_W_Z: src = s->ag[3][1], c = 3; break;
+        case FS_PIPE_ARR_VIS_U_Y: src = s->vg[1][0], c = 1; break;
+        case FS_PIPE_ARR_VIS_U_Z: src = s->vg[1][1], c = 1; break;
+        case FS_PIPE_ARR_VIS_V_Y: src = s->vg[2][0], c = 2; break;
+        case FS_PIPE_ARR_VIS_V_Z: src = s->vg[2][1], c = 2; break;
+        case FS_PIPE_ARR_VIS_W_Y: src = s->vg[3][0], c = 3; break;
+        case FS_PIPE_ARR_VIS_W_Z: src = s->vg[3][1], c = 3; break;
+        case FS_PIPE_ARR_BOUNDARY: src = s->bls; break;
+        case FS_PIPE_ARR_DIVERGENCE: src = s->div; break;
+        case FS_PIPE_ARR_JUMP: src = s->jc; break;
+        case FS_PIPE_ARR_PRESSURE: src = s->p; break;
+        default: throw FsPipeError{FS_ERR_BAD_ARG, "unknown array"};
+    }
+    s->sync();
+    std::vector<double> h(s->L.total);
+    PCHECK(hipMemcpy(h.data(), src, sizeof(double) * s->L.total, hipMemcpyDeviceToHost));
+    FsBox b = s->box(c);
+    long q = 0;
+    for (int k = -FS_G; k < b.nk + FS_G; k++)
+        for (int j = -FS_G; j < b.nj + FS_G; j++)
+            for (int i = -FS_G; i < b.ni + FS_G; i++) dst[q++] = h[s->L.idx(i, j, k)];
+}
+
+void fs_pipe_async(FsPipe* s, bool async) { s->async = async; }
+void fs_pipe_sync(FsPipe* s) {
+    s->sync();
+    s->check_launch();
+}
+hipStream_t fs_pipe_stream(FsPipe* s) { return s->st; }

@@ -467,6 +467,109 @@ fs_solver* fs_batch_scene(fs_batch* b, int i);
 int  fs_batch_advance_frame(fs_batch* b, const double* dt, fs_step_stats* stats, int* rcs);
 const char* fs_batch_last_error(const fs_batch* b);
 
+/* ---- oil/water pipe mode ------------------------------------------------------
+ * EULERIAN_FLUID_SOLVER_3D in oil_water_simulation mode with the pipe along x
+ * (PIPE_OPTIONS is_parallel, WORLD_DISCRETIZATION_3D.h:91-110): core-annular
+ * flow of oil (phi <= 0) inside water in a cylinder of radius a (R2/R1, the
+ * pipe wall at sqrt(y^2 + z^2) = a in units of the core radius R1), periodic
+ * along x with the node-periodic ghost copies of
+ * FIELD_STRUCTURE_3D::FillGhostCellsPeriodicInXDirection (FIELD_STRUCTURE_3D.h:
+ * 1088-1327), the wall by theta-extrapolated ghost velocities
+ * (ADVECTION_3D.h:380-993, VISCOSITY_3D.h:2048-2625, PROJECTION_3D.h:
+ * 1521-1690), the centreline faces held fixed (ADVECTION_3D.h:279-307), the
+ * dimensionless driving pressure and gravity (EULERIAN_FLUID_SOLVER_3D.h:
+ * 2277-2433), the semi-implicit viscosity with its x-periodic rows
+ * (VISCOSITY_3D.h:565-989, POISSON_SOLVER_3D.h:342-378, 3641-3828) and the
+ * masked GFM projection with the curvature jump 1/We kappa (PROJECTION_3D.h:
+ * 414-432, 642-848, 901-1097, 1101-1449; POISSON_SOLVER_3D.h:1360-1815).
+ *
+ * Scope: the shipped script's configuration -- parallel pipe, cylindrical
+ * boundary, Neumann pressure, dimensionless form, semi-implicit viscosity,
+ * ENO3 velocity advection, WENO5 level set, boundary_order_velocity 1, the
+ * "old_fashioned" pressure assembly.  The vertical pipe, regular_boundary,
+ * the dimensional form, explicit viscosity, the CSF model, purvis_style and
+ * second-order wall extrapolation return FS_ERR_UNSUPPORTED.  The pipe must
+ * lie strictly inside the box: the first and last two cells and faces of
+ * every y and z line outside the wall (FS_ERR_BAD_ARG otherwise), nx >= 8.
+ *
+ * fs_params keys used: water_density, water_viscosity, gravity,
+ * surface_tension, order_for_time_advancing, the reinitialization keys,
+ * levelset_epsilon, tolerance, max_iteration, use_previous_pressure,
+ * stop_rule, rel_tolerance and the debugging switches; precision must be
+ * FS_FP64 and schedule FS_SCHEDULE_COMPAT (the reference's order and quirks).
+ */
+typedef struct fs_pipe_params {
+    int    is_parallel;          /* 1: pipe along x (0, the vertical pipe, is refused) */
+    double a;                    /* R2/R1 (SIMULATION_WORLD.h:124) */
+    double R2;                   /* pipe radius (:125) */
+    double A_0;                  /* interface amplitude as the script gives it; scaled by a/R2 (:126-127) */
+    double alpha;                /* interface wave number (:128) */
+    double m, eta, K, g;         /* FLUID_SOLVER_UNIFORM_OIL_WATER keys (EULERIAN_FLUID_SOLVER_3D.h:503-506) */
+    double oil_density, oil_viscosity;
+    int    dimensionless_form;   /* 1 only */
+    int    cylindrical_boundary; /* 1 only (PROJECTION_WATER cylindrical_boundary) */
+    int    boundary_order_velocity; /* 1 only (VELOCITY_ADVECTION boundary_order_velocity) */
+} fs_pipe_params;
+
+/* Replaces EULERIAN_FLUID_SOLVER_3D::InitializeFromScriptBlock in oil/water
+ * mode (EULERIAN_FLUID_SOLVER_3D.h:192-800): the derived constants R1 = R2/a,
+ * f, A, V0, J, Re, We of :496-520 and the device fields.  The handle then
+ * serves fs_set_fields / fs_get_fields (interior layout), fs_step,
+ * fs_advance_frame, fs_cfl_dt, fs_set_async / fs_synchronize /
+ * fs_get_stream and fs_destroy; every other fs_* entry point refuses it with
+ * FS_ERR_UNSUPPORTED. */
+int  fs_create_pipe(const fs_params* params, const fs_pipe_params* pipe, const fs_grid* grid, int device,
+                    fs_solver** out);
+/* The reference's initial state of the scene: the core-annular velocity
+ * profile on the x faces (EULERIAN_FLUID_SOLVER_3D.h:536-563), v = w = 0,
+ * the perturbed interface phi = sqrt(y^2 + z^2) - (A_0 cos(alpha x) + 1)
+ * (SIMULATION_WORLD.cpp:290-300), the pipe's level set sqrt(y^2 + z^2) - a
+ * (:305-319), p = 0. */
+int  fs_pipe_initial_fields(fs_solver* s);
+/* Derived constants: out[0..6] = R1, f, A, V0, J, Re, We. */
+int  fs_pipe_constants(fs_solver* s, double* out);
+
+/* One phase of the step alone, for parity tests and callers that schedule
+ * phases themselves: FS_PIPE_LEVELSET_ADVECTION (ADVECTION_3D::Solve_Levelset),
+ * FS_PIPE_VELOCITY_ADVECTION (Solve_Velocity), FS_PIPE_GRAVITY (ApplyGravity),
+ * FS_PIPE_SOURCING (Sourcing: the semi-implicit viscosity),
+ * FS_PIPE_PROJECTION (PROJECTION_3D::Solve), FS_PIPE_REINITIALIZATION
+ * (Reinitialization, one call of iteration_number sweeps). */
+#define FS_PIPE_LEVELSET_ADVECTION  0
+#define FS_PIPE_VELOCITY_ADVECTION  1
+#define FS_PIPE_GRAVITY             2
+#define FS_PIPE_SOURCING            3
+#define FS_PIPE_PROJECTION          4
+#define FS_PIPE_REINITIALIZATION    5
+int  fs_pipe_phase(fs_solver* s, int phase, double dt, fs_step_stats* stats);
+
+/* A persistent array of the step, copied over its whole box grown by 3 ghost
+ * layers ((ni + 6) * (nj + 6) * (nk + 6) doubles, x fastest; ni = nx + 1 on
+ * x faces, ...), for tests that check the reference's ghost copies. */
+#define FS_PIPE_ARR_PHI         0   /* level set with its ghosts */
+#define FS_PIPE_ARR_SCALAR_GHOST 1  /* scalar_field_ghost */
+#define FS_PIPE_ARR_GHOST_U     2   /* vector_field_mac_ghost_x, _y, _z */
+#define FS_PIPE_ARR_GHOST_V     3
+#define FS_PIPE_ARR_GHOST_W     4
+#define FS_PIPE_ARR_ADV_U_Y     5   /* ADVECTION_3D's velocity_field_mac_ghost_x_y, x_z, y_y, y_z, z_y, z_z */
+#define FS_PIPE_ARR_ADV_U_Z     6
+#define FS_PIPE_ARR_ADV_V_Y     7
+#define FS_PIPE_ARR_ADV_V_Z     8
+#define FS_PIPE_ARR_ADV_W_Y     9
+#define FS_PIPE_ARR_ADV_W_Z    10
+#define FS_PIPE_ARR_VIS_U_Y    11   /* VISCOSITY_3D's copies of the same six */
+#define FS_PIPE_ARR_VIS_U_Z    12
+#define FS_PIPE_ARR_VIS_V_Y    13
+#define FS_PIPE_ARR_VIS_V_Z    14
+#define FS_PIPE_ARR_VIS_W_Y    15
+#define FS_PIPE_ARR_VIS_W_Z    16
+#define FS_PIPE_ARR_BOUNDARY   17   /* the pipe's level set (cell) */
+#define FS_PIPE_ARR_DIVERGENCE 18
+#define FS_PIPE_ARR_JUMP       19   /* jc_on_solution */
+#define FS_PIPE_ARR_PRESSURE   20
+#define FS_PIPE_ARR_COUNT      21
+int  fs_pipe_get_array(fs_solver* s, int which, double* dst);
+
 #ifdef __cplusplus
 }
 #endif

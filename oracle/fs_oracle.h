@@ -92,6 +92,30 @@ int  fso_solve_poisson(fso_state* s, const double* source, const double* jump, d
 /* The persistent ghost copies over their boxes grown by 3 (NULL skips). */
 int  fso_get_ghost_copies(fso_state* s, double* sghost, double* gu, double* gv, double* gw);
 
+/* ---- oil/water pipe mode (fs_oracle_pipe.c) ---------------------------------
+ * The same entry points as the HIP library's fs_create_pipe handle
+ * (fs_pipe_* in include/fs_abi.h), prefix fsop_. */
+typedef struct fsop_state fsop_state;
+int  fsop_create(const fs_params* params, const fs_pipe_params* pipe, const fs_grid* grid, fsop_state** out);
+void fsop_destroy(fsop_state* s);
+int  fsop_constants(fsop_state* s, double* out);
+int  fsop_initial_fields(fsop_state* s);
+int  fsop_set_fields(fsop_state* s, const double* phi, const double* u, const double* v, const double* w,
+                     const double* p);
+int  fsop_get_fields(fsop_state* s, double* phi, double* u, double* v, double* w, double* p);
+int  fsop_phase(fsop_state* s, int phase, double dt, fs_step_stats* st);
+int  fsop_step(fsop_state* s, double dt, fs_step_stats* st);
+int  fsop_advance_frame(fsop_state* s, double dt, fs_step_stats* st);
+int  fsop_compute_curvatures(fsop_state* s);
+int  fsop_cfl_dt(fsop_state* s, double cfl, double* dt);
+int  fsop_get_array(fsop_state* s, int which, double* dst);
+int  fsop_levelset_advection(fsop_state* s, double dt);
+int  fsop_velocity_advection(fsop_state* s, double dt);
+int  fsop_apply_gravity(fsop_state* s, double dt);
+int  fsop_sourcing(fsop_state* s, double dt, fs_step_stats* st);
+int  fsop_projection(fsop_state* s, double dt, fs_step_stats* st);
+int  fsop_reinitialization(fsop_state* s, double dt);
+
 #ifdef __cplusplus
 }
 #endif
