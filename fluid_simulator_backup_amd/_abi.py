@@ -171,10 +171,7 @@ class FsWorldOptions(C.Structure):
     _fields_ = [("precision", _I), ("stop_rule", _I), ("rel_tolerance", _D), ("schedule", _I)]
 
 
-class FsWorldInfo(C.Structure):
-    _fields_ = [("params", FsParams), ("grid", FsGrid),
-                ("dt", _D), ("max_dt", _D), ("cfl", _D), ("frame_rate", _D), ("accu_dt", _D),
-                ("num_substeps", _I), ("frame", _I), ("last_frame", _I), ("n_sources", _I)]
+
 
 
 def _ptr(a):
@@ -680,6 +677,13 @@ class FsPipeParams(C.Structure):
                 ("dimensionless_form", _I), ("cylindrical_boundary", _I), ("boundary_order_velocity", _I)]
 
 
+class FsWorldInfo(C.Structure):
+    _fields_ = [("params", FsParams), ("grid", FsGrid),
+                ("dt", _D), ("max_dt", _D), ("cfl", _D), ("frame_rate", _D), ("accu_dt", _D),
+                ("num_substeps", _I), ("frame", _I), ("last_frame", _I), ("n_sources", _I),
+                ("oil_water", _I), ("pipe", FsPipeParams)]
+
+
 def pipe_params(**kw):
     """fs_params and fs_pipe_params of the reference's oil/water script
     (3D_script.sim:32-40, 195-290): core-annular flow, dimensionless form."""
@@ -754,10 +758,14 @@ class Pipe:
               "gravity": FS_PIPE_GRAVITY, "sourcing": FS_PIPE_SOURCING, "projection": FS_PIPE_PROJECTION,
               "reinitialization": FS_PIPE_REINITIALIZATION}
 
-    def __init__(self, lib, prefix, params, pipe, grid, device=0):
+    def __init__(self, lib, prefix, params, pipe, grid, device=0, handle=None):
         self.lib, self.prefix, self.params, self.pipe, self.grid = lib, prefix, params, pipe, grid
         self.nx, self.ny, self.nz = grid.nx, grid.ny, grid.nz
         self.names = bind_pipe(lib, prefix)
+        self.owned = handle is None
+        if handle is not None:  # a world's solver: the world owns it
+            self.h = C.c_void_p(handle)
+            return
         h = C.c_void_p()
         if prefix == "fs_":
             rc = lib.fs_create_pipe(C.byref(params), C.byref(pipe), C.byref(grid), device, C.byref(h))
@@ -779,7 +787,8 @@ class Pipe:
 
     def close(self):
         if getattr(self, "h", None):
-            getattr(self.lib, self.names["destroy"])(self.h)
+            if self.owned:
+                getattr(self.lib, self.names["destroy"])(self.h)
             self.h = None
 
     def __del__(self):

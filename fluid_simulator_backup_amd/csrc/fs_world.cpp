@@ -1,5 +1,5 @@
-// Scene layer: SIMULATION_WORLD for air/water scripts, over the C-ABI of the
-// timestep library.  The script keys, their defaults and the order of the
+// Scene layer: SIMULATION_WORLD for air/water and oil/water (pipe) scripts,
+// over the C-ABI of the timestep library.  The script keys, their defaults and the order of the
 // set-up follow the reference files cited at each step.
 #include "../../include/fs_world.h"
 
@@ -32,6 +32,8 @@ struct WorldSetup {
     int num_substeps = 1, last_frame = 0;
     bool water_object = false;
     std::vector<fs_levelset_source> sources;
+    bool oil_water = false;  // the pipe mode (fs_create_pipe)
+    fs_pipe_params Q{};
 };
 
 // GRID_STRUCTURE_3D::InitializeFromBlock (GRID_STRUCTURE_3D.h:201-219); the
@@ -68,6 +70,113 @@ fs_grid grid_from_block(const fs::ScriptBlock& b, double solver_scale, double* w
     return g;
 }
 
+// the options of this build shared by both modes
+void build_options(fs_params& P, const fs_world_options* opts) {
+    P.stop_rule = opts ? opts->stop_rule : FS_STOP_ABS_PRECOND;
+    P.rel_tolerance = opts ? opts->rel_tolerance : 1e-8;
+    P.precision = opts ? opts->precision : FS_FP64;
+    P.schedule = opts ? opts->schedule : FS_SCHEDULE_COMPAT;
+}
+
+// SIMULATION_WORLD::InitializeFormScript for oil_water_simulation scripts:
+// the pipe (SIMULATION_WORLD.h:122-129), WORLD_DISCRETIZATION_OIL_WATER
+// (WORLD_DISCRETIZATION_3D.h:91-110) and FLUID_SOLVER_UNIFORM_OIL_WATER
+// (EULERIAN_FLUID_SOLVER_3D.h:192-260, 496-520, ADVECTION_3D.h:100-113,
+// PROJECTION_3D.h:178-228, VISCOSITY_3D.h:184-231)
+WorldSetup parse_oil_water(const fs::Script& sc, const fs::ScriptBlock& world, const fs_world_options* opts,
+                           WorldSetup w) {
+    w.oil_water = true;
+    fs_pipe_params& Q = w.Q;
+    Q.a = world.GetFloat("a", 1);
+    Q.R2 = world.GetFloat("R2", 1);
+    Q.A_0 = world.GetFloat("A_0", 1);  // scaled by a/R2 in fs_create_pipe, SIMULATION_WORLD.h:127
+    Q.alpha = world.GetFloat("alpha", 1);
+    const fs::ScriptBlock& wd = world.Find("WORLD_DISCRETIZATION_OIL_WATER");
+    const fs::ScriptBlock& po = wd.Find("PIPE_OPTIONS");
+    const bool vertical = po.GetBoolean("is_vertical", false), parallel = po.GetBoolean("is_parallel", false);
+    if (vertical) unsupported("the vertical pipe (PIPE_OPTIONS is_vertical)");
+    if (!parallel) unsupported("PIPE_OPTIONS sets neither is_vertical nor is_parallel");
+    Q.is_parallel = 1;
+    const fs::ScriptBlock& fb = sc.Find("FLUID_SOLVER_UNIFORM_OIL_WATER");
+    double wdx = 0;
+    w.G = grid_from_block(wd.Find("GRID_STRUCTURE_3D_PARALLEL"), fb.GetFloat("resolution_scale", 1.0), &wdx);
+    if (w.G.nx < 8 || w.G.ny < 8 || w.G.nz < 8) throw WorldFail{FS_ERR_BAD_ARG, "pipe grid below 8 cells per axis"};
+    fs_params& P = w.P;
+    if (!fb.GetBoolean("use_water_solver", true)) unsupported("use_water_solver = false");
+    if (fb.GetInteger("ghost_width", 3) != 3) unsupported("ghost_width other than 3");
+    P.surface_tension = fb.GetFloat("surface_tension", 0);
+    if (fb.GetBoolean("fastsweeping_reinitialization", false)) unsupported("fastsweeping_reinitialization");
+    P.sussmanstyle_reinitialization = fb.GetBoolean("sussmanstyle_reinitialization", false);
+    P.scaling_number_for_reinitialization = fb.GetInteger("scaling_number_for_reinitialization", 5);
+    P.iteration_number_for_reinitialization = fb.GetInteger("iteration_number_for_reinitialization", 10);
+    P.use_delta_function_formulation = fb.GetBoolean("use_delta_function_formulation", false);
+    if (!P.use_delta_function_formulation)
+        unsupported("use_delta_function_formulation = false in the pipe mode (no viscosity and no jump)");
+    if (fb.GetBoolean("CSF_model", false)) unsupported("CSF_model in the pipe mode");
+    if (fb.GetBoolean("curvature_by_normal_vector", false) || !fb.GetBoolean("curvature_by_levelset", false))
+        unsupported("curvature other than by level set in the pipe mode");
+    P.curvature_method = FS_CURV_LEVELSET;
+    P.water_viscosity = fb.GetFloat("water_viscosity", 1);
+    P.water_density = fb.GetFloat("water_density", 1);
+    Q.oil_viscosity = fb.GetFloat("oil_viscosity", 0);
+    Q.oil_density = fb.GetFloat("oil_density", 1);
+    const double g0[3] = {0, -9.8, 0};
+    fb.GetVector3("gravity", g0, P.gravity);
+    P.order_for_time_advancing = fb.GetInteger("order_for_time_advancing", 1);
+    P.is_levelset_advection_active = fb.GetBoolean("is_Levelset_advection_active", false);
+    P.is_velocity_advection_active = fb.GetBoolean("is_Velocity_advection_active", false);
+    P.is_external_force_active = fb.GetBoolean("is_external_force_active", false);
+    P.is_sourcing_active = fb.GetBoolean("is_sourcing_active", false);
+    P.is_projection_active = fb.GetBoolean("is_projection_active", false);
+    if (!fb.GetBoolean("semi_implicit_approach", false))
+        unsupported("the explicit viscosity of the pipe mode (semi_implicit_approach = false)");
+    P.semi_implicit_viscosity = 1;
+    Q.dimensionless_form = fb.GetBoolean("dimensionless_form", false) ? 1 : 0;
+    if (!Q.dimensionless_form) unsupported("the dimensional form of the pipe mode (dimensionless_form = false)");
+    if (!fb.GetBoolean("is_periodic", false)) unsupported("a pipe that is not periodic (is_periodic = false)");
+    Q.m = fb.GetFloat("m", 1);
+    Q.eta = fb.GetFloat("eta", 1);
+    Q.K = fb.GetFloat("K", 1);
+    Q.g = fb.GetFloat("g", 1);
+    const fs::ScriptBlock& ls = fb.Find("LEVELSET_ADVECTION");
+    if (!ls.GetBoolean("use_5th_weno", false)) unsupported("level-set advection other than use_5th_weno");
+    P.levelset_epsilon = ls.GetFloat("epsilon", 10e-6);
+    const fs::ScriptBlock& va = fb.Find("VELOCITY_ADVECTION");
+    if (va.GetBoolean("use_5th_weno_v", false) || !va.GetBoolean("use_3rd_eno_v", false))
+        unsupported("velocity advection other than use_3rd_eno_v in the pipe mode");
+    P.use_5th_weno_v = 0;
+    P.velocity_epsilon = va.GetFloat("epsilon", 10e-6);
+    Q.boundary_order_velocity = va.GetInteger("boundary_order_velocity", 1);
+    if (Q.boundary_order_velocity != 1) unsupported("boundary_order_velocity other than 1");
+    const fs::ScriptBlock& pw = fb.Find("PROJECTION_WATER");
+    P.tolerance = pw.GetFloat("tolerance", 1e-4);
+    P.max_iteration = pw.GetInteger("max_iteration", 30);
+    P.use_previous_pressure = pw.GetBoolean("use_previous_pressure", false);
+    if (!P.use_previous_pressure) unsupported("use_previous_pressure = false in the pipe mode");
+    if (pw.GetString("poisson_solver_type", "NULL") != "PCG")
+        unsupported("poisson_solver_type other than \"PCG\" (CG, multigrid, hybrid, Gauss-Seidel)");
+    if (!pw.GetBoolean("use_variable_density", false)) unsupported("use_variable_density = false");
+    if (pw.GetBoolean("Dirichlet_Boundary_Condition", false) || !pw.GetBoolean("Neumann_Boundary_Condition", false))
+        unsupported("pressure boundary conditions other than Neumann");
+    if (pw.GetBoolean("regular_boundary", false) || !pw.GetBoolean("cylindrical_boundary", false))
+        unsupported("a pipe boundary other than cylindrical_boundary");
+    Q.cylindrical_boundary = 1;
+    if (!pw.GetBoolean("old_fashioned", true) || pw.GetBoolean("purvis_style", false))
+        unsupported("the Purvis-style jump discretisation");
+    build_options(P, opts);
+    // the WATER object seeds the interface (SIMULATION_WORLD.cpp:130-319);
+    // other objects are outside this mode
+    const fs::ScriptBlock& objs = sc.Find("OBJECT_LIST");
+    for (const fs::ScriptBlock& o : objs.children) {
+        const std::string shape = o.GetString("shape_type", "NONE"), type = o.GetString("object_type", "NONE");
+        if (shape == "WATER") w.water_object = true;
+        else if (type == "RIGID" || shape == "CYLINDER" || shape == "OBJ" || shape == "SPHERE")
+            unsupported("objects other than the WATER block in the pipe mode (" + o.name + ")");
+    }
+    if (!w.water_object) unsupported("no OBJECT_LIST block of shape_type WATER (the interface would not be seeded)");
+    return w;
+}
+
 WorldSetup parse(const std::string& path, const fs_world_options* opts) {
     fs::Script sc;
     std::string err;
@@ -85,7 +194,8 @@ WorldSetup parse(const std::string& path, const fs_world_options* opts) {
     if (!world.GetBoolean("use_eulerian_solver")) unsupported("use_eulerian_solver = false (no solver to advance)");
     const bool air_water = world.GetBoolean("air_water_simulation", false);
     const bool oil_water = world.GetBoolean("oil_water_simulation", false);
-    if (oil_water) unsupported("oil_water_simulation (the pipe mode) is outside this build");
+    if (air_water && oil_water) unsupported("both air_water_simulation and oil_water_simulation are set");
+    if (oil_water) return parse_oil_water(sc, world, opts, w);
     if (!air_water) unsupported("neither air_water_simulation nor oil_water_simulation is set");
 
     // the solver block, EULERIAN_FLUID_SOLVER_3D::InitializeFromScriptBlock,
@@ -169,11 +279,7 @@ WorldSetup parse(const std::string& path, const fs_world_options* opts) {
     if (!pw.GetBoolean("old_fashioned", true) || pw.GetBoolean("purvis_style", false))
         unsupported("the Purvis-style jump discretisation");
 
-    // options of this build
-    P.stop_rule = opts ? opts->stop_rule : FS_STOP_ABS_PRECOND;
-    P.rel_tolerance = opts ? opts->rel_tolerance : 1e-8;
-    P.precision = opts ? opts->precision : FS_FP64;
-    P.schedule = opts ? opts->schedule : FS_SCHEDULE_COMPAT;
+    build_options(P, opts);
 
     // SIMULATION_WORLD::InitializeObjectListFromScript, SIMULATION_WORLD.cpp:4-480
     const fs::ScriptBlock& objs = sc.Find("OBJECT_LIST");
@@ -216,6 +322,8 @@ void fill_info(const WorldSetup& w, fs_world_info* info) {
     info->num_substeps = w.num_substeps;
     info->last_frame = w.last_frame;
     info->n_sources = (int)w.sources.size();
+    info->oil_water = w.oil_water ? 1 : 0;
+    info->pipe = w.Q;
 }
 
 }  // namespace
@@ -350,7 +458,16 @@ int fs_world_create(const char* script_path, const fs_world_options* opts, int d
         return FS_ERR_OOM;
     }
     const WorldSetup& S = w->setup;
-    int rc = fs_create(&S.P, &S.G, device, &w->solver);
+    int rc;
+    if (S.oil_water) {
+        // the pipe's level set, the interface and the core-annular flow
+        // (EULERIAN_FLUID_SOLVER_3D.h:496-563, SIMULATION_WORLD.cpp:288-319)
+        if ((rc = fs_create_pipe(&S.P, &S.Q, &S.G, device, &w->solver)) != FS_OK) return rc;
+        if ((rc = fs_pipe_initial_fields(w->solver)) != FS_OK) return rc;
+        *out = w.release();
+        return FS_OK;
+    }
+    rc = fs_create(&S.P, &S.G, device, &w->solver);
     if (rc != FS_OK) return rc;
     // SIMULATION_WORLD.cpp:116-128: in air/water mode the WATER object's level
     // set is the sphere of radius 1/3 about the origin, sampled at the
@@ -407,14 +524,21 @@ int fs_world_get_info(const fs_world* w, fs_world_info* info) {
 int fs_world_advance_frame(fs_world* w, fs_step_stats* stats) {
     if (!w) return FS_ERR_BAD_ARG;
     const WorldSetup& S = w->setup;
-    int rc = fs_set_frame(w->solver, w->frame);
-    if (rc != FS_OK) return rc;
-    // SIMULATION_WORLD::AdvanceOneFrame, SIMULATION_WORLD.h:320-330
-    if ((rc = fs_compute_curvatures(w->solver)) != FS_OK) return rc;
+    int rc;
+    // SIMULATION_WORLD::AdvanceOneFrame, SIMULATION_WORLD.h:320-330: the
+    // level set's ComputeCurvatures first.  A pipe handle has no level-set
+    // sources; its fs_advance_frame is ComputeCurvatures and a step, and its
+    // CFL takes the curvature maximum of the current level set itself.
+    bool curvature_pending = S.oil_water;
+    if (!S.oil_water) {
+        if ((rc = fs_set_frame(w->solver, w->frame)) != FS_OK) return rc;
+        if ((rc = fs_compute_curvatures(w->solver)) != FS_OK) return rc;
+    }
     fs_step_stats sum{}, st{};
     int result = FS_OK;
     auto step = [&](double dt) {
-        int r = fs_step(w->solver, dt, &st);
+        int r = curvature_pending ? fs_advance_frame(w->solver, dt, &st) : fs_step(w->solver, dt, &st);
+        curvature_pending = false;
         if (r != FS_OK && r != FS_ERR_NOT_CONVERGED) return r;
         if (r == FS_ERR_NOT_CONVERGED) result = r;
         int its = sum.pcg_iterations + st.pcg_iterations, vits = sum.viscosity_iterations + st.viscosity_iterations;

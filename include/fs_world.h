@@ -1,7 +1,7 @@
 /*
  * fs_world.h -- C-ABI of the scene layer above the timestep library: the
  * reference's SIMULATION_WORLD (SIMULATION_WORLD.h, SIMULATION_WORLD.cpp)
- * for air/water scenes, driven by its .sim scripts, and the script reader
+ * for air/water and oil/water pipe scenes, driven by its .sim scripts, and the script reader
  * (SCRIPT_READER.h) it parses them with.
  *
  * A world owns one fs_solver (fs_abi.h) and advances it frame by frame as
@@ -64,8 +64,16 @@ int  fs_script_child_name(const fs_script* s, const char* block_path, int i, cha
  *     (fs_set_levelset_sources) with their FLUID_CONTROL_OPTIONS
  *     (FLUID_CONTROL_OPTIONS.h:45-61), applied once at frame 0
  *     (SIMULATION_WORLD.h:224-227) and in every Sourcing phase.
+ * For oil_water_simulation scripts (the shipped 3D_script.sim) it sets up
+ * the pipe instead (fs_create_pipe, fs_pipe_initial_fields): a, R2, A_0,
+ * alpha of block SIMULATION_WORLD (SIMULATION_WORLD.h:122-129), the grid of
+ * WORLD_DISCRETIZATION_OIL_WATER.GRID_STRUCTURE_3D_PARALLEL
+ * (WORLD_DISCRETIZATION_3D.h:91-110), the keys of
+ * FLUID_SOLVER_UNIFORM_OIL_WATER and its blocks (m, eta, K, g, oil_density,
+ * oil_viscosity, boundary_order_velocity, cylindrical_boundary, ...), and the
+ * interface and flow the scene starts from (SIMULATION_WORLD.cpp:130-319).
  * FS_ERR_UNSUPPORTED names what a script asks for that this build does not
- * run: oil/water pipe scenes, rigid, cylinder or mesh objects, other Poisson
+ * run: the pipe mode outside fs_create_pipe's scope, rigid, cylinder or mesh objects, other Poisson
  * solvers or boundary conditions, curvature by normals, the CSF model, a
  * missing WATER object (the level set would keep its zeros).
  * opts may be NULL (FP64, the reference's stop rule, compatible schedule). */
@@ -86,6 +94,8 @@ typedef struct fs_world_info {
     int    frame;           /* frames advanced */
     int    last_frame;
     int    n_sources;
+    int    oil_water;       /* 1: an oil_water_simulation script (a pipe handle, fs_create_pipe) */
+    fs_pipe_params pipe;    /* its pipe keys (oil_water only) */
 } fs_world_info;
 
 int  fs_world_create(const char* script_path, const fs_world_options* opts, int device, fs_world** out);

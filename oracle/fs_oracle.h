@@ -115,6 +115,8 @@ int  fsop_apply_gravity(fsop_state* s, double dt);
 int  fsop_sourcing(fsop_state* s, double dt, fs_step_stats* st);
 int  fsop_projection(fsop_state* s, double dt, fs_step_stats* st);
 int  fsop_reinitialization(fsop_state* s, double dt);
+/* the pipe mode's sin polynomial (|x| <= pi) */
+double fsop_sin(double x);
 
 #ifdef __cplusplus
 }

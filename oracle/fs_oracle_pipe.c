@@ -729,6 +729,9 @@ static double pipe_sin(double x) {
     return neg ? -r : r;
 }
 
+/* the polynomial alone, for the tests' accuracy check */
+double fsop_sin(double x) { return pipe_sin(x); }
+
 /* EULERIAN_FLUID_SOLVER_3D::HeavisideFunction (:2459-2479), the form
  * PROJECTION_3D::HeavisideFunction also uses (PROJECTION_3D.h:1452-1472) */
 static double heav_solver(double phi, double eps) {

@@ -115,3 +115,18 @@ def test_pipe_refusals(hip_lib):
     # the air/water-only entry points refuse a pipe handle
     rc = hip_lib.fs_compute_curvatures(s.h)
     assert rc == _abi.FS_ERR_UNSUPPORTED
+
+
+def test_pipe_shipped_scene_at_its_size(hip_lib, oracle_lib):
+    """The reference script's scene at its own 51^3 grid, eight frames with
+    its CFL 0.9: bit for bit every frame, and each projection converged to
+    the script's tolerance (1e-6 on sqrt(r^T D^-1 r))."""
+    gpu, cpu = pair(hip_lib, oracle_lib, n=51)
+    for f in range(8):
+        dt = cpu.cfl_dt(0.9)
+        assert gpu.cfl_dt(0.9) == dt
+        say(f"[pipe] 51^3 frame {f} dt={dt}")
+        sg, sc = gpu.advance_frame(dt), cpu.advance_frame(dt)
+        assert sg["pcg_iterations"] == sc["pcg_iterations"]
+        assert sg["residual_precond"] < 1e-6
+        assert_fields_equal(gpu, cpu, f"51^3 frame {f}")

@@ -123,3 +123,31 @@ def test_world_fp32_refine_and_fixed_schedule(hip_lib, precision):
     f = world.solver.get_fields()
     assert all(np.isfinite(f[c]).all() for c in f)
     world.close()
+
+
+def test_pipe_world_matches_checker(hip_lib, oracle_lib):
+    """An oil_water_simulation script (the pipe) through fs_world_*: the
+    world's pipe handle against the CPU checker's pipe driven through the same
+    loop (the CFL step with the curvature of the current level set, then
+    ComputeCurvatures and the step), bit for bit over three frames."""
+    path = os.path.join(SCRIPTS, "oil_water_pipe.sim")
+    world = _abi.World(hip_lib, path, 0)
+    info = world.info()
+    assert info.oil_water == 1 and info.cfl == 0.9
+    gpu = _abi.Pipe(hip_lib, "fs_", info.params, info.pipe, info.grid, handle=hip_lib.fs_world_solver(world.h))
+    cpu = _abi.Pipe(oracle_lib, "fsop_", info.params, info.pipe, info.grid)
+    cpu.initial_fields()
+    for c, a in gpu.get_fields().items():
+        assert np.array_equal(a, cpu.get_fields()[c]), c
+    accu = 0.0
+    for f in range(3):
+        dt = cpu.cfl_dt(info.cfl)
+        sc = cpu.advance_frame(dt)
+        sg = world.advance_frame()
+        accu += dt
+        assert world.info().accu_dt == accu
+        assert sg["pcg_iterations"] == sc["pcg_iterations"]
+    fg, fc = gpu.get_fields(), cpu.get_fields()
+    for c in fc:
+        assert np.array_equal(fg[c], fc[c]), c
+    world.close()

@@ -136,8 +136,8 @@ def test_unsupported_scenes_are_named(host_lib, tmp_path):
 @pytest.mark.skipif(not os.path.exists(REFERENCE_SCRIPT), reason="reference scripts not present")
 def test_reference_script_parses(host_lib):
     """The reference's own script, read where it lies: its air/water solver
-    block and grid come out as written, and the scene it selects (the
-    oil/water pipe) is reported as outside this build."""
+    block and grid come out as written, and the scene it selects, the
+    oil/water pipe, sets up the pipe handle's keys as written."""
     s = _abi.Script(host_lib, REFERENCE_SCRIPT)
     assert s.float("FLUID_SOLVER_UNIFORM_AIR_WATER", "surface_tension") == 0.0728
     assert s.vector3("FLUID_SOLVER_UNIFORM_AIR_WATER", "gravity") == (0.0, -9.8, 0.0)
@@ -149,6 +149,15 @@ def test_reference_script_parses(host_lib):
     # the commented-out SOURCE_SPHERE block is gone; WATER stays
     assert s.children("OBJECT_LIST") == ["WATER"]
     s.close()
-    with pytest.raises(_abi.FsError) as e:
-        _abi.parse_world(host_lib, REFERENCE_SCRIPT)
-    assert e.value.code == _abi.FS_ERR_UNSUPPORTED
+    info, _ = _abi.parse_world(host_lib, REFERENCE_SCRIPT)
+    assert info.oil_water == 1
+    g, q, p = info.grid, info.pipe, info.params
+    assert (g.nx, g.ny, g.nz) == (51, 51, 51)
+    assert (g.x_min, g.x_max, g.y_min, g.z_max) == (0.0, 3.141592, -1.570796, 1.570796)
+    assert (q.a, q.R2, q.A_0, q.alpha) == (1.28, 0.47625, 0.001, 2.0)
+    assert (q.m, q.eta, q.K, q.g) == (0.00166, 1.1, -0.4552, -980.0)
+    assert (q.oil_density, q.oil_viscosity, q.is_parallel, q.dimensionless_form) == (0.905, 6.01, 1, 1)
+    assert (p.water_density, p.water_viscosity, p.surface_tension) == (0.995, 0.01, 8.54)
+    assert tuple(p.gravity) == (0.0, -980.0, 0.0) and p.order_for_time_advancing == 2
+    assert (p.scaling_number_for_reinitialization, p.iteration_number_for_reinitialization) == (2, 10)
+    assert p.tolerance == 1e-6 and info.cfl == 0.9
