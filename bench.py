@@ -30,7 +30,8 @@ Other workloads (each its own JSON line): `--workload batch` (48 uneven
 scenes advanced together against one at a time), `--workload adjoint` (the
 checkpointed rollout gradient against central differences), `--workload
 projection` (one projection of the 64-bubble pool, density ratio 1000, at
-512^3 per path).
+512^3 per path), `--workload pipe` (the reference script's oil/water pipe
+scene, 51^3 by default).
 """
 import argparse
 import json
@@ -66,10 +67,11 @@ def parse():
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run a 1 GiB device copy first (profiler byte-counter calibration)")
     ap.add_argument("--workload", default="timestep",
-                    choices=["timestep", "batch", "adjoint", "projection", "sensitivity"],
+                    choices=["timestep", "batch", "adjoint", "projection", "sensitivity", "pipe"],
                     help="batch: 48 uneven scenes advanced together vs one at a time; adjoint: checkpointed "
                          "rollout gradient checked against central differences; projection: one projection of "
-                         "the 64-bubble pool (default 512^3) per path (each its own JSON line)")
+                         "the 64-bubble pool (default 512^3) per path; pipe: the oil/water pipe scene (default "
+                         "51^3) (each its own JSON line)")
     ap.add_argument("--max-iteration", type=int, default=40000, help="projection workload: PCG iteration cap")
     ap.add_argument("--batch-path", default="mg", choices=["mg", "refine"],
                     help="batch workload: the FP32 projection inside FP64 refinement (multigrid or Jacobi)")
@@ -738,6 +740,70 @@ def jacobi_roofline(R, head, scene, n):
     return roof
 
 
+def pipe_main(a):
+    """The oil/water pipe scene of the reference's script (core-annular flow,
+    3D_script.sim:32-99, 195-290) at n^3 (default its 51^3): CFL 0.9 before
+    every frame as SIMULATION_WORLD does, RK2, the semi-implicit viscosity and
+    the projection solved to the script's tolerance.  CPU baseline: the CPU
+    checker's frames of the same scene from the same state."""
+    import torch
+
+    import fluid_simulator_backup_amd as fsa
+    from fluid_simulator_backup_amd import _abi
+    torch.cuda.set_device(0)
+    lib = fsa.load_library()
+    n = a.n or 51
+    P, Q = _abi.pipe_params()
+    g = _abi.pipe_grid(n)
+    s = _abi.Pipe(lib, "fs_", P, Q, g)
+    s.initial_fields()
+    print(f"[bench] pipe {n}^3 warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    stats = []
+    for _ in range(a.warmup):
+        s.advance_frame(s.cfl_dt(0.9))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for f in range(a.steps):
+        stats.append(s.advance_frame(s.cfl_dt(0.9)))
+        if (f + 1) % 10 == 0:
+            print(f"[bench] pipe frame {f + 1}/{a.steps}", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    t = time.perf_counter() - t0
+    ms = t * 1e3 / a.steps
+    fields = s.get_fields()
+    s.close()
+    cells = n ** 3
+    out = {
+        "metric": "ms/frame, oil/water pipe (core-annular flow), the reference script's scene",
+        "value": round(ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
+        "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
+        "dtype": "f64", "data": "synthetic (the script's analytic initial state)",
+        "config": {"workload": f"oil/water pipe {n}^3, parallel pipe a = 1.28, dimensionless, RK2, CFL 0.9, "
+                               f"projection and semi-implicit viscosity to the script's tolerance 1e-6",
+                   "cells": cells},
+        "pcg_iterations_per_step": float(np.mean([st["pcg_iterations"] for st in stats])),
+        "viscosity_iterations_per_step": float(np.mean([st["viscosity_iterations"] for st in stats])),
+        "finite": bool(all(np.all(np.isfinite(v)) for v in fields.values())),
+    }
+    if not a.no_cpu_baseline:
+        orc = _abi.load(os.path.join(ROOT, "oracle", "libfs_oracle.so"), "fso_")
+        c = _abi.Pipe(orc, "fsop_", P, Q, g)
+        c.initial_fields()
+        for _ in range(a.warmup):
+            c.advance_frame(c.cfl_dt(0.9))
+        frames = max(1, min(a.steps, int(8 * (51 / n) ** 3) or 1))
+        t0 = time.perf_counter()
+        for _ in range(frames):
+            c.advance_frame(c.cfl_dt(0.9))
+        tc = time.perf_counter() - t0
+        c.close()
+        out["cpu_baseline"] = {"value": round(tc * 1e3 / frames, 1), "unit": "ms/step", "cores": 1, "kind": "port",
+                               "sample": f"CPU checker (C, FP64, one thread) of the same scene: frames "
+                                         f"{a.warmup + 1}-{a.warmup + frames} after the same warmup, timed end to end"}
+        out["speedup_vs_cpu"] = round(tc * 1e3 / frames / ms, 2)
+    print(json.dumps(out))
+
+
 def main():
     a = parse()
     if a.gpus != 1:
@@ -755,6 +821,9 @@ def main():
         return
     if a.workload == "sensitivity":
         sensitivity_main(a)
+        return
+    if a.workload == "pipe":
+        pipe_main(a)
         return
     a.n = a.n or 256
     import torch

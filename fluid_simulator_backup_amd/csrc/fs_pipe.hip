@@ -593,6 +593,17 @@ struct FsPipePcg {
     double tol2, rtol;
 };
 
+// the second level of the fixed order: lane t adds the block sums t, t +
+// RBLK, ... in order, then the binary tree over the lanes; every thread of
+// the (single, RBLK-thread) workgroup returns the total
+__device__ __forceinline__ double sum_parts(const double* __restrict__ part, int nb, double* sh) {
+    double v = 0;
+    for (int q = threadIdx.x; q < nb; q += RBLK) v += part[q];
+    double t = block_tree(v, sh);
+    __syncthreads();
+    return t;
+}
+
 // partial[blk] = tree over the block of a*b (b == NULL: a)
 __global__ void __launch_bounds__(RBLK) k_dot_part(const double* __restrict__ a, const double* __restrict__ b, int N,
                                                    double* __restrict__ part) {
@@ -602,12 +613,11 @@ __global__ void __launch_bounds__(RBLK) k_dot_part(const double* __restrict__ a,
     double t = block_tree(v, sh);
     if (threadIdx.x == 0) part[blockIdx.x] = t;
 }
-// the block sums in block order into *out
-__global__ void k_sum_parts(const double* __restrict__ part, int nb, double* out) {
-    if (threadIdx.x != 0) return;
-    double total = 0;
-    for (int q = 0; q < nb; q++) total += part[q];
-    *out = total;
+// the total of the block sums into *out (one RBLK-thread workgroup)
+__global__ void __launch_bounds__(RBLK) k_sum_parts(const double* __restrict__ part, int nb, double* out) {
+    __shared__ double sh[RBLK];
+    double t = sum_parts(part, nb, sh);
+    if (threadIdx.x == 0) *out = t;
 }
 
 // b -= (1/N) sum(b), POISSON_SOLVER_3D.h:221-246
@@ -666,12 +676,10 @@ __global__ void __launch_bounds__(RBLK) k_pcg_begin(const double* __restrict__ b
         part[2 * nb + blockIdx.x] = t2;
     }
 }
-__global__ void k_pcg_begin_scalars(const double* __restrict__ part, int nb, FsPipePcg* S) {
+__global__ void __launch_bounds__(RBLK) k_pcg_begin_scalars(const double* __restrict__ part, int nb, FsPipePcg* S) {
+    __shared__ double sh[RBLK];
+    const double a = sum_parts(part, nb, sh), b = sum_parts(part + nb, nb, sh), c = sum_parts(part + 2 * nb, nb, sh);
     if (threadIdx.x != 0) return;
-    double a = 0, b = 0, c = 0;
-    for (int q = 0; q < nb; q++) a += part[q];
-    for (int q = 0; q < nb; q++) b += part[nb + q];
-    for (int q = 0; q < nb; q++) c += part[2 * nb + q];
     S->rz = a;
     S->bb = b;
     S->rr = c;
@@ -682,10 +690,11 @@ __global__ void k_pcg_begin_scalars(const double* __restrict__ part, int nb, FsP
     if (S->max_iter <= 0) S->done = S->done ? S->done : 2;
 }
 // alpha = r.z / p.q
-__global__ void k_pcg_alpha(const double* __restrict__ part, int nb, FsPipePcg* S) {
-    if (threadIdx.x != 0 || S->done) return;
-    double t = 0;
-    for (int q = 0; q < nb; q++) t += part[q];
+__global__ void __launch_bounds__(RBLK) k_pcg_alpha(const double* __restrict__ part, int nb, FsPipePcg* S) {
+    __shared__ double sh[RBLK];
+    if (S->done) return;
+    const double t = sum_parts(part, nb, sh);
+    if (threadIdx.x != 0) return;
     S->pq = t;
     S->alpha = S->rz / t;
 }
@@ -715,15 +724,15 @@ __global__ void __launch_bounds__(RBLK) k_pcg_update(double* __restrict__ x, dou
     }
 }
 // the stop test and beta (PCG_METHOD.cpp:81-101)
-__global__ void k_pcg_tail(const double* __restrict__ part, int nb, FsPipePcg* S) {
-    if (threadIdx.x != 0 || S->done) return;
-    double t = 0;
-    for (int q = 0; q < nb; q++) t += part[q];
+__global__ void __launch_bounds__(RBLK) k_pcg_tail(const double* __restrict__ part, int nb, FsPipePcg* S) {
+    __shared__ double sh[RBLK];
+    if (S->done) return;
+    const double t = sum_parts(part, nb, sh);
+    const double rr = S->rel ? sum_parts(part + nb, nb, sh) : 0;
+    if (threadIdx.x != 0) return;
     S->rz_old = S->rz;
     S->rz = t;
     if (S->rel) {
-        double rr = 0;
-        for (int q = 0; q < nb; q++) rr += part[nb + q];
         S->rr = rr;
         if (sqrt(rr) < S->rtol * sqrt(S->bb)) {
             S->done = 1;
@@ -1346,14 +1355,14 @@ struct FsPipe {
         PCHECK(hipMemcpyAsync(pcg, host_pcg, sizeof(FsPipePcg), hipMemcpyHostToDevice, st));
         hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv, nullptr, nullptr);
         hipLaunchKernelGGL(k_pcg_begin, dim3(nb), dim3(RBLK), 0, st, rhs, qv, dg, rv, zv, pv, N, part, nb);
-        hipLaunchKernelGGL(k_pcg_begin_scalars, dim3(1), dim3(64), 0, st, part, nb, pcg);
+        hipLaunchKernelGGL(k_pcg_begin_scalars, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
         int batch = 8, enq = 0;
         while (true) {
             for (int q = 0; q < batch; q++) {
                 hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, pv, qv, part, pcg);
-                hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(64), 0, st, part, nb, pcg);
+                hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
                 hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(RBLK), 0, st, xv, rv, zv, pv, qv, dg, N, part, nb, pcg);
-                hipLaunchKernelGGL(k_pcg_tail, dim3(1), dim3(64), 0, st, part, nb, pcg);
+                hipLaunchKernelGGL(k_pcg_tail, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
                 hipLaunchKernelGGL(k_pcg_direction, dim3(nb), dim3(RBLK), 0, st, pv, zv, N, pcg);
             }
             enq += batch;
@@ -1370,7 +1379,7 @@ struct FsPipe {
         hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv, nullptr, nullptr);
         hipLaunchKernelGGL(k_residual, dim3(nb), dim3(RBLK), 0, st, rhs, qv, rv, N);
         hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rv, rv, N, part);
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part, nb, dscal);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, part, nb, dscal);
         PCHECK(hipMemcpyAsync(host_bits, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
         sync();
         const double bnorm = std::sqrt(host_pcg->bb);
@@ -1461,7 +1470,7 @@ struct FsPipe {
                            p, val, col, cnt, dg, rhs, xv, L, S);
         // Neumann compatibility, POISSON_SOLVER_3D.h:221-246
         hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rhs, nullptr, N, part);
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(64), 0, st, part, nb, dscal);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, part, nb, dscal);
         hipLaunchKernelGGL(k_remove_mean, dim3(nb), dim3(RBLK), 0, st, rhs, N, dscal);
         last_iters = Pcg(N);
         hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, p, xv, sys[0].rows, N, L, 0);

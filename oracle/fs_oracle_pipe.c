@@ -822,21 +822,34 @@ static double sys_diag(const psys_t* S, int r) {
  * kernels (the reference's order depends on its thread count,
  * MULTITHREADING.h:229-250): rows in blocks of 256, each block summed by a
  * binary tree (element t += element t + h for h = 128, 64, ..., 1; rows past
- * the end count as +0), the block sums added in block order.  b == NULL
- * sums a. */
+ * the end count as +0) into a block sum; then 256 lanes, lane t adding the
+ * block sums t, t + 256, t + 512, ... in order from +0, and the same binary
+ * tree over the 256 lanes.  b == NULL sums a. */
 #define RBLK 256
+static void tree256(double* t) {
+    for (int h = RBLK / 2; h >= 1; h /= 2)
+        for (int q = 0; q < h; q++) t[q] += t[q + h];
+}
 static double dot_(const double* a, const double* b, int n) {
-    double total = 0, t[RBLK];
-    for (int b0 = 0; b0 < n; b0 += RBLK) {
+    const int nb = (n + RBLK - 1) / RBLK;
+    double* part = (double*)malloc(sizeof(double) * (size_t)(nb > 0 ? nb : 1));
+    double t[RBLK];
+    for (int blk = 0; blk < nb; blk++) {
         for (int q = 0; q < RBLK; q++) {
-            int r = b0 + q;
+            int r = blk * RBLK + q;
             t[q] = r < n ? (b ? a[r] * b[r] : a[r]) : (double)0;
         }
-        for (int h = RBLK / 2; h >= 1; h /= 2)
-            for (int q = 0; q < h; q++) t[q] += t[q + h];
-        total += t[0];
+        tree256(t);
+        part[blk] = t[0];
     }
-    return total;
+    for (int q = 0; q < RBLK; q++) {
+        double v = 0;
+        for (int p = q; p < nb; p += RBLK) v += part[p];
+        t[q] = v;
+    }
+    tree256(t);
+    free(part);
+    return t[0];
 }
 
 /* PCG_METHOD::PCGMethod with the Jacobi preconditioner (PCG_METHOD.cpp:4-160)
