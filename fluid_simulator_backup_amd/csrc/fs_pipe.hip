@@ -498,6 +498,7 @@ __global__ void k_visc_assemble(const long* __restrict__ rows, int N, const int*
     n++;
     cnt[r] = n;
     dg[r] = d;
+    for (int e = n; e < PSLOTS; e++) val[e * N + r] = 0, col[e * N + r] = r;  // padding the PCG loop reads
     double bv = expl[o];
     const double* dir[6] = {vx, vx, vy, vy, vz, vz};
     for (int e = 0; e < 6; e++)
@@ -550,6 +551,7 @@ __global__ void k_pres_assemble(const long* __restrict__ rows, int N, const int*
     n++;
     cnt[r] = n;
     dg[r] = d;
+    for (int e = n; e < PSLOTS; e++) val[e * N + r] = 0, col[e * N + r] = r;  // padding the PCG loop reads
     const double o2[6] = {S.odx2, S.odx2, S.ody2, S.ody2, S.odz2, S.odz2}, o1[6] = {S.odx, S.odx, S.ody, S.ody, S.odz, S.odz};
     const double jcc = JC[o];
     double F[6];
@@ -589,6 +591,7 @@ __device__ __forceinline__ double block_tree(double v, double* sh) {
 // PCG scalars on the device.  done: 1 converged, 2 max_iteration reached.
 struct FsPipePcg {
     double rz, rz_old, pq, alpha, beta, rr, bb, sum;
+    double rzs[2];  // r.z of the last two iterations, by parity (the loop kernels' own copy)
     int iter, done, max_iter, rel;
     double tol2, rtol;
 };
@@ -629,24 +632,16 @@ __global__ void k_remove_mean(double* __restrict__ b, int N, const double* __res
 }
 
 // y = A x in each row's entry order (CSR_MATRIX::Multiply, CSR_MATRIX.h:
-// 236-260); with `part`, the block trees of x.y too
+// 236-260)
 __global__ void __launch_bounds__(RBLK) k_spmv(const double* __restrict__ val, const int* __restrict__ col,
                                                const int* __restrict__ cnt, int N, const double* __restrict__ x,
-                                               double* __restrict__ y, double* __restrict__ part,
-                                               const FsPipePcg* __restrict__ S) {
-    __shared__ double sh[RBLK];
-    if (S && S->done) return;
+                                               double* __restrict__ y) {
     int r = blockIdx.x * RBLK + threadIdx.x;
+    if (r >= N) return;
+    const int n = cnt[r];
     double v = 0;
-    if (r < N) {
-        const int n = cnt[r];
-        for (int e = 0; e < n; e++) v += val[e * N + r] * x[col[e * N + r]];
-        y[r] = v;
-    }
-    if (part) {
-        double t = block_tree(r < N ? x[r] * v : (double)0, sh);
-        if (threadIdx.x == 0) part[blockIdx.x] = t;
-    }
+    for (int e = 0; e < n; e++) v += val[e * N + r] * x[col[e * N + r]];
+    y[r] = v;
 }
 
 // r = b - q, z = r / D, p = z; partials of r.z, b.b, r.r
@@ -681,6 +676,7 @@ __global__ void __launch_bounds__(RBLK) k_pcg_begin_scalars(const double* __rest
     const double a = sum_parts(part, nb, sh), b = sum_parts(part + nb, nb, sh), c = sum_parts(part + 2 * nb, nb, sh);
     if (threadIdx.x != 0) return;
     S->rz = a;
+    S->rzs[1] = a;
     S->bb = b;
     S->rr = c;
     S->iter = 0;
@@ -689,76 +685,291 @@ __global__ void __launch_bounds__(RBLK) k_pcg_begin_scalars(const double* __rest
     if (S->rel && (bnorm == 0 || sqrt(c) <= S->rtol * bnorm)) S->done = 1;
     if (S->max_iter <= 0) S->done = S->done ? S->done : 2;
 }
-// alpha = r.z / p.q
-__global__ void __launch_bounds__(RBLK) k_pcg_alpha(const double* __restrict__ part, int nb, FsPipePcg* S) {
-    __shared__ double sh[RBLK];
-    if (S->done) return;
-    const double t = sum_parts(part, nb, sh);
-    if (threadIdx.x != 0) return;
-    S->pq = t;
-    S->alpha = S->rz / t;
+// ---- the PCG loop in two launches per iteration ------------------------------
+//
+// The fixed order's second level has RBLK lanes: lane t adds the block sums
+// t, t + RBLK, ... in order.  The loop kernels run exactly RBLK workgroups of
+// PWG threads; workgroup w walks the blocks w, w + RBLK, ... (PWG / RBLK of
+// them at a time, added in block order), so its running sum IS lane w's, and
+// a consumer needs only the binary tree of the RBLK workgroup sums -- every
+// workgroup of the next kernel re-forms the scalar from them, in the same
+// order, without a fence or atomic.
+#define PWG 1024
+#define PSUB (PWG / RBLK)
+
+// The binary tree of one block's RBLK values (t[q] += t[q + h], h = RBLK/2
+// .. 1) from LDS, called by the 64 lanes of one wave: the first two levels
+// from the four quarters, the last six by lane shuffles (lane q adds lane
+// q + h, the same pairs).  Lane 0 returns the total.
+__device__ __forceinline__ double tree_lds(const double* t) {
+    const int l = threadIdx.x % 64;
+    const double a = t[l] + t[l + 128], b = t[l + 64] + t[l + 192];
+    double v = a + b;
+    for (int h = 32; h >= 1; h /= 2) v += __shfl_down(v, h, 64);
+    return v;
 }
-// x += alpha p, r -= alpha q, z = r / D; partials of r.z (and r.r)
-__global__ void __launch_bounds__(RBLK) k_pcg_update(double* __restrict__ x, double* __restrict__ r,
-                                                     double* __restrict__ z, const double* __restrict__ p,
-                                                     const double* __restrict__ q, const double* __restrict__ dg, int N,
-                                                     double* __restrict__ part, int nb, const FsPipePcg* S) {
-    __shared__ double sh[RBLK];
-    if (S->done) return;
-    const double alpha = S->alpha;
-    int i = blockIdx.x * RBLK + threadIdx.x;
-    double rv = 0, zv = 0;
-    if (i < N) {
-        x[i] += alpha * p[i];
-        rv = r[i] - alpha * q[i];
-        r[i] = rv;
-        zv = rv / dg[i];
-        z[i] = zv;
+
+// the stop test and beta of the iteration whose r.z (and r.r) workgroup sums
+// are `lp` (PCG_METHOD.cpp:81-101); m is that iteration's number.  Returns 0
+// to go on (beta set), 1 converged, 2 max_iteration reached.  Every thread
+// of the workgroup takes part and gets the same result.
+__device__ __forceinline__ int pcg_tail(const double* __restrict__ lp, const FsPipePcg* S, int m, double rz_old,
+                                        double* sh, double* res, double& t, double& rr, double& beta) {
+    const int rel = S->rel;
+    if (threadIdx.x < RBLK) {
+        sh[threadIdx.x] = lp[threadIdx.x];
+        if (rel) sh[RBLK + threadIdx.x] = lp[RBLK + threadIdx.x];
     }
-    double t0 = block_tree(i < N ? rv * zv : (double)0, sh);
-    if (threadIdx.x == 0) part[blockIdx.x] = t0;
-    if (S->rel) {
-        __syncthreads();
-        double t1 = block_tree(i < N ? rv * rv : (double)0, sh);
-        if (threadIdx.x == 0) part[nb + blockIdx.x] = t1;
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const double a = tree_lds(sh), b = rel ? tree_lds(sh + RBLK) : (double)0;
+        if (threadIdx.x == 0) res[0] = a, res[1] = b;
     }
-}
-// the stop test and beta (PCG_METHOD.cpp:81-101)
-__global__ void __launch_bounds__(RBLK) k_pcg_tail(const double* __restrict__ part, int nb, FsPipePcg* S) {
-    __shared__ double sh[RBLK];
-    if (S->done) return;
-    const double t = sum_parts(part, nb, sh);
-    const double rr = S->rel ? sum_parts(part + nb, nb, sh) : 0;
-    if (threadIdx.x != 0) return;
-    S->rz_old = S->rz;
-    S->rz = t;
-    if (S->rel) {
-        S->rr = rr;
-        if (sqrt(rr) < S->rtol * sqrt(S->bb)) {
-            S->done = 1;
-            return;
-        }
+    __syncthreads();
+    t = res[0];
+    rr = res[1];
+    if (rel) {
+        if (sqrt(rr) < S->rtol * sqrt(S->bb)) return 1;
     } else if (t < S->tol2) {
+        return 1;
+    }
+    beta = t / rz_old;
+    return m >= S->max_iter ? 2 : 0;
+}
+// the host-facing record of a tail decision (one thread)
+__device__ __forceinline__ void pcg_record(FsPipePcg* S, int dec, int m, double rz_old, double t, double rr,
+                                          double beta) {
+    S->rz_old = rz_old;
+    S->rz = t;
+    if (S->rel) S->rr = rr;
+    if (dec == 1) {
+        S->iter = m - 1;
         S->done = 1;
         return;
     }
-    S->beta = S->rz / S->rz_old;
-    S->iter++;
-    if (S->iter >= S->max_iter) S->done = 2;
+    S->beta = beta;
+    S->iter = m;
+    if (dec == 2) S->done = 2;
 }
-// p = beta p + z (PCG_METHOD.cpp:104-108: p *= beta; p += z)
-__global__ void __launch_bounds__(RBLK) k_pcg_direction(double* __restrict__ p, const double* __restrict__ z, int N,
-                                                        const FsPipePcg* S) {
-    // stopped (converged, or max_iteration reached: the reference's last p
-    // update is never read)
+
+// After a pass has put its NS x PWG row values in sh (step s, sub-block q,
+// row l at sh[s * PWG + q * RBLK + l]): each block's tree, then thread 0
+// adds the trees of the blocks that exist (index c0 + RBLK (PSUB s + q) <
+// nb) to acc in block order
+template <int NS>
+__device__ __forceinline__ void pass_trees(const double* sh, double* res, int c0, int nb, double& acc) {
+    __syncthreads();
+    if (threadIdx.x % RBLK < 64) {
+        const int q = threadIdx.x / RBLK;
+        for (int s = 0; s < NS; s++) {
+            const double v = tree_lds(sh + s * PWG + q * RBLK);
+            if (threadIdx.x % RBLK == 0) res[s * PSUB + q] = v;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        for (int s = 0; s < NS; s++)
+            for (int q = 0; q < PSUB; q++)
+                if (c0 + RBLK * (PSUB * s + q) < nb) acc += res[s * PSUB + q];
+}
+
+// One row's operands of the fused direction + SpMV step, loaded apart from
+// their use so the loads can be issued before the tail that gives beta.
+// ELL slots past the row's count hold (column r, value 0), so every load is
+// in bounds.
+template <bool FIRST>
+struct PcgRow {
+    int n;
+    double a[PSLOTS], pc[PSLOTS], zc[PSLOTS], pr, zr;
+    __device__ __forceinline__ void load(const double* __restrict__ val, const int* __restrict__ col,
+                                         const int* __restrict__ cnt, int N, const double* __restrict__ po,
+                                         const double* __restrict__ z, int rc) {
+        n = cnt[rc];
+#pragma unroll
+        for (int e = 0; e < PSLOTS; e++) {
+            const int c = col[e * N + rc];
+            a[e] = val[e * N + rc];
+            pc[e] = po[c];
+            zc[e] = FIRST ? (double)0 : z[c];
+        }
+        pr = po[rc];
+        zr = FIRST ? (double)0 : z[rc];
+    }
+    // q = sum over the row's entries in order of a_e (p beta + z)_e; the
+    // row's own new direction in p_row
+    __device__ __forceinline__ double apply(double beta, double& p_row) const {
+        double v = 0;
+#pragma unroll
+        for (int e = 0; e < PSLOTS; e++) {
+            double x = pc[e];
+            if (!FIRST) {
+                x *= beta;
+                x += zc[e];
+            }
+            const double t = v + a[e] * x;
+            v = e < n ? t : v;
+        }
+        p_row = pr;
+        if (!FIRST) {
+            p_row *= beta;
+            p_row += zr;
+        }
+        return v;
+    }
+};
+
+// Iteration k, first launch: the tail of iteration k - 1 (k > 1), the new
+// direction p_k = p_{k-1} beta + z (PCG_METHOD.cpp:104-108, formed again for
+// every neighbour a row reads, the same bits), q = A p_k in each row's entry
+// order (CSR_MATRIX.h:236-260), and the workgroup sums of p.q.  p_1 = z is
+// in `po` already; for k > 1 p_k goes to `pn`.  S->rzs[k & 1] receives r.z
+// of iteration k - 1 for the update.  A pass covers NS steps of PWG rows,
+// their loads in flight together; with NS = 1 (one step covers the system)
+// the row's loads go out before the tail.
+template <int NS, bool FIRST>
+__global__ void __launch_bounds__(PWG) k_pcg_spmv_dir(const double* __restrict__ val, const int* __restrict__ col,
+                                                      const int* __restrict__ cnt, int N, int nb,
+                                                      const double* __restrict__ po, double* __restrict__ pn,
+                                                      const double* __restrict__ z, double* __restrict__ q,
+                                                      double* __restrict__ lp, FsPipePcg* S, int k) {
+    __shared__ double sh[NS * PWG];
+    __shared__ double res[NS * PSUB];
     if (S->done) return;
-    int i = blockIdx.x * RBLK + threadIdx.x;
-    if (i >= N) return;
-    double v = p[i];
-    v *= S->beta;
-    v += z[i];
-    p[i] = v;
+    const int sub = threadIdx.x / RBLK, l = threadIdx.x % RBLK;
+    PcgRow<FIRST> row1;
+    const int b1 = blockIdx.x + RBLK * sub, r1 = b1 * RBLK + l;
+    const bool in1 = b1 < nb && r1 < N;
+    if (NS == 1) row1.load(val, col, cnt, N, po, z, in1 ? r1 : 0);
+    double beta = 0;
+    if (!FIRST) {
+        const double rz_old = S->rzs[(k - 1) & 1];
+        double t, rr;
+        const int dec = pcg_tail(lp + RBLK, S, k - 1, rz_old, sh, res, t, rr, beta);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            pcg_record(S, dec, k - 1, rz_old, t, rr, beta);
+            if (dec == 0) S->rzs[k & 1] = t;
+        }
+        // stopped: the reference's last direction is never read
+        if (dec) return;
+    }
+    double acc = 0;
+    if (NS == 1) {
+        double pr;
+        const double v = row1.apply(beta, pr);
+        if (in1) {
+            q[r1] = v;
+            if (!FIRST) pn[r1] = pr;
+        }
+        sh[threadIdx.x] = in1 ? pr * v : (double)0;
+        pass_trees<NS>(sh, res, blockIdx.x, nb, acc);
+    } else {
+        for (int c0 = blockIdx.x; c0 < nb; c0 += RBLK * PSUB * NS) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int b = c0 + RBLK * (PSUB * s + sub), r = b * RBLK + l;
+                const bool in = b < nb && r < N;
+                PcgRow<FIRST> row;
+                row.load(val, col, cnt, N, po, z, in ? r : 0);
+                double pr;
+                const double v = row.apply(beta, pr);
+                if (in) {
+                    q[r] = v;
+                    if (!FIRST) pn[r] = pr;
+                }
+                sh[s * PWG + threadIdx.x] = in ? pr * v : (double)0;
+            }
+            pass_trees<NS>(sh, res, c0, nb, acc);
+        }
+    }
+    if (threadIdx.x == 0) lp[blockIdx.x] = acc;
 }
+
+// Iteration k, second launch: alpha = r.z / p.q, x += alpha p, r -= alpha q,
+// z = r / D, and the workgroup sums of r.z (and r.r) for the next tail
+template <int NS>
+__global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, double* __restrict__ r,
+                                                     double* __restrict__ z, const double* __restrict__ p,
+                                                     const double* __restrict__ q, const double* __restrict__ dg,
+                                                     int N, int nb, double* __restrict__ lp, FsPipePcg* S, int k) {
+    __shared__ double sh[2 * NS * PWG];
+    __shared__ double res[NS * PSUB];
+    if (S->done) return;
+    const int rel = S->rel;
+    const int sub = threadIdx.x / RBLK, l = threadIdx.x % RBLK;
+    // one-step passes: the row's loads go out before the tree of p.q
+    const int b1 = blockIdx.x + RBLK * sub, i1 = b1 * RBLK + l;
+    const bool in1 = b1 < nb && i1 < N;
+    const int ic1 = in1 ? i1 : 0;
+    double x1 = 0, p1 = 0, r1 = 0, q1 = 0, d1 = 1;
+    if (NS == 1) x1 = x[ic1], p1 = p[ic1], r1 = r[ic1], q1 = q[ic1], d1 = dg[ic1];
+    if (threadIdx.x < RBLK) sh[threadIdx.x] = lp[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const double t = tree_lds(sh);
+        if (threadIdx.x == 0) res[0] = t;
+    }
+    __syncthreads();
+    const double pq = res[0];
+    const double alpha = S->rzs[k & 1] / pq;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        S->pq = pq;
+        S->alpha = alpha;
+    }
+    double acc0 = 0, acc1 = 0;
+    if (NS == 1) {
+        const double xv = x1 + alpha * p1;
+        const double rv = r1 - alpha * q1;
+        const double zv = rv / d1;
+        if (in1) {
+            x[i1] = xv;
+            r[i1] = rv;
+            z[i1] = zv;
+        }
+        sh[threadIdx.x] = in1 ? rv * zv : (double)0;
+        sh[PWG + threadIdx.x] = in1 ? rv * rv : (double)0;
+        pass_trees<NS>(sh, res, blockIdx.x, nb, acc0);
+        if (rel) pass_trees<NS>(sh + PWG, res, blockIdx.x, nb, acc1);
+    } else {
+        for (int c0 = blockIdx.x; c0 < nb; c0 += RBLK * PSUB * NS) {
+#pragma unroll
+            for (int s = 0; s < NS; s++) {
+                const int b = c0 + RBLK * (PSUB * s + sub), i = b * RBLK + l;
+                const bool in = b < nb && i < N;
+                const int ic = in ? i : 0;
+                const double xv = x[ic] + alpha * p[ic];
+                const double rv = r[ic] - alpha * q[ic];
+                const double zv = rv / dg[ic];
+                if (in) {
+                    x[i] = xv;
+                    r[i] = rv;
+                    z[i] = zv;
+                }
+                sh[s * PWG + threadIdx.x] = in ? rv * zv : (double)0;
+                sh[(NS + s) * PWG + threadIdx.x] = in ? rv * rv : (double)0;
+            }
+            pass_trees<NS>(sh, res, c0, nb, acc0);
+            if (rel) pass_trees<NS>(sh + NS * PWG, res, c0, nb, acc1);
+        }
+    }
+    if (threadIdx.x == 0) {
+        lp[RBLK + blockIdx.x] = acc0;
+        lp[2 * RBLK + blockIdx.x] = acc1;
+    }
+}
+
+// the tail of iteration m at the end of a launch batch, so the host sees the
+// stop without another batch (one RBLK-thread workgroup; the next batch's
+// first launch repeats the same decision when it goes on)
+__global__ void __launch_bounds__(RBLK) k_pcg_tail_end(const double* __restrict__ lp, FsPipePcg* S, int m) {
+    __shared__ double sh[2 * RBLK];
+    __shared__ double res[2];
+    if (S->done || m < 1) return;
+    const double rz_old = S->rzs[m & 1];
+    double t, rr, beta = 0;
+    const int dec = pcg_tail(lp + RBLK, S, m, rz_old, sh, res, t, rr, beta);
+    if (threadIdx.x == 0) pcg_record(S, dec, m, rz_old, t, rr, beta);
+}
+
 // r = b - q (the true residual after the solve)
 __global__ void k_residual(const double* __restrict__ b, const double* __restrict__ q, double* __restrict__ r, int N) {
     int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -959,7 +1170,7 @@ struct FsPipe {
     PipeSys sys[4];
     int Nmax = 0;
     double *val = nullptr, *dg = nullptr, *rhs = nullptr, *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv = nullptr,
-           *qv = nullptr, *part = nullptr;
+           *pv2 = nullptr, *qv = nullptr, *part = nullptr, *lpart = nullptr;
     int *col = nullptr, *cnt = nullptr;
     FsPipePcg* pcg = nullptr;
     double* dscal = nullptr;             // device scalars: [0] a sum of the fixed-order reductions
@@ -990,8 +1201,9 @@ struct FsPipe {
             (void)hipFree(sys[s].bc);
             (void)hipFree(sys[s].rows);
         }
-        for (void* d : {(void*)val, (void*)dg, (void*)rhs, (void*)xv, (void*)rv, (void*)zv, (void*)pv, (void*)qv,
-                        (void*)part, (void*)col, (void*)cnt, (void*)pcg, (void*)bits, (void*)dscal})
+        for (void* d : {(void*)val, (void*)dg, (void*)rhs, (void*)xv, (void*)rv, (void*)zv, (void*)pv, (void*)pv2,
+                        (void*)qv, (void*)part, (void*)lpart, (void*)col, (void*)cnt, (void*)pcg, (void*)bits,
+                        (void*)dscal})
             (void)hipFree(d);
         if (host_pcg) (void)hipHostFree(host_pcg);
         if (host_bits) (void)hipHostFree(host_bits);
@@ -1167,8 +1379,9 @@ struct FsPipe {
         PCHECK(hipMalloc(&val, sizeof(double) * N * PSLOTS));
         PCHECK(hipMalloc(&col, sizeof(int) * N * PSLOTS));
         PCHECK(hipMalloc(&cnt, sizeof(int) * N));
-        for (double** d : {&dg, &rhs, &xv, &rv, &zv, &pv, &qv}) PCHECK(hipMalloc(d, sizeof(double) * N));
+        for (double** d : {&dg, &rhs, &xv, &rv, &zv, &pv, &pv2, &qv}) PCHECK(hipMalloc(d, sizeof(double) * N));
         PCHECK(hipMalloc(&part, sizeof(double) * 3 * (nb + 1)));
+        PCHECK(hipMalloc(&lpart, sizeof(double) * 3 * RBLK));  // the loop's workgroup sums: p.q, r.z, r.r
     }
 
     // FillGhostCellsContinuousDerivativesFrom of a cell field onto itself,
@@ -1353,30 +1566,36 @@ struct FsPipe {
         host_pcg->tol2 = P.tolerance * P.tolerance;
         host_pcg->rtol = P.rel_tolerance;
         PCHECK(hipMemcpyAsync(pcg, host_pcg, sizeof(FsPipePcg), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv, nullptr, nullptr);
+        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv);
         hipLaunchKernelGGL(k_pcg_begin, dim3(nb), dim3(RBLK), 0, st, rhs, qv, dg, rv, zv, pv, N, part, nb);
         hipLaunchKernelGGL(k_pcg_begin_scalars, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
-        int batch = 8, enq = 0;
+        // p_k lives in pb[k & 1]; p_1 = z, which k_pcg_begin wrote to pv
+        double* pb[2] = {pv2, pv};
+        // one step of PWG rows per workgroup covers the system: one-step passes
+        const bool one = nb <= RBLK * PSUB;
+        int batch = 8, k = 1;
         while (true) {
-            for (int q = 0; q < batch; q++) {
-                hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, pv, qv, part, pcg);
-                hipLaunchKernelGGL(k_pcg_alpha, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
-                hipLaunchKernelGGL(k_pcg_update, dim3(nb), dim3(RBLK), 0, st, xv, rv, zv, pv, qv, dg, N, part, nb, pcg);
-                hipLaunchKernelGGL(k_pcg_tail, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
-                hipLaunchKernelGGL(k_pcg_direction, dim3(nb), dim3(RBLK), 0, st, pv, zv, N, pcg);
+            for (int q = 0; q < batch; q++, k++) {
+                const double* po = pb[k == 1 ? 1 : (k - 1) & 1];
+                auto ka = one ? (k == 1 ? k_pcg_spmv_dir<1, true> : k_pcg_spmv_dir<1, false>)
+                              : (k == 1 ? k_pcg_spmv_dir<4, true> : k_pcg_spmv_dir<4, false>);
+                hipLaunchKernelGGL(ka, dim3(RBLK), dim3(PWG), 0, st, val, col, cnt, N, nb, po, pb[k & 1], zv, qv,
+                                   lpart, pcg, k);
+                hipLaunchKernelGGL(one ? k_pcg_update2<1> : k_pcg_update2<4>, dim3(RBLK), dim3(PWG), 0, st, xv, rv, zv,
+                                   pb[k & 1], qv, dg, N, nb, lpart, pcg, k);
             }
-            enq += batch;
+            hipLaunchKernelGGL(k_pcg_tail_end, dim3(1), dim3(RBLK), 0, st, lpart, pcg, k - 1);
             PCHECK(hipMemcpyAsync(host_pcg, pcg, sizeof(FsPipePcg), hipMemcpyDeviceToHost, st));
             sync();
             if (host_pcg->done) break;
-            if (enq > P.max_iteration + 2) throw FsPipeError{FS_ERR_HIP, "pipe PCG: iteration count out of step"};
+            if (k > P.max_iteration + 3) throw FsPipeError{FS_ERR_HIP, "pipe PCG: iteration count out of step"};
             batch = std::min(64, batch * 2);
         }
         check_launch();
         const int it = host_pcg->iter;
         last_res_abs = std::sqrt(host_pcg->rz);
         // the true residual for the report
-        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv, nullptr, nullptr);
+        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv);
         hipLaunchKernelGGL(k_residual, dim3(nb), dim3(RBLK), 0, st, rhs, qv, rv, N);
         hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rv, rv, N, part);
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, part, nb, dscal);
