@@ -761,6 +761,9 @@ def pipe_main(a):
     stats = []
     for _ in range(a.warmup):
         s.advance_frame(s.cfl_dt(0.9))
+    # one launch of each PCG loop kernel per launch batch timed with HIP
+    # events on the solver's stream, inside the timed frames
+    s.set_sampling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f in range(a.steps):
@@ -770,8 +773,16 @@ def pipe_main(a):
     torch.cuda.synchronize()
     t = time.perf_counter() - t0
     ms = t * 1e3 / a.steps
+    ks = {name: s.kernel_samples(k) for name, k in (("k_pcg_spmv_dir", _abi.FS_PIPE_KS_SPMV),
+                                                     ("k_pcg_update2", _abi.FS_PIPE_KS_UPDATE))}
     fields = s.get_fields()
     s.close()
+    kernels = {}
+    for name, (kms, kbytes, nsamp, launches) in ks.items():
+        gbs = kbytes / (kms * 1e6) if kms > 0 else None
+        kernels[name] = {"avg_us": round(kms * 1e3, 3), "algorithmic_bytes": round(kbytes), "GB/s": gbs and round(gbs, 1),
+                         "samples": nsamp, "launches": launches}
+    dom = kernels["k_pcg_spmv_dir"]
     cells = n ** 3
     out = {
         "metric": "ms/frame, oil/water pipe (core-annular flow), the reference script's scene",
@@ -784,6 +795,13 @@ def pipe_main(a):
         "pcg_iterations_per_step": float(np.mean([st["pcg_iterations"] for st in stats])),
         "viscosity_iterations_per_step": float(np.mean([st["viscosity_iterations"] for st in stats])),
         "finite": bool(all(np.all(np.isfinite(v)) for v in fields.values())),
+        # the dominant kernel: 120 algorithmic bytes per row of the solved
+        # system (7 ELL values and columns, the count, p, z, new p and q)
+        "roofline": {"kernel": "k_pcg_spmv_dir", "bound": "hbm", "achieved": dom["GB/s"], "peak": 8000.0,
+                     "unit": "GB/s", "frac": dom["GB/s"] and round(dom["GB/s"] / 8000.0, 4), "traffic": None,
+                     "rows_per_launch": round(ks["k_pcg_spmv_dir"][1] / 120),
+                     "limiter": "launch and load latency of the dependent PCG iterations (two launches each)"},
+        "kernels": kernels,
     }
     if not a.no_cpu_baseline:
         orc = _abi.load(os.path.join(ROOT, "oracle", "libfs_oracle.so"), "fso_")

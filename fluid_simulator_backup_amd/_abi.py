@@ -661,6 +661,8 @@ class World:
 
 FS_PIPE_LEVELSET_ADVECTION, FS_PIPE_VELOCITY_ADVECTION, FS_PIPE_GRAVITY = 0, 1, 2
 FS_PIPE_SOURCING, FS_PIPE_PROJECTION, FS_PIPE_REINITIALIZATION = 3, 4, 5
+FS_PIPE_KS_SPMV = 0
+FS_PIPE_KS_UPDATE = 1
 PIPE_ARRAYS = ["phi", "scalar_ghost", "ghost_u", "ghost_v", "ghost_w",
                "adv_u_y", "adv_u_z", "adv_v_y", "adv_v_z", "adv_w_y", "adv_w_z",
                "vis_u_y", "vis_u_z", "vis_v_y", "vis_v_z", "vis_w_y", "vis_w_z",
@@ -730,6 +732,8 @@ def bind_pipe(lib, prefix):
         getattr(lib, names["init"]).argtypes = [vp]
         getattr(lib, names["constants"]).argtypes = [vp, _PD]
         lib.fs_cfl_dt.argtypes = [vp, _D, _I, C.POINTER(_D)]
+        lib.fs_set_kernel_sampling.argtypes = [vp, _I]
+        lib.fs_pipe_kernel_samples.argtypes = [vp, _I, _PD, _PD, C.POINTER(C.c_longlong), C.POINTER(C.c_longlong)]
     else:
         lib.fsop_create.argtypes = [C.POINTER(FsParams), C.POINTER(FsPipeParams), C.POINTER(FsGrid), C.POINTER(vp)]
         names = {"init": "fsop_initial_fields", "constants": "fsop_constants", "phase": "fsop_phase",
@@ -842,6 +846,22 @@ class Pipe:
         if rc != FS_OK:
             raise FsError(rc, "cfl_dt")
         return d.value
+
+    def set_sampling(self, enable):
+        """In-loop timing of the PCG loop kernels (HIP library only)."""
+        rc = self.lib.fs_set_kernel_sampling(self.h, int(bool(enable)))
+        if rc != FS_OK:
+            raise FsError(rc, "set_kernel_sampling")
+
+    def kernel_samples(self, kernel):
+        """(mean ms, mean algorithmic bytes, timed launches, launches) of
+        FS_PIPE_KS_SPMV or FS_PIPE_KS_UPDATE since sampling was enabled."""
+        ms, by = C.c_double(), C.c_double()
+        n, launches = C.c_longlong(), C.c_longlong()
+        rc = self.lib.fs_pipe_kernel_samples(self.h, kernel, C.byref(ms), C.byref(by), C.byref(n), C.byref(launches))
+        if rc != FS_OK:
+            raise FsError(rc, "pipe_kernel_samples")
+        return ms.value, by.value, n.value, launches.value
 
     def array(self, name):
         """A persistent array over its box grown by 3 ghost layers, [k, j, i]."""

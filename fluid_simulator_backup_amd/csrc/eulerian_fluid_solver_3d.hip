@@ -2041,6 +2041,15 @@ int fs_pipe_get_array(fs_solver* s, int which, double* dst) {
     });
 }
 
+int fs_pipe_kernel_samples(fs_solver* s, int kernel, double* avg_ms, double* avg_bytes, long long* samples,
+                           long long* launches) {
+    if (!s || !s->pipe || !avg_ms || !avg_bytes || !samples || !launches) return FS_ERR_BAD_ARG;
+    return pipe_guarded(s, [&](FsPipe& P) {
+        fs_pipe_samples(&P, kernel, avg_ms, avg_bytes, samples, launches);
+        return FS_OK;
+    });
+}
+
 int fs_create(const fs_params* params, const fs_grid* grid, int device, fs_solver** out) {
     if (!params || !grid || !out) return FS_ERR_BAD_ARG;
     *out = nullptr;
@@ -2835,6 +2844,11 @@ int fs_time_pcg_kernels(fs_solver* s, int precision, int reps, double* avg_ms) {
 }
 
 int fs_set_kernel_sampling(fs_solver* s, int enable) {
+    if (s && s->pipe)
+        return pipe_guarded(s, [&](FsPipe& P) {
+            fs_pipe_sampling(&P, enable != 0);
+            return FS_OK;
+        });
     return guarded(s, [&](EulerianFluidSolver3D& S) {
         S.sampling = enable != 0;
         S.probe.recorded = false;

@@ -29,3 +29,6 @@ void fs_pipe_array(FsPipe* s, int which, double* dst);
 void fs_pipe_async(FsPipe* s, bool async);
 void fs_pipe_sync(FsPipe* s);
 hipStream_t fs_pipe_stream(FsPipe* s);
+// in-loop samples of the PCG loop kernels (0 k_pcg_spmv_dir, 1 k_pcg_update2)
+void fs_pipe_sampling(FsPipe* s, bool on);
+void fs_pipe_samples(FsPipe* s, int kernel, double* avg_ms, double* avg_bytes, long long* samples, long long* launches);

@@ -1182,6 +1182,11 @@ struct FsPipe {
     int last_iters = 0, visc_iters = 0;
     double last_res_abs = 0, last_res_rel = 0;
     bool not_converged = false;
+    // in-loop samples of the PCG loop kernels (0 spmv_dir, 1 update2)
+    bool sampling = false;
+    hipEvent_t kev[3] = {};
+    double ks_ms[2] = {}, ks_bytes[2] = {};
+    long long ks_n[2] = {}, ks_launch[2] = {};
 
     FsBox box(int c) const { return fs_box(L, c); }
     double* field(int c) { return c == 1 ? u : c == 2 ? v : w; }
@@ -1207,6 +1212,8 @@ struct FsPipe {
             (void)hipFree(d);
         if (host_pcg) (void)hipHostFree(host_pcg);
         if (host_bits) (void)hipHostFree(host_bits);
+        for (hipEvent_t e : kev)
+            if (e) (void)hipEventDestroy(e);
         if (st) (void)hipStreamDestroy(st);
     }
 
@@ -1556,6 +1563,21 @@ struct FsPipe {
                                c, Gp);
     }
 
+    // Adds the timed launches of iteration k (of a system of N rows) to the
+    // samples if they ran in full: iteration k's launches return at once when
+    // the tail of iteration k - 1 stopped the solve
+    void sample(int k, int N) {
+        const FsPipePcg& h = *host_pcg;
+        const bool ran = h.done == 0 || (h.done == 1 ? k - 1 <= h.iter : k <= h.iter);
+        if (!ran) return;
+        float a = 0, b = 0;
+        PCHECK(hipEventElapsedTime(&a, kev[0], kev[1]));
+        PCHECK(hipEventElapsedTime(&b, kev[1], kev[2]));
+        ks_ms[0] += a, ks_ms[1] += b;
+        ks_bytes[0] += 120.0 * N, ks_bytes[1] += 64.0 * N;
+        ks_n[0]++, ks_n[1]++;
+    }
+
     // the Jacobi-PCG of PCG_METHOD.cpp:4-160 on the assembled ELL system of
     // N rows (x holds x0); returns the iterations
     int Pcg(int N) {
@@ -1575,18 +1597,28 @@ struct FsPipe {
         const bool one = nb <= RBLK * PSUB;
         int batch = 8, k = 1;
         while (true) {
+            int ks = 0;  // the iteration whose two launches are timed (0: none)
             for (int q = 0; q < batch; q++, k++) {
                 const double* po = pb[k == 1 ? 1 : (k - 1) & 1];
+                const bool timed = sampling && q == 1;
+                if (timed) {
+                    ks = k;
+                    PCHECK(hipEventRecord(kev[0], st));
+                }
                 auto ka = one ? (k == 1 ? k_pcg_spmv_dir<1, true> : k_pcg_spmv_dir<1, false>)
                               : (k == 1 ? k_pcg_spmv_dir<4, true> : k_pcg_spmv_dir<4, false>);
                 hipLaunchKernelGGL(ka, dim3(RBLK), dim3(PWG), 0, st, val, col, cnt, N, nb, po, pb[k & 1], zv, qv,
                                    lpart, pcg, k);
+                if (timed) PCHECK(hipEventRecord(kev[1], st));
                 hipLaunchKernelGGL(one ? k_pcg_update2<1> : k_pcg_update2<4>, dim3(RBLK), dim3(PWG), 0, st, xv, rv, zv,
                                    pb[k & 1], qv, dg, N, nb, lpart, pcg, k);
+                if (timed) PCHECK(hipEventRecord(kev[2], st));
             }
+            if (sampling) ks_launch[0] += batch, ks_launch[1] += batch;
             hipLaunchKernelGGL(k_pcg_tail_end, dim3(1), dim3(RBLK), 0, st, lpart, pcg, k - 1);
             PCHECK(hipMemcpyAsync(host_pcg, pcg, sizeof(FsPipePcg), hipMemcpyDeviceToHost, st));
             sync();
+            if (ks) sample(ks, N);
             if (host_pcg->done) break;
             if (k > P.max_iteration + 3) throw FsPipeError{FS_ERR_HIP, "pipe PCG: iteration count out of step"};
             batch = std::min(64, batch * 2);
@@ -1941,3 +1973,19 @@ void fs_pipe_sync(FsPipe* s) {
     s->check_launch();
 }
 hipStream_t fs_pipe_stream(FsPipe* s) { return s->st; }
+void fs_pipe_sampling(FsPipe* s, bool on) {
+    s->sync();
+    for (hipEvent_t& e : s->kev)
+        if (!e) PCHECK(hipEventCreate(&e));
+    s->sampling = on;
+    for (int i = 0; i < 2; i++) s->ks_ms[i] = s->ks_bytes[i] = 0, s->ks_n[i] = s->ks_launch[i] = 0;
+}
+void fs_pipe_samples(FsPipe* s, int kernel, double* avg_ms, double* avg_bytes, long long* samples,
+                     long long* launches) {
+    if (kernel < 0 || kernel > 1) throw FsPipeError{FS_ERR_BAD_ARG, "pipe kernel samples: kernel must be 0 or 1"};
+    const long long n = s->ks_n[kernel];
+    *avg_ms = n ? s->ks_ms[kernel] / n : 0;
+    *avg_bytes = n ? s->ks_bytes[kernel] / n : 0;
+    *samples = n;
+    *launches = s->ks_launch[kernel];
+}
