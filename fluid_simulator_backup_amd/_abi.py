@@ -686,17 +686,28 @@ class FsWorldInfo(C.Structure):
                 ("oil_water", _I), ("pipe", FsPipeParams)]
 
 
-def pipe_params(**kw):
-    """fs_params and fs_pipe_params of the reference's oil/water script
-    (3D_script.sim:32-40, 195-290): core-annular flow, dimensionless form."""
+# the pipe keys of the shipped scripts: (is_parallel, a, A_0, alpha, K)
+PIPE_SCRIPTS = {
+    "3D_script.sim": (1, 1.28, 0.001, 2.0, -0.4552),          # parallel, 3D_script.sim:35-39, 229
+    "3D_script_1.sim": (0, 1.61, 0.1, 2.4, -2.03030303),     # vertical, 3D_script_1.sim:35-39, 229
+    "3D_script_3.sim": (0, 1.7, 0.01, 1.975, -0.542709),     # vertical, 3D_script_3.sim:35-38, 229
+}
+
+
+def pipe_params(script="3D_script.sim", **kw):
+    """fs_params and fs_pipe_params of one of the reference's oil/water
+    scripts (3D_script.sim:32-40, 195-290 and the vertical pipes of
+    3D_script_1.sim and 3D_script_3.sim): core-annular flow, dimensionless
+    form.  3D_script_2.sim is 3D_script.sim's pipe with the vertical grid."""
     p = default_params(water_density=0.995, water_viscosity=0.01, surface_tension=8.54,
                        gravity=(0.0, -980.0, 0.0), order_for_time_advancing=2,
                        scaling_number_for_reinitialization=2, iteration_number_for_reinitialization=10,
                        tolerance=1e-6)
     q = FsPipeParams()
-    q.is_parallel = 1
-    q.a, q.R2, q.A_0, q.alpha = 1.28, 0.47625, 0.001, 2.0
-    q.m, q.eta, q.K, q.g = 0.00166, 1.1, -0.4552, -980.0
+    par, a, A_0, alpha, K = PIPE_SCRIPTS[script]
+    q.is_parallel = par
+    q.a, q.R2, q.A_0, q.alpha = a, 0.47625, A_0, alpha
+    q.m, q.eta, q.K, q.g = 0.00166, 1.1, K, -980.0
     q.oil_density, q.oil_viscosity = 0.905, 6.01
     q.dimensionless_form, q.cylindrical_boundary, q.boundary_order_velocity = 1, 1, 1
     for k, v in kw.items():
@@ -716,6 +727,17 @@ def pipe_grid(n=51):
     g.nx = g.ny = g.nz = n
     g.x_min, g.y_min, g.z_min = 0.0, -1.570796, -1.570796
     g.x_max, g.y_max, g.z_max = 3.141592, 1.570796, 1.570796
+    return g
+
+
+def vertical_pipe_grid(nxz=71, ny=51, half=1.8326, length=2.618):
+    """GRID_STRUCTURE_3D_VERTICAL of 3D_script_1.sim (:78-85): nxz x ny x nxz
+    nodes, x and z in [-half, half], y (the pipe's axis) in [0, length];
+    3D_script_3.sim has half 2.2270, length 3.1814."""
+    g = FsGrid()
+    g.nx, g.ny, g.nz = nxz, ny, nxz
+    g.x_min, g.y_min, g.z_min = -half, 0.0, -half
+    g.x_max, g.y_max, g.z_max = half, length, half
     return g
 
 

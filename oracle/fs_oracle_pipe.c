@@ -70,7 +70,8 @@ struct fsop_state {
     int nx, ny, nz;
     long sx, sy, ntot;
     pgc_t cell, mac[4]; /* mac[1..3]: the x-, y-, z-face grids */
-    double y0f[4], z0f[4]; /* y_min, z_min of each component's grid */
+    double x0f[4], y0f[4], z0f[4]; /* x_min, y_min, z_min of each component's grid */
+    int vert;                      /* 1: the vertical pipe (along y), 0: the parallel pipe (along x) */
     /* derived constants, EULERIAN_FLUID_SOLVER_3D.h:496-520 */
     double R1, f, A, V0, J, Re, We, one_over_We, A0;
     /* state */
@@ -112,10 +113,26 @@ static void box(const fsop_state* s, int c, int* ni, int* nj, int* nk) {
 }
 
 /* the centreline faces ADVECTION_3D::Solve_Velocity fixes on its first call
- * (ADVECTION_3D.h:279-307, is_parallel); set from creation: every phase that
- * reads them runs after that call in a step */
-static int fixed_face(const fsop_state* s, int c, int j, int k) {
-    int je, ke;
+ * (ADVECTION_3D.h:248-307: along x for the parallel pipe, along y for the
+ * vertical one); set from creation: every phase that reads them runs after
+ * that call in a step */
+static int fixed_face(const fsop_state* s, int c, int i, int j, int k) {
+    int ie, je, ke;
+    if (s->vert) {
+        if (c == 1) {
+            ie = s->nx, ke = s->nz - 1;
+            return (i == ie / 2 || i == ie / 2 + 1) && k == ke / 2;
+        }
+        if (c == 2) {
+            ie = s->nx - 1, ke = s->nz - 1;
+            return i == ie / 2 && k == ke / 2;
+        }
+        if (c == 3) {
+            ie = s->nx - 1, ke = s->nz;
+            return i == ie / 2 && (k == ke / 2 || k == ke / 2 + 1);
+        }
+        return 0;
+    }
     if (c == 1) {
         je = s->ny - 1, ke = s->nz - 1;
         return j == je / 2 && k == ke / 2;
@@ -170,6 +187,87 @@ static void fill_periodic_x(fsop_state* s, double* dst, const double* src, int c
                     if (!mask || P3(mask, i, j, k) < (double)1e-8) P3(dst, i, j, k) = P3(src, 0 + (i - ie), j, k);
             }
         }
+}
+
+/* ---- y-periodic ghost fills, FIELD_STRUCTURE_3D.h:1328-1710 ---------------
+ * Unmasked (:1328-1430): interior copy when `copy`, then the y ghosts from
+ * the node-periodic image: a y-face field's two end faces and ghosts as the
+ * x fill's x faces, other fields' ghosts j < 0 and j > je only (unlike the x
+ * fill, the first row is not rewritten).  Masked (:1432-1710): the interior
+ * copy where the mask is < 1e-8 and ZERO elsewhere, the ghost rows where
+ * the mask at the written cell is < 1e-8, then four blocks of linear
+ * extrapolation of the source beyond the wall along x and z, in the order
+ * the reference's loop runs them on one thread: from each cell outside the
+ * wall whose x neighbour below (block 2) or above (block 3) or z neighbour
+ * below or above (the two blocks 4) is inside, every cell from it to the
+ * box edge (ghosts included) takes 2 src(t -/+ 1) - src(t -/+ 2). */
+static void fill_periodic_y(fsop_state* s, double* dst, const double* src, int c, int gw, int copy, const double* mask) {
+    int ni, nj, nk;
+    box(s, c, &ni, &nj, &nk);
+    const int ie = ni - 1, je = nj - 1, ke = nk - 1;
+    if (copy)
+        for (int k = 0; k < nk; k++)
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++) {
+                    if (!mask || P3(mask, i, j, k) < (double)1e-8)
+                        P3(dst, i, j, k) = P3(src, i, j, k);
+                    else
+                        P3(dst, i, j, k) = 0;
+                }
+    for (int k = 0; k < nk; k++)
+        for (int i = 0; i < ni; i++) {
+            if (c == 2) {
+                for (int j = 0; j >= -gw; j--)
+                    if (!mask || P3(mask, i, j, k) < (double)1e-8) P3(dst, i, j, k) = P3(src, i, je + (j - (0 + 1)), k);
+            } else {
+                for (int j = 0 - 1; j >= -gw; j--)
+                    if (!mask || P3(mask, i, j, k) < (double)1e-8) P3(dst, i, j, k) = P3(src, i, je + (j - 0), k);
+            }
+        }
+    for (int k = 0; k < nk; k++)
+        for (int i = 0; i < ni; i++) {
+            if (c == 2) {
+                for (int j = je; j <= je + gw; j++)
+                    if (!mask || P3(mask, i, j, k) < (double)1e-8) P3(dst, i, j, k) = P3(src, i, 0 + (j - (je - 1)), k);
+            } else {
+                for (int j = je + 1; j <= je + gw; j++)
+                    if (!mask || P3(mask, i, j, k) < (double)1e-8) P3(dst, i, j, k) = P3(src, i, 0 + (j - je), k);
+            }
+        }
+    if (!mask) return;
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (P3(mask, i - 1, j, k) < 0 && P3(mask, i, j, k) > 0)
+                    for (int t = i; t <= ie + gw; t++)
+                        P3(dst, t, j, k) = (double)2 * P3(src, t - 1, j, k) - P3(src, t - 2, j, k);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (P3(mask, i + 1, j, k) < 0 && P3(mask, i, j, k) > 0)
+                    for (int t = i; t >= 0 - gw; t--)
+                        P3(dst, t, j, k) = (double)2 * P3(src, t + 1, j, k) - P3(src, t + 2, j, k);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (P3(mask, i, j, k - 1) < 0 && P3(mask, i, j, k) > 0)
+                    for (int t = k; t <= ke + gw; t++)
+                        P3(dst, i, j, t) = (double)2 * P3(src, i, j, t - 1) - P3(src, i, j, t - 2);
+    for (int k = 0; k < nk; k++)
+        for (int j = 0; j < nj; j++)
+            for (int i = 0; i < ni; i++)
+                if (P3(mask, i, j, k + 1) < 0 && P3(mask, i, j, k) > 0)
+                    for (int t = k; t >= 0 - gw; t--)
+                        P3(dst, i, j, t) = (double)2 * P3(src, i, j, t + 1) - P3(src, i, j, t + 2);
+}
+
+/* the fill along the pipe's axis: x for the parallel pipe, y for the
+ * vertical one (every periodic fill of the step has both branches) */
+static void fill_periodic(fsop_state* s, double* dst, const double* src, int c, int gw, int copy, const double* mask) {
+    if (s->vert)
+        fill_periodic_y(s, dst, src, c, gw, copy, mask);
+    else
+        fill_periodic_x(s, dst, src, c, gw, copy, mask);
 }
 
 /* FillGhostCellsFrom (constant extrapolation, FIELD_STRUCTURE_3D.h:663-1086)
@@ -227,7 +325,8 @@ static void fill_linear_self(fsop_state* s, double* a, int d) {
 
 int fsop_create(const fs_params* P, const fs_pipe_params* Q, const fs_grid* g, fsop_state** out) {
     if (!P || !Q || !g || !out) return FS_ERR_BAD_ARG;
-    if (!Q->is_parallel || !Q->dimensionless_form || !Q->cylindrical_boundary || Q->boundary_order_velocity != 1)
+    if ((Q->is_parallel != 0 && Q->is_parallel != 1) || !Q->dimensionless_form || !Q->cylindrical_boundary ||
+        Q->boundary_order_velocity != 1)
         return FS_ERR_UNSUPPORTED;
     if (g->nx < 8 || g->ny < 8 || g->nz < 8) return FS_ERR_BAD_ARG;
     if (P->order_for_time_advancing < 1 || P->order_for_time_advancing > 3) return FS_ERR_BAD_ARG;
@@ -235,6 +334,7 @@ int fsop_create(const fs_params* P, const fs_pipe_params* Q, const fs_grid* g, f
     s->P = *P;
     s->Q = *Q;
     s->gr = *g;
+    s->vert = !Q->is_parallel;
     s->nx = g->nx, s->ny = g->ny, s->nz = g->nz;
     s->sx = g->nx + 1 + 2 * PG;
     s->sy = g->ny + 1 + 2 * PG;
@@ -245,6 +345,7 @@ int fsop_create(const fs_params* P, const fs_pipe_params* Q, const fs_grid* g, f
     double dx = s->cell.x.d, dy = s->cell.y.d, dz = s->cell.z.d;
     for (int c = 0; c < 4; c++) {
         s->mac[c] = s->cell;
+        s->x0f[c] = g->x_min;
         s->y0f[c] = g->y_min;
         s->z0f[c] = g->z_min;
     }
@@ -252,6 +353,7 @@ int fsop_create(const fs_params* P, const fs_pipe_params* Q, const fs_grid* g, f
     s->mac[1].x = mkax(g->x_min - (double)0.5 * dx, g->x_max + (double)0.5 * dx, g->nx + 1);
     s->mac[2].y = mkax(g->y_min - (double)0.5 * dy, g->y_max + (double)0.5 * dy, g->ny + 1);
     s->mac[3].z = mkax(g->z_min - (double)0.5 * dz, g->z_max + (double)0.5 * dz, g->nz + 1);
+    s->x0f[1] = g->x_min - (double)0.5 * dx;
     s->y0f[2] = g->y_min - (double)0.5 * dy;
     s->z0f[3] = g->z_min - (double)0.5 * dz;
     /* EULERIAN_FLUID_SOLVER_3D.h:496-520 */
@@ -286,7 +388,8 @@ int fsop_create(const fs_params* P, const fs_pipe_params* Q, const fs_grid* g, f
     s->pbc = (int*)calloc((size_t)n, sizeof(int));
     s->vbc = (int*)calloc((size_t)n, sizeof(int));
     /* boundary_phi_<c> on the interior of each face grid: sqrt(y^2 + z^2) - a
-     * (ADVECTION_3D.h:412-432 and its copies in VISCOSITY_3D, PROJECTION_3D and
+     * for the parallel pipe, sqrt(x^2 + z^2) - a for the vertical one
+     * (ADVECTION_3D.h:380-432 and its copies in VISCOSITY_3D, PROJECTION_3D and
      * EULERIAN_FLUID_SOLVER_3D) */
     for (int c = 1; c < 4; c++) {
         int ni, nj, nk;
@@ -294,23 +397,27 @@ int fsop_create(const fs_params* P, const fs_pipe_params* Q, const fs_grid* g, f
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) {
-                    double y = s->y0f[c] + j * s->mac[c].y.d, z = s->z0f[c] + k * s->mac[c].z.d;
-                    P3(s->bp[c], i, j, k) = sqrt(y * y + z * z) - a;
+                    double w1 = s->vert ? s->x0f[c] + i * s->mac[c].x.d : s->y0f[c] + j * s->mac[c].y.d;
+                    double z = s->z0f[c] + k * s->mac[c].z.d;
+                    P3(s->bp[c], i, j, k) = sqrt(w1 * w1 + z * z) - a;
                 }
     }
     /* the pipe inside the box: the first and last two cells and faces of
-     * every y and z line are outside the wall */
+     * every line across it (y and z for the parallel pipe, x and z for the
+     * vertical one) are outside the wall */
     for (int c = 0; c < 4; c++) {
         int ni, nj, nk;
         box(s, c, &ni, &nj, &nk);
-        double y0 = s->y0f[c], z0 = s->z0f[c], ddy = s->mac[c].y.d, ddz = s->mac[c].z.d;
         int bad = 0;
         for (int k = 0; k < nk; k++)
-            for (int j = 0; j < nj; j++) {
-                if (!(j < 2 || j >= nj - 2 || k < 2 || k >= nk - 2)) continue;
-                double y = y0 + j * ddy, z = z0 + k * ddz;
-                if (!(sqrt(y * y + z * z) - a > 0)) bad = 1;
-            }
+            for (int j = 0; j < nj; j++)
+                for (int i = 0; i < ni; i++) {
+                    int q = s->vert ? i : j, nq = s->vert ? ni : nj;
+                    if (!(q < 2 || q >= nq - 2 || k < 2 || k >= nk - 2)) continue;
+                    double w1 = s->vert ? s->x0f[c] + i * s->mac[c].x.d : s->y0f[c] + j * s->mac[c].y.d;
+                    double z = s->z0f[c] + k * s->mac[c].z.d;
+                    if (!(sqrt(w1 * w1 + z * z) - a > 0)) bad = 1;
+                }
         if (bad) {
             fsop_destroy(s);
             return FS_ERR_BAD_ARG;
@@ -350,33 +457,58 @@ int fsop_constants(fsop_state* s, double* out) {
     return FS_OK;
 }
 
-/* The scene's initial state: EULERIAN_FLUID_SOLVER_3D.h:536-563 (x faces of
- * the dimensionless parallel pipe, ghost layers of width 2 included),
- * :453-460 and :755-757 (the level sets' ghosts from a constant 3 dx), then
- * SIMULATION_WORLD.cpp:288-319: phi = sqrt(y^2+z^2) - (A_0 cos(alpha x) + 1)
- * on the interior, its ghosts the linear extrapolation of the pipe level
- * set's values at that moment (:303, all 3 dx, so 3 dx), and the pipe level
- * set sqrt(y^2+z^2) - a with its ghosts linearly extrapolated (:319). */
+/* The scene's initial state: EULERIAN_FLUID_SOLVER_3D.h:536-615, the
+ * dimensionless core-annular profile -- on the x faces with their ghost
+ * layers of width 2 for the parallel pipe; on the y faces' interior, with
+ * the sign of V0, for the vertical one --, :453-460 and :755-757 (the level
+ * sets' ghosts from a constant 3 dx), then SIMULATION_WORLD.cpp:288-319:
+ * phi = r - (A_0 cos(alpha s) + 1) on the interior (r the distance from the
+ * pipe's axis, s the coordinate along it), its ghosts the linear
+ * extrapolation of the pipe level set's values at that moment (:303, all
+ * 3 dx, so 3 dx), and the pipe level set r - a with its ghosts linearly
+ * extrapolated (:319). */
 int fsop_initial_fields(fsop_state* s) {
     const double a = s->Q.a, A = s->A, K = s->Q.K, m = s->Q.m;
     memset(s->u, 0, sizeof(double) * s->ntot);
     memset(s->v, 0, sizeof(double) * s->ntot);
     memset(s->w, 0, sizeof(double) * s->ntot);
     memset(s->p, 0, sizeof(double) * s->ntot);
-    for (int k = -2; k <= s->nz - 1 + 2; k++)
-        for (int j = -2; j <= s->ny - 1 + 2; j++)
-            for (int i = -2; i <= s->nx + 2; i++) {
-                double y = s->gr.y_min + j * s->mac[1].y.d, z = s->gr.z_min + k * s->mac[1].z.d;
-                double magnitude = sqrt(y * y + z * z);
-                double val;
-                if ((magnitude >= 1) && (magnitude <= a))
-                    val = (double)1 / A * (a * a - (y * y + z * z) - (double)2 * (K - 1) * log(sqrt(y * y + z * z) / a));
-                else if (magnitude < 1)
-                    val = (double)1 - (double)1 / A * (m * (y * y + z * z) * K);
-                else
-                    val = 0;
-                P3(s->u, i, j, k) = val;
-            }
+    if (s->vert) {
+        for (int k = 0; k <= s->nz - 1; k++)
+            for (int j = 0; j <= s->ny; j++)
+                for (int i = 0; i <= s->nx - 1; i++) {
+                    double x = s->gr.x_min + i * s->mac[2].x.d, z = s->gr.z_min + k * s->mac[2].z.d;
+                    double magnitude = sqrt(x * x + z * z);
+                    double val = 0;
+                    if ((magnitude >= 1) && (magnitude <= a)) {
+                        if (s->V0 > 0)
+                            val = (double)1 / A * (a * a - (x * x + z * z) - (double)2 * (K - 1) * log(sqrt(x * x + z * z) / a));
+                        else
+                            val = -(double)1 / A * (a * a - (x * x + z * z) - (double)2 * (K - 1) * log(sqrt(x * x + z * z) / a));
+                    } else if (magnitude < 1) {
+                        if (s->V0 > 0)
+                            val = (double)1 - (double)1 / A * (m * (x * x + z * z) * K);
+                        else
+                            val = -((double)1 - (double)1 / A * (m * (x * x + z * z) * K));
+                    }
+                    P3(s->v, i, j, k) = val;
+                }
+    } else {
+        for (int k = -2; k <= s->nz - 1 + 2; k++)
+            for (int j = -2; j <= s->ny - 1 + 2; j++)
+                for (int i = -2; i <= s->nx + 2; i++) {
+                    double y = s->gr.y_min + j * s->mac[1].y.d, z = s->gr.z_min + k * s->mac[1].z.d;
+                    double magnitude = sqrt(y * y + z * z);
+                    double val;
+                    if ((magnitude >= 1) && (magnitude <= a))
+                        val = (double)1 / A * (a * a - (y * y + z * z) - (double)2 * (K - 1) * log(sqrt(y * y + z * z) / a));
+                    else if (magnitude < 1)
+                        val = (double)1 - (double)1 / A * (m * (y * y + z * z) * K);
+                    else
+                        val = 0;
+                    P3(s->u, i, j, k) = val;
+                }
+    }
     const double three_dx = s->cell.x.d * (double)3;
     for (long q = 0; q < s->ntot; q++) s->phi[q] = three_dx;
     for (int k = 0; k < s->nz; k++)
@@ -384,8 +516,13 @@ int fsop_initial_fields(fsop_state* s) {
             for (int i = 0; i < s->nx; i++) {
                 double x = s->gr.x_min + i * s->cell.x.d, y = s->gr.y_min + j * s->cell.y.d,
                        z = s->gr.z_min + k * s->cell.z.d;
-                P3(s->phi, i, j, k) = sqrt(y * y + z * z) - (s->A0 * cos(s->Q.alpha * x) + (double)1);
-                P3(s->bls, i, j, k) = sqrt(y * y + z * z) - a;
+                if (s->vert) {
+                    P3(s->phi, i, j, k) = sqrt(x * x + z * z) - (s->A0 * cos(s->Q.alpha * y) + (double)1);
+                    P3(s->bls, i, j, k) = sqrt(x * x + z * z) - a;
+                } else {
+                    P3(s->phi, i, j, k) = sqrt(y * y + z * z) - (s->A0 * cos(s->Q.alpha * x) + (double)1);
+                    P3(s->bls, i, j, k) = sqrt(y * y + z * z) - a;
+                }
             }
     fill_linear_self(s, s->bls, 2);
     return FS_OK;
@@ -428,17 +565,56 @@ int fsop_get_fields(fsop_state* s, double* phi, double* u, double* v, double* w,
 
 static double theta_of(double near, double far) { return fabs(near) / (fabs(near) + fabs(far)); }
 
-/* ADVECTION_3D::SetupBoundaryConditionForVelocity, level-set branch,
- * is_parallel (ADVECTION_3D.h:494-550): masked periodic copies of u, v, w
+/* ADVECTION_3D::SetupBoundaryConditionForVelocity, level-set branch
+ * (ADVECTION_3D.h:436-550); is_parallel (:494-550): masked periodic copies of u, v, w
  * into the shared ghost copies, then on y and z faces outside the pipe the
  * velocity is zeroed and the ghost copy extrapolated from inside.  On y
  * faces the second test repeats `phi(j-1) < 0` (:528) and takes the j+1
  * side, overwriting the first: the value read at j+1 is the one before this
  * loop reaches j+1. */
 static void bc_levelset_advection(fsop_state* s) {
-    fill_periodic_x(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
-    fill_periodic_x(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
-    fill_periodic_x(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
+    fill_periodic(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
+    fill_periodic(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
+    fill_periodic(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
+    if (s->vert) {
+        /* is_vertical (:438-491): x faces with both x tests as written, then
+         * z faces, the all-outside test over x and z neighbours */
+        const double* b = s->bp[1];
+        for (int k = 0; k < s->nz; k++)
+            for (int j = 0; j < s->ny; j++)
+                for (int i = 0; i < s->nx + 1; i++) {
+                    if (!(P3(b, i, j, k) > 0)) continue;
+                    P3(s->u, i, j, k) = 0;
+                    if (P3(b, i - 1, j, k) > 0 && P3(b, i + 1, j, k) > 0 && P3(b, i, j, k - 1) > 0 && P3(b, i, j, k + 1) > 0)
+                        P3(s->gx, i, j, k) = 0;
+                    if (P3(b, i - 1, j, k) < 0) {
+                        double th = theta_of(P3(b, i - 1, j, k), P3(b, i, j, k));
+                        P3(s->gx, i, j, k) = (th - 1) / th * P3(s->gx, i - 1, j, k);
+                    }
+                    if (P3(b, i + 1, j, k) < 0) {
+                        double th = theta_of(P3(b, i + 1, j, k), P3(b, i, j, k));
+                        P3(s->gx, i, j, k) = (th - 1) / th * P3(s->gx, i + 1, j, k);
+                    }
+                }
+        b = s->bp[3];
+        for (int k = 0; k < s->nz + 1; k++)
+            for (int j = 0; j < s->ny; j++)
+                for (int i = 0; i < s->nx; i++) {
+                    if (!(P3(b, i, j, k) > 0)) continue;
+                    P3(s->w, i, j, k) = 0;
+                    if (P3(b, i - 1, j, k) > 0 && P3(b, i + 1, j, k) > 0 && P3(b, i, j, k - 1) > 0 && P3(b, i, j, k + 1) > 0)
+                        P3(s->gz, i, j, k) = 0;
+                    if (P3(b, i, j, k - 1) < 0) {
+                        double th = theta_of(P3(b, i, j, k - 1), P3(b, i, j, k));
+                        P3(s->gz, i, j, k) = (th - 1) / th * P3(s->gz, i, j, k - 1);
+                    }
+                    if (P3(b, i, j, k + 1) < 0) {
+                        double th = theta_of(P3(b, i, j, k + 1), P3(b, i, j, k));
+                        P3(s->gz, i, j, k) = (th - 1) / th * P3(s->gz, i, j, k + 1);
+                    }
+                }
+        return;
+    }
     const double* b = s->bp[2];
     for (int k = 0; k < s->nz; k++)
         for (int j = 0; j < s->ny + 1; j++)
@@ -483,14 +659,17 @@ static void bc_levelset_advection(fsop_state* s) {
  * the next three are pow(10 t, t), t = 2, 3, 4 (so ENO3 never picks a
  * stencil through the wall).  x faces, then z, then y (:716-851). */
 static void bc_velocity_advection(fsop_state* s) {
-    fill_periodic_x(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
-    fill_periodic_x(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
-    fill_periodic_x(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
+    fill_periodic(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
+    fill_periodic(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
+    fill_periodic(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
     const int order[3] = {1, 3, 2};
     for (int c = 1; c < 4; c++) {
-        fill_periodic_x(s, s->ag[c - 1][0], comp_field(s, c), c, 3, 1, s->bp[c]);
-        fill_periodic_x(s, s->ag[c - 1][1], comp_field(s, c), c, 3, 1, s->bp[c]);
+        fill_periodic(s, s->ag[c - 1][0], comp_field(s, c), c, 3, 1, s->bp[c]);
+        fill_periodic(s, s->ag[c - 1][1], comp_field(s, c), c, 3, 1, s->bp[c]);
     }
+    /* the wall's first axis: y for the parallel pipe, x for the vertical one
+     * (:553-701 writes the x copies <c>_x in place of the y copies) */
+    const int ui = s->vert, uj = !s->vert;
     for (int oc = 0; oc < 3; oc++) {
         int c = order[oc], ni, nj, nk;
         box(s, c, &ni, &nj, &nk);
@@ -501,15 +680,15 @@ static void bc_velocity_advection(fsop_state* s) {
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) {
                     double bc0 = P3(b, i, j, k);
-                    if (bc0 < 0 && P3(b, i, j + 1, k) > 0) {
-                        double th = theta_of(bc0, P3(b, i, j + 1, k));
-                        P3(ay, i, j + 1, k) = (th - 1) / th * P3(g, i, j, k);
-                        for (int t = 2; t <= 4; t++) P3(ay, i, j + t, k) = pow(10 * t, t);
+                    if (bc0 < 0 && P3(b, i + ui, j + uj, k) > 0) {
+                        double th = theta_of(bc0, P3(b, i + ui, j + uj, k));
+                        P3(ay, i + ui, j + uj, k) = (th - 1) / th * P3(g, i, j, k);
+                        for (int t = 2; t <= 4; t++) P3(ay, i + t * ui, j + t * uj, k) = pow(10 * t, t);
                     }
-                    if (bc0 < 0 && P3(b, i, j - 1, k) > 0) {
-                        double th = theta_of(bc0, P3(b, i, j - 1, k));
-                        P3(ay, i, j - 1, k) = (th - 1) / th * P3(g, i, j, k);
-                        for (int t = 2; t <= 4; t++) P3(ay, i, j - t, k) = pow(10 * t, t);
+                    if (bc0 < 0 && P3(b, i - ui, j - uj, k) > 0) {
+                        double th = theta_of(bc0, P3(b, i - ui, j - uj, k));
+                        P3(ay, i - ui, j - uj, k) = (th - 1) / th * P3(g, i, j, k);
+                        for (int t = 2; t <= 4; t++) P3(ay, i - t * ui, j - t * uj, k) = pow(10 * t, t);
                     }
                     if (bc0 < 0 && P3(b, i, j, k + 1) > 0) {
                         double th = theta_of(bc0, P3(b, i, j, k + 1));
@@ -532,35 +711,40 @@ static void bc_velocity_advection(fsop_state* s) {
  * y/z neighbours outside), the extrapolation from j-1, j+1, k-1 or k+1 (the
  * first inside one) into its y or z copy, or zero in both. */
 static void bc_viscosity(fsop_state* s) {
-    fill_periodic_x(s, s->sfg, s->phi, 0, 3, 1, NULL);
-    fill_periodic_x(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
-    fill_periodic_x(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
-    fill_periodic_x(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
+    fill_periodic(s, s->sfg, s->phi, 0, 3, 1, NULL);
+    fill_periodic(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
+    fill_periodic(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
+    fill_periodic(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
     for (int c = 1; c < 4; c++) {
-        fill_periodic_x(s, s->vg[c - 1][0], comp_field(s, c), c, 3, 1, s->bp[c]);
-        fill_periodic_x(s, s->vg[c - 1][1], comp_field(s, c), c, 3, 1, s->bp[c]);
+        fill_periodic(s, s->vg[c - 1][0], comp_field(s, c), c, 3, 1, s->bp[c]);
+        fill_periodic(s, s->vg[c - 1][1], comp_field(s, c), c, 3, 1, s->bp[c]);
     }
     const int order[3] = {1, 3, 2};
+    /* the wall's first axis: y (parallel), x (vertical, :2106-2233, the x
+     * copies <c>_x; its y-face branch extrapolates from v itself, which
+     * equals its shared copy on the inside faces it reads) */
+    const int ui = s->vert, uj = !s->vert;
     for (int oc = 0; oc < 3; oc++) {
         int c = order[oc], ni, nj, nk;
         box(s, c, &ni, &nj, &nk);
         const double* b = s->bp[c];
-        const double* g = ghost_comp(s, c);
+        const double* g = s->vert && c == 2 ? s->v : ghost_comp(s, c);
         double *ay = s->vg[c - 1][0], *az = s->vg[c - 1][1];
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) {
                     double b0 = P3(b, i, j, k);
                     if (!(b0 > 0)) continue;
-                    if (P3(b, i, j - 1, k) > 0 && P3(b, i, j + 1, k) > 0 && P3(b, i, j, k - 1) > 0 && P3(b, i, j, k + 1) > 0) {
+                    if (P3(b, i - ui, j - uj, k) > 0 && P3(b, i + ui, j + uj, k) > 0 && P3(b, i, j, k - 1) > 0 &&
+                        P3(b, i, j, k + 1) > 0) {
                         P3(ay, i, j, k) = 0;
                         P3(az, i, j, k) = 0;
-                    } else if (P3(b, i, j - 1, k) < -0) {
-                        double th = theta_of(P3(b, i, j - 1, k), b0);
-                        P3(ay, i, j, k) = (th - 1) / th * P3(g, i, j - 1, k);
-                    } else if (P3(b, i, j + 1, k) < -0) {
-                        double th = theta_of(P3(b, i, j + 1, k), b0);
-                        P3(ay, i, j, k) = (th - 1) / th * P3(g, i, j + 1, k);
+                    } else if (P3(b, i - ui, j - uj, k) < -0) {
+                        double th = theta_of(P3(b, i - ui, j - uj, k), b0);
+                        P3(ay, i, j, k) = (th - 1) / th * P3(g, i - ui, j - uj, k);
+                    } else if (P3(b, i + ui, j + uj, k) < -0) {
+                        double th = theta_of(P3(b, i + ui, j + uj, k), b0);
+                        P3(ay, i, j, k) = (th - 1) / th * P3(g, i + ui, j + uj, k);
                     } else if (P3(b, i, j, k - 1) < -0) {
                         double th = theta_of(P3(b, i, j, k - 1), b0);
                         P3(az, i, j, k) = (th - 1) / th * P3(g, i, j, k - 1);
@@ -580,11 +764,14 @@ static void bc_viscosity(fsop_state* s) {
  * outside the pipe: zero (all four outside), the extrapolation from the
  * first inside neighbour along the face's own axis, or zero. */
 static void bc_divergence(fsop_state* s) {
-    fill_periodic_x(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
-    fill_periodic_x(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
-    fill_periodic_x(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
-    for (int c = 3; c >= 2; c--) {
-        int ni, nj, nk;
+    fill_periodic(s, s->gx, s->u, 1, 3, 1, s->bp[1]);
+    fill_periodic(s, s->gy, s->v, 2, 3, 1, s->bp[2]);
+    fill_periodic(s, s->gz, s->w, 3, 3, 1, s->bp[3]);
+    /* parallel: z faces, then y; vertical (:1566-1627): x faces, then z */
+    const int ui = s->vert, uj = !s->vert;
+    const int comps[2] = {s->vert ? 1 : 3, s->vert ? 3 : 2};
+    for (int q = 0; q < 2; q++) {
+        int c = comps[q], ni, nj, nk;
         box(s, c, &ni, &nj, &nk);
         const double* b = s->bp[c];
         double* g = ghost_comp(s, c);
@@ -593,15 +780,16 @@ static void bc_divergence(fsop_state* s) {
                 for (int i = 0; i < ni; i++) {
                     double b0 = P3(b, i, j, k);
                     if (!(b0 > 0)) continue;
-                    int dj = c == 2, dk = c == 3;
-                    if (P3(b, i, j - 1, k) > 0 && P3(b, i, j + 1, k) > 0 && P3(b, i, j, k - 1) > 0 && P3(b, i, j, k + 1) > 0)
+                    int di = c == 1, dj = c == 2, dk = c == 3;
+                    if (P3(b, i - ui, j - uj, k) > 0 && P3(b, i + ui, j + uj, k) > 0 && P3(b, i, j, k - 1) > 0 &&
+                        P3(b, i, j, k + 1) > 0)
                         P3(g, i, j, k) = 0;
-                    else if (P3(b, i, j - dj, k - dk) < 0) {
-                        double th = theta_of(P3(b, i, j - dj, k - dk), b0);
-                        P3(g, i, j, k) = (th - (double)1) / th * P3(g, i, j - dj, k - dk);
-                    } else if (P3(b, i, j + dj, k + dk) < 0) {
-                        double th = theta_of(P3(b, i, j + dj, k + dk), b0);
-                        P3(g, i, j, k) = (th - (double)1) / th * P3(g, i, j + dj, k + dk);
+                    else if (P3(b, i - di, j - dj, k - dk) < 0) {
+                        double th = theta_of(P3(b, i - di, j - dj, k - dk), b0);
+                        P3(g, i, j, k) = (th - (double)1) / th * P3(g, i - di, j - dj, k - dk);
+                    } else if (P3(b, i + di, j + dj, k + dk) < 0) {
+                        double th = theta_of(P3(b, i + di, j + dj, k + dk), b0);
+                        P3(g, i, j, k) = (th - (double)1) / th * P3(g, i + di, j + dj, k + dk);
                     } else
                         P3(g, i, j, k) = 0;
                 }
@@ -652,7 +840,7 @@ static double upwind_(double rho, double dt, double uu, double vv, double ww, do
  * (ADVECTION_METHOD_3D.h:281-523): cells outside the pipe are skipped. */
 int fsop_levelset_advection(fsop_state* s, double dt) {
     bc_levelset_advection(s);
-    fill_periodic_x(s, s->sfg, s->phi, 0, 3, 1, NULL);
+    fill_periodic(s, s->sfg, s->phi, 0, 3, 1, NULL);
     const double eps = s->P.levelset_epsilon;
     const pgc_t* gc = &s->cell;
     for (int k = 0; k < s->nz; k++)
@@ -686,12 +874,15 @@ int fsop_velocity_advection(fsop_state* s, double dt) {
         const pgc_t* g = &s->mac[c];
         const double inv3x = g->x.inv * (double)1 / 3, inv3y = g->y.inv * (double)1 / 3,
                      inv3z = g->z.inv * (double)1 / 3;
-        const double *sx_ = ghost_comp(s, c), *sy_ = s->ag[c - 1][0], *sz_ = s->ag[c - 1][1];
+        /* the derivative along the pipe from the shared copy, across it from
+         * the per-axis copies (:316-330) */
+        const double *sx_ = s->vert ? s->ag[c - 1][0] : ghost_comp(s, c),
+                     *sy_ = s->vert ? ghost_comp(s, c) : s->ag[c - 1][0], *sz_ = s->ag[c - 1][1];
         double* rho = comp_field(s, c);
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) {
-                    if (fixed_face(s, c, j, k)) continue;
+                    if (fixed_face(s, c, i, j, k)) continue;
                     if (!(P3(s->bp[c], i, j, k) < (double)0)) continue;
                     double f[7], mx, px, my, py, mz, pz_, uu, vv, ww;
                     gather7_(s, sx_, i, j, k, 0, f);
@@ -758,13 +949,29 @@ int fsop_apply_gravity(fsop_state* s, double dt) {
     const double drive = dt * s->f;
     const double eta = s->Q.eta, ro = s->Q.oil_density, R1 = s->R1, V0 = s->V0;
     const double eps = (double)1.5 * s->cell.x.d;
-    fill_periodic_x(s, s->sfg, s->phi, 0, 3, 1, NULL);
+    fill_periodic(s, s->sfg, s->phi, 0, 3, 1, NULL);
+    if (s->vert) {
+        /* is_vertical (:2327-2360): the driving pressure along y, on the y
+         * faces only, density at the y faces (eps 1.5 dy) */
+        const double epsy = (double)1.5 * s->cell.y.d;
+        for (int k = 0; k < s->nz; k++)
+            for (int j = 0; j < s->ny + 1; j++)
+                for (int i = 0; i < s->nx; i++) {
+                    double lc = (double)0.5 * (P3(s->sfg, i, j, k) + P3(s->sfg, i, j - 1, k));
+                    double dh = (double)1 + (eta - (double)1) * heav_solver(lc, epsy);
+                    if (fixed_face(s, 2, i, j, k)) continue;
+                    if (P3(s->bp[2], i, j, k) < 0)
+                        P3(s->v, i, j, k) += R1 / (V0 * V0) * (gdy + (double)1 / (ro * dh) * drive);
+                }
+        (void)gdx, (void)gdz, (void)eps;
+        return FS_OK;
+    }
     for (int k = 0; k < s->nz; k++)
         for (int j = 0; j < s->ny; j++)
             for (int i = 0; i < s->nx + 1; i++) {
                 double lc = (double)0.5 * (P3(s->sfg, i, j, k) + P3(s->sfg, i - 1, j, k));
                 double dh = (double)1 + (eta - (double)1) * heav_solver(lc, eps);
-                if (fixed_face(s, 1, j, k)) continue;
+                if (fixed_face(s, 1, i, j, k)) continue;
                 if (P3(s->bp[1], i, j, k) < 0)
                     P3(s->u, i, j, k) += R1 / (V0 * V0) * (gdx + ((double)1 / (dh * ro)) * drive);
             }
@@ -776,7 +983,7 @@ int fsop_apply_gravity(fsop_state* s, double dt) {
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) {
-                    if (fixed_face(s, c, j, k)) continue;
+                    if (fixed_face(s, c, i, j, k)) continue;
                     if (P3(s->bp[c], i, j, k) < 0) P3(vel, i, j, k) += R1 / (V0 * V0) * gd;
                 }
     }
@@ -916,9 +1123,9 @@ static int number_rows(fsop_state* s, int* bc, int c) {
 /* ---- semi-implicit viscosity ---------------------------------------------- */
 
 /* VISCOSITY_3D::SetupBoundaryConditionsForVelocity(velocity, bc),
- * cylindrical, is_parallel (VISCOSITY_3D.h:1350-1546): over the face box
- * grown by one, DIR outside the pipe, PER beyond the x ends (from the last
- * face on for x faces), FULL elsewhere. */
+ * cylindrical (VISCOSITY_3D.h:1350-1546): over the face box grown by one,
+ * DIR outside the pipe, PER beyond the ends along the pipe (from the last
+ * face on for faces normal to it), FULL elsewhere. */
 static void viscosity_bc(fsop_state* s, int c) {
     int ni, nj, nk;
     box(s, c, &ni, &nj, &nk);
@@ -928,17 +1135,17 @@ static void viscosity_bc(fsop_state* s, int c) {
             for (int i = -1; i <= ni; i++) {
                 int v;
                 if (P3(b, i, j, k) > 0) v = BC_DIR;
-                else if (i < 0) v = BC_PER;
-                else if (c == 1 ? i >= ni - 1 : i > ni - 1) v = BC_PER;
+                else if (s->vert ? j < 0 : i < 0) v = BC_PER;
+                else if (s->vert ? (c == 2 ? j >= nj - 1 : j > nj - 1) : (c == 1 ? i >= ni - 1 : i > ni - 1)) v = BC_PER;
                 else v = BC_FULL;
                 P3(s->vbc, i, j, k) = v;
             }
 }
 
 /* POISSON_SOLVER_3D::BuildLinearSystemNodeForSemiImplicitViscosity with
- * per-axis ghost arrays (:3641-3828), then Solve and VectorToGrid (fixed
- * faces keep their value, :4060-4078) and the x-periodic copy of the last
- * x face (:366-376). */
+ * per-axis ghost arrays (:3641-3828, periodic entries along x, then y),
+ * then Solve and VectorToGrid (fixed faces keep their value, :4060-4078) and
+ * the periodic copy of the last face normal to the pipe (:354-376). */
 static void viscosity_solve(fsop_state* s, int c, const double* vx, const double* vy, const double* vz) {
     int ni, nj, nk;
     box(s, c, &ni, &nj, &nk);
@@ -949,7 +1156,7 @@ static void viscosity_solve(fsop_state* s, int c, const double* vx, const double
     const pgc_t* g = &s->mac[c];
     const double inv_dxdx = (double)1 / (g->x.d * g->x.d), inv_dydy = (double)1 / (g->y.d * g->y.d),
                  inv_dzdz = (double)1 / (g->z.d * g->z.d);
-    const int ie = ni - 1;
+    const int ie = ni - 1, je = nj - 1;
     double* vel = comp_field(s, c);
     double* const* cf = s->coef;
 #define BC(a, b, cc) P3(bc, a, b, cc)
@@ -975,6 +1182,16 @@ static void viscosity_solve(fsop_state* s, int c, const double* vx, const double
                     coef_ijk += CF(0);
                     int col = c == 1 ? BC(0 + (i + 1 - (ie - 1)), j, k) : BC(0 + (i + 1 - ie), j, k);
                     sys_push(S, col, -CF(0) * inv_dxdx);
+                }
+                if (BC(i, j - 1, k) == BC_PER) {
+                    coef_ijk += CF(3);
+                    int col = c == 2 ? BC(i, je + (j - 1 - (0 + 1)), k) : BC(i, je + (j - 1 - 0), k);
+                    sys_push(S, col, -CF(3) * inv_dydy);
+                }
+                if (BC(i, j + 1, k) == BC_PER) {
+                    coef_ijk += CF(2);
+                    int col = c == 2 ? BC(i, 0 + (j + 1 - (je - 1)), k) : BC(i, 0 + (j + 1 - je), k);
+                    sys_push(S, col, -CF(2) * inv_dydy);
                 }
                 if (BC(i - 1, j, k) == BC_DIR) coef_ijk += CF(1);
                 if (BC(i + 1, j, k) == BC_DIR) coef_ijk += CF(0);
@@ -1003,12 +1220,17 @@ static void viscosity_solve(fsop_state* s, int c, const double* vx, const double
     for (int k = 0; k < nk; k++)
         for (int j = 0; j < nj; j++)
             for (int i = 0; i < ni; i++) {
-                if (fixed_face(s, c, j, k)) continue;
+                if (fixed_face(s, c, i, j, k)) continue;
                 if (BC(i, j, k) < 0) continue;
                 P3(vel, i, j, k) = S->x[BC(i, j, k)];
             }
 #undef BC
-    if (c == 1)
+    if (s->vert) {
+        if (c == 2)
+            for (int k = 0; k < nk; k++)
+                for (int j = 0; j < nj; j++)
+                    for (int i = 0; i < ni; i++) P3(vel, i, je, k) = P3(vel, i, 0 + 1, k);
+    } else if (c == 1)
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++)
                 for (int i = 0; i < ni; i++) P3(vel, ie, j, k) = P3(vel, 0 + 1, j, k);
@@ -1066,13 +1288,16 @@ static int viscosity(fsop_state* s, double dt) {
                         P3(s->coef[e], i, j, k) = nrm ? (double)2 * dt / (Re * rho_h) * mu[e] : dt / (Re * rho_h) * mu[e];
                     }
                     double ex;
+                    /* the per-axis copies the cross terms read: the parallel pipe's
+                     * x term reads y_x and z_x, which it never writes (zero); the
+                     * vertical pipe's are its x copies (:615-621, 754-760, 821-827) */
                     if (c == 1) {
-                        const double *Y = s->vgyx, *Z = s->vgzx;
+                        const double *Y = s->vert ? s->vg[1][0] : s->vgyx, *Z = s->vert ? s->vg[2][0] : s->vgzx;
                         ex = P3(s->gx, i, j, k) +
                              dt / (Re * rho_h) * (oy * (mu[2] * ox * (P3(Y, i, j + 1, k) - P3(Y, i - 1, j + 1, k)) - mu[3] * ox * (P3(Y, i, j, k) - P3(Y, i - 1, j, k)))) +
                              dt / (Re * rho_h) * (oz * (mu[4] * ox * (P3(Z, i, j, k + 1) - P3(Z, i - 1, j, k + 1)) - mu[5] * ox * (P3(Z, i, j, k) - P3(Z, i - 1, j, k))));
                     } else if (c == 2) {
-                        const double *X = s->vg[0][0], *Z = s->vg[2][0];
+                        const double *X = s->vg[0][0], *Z = s->vert ? s->vg[2][1] : s->vg[2][0];
                         ex = P3(s->gy, i, j, k) +
                              dt / (Re * rho_h) * (ox * (mu[0] * oy * (P3(X, i + 1, j, k) - P3(X, i + 1, j - 1, k)) - mu[1] * oy * (P3(X, i, j, k) - P3(X, i, j - 1, k)))) +
                              dt / (Re * rho_h) * (oz * (mu[4] * oy * (P3(Z, i, j, k + 1) - P3(Z, i, j - 1, k + 1)) - mu[5] * ox * (P3(Z, i, j, k) - P3(Z, i, j - 1, k))));
@@ -1080,12 +1305,17 @@ static int viscosity(fsop_state* s, double dt) {
                         const double *X = s->vg[0][1], *Y = s->vg[1][1];
                         ex = P3(s->gz, i, j, k) +
                              dt / (Re * rho_h) * (ox * (oz * (mu[0] * (P3(X, i + 1, j, k) - P3(X, i + 1, j, k - 1))) - oz * (mu[1] * (P3(X, i, j, k) - P3(X, i, j, k - 1))))) +
-                             dt / (Re * rho_h) * (oy * (oz * (mu[2] * (P3(Y, i, j + 1, k) - P3(Y, i, j + 1, k - 1))) - ox * (mu[3] * (P3(Y, i, j, k) - P3(Y, i, j, k - 1)))));
+                             dt / (Re * rho_h) * (oy * (oz * (mu[2] * (P3(Y, i, j + 1, k) - P3(Y, i, j + 1, k - 1))) - (s->vert ? oz : ox) * (mu[3] * (P3(Y, i, j, k) - P3(Y, i, j, k - 1)))));
                     }
                     P3(s->expl, i, j, k) = ex;
                 }
         viscosity_bc(s, c);
-        viscosity_solve(s, c, ghost_comp(s, c), s->vg[c - 1][0], s->vg[c - 1][1]);
+        /* the Dirichlet values along x, y, z: the shared copy along the pipe,
+         * the per-axis copies across it (:712-716, 778-782, 844-848) */
+        if (s->vert)
+            viscosity_solve(s, c, s->vg[c - 1][0], ghost_comp(s, c), s->vg[c - 1][1]);
+        else
+            viscosity_solve(s, c, ghost_comp(s, c), s->vg[c - 1][0], s->vg[c - 1][1]);
     }
     return FS_OK;
 }
@@ -1164,6 +1394,16 @@ static void pressure_bc(fsop_state* s) {
         for (int j = -1; j <= s->ny; j++)
             for (int i = -1; i <= s->nx; i++) {
                 int v;
+                if (s->vert) {
+                    /* is_vertical (:1025-1043): every cell outside the pipe and
+                     * every y ghost NEUM */
+                    if (P3(b, i, j, k) > 0) v = BC_NEUM;
+                    else if (j < 0) v = BC_NEUM;
+                    else if (j > s->ny - 1) v = BC_NEUM;
+                    else v = BC_FULL;
+                    P3(s->pbc, i, j, k) = v;
+                    continue;
+                }
                 if (P3(b, i, j, k) > 0) {
                     if (P3(b, i, j - 1, k) < 0 || P3(b, i, j + 1, k) < 0 || P3(b, i, j, k - 1) < 0 || P3(b, i, j, k + 1) < 0)
                         v = BC_NEUM;
@@ -1276,7 +1516,7 @@ static void pressure_solve(fsop_state* s) {
  * or interface update, the max |u|, and the x-periodic copies of the end
  * faces. */
 static void velocity_update(fsop_state* s, double dt) {
-    fill_periodic_x(s, s->psfg, s->phi, 0, 2, 1, NULL);
+    fill_periodic(s, s->psfg, s->phi, 0, 2, 1, NULL);
     const double rr = s->P.water_density / s->Q.oil_density;
     for (int c = 1; c < 4; c++) {
         int ni, nj, nk;
@@ -1292,7 +1532,7 @@ static void velocity_update(fsop_state* s, double dt) {
                 for (int i = 0; i < ni; i++) {
                     double lcf = (double)0.5 * (P3(s->psfg, i, j, k) + P3(s->psfg, i - di, j - dj, k - dk));
                     double dh = 1 + (rr - (double)1) * heav_solver(lcf, eps);
-                    if (fixed_face(s, c, j, k)) continue;
+                    if (fixed_face(s, c, i, j, k)) continue;
                     if (P3(s->pbc, i, j, k) < 0) continue;
                     double* vp = &P3(vel, i, j, k);
                     const double density_ijk = P3(s->rho, i, j, k);
@@ -1304,7 +1544,18 @@ static void velocity_update(fsop_state* s, double dt) {
                     const double one_over_half = 1 / dh;
                     const double bc = (double)P3(s->pbc, i, j, k), bcn = (double)P3(s->pbc, i - di, j - dj, k - dk);
                     const double dp = P3(s->p, i, j, k) - P3(s->p, i - di, j - dj, k - dk);
-                    if (c == 1) {
+                    if (s->vert && c != 3) {
+                        /* is_vertical (:1188-1205, 1297-1318); the y faces skip the
+                         * first and the last face */
+                        if (bc * bcn >= 0 && (c == 1 || (j != 0 && j <= nj - 1 - 1))) {
+                            if (lv * ln > 0)
+                                *vp -= dt * dp * one_over_density_ijk * od;
+                            else if (lv <= 0)
+                                *vp -= dt * (dp + gamma) * one_over_half * od;
+                            else if (lv > 0)
+                                *vp -= dt * (dp - gamma) * one_over_half * od;
+                        }
+                    } else if (c == 1) {
                         if (blv * bln >= 0) {
                             if (bc * bcn > 0)
                                 *vp -= dt * dp * one_over_density_ijk * od;
@@ -1335,7 +1586,18 @@ static void velocity_update(fsop_state* s, double dt) {
                     vmax = vmax > fabs(*vp) ? vmax : fabs(*vp);
                 }
         s->max_vel[c - 1] = vmax;
-        /* :1248-1256, :1355-1363, :1436-1444 */
+        /* :1238-1256, :1345-1363, :1429-1444 */
+        if (s->vert) {
+            for (int k = 0; k < nk; k++)
+                for (int i = 0; i < ni; i++) {
+                    if (c == 2) {
+                        P3(vel, i, nj - 1, k) = P3(vel, i, 0 + 1, k);
+                        P3(vel, i, 0, k) = P3(vel, i, nj - 1 - 1, k);
+                    } else
+                        P3(vel, i, nj - 1, k) = P3(vel, i, 0, k);
+                }
+            continue;
+        }
         for (int k = 0; k < nk; k++)
             for (int j = 0; j < nj; j++) {
                 if (c == 1) {
@@ -1350,7 +1612,7 @@ static void velocity_update(fsop_state* s, double dt) {
 /* PROJECTION_3D::Solve, oil/water, is_parallel (PROJECTION_3D.h:414-432): */
 int fsop_projection(fsop_state* s, double dt, fs_step_stats* st) {
     /* phi's own x ghosts and first cell from the periodic image (:424-427) */
-    fill_periodic_x(s, s->phi, s->phi, 0, 2, 0, NULL);
+    fill_periodic(s, s->phi, s->phi, 0, 2, 0, NULL);
     /* DetermineProjectionDensity, dimensionless (:642-668) */
     const double rr = s->P.water_density / s->Q.oil_density;
     for (int k = 0; k < s->nz; k++)
@@ -1382,11 +1644,11 @@ int fsop_projection(fsop_state* s, double dt, fs_step_stats* st) {
     for (int k = 0; k < s->nz; k++)
         for (int j = 0; j < s->ny; j++)
             for (int i = 0; i < s->nx; i++) P3(s->jc, i, j, k) = (double)1 / s->We * P3(s->kappa, i, j, k);
-    fill_periodic_x(s, s->jc, s->jc, 0, 1, 1, NULL);
+    fill_periodic(s, s->jc, s->jc, 0, 1, 1, NULL);
     /* DeterminePressure (:794-834) */
     pressure_bc(s);
     fill_const_self(s, s->phi, 2); /* water_levelset.ComputeNormals (LEVELSET_3D.h:931) */
-    fill_periodic_x(s, s->irho, s->irho, 0, 2, 0, NULL);
+    fill_periodic(s, s->irho, s->irho, 0, 2, 0, NULL);
     fill_const_self(s, s->bls, 2); /* boundary_levelset.ComputeNormals */
     pressure_solve(s);
     velocity_update(s, dt);
@@ -1421,7 +1683,7 @@ int fsop_reinitialization(fsop_state* s, double dt) {
             }
     const double eps = s->P.levelset_epsilon;
     for (int it = 0; it < s->P.iteration_number_for_reinitialization; it++) {
-        fill_periodic_x(s, s->sfg, s->phi, 0, 3, 1, NULL);
+        fill_periodic(s, s->sfg, s->phi, 0, 3, 1, NULL);
         for (int k = 0; k < s->nz; k++)
             for (int j = 0; j < s->ny; j++)
                 for (int i = 0; i < s->nx; i++) {
