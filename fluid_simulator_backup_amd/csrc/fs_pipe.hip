@@ -60,10 +60,17 @@ inline dim3 box_grid(FsBox b, int lo = 0) {
     return dim3((b.ni + 2 * lo + PBX - 1) / PBX, (b.nj + PBY - 1) / PBY, b.nk);
 }
 
-// the centreline faces ADVECTION_3D::Solve_Velocity holds fixed, is_parallel
-// (ADVECTION_3D.h:279-307); set from creation: every phase that reads them
-// runs after that call in a step
-__host__ __device__ inline bool fixed_face(int c, int j, int k, int ny, int nz) {
+// the centreline faces ADVECTION_3D::Solve_Velocity holds fixed
+// (ADVECTION_3D.h:248-307: along y for the vertical pipe, along x for the
+// parallel one); set from creation: every phase that reads them runs after
+// that call in a step
+__host__ __device__ inline bool fixed_face(int c, int i, int j, int k, int nx, int ny, int nz, int vert) {
+    if (vert) {
+        if (c == 1) return (i == nx / 2 || i == nx / 2 + 1) && k == (nz - 1) / 2;
+        if (c == 2) return i == (nx - 1) / 2 && k == (nz - 1) / 2;
+        if (c == 3) return i == (nx - 1) / 2 && (k == nz / 2 || k == nz / 2 + 1);
+        return false;
+    }
     if (c == 1) return j == (ny - 1) / 2 && k == (nz - 1) / 2;
     if (c == 2) return (j == ny / 2 || j == ny / 2 + 1) && k == (nz - 1) / 2;
     if (c == 3) return j == (ny - 1) / 2 && (k == nz / 2 || k == nz / 2 + 1);
@@ -94,6 +101,60 @@ __global__ void k_periodic_x(double* dst, const double* src, const double* __res
         if (mask && !(mask[o] <= 0)) return;
         dst[o] = src[o];
     }
+}
+
+// ---- y-periodic ghost copies (the vertical pipe), FIELD_STRUCTURE_3D.h:1328-1710
+// One thread per cell of the column segment j in [-gw, nj-1+gw] of every
+// interior (i, k).  The interior copy (masked: src where the mask is < 1e-8,
+// zero elsewhere), then the y ghosts -- for y faces the two end faces and
+// beyond, for other fields j < 0 and j > je only -- from the node-periodic
+// image, with a mask only where it is < 1e-8 at the written cell.  Reads
+// and writes never meet for ny >= 8, so dst == src works in one pass.
+__global__ void k_periodic_y(double* dst, const double* src, const double* __restrict__ mask, FsLayout L, FsBox b,
+                             int c, int gw, int copy) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y - gw, k = blockIdx.z;
+    if (i >= b.ni || j > b.nj - 1 + gw) return;
+    const int je = b.nj - 1;
+    const long o = L.idx(i, j, k);
+    const bool left = c == 2 ? j <= 0 : j < 0, right = c == 2 ? j >= je : j > je;
+    if (left || right) {
+        if (!mask || mask[o] < (double)1e-8) {
+            int sj = left ? (c == 2 ? je + (j - (0 + 1)) : je + (j - 0)) : (c == 2 ? 0 + (j - (je - 1)) : 0 + (j - je));
+            dst[o] = src[L.idx(i, sj, k)];
+            return;
+        }
+    }
+    if (copy && j >= 0 && j <= je) {
+        if (!mask || mask[o] < (double)1e-8)
+            dst[o] = src[o];
+        else
+            dst[o] = 0;
+    }
+}
+
+// The masked fill's four extrapolation blocks (FIELD_STRUCTURE_3D.h:1568-1670),
+// run on one thread in the order x-right, x-left, z-top, z-bottom: from each
+// cell outside the wall whose neighbour below (above) is inside, every cell
+// to the box's edge, ghosts included, takes 2 src(t -/+ 1) - src(t -/+ 2).
+// The value does not depend on which crossing wrote it, so a cell takes the
+// last block that reaches it: xr[j, k] (xl[j, k]) is the first (last)
+// right (left) crossing of the x row, zt[i, j] (zb[i, j]) the first (last)
+// top (bottom) crossing of the z column, or +-2^30 where there is none.
+__global__ void k_periodic_y_ext(double* __restrict__ dst, const double* __restrict__ src,
+                                 const int* __restrict__ xr, const int* __restrict__ xl, const int* __restrict__ zt,
+                                 const int* __restrict__ zb, FsLayout L, FsBox b, int gw) {
+    int i = blockIdx.x * PBX + threadIdx.x - gw, j = blockIdx.y * PBY + threadIdx.y, k = (int)blockIdx.z - gw;
+    if (i > b.ni - 1 + gw || j >= b.nj) return;
+    const long o = L.idx(i, j, k), sy = L.sy;
+    const bool icol = i >= 0 && i < b.ni, krow = k >= 0 && k < b.nk;
+    if (icol && k <= zb[j * b.ni + i])
+        dst[o] = (double)2 * src[o + sy] - src[o + 2 * sy];
+    else if (icol && zt[j * b.ni + i] <= k)
+        dst[o] = (double)2 * src[o - sy] - src[o - 2 * sy];
+    else if (krow && i <= xl[k * b.nj + j])
+        dst[o] = (double)2 * src[o + 1] - src[o + 2];
+    else if (krow && xr[k * b.nj + j] <= i)
+        dst[o] = (double)2 * src[o - 1] - src[o - 2];
 }
 
 // ---- wall conditions of the velocity ghost copies --------------------------
@@ -127,18 +188,46 @@ __global__ void k_lsbc_y(double* __restrict__ v, double* __restrict__ g, const d
     if (wr) g[o] = val;
 }
 
-// the same for z faces (ADVECTION_3D.h:533-550): w = 0 outside, the copy
-// zeroed or extrapolated from the inside neighbour below or above (inside
-// faces are never written here, so the loop reads no value it wrote)
-__global__ void k_lsbc_z(double* __restrict__ w, double* __restrict__ g, const double* __restrict__ b, FsLayout L,
+// the vertical pipe's x faces (ADVECTION_3D.h:445-466): outside the pipe
+// u = 0 and the copy zeroed (all four x/z neighbours outside) or
+// extrapolated from the inside neighbour left, then right (inside faces are
+// never written here, so the loop reads no value it wrote)
+__global__ void k_lsbc_x(double* __restrict__ u, double* __restrict__ g, const double* __restrict__ b, FsLayout L,
                          FsBox bx) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     if (i >= bx.ni || j >= bx.nj) return;
-    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const long o = L.idx(i, j, k), sy = L.sy;
+    const double b0 = b[o];
+    if (!(b0 > 0)) return;
+    u[o] = 0;
+    double val = 0;
+    bool wr = false;
+    if (b[o - 1] > 0 && b[o + 1] > 0 && b[o - sy] > 0 && b[o + sy] > 0) val = 0, wr = true;
+    if (b[o - 1] < 0) {
+        double th = theta_of(b[o - 1], b0);
+        val = (th - 1) / th * g[o - 1];
+        wr = true;
+    }
+    if (b[o + 1] < 0) {
+        double th = theta_of(b[o + 1], b0);
+        val = (th - 1) / th * g[o + 1];
+        wr = true;
+    }
+    if (wr) g[o] = val;
+}
+
+// the same for z faces (ADVECTION_3D.h:533-550, and :468-489 for the vertical pipe, s1 = 1): w = 0 outside, the copy
+// zeroed or extrapolated from the inside neighbour below or above (inside
+// faces are never written here, so the loop reads no value it wrote)
+__global__ void k_lsbc_z(double* __restrict__ w, double* __restrict__ g, const double* __restrict__ b, FsLayout L,
+                         FsBox bx, long s1) {
+    int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
+    if (i >= bx.ni || j >= bx.nj) return;
+    const long o = L.idx(i, j, k), sy = L.sy;
     const double b0 = b[o];
     if (!(b0 > 0)) return;
     w[o] = 0;
-    if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0) g[o] = 0;
+    if (b[o - s1] > 0 && b[o + s1] > 0 && b[o - sy] > 0 && b[o + sy] > 0) g[o] = 0;
     if (b[o - sy] < 0) {
         double th = theta_of(b[o - sy], b0);
         g[o] = (th - 1) / th * g[o - sy];
@@ -156,10 +245,10 @@ __global__ void k_lsbc_z(double* __restrict__ w, double* __restrict__ g, const d
 // are pow(10 t, t) = 400, 27000, 2560000.  Each written cell belongs to one
 // crossing (a y or z line meets the convex pipe in one interval).
 __global__ void k_velbc_cross(double* __restrict__ ay, double* __restrict__ az, const double* __restrict__ g,
-                              const double* __restrict__ b, FsLayout L, FsBox bx) {
+                              const double* __restrict__ b, FsLayout L, FsBox bx, long sx) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     if (i >= bx.ni || j >= bx.nj) return;
-    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const long o = L.idx(i, j, k), sy = L.sy;
     const double b0 = b[o];
     if (!(b0 < 0)) return;
     const double sent[3] = {400.0, 27000.0, 2560000.0};
@@ -191,11 +280,14 @@ __global__ void k_velbc_cross(double* __restrict__ ay, double* __restrict__ az, 
 // the pipe takes, in the reference's else-if order, zero in both copies
 // (all four neighbours outside), the extrapolation from the first inside
 // neighbour j-1, j+1 (y copy), k-1, k+1 (z copy), or zero in both.
+// With the vertical pipe (:2120-2232) the wall's first axis is x (sx = 1,
+// ay the x copy), and the y faces extrapolate from v itself (g = v), which
+// equals its shared copy on the inside faces read.
 __global__ void k_viscbc(double* __restrict__ ay, double* __restrict__ az, const double* __restrict__ g,
-                         const double* __restrict__ b, FsLayout L, FsBox bx) {
+                         const double* __restrict__ b, FsLayout L, FsBox bx, long sx) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     if (i >= bx.ni || j >= bx.nj) return;
-    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const long o = L.idx(i, j, k), sy = L.sy;
     const double b0 = b[o];
     if (!(b0 > 0)) return;
     if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0) {
@@ -219,14 +311,16 @@ __global__ void k_viscbc(double* __restrict__ ay, double* __restrict__ az, const
     }
 }
 
-// PROJECTION_3D::SetupBoundaryConditionsForVelocityDivergence, is_parallel,
-// y (st = sx) or z (st = sy) faces (PROJECTION_3D.h:1629-1685): zero, the
-// extrapolation from the inside neighbour along the face's axis, or zero.
-// Inside faces are not written, so reading them in place is safe.
-__global__ void k_divbc(double* __restrict__ g, const double* __restrict__ b, FsLayout L, FsBox bx, long st) {
+// PROJECTION_3D::SetupBoundaryConditionsForVelocityDivergence
+// (PROJECTION_3D.h:1566-1685), one face component: zero (all four neighbours
+// across the pipe outside: y and z for the parallel pipe, s1 = sx; x and z
+// for the vertical one, s1 = 1), the extrapolation from the inside
+// neighbour along the face's own axis (stride st), or zero.  Inside faces
+// are not written, so reading them in place is safe.
+__global__ void k_divbc(double* __restrict__ g, const double* __restrict__ b, FsLayout L, FsBox bx, long st, long sx) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     if (i >= bx.ni || j >= bx.nj) return;
-    const long o = L.idx(i, j, k), sx = L.sx, sy = L.sy;
+    const long o = L.idx(i, j, k), sy = L.sy;
     const double b0 = b[o];
     if (!(b0 > 0)) return;
     if (b[o - sx] > 0 && b[o + sx] > 0 && b[o - sy] > 0 && b[o + sy] > 0)
@@ -278,10 +372,11 @@ __global__ void __launch_bounds__(PBX* PBY) k_eno3_pipe(double* __restrict__ rho
                                                         const double* __restrict__ sya, const double* __restrict__ sza,
                                                         const double* __restrict__ gu, const double* __restrict__ gv,
                                                         const double* __restrict__ gw, const double* __restrict__ b,
-                                                        FsLayout L, FsBox bx, int comp, FsGridConst gc, double dt) {
+                                                        FsLayout L, FsBox bx, int comp, FsGridConst gc, double dt,
+                                                        int vert) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     if (i >= bx.ni || j >= bx.nj) return;
-    if (fixed_face(comp, j, k, L.ny, L.nz)) return;
+    if (fixed_face(comp, i, j, k, L.nx, L.ny, L.nz, vert)) return;
     const long c = L.idx(i, j, k);
     if (!(b[c] < (double)0)) return;
     const double inv3x = gc.x.inv * (double)1 / 3, inv3y = gc.y.inv * (double)1 / 3, inv3z = gc.z.inv * (double)1 / 3;
@@ -340,17 +435,24 @@ __device__ __forceinline__ double heav_visc(double phi, double eps) {
 // (:2340-2393): inside the pipe, fixed faces excepted, u += R1/V0^2 (dt g_x
 // + 1/(rho_half rho_oil) dt f) with rho_half = 1 + (eta - 1) H of the face
 // level set (eps 1.5 dx, :2522-2561); v, w += R1/V0^2 dt g
+// The vertical pipe (:2327-2360) drives v alone: v += R1/V0^2 (dt g_y +
+// 1/(rho_oil rho_half) dt f), rho_half at the y faces (eps 1.5 dy).
 struct FsGravPipe {
     double gd[3], drive, eta, ro, R1, V0, eps;
+    int vert;
 };
 __global__ void k_gravity_pipe(double* __restrict__ vel, const double* __restrict__ sfg, const double* __restrict__ b,
                                FsLayout L, FsBox bx, int comp, FsGravPipe G) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     if (i >= bx.ni || j >= bx.nj) return;
-    if (fixed_face(comp, j, k, L.ny, L.nz)) return;
+    if (fixed_face(comp, i, j, k, L.nx, L.ny, L.nz, G.vert)) return;
     const long o = L.idx(i, j, k);
     if (!(b[o] < 0)) return;
-    if (comp == 1) {
+    if (G.vert) {
+        double lc = (double)0.5 * (sfg[o] + sfg[o - L.sx]);
+        double dh = (double)1 + (G.eta - (double)1) * heav_solver(lc, G.eps);
+        vel[o] += G.R1 / (G.V0 * G.V0) * (G.gd[1] + (double)1 / (G.ro * dh) * G.drive);
+    } else if (comp == 1) {
         double lc = (double)0.5 * (sfg[o] + sfg[o - 1]);
         double dh = (double)1 + (G.eta - (double)1) * heav_solver(lc, G.eps);
         vel[o] += G.R1 / (G.V0 * G.V0) * (G.gd[0] + ((double)1 / (dh * G.ro)) * G.drive);
@@ -371,6 +473,7 @@ __global__ void k_gravity_pipe(double* __restrict__ vel, const double* __restric
 // reads).
 struct FsViscPipe {
     double dt, Re, eta, m, eps;
+    int vert;  // the vertical pipe's z term takes 1/dz where the parallel one takes 1/dx (:827 vs :835)
 };
 __global__ void k_visc_coef(double* __restrict__ c0, double* __restrict__ c1, double* __restrict__ c2,
                             double* __restrict__ c3, double* __restrict__ c4, double* __restrict__ c5,
@@ -427,7 +530,7 @@ __global__ void k_visc_coef(double* __restrict__ c0, double* __restrict__ c1, do
     } else {
         ex = self[o] +
              dt / (Re * rho_h) * (ox * (oz * (mu[0] * (A[o + 1] - A[o + 1 - sy])) - oz * (mu[1] * (A[o] - A[o - sy])))) +
-             dt / (Re * rho_h) * (oy * (oz * (mu[2] * (B[o + sx] - B[o + sx - sy])) - ox * (mu[3] * (B[o] - B[o - sy]))));
+             dt / (Re * rho_h) * (oy * (oz * (mu[2] * (B[o + sx] - B[o + sx - sy])) - (V.vert ? oz : ox) * (mu[3] * (B[o] - B[o - sy]))));
     }
     expl[o] = ex;
 }
@@ -442,7 +545,7 @@ __global__ void k_visc_coef(double* __restrict__ c0, double* __restrict__ c1, do
 // component's bc field after row numbering (FULL cells hold their row).
 struct FsViscSys {
     double inv_dxdx, inv_dydy, inv_dzdz;
-    int comp, ie;
+    int comp, ie, je;
 };
 __global__ void k_visc_assemble(const long* __restrict__ rows, int N, const int* __restrict__ bc,
                                 const double* __restrict__ c0, const double* __restrict__ c1,
@@ -473,8 +576,8 @@ __global__ void k_visc_assemble(const long* __restrict__ rows, int N, const int*
             col[n * N + r] = nb[e];
             n++;
         }
-    // x-periodic neighbours (:3721-3746)
-    const int i = (int)((o - L.off) % sx) - FS_G;
+    // periodic neighbours along x, then y (:3721-3770)
+    const int i = (int)((o - L.off) % sx) - FS_G, j = (int)(((o - L.off) / sx) % (sy / sx)) - FS_G;
     if (nb[0] == BC_PER) {
         coef_ijk += cf[1];
         int pi = S.comp == 1 ? S.ie + (i - 1 - (0 + 1)) : S.ie + (i - 1 - 0);
@@ -487,6 +590,20 @@ __global__ void k_visc_assemble(const long* __restrict__ rows, int N, const int*
         int pi = S.comp == 1 ? 0 + (i + 1 - (S.ie - 1)) : 0 + (i + 1 - S.ie);
         val[n * N + r] = -cf[0] * S.inv_dxdx;
         col[n * N + r] = bc[o + (pi - i)];
+        n++;
+    }
+    if (nb[2] == BC_PER) {
+        coef_ijk += cf[3];
+        int pj = S.comp == 2 ? S.je + (j - 1 - (0 + 1)) : S.je + (j - 1 - 0);
+        val[n * N + r] = -cf[3] * S.inv_dydy;
+        col[n * N + r] = bc[o + (long)(pj - j) * sx];
+        n++;
+    }
+    if (nb[3] == BC_PER) {
+        coef_ijk += cf[2];
+        int pj = S.comp == 2 ? 0 + (j + 1 - (S.je - 1)) : 0 + (j + 1 - S.je);
+        val[n * N + r] = -cf[2] * S.inv_dydy;
+        col[n * N + r] = bc[o + (long)(pj - j) * sx];
         n++;
     }
     for (int e = 0; e < 6; e++)
@@ -979,23 +1096,30 @@ __global__ void k_residual(const double* __restrict__ b, const double* __restric
 // VectorToGrid (POISSON_SOLVER_3D.h:4060-4078): FULL cells take x, fixed
 // faces excepted
 __global__ void k_scatter(double* __restrict__ dst, const double* __restrict__ x, const long* __restrict__ rows, int N,
-                          FsLayout L, int comp) {
+                          FsLayout L, int comp, int vert) {
     int r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= N) return;
     const long o = rows[r];
     if (comp) {
         long q = o - L.off;
-        int j = (int)((q / L.sx) % (L.sy / L.sx)) - FS_G, k = (int)(q / L.sy) - FS_G;
-        if (fixed_face(comp, j, k, L.ny, L.nz)) return;
+        int i = (int)(q % L.sx) - FS_G, j = (int)((q / L.sx) % (L.sy / L.sx)) - FS_G, k = (int)(q / L.sy) - FS_G;
+        if (fixed_face(comp, i, j, k, L.nx, L.ny, L.nz, vert)) return;
     }
     dst[o] = x[r];
 }
 
-// u(ie, j, k) = u(1, j, k) after the x solve (POISSON_SOLVER_3D.h:366-376)
-__global__ void k_xfix_last(double* __restrict__ u, FsLayout L, FsBox bx) {
-    int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
-    if (j >= bx.nj) return;
-    u[L.idx(bx.ni - 1, j, k)] = u[L.idx(0 + 1, j, k)];
+// after the solve of the face component normal to the pipe
+// (POISSON_SOLVER_3D.h:354-376): u(ie, j, k) = u(1, j, k) for the parallel
+// pipe, v(i, je, k) = v(i, 1, k) for the vertical one
+__global__ void k_fix_last(double* __restrict__ vel, FsLayout L, FsBox bx, int vert) {
+    int q = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+    if (vert) {
+        if (q >= bx.ni) return;
+        vel[L.idx(q, bx.nj - 1, k)] = vel[L.idx(q, 0 + 1, k)];
+    } else {
+        if (q >= bx.nj) return;
+        vel[L.idx(bx.ni - 1, q, k)] = vel[L.idx(0 + 1, q, k)];
+    }
 }
 
 // ---- projection -----------------------------------------------------------------
@@ -1039,10 +1163,10 @@ struct FsPresUpd {
 __global__ void k_pupdate(double* __restrict__ vel, const double* __restrict__ psfg, const double* __restrict__ phi,
                           const double* __restrict__ bls, const double* __restrict__ jc,
                           const double* __restrict__ rho, const double* __restrict__ p, const int* __restrict__ bc,
-                          FsLayout L, FsBox bx, int comp, FsPresUpd U, unsigned long long* vmax) {
+                          FsLayout L, FsBox bx, int comp, FsPresUpd U, unsigned long long* vmax, int vert) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
     double mine = 0;
-    if (i < bx.ni && j < bx.nj && !fixed_face(comp, j, k, L.ny, L.nz)) {
+    if (i < bx.ni && j < bx.nj && !fixed_face(comp, i, j, k, L.nx, L.ny, L.nz, vert)) {
         const long o = L.idx(i, j, k);
         const long n = o - (comp == 1 ? 1 : comp == 2 ? L.sx : L.sy);
         if (bc[o] >= 0) {
@@ -1055,7 +1179,18 @@ __global__ void k_pupdate(double* __restrict__ vel, const double* __restrict__ p
             const double one_over_half = 1 / dh;
             const double bcv = (double)bc[o], bcn = (double)bc[n];
             const double dp = p[o] - p[n];
-            if (comp == 1) {
+            if (vert && comp != 3) {
+                // the vertical pipe's x and y faces (:1188-1205, 1297-1318);
+                // the y faces skip the first and the last face
+                if (bcv * bcn >= 0 && (comp == 1 || (j != 0 && j <= bx.nj - 1 - 1))) {
+                    if (lv * ln > 0)
+                        vp -= U.dt * dp * one_over_density_ijk * U.od;
+                    else if (lv <= 0)
+                        vp -= U.dt * (dp + gamma) * one_over_half * U.od;
+                    else if (lv > 0)
+                        vp -= U.dt * (dp - gamma) * one_over_half * U.od;
+                }
+            } else if (comp == 1) {
                 if (bls[o] * bls[n] >= 0) {
                     if (bcv * bcn > 0)
                         vp -= U.dt * dp * one_over_density_ijk * U.od;
@@ -1099,10 +1234,23 @@ __global__ void k_pupdate(double* __restrict__ vel, const double* __restrict__ p
     if (t == 0) atomicMax(vmax, (unsigned long long)__double_as_longlong(sh[0]));
 }
 
-// the x-periodic copies of the end faces after the update (PROJECTION_3D.h:
-// 1248-1256, 1355-1363, 1436-1444): x faces u(ie) = u(1), u(0) = u(ie-1);
-// y and z faces v(ie) = v(0)
-__global__ void k_pfix(double* __restrict__ vel, FsLayout L, FsBox bx, int comp) {
+// the periodic copies of the end faces after the update (PROJECTION_3D.h:
+// 1238-1256, 1345-1363, 1429-1444): parallel pipe, x faces u(ie) = u(1),
+// u(0) = u(ie-1), y and z faces v(ie) = v(0); vertical pipe, the same along
+// y with the y faces in the role of the x faces
+__global__ void k_pfix(double* __restrict__ vel, FsLayout L, FsBox bx, int comp, int vert) {
+    if (vert) {
+        int i = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
+        if (i >= bx.ni) return;
+        const int je = bx.nj - 1;
+        if (comp == 2) {
+            double a = vel[L.idx(i, 0 + 1, k)], b = vel[L.idx(i, je - 1, k)];
+            vel[L.idx(i, je, k)] = a;
+            vel[L.idx(i, 0, k)] = b;
+        } else
+            vel[L.idx(i, je, k)] = vel[L.idx(i, 0, k)];
+        return;
+    }
     int j = blockIdx.x * blockDim.x + threadIdx.x, k = blockIdx.y;
     if (j >= bx.nj) return;
     const int ie = bx.ni - 1;
@@ -1150,7 +1298,12 @@ struct FsPipe {
     fs_grid G;
     FsLayout L;
     FsGridConst cell, mac[4];
-    double y0f[4], z0f[4];
+    double x0f[4], y0f[4], z0f[4];
+    int vert = 0;  // 1: the vertical pipe (along y), 0: the parallel pipe (along x)
+    // crossing tables of the masked y fill's extrapolation, per face component:
+    // [0] x rows' first right crossing, [1] last left crossing (nk x nj),
+    // [2] z columns' first top crossing, [3] last bottom crossing (nj x ni)
+    int* xz[4][4] = {};
     double R1, f, A, V0, J, Re, We, A0;
     int device = 0;
     hipStream_t st = nullptr;
@@ -1160,7 +1313,7 @@ struct FsPipe {
     double *sfg = nullptr, *g[4] = {nullptr, nullptr, nullptr, nullptr};
     double* ag[4][2] = {};
     double* vg[4][2] = {};
-    double* zero = nullptr;  // VISCOSITY_3D's y_x and z_x copies, never written in this mode
+    double* zero = nullptr;  // the parallel pipe reads VISCOSITY_3D's y_x and z_x copies, never written there
     double* bp[4] = {};
     double *psfg = nullptr, *rho = nullptr, *irho = nullptr, *div = nullptr, *jc = nullptr, *kappa = nullptr;
     double *sgn = nullptr, *tmp = nullptr, *scr = nullptr, *rk[4] = {};
@@ -1205,6 +1358,7 @@ struct FsPipe {
         for (int s = 0; s < 4; s++) {
             (void)hipFree(sys[s].bc);
             (void)hipFree(sys[s].rows);
+            for (int t = 0; t < 4; t++) (void)hipFree(xz[s][t]);
         }
         for (void* d : {(void*)val, (void*)dg, (void*)rhs, (void*)xv, (void*)rv, (void*)zv, (void*)pv, (void*)pv2,
                         (void*)qv, (void*)part, (void*)lpart, (void*)col, (void*)cnt, (void*)pcg, (void*)bits,
@@ -1223,10 +1377,12 @@ struct FsPipe {
     // ---- set-up --------------------------------------------------------------
     void init(const fs_params& P_, const fs_pipe_params& Q_, const fs_grid& G_, int dev) {
         P = P_, Q = Q_, G = G_;
-        if (!Q.is_parallel || !Q.dimensionless_form || !Q.cylindrical_boundary || Q.boundary_order_velocity != 1)
+        if ((Q.is_parallel != 0 && Q.is_parallel != 1) || !Q.dimensionless_form || !Q.cylindrical_boundary ||
+            Q.boundary_order_velocity != 1)
             throw FsPipeError{FS_ERR_UNSUPPORTED,
-                              "pipe mode: only the parallel pipe with a cylindrical boundary, the dimensionless form "
-                              "and first-order wall extrapolation are implemented"};
+                              "pipe mode: only the parallel or vertical pipe with a cylindrical boundary, the "
+                              "dimensionless form and first-order wall extrapolation are implemented"};
+        vert = !Q.is_parallel;
         if (P.precision != FS_FP64 || P.schedule != FS_SCHEDULE_COMPAT)
             throw FsPipeError{FS_ERR_UNSUPPORTED, "pipe mode: FS_FP64 and FS_SCHEDULE_COMPAT only"};
         if (P.csf_model) throw FsPipeError{FS_ERR_UNSUPPORTED, "pipe mode: CSF_model is not implemented"};
@@ -1248,11 +1404,12 @@ struct FsPipe {
         cell.y = fs_make_axis(G.y_min, G.y_max, G.ny);
         cell.z = fs_make_axis(G.z_min, G.z_max, G.nz);
         const double dx = cell.x.d, dy = cell.y.d, dz = cell.z.d;
-        for (int c = 0; c < 4; c++) mac[c] = cell, y0f[c] = G.y_min, z0f[c] = G.z_min;
+        for (int c = 0; c < 4; c++) mac[c] = cell, x0f[c] = G.x_min, y0f[c] = G.y_min, z0f[c] = G.z_min;
         // MAC grids, EULERIAN_FLUID_SOLVER_3D.h:407,415,423
         mac[1].x = fs_make_axis(G.x_min - (double)0.5 * dx, G.x_max + (double)0.5 * dx, G.nx + 1);
         mac[2].y = fs_make_axis(G.y_min - (double)0.5 * dy, G.y_max + (double)0.5 * dy, G.ny + 1);
         mac[3].z = fs_make_axis(G.z_min - (double)0.5 * dz, G.z_max + (double)0.5 * dz, G.nz + 1);
+        x0f[1] = G.x_min - (double)0.5 * dx;
         y0f[2] = G.y_min - (double)0.5 * dy;
         z0f[3] = G.z_min - (double)0.5 * dz;
         // EULERIAN_FLUID_SOLVER_3D.h:496-520
@@ -1299,56 +1456,61 @@ struct FsPipe {
     void static_geometry() {
         const long n = L.total;
         std::vector<double> h(n);
-        // boundary_phi_<c> on the interior of each face grid (ADVECTION_3D.h:
-        // 412-432 and its copies in the other modules)
+        // boundary_phi_<c> on the interior of each face grid: the distance from
+        // the pipe's axis less a, with r^2 = y^2 + z^2 for the parallel pipe
+        // and x^2 + z^2 for the vertical one (ADVECTION_3D.h:380-432 and its
+        // copies in the other modules)
         std::vector<std::vector<double>> hb(4);
         for (int c = 1; c < 4; c++) {
             std::fill(h.begin(), h.end(), 0.0);
             FsBox b = box(c);
-            const double ddy = mac[c].y.d, ddz = mac[c].z.d;
             for (int k = 0; k < b.nk; k++)
                 for (int j = 0; j < b.nj; j++)
                     for (int i = 0; i < b.ni; i++) {
-                        double y = y0f[c] + j * ddy, z = z0f[c] + k * ddz;
-                        h[L.idx(i, j, k)] = std::sqrt(y * y + z * z) - Q.a;
+                        double w1 = vert ? x0f[c] + i * mac[c].x.d : y0f[c] + j * mac[c].y.d, z = z0f[c] + k * mac[c].z.d;
+                        h[L.idx(i, j, k)] = std::sqrt(w1 * w1 + z * z) - Q.a;
                     }
             PCHECK(hipMemcpy(bp[c], h.data(), sizeof(double) * n, hipMemcpyHostToDevice));
             hb[c] = h;
+            if (vert) crossing_tables(c, h);
         }
         // the pipe inside the box: the first and last two cells and faces of
-        // every y and z line outside the wall
+        // every line across it outside the wall
         for (int c = 0; c < 4; c++) {
             FsBox b = box(c);
-            const double ddy = mac[c].y.d, ddz = mac[c].z.d;
             for (int k = 0; k < b.nk; k++)
-                for (int j = 0; j < b.nj; j++) {
-                    if (!(j < 2 || j >= b.nj - 2 || k < 2 || k >= b.nk - 2)) continue;
-                    double y = y0f[c] + j * ddy, z = z0f[c] + k * ddz;
-                    if (!(std::sqrt(y * y + z * z) - Q.a > 0))
-                        throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: the pipe wall must lie inside the box, two cells "
-                                                          "from every y and z face"};
-                }
+                for (int j = 0; j < b.nj; j++)
+                    for (int i = 0; i < b.ni; i++) {
+                        const int q = vert ? i : j, nq = vert ? b.ni : b.nj;
+                        if (!(q < 2 || q >= nq - 2 || k < 2 || k >= b.nk - 2)) continue;
+                        double w1 = vert ? x0f[c] + i * mac[c].x.d : y0f[c] + j * mac[c].y.d, z = z0f[c] + k * mac[c].z.d;
+                        if (!(std::sqrt(w1 * w1 + z * z) - Q.a > 0))
+                            throw FsPipeError{FS_ERR_BAD_ARG, "pipe mode: the pipe wall must lie inside the box, two "
+                                                              "cells from every face across the pipe"};
+                    }
         }
-        // the pipe's level set, SIMULATION_WORLD.cpp:305-319: sqrt(y^2 + z^2)
-        // - a on the cells, ghosts by linear extrapolation
+        // the pipe's level set, SIMULATION_WORLD.cpp:305-319: r - a on the
+        // cells, ghosts by linear extrapolation
         std::fill(h.begin(), h.end(), 0.0);
         for (int k = 0; k < G.nz; k++)
             for (int j = 0; j < G.ny; j++)
                 for (int i = 0; i < G.nx; i++) {
-                    double y = G.y_min + j * cell.y.d, z = G.z_min + k * cell.z.d;
-                    h[L.idx(i, j, k)] = std::sqrt(y * y + z * z) - Q.a;
+                    double w1 = vert ? G.x_min + i * cell.x.d : G.y_min + j * cell.y.d, z = G.z_min + k * cell.z.d;
+                    h[L.idx(i, j, k)] = std::sqrt(w1 * w1 + z * z) - Q.a;
                 }
         std::vector<double> hbls = linear_ghosts(h);
         PCHECK(hipMemcpy(bls, hbls.data(), sizeof(double) * n, hipMemcpyHostToDevice));
         // pressure bc, PROJECTION_3D::SetupBoundaryConditions, Neumann,
-        // cylindrical, is_parallel (:1017-1065)
+        // cylindrical (:1017-1065)
         std::vector<int> bc(n, 0);
         auto B = [&](int i, int j, int k) { return hbls[L.idx(i, j, k)]; };
         for (int k = -1; k <= G.nz; k++)
             for (int j = -1; j <= G.ny; j++)
                 for (int i = -1; i <= G.nx; i++) {
                     int val;
-                    if (B(i, j, k) > 0)
+                    if (vert)  // is_vertical (:1025-1043): outside and the y ghosts NEUM
+                        val = (B(i, j, k) > 0 || j < 0 || j > G.ny - 1) ? BC_NEUM : BC_FULL;
+                    else if (B(i, j, k) > 0)
                         val = (B(i, j - 1, k) < 0 || B(i, j + 1, k) < 0 || B(i, j, k - 1) < 0 || B(i, j, k + 1) < 0)
                                   ? BC_NEUM : BC_OTHER;
                     else if (i == -1 || i == G.nx)
@@ -1359,7 +1521,7 @@ struct FsPipe {
                 }
         upload_system(0, bc, box(0));
         // viscosity bc, VISCOSITY_3D::SetupBoundaryConditionsForVelocity,
-        // cylindrical, is_parallel (:1482-1539)
+        // cylindrical (:1422-1539): PER beyond the ends along the pipe
         for (int c = 1; c < 4; c++) {
             std::fill(bc.begin(), bc.end(), 0);
             FsBox b = box(c);
@@ -1370,9 +1532,9 @@ struct FsPipe {
                         int val;
                         if (hbp[L.idx(i, j, k)] > 0)
                             val = BC_DIR;
-                        else if (i < 0)
+                        else if (vert ? j < 0 : i < 0)
                             val = BC_PER;
-                        else if (c == 1 ? i >= b.ni - 1 : i > b.ni - 1)
+                        else if (vert ? (c == 2 ? j >= b.nj - 1 : j > b.nj - 1) : (c == 1 ? i >= b.ni - 1 : i > b.ni - 1))
                             val = BC_PER;
                         else
                             val = BC_FULL;
@@ -1389,6 +1551,30 @@ struct FsPipe {
         for (double** d : {&dg, &rhs, &xv, &rv, &zv, &pv, &pv2, &qv}) PCHECK(hipMalloc(d, sizeof(double) * N));
         PCHECK(hipMalloc(&part, sizeof(double) * 3 * (nb + 1)));
         PCHECK(hipMalloc(&lpart, sizeof(double) * 3 * RBLK));  // the loop's workgroup sums: p.q, r.z, r.r
+    }
+
+    // the masked y fill's crossings (k_periodic_y_ext) of face component c
+    // from its boundary_phi hb (zero ghosts, as on the device)
+    void crossing_tables(int c, const std::vector<double>& hb) {
+        FsBox b = box(c);
+        const int big = 1 << 30;
+        std::vector<int> xr((size_t)b.nk * b.nj, big), xl((size_t)b.nk * b.nj, -big), zt((size_t)b.nj * b.ni, big),
+            zb((size_t)b.nj * b.ni, -big);
+        auto M = [&](int i, int j, int k) { return hb[L.idx(i, j, k)]; };
+        for (int k = 0; k < b.nk; k++)
+            for (int j = 0; j < b.nj; j++)
+                for (int i = 0; i < b.ni; i++) {
+                    if (!(M(i, j, k) > 0)) continue;
+                    if (M(i - 1, j, k) < 0) xr[k * b.nj + j] = std::min(xr[k * b.nj + j], i);
+                    if (M(i + 1, j, k) < 0) xl[k * b.nj + j] = std::max(xl[k * b.nj + j], i);
+                    if (M(i, j, k - 1) < 0) zt[j * b.ni + i] = std::min(zt[j * b.ni + i], k);
+                    if (M(i, j, k + 1) < 0) zb[j * b.ni + i] = std::max(zb[j * b.ni + i], k);
+                }
+        const std::vector<int>* tabs[4] = {&xr, &xl, &zt, &zb};
+        for (int t = 0; t < 4; t++) {
+            PCHECK(hipMalloc(&xz[c][t], sizeof(int) * tabs[t]->size()));
+            PCHECK(hipMemcpy(xz[c][t], tabs[t]->data(), sizeof(int) * tabs[t]->size(), hipMemcpyHostToDevice));
+        }
     }
 
     // FillGhostCellsContinuousDerivativesFrom of a cell field onto itself,
@@ -1457,7 +1643,30 @@ struct FsPipe {
         const long n = L.total;
         std::vector<double> hu(n, 0.0), hphi(n, 0.0);
         const double a = Q.a, K = Q.K, m = Q.m;
-        // EULERIAN_FLUID_SOLVER_3D.h:536-563: x faces, ghost layers of width 2
+        // EULERIAN_FLUID_SOLVER_3D.h:566-611: the vertical pipe's y faces,
+        // interior only, with the sign of V0
+        if (vert)
+            for (int k = 0; k <= G.nz - 1; k++)
+                for (int j = 0; j <= G.ny; j++)
+                    for (int i = 0; i <= G.nx - 1; i++) {
+                        double x = G.x_min + i * mac[2].x.d, z = G.z_min + k * mac[2].z.d;
+                        double magnitude = std::sqrt(x * x + z * z), val = 0;
+                        if ((magnitude >= 1) && (magnitude <= a)) {
+                            if (V0 > 0)
+                                val = (double)1 / A * (a * a - (x * x + z * z) - (double)2 * (K - 1) * std::log(std::sqrt(x * x + z * z) / a));
+                            else
+                                val = -(double)1 / A * (a * a - (x * x + z * z) - (double)2 * (K - 1) * std::log(std::sqrt(x * x + z * z) / a));
+                        } else if (magnitude < 1) {
+                            if (V0 > 0)
+                                val = (double)1 - (double)1 / A * (m * (x * x + z * z) * K);
+                            else
+                                val = -((double)1 - (double)1 / A * (m * (x * x + z * z) * K));
+                        }
+                        hu[L.idx(i, j, k)] = val;
+                    }
+        // EULERIAN_FLUID_SOLVER_3D.h:536-563: the parallel pipe's x faces,
+        // ghost layers of width 2
+        if (!vert)
         for (int k = -2; k <= G.nz - 1 + 2; k++)
             for (int j = -2; j <= G.ny - 1 + 2; j++)
                 for (int i = -2; i <= G.nx + 2; i++) {
@@ -1479,11 +1688,13 @@ struct FsPipe {
             for (int j = 0; j < G.ny; j++)
                 for (int i = 0; i < G.nx; i++) {
                     double x = G.x_min + i * cell.x.d, y = G.y_min + j * cell.y.d, z = G.z_min + k * cell.z.d;
-                    hphi[L.idx(i, j, k)] = std::sqrt(y * y + z * z) - (A0 * std::cos(Q.alpha * x) + (double)1);
+                    hphi[L.idx(i, j, k)] = vert ? std::sqrt(x * x + z * z) - (A0 * std::cos(Q.alpha * y) + (double)1)
+                                                : std::sqrt(y * y + z * z) - (A0 * std::cos(Q.alpha * x) + (double)1);
                 }
-        PCHECK(hipMemcpyAsync(u, hu.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
-        PCHECK(hipMemcpyAsync(phi, hphi.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        PCHECK(hipMemsetAsync(u, 0, sizeof(double) * n, st));
         PCHECK(hipMemsetAsync(v, 0, sizeof(double) * n, st));
+        PCHECK(hipMemcpyAsync(vert ? v : u, hu.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
+        PCHECK(hipMemcpyAsync(phi, hphi.data(), sizeof(double) * n, hipMemcpyHostToDevice, st));
         PCHECK(hipMemsetAsync(w, 0, sizeof(double) * n, st));
         PCHECK(hipMemsetAsync(p, 0, sizeof(double) * n, st));
         sync();
@@ -1511,21 +1722,40 @@ struct FsPipe {
     }
 
     // ---- launch helpers ----------------------------------------------------------
+    // the periodic fill along the pipe: x (parallel) or y (vertical, the
+    // masked fill then extrapolates beyond the wall)
     void periodic(double* dst, const double* src, int c, int gw, int copy, const double* mask) {
         FsBox b = box(c);
-        hipLaunchKernelGGL(k_periodic_x, box_grid(b, gw), dim3(PBX, PBY), 0, st, dst, src, mask, L, b, c, gw, copy);
+        if (!vert) {
+            hipLaunchKernelGGL(k_periodic_x, box_grid(b, gw), dim3(PBX, PBY), 0, st, dst, src, mask, L, b, c, gw,
+                               copy);
+            return;
+        }
+        hipLaunchKernelGGL(k_periodic_y, dim3((b.ni + PBX - 1) / PBX, (b.nj + 2 * gw + PBY - 1) / PBY, b.nk),
+                           dim3(PBX, PBY), 0, st, dst, src, mask, L, b, c, gw, copy);
+        if (mask)
+            hipLaunchKernelGGL(k_periodic_y_ext, dim3((b.ni + 2 * gw + PBX - 1) / PBX, (b.nj + PBY - 1) / PBY,
+                                                      b.nk + 2 * gw),
+                               dim3(PBX, PBY), 0, st, dst, src, xz[c][0], xz[c][1], xz[c][2], xz[c][3], L, b, gw);
     }
     void masked_copies(double* const* dsts3) {  // dsts3[1..3] <- u, v, w masked by their boundary_phi
         for (int c = 1; c < 4; c++) periodic(dsts3[c], field(c), c, 3, 1, bp[c]);
     }
 
+    // the stride of the wall's first axis: y (parallel pipe) or x (vertical)
+    long s1() const { return vert ? 1L : (long)L.sx; }
+
     // ---- phases ------------------------------------------------------------------
     // ADVECTION_3D::Solve_Levelset, oil/water (ADVECTION_3D.h:204-242)
     void LevelsetAdvection(double dt) {
         masked_copies(g);
-        PCHECK(hipMemcpyAsync(scr, g[2], sizeof(double) * L.total, hipMemcpyDeviceToDevice, st));
-        hipLaunchKernelGGL(k_lsbc_y, box_grid(box(2)), dim3(PBX, PBY), 0, st, v, g[2], scr, bp[2], L, box(2));
-        hipLaunchKernelGGL(k_lsbc_z, box_grid(box(3)), dim3(PBX, PBY), 0, st, w, g[3], bp[3], L, box(3));
+        if (vert) {
+            hipLaunchKernelGGL(k_lsbc_x, box_grid(box(1)), dim3(PBX, PBY), 0, st, u, g[1], bp[1], L, box(1));
+        } else {
+            PCHECK(hipMemcpyAsync(scr, g[2], sizeof(double) * L.total, hipMemcpyDeviceToDevice, st));
+            hipLaunchKernelGGL(k_lsbc_y, box_grid(box(2)), dim3(PBX, PBY), 0, st, v, g[2], scr, bp[2], L, box(2));
+        }
+        hipLaunchKernelGGL(k_lsbc_z, box_grid(box(3)), dim3(PBX, PBY), 0, st, w, g[3], bp[3], L, box(3), s1());
         periodic(sfg, phi, 0, 3, 1, nullptr);
         hipLaunchKernelGGL(k_weno5_pipe, box_grid(box(0)), dim3(PBX, PBY), 0, st, phi, sfg, g[1], g[2], g[3], bls, L,
                            cell.x.inv, cell.y.inv, cell.z.inv, P.levelset_epsilon, dt);
@@ -1542,11 +1772,14 @@ struct FsPipe {
         for (int oc = 0; oc < 3; oc++) {
             int c = order[oc];
             hipLaunchKernelGGL(k_velbc_cross, box_grid(box(c)), dim3(PBX, PBY), 0, st, ag[c][0], ag[c][1], g[c], bp[c],
-                               L, box(c));
+                               L, box(c), s1());
         }
+        // derivatives along the pipe from the shared copy, across it from the
+        // per-axis copies (ADVECTION_3D.h:316-330)
         for (int c = 1; c < 4; c++)
-            hipLaunchKernelGGL(k_eno3_pipe, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), g[c], ag[c][0],
-                               ag[c][1], g[1], g[2], g[3], bp[c], L, box(c), c, mac[c], dt);
+            hipLaunchKernelGGL(k_eno3_pipe, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), vert ? ag[c][0] : g[c],
+                               vert ? g[c] : ag[c][0], ag[c][1], g[1], g[2], g[3], bp[c], L, box(c), c, mac[c], dt,
+                               vert);
     }
 
     // EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless, is_parallel
@@ -1556,9 +1789,11 @@ struct FsPipe {
         Gp.gd[1] = dt * P.gravity[1];
         Gp.gd[2] = dt * P.gravity[2];
         Gp.drive = dt * f;
-        Gp.eta = Q.eta, Gp.ro = Q.oil_density, Gp.R1 = R1, Gp.V0 = V0, Gp.eps = (double)1.5 * cell.x.d;
+        Gp.eta = Q.eta, Gp.ro = Q.oil_density, Gp.R1 = R1, Gp.V0 = V0;
+        Gp.eps = (double)1.5 * (vert ? cell.y.d : cell.x.d);
+        Gp.vert = vert;
         periodic(sfg, phi, 0, 3, 1, nullptr);
-        for (int c = 1; c < 4; c++)
+        for (int c = vert ? 2 : 1; c <= (vert ? 2 : 3); c++)
             hipLaunchKernelGGL(k_gravity_pipe, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), sfg, bp[c], L, box(c),
                                c, Gp);
     }
@@ -1653,15 +1888,18 @@ struct FsPipe {
         const int order[3] = {1, 3, 2};
         for (int oc = 0; oc < 3; oc++) {
             int c = order[oc];
-            hipLaunchKernelGGL(k_viscbc, box_grid(box(c)), dim3(PBX, PBY), 0, st, vg[c][0], vg[c][1], g[c], bp[c], L,
-                               box(c));
+            hipLaunchKernelGGL(k_viscbc, box_grid(box(c)), dim3(PBX, PBY), 0, st, vg[c][0], vg[c][1],
+                               vert && c == 2 ? v : g[c], bp[c], L, box(c), s1());
         }
-        FsViscPipe Vp{dt, Re, Q.eta, Q.m, (double)1.5 * cell.x.d};
+        FsViscPipe Vp{dt, Re, Q.eta, Q.m, (double)1.5 * cell.x.d, vert};
         for (int c = 1; c < 4; c++) {
-            // cross-component copies each explicit term reads: x reads y_x,
-            // z_x (zero here), y reads x_y and z_y, z reads x_z and y_z
-            const double* A_ = c == 1 ? zero : c == 2 ? vg[1][0] : vg[1][1];
-            const double* B_ = c == 1 ? zero : c == 2 ? vg[3][0] : vg[2][1];
+            // cross-component copies each explicit term reads (VISCOSITY_3D.h:
+            // 615-631, 754-767, 821-834): parallel pipe, x reads y_x and z_x
+            // (never written: zero), y reads x_y and z_y, z reads x_z and y_z;
+            // vertical pipe, x reads y_x and z_x, y reads x_x and z_z, z reads
+            // x_z and y_z (index 0: the wall's first axis, y or x)
+            const double* A_ = c == 1 ? (vert ? vg[2][0] : zero) : c == 2 ? vg[1][0] : vg[1][1];
+            const double* B_ = c == 1 ? (vert ? vg[3][0] : zero) : c == 2 ? (vert ? vg[3][1] : vg[3][0]) : vg[2][1];
             hipLaunchKernelGGL(k_visc_coef, box_grid(box(c)), dim3(PBX, PBY), 0, st, coef[0], coef[1], coef[2], coef[3],
                                coef[4], coef[5], expl, sfg, g[c], A_, B_, L, box(c), c, mac[c], Vp);
             FsViscSys S;
@@ -1670,14 +1908,22 @@ struct FsPipe {
             S.inv_dzdz = (double)1 / (mac[c].z.d * mac[c].z.d);
             S.comp = c;
             S.ie = box(c).ni - 1;
+            S.je = box(c).nj - 1;
             const int N = sys[c].N, nb = (N + RBLK - 1) / RBLK;
+            // the Dirichlet values along x, y, z: the shared copy along the
+            // pipe, the per-axis copies across it (:712-716, 778-782, 844-848)
+            const double* dx_ = vert ? vg[c][0] : g[c];
+            const double* dy_ = vert ? g[c] : vg[c][0];
             hipLaunchKernelGGL(k_visc_assemble, dim3(nb), dim3(RBLK), 0, st, sys[c].rows, N, sys[c].bc, coef[0],
-                               coef[1], coef[2], coef[3], coef[4], coef[5], expl, g[c], vg[c][0], vg[c][1], field(c),
-                               val, col, cnt, dg, rhs, xv, L, S);
+                               coef[1], coef[2], coef[3], coef[4], coef[5], expl, dx_, dy_, vg[c][1], field(c), val,
+                               col, cnt, dg, rhs, xv, L, S);
             visc_iters += Pcg(N);
-            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, field(c), xv, sys[c].rows, N, L, c);
-            if (c == 1)
-                hipLaunchKernelGGL(k_xfix_last, dim3((box(1).nj + 63) / 64, box(1).nk), dim3(64), 0, st, u, L, box(1));
+            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, field(c), xv, sys[c].rows, N, L, c, vert);
+            if (c == (vert ? 2 : 1)) {
+                FsBox b = box(c);
+                hipLaunchKernelGGL(k_fix_last, dim3(((vert ? b.ni : b.nj) + 63) / 64, b.nk), dim3(64), 0, st, field(c), L,
+                                   b, vert);
+            }
         }
     }
 
@@ -1700,8 +1946,16 @@ struct FsPipe {
         hipLaunchKernelGGL(k_pdensity, box_grid(box(0)), dim3(PBX, PBY), 0, st, phi, rho, irho, L, rr);
         // DetermineDivergence (:697-725)
         masked_copies(g);
-        hipLaunchKernelGGL(k_divbc, box_grid(box(3)), dim3(PBX, PBY), 0, st, g[3], bp[3], L, box(3), (long)L.sy);
-        hipLaunchKernelGGL(k_divbc, box_grid(box(2)), dim3(PBX, PBY), 0, st, g[2], bp[2], L, box(2), (long)L.sx);
+        if (vert) {  // x faces, then z (:1566-1627)
+            hipLaunchKernelGGL(k_divbc, box_grid(box(1)), dim3(PBX, PBY), 0, st, g[1], bp[1], L, box(1), 1L, 1L);
+            hipLaunchKernelGGL(k_divbc, box_grid(box(3)), dim3(PBX, PBY), 0, st, g[3], bp[3], L, box(3), (long)L.sy,
+                               1L);
+        } else {  // z faces, then y (:1629-1685)
+            hipLaunchKernelGGL(k_divbc, box_grid(box(3)), dim3(PBX, PBY), 0, st, g[3], bp[3], L, box(3), (long)L.sy,
+                               (long)L.sx);
+            hipLaunchKernelGGL(k_divbc, box_grid(box(2)), dim3(PBX, PBY), 0, st, g[2], bp[2], L, box(2), (long)L.sx,
+                               (long)L.sx);
+        }
         hipLaunchKernelGGL(k_pdiv, box_grid(box(0)), dim3(PBX, PBY), 0, st, g[1], g[2], g[3], bls, div, L, cell, 1 / dt);
         // DetermineJumpConditionField (:746-792): jc = kappa/We, x-periodic
         Curvatures(phi);
@@ -1724,7 +1978,7 @@ struct FsPipe {
         hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, part, nb, dscal);
         hipLaunchKernelGGL(k_remove_mean, dim3(nb), dim3(RBLK), 0, st, rhs, N, dscal);
         last_iters = Pcg(N);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, p, xv, sys[0].rows, N, L, 0);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, p, xv, sys[0].rows, N, L, 0, vert);
         // UpdateVelocity (:1101-1449)
         periodic(psfg, phi, 0, 2, 1, nullptr);
         PCHECK(hipMemsetAsync(bits + 2, 0, 3 * sizeof(unsigned long long), st));
@@ -1735,8 +1989,9 @@ struct FsPipe {
             U.eps = (double)1.5 * (c == 1 ? cell.x.d : c == 2 ? cell.y.d : cell.z.d);
             U.rr = rr;
             hipLaunchKernelGGL(k_pupdate, box_grid(box(c)), dim3(PBX, PBY), 0, st, field(c), psfg, phi, bls, jc, rho, p,
-                               sys[0].bc, L, box(c), c, U, bits + 1 + c);
-            hipLaunchKernelGGL(k_pfix, dim3((box(c).nj + 63) / 64, box(c).nk), dim3(64), 0, st, field(c), L, box(c), c);
+                               sys[0].bc, L, box(c), c, U, bits + 1 + c, vert);
+            hipLaunchKernelGGL(k_pfix, dim3(((vert ? box(c).ni : box(c).nj) + 63) / 64, box(c).nk), dim3(64), 0, st,
+                               field(c), L, box(c), c, vert);
         }
         PCHECK(hipMemcpyAsync(host_bits + 2, bits + 2, 3 * sizeof(unsigned long long), hipMemcpyDeviceToHost, st));
         sync();

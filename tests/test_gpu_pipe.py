@@ -1,5 +1,7 @@
 """Oil/water pipe mode (fs_create_pipe) on the GPU against the CPU checker
-(oracle/fs_oracle_pipe.c), phase by phase and frame by frame.
+(oracle/fs_oracle_pipe.c), phase by phase and frame by frame, for the
+parallel pipe (3D_script.sim) and the vertical one (3D_script_1.sim,
+3D_script_3.sim).
 
 Every kernel keeps the reference's operation order and both sides sum in the
 same fixed order, so the comparisons are bit for bit: fields after each phase,
@@ -32,9 +34,15 @@ def say(*a):
     sys.stdout.flush()
 
 
-def pair(hip_lib, oracle_lib, n=20, **kw):
-    P, Q = _abi.pipe_params(**kw)
-    g = _abi.pipe_grid(n)
+def pair(hip_lib, oracle_lib, n=20, vertical=False, **kw):
+    """The same scene on both libraries: the parallel pipe at n^3, or the
+    vertical pipe of 3D_script_1.sim at (n + 8) x n x (n + 8)."""
+    if vertical:
+        P, Q = _abi.pipe_params("3D_script_1.sim", **kw)
+        g = _abi.vertical_pipe_grid(n + 8, n)
+    else:
+        P, Q = _abi.pipe_params(**kw)
+        g = _abi.pipe_grid(n)
     gpu = _abi.Pipe(hip_lib, "fs_", P, Q, g)
     cpu = _abi.Pipe(oracle_lib, "fsop_", P, Q, g)
     for s in (gpu, cpu):
@@ -49,20 +57,22 @@ def assert_fields_equal(gpu, cpu, what):
         assert np.array_equal(fg[c], fc[c]), f"{what}: {c} differs by {d}"
 
 
-def test_pipe_constants_and_initial_state(hip_lib, oracle_lib):
-    say("[pipe] constants and initial state, 20^3")
-    gpu, cpu = pair(hip_lib, oracle_lib)
+@pytest.mark.parametrize("vertical", [False, True])
+def test_pipe_constants_and_initial_state(hip_lib, oracle_lib, vertical):
+    say(f"[pipe] constants and initial state, vertical={vertical}")
+    gpu, cpu = pair(hip_lib, oracle_lib, vertical=vertical)
     assert gpu.constants() == cpu.constants()
     assert_fields_equal(gpu, cpu, "initial")
     assert np.array_equal(gpu.array("boundary"), cpu.array("boundary"))
 
 
+@pytest.mark.parametrize("vertical", [False, True])
 @pytest.mark.parametrize("warm", [0, 1])
-def test_pipe_phases_bit_exact(hip_lib, oracle_lib, warm):
+def test_pipe_phases_bit_exact(hip_lib, oracle_lib, warm, vertical):
     """Each phase alone from the same state: from the initial state and after
-    one full RK2 frame (v, w non-zero, the ghost copies holding the values
-    the phases left)."""
-    gpu, cpu = pair(hip_lib, oracle_lib)
+    one full RK2 frame (the cross velocities non-zero, the ghost copies
+    holding the values the phases left)."""
+    gpu, cpu = pair(hip_lib, oracle_lib, vertical=vertical)
     dt = cpu.cfl_dt(0.9)
     assert gpu.cfl_dt(0.9) == dt
     for _ in range(warm):
@@ -71,7 +81,7 @@ def test_pipe_phases_bit_exact(hip_lib, oracle_lib, warm):
         assert sg["pcg_iterations"] == sc["pcg_iterations"]
         assert_fields_equal(gpu, cpu, "warm frame")
     for name in PHASES:
-        say(f"[pipe] phase {name} warm={warm}")
+        say(f"[pipe] phase {name} warm={warm} vertical={vertical}")
         sg, sc = gpu.phase(name, dt), cpu.phase(name, dt)
         assert_fields_equal(gpu, cpu, name)
         for arr in PHASE_ARRAYS[name]:
@@ -82,14 +92,17 @@ def test_pipe_phases_bit_exact(hip_lib, oracle_lib, warm):
             assert sg["pcg_iterations"] == sc["pcg_iterations"]
 
 
+@pytest.mark.parametrize("vertical", [False, True])
 @pytest.mark.parametrize("order", [1, 2, 3])
-def test_pipe_frames_bit_exact(hip_lib, oracle_lib, order):
-    """Three frames with the CFL step before each (CFL 0.9, the script's)."""
-    gpu, cpu = pair(hip_lib, oracle_lib, n=24, order_for_time_advancing=order)
+def test_pipe_frames_bit_exact(hip_lib, oracle_lib, order, vertical):
+    """Three frames with the CFL step before each (the scripts' CFL: 0.9
+    parallel, 0.8 vertical)."""
+    gpu, cpu = pair(hip_lib, oracle_lib, n=24, vertical=vertical, order_for_time_advancing=order)
+    cfl = 0.8 if vertical else 0.9
     for f in range(3):
-        dt = cpu.cfl_dt(0.9)
-        assert gpu.cfl_dt(0.9) == dt
-        say(f"[pipe] order {order} frame {f} dt={dt}")
+        dt = cpu.cfl_dt(cfl)
+        assert gpu.cfl_dt(cfl) == dt
+        say(f"[pipe] order {order} vertical={vertical} frame {f} dt={dt}")
         sg, sc = gpu.advance_frame(dt), cpu.advance_frame(dt)
         assert sg["pcg_iterations"] == sc["pcg_iterations"]
         assert sg["viscosity_iterations"] == sc["viscosity_iterations"]
@@ -100,7 +113,7 @@ def test_pipe_frames_bit_exact(hip_lib, oracle_lib, order):
 def test_pipe_refusals(hip_lib):
     say("[pipe] refusals")
     g = _abi.pipe_grid(20)
-    for kw, code in (({"is_parallel": 0}, _abi.FS_ERR_UNSUPPORTED),
+    for kw, code in (({"is_parallel": 2}, _abi.FS_ERR_UNSUPPORTED),
                      ({"dimensionless_form": 0}, _abi.FS_ERR_UNSUPPORTED),
                      ({"boundary_order_velocity": 2}, _abi.FS_ERR_UNSUPPORTED),
                      ({"csf_model": 1}, _abi.FS_ERR_UNSUPPORTED),
@@ -130,3 +143,26 @@ def test_pipe_shipped_scene_at_its_size(hip_lib, oracle_lib):
         assert sg["pcg_iterations"] == sc["pcg_iterations"]
         assert sg["residual_precond"] < 1e-6
         assert_fields_equal(gpu, cpu, f"51^3 frame {f}")
+
+
+@pytest.mark.parametrize("script", ["3D_script_1.sim", "3D_script_3.sim"])
+def test_vertical_pipe_shipped_scenes_at_their_size(hip_lib, oracle_lib, script):
+    """The vertical scripts' scenes at their own 71 x 51 x 71 grids, four
+    frames with their CFL 0.8, bit for bit every frame, every projection
+    converged to the scripts' 1e-6 (3D_script_3.sim's V0 < 0: the profile
+    runs the other way)."""
+    half, length = (1.8326, 2.618) if script == "3D_script_1.sim" else (2.2270, 3.1814)
+    P, Q = _abi.pipe_params(script)
+    g = _abi.vertical_pipe_grid(71, 51, half, length)
+    gpu = _abi.Pipe(hip_lib, "fs_", P, Q, g)
+    cpu = _abi.Pipe(oracle_lib, "fsop_", P, Q, g)
+    for s in (gpu, cpu):
+        s.initial_fields()
+    for f in range(4):
+        dt = cpu.cfl_dt(0.8)
+        assert gpu.cfl_dt(0.8) == dt
+        say(f"[pipe] {script} 71x51x71 frame {f} dt={dt}")
+        sg, sc = gpu.advance_frame(dt), cpu.advance_frame(dt)
+        assert sg["pcg_iterations"] == sc["pcg_iterations"]
+        assert sg["residual_precond"] < 1e-6
+        assert_fields_equal(gpu, cpu, f"{script} frame {f}")
