@@ -30,8 +30,9 @@ Other workloads (each its own JSON line): `--workload batch` (48 uneven
 scenes advanced together against one at a time), `--workload adjoint` (the
 checkpointed rollout gradient against central differences), `--workload
 projection` (one projection of the 64-bubble pool, density ratio 1000, at
-512^3 per path), `--workload pipe` (the reference script's oil/water pipe
-scene, 51^3 by default).
+512^3 per path), `--workload pipe` (the reference scripts' oil/water pipe
+scenes: 3D_script.sim's parallel pipe at 51^3 by default, `--pipe-script
+3D_script_1.sim` or `3D_script_3.sim` the vertical pipe at 71 x 51 x 71).
 """
 import argparse
 import json
@@ -81,6 +82,9 @@ def parse():
     ap.add_argument("--density-ratio", type=float, default=1000.0,
                     help="adjoint workload: water/air density ratio (water 1000 kg/m^3)")
     ap.add_argument("--scenes", type=int, default=48)
+    ap.add_argument("--pipe-script", default="3D_script.sim", choices=["3D_script.sim", "3D_script_1.sim", "3D_script_3.sim"],
+                    help="pipe workload: the reference script whose pipe runs (3D_script.sim the parallel pipe at "
+                         "n^3, default 51; 3D_script_1.sim and 3D_script_3.sim the vertical pipe at 71 x 51 x 71)")
     ap.add_argument("--threads", type=int, default=0)
     return ap.parse_args()
 
@@ -741,10 +745,12 @@ def jacobi_roofline(R, head, scene, n):
 
 
 def pipe_main(a):
-    """The oil/water pipe scene of the reference's script (core-annular flow,
-    3D_script.sim:32-99, 195-290) at n^3 (default its 51^3): CFL 0.9 before
-    every frame as SIMULATION_WORLD does, RK2, the semi-implicit viscosity and
-    the projection solved to the script's tolerance.  CPU baseline: the CPU
+    """The oil/water pipe scene of one of the reference's scripts (core-annular
+    flow): 3D_script.sim's parallel pipe at n^3 (default its 51^3, CFL 0.9),
+    or the vertical pipe of 3D_script_1.sim / 3D_script_3.sim at their
+    71 x 51 x 71 (CFL 0.8); the CFL step before every frame as
+    SIMULATION_WORLD does, RK2, the semi-implicit viscosity and the
+    projection solved to the script's tolerance.  CPU baseline: the CPU
     checker's frames of the same scene from the same state."""
     import torch
 
@@ -752,22 +758,31 @@ def pipe_main(a):
     from fluid_simulator_backup_amd import _abi
     torch.cuda.set_device(0)
     lib = fsa.load_library()
-    n = a.n or 51
-    P, Q = _abi.pipe_params()
-    g = _abi.pipe_grid(n)
+    P, Q = _abi.pipe_params(a.pipe_script)
+    vertical = Q.is_parallel == 0
+    if vertical:
+        half, length = (1.8326, 2.618) if a.pipe_script == "3D_script_1.sim" else (2.2270, 3.1814)
+        nxz = a.n or 71
+        g = _abi.vertical_pipe_grid(nxz, (nxz * 51) // 71, half, length)
+        cfl, shape = 0.8, f"{g.nx}x{g.ny}x{g.nz}"
+    else:
+        n = a.n or 51
+        g = _abi.pipe_grid(n)
+        cfl, shape = 0.9, f"{n}^3"
+    cells = g.nx * g.ny * g.nz
     s = _abi.Pipe(lib, "fs_", P, Q, g)
     s.initial_fields()
-    print(f"[bench] pipe {n}^3 warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
+    print(f"[bench] pipe {a.pipe_script} {shape} warmup={a.warmup} steps={a.steps}", file=sys.stderr, flush=True)
     stats = []
     for _ in range(a.warmup):
-        s.advance_frame(s.cfl_dt(0.9))
+        s.advance_frame(s.cfl_dt(cfl))
     # one launch of each PCG loop kernel per launch batch timed with HIP
     # events on the solver's stream, inside the timed frames
     s.set_sampling(True)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for f in range(a.steps):
-        stats.append(s.advance_frame(s.cfl_dt(0.9)))
+        stats.append(s.advance_frame(s.cfl_dt(cfl)))
         if (f + 1) % 10 == 0:
             print(f"[bench] pipe frame {f + 1}/{a.steps}", file=sys.stderr, flush=True)
     torch.cuda.synchronize()
@@ -783,14 +798,15 @@ def pipe_main(a):
         kernels[name] = {"avg_us": round(kms * 1e3, 3), "algorithmic_bytes": round(kbytes), "GB/s": gbs and round(gbs, 1),
                          "samples": nsamp, "launches": launches}
     dom = kernels["k_pcg_spmv_dir"]
-    cells = n ** 3
     out = {
         "metric": "ms/frame, oil/water pipe (core-annular flow), the reference script's scene",
         "value": round(ms, 3), "unit": "ms/step", "n_gpus": 1, "steps": a.steps, "warmup": a.warmup,
         "ms_per_step": round(ms, 3), "higher_is_better": False, "scaling": "strong", "vs_baseline": None,
         "dtype": "f64", "data": "synthetic (the script's analytic initial state)",
-        "config": {"workload": f"oil/water pipe {n}^3, parallel pipe a = 1.28, dimensionless, RK2, CFL 0.9, "
-                               f"projection and semi-implicit viscosity to the script's tolerance 1e-6",
+        "config": {"workload": f"oil/water pipe of {a.pipe_script}, {shape}, "
+                               f"{'vertical' if vertical else 'parallel'} pipe a = {Q.a}, dimensionless, "
+                               f"RK{P.order_for_time_advancing}, CFL {cfl}, projection and semi-implicit viscosity to "
+                               f"the script's tolerance 1e-6",
                    "cells": cells},
         "pcg_iterations_per_step": float(np.mean([st["pcg_iterations"] for st in stats])),
         "viscosity_iterations_per_step": float(np.mean([st["viscosity_iterations"] for st in stats])),
@@ -808,11 +824,11 @@ def pipe_main(a):
         c = _abi.Pipe(orc, "fsop_", P, Q, g)
         c.initial_fields()
         for _ in range(a.warmup):
-            c.advance_frame(c.cfl_dt(0.9))
-        frames = max(1, min(a.steps, int(8 * (51 / n) ** 3) or 1))
+            c.advance_frame(c.cfl_dt(cfl))
+        frames = max(1, min(a.steps, int(8 * 51 ** 3 / cells) or 1))
         t0 = time.perf_counter()
         for _ in range(frames):
-            c.advance_frame(c.cfl_dt(0.9))
+            c.advance_frame(c.cfl_dt(cfl))
         tc = time.perf_counter() - t0
         c.close()
         out["cpu_baseline"] = {"value": round(tc * 1e3 / frames, 1), "unit": "ms/step", "cores": 1, "kind": "port",

@@ -94,12 +94,16 @@ WorldSetup parse_oil_water(const fs::Script& sc, const fs::ScriptBlock& world, c
     const fs::ScriptBlock& wd = world.Find("WORLD_DISCRETIZATION_OIL_WATER");
     const fs::ScriptBlock& po = wd.Find("PIPE_OPTIONS");
     const bool vertical = po.GetBoolean("is_vertical", false), parallel = po.GetBoolean("is_parallel", false);
-    if (vertical) unsupported("the vertical pipe (PIPE_OPTIONS is_vertical)");
-    if (!parallel) unsupported("PIPE_OPTIONS sets neither is_vertical nor is_parallel");
-    Q.is_parallel = 1;
+    if (vertical && parallel)
+        unsupported("PIPE_OPTIONS sets both is_vertical and is_parallel (the reference would run both pipes' "
+                    "boundary conditions on one grid)");
+    if (!vertical && !parallel) unsupported("PIPE_OPTIONS sets neither is_vertical nor is_parallel");
+    Q.is_parallel = parallel ? 1 : 0;
     const fs::ScriptBlock& fb = sc.Find("FLUID_SOLVER_UNIFORM_OIL_WATER");
     double wdx = 0;
-    w.G = grid_from_block(wd.Find("GRID_STRUCTURE_3D_PARALLEL"), fb.GetFloat("resolution_scale", 1.0), &wdx);
+    // WORLD_DISCRETIZATION_3D.h:96-109: each pipe reads its own grid block
+    w.G = grid_from_block(wd.Find(vertical ? "GRID_STRUCTURE_3D_VERTICAL" : "GRID_STRUCTURE_3D_PARALLEL"),
+                          fb.GetFloat("resolution_scale", 1.0), &wdx);
     if (w.G.nx < 8 || w.G.ny < 8 || w.G.nz < 8) throw WorldFail{FS_ERR_BAD_ARG, "pipe grid below 8 cells per axis"};
     fs_params& P = w.P;
     if (!fb.GetBoolean("use_water_solver", true)) unsupported("use_water_solver = false");

@@ -125,15 +125,18 @@ def test_world_fp32_refine_and_fixed_schedule(hip_lib, precision):
     world.close()
 
 
-def test_pipe_world_matches_checker(hip_lib, oracle_lib):
-    """An oil_water_simulation script (the pipe) through fs_world_*: the
-    world's pipe handle against the CPU checker's pipe driven through the same
-    loop (the CFL step with the curvature of the current level set, then
-    ComputeCurvatures and the step), bit for bit over three frames."""
-    path = os.path.join(SCRIPTS, "oil_water_pipe.sim")
+@pytest.mark.parametrize("script,parallel,cfl", [("oil_water_pipe.sim", 1, 0.9),
+                                                 ("oil_water_pipe_vertical.sim", 0, 0.8)])
+def test_pipe_world_matches_checker(hip_lib, oracle_lib, script, parallel, cfl):
+    """An oil_water_simulation script (the parallel or the vertical pipe)
+    through fs_world_*: the world's pipe handle against the CPU checker's pipe
+    driven through the same loop (the CFL step with the curvature of the
+    current level set, then ComputeCurvatures and the step), bit for bit over
+    three frames."""
+    path = os.path.join(SCRIPTS, script)
     world = _abi.World(hip_lib, path, 0)
     info = world.info()
-    assert info.oil_water == 1 and info.cfl == 0.9
+    assert info.oil_water == 1 and info.cfl == cfl and info.pipe.is_parallel == parallel
     gpu = _abi.Pipe(hip_lib, "fs_", info.params, info.pipe, info.grid, handle=hip_lib.fs_world_solver(world.h))
     cpu = _abi.Pipe(oracle_lib, "fsop_", info.params, info.pipe, info.grid)
     cpu.initial_fields()

@@ -161,3 +161,23 @@ def test_reference_script_parses(host_lib):
     assert tuple(p.gravity) == (0.0, -980.0, 0.0) and p.order_for_time_advancing == 2
     assert (p.scaling_number_for_reinitialization, p.iteration_number_for_reinitialization) == (2, 10)
     assert p.tolerance == 1e-6 and info.cfl == 0.9
+
+
+@pytest.mark.skipif(not os.path.exists(REFERENCE_SCRIPT), reason="reference scripts not present")
+@pytest.mark.parametrize("name,grid,keys,order", [
+    ("3D_script_1.sim", (71, 51, 71, -1.8326, 1.8326, 0.0, 2.618), (1.61, 0.1, 2.4, -2.03030303), 2),
+    ("3D_script_2.sim", (51, 51, 51, -1.5707645, 1.5707645, 0.0, 3.141529), (1.28, 0.001, 2.0, -0.4552), 1),
+    ("3D_script_3.sim", (71, 51, 71, -2.2270, 2.2270, 0.0, 3.1814), (1.7, 0.01, 1.975, -0.542709), 2)])
+def test_reference_vertical_scripts_parse(host_lib, name, grid, keys, order):
+    """The reference's three vertical pipe scripts, read where they lie:
+    PIPE_OPTIONS is_vertical selects GRID_STRUCTURE_3D_VERTICAL
+    (WORLD_DISCRETIZATION_3D.h:96-101) and the pipe handle's keys come out as
+    written (3D_script_2.sim sets use_rk_3rd instead of
+    order_for_time_advancing, so its order is the default 1)."""
+    info, _ = _abi.parse_world(host_lib, os.path.join(os.path.dirname(REFERENCE_SCRIPT), name))
+    assert info.oil_water == 1 and info.pipe.is_parallel == 0
+    g, q, p = info.grid, info.pipe, info.params
+    assert (g.nx, g.ny, g.nz, g.x_min, g.x_max, g.y_min, g.y_max) == grid
+    assert (g.z_min, g.z_max) == (grid[3], grid[4])
+    assert (q.a, q.A_0, q.alpha, q.K) == keys
+    assert p.order_for_time_advancing == order and info.cfl == 0.8 and p.tolerance == 1e-6
