@@ -468,13 +468,16 @@ int  fs_batch_advance_frame(fs_batch* b, const double* dt, fs_step_stats* stats,
 const char* fs_batch_last_error(const fs_batch* b);
 
 /* ---- oil/water pipe mode ------------------------------------------------------
- * EULERIAN_FLUID_SOLVER_3D in oil_water_simulation mode with the pipe along x
- * (PIPE_OPTIONS is_parallel, WORLD_DISCRETIZATION_3D.h:91-110): core-annular
- * flow of oil (phi <= 0) inside water in a cylinder of radius a (R2/R1, the
- * pipe wall at sqrt(y^2 + z^2) = a in units of the core radius R1), periodic
- * along x with the node-periodic ghost copies of
- * FIELD_STRUCTURE_3D::FillGhostCellsPeriodicInXDirection (FIELD_STRUCTURE_3D.h:
- * 1088-1327), the wall by theta-extrapolated ghost velocities
+ * EULERIAN_FLUID_SOLVER_3D in oil_water_simulation mode (PIPE_OPTIONS,
+ * WORLD_DISCRETIZATION_3D.h:91-110): core-annular flow of oil (phi <= 0)
+ * inside water in a cylinder of radius a (R2/R1 in units of the core radius
+ * R1), the pipe along x (is_parallel: wall at sqrt(y^2 + z^2) = a, periodic
+ * in x with FillGhostCellsPeriodicInXDirection, FIELD_STRUCTURE_3D.h:
+ * 1088-1327) or along y (is_vertical: wall at sqrt(x^2 + z^2) = a, periodic
+ * in y with FillGhostCellsPeriodicInYDirection, :1328-1710, whose masked form
+ * zeroes the copy outside the wall and extrapolates the velocity linearly
+ * beyond it), the node-periodic ghost copies, the wall by theta-extrapolated
+ * ghost velocities
  * (ADVECTION_3D.h:380-993, VISCOSITY_3D.h:2048-2625, PROJECTION_3D.h:
  * 1521-1690), the centreline faces held fixed (ADVECTION_3D.h:279-307), the
  * dimensionless driving pressure and gravity (EULERIAN_FLUID_SOLVER_3D.h:
@@ -483,14 +486,15 @@ const char* fs_batch_last_error(const fs_batch* b);
  * masked GFM projection with the curvature jump 1/We kappa (PROJECTION_3D.h:
  * 414-432, 642-848, 901-1097, 1101-1449; POISSON_SOLVER_3D.h:1360-1815).
  *
- * Scope: the shipped script's configuration -- parallel pipe, cylindrical
- * boundary, Neumann pressure, dimensionless form, semi-implicit viscosity,
- * ENO3 velocity advection, WENO5 level set, boundary_order_velocity 1, the
- * "old_fashioned" pressure assembly.  The vertical pipe, regular_boundary,
- * the dimensional form, explicit viscosity, the CSF model, purvis_style and
- * second-order wall extrapolation return FS_ERR_UNSUPPORTED.  The pipe must
- * lie strictly inside the box: the first and last two cells and faces of
- * every y and z line outside the wall (FS_ERR_BAD_ARG otherwise), nx >= 8.
+ * Scope: the shipped scripts' configuration (3D_script.sim parallel,
+ * 3D_script_1/2/3.sim vertical) -- cylindrical boundary, Neumann pressure,
+ * dimensionless form, semi-implicit viscosity, ENO3 velocity advection,
+ * WENO5 level set, boundary_order_velocity 1, the "old_fashioned" pressure
+ * assembly.  regular_boundary, the dimensional form, explicit viscosity, the
+ * CSF model, purvis_style and second-order wall extrapolation return
+ * FS_ERR_UNSUPPORTED.  The pipe must lie strictly inside the box: the first
+ * and last two cells and faces of every line across the pipe outside the
+ * wall (FS_ERR_BAD_ARG otherwise); every extent >= 8.
  *
  * fs_params keys used: water_density, water_viscosity, gravity,
  * surface_tension, order_for_time_advancing, the reinitialization keys,
@@ -499,7 +503,7 @@ const char* fs_batch_last_error(const fs_batch* b);
  * FS_FP64 and schedule FS_SCHEDULE_COMPAT (the reference's order and quirks).
  */
 typedef struct fs_pipe_params {
-    int    is_parallel;          /* 1: pipe along x (0, the vertical pipe, is refused) */
+    int    is_parallel;          /* 1: pipe along x (is_parallel); 0: pipe along y (is_vertical) */
     double a;                    /* R2/R1 (SIMULATION_WORLD.h:124) */
     double R2;                   /* pipe radius (:125) */
     double A_0;                  /* interface amplitude as the script gives it; scaled by a/R2 (:126-127) */
@@ -551,7 +555,8 @@ int  fs_pipe_phase(fs_solver* s, int phase, double dt, fs_step_stats* stats);
 #define FS_PIPE_ARR_GHOST_U     2   /* vector_field_mac_ghost_x, _y, _z */
 #define FS_PIPE_ARR_GHOST_V     3
 #define FS_PIPE_ARR_GHOST_W     4
-#define FS_PIPE_ARR_ADV_U_Y     5   /* ADVECTION_3D's velocity_field_mac_ghost_x_y, x_z, y_y, y_z, z_y, z_z */
+#define FS_PIPE_ARR_ADV_U_Y     5   /* ADVECTION_3D's velocity_field_mac_ghost_x_y, x_z, y_y, y_z, z_y, z_z
+                                        (the vertical pipe: x_x, x_z, y_x, y_z, z_x, z_z) */
 #define FS_PIPE_ARR_ADV_U_Z     6
 #define FS_PIPE_ARR_ADV_V_Y     7
 #define FS_PIPE_ARR_ADV_V_Z     8
