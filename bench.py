@@ -718,7 +718,7 @@ def frame_roofline(R, scene, n, steps, fast=False):
             "bytes_per_launch": bytes_launch, "moved_bytes_per_launch": mov * cells,
             "moved_frac": round(mov * cells / (avg * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "avg_launch_ms": round(avg, 5),
-            "timing": f"HIP events on the solver stream around one launch per call (a different sweep of the call each time) of the timed frames "
+            "timing": f"HIP events on the solver stream around one launch per call (the call's first sweep) of the timed frames "
                       f"({n_s} samples of {launches} launches)",
             "frame_kernels": {k: rates(k) for k in FRAME_KERNELS}}
 
