@@ -709,6 +709,7 @@ __device__ __forceinline__ double block_tree(double v, double* sh) {
 struct FsPipePcg {
     double rz, rz_old, pq, alpha, beta, rr, bb, sum;
     double rzs[2];  // r.z of the last two iterations, by parity (the loop kernels' own copy)
+    int stop[2];    // the loop's stop decision of iteration k in stop[k & 1] (0 go on, else done's value)
     int iter, done, max_iter, rel;
     double tol2, rtol;
 };
@@ -801,6 +802,8 @@ __global__ void __launch_bounds__(RBLK) k_pcg_begin_scalars(const double* __rest
     double bnorm = sqrt(b);
     if (S->rel && (bnorm == 0 || sqrt(c) <= S->rtol * bnorm)) S->done = 1;
     if (S->max_iter <= 0) S->done = S->done ? S->done : 2;
+    S->stop[0] = S->done;  // read by iteration 1's first launch
+    S->stop[1] = 0;
 }
 // ---- the PCG loop in two launches per iteration ------------------------------
 //
@@ -951,19 +954,28 @@ __global__ void __launch_bounds__(PWG) k_pcg_spmv_dir(const double* __restrict__
                                                       double* __restrict__ lp, FsPipePcg* S, int k) {
     __shared__ double sh[NS * PWG];
     __shared__ double res[NS * PSUB];
-    if (S->done) return;
+    // an earlier stop is carried forward; this launch reads slot k - 1 and
+    // writes slot k only, so no flag is read and written in one launch
+    const int prev = S->stop[(k - 1) & 1];
+    if (prev) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->stop[k & 1] = prev;
+        return;
+    }
     const int sub = threadIdx.x / RBLK, l = threadIdx.x % RBLK;
     PcgRow<FIRST> row1;
     const int b1 = blockIdx.x + RBLK * sub, r1 = b1 * RBLK + l;
     const bool in1 = b1 < nb && r1 < N;
     if (NS == 1) row1.load(val, col, cnt, N, po, z, in1 ? r1 : 0);
     double beta = 0;
-    if (!FIRST) {
+    if (FIRST) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->stop[k & 1] = 0;
+    } else {
         const double rz_old = S->rzs[(k - 1) & 1];
         double t, rr;
         const int dec = pcg_tail(lp + RBLK, S, k - 1, rz_old, sh, res, t, rr, beta);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             pcg_record(S, dec, k - 1, rz_old, t, rr, beta);
+            S->stop[k & 1] = dec;
             if (dec == 0) S->rzs[k & 1] = t;
         }
         // stopped: the reference's last direction is never read
@@ -1010,7 +1022,7 @@ __global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, dou
                                                      int N, int nb, double* __restrict__ lp, FsPipePcg* S, int k) {
     __shared__ double sh[2 * NS * PWG];
     __shared__ double res[NS * PSUB];
-    if (S->done) return;
+    if (S->stop[k & 1]) return;
     const int rel = S->rel;
     const int sub = threadIdx.x / RBLK, l = threadIdx.x % RBLK;
     // one-step passes: the row's loads go out before the tree of p.q
@@ -1080,11 +1092,14 @@ __global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, dou
 __global__ void __launch_bounds__(RBLK) k_pcg_tail_end(const double* __restrict__ lp, FsPipePcg* S, int m) {
     __shared__ double sh[2 * RBLK];
     __shared__ double res[2];
-    if (S->done || m < 1) return;
+    if (m < 1 || S->stop[m & 1]) return;
     const double rz_old = S->rzs[m & 1];
     double t, rr, beta = 0;
     const int dec = pcg_tail(lp + RBLK, S, m, rz_old, sh, res, t, rr, beta);
-    if (threadIdx.x == 0) pcg_record(S, dec, m, rz_old, t, rr, beta);
+    if (threadIdx.x == 0) {
+        pcg_record(S, dec, m, rz_old, t, rr, beta);
+        if (dec) S->stop[m & 1] = dec;  // the next batch's first launch reads it
+    }
 }
 
 // r = b - q (the true residual after the solve)
