@@ -1,8 +1,10 @@
 // fs_pipe.hip -- the oil/water pipe mode on gfx950: core-annular flow in a
-// cylinder along x (EULERIAN_FLUID_SOLVER_3D in oil_water_simulation mode,
-// is_parallel, cylindrical_boundary, dimensionless_form,
-// semi_implicit_approach; the configuration of the reference's shipped
-// script, 3D_script.sim:32-99, 195-290).
+// cylinder along x (is_parallel) or y (is_vertical) -- EULERIAN_FLUID_SOLVER_3D
+// in oil_water_simulation mode, cylindrical_boundary, dimensionless_form,
+// semi_implicit_approach: the configuration of the reference's shipped
+// scripts (3D_script.sim:32-99, 195-290 parallel; 3D_script_1/2/3.sim
+// vertical).  The vertical pipe is the parallel one's axis mirror except
+// where the reference's vertical branches differ, which each kernel notes.
 //
 // The reference's modules keep persistent ghosted arrays: the solver's
 // scalar_field_ghost and vector_field_mac_ghost_x/y/z are written by
@@ -13,7 +15,7 @@
 // refresh points, in the padded layout of fs_layout.h (ghost width 3), zeroed
 // at creation as ARRAY_3D::Initialize(..., true) zeroes them.  The arrays the
 // reference computes only on their interior (boundary_phi_<c>, the pipe's
-// level set sqrt(y^2 + z^2) - a at the faces) are static and computed once;
+// level set r - a at the faces) are static and computed once;
 // their ghost cells read 0, as the reference's do.
 //
 // Every kernel keeps the reference's operation order (-ffp-contract=off).
@@ -239,11 +241,12 @@ __global__ void k_lsbc_z(double* __restrict__ w, double* __restrict__ g, const d
 }
 
 // ADVECTION_3D::SetupBoundaryConditionForVelocity, velocity branch,
-// is_parallel, boundary_order_velocity 1 (ADVECTION_3D.h:716-851): at every
-// crossing inside -> outside along y or z the first outside value of the
+// boundary_order_velocity 1 (ADVECTION_3D.h:553-851): at every crossing
+// inside -> outside along the wall's first axis (y for the parallel pipe,
+// x for the vertical one: stride sx) or z, the first outside value of the
 // axis' copy is (theta - 1)/theta times the inside value and the next three
 // are pow(10 t, t) = 400, 27000, 2560000.  Each written cell belongs to one
-// crossing (a y or z line meets the convex pipe in one interval).
+// crossing (a line across the convex pipe meets it in one interval).
 __global__ void k_velbc_cross(double* __restrict__ ay, double* __restrict__ az, const double* __restrict__ g,
                               const double* __restrict__ b, FsLayout L, FsBox bx, long sx) {
     int i = blockIdx.x * PBX + threadIdx.x, j = blockIdx.y * PBY + threadIdx.y, k = blockIdx.z;
@@ -275,7 +278,7 @@ __global__ void k_velbc_cross(double* __restrict__ ay, double* __restrict__ az, 
     }
 }
 
-// VISCOSITY_3D::SetupBoundaryConditionsForVelocity(thread_id), is_parallel,
+// VISCOSITY_3D::SetupBoundaryConditionsForVelocity(thread_id),
 // boundary_order_velocity 1 (VISCOSITY_3D.h:2255-2352): each face outside
 // the pipe takes, in the reference's else-if order, zero in both copies
 // (all four neighbours outside), the extrapolation from the first inside
@@ -431,8 +434,8 @@ __device__ __forceinline__ double heav_visc(double phi, double eps) {
     return (double)0.5 * ((double)1 + phi * one_over_epsilon + one_over_pi * pipe_sin(FS_PI * phi * one_over_epsilon));
 }
 
-// EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless, is_parallel
-// (:2340-2393): inside the pipe, fixed faces excepted, u += R1/V0^2 (dt g_x
+// EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless; the parallel pipe
+// (:2362-2393): inside the pipe, fixed faces excepted, u += R1/V0^2 (dt g_x
 // + 1/(rho_half rho_oil) dt f) with rho_half = 1 + (eta - 1) H of the face
 // level set (eps 1.5 dx, :2522-2561); v, w += R1/V0^2 dt g
 // The vertical pipe (:2327-2360) drives v alone: v += R1/V0^2 (dt g_y +
@@ -463,8 +466,8 @@ __global__ void k_gravity_pipe(double* __restrict__ vel, const double* __restric
 
 // ---- semi-implicit viscosity: coefficients and explicit terms ----------------
 
-// VISCOSITY_3D::ApplyViscosity, dimensionless, is_parallel
-// (VISCOSITY_3D.h:579-852), one component: the level set at the face, edge
+// VISCOSITY_3D::ApplyViscosity, dimensionless (VISCOSITY_3D.h:579-852),
+// one component: the level set at the face, edge
 // centres and the two cells about it from the shared scalar ghost copy,
 // density 1 + (eta - 1) H, viscosities 1 + (m - 1) H (eps 1.5 dx,
 // :1258-1347), coefficients 2 dt/(Re rho) mu for the normal pair and
@@ -1169,7 +1172,7 @@ __global__ void k_pdiv(const double* __restrict__ gu, const double* __restrict__
     div[o] = dv * one_over_dt;
 }
 
-// UpdateVelocityByPressureGradientVariableDensity, oil/water, is_parallel
+// UpdateVelocityByPressureGradientVariableDensity, oil/water
 // (PROJECTION_3D.h:1117-1427), one component; vmax: bits of max |u| over the
 // faces updated
 struct FsPresUpd {
@@ -1797,7 +1800,7 @@ struct FsPipe {
                                vert);
     }
 
-    // EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless, is_parallel
+    // EULERIAN_FLUID_SOLVER_3D::ApplyGravity, dimensionless
     void ApplyGravity(double dt) {
         FsGravPipe Gp;
         Gp.gd[0] = dt * P.gravity[0];
@@ -1889,7 +1892,7 @@ struct FsPipe {
         return it;
     }
 
-    // VISCOSITY_3D::ApplyViscosity, semi-implicit, dimensionless, is_parallel
+    // VISCOSITY_3D::ApplyViscosity, semi-implicit, dimensionless
     // (VISCOSITY_3D.h:565-852): the boundary conditions (VISCOSITY_3D.h:
     // 2048-2106, 2235-2356) once, then each component's coefficients,
     // explicit term, system and solve
@@ -1953,7 +1956,7 @@ struct FsPipe {
         kmax = host_bits[0];
     }
 
-    // PROJECTION_3D::Solve, oil/water, is_parallel (PROJECTION_3D.h:414-432)
+    // PROJECTION_3D::Solve, oil/water (PROJECTION_3D.h:414-432)
     void Projection(double dt, fs_step_stats* stt) {
         // phi's own x ghosts and first cell from the periodic image (:424-427)
         periodic(phi, phi, 0, 2, 0, nullptr);
@@ -1972,7 +1975,7 @@ struct FsPipe {
                                (long)L.sx);
         }
         hipLaunchKernelGGL(k_pdiv, box_grid(box(0)), dim3(PBX, PBY), 0, st, g[1], g[2], g[3], bls, div, L, cell, 1 / dt);
-        // DetermineJumpConditionField (:746-792): jc = kappa/We, x-periodic
+        // DetermineJumpConditionField (:746-792): jc = kappa/We, periodic along the pipe
         Curvatures(phi);
         periodic(jc, jc, 0, 1, 1, nullptr);
         // DeterminePressure (:794-834)
@@ -2020,7 +2023,7 @@ struct FsPipe {
 
     // EULERIAN_FLUID_SOLVER_3D::Reinitialization with ReinitializationBySussman,
     // oil/water (ADVECTION_3D.h:360-378): each sweep copies phi into the
-    // shared scalar ghost copy, x-periodic, and takes one WENO5 Sussman step
+    // shared scalar ghost copy, periodic along the pipe, and takes one WENO5 Sussman step
     void Reinitialization() {
         if (!P.sussmanstyle_reinitialization) return;
         const double fictious_dt = cell.x.d / P.scaling_number_for_reinitialization;

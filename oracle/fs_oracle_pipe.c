@@ -4,10 +4,11 @@
  *
  * A plain-C, FP64, single-thread restatement of
  * EULERIAN_FLUID_SOLVER_3D::AdvanceOneTimeStepThread (EULERIAN_FLUID_SOLVER_3D.h:
- * 803-1111) in oil_water_simulation mode with the pipe along x (is_parallel),
- * cylindrical_boundary, Neumann pressure, dimensionless_form,
- * semi_implicit_approach and boundary_order_velocity 1 -- the configuration of
- * the reference's shipped script (3D_script.sim:32-99, 195-290).  Each function
+ * 803-1111) in oil_water_simulation mode with the pipe along x (is_parallel)
+ * or y (is_vertical), cylindrical_boundary, Neumann pressure,
+ * dimensionless_form, semi_implicit_approach and boundary_order_velocity 1 --
+ * the configuration of the reference's shipped scripts (3D_script.sim:32-99,
+ * 195-290 parallel; 3D_script_1/2/3.sim vertical).  Each function
  * cites the lines it restates; expressions keep the reference's operation
  * order (build with -ffp-contract=off).
  *
