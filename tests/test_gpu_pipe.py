@@ -7,6 +7,7 @@ Every kernel keeps the reference's operation order and both sides sum in the
 same fixed order, so the comparisons are bit for bit: fields after each phase,
 the persistent ghost copies each phase leaves (the reference's stale values
 outside the pipe included), the pressure and the solves' iteration counts."""
+import ctypes as C
 import sys
 
 import numpy as np
@@ -166,3 +167,19 @@ def test_vertical_pipe_shipped_scenes_at_their_size(hip_lib, oracle_lib, script)
         assert sg["pcg_iterations"] == sc["pcg_iterations"]
         assert sg["residual_precond"] < 1e-6
         assert_fields_equal(gpu, cpu, f"{script} frame {f}")
+
+
+@pytest.mark.parametrize("max_iteration", [0, 1, 7])
+def test_pipe_pcg_at_max_iteration(hip_lib, oracle_lib, max_iteration):
+    """A projection whose solve stops at max_iteration (PCG_METHOD.cpp's
+    iteration cap) stops at the checker's iterate, bit for bit, and reports
+    FS_ERR_NOT_CONVERGED."""
+    gpu, cpu = pair(hip_lib, oracle_lib, max_iteration=max_iteration)
+    dt = cpu.cfl_dt(0.9)
+    say(f"[pipe] max_iteration {max_iteration}, dt={dt}")
+    sc = cpu.phase("projection", dt)
+    st = _abi.FsStepStats()
+    rc = gpu._call("phase", _abi.FS_PIPE_PROJECTION, dt, C.byref(st), allow=(_abi.FS_ERR_NOT_CONVERGED,))
+    assert rc == _abi.FS_ERR_NOT_CONVERGED
+    assert st.as_dict()["pcg_iterations"] == sc["pcg_iterations"] == max_iteration
+    assert_fields_equal(gpu, cpu, f"max_iteration {max_iteration}")
