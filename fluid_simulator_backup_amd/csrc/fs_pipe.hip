@@ -812,13 +812,32 @@ __global__ void __launch_bounds__(RBLK) k_pcg_begin_scalars(const double* __rest
 //
 // The fixed order's second level has RBLK lanes: lane t adds the block sums
 // t, t + RBLK, ... in order.  The loop kernels run exactly RBLK workgroups of
-// PWG threads; workgroup w walks the blocks w, w + RBLK, ... (PWG / RBLK of
-// them at a time, added in block order), so its running sum IS lane w's, and
-// a consumer needs only the binary tree of the RBLK workgroup sums -- every
-// workgroup of the next kernel re-forms the scalar from them, in the same
-// order, without a fence or atomic.
+// W threads (a multiple of RBLK up to PWG) per system; workgroup w walks the blocks w, w +
+// RBLK, ... (W / RBLK of them at a time, added in block order), so its
+// running sum IS lane w's, whatever W, and a consumer needs only the binary
+// tree of the RBLK workgroup sums -- every workgroup of the next kernel
+// re-forms the scalar from them, in the same order, without a fence or atomic.
 #define PWG 1024
 #define PSUB (PWG / RBLK)
+
+// Up to PMAXSYS systems solved together: launch row y of the loop kernels
+// iterates system y with its own rows, sums, scalars and stop flags (the
+// semi-implicit viscosity's three independent components share launches)
+#define PMAXSYS 3
+struct PcgBatch {
+    const double* val[PMAXSYS];
+    const int* col[PMAXSYS];
+    const int* cnt[PMAXSYS];
+    const double* dg[PMAXSYS];
+    double* x[PMAXSYS];
+    double* r[PMAXSYS];
+    double* z[PMAXSYS];
+    double* p[PMAXSYS][2];  // p_k in p[k & 1]; p_1 = z in p[1]
+    double* q[PMAXSYS];
+    double* lp[PMAXSYS];  // the loop's workgroup sums: p.q, r.z, r.r (RBLK each)
+    FsPipePcg* S[PMAXSYS];
+    int N[PMAXSYS], nb[PMAXSYS];
+};
 
 // The binary tree of one block's RBLK values (t[q] += t[q + h], h = RBLK/2
 // .. 1) from LDS, called by the 64 lanes of one wave: the first two levels
@@ -859,6 +878,45 @@ __device__ __forceinline__ int pcg_tail(const double* __restrict__ lp, const FsP
     beta = t / rz_old;
     return m >= S->max_iter ? 2 : 0;
 }
+// the stop rule's constants, read before the launch stores to S
+struct PcgRule {
+    int rel, max_iter;
+    double rtol, bb, tol2;
+};
+__device__ __forceinline__ PcgRule pcg_rule(const FsPipePcg* S) {
+    return PcgRule{S->rel, S->max_iter, S->rtol, S->bb, S->tol2};
+}
+
+// pcg_tail with the workgroup sums of r.z and r.r already loaded by threads
+// t < RBLK (l0 = lp[t], l1 = lp[RBLK + t] of the sums' array)
+__device__ __forceinline__ int pcg_tail_pre(double l0, double l1, const PcgRule& R, int m, double rz_old,
+                                            double* sh, double* res, double& t, double& rr, double& beta) {
+    const int rel = R.rel;
+    if (threadIdx.x < RBLK) {
+        sh[threadIdx.x] = l0;
+        sh[RBLK + threadIdx.x] = l1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const double a = tree_lds(sh), b = rel ? tree_lds(sh + RBLK) : (double)0;
+        if (threadIdx.x == 0) res[0] = a, res[1] = b;
+    }
+    __syncthreads();
+    t = res[0];
+    rr = res[1];
+    if (rel) {
+        if (sqrt(rr) < R.rtol * sqrt(R.bb)) return 1;
+    } else if (t < R.tol2) {
+        return 1;
+    }
+    beta = t / rz_old;
+    return m >= R.max_iter ? 2 : 0;
+}
+
+// Keeps loads issued above a branch: the compiler may not move a load whose
+// value an asm statement reads below that statement
+#define PIN2(a, b) asm volatile("" ::"v"(a), "v"(b))
+
 // the host-facing record of a tail decision (one thread)
 __device__ __forceinline__ void pcg_record(FsPipePcg* S, int dec, int m, double rz_old, double t, double rr,
                                           double beta) {
@@ -875,25 +933,25 @@ __device__ __forceinline__ void pcg_record(FsPipePcg* S, int dec, int m, double 
     if (dec == 2) S->done = 2;
 }
 
-// After a pass has put its NS x PWG row values in sh (step s, sub-block q,
-// row l at sh[s * PWG + q * RBLK + l]): each block's tree, then thread 0
-// adds the trees of the blocks that exist (index c0 + RBLK (PSUB s + q) <
+// After a pass has put its NS x W row values in sh (step s, sub-block q,
+// row l at sh[s * W + q * RBLK + l]): each block's tree, then thread 0
+// adds the trees of the blocks that exist (index c0 + RBLK ((W / RBLK) s + q) <
 // nb) to acc in block order
-template <int NS>
+template <int NS, int W>
 __device__ __forceinline__ void pass_trees(const double* sh, double* res, int c0, int nb, double& acc) {
     __syncthreads();
     if (threadIdx.x % RBLK < 64) {
         const int q = threadIdx.x / RBLK;
         for (int s = 0; s < NS; s++) {
-            const double v = tree_lds(sh + s * PWG + q * RBLK);
-            if (threadIdx.x % RBLK == 0) res[s * PSUB + q] = v;
+            const double v = tree_lds(sh + s * W + q * RBLK);
+            if (threadIdx.x % RBLK == 0) res[s * (W / RBLK) + q] = v;
         }
     }
     __syncthreads();
     if (threadIdx.x == 0)
         for (int s = 0; s < NS; s++)
-            for (int q = 0; q < PSUB; q++)
-                if (c0 + RBLK * (PSUB * s + q) < nb) acc += res[s * PSUB + q];
+            for (int q = 0; q < (W / RBLK); q++)
+                if (c0 + RBLK * ((W / RBLK) * s + q) < nb) acc += res[s * (W / RBLK) + q];
 }
 
 // One row's operands of the fused direction + SpMV step, loaded apart from
@@ -902,21 +960,41 @@ __device__ __forceinline__ void pass_trees(const double* sh, double* res, int c0
 // in bounds.
 template <bool FIRST>
 struct PcgRow {
-    int n;
+    int n, c[PSLOTS];
     double a[PSLOTS], pc[PSLOTS], zc[PSLOTS], pr, zr;
-    __device__ __forceinline__ void load(const double* __restrict__ val, const int* __restrict__ col,
-                                         const int* __restrict__ cnt, int N, const double* __restrict__ po,
-                                         const double* __restrict__ z, int rc) {
+    // the row's own entries, count, p and z (no dependent loads)
+    __device__ __forceinline__ void load_row(const double* __restrict__ val, const int* __restrict__ col,
+                                             const int* __restrict__ cnt, int N, const double* __restrict__ po,
+                                             const double* __restrict__ z, int rc) {
         n = cnt[rc];
 #pragma unroll
         for (int e = 0; e < PSLOTS; e++) {
-            const int c = col[e * N + rc];
+            c[e] = col[e * N + rc];
             a[e] = val[e * N + rc];
-            pc[e] = po[c];
-            zc[e] = FIRST ? (double)0 : z[c];
         }
         pr = po[rc];
         zr = FIRST ? (double)0 : z[rc];
+    }
+    // the neighbours' p and z, by the row's columns
+    __device__ __forceinline__ void gather(const double* __restrict__ po, const double* __restrict__ z) {
+#pragma unroll
+        for (int e = 0; e < PSLOTS; e++) {
+            pc[e] = po[c[e]];
+            zc[e] = FIRST ? (double)0 : z[c[e]];
+        }
+    }
+    // the columns are read: the gathers can go out
+    __device__ __forceinline__ void pin_columns() const {
+        PIN2(c[0], c[1]);
+        PIN2(c[2], c[3]);
+        PIN2(c[4], c[5]);
+        PIN2(c[6], n);
+    }
+    __device__ __forceinline__ void load(const double* __restrict__ val, const int* __restrict__ col,
+                                         const int* __restrict__ cnt, int N, const double* __restrict__ po,
+                                         const double* __restrict__ z, int rc) {
+        load_row(val, col, cnt, N, po, z, rc);
+        gather(po, z);
     }
     // q = sum over the row's entries in order of a_e (p beta + z)_e; the
     // row's own new direction in p_row
@@ -946,36 +1024,59 @@ struct PcgRow {
 // every neighbour a row reads, the same bits), q = A p_k in each row's entry
 // order (CSR_MATRIX.h:236-260), and the workgroup sums of p.q.  p_1 = z is
 // in `po` already; for k > 1 p_k goes to `pn`.  S->rzs[k & 1] receives r.z
-// of iteration k - 1 for the update.  A pass covers NS steps of PWG rows,
+// of iteration k - 1 for the update.  A pass covers NS steps of W rows,
 // their loads in flight together; with NS = 1 (one step covers the system)
 // the row's loads go out before the tail.
-template <int NS, bool FIRST>
-__global__ void __launch_bounds__(PWG) k_pcg_spmv_dir(const double* __restrict__ val, const int* __restrict__ col,
-                                                      const int* __restrict__ cnt, int N, int nb,
-                                                      const double* __restrict__ po, double* __restrict__ pn,
-                                                      const double* __restrict__ z, double* __restrict__ q,
-                                                      double* __restrict__ lp, FsPipePcg* S, int k) {
-    __shared__ double sh[NS * PWG];
-    __shared__ double res[NS * PSUB];
+template <int NS, bool FIRST, int W>
+__global__ void __launch_bounds__(W) k_pcg_spmv_dir(const PcgBatch B, int k) {
+    const int y = blockIdx.y;
+    const double* __restrict__ val = B.val[y];
+    const int* __restrict__ col = B.col[y];
+    const int* __restrict__ cnt = B.cnt[y];
+    const int N = B.N[y], nb = B.nb[y];
+    const double* __restrict__ po = B.p[y][k == 1 ? 1 : (k - 1) & 1];
+    double* __restrict__ pn = B.p[y][k & 1];
+    const double* __restrict__ z = B.z[y];
+    double* __restrict__ q = B.q[y];
+    double* __restrict__ lp = B.lp[y];
+    FsPipePcg* S = B.S[y];
+    // the rows' values of a pass, and the tail's two sets of RBLK sums
+    __shared__ double sh[NS * W > 2 * RBLK ? NS * W : 2 * RBLK];
+    __shared__ double res[NS * (W / RBLK) > 2 ? NS * (W / RBLK) : 2];
     // an earlier stop is carried forward; this launch reads slot k - 1 and
     // writes slot k only, so no flag is read and written in one launch
     const int prev = S->stop[(k - 1) & 1];
-    if (prev) {
-        if (blockIdx.x == 0 && threadIdx.x == 0) S->stop[k & 1] = prev;
-        return;
-    }
+    const PcgRule R = pcg_rule(S);
+    const double rz_old = FIRST ? (double)0 : S->rzs[(k - 1) & 1];
     const int sub = threadIdx.x / RBLK, l = threadIdx.x % RBLK;
     PcgRow<FIRST> row1;
     const int b1 = blockIdx.x + RBLK * sub, r1 = b1 * RBLK + l;
     const bool in1 = b1 < nb && r1 < N;
-    if (NS == 1) row1.load(val, col, cnt, N, po, z, in1 ? r1 : 0);
+    // one-step passes: the flag, the row and the last iteration's sums are
+    // independent loads, all in bounds whatever the flag says; they go out
+    // together, ahead of the branch on the flag
+    double l0 = 0, l1 = 0;
+    if (NS == 1) {
+        row1.load_row(val, col, cnt, N, po, z, in1 ? r1 : 0);
+        if (!FIRST && threadIdx.x < RBLK) {
+            l0 = lp[RBLK + threadIdx.x];
+            l1 = lp[2 * RBLK + threadIdx.x];
+        }
+        row1.pin_columns();
+        PIN2(l0, l1);
+    }
+    if (prev) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) S->stop[k & 1] = prev;
+        return;
+    }
+    if (NS == 1) row1.gather(po, z);
     double beta = 0;
     if (FIRST) {
         if (blockIdx.x == 0 && threadIdx.x == 0) S->stop[k & 1] = 0;
     } else {
-        const double rz_old = S->rzs[(k - 1) & 1];
         double t, rr;
-        const int dec = pcg_tail(lp + RBLK, S, k - 1, rz_old, sh, res, t, rr, beta);
+        const int dec = NS == 1 ? pcg_tail_pre(l0, l1, R, k - 1, rz_old, sh, res, t, rr, beta)
+                                : pcg_tail(lp + RBLK, S, k - 1, rz_old, sh, res, t, rr, beta);
         if (blockIdx.x == 0 && threadIdx.x == 0) {
             pcg_record(S, dec, k - 1, rz_old, t, rr, beta);
             S->stop[k & 1] = dec;
@@ -993,12 +1094,12 @@ __global__ void __launch_bounds__(PWG) k_pcg_spmv_dir(const double* __restrict__
             if (!FIRST) pn[r1] = pr;
         }
         sh[threadIdx.x] = in1 ? pr * v : (double)0;
-        pass_trees<NS>(sh, res, blockIdx.x, nb, acc);
+        pass_trees<NS, W>(sh, res, blockIdx.x, nb, acc);
     } else {
-        for (int c0 = blockIdx.x; c0 < nb; c0 += RBLK * PSUB * NS) {
+        for (int c0 = blockIdx.x; c0 < nb; c0 += RBLK * (W / RBLK) * NS) {
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                const int b = c0 + RBLK * (PSUB * s + sub), r = b * RBLK + l;
+                const int b = c0 + RBLK * ((W / RBLK) * s + sub), r = b * RBLK + l;
                 const bool in = b < nb && r < N;
                 PcgRow<FIRST> row;
                 row.load(val, col, cnt, N, po, z, in ? r : 0);
@@ -1008,9 +1109,9 @@ __global__ void __launch_bounds__(PWG) k_pcg_spmv_dir(const double* __restrict__
                     q[r] = v;
                     if (!FIRST) pn[r] = pr;
                 }
-                sh[s * PWG + threadIdx.x] = in ? pr * v : (double)0;
+                sh[s * W + threadIdx.x] = in ? pr * v : (double)0;
             }
-            pass_trees<NS>(sh, res, c0, nb, acc);
+            pass_trees<NS, W>(sh, res, c0, nb, acc);
         }
     }
     if (threadIdx.x == 0) lp[blockIdx.x] = acc;
@@ -1018,23 +1119,39 @@ __global__ void __launch_bounds__(PWG) k_pcg_spmv_dir(const double* __restrict__
 
 // Iteration k, second launch: alpha = r.z / p.q, x += alpha p, r -= alpha q,
 // z = r / D, and the workgroup sums of r.z (and r.r) for the next tail
-template <int NS>
-__global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, double* __restrict__ r,
-                                                     double* __restrict__ z, const double* __restrict__ p,
-                                                     const double* __restrict__ q, const double* __restrict__ dg,
-                                                     int N, int nb, double* __restrict__ lp, FsPipePcg* S, int k) {
-    __shared__ double sh[2 * NS * PWG];
-    __shared__ double res[NS * PSUB];
-    if (S->stop[k & 1]) return;
+template <int NS, int W>
+__global__ void __launch_bounds__(W) k_pcg_update2(const PcgBatch B, int k) {
+    const int y = blockIdx.y;
+    double* __restrict__ x = B.x[y];
+    double* __restrict__ r = B.r[y];
+    double* __restrict__ z = B.z[y];
+    const double* __restrict__ p = B.p[y][k & 1];
+    const double* __restrict__ q = B.q[y];
+    const double* __restrict__ dg = B.dg[y];
+    const int N = B.N[y], nb = B.nb[y];
+    double* __restrict__ lp = B.lp[y];
+    FsPipePcg* S = B.S[y];
+    __shared__ double sh[2 * NS * W];
+    __shared__ double res[NS * (W / RBLK)];
+    const int stop = S->stop[k & 1];
     const int rel = S->rel;
+    const double rz = S->rzs[k & 1];
     const int sub = threadIdx.x / RBLK, l = threadIdx.x % RBLK;
-    // one-step passes: the row's loads go out before the tree of p.q
+    // one-step passes: the row's loads and the sums of p.q go out together,
+    // ahead of the branch on the stop flag (all in bounds whatever it says)
     const int b1 = blockIdx.x + RBLK * sub, i1 = b1 * RBLK + l;
     const bool in1 = b1 < nb && i1 < N;
     const int ic1 = in1 ? i1 : 0;
-    double x1 = 0, p1 = 0, r1 = 0, q1 = 0, d1 = 1;
-    if (NS == 1) x1 = x[ic1], p1 = p[ic1], r1 = r[ic1], q1 = q[ic1], d1 = dg[ic1];
-    if (threadIdx.x < RBLK) sh[threadIdx.x] = lp[threadIdx.x];
+    double x1 = 0, p1 = 0, r1 = 0, q1 = 0, d1 = 1, l0 = 0;
+    if (NS == 1) {
+        x1 = x[ic1], p1 = p[ic1], r1 = r[ic1], q1 = q[ic1], d1 = dg[ic1];
+        if (threadIdx.x < RBLK) l0 = lp[threadIdx.x];
+        PIN2(p1, q1);
+        PIN2(l0, r1);
+        PIN2(rz, rel);
+    }
+    if (stop) return;
+    if (threadIdx.x < RBLK) sh[threadIdx.x] = NS == 1 ? l0 : lp[threadIdx.x];
     __syncthreads();
     if (threadIdx.x < 64) {
         const double t = tree_lds(sh);
@@ -1042,7 +1159,7 @@ __global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, dou
     }
     __syncthreads();
     const double pq = res[0];
-    const double alpha = S->rzs[k & 1] / pq;
+    const double alpha = rz / pq;
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         S->pq = pq;
         S->alpha = alpha;
@@ -1058,14 +1175,14 @@ __global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, dou
             z[i1] = zv;
         }
         sh[threadIdx.x] = in1 ? rv * zv : (double)0;
-        sh[PWG + threadIdx.x] = in1 ? rv * rv : (double)0;
-        pass_trees<NS>(sh, res, blockIdx.x, nb, acc0);
-        if (rel) pass_trees<NS>(sh + PWG, res, blockIdx.x, nb, acc1);
+        sh[W + threadIdx.x] = in1 ? rv * rv : (double)0;
+        pass_trees<NS, W>(sh, res, blockIdx.x, nb, acc0);
+        if (rel) pass_trees<NS, W>(sh + W, res, blockIdx.x, nb, acc1);
     } else {
-        for (int c0 = blockIdx.x; c0 < nb; c0 += RBLK * PSUB * NS) {
+        for (int c0 = blockIdx.x; c0 < nb; c0 += RBLK * (W / RBLK) * NS) {
 #pragma unroll
             for (int s = 0; s < NS; s++) {
-                const int b = c0 + RBLK * (PSUB * s + sub), i = b * RBLK + l;
+                const int b = c0 + RBLK * ((W / RBLK) * s + sub), i = b * RBLK + l;
                 const bool in = b < nb && i < N;
                 const int ic = in ? i : 0;
                 const double xv = x[ic] + alpha * p[ic];
@@ -1076,11 +1193,11 @@ __global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, dou
                     r[i] = rv;
                     z[i] = zv;
                 }
-                sh[s * PWG + threadIdx.x] = in ? rv * zv : (double)0;
-                sh[(NS + s) * PWG + threadIdx.x] = in ? rv * rv : (double)0;
+                sh[s * W + threadIdx.x] = in ? rv * zv : (double)0;
+                sh[(NS + s) * W + threadIdx.x] = in ? rv * rv : (double)0;
             }
-            pass_trees<NS>(sh, res, c0, nb, acc0);
-            if (rel) pass_trees<NS>(sh + NS * PWG, res, c0, nb, acc1);
+            pass_trees<NS, W>(sh, res, c0, nb, acc0);
+            if (rel) pass_trees<NS, W>(sh + NS * W, res, c0, nb, acc1);
         }
     }
     if (threadIdx.x == 0) {
@@ -1090,9 +1207,11 @@ __global__ void __launch_bounds__(PWG) k_pcg_update2(double* __restrict__ x, dou
 }
 
 // the tail of iteration m at the end of a launch batch, so the host sees the
-// stop without another batch (one RBLK-thread workgroup; the next batch's
-// first launch repeats the same decision when it goes on)
-__global__ void __launch_bounds__(RBLK) k_pcg_tail_end(const double* __restrict__ lp, FsPipePcg* S, int m) {
+// stop without another batch (one RBLK-thread workgroup per system; the next
+// batch's first launch repeats the same decision when it goes on)
+__global__ void __launch_bounds__(RBLK) k_pcg_tail_end(const PcgBatch B, int m) {
+    const double* __restrict__ lp = B.lp[blockIdx.y];
+    FsPipePcg* S = B.S[blockIdx.y];
     __shared__ double sh[2 * RBLK];
     __shared__ double res[2];
     if (m < 1 || S->stop[m & 1]) return;
@@ -1340,10 +1459,20 @@ struct FsPipe {
     // systems: 0 pressure, 1..3 viscosity x, y, z
     PipeSys sys[4];
     int Nmax = 0;
-    double *val = nullptr, *dg = nullptr, *rhs = nullptr, *xv = nullptr, *rv = nullptr, *zv = nullptr, *pv = nullptr,
-           *pv2 = nullptr, *qv = nullptr, *part = nullptr, *lpart = nullptr;
-    int *col = nullptr, *cnt = nullptr;
-    FsPipePcg* pcg = nullptr;
+    // one PCG system's buffers (ELL rows of at most Nmax, its vectors and
+    // sums); set 0 takes the pressure, sets 0-2 the viscosity components,
+    // which are solved together
+    struct PcgSet {
+        double *val = nullptr, *dg = nullptr, *rhs = nullptr, *x = nullptr, *r = nullptr, *z = nullptr, *p = nullptr,
+               *p2 = nullptr, *q = nullptr, *part = nullptr, *lp = nullptr;
+        int *col = nullptr, *cnt = nullptr;
+        int N = 0;
+    };
+    PcgSet ps[PMAXSYS];
+    FsPipePcg* pcg = nullptr;  // PMAXSYS scalar records, one per set
+    // iterations of the last pressure (0) and viscosity (1) solve: the next
+    // solve of the kind checks for its stop first after about that many
+    int pcg_hint[2] = {0, 0};
     double* dscal = nullptr;             // device scalars: [0] a sum of the fixed-order reductions
     unsigned long long* bits = nullptr;  // [0] kappa max, [1] curvature failures, [2..4] max |u|, |v|, |w|
     FsPipePcg* host_pcg = nullptr;
@@ -1378,10 +1507,11 @@ struct FsPipe {
             (void)hipFree(sys[s].rows);
             for (int t = 0; t < 4; t++) (void)hipFree(xz[s][t]);
         }
-        for (void* d : {(void*)val, (void*)dg, (void*)rhs, (void*)xv, (void*)rv, (void*)zv, (void*)pv, (void*)pv2,
-                        (void*)qv, (void*)part, (void*)lpart, (void*)col, (void*)cnt, (void*)pcg, (void*)bits,
-                        (void*)dscal})
-            (void)hipFree(d);
+        for (PcgSet& q : ps)
+            for (void* d : {(void*)q.val, (void*)q.dg, (void*)q.rhs, (void*)q.x, (void*)q.r, (void*)q.z, (void*)q.p,
+                            (void*)q.p2, (void*)q.q, (void*)q.part, (void*)q.lp, (void*)q.col, (void*)q.cnt})
+                (void)hipFree(d);
+        for (void* d : {(void*)pcg, (void*)bits, (void*)dscal}) (void)hipFree(d);
         if (host_pcg) (void)hipHostFree(host_pcg);
         if (host_bits) (void)hipHostFree(host_bits);
         for (hipEvent_t e : kev)
@@ -1459,10 +1589,10 @@ struct FsPipe {
         kappa = alloc_field(); sgn = alloc_field(); tmp = alloc_field(); scr = alloc_field();
         for (int e = 0; e < 6; e++) coef[e] = alloc_field();
         expl = alloc_field();
-        PCHECK(hipMalloc(&pcg, sizeof(FsPipePcg)));
+        PCHECK(hipMalloc(&pcg, PMAXSYS * sizeof(FsPipePcg)));
         PCHECK(hipMalloc(&bits, 8 * sizeof(unsigned long long)));
         PCHECK(hipMalloc(&dscal, 8 * sizeof(double)));
-        PCHECK(hipHostMalloc(&host_pcg, sizeof(FsPipePcg)));
+        PCHECK(hipHostMalloc(&host_pcg, PMAXSYS * sizeof(FsPipePcg)));
         PCHECK(hipHostMalloc(&host_bits, 8 * sizeof(double)));
         static_geometry();
         sync();
@@ -1563,12 +1693,15 @@ struct FsPipe {
         // ELL storage for the largest system
         const size_t N = (size_t)std::max(Nmax, 1);
         const int nb = (int)((N + RBLK - 1) / RBLK);
-        PCHECK(hipMalloc(&val, sizeof(double) * N * PSLOTS));
-        PCHECK(hipMalloc(&col, sizeof(int) * N * PSLOTS));
-        PCHECK(hipMalloc(&cnt, sizeof(int) * N));
-        for (double** d : {&dg, &rhs, &xv, &rv, &zv, &pv, &pv2, &qv}) PCHECK(hipMalloc(d, sizeof(double) * N));
-        PCHECK(hipMalloc(&part, sizeof(double) * 3 * (nb + 1)));
-        PCHECK(hipMalloc(&lpart, sizeof(double) * 3 * RBLK));  // the loop's workgroup sums: p.q, r.z, r.r
+        for (PcgSet& q : ps) {
+            PCHECK(hipMalloc(&q.val, sizeof(double) * N * PSLOTS));
+            PCHECK(hipMalloc(&q.col, sizeof(int) * N * PSLOTS));
+            PCHECK(hipMalloc(&q.cnt, sizeof(int) * N));
+            for (double** d : {&q.dg, &q.rhs, &q.x, &q.r, &q.z, &q.p, &q.p2, &q.q})
+                PCHECK(hipMalloc(d, sizeof(double) * N));
+            PCHECK(hipMalloc(&q.part, sizeof(double) * 3 * (nb + 1)));
+            PCHECK(hipMalloc(&q.lp, sizeof(double) * 3 * RBLK));  // the loop's workgroup sums: p.q, r.z, r.r
+        }
     }
 
     // the masked y fill's crossings (k_periodic_y_ext) of face component c
@@ -1816,80 +1949,127 @@ struct FsPipe {
                                c, Gp);
     }
 
-    // Adds the timed launches of iteration k (of a system of N rows) to the
-    // samples if they ran in full: iteration k's launches return at once when
-    // the tail of iteration k - 1 stopped the solve
-    void sample(int k, int N) {
-        const FsPipePcg& h = *host_pcg;
-        const bool ran = h.done == 0 || (h.done == 1 ? k - 1 <= h.iter : k <= h.iter);
-        if (!ran) return;
+    // Adds the timed launches of iteration k to the samples, counting the rows
+    // of the systems for which they ran in full: iteration k's launches
+    // return at once for a system whose tail of iteration k - 1 stopped it
+    void sample(int k, int s0, int ns) {
+        double rows = 0;
+        for (int i = 0; i < ns; i++) {
+            const FsPipePcg& h = host_pcg[i];
+            if (h.done == 0 || (h.done == 1 ? k - 1 <= h.iter : k <= h.iter)) rows += ps[s0 + i].N;
+        }
+        if (rows == 0) return;
         float a = 0, b = 0;
         PCHECK(hipEventElapsedTime(&a, kev[0], kev[1]));
         PCHECK(hipEventElapsedTime(&b, kev[1], kev[2]));
         ks_ms[0] += a, ks_ms[1] += b;
-        ks_bytes[0] += 120.0 * N, ks_bytes[1] += 64.0 * N;
+        ks_bytes[0] += 120.0 * rows, ks_bytes[1] += 64.0 * rows;
         ks_n[0]++, ks_n[1]++;
     }
 
-    // the Jacobi-PCG of PCG_METHOD.cpp:4-160 on the assembled ELL system of
-    // N rows (x holds x0); returns the iterations
-    int Pcg(int N) {
-        const int nb = (N + RBLK - 1) / RBLK;
-        *host_pcg = FsPipePcg{};
-        host_pcg->max_iter = P.max_iteration;
-        host_pcg->rel = P.stop_rule == FS_STOP_REL_RESIDUAL;
-        host_pcg->tol2 = P.tolerance * P.tolerance;
-        host_pcg->rtol = P.rel_tolerance;
-        PCHECK(hipMemcpyAsync(pcg, host_pcg, sizeof(FsPipePcg), hipMemcpyHostToDevice, st));
-        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv);
-        hipLaunchKernelGGL(k_pcg_begin, dim3(nb), dim3(RBLK), 0, st, rhs, qv, dg, rv, zv, pv, N, part, nb);
-        hipLaunchKernelGGL(k_pcg_begin_scalars, dim3(1), dim3(RBLK), 0, st, part, nb, pcg);
-        // p_k lives in pb[k & 1]; p_1 = z, which k_pcg_begin wrote to pv
-        double* pb[2] = {pv2, pv};
-        // one step of PWG rows per workgroup covers the system: one-step passes
-        const bool one = nb <= RBLK * PSUB;
-        int batch = 8, k = 1;
+    // The Jacobi-PCG of PCG_METHOD.cpp:4-160 on the assembled ELL systems of
+    // sets s0 .. s0 + ns - 1 (each x holds its x0), solved together: every
+    // launch of the loop iterates each system still running, each with its
+    // own sums and stop test, so each takes the iterates of a solve alone.
+    // iters[i] receives set s0 + i's iterations; kind 0 pressure, 1 viscosity
+    // (the host checks for the stop between launch batches: the first one
+    // runs to a little short of the last solve of the kind, the next ones
+    // 8, 16, 32, 64 iterations).
+    void PcgSolve(int s0, int ns, int* iters, int kind) {
+        PcgBatch B{};
+        int nbmax = 0;
+        for (int i = 0; i < ns; i++) {
+            PcgSet& q = ps[s0 + i];
+            const int N = q.N, nb = (N + RBLK - 1) / RBLK;
+            nbmax = std::max(nbmax, nb);
+            FsPipePcg& hp = host_pcg[i];
+            hp = FsPipePcg{};
+            hp.max_iter = P.max_iteration;
+            hp.rel = P.stop_rule == FS_STOP_REL_RESIDUAL;
+            hp.tol2 = P.tolerance * P.tolerance;
+            hp.rtol = P.rel_tolerance;
+            B.val[i] = q.val, B.col[i] = q.col, B.cnt[i] = q.cnt, B.dg[i] = q.dg;
+            B.x[i] = q.x, B.r[i] = q.r, B.z[i] = q.z, B.q[i] = q.q, B.lp[i] = q.lp;
+            // p_1 = z, which k_pcg_begin writes to q.p
+            B.p[i][0] = q.p2, B.p[i][1] = q.p;
+            B.S[i] = pcg + i;
+            B.N[i] = N, B.nb[i] = nb;
+        }
+        PCHECK(hipMemcpyAsync(pcg, host_pcg, ns * sizeof(FsPipePcg), hipMemcpyHostToDevice, st));
+        for (int i = 0; i < ns; i++) {
+            PcgSet& q = ps[s0 + i];
+            const int N = q.N, nb = B.nb[i];
+            hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, q.val, q.col, q.cnt, N, q.x, q.q);
+            hipLaunchKernelGGL(k_pcg_begin, dim3(nb), dim3(RBLK), 0, st, q.rhs, q.q, q.dg, q.r, q.z, q.p, N, q.part, nb);
+            hipLaunchKernelGGL(k_pcg_begin_scalars, dim3(1), dim3(RBLK), 0, st, q.part, nb, pcg + i);
+        }
+        // one step of a workgroup's rows covers every system (one-step
+        // passes) in the narrowest workgroup that does -- RBLK threads per
+        // block of a step -- so that three systems' workgroups fit the GPU at
+        // once; larger systems take steps of PWG rows
+        const int steps = nbmax <= RBLK ? 1 : nbmax <= 2 * RBLK ? 2 : nbmax <= 3 * RBLK ? 3 : 4;
+        const bool one = nbmax <= RBLK * PSUB;
+        const dim3 grid(RBLK, ns), wg(one ? steps * RBLK : PWG);
+        using KA = void (*)(const PcgBatch, int);
+        const KA spmv1[4] = {k_pcg_spmv_dir<1, true, RBLK>, k_pcg_spmv_dir<1, true, 2 * RBLK>,
+                             k_pcg_spmv_dir<1, true, 3 * RBLK>, k_pcg_spmv_dir<1, true, 4 * RBLK>};
+        const KA spmvk[4] = {k_pcg_spmv_dir<1, false, RBLK>, k_pcg_spmv_dir<1, false, 2 * RBLK>,
+                             k_pcg_spmv_dir<1, false, 3 * RBLK>, k_pcg_spmv_dir<1, false, 4 * RBLK>};
+        const KA upd[4] = {k_pcg_update2<1, RBLK>, k_pcg_update2<1, 2 * RBLK>, k_pcg_update2<1, 3 * RBLK>,
+                           k_pcg_update2<1, 4 * RBLK>};
+        int batch = std::max(8, pcg_hint[kind] - 8), next = 8, k = 1;
         while (true) {
             int ks = 0;  // the iteration whose two launches are timed (0: none)
-            for (int q = 0; q < batch; q++, k++) {
-                const double* po = pb[k == 1 ? 1 : (k - 1) & 1];
-                const bool timed = sampling && q == 1;
+            for (int j = 0; j < batch; j++, k++) {
+                const bool timed = sampling && j == 1;
                 if (timed) {
                     ks = k;
                     PCHECK(hipEventRecord(kev[0], st));
                 }
-                auto ka = one ? (k == 1 ? k_pcg_spmv_dir<1, true> : k_pcg_spmv_dir<1, false>)
-                              : (k == 1 ? k_pcg_spmv_dir<4, true> : k_pcg_spmv_dir<4, false>);
-                hipLaunchKernelGGL(ka, dim3(RBLK), dim3(PWG), 0, st, val, col, cnt, N, nb, po, pb[k & 1], zv, qv,
-                                   lpart, pcg, k);
+                const KA ka = one ? (k == 1 ? spmv1 : spmvk)[steps - 1]
+                                  : (k == 1 ? k_pcg_spmv_dir<4, true, PWG> : k_pcg_spmv_dir<4, false, PWG>);
+                const KA kb = one ? upd[steps - 1] : k_pcg_update2<4, PWG>;
+                hipLaunchKernelGGL(ka, grid, wg, 0, st, B, k);
                 if (timed) PCHECK(hipEventRecord(kev[1], st));
-                hipLaunchKernelGGL(one ? k_pcg_update2<1> : k_pcg_update2<4>, dim3(RBLK), dim3(PWG), 0, st, xv, rv, zv,
-                                   pb[k & 1], qv, dg, N, nb, lpart, pcg, k);
+                hipLaunchKernelGGL(kb, grid, wg, 0, st, B, k);
                 if (timed) PCHECK(hipEventRecord(kev[2], st));
             }
             if (sampling) ks_launch[0] += batch, ks_launch[1] += batch;
-            hipLaunchKernelGGL(k_pcg_tail_end, dim3(1), dim3(RBLK), 0, st, lpart, pcg, k - 1);
-            PCHECK(hipMemcpyAsync(host_pcg, pcg, sizeof(FsPipePcg), hipMemcpyDeviceToHost, st));
+            hipLaunchKernelGGL(k_pcg_tail_end, dim3(1, ns), dim3(RBLK), 0, st, B, k - 1);
+            PCHECK(hipMemcpyAsync(host_pcg, pcg, ns * sizeof(FsPipePcg), hipMemcpyDeviceToHost, st));
             sync();
-            if (ks) sample(ks, N);
-            if (host_pcg->done) break;
+            if (ks) sample(ks, s0, ns);
+            bool done = true;
+            for (int i = 0; i < ns; i++) done = done && host_pcg[i].done;
+            if (done) break;
             if (k > P.max_iteration + 3) throw FsPipeError{FS_ERR_HIP, "pipe PCG: iteration count out of step"};
-            batch = std::min(64, batch * 2);
+            batch = next;
+            next = std::min(64, next * 2);
         }
         check_launch();
-        const int it = host_pcg->iter;
-        last_res_abs = std::sqrt(host_pcg->rz);
-        // the true residual for the report
-        hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, val, col, cnt, N, xv, qv);
-        hipLaunchKernelGGL(k_residual, dim3(nb), dim3(RBLK), 0, st, rhs, qv, rv, N);
-        hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rv, rv, N, part);
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, part, nb, dscal);
-        PCHECK(hipMemcpyAsync(host_bits, dscal, sizeof(double), hipMemcpyDeviceToHost, st));
+        // the true residuals for the report
+        for (int i = 0; i < ns; i++) {
+            PcgSet& q = ps[s0 + i];
+            const int N = q.N, nb = B.nb[i];
+            hipLaunchKernelGGL(k_spmv, dim3(nb), dim3(RBLK), 0, st, q.val, q.col, q.cnt, N, q.x, q.q);
+            hipLaunchKernelGGL(k_residual, dim3(nb), dim3(RBLK), 0, st, q.rhs, q.q, q.r, N);
+            hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, q.r, q.r, N, q.part);
+            hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, q.part, nb, dscal + i);
+        }
+        PCHECK(hipMemcpyAsync(host_bits, dscal, ns * sizeof(double), hipMemcpyDeviceToHost, st));
         sync();
-        const double bnorm = std::sqrt(host_pcg->bb);
-        last_res_rel = bnorm > 0 ? std::sqrt(host_bits[0]) / bnorm : 0;
-        if (host_pcg->done == 2) not_converged = true;
-        return it;
+        // the report keeps the last system's residuals, as one solve after
+        // another would
+        pcg_hint[kind] = 0;
+        for (int i = 0; i < ns; i++) {
+            const FsPipePcg& hp = host_pcg[i];
+            iters[i] = hp.iter;
+            pcg_hint[kind] = std::max(pcg_hint[kind], hp.iter);
+            last_res_abs = std::sqrt(hp.rz);
+            const double bnorm = std::sqrt(hp.bb);
+            last_res_rel = bnorm > 0 ? std::sqrt(host_bits[i]) / bnorm : 0;
+            if (hp.done == 2) not_converged = true;
+        }
     }
 
     // VISCOSITY_3D::ApplyViscosity, semi-implicit, dimensionless
@@ -1932,11 +2112,21 @@ struct FsPipe {
             // pipe, the per-axis copies across it (:712-716, 778-782, 844-848)
             const double* dx_ = vert ? vg[c][0] : g[c];
             const double* dy_ = vert ? g[c] : vg[c][0];
+            PcgSet& q = ps[c - 1];
+            q.N = N;
             hipLaunchKernelGGL(k_visc_assemble, dim3(nb), dim3(RBLK), 0, st, sys[c].rows, N, sys[c].bc, coef[0],
-                               coef[1], coef[2], coef[3], coef[4], coef[5], expl, dx_, dy_, vg[c][1], field(c), val,
-                               col, cnt, dg, rhs, xv, L, S);
-            visc_iters += Pcg(N);
-            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, field(c), xv, sys[c].rows, N, L, c, vert);
+                               coef[1], coef[2], coef[3], coef[4], coef[5], expl, dx_, dy_, vg[c][1], field(c), q.val,
+                               q.col, q.cnt, q.dg, q.rhs, q.x, L, S);
+        }
+        // the components' systems read only the copies made above, and each
+        // solve writes its own component: the three are solved together
+        int it[3];
+        PcgSolve(0, 3, it, 1);
+        visc_iters += it[0] + it[1] + it[2];
+        for (int c = 1; c < 4; c++) {
+            const int N = sys[c].N, nb = (N + RBLK - 1) / RBLK;
+            hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, field(c), ps[c - 1].x, sys[c].rows, N, L, c,
+                               vert);
             if (c == (vert ? 2 : 1)) {
                 FsBox b = box(c);
                 hipLaunchKernelGGL(k_fix_last, dim3(((vert ? b.ni : b.nj) + 63) / 64, b.nk), dim3(64), 0, st, field(c), L,
@@ -1989,14 +2179,16 @@ struct FsPipe {
         S.odx = cell.x.inv, S.ody = cell.y.inv, S.odz = cell.z.inv;
         S.odx2 = cell.x.inv_d2, S.ody2 = cell.y.inv_d2, S.odz2 = cell.z.inv_d2;
         const int N = sys[0].N, nb = (N + RBLK - 1) / RBLK;
+        PcgSet& q = ps[0];
+        q.N = N;
         hipLaunchKernelGGL(k_pres_assemble, dim3(nb), dim3(RBLK), 0, st, sys[0].rows, N, sys[0].bc, phi, irho, jc, div,
-                           p, val, col, cnt, dg, rhs, xv, L, S);
+                           p, q.val, q.col, q.cnt, q.dg, q.rhs, q.x, L, S);
         // Neumann compatibility, POISSON_SOLVER_3D.h:221-246
-        hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, rhs, nullptr, N, part);
-        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, part, nb, dscal);
-        hipLaunchKernelGGL(k_remove_mean, dim3(nb), dim3(RBLK), 0, st, rhs, N, dscal);
-        last_iters = Pcg(N);
-        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, p, xv, sys[0].rows, N, L, 0, vert);
+        hipLaunchKernelGGL(k_dot_part, dim3(nb), dim3(RBLK), 0, st, q.rhs, nullptr, N, q.part);
+        hipLaunchKernelGGL(k_sum_parts, dim3(1), dim3(RBLK), 0, st, q.part, nb, dscal);
+        hipLaunchKernelGGL(k_remove_mean, dim3(nb), dim3(RBLK), 0, st, q.rhs, N, dscal);
+        PcgSolve(0, 1, &last_iters, 0);
+        hipLaunchKernelGGL(k_scatter, dim3(nb), dim3(RBLK), 0, st, p, q.x, sys[0].rows, N, L, 0, vert);
         // UpdateVelocity (:1101-1449)
         periodic(psfg, phi, 0, 2, 1, nullptr);
         PCHECK(hipMemsetAsync(bits + 2, 0, 3 * sizeof(unsigned long long), st));

@@ -578,12 +578,14 @@ int  fs_pipe_get_array(fs_solver* s, int which, double* dst);
 /* In-loop samples of the pipe solves' two loop kernels while
  * fs_set_kernel_sampling is on: one launch of each per launch batch of every
  * solve is timed with HIP events on the handle's stream (a launch that the
- * stop test ended early is dropped).  Returns the mean duration (ms), the
- * mean algorithmic bytes of the timed launches (FS_PIPE_KS_SPMV, k_pcg_spmv_dir:
- * 120 per row -- 7 ELL values and columns, the count, p, z and the new p and
- * q; FS_PIPE_KS_UPDATE, k_pcg_update2: 64 per row -- x, r read and written,
- * p, q, D read, z written), the number of timed launches and the number of
- * launches since sampling was enabled.  No reference counterpart:
+ * stop test ended early is dropped).  The three viscosity components are
+ * solved together, one launch iterating each system still running.  Returns
+ * the mean duration (ms), the mean algorithmic bytes of the timed launches
+ * over the rows of the systems they iterated (FS_PIPE_KS_SPMV,
+ * k_pcg_spmv_dir: 120 per row -- 7 ELL values and columns, the count, p, z
+ * and the new p and q; FS_PIPE_KS_UPDATE, k_pcg_update2: 64 per row -- x, r
+ * read and written, p, q, D read, z written), the number of timed launches
+ * and the number of launches since sampling was enabled.  No reference counterpart:
  * measurement hooks beside fs_get_frame_kernel_samples. */
 #define FS_PIPE_KS_SPMV   0
 #define FS_PIPE_KS_UPDATE 1
