@@ -183,3 +183,22 @@ def test_pipe_pcg_at_max_iteration(hip_lib, oracle_lib, max_iteration):
     assert rc == _abi.FS_ERR_NOT_CONVERGED
     assert st.as_dict()["pcg_iterations"] == sc["pcg_iterations"] == max_iteration
     assert_fields_equal(gpu, cpu, f"max_iteration {max_iteration}")
+
+
+@pytest.mark.parametrize("vertical", [False, True])
+def test_pipe_frame_at_max_iteration(hip_lib, oracle_lib, vertical):
+    """A whole frame whose every solve stops at max_iteration = 5: the three
+    viscosity systems, solved in shared launches, and the projection stop at
+    the checker's iterates, bit for bit."""
+    gpu, cpu = pair(hip_lib, oracle_lib, vertical=vertical, max_iteration=5)
+    dt = cpu.cfl_dt(0.8 if vertical else 0.9)
+    say(f"[pipe] frame at max_iteration 5, vertical={vertical}, dt={dt}")
+    sc = cpu.advance_frame(dt)
+    st = _abi.FsStepStats()
+    rc = gpu._call("frame", dt, C.byref(st), allow=(_abi.FS_ERR_NOT_CONVERGED,))
+    assert rc == _abi.FS_ERR_NOT_CONVERGED
+    sg = st.as_dict()
+    # RK2: two projections and two sets of three viscosity solves
+    assert sg["pcg_iterations"] == sc["pcg_iterations"] == 10
+    assert sg["viscosity_iterations"] == sc["viscosity_iterations"] == 30
+    assert_fields_equal(gpu, cpu, "frame at max_iteration 5")
