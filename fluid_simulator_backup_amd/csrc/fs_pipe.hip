@@ -1973,8 +1973,8 @@ struct FsPipe {
     // own sums and stop test, so each takes the iterates of a solve alone.
     // iters[i] receives set s0 + i's iterations; kind 0 pressure, 1 viscosity
     // (the host checks for the stop between launch batches: the first one
-    // runs to a little short of the last solve of the kind, the next ones
-    // 8, 16, 32, 64 iterations).
+    // runs to a little short of the last solve of the kind, at most 1024
+    // iterations, the next ones 8, 16, 32, 64).
     void PcgSolve(int s0, int ns, int* iters, int kind) {
         PcgBatch B{};
         int nbmax = 0;
@@ -2017,7 +2017,7 @@ struct FsPipe {
                              k_pcg_spmv_dir<1, false, 3 * RBLK>, k_pcg_spmv_dir<1, false, 4 * RBLK>};
         const KA upd[4] = {k_pcg_update2<1, RBLK>, k_pcg_update2<1, 2 * RBLK>, k_pcg_update2<1, 3 * RBLK>,
                            k_pcg_update2<1, 4 * RBLK>};
-        int batch = std::max(8, pcg_hint[kind] - 8), next = 8, k = 1;
+        int batch = std::min(1024, std::max(8, pcg_hint[kind] - 8)), next = 8, k = 1;
         while (true) {
             int ks = 0;  // the iteration whose two launches are timed (0: none)
             for (int j = 0; j < batch; j++, k++) {
