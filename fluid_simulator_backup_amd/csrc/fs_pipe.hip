@@ -1971,7 +1971,9 @@ struct FsPipe {
     // sets s0 .. s0 + ns - 1 (each x holds its x0), solved together: every
     // launch of the loop iterates each system still running, each with its
     // own sums and stop test, so each takes the iterates of a solve alone.
-    // iters[i] receives set s0 + i's iterations; kind 0 pressure, 1 viscosity
+    // iters[i] receives set s0 + i's iterations; kind 0 pressure (whose true
+    // residual the step reports), 1 viscosity (no report: the next
+    // projection's replaces it)
     // (the host checks for the stop between launch batches: the first one
     // runs to a little short of the last solve of the kind, at most 1024
     // iterations, the next ones 8, 16, 32, 64).
@@ -2047,6 +2049,14 @@ struct FsPipe {
             next = std::min(64, next * 2);
         }
         check_launch();
+        pcg_hint[kind] = 0;
+        for (int i = 0; i < ns; i++) {
+            const FsPipePcg& hp = host_pcg[i];
+            iters[i] = hp.iter;
+            pcg_hint[kind] = std::max(pcg_hint[kind], hp.iter);
+            if (hp.done == 2) not_converged = true;
+        }
+        if (kind != 0) return;
         // the true residuals for the report
         for (int i = 0; i < ns; i++) {
             PcgSet& q = ps[s0 + i];
@@ -2060,15 +2070,11 @@ struct FsPipe {
         sync();
         // the report keeps the last system's residuals, as one solve after
         // another would
-        pcg_hint[kind] = 0;
         for (int i = 0; i < ns; i++) {
             const FsPipePcg& hp = host_pcg[i];
-            iters[i] = hp.iter;
-            pcg_hint[kind] = std::max(pcg_hint[kind], hp.iter);
             last_res_abs = std::sqrt(hp.rz);
             const double bnorm = std::sqrt(hp.bb);
             last_res_rel = bnorm > 0 ? std::sqrt(host_bits[i]) / bnorm : 0;
-            if (hp.done == 2) not_converged = true;
         }
     }
 
